@@ -1,0 +1,148 @@
+"""Benchmark of the ELAS path on one MI355X.
+
+Workload (the headline metric's single-GPU configuration): one synthetic
+1242x375 uint8 stereo pair, disparities 0..127, seed 1 (16 slanted-plane
+regions, 5% occluded strips), reference-effective parameters.  A step is one
+Elas::process of that pair with images and disparity maps resident in HBM
+(elas_hip_process_device), host Delaunay included.
+
+value = Mpx*disp/s = W*H*(disp_max+1)/1e6 per pair / wall seconds.
+Also reported: pairs/s, per-stage device times (HIP events on the pipeline
+stream), the roofline of the dominant kernel, the batched-vs-plain pairs/s
+on the mixed 64-pair batch, and the oracle (the reference algorithm restated
+on the CPU) timed on this host on a bounded sample.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+import __graft_entry__  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+# v_sad_u8 (4 byte-|diff| per lane-op): 256 CU x 64 lanes x 2.4 GHz x 4
+SAD_PEAK_TBYTE = 256 * 64 * 2.4e9 * 4 / 1e12
+
+
+def work_counts(oracle, left, right, dmax):
+    """Algorithmic work of the kernels, counted by the oracle on this input:
+    (candidate, d) pairs of each support pass and (pixel, d) pairs of dense
+    matching."""
+    prm = oracle.params(disp_max=dmax)
+    d0 = oracle.descriptor(left)
+    d1 = oracle.descriptor(right)
+    H, W = left.shape
+    can = np.full((H // prm.candidate_stepsize, W // prm.candidate_stepsize), -1, np.int32)
+    w_left = oracle.support_pass(d0, d1, prm, 0, can)
+    w_right = oracle.support_pass(d0, d1, prm, 1, can)
+    out = oracle.process(left, right, prm)
+    return dict(support_left=w_left, support_right=w_right, dense=out["work_dense"])
+
+
+def cpu_baseline(oracle, left, right, dmax, budget_s=12.0):
+    prm = oracle.params(disp_max=dmax)
+    n, t0 = 0, time.perf_counter()
+    while True:
+        oracle.process(left, right, prm, dump=False)
+        n += 1
+        el = time.perf_counter() - t0
+        if (el > budget_s and n >= 2) or n >= 1000:
+            break
+    return n, el
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched comparison")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only (pairs would shard across GPUs with no "
+              "exchange; no multi-GPU path is built)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    pkg = __graft_entry__.load_package()
+    synth = __graft_entry__.load_synth()
+    import oracle
+
+    left, right, _, dmax = synth.config_pair("c1")
+    H, W = left.shape
+    mpxd = W * H * (dmax + 1) / 1e6
+    el = pkg.Elas(pkg.default_params(disp_max=dmax), device=0)
+    dev = torch.device("cuda:0")
+    L = torch.from_numpy(left).to(dev)
+    R = torch.from_numpy(right).to(dev)
+    D1 = torch.empty((H, W), dtype=torch.float32, device=dev)
+    D2 = torch.empty_like(D1)
+    torch.cuda.synchronize()
+    for _ in range(args.warmup):
+        el.process_device(L, R, D1, D2)
+    torch.cuda.synchronize()
+    stage_sum = {}
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        el.process_device(L, R, D1, D2)
+        for k, v in el.stage_times().items():
+            stage_sum[k] = stage_sum.get(k, 0.0) + v
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    stages = {k: v / args.steps for k, v in stage_sum.items()}
+    value = args.steps * mpxd / wall
+
+    wc = work_counts(oracle, left, right, dmax)
+    sup_ms = stages["support"]
+    sup_ops = 64.0 * (wc["support_left"] + wc["support_right"])
+    den_ops = 16.0 * wc["dense"]
+    desc_bytes = 2 * 17.0 * W * H
+    kernels = {
+        "support": dict(bound="valu_sad", achieved=sup_ops / (sup_ms * 1e-3) / 1e12, peak=SAD_PEAK_TBYTE,
+                        unit="Tbyte-diff/s", ms=sup_ms),
+        "dense": dict(bound="valu_sad", achieved=den_ops / (stages["dense"] * 1e-3) / 1e12,
+                      peak=SAD_PEAK_TBYTE, unit="Tbyte-diff/s", ms=stages["dense"]),
+        "descriptor": dict(bound="hbm", achieved=desc_bytes / (stages["descriptor"] * 1e-3) / 1e9,
+                           peak=HBM_PEAK_GBS, unit="GB/s", ms=stages["descriptor"]),
+    }
+    for k in kernels.values():
+        k["frac"] = k["achieved"] / k["peak"]
+    dom = max(("support", "dense", "descriptor"), key=lambda k: kernels[k]["ms"])
+    d = kernels[dom]
+    roofline = {"bound": d["bound"], "achieved": round(d["achieved"], 4), "peak": round(d["peak"], 3),
+                "unit": d["unit"], "frac": round(d["frac"], 5), "traffic": None, "kernel": dom}
+
+    res = {
+        "metric": "Mpx*disp/s", "value": round(value, 2), "unit": "Mpx*disp/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(wall / args.steps * 1e3, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u8",
+        "data": "synthetic",
+        "config": {"workload": "elas_plain_1242x375_d128_device_resident", "width": W, "height": H,
+                   "disp_max": dmax, "pairs_per_step": 1, "host_delaunay": "included"},
+        "pairs_per_s": round(args.steps / wall, 2),
+        "stages_ms": {k: round(v, 5) for k, v in stages.items()},
+        "gpu_ms_per_pair": round(sum(stages[k] for k in ("descriptor", "support", "filter", "compact",
+                                                         "plane", "dense", "lr")), 5),
+        "roofline": roofline,
+        "kernels": {k: {kk: (round(vv, 5) if isinstance(vv, float) else vv) for kk, vv in v.items()}
+                    for k, v in kernels.items()},
+        "work": wc,
+    }
+    if not args.no_cpu:
+        n, el_s = cpu_baseline(oracle, left, right, dmax, args.cpu_seconds)
+        res["cpu_baseline"] = {"value": round(n * mpxd / el_s, 3), "unit": "Mpx*disp/s", "cores": 1,
+                               "kind": "port", "sample": "%d full pipelines of the same 1242x375 d128 pair "
+                               "(oracle, 1 thread, %.1f s)" % (n, el_s)}
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

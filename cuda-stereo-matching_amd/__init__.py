@@ -1,0 +1,249 @@
+"""MI355X-native ELAS stereo path: Python front end over libelas_hip.so.
+
+Mirrors the reference's interface (class Elas, src/Elas.h:11-167):
+``Elas(params).process(left, right) -> (D1, D2)`` with the same parameter
+struct, plus the batched and device-resident forms.  Every call goes to the
+gfx950 kernels through the C ABI of include/elas_hip.h; there is no CPU
+fallback: a missing library or device raises.
+
+The directory name is not a Python identifier; load it with
+``importlib`` (see ``__graft_entry__.load_package``) under the name
+``cuda_stereo_matching_amd``.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libelas_hip.so")
+
+STATUS = {0: "ok", 1: "invalid argument", 2: "HIP error", 3: "out of memory",
+          4: "too few support points", 5: "no gfx950 device", 6: "unsupported", 7: "busy"}
+ELAS_OK, ELAS_ERR_TOO_FEW_POINTS = 0, 4
+
+STAGES = ["desc0", "desc1", "can_left", "can_lr", "can_incon", "can_final", "sp0", "sp1", "grid0",
+          "grid1", "tri0", "tri1", "mean0", "mean1", "d1_raw", "d2_raw", "D1", "D2"]
+TIMERS = ["descriptor", "support", "filter", "compact", "grid", "triangle", "plane", "dense", "lr",
+          "post", "total"]
+
+
+class ElasParams(ctypes.Structure):
+    """elas_params: same field order and types as Elas::parameters (src/Elas.h:15-37)."""
+    _fields_ = [
+        ("disp_min", ctypes.c_int32), ("disp_max", ctypes.c_int32),
+        ("support_threshold", ctypes.c_float), ("support_texture", ctypes.c_int32),
+        ("candidate_stepsize", ctypes.c_int32), ("incon_window_size", ctypes.c_int32),
+        ("incon_threshold", ctypes.c_int32), ("incon_min_support", ctypes.c_int32),
+        ("add_corners", ctypes.c_bool), ("grid_size", ctypes.c_int32),
+        ("beta", ctypes.c_float), ("gamma", ctypes.c_float), ("sigma", ctypes.c_float),
+        ("sradius", ctypes.c_float), ("match_texture", ctypes.c_int32),
+        ("lr_threshold", ctypes.c_int32), ("speckle_sim_threshold", ctypes.c_float),
+        ("speckle_size", ctypes.c_int32), ("ipol_gap_width", ctypes.c_int32),
+        ("filter_median", ctypes.c_bool), ("filter_adaptive_mean", ctypes.c_bool),
+        ("postprocess_only_left", ctypes.c_bool), ("subsampling", ctypes.c_bool),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class ShapeInfo(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_int32) for k in ("width", "height", "can_width", "can_height", "grid_width",
+                                               "grid_height", "num_support", "num_tri0", "num_tri1",
+                                               "status")]
+
+
+# Exported symbols and their signatures (include/elas_hip.h).
+_P = ctypes.c_void_p
+SIGNATURES = {
+    "elas_params_default": (None, [ctypes.POINTER(ElasParams)]),
+    "elas_params_declared": (None, [ctypes.POINTER(ElasParams)]),
+    "elas_hip_create": (ctypes.c_int, [ctypes.POINTER(ElasParams), ctypes.c_int, ctypes.POINTER(_P)]),
+    "elas_hip_destroy": (ctypes.c_int, [_P]),
+    "elas_hip_set_params": (ctypes.c_int, [_P, ctypes.POINTER(ElasParams)]),
+    "elas_hip_last_error": (ctypes.c_char_p, [_P]),
+    "elas_hip_stream": (_P, [_P]),
+    "elas_hip_process": (ctypes.c_int, [_P, _P, _P, _P, _P, _P]),
+    "elas_hip_process_device": (ctypes.c_int, [_P, _P, _P, _P, _P, ctypes.c_int32, ctypes.c_int32]),
+    "elas_hip_process_batch": (ctypes.c_int, [_P, ctypes.c_int32, _P, _P, _P, _P, _P, _P]),
+    "elas_hip_set_debug": (ctypes.c_int, [_P, ctypes.c_int]),
+    "elas_hip_debug_fetch": (ctypes.c_int, [_P, ctypes.c_int, _P, ctypes.c_size_t,
+                                            ctypes.POINTER(ctypes.c_size_t)]),
+    "elas_hip_shape_info": (ctypes.c_int, [_P, ctypes.POINTER(ShapeInfo)]),
+    "elas_hip_stage_times": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_float), ctypes.c_int32]),
+}
+
+_lib = None
+
+
+def load_library(path=LIB_PATH):
+    """Load libelas_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.isfile(path):
+        raise RuntimeError("libelas_hip.so is not built (%s): run __graft_entry__.build()" % path)
+    lib = ctypes.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def default_params(**overrides):
+    """Effective reference defaults (src/Kernels/KernelsCommon.h:8-27) with overrides."""
+    p = ElasParams()
+    load_library().elas_params_default(ctypes.byref(p))
+    for k, v in overrides.items():
+        setattr(p, k, v)
+    return p
+
+
+def declared_params(**overrides):
+    """The initialisers of src/Elas.h:42-66 with overrides."""
+    p = ElasParams()
+    load_library().elas_params_declared(ctypes.byref(p))
+    for k, v in overrides.items():
+        setattr(p, k, v)
+    return p
+
+
+class ElasError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("%s (%d): %s" % (STATUS.get(code, "error"), code, msg))
+        self.code = code
+
+
+def _u8(a):
+    a = np.ascontiguousarray(a, dtype=np.uint8)
+    if a.ndim != 2:
+        raise ValueError("images must be 2-D uint8 arrays")
+    return a
+
+
+class Elas:
+    """Drop-in for the reference's class Elas on one gfx950 GPU."""
+
+    def __init__(self, params=None, device=0):
+        self._lib = load_library()
+        self.params = params if params is not None else default_params()
+        self._ctx = ctypes.c_void_p()
+        st = self._lib.elas_hip_create(ctypes.byref(self.params), int(device), ctypes.byref(self._ctx))
+        if st != 0:
+            raise ElasError(st, "elas_hip_create(device=%d)" % device)
+        self.status = 0
+
+    def close(self):
+        if self._ctx:
+            self._lib.elas_hip_destroy(self._ctx)
+            self._ctx = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, st, allow_few=False):
+        self.status = st
+        if st == 0 or (allow_few and st == ELAS_ERR_TOO_FEW_POINTS):
+            return st
+        raise ElasError(st, (self._lib.elas_hip_last_error(self._ctx) or b"").decode())
+
+    def last_error(self):
+        return (self._lib.elas_hip_last_error(self._ctx) or b"").decode()
+
+    def set_params(self, params):
+        self._check(self._lib.elas_hip_set_params(self._ctx, ctypes.byref(params)))
+        self.params = params
+
+    def process(self, left, right, D1=None, D2=None):
+        """Elas::process (src/Elas.h:76-83): host uint8 [H,W] in, float32 D1/D2 out.
+        Fewer than 3 support points returns all -1 maps with status 4 (the
+        reference exits)."""
+        left, right = _u8(left), _u8(right)
+        if left.shape != right.shape:
+            raise ValueError("images must have the same size")
+        H, W = left.shape
+        D1 = np.empty((H, W), np.float32) if D1 is None else D1
+        D2 = np.empty((H, W), np.float32) if D2 is None else D2
+        dims = (ctypes.c_int32 * 3)(W, H, W)
+        st = self._lib.elas_hip_process(self._ctx, left.ctypes.data, right.ctypes.data, D1.ctypes.data,
+                                        D2.ctypes.data, ctypes.cast(dims, ctypes.c_void_p))
+        self._check(st, allow_few=True)
+        return D1, D2
+
+    def process_device(self, left, right, D1, D2):
+        """Device-resident form: torch uint8 [H,W] cuda tensors in, float32 out."""
+        H, W = int(left.shape[0]), int(left.shape[1])
+        for t, dt in ((left, "uint8"), (right, "uint8"), (D1, "float32"), (D2, "float32")):
+            if not t.is_cuda or not t.is_contiguous() or str(t.dtype) != "torch." + dt or tuple(t.shape) != (H, W):
+                raise ValueError("process_device expects contiguous cuda tensors of shape [H, W]")
+        st = self._lib.elas_hip_process_device(self._ctx, left.data_ptr(), right.data_ptr(), D1.data_ptr(),
+                                               D2.data_ptr(), W, H)
+        return self._check(st, allow_few=True)
+
+    def process_batch(self, pairs):
+        """pairs: list of (left, right, disp_max or None).  One launch per stage
+        over all pairs.  Returns list of (D1, D2, status)."""
+        n = len(pairs)
+        ls = [_u8(p[0]) for p in pairs]
+        rs = [_u8(p[1]) for p in pairs]
+        d1 = [np.empty(l.shape, np.float32) for l in ls]
+        d2 = [np.empty(l.shape, np.float32) for l in ls]
+        arr = lambda xs: (ctypes.c_void_p * n)(*[x.ctypes.data for x in xs])
+        dims = (ctypes.c_int32 * (3 * n))()
+        for i, (l, p) in enumerate(zip(ls, pairs)):
+            dims[3 * i], dims[3 * i + 1] = l.shape[1], l.shape[0]
+            dims[3 * i + 2] = -1 if len(p) < 3 or p[2] is None else int(p[2])
+        stat = (ctypes.c_int32 * n)()
+        st = self._lib.elas_hip_process_batch(self._ctx, n, ctypes.cast(arr(ls), _P), ctypes.cast(arr(rs), _P),
+                                              ctypes.cast(arr(d1), _P), ctypes.cast(arr(d2), _P),
+                                              ctypes.cast(dims, _P), ctypes.cast(stat, _P))
+        self._check(st)
+        return [(d1[i], d2[i], int(stat[i])) for i in range(n)]
+
+    def set_debug(self, on=True):
+        self._check(self._lib.elas_hip_set_debug(self._ctx, 1 if on else 0))
+
+    def shape_info(self):
+        info = ShapeInfo()
+        self._check(self._lib.elas_hip_shape_info(self._ctx, ctypes.byref(info)))
+        return {k: getattr(info, k) for k, _ in info._fields_}
+
+    def stage_times(self):
+        ms = (ctypes.c_float * len(TIMERS))()
+        self._check(self._lib.elas_hip_stage_times(self._ctx, ms, len(TIMERS)))
+        return dict(zip(TIMERS, list(ms)))
+
+    def fetch(self, stage):
+        """Intermediate of the last call (debug mode), shaped like the reference buffer."""
+        k = STAGES.index(stage)
+        sz = ctypes.c_size_t(0)
+        self._check(self._lib.elas_hip_debug_fetch(self._ctx, k, None, 0, ctypes.byref(sz)))
+        info = self.shape_info()
+        W, H = info["width"], info["height"]
+        cw, ch = info["can_width"], info["can_height"]
+        gw, gh = info["grid_width"], info["grid_height"]
+        if stage.startswith("desc"):
+            out = np.zeros((H, W, 16), np.uint8)
+        elif stage.startswith("can"):
+            out = np.zeros((ch, cw), np.int32)
+        elif stage.startswith("sp"):
+            out = np.zeros((info["num_support"], 2), np.float32)
+        elif stage.startswith("grid"):
+            out = np.zeros((gh, gw, sz.value // 4 // max(1, gw * gh)), np.int32)
+        elif stage == "tri0":
+            out = np.zeros((info["num_tri0"], 3), np.int32)
+        elif stage == "tri1":
+            out = np.zeros((info["num_tri1"], 3), np.int32)
+        else:
+            out = np.zeros((H, W), np.float32)
+        if out.nbytes != sz.value:
+            raise RuntimeError("stage %s: size %d != expected %d" % (stage, sz.value, out.nbytes))
+        if out.nbytes:
+            self._check(self._lib.elas_hip_debug_fetch(self._ctx, k, out.ctypes.data, out.nbytes, None))
+        return out

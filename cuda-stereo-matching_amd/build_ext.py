@@ -1,0 +1,92 @@
+"""Build recipe of libelas_hip.so (gfx950 kernels + host engine + C ABI).
+
+  * csrc/*.hip      hipcc --offload-arch=gfx950, FP contraction off, packed
+                    fp32 ops off;
+  * csrc/*.cpp      g++ against the HIP runtime headers (host code only);
+  * Triangle        the reference's src/triangle.cpp, compiled where it lies
+                    (never copied), ELAS_TRI_INSTANCES times, each under its
+                    own namespace elas_triK (-DTRILIB=elas_triK) so that each
+                    host worker owns an independent, non-shared Triangle state;
+linked by hipcc into lib/libelas_hip.so (in-tree, loaded by ctypes).
+Without the reference checkout an existing library is kept as it is.
+"""
+import concurrent.futures as cf
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+BUILD = os.path.join(HERE, "_build")
+LIBDIR = os.path.join(HERE, "lib")
+LIB = os.path.join(LIBDIR, "libelas_hip.so")
+REF_SRC = "/root/reference/src"
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
+TRI_INSTANCES = 16
+
+HIP_SOURCES = ["k_desc.hip", "k_support.hip", "k_dense.hip"]
+HOST_SOURCES = ["elas_engine.cpp", "elas_capi.cpp", "Elas.cpp"]
+HIPFLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",
+            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+            "-I", CSRC, "-I", os.path.join(ROOT, "include")]
+HOSTFLAGS = ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-D__HIP_PLATFORM_AMD__",
+             "-I", os.path.join(ROCM, "include"), "-I", CSRC, "-I", os.path.join(ROOT, "include")]
+
+
+def _cmds():
+    cmds, objs = [], []
+    for s in HIP_SOURCES:
+        o = os.path.join(BUILD, s + ".o")
+        cmds.append(["hipcc"] + HIPFLAGS + ["-c", os.path.join(CSRC, s), "-o", o])
+        objs.append(o)
+    for s in HOST_SOURCES:
+        o = os.path.join(BUILD, s + ".o")
+        cmds.append(["g++"] + HOSTFLAGS + ["-c", os.path.join(CSRC, s), "-o", o])
+        objs.append(o)
+    o = os.path.join(BUILD, "elas_triangle.cpp.o")
+    cmds.append(["g++"] + HOSTFLAGS + ["-I", REF_SRC, "-c", os.path.join(CSRC, "elas_triangle.cpp"), "-o", o])
+    objs.append(o)
+    for k in range(TRI_INSTANCES):
+        o = os.path.join(BUILD, "triangle_%d.o" % k)
+        # Triangle's exact predicates need strict IEEE float: no contraction, no fast-math.
+        cmds.append(["g++", "-O2", "-ffp-contract=off", "-fPIC", "-w", "-DTRILIB=elas_tri%d" % k, "-c",
+                     os.path.join(REF_SRC, "triangle.cpp"), "-o", o])
+        objs.append(o)
+    return cmds, objs
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    return cmd, r.returncode, r.stdout + r.stderr
+
+
+def build(verbose=False, jobs=None):
+    """Compile and link; returns the library path."""
+    if not os.path.isfile(os.path.join(REF_SRC, "triangle.cpp")):
+        if os.path.isfile(LIB):
+            return LIB
+        raise RuntimeError("reference checkout missing and no prebuilt library at " + LIB)
+    os.makedirs(BUILD, exist_ok=True)
+    os.makedirs(LIBDIR, exist_ok=True)
+    cmds, objs = _cmds()
+    jobs = jobs or min(16, os.cpu_count() or 4)
+    with cf.ThreadPoolExecutor(jobs) as ex:
+        for cmd, rc, out in ex.map(_run, cmds):
+            if rc != 0:
+                raise RuntimeError("build failed: %s\n%s" % (" ".join(cmd), out))
+            if verbose and out.strip():
+                lines = [l for l in out.splitlines() if "packed-fp32-ops" not in l and l.strip()]
+                if lines:
+                    print("\n".join(lines), file=sys.stderr)
+    link = ["hipcc", "--offload-arch=gfx950", "-shared", "-fPIC", "-o", LIB + ".tmp"] + objs + [
+        "-L", os.path.join(ROCM, "lib"), "-lamdhip64", "-lpthread"]
+    cmd, rc, out = _run(link)
+    if rc != 0:
+        raise RuntimeError("link failed: %s\n%s" % (" ".join(cmd), out))
+    os.replace(LIB + ".tmp", LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose=True))

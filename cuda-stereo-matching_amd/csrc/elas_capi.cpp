@@ -1,0 +1,205 @@
+// elas_capi.cpp -- extern "C" entry points declared in include/elas_hip.h.
+#include <hip/hip_runtime_api.h>
+
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "elas_engine.h"
+#include "elas_hip.h"
+
+extern "C" {
+
+// Effective values of the reference kernels (src/Kernels/KernelsCommon.h:8-27);
+// post-processing off because the reference pipeline ends at the LR check.
+void elas_params_default(elas_params* p) {
+    if (!p) return;
+    memset(p, 0, sizeof(*p));
+    p->disp_min = 0;
+    p->disp_max = 255;
+    p->support_threshold = 0.85f;
+    p->support_texture = 10;
+    p->candidate_stepsize = 5;
+    p->incon_window_size = 5;
+    p->incon_threshold = 5;
+    p->incon_min_support = 5;
+    p->add_corners = false;
+    p->grid_size = 20;
+    p->beta = 0.02f;
+    p->gamma = 3.0f;
+    p->sigma = 1.0f;
+    p->sradius = 2.0f;
+    p->match_texture = 0;
+    p->lr_threshold = 2;
+    p->speckle_sim_threshold = 1.0f;
+    p->speckle_size = 0;
+    p->ipol_gap_width = 0;
+    p->filter_median = false;
+    p->filter_adaptive_mean = false;
+    p->postprocess_only_left = false;
+    p->subsampling = false;
+}
+
+// The initialisers written in Elas::parameters() (src/Elas.h:42-66), which
+// the reference kernels never read.
+void elas_params_declared(elas_params* p) {
+    if (!p) return;
+    memset(p, 0, sizeof(*p));
+    p->disp_min = 0;
+    p->disp_max = 255;
+    p->support_threshold = 0.95f;
+    p->support_texture = 10;
+    p->candidate_stepsize = 5;
+    p->incon_window_size = 5;
+    p->incon_threshold = 5;
+    p->incon_min_support = 5;
+    p->add_corners = true;
+    p->grid_size = 20;
+    p->beta = 0.02f;
+    p->gamma = 5.0f;
+    p->sigma = 1.0f;
+    p->sradius = 3.0f;
+    p->match_texture = 0;
+    p->lr_threshold = 2;
+    p->speckle_sim_threshold = 1.0f;
+    p->speckle_size = 200;
+    p->ipol_gap_width = 5000;
+    p->filter_median = true;
+    p->filter_adaptive_mean = false;
+    p->postprocess_only_left = false;
+    p->subsampling = false;
+}
+
+int elas_hip_create(const elas_params* params, int device, elas_ctx** out) {
+    if (!out) return ELAS_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+        (void)hipGetLastError();
+        return ELAS_ERR_NO_DEVICE;
+    }
+    if (device < 0 || device >= ndev) return ELAS_ERR_NO_DEVICE;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return ELAS_ERR_NO_DEVICE;
+    if (strncmp(prop.gcnArchName, "gfx950", 6) != 0) return ELAS_ERR_NO_DEVICE;
+    elas_ctx* c = new (std::nothrow) elas_ctx();
+    if (!c) return ELAS_ERR_OUT_OF_MEMORY;
+    c->device = device;
+    c->num_cu = prop.multiProcessorCount;
+    elas_params p;
+    if (params) p = *params;
+    else elas_params_default(&p);
+    int st = elas_make_kparams(&p, &c->kp, &c->err);
+    if (st != ELAS_OK) {
+        delete c;
+        return st;
+    }
+    c->prm = p;
+    if (hipSetDevice(device) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking) != hipSuccess) {
+        (void)hipGetLastError();
+        delete c;
+        return ELAS_ERR_HIP;
+    }
+    for (auto& e : c->ev)
+        if (hipEventCreate(&e) != hipSuccess) {
+            (void)hipGetLastError();
+            delete c;
+            return ELAS_ERR_HIP;
+        }
+    *out = c;
+    return ELAS_OK;
+}
+
+int elas_hip_destroy(elas_ctx* c) {
+    if (!c) return ELAS_ERR_INVALID_ARGUMENT;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->stream2) (void)hipStreamSynchronize(c->stream2);
+    for (auto& e : c->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (c->arena) (void)hipFree(c->arena);
+    if (c->table) (void)hipFree(c->table);
+    if (c->pin) (void)hipHostFree(c->pin);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    if (c->stream2) (void)hipStreamDestroy(c->stream2);
+    delete c;
+    return ELAS_OK;
+}
+
+int elas_hip_set_params(elas_ctx* c, const elas_params* p) {
+    if (!c || !p) return ELAS_ERR_INVALID_ARGUMENT;
+    ElasKParams kp;
+    int st = elas_make_kparams(p, &kp, &c->err);
+    if (st != ELAS_OK) return st;
+    c->prm = *p;
+    c->kp = kp;
+    return ELAS_OK;
+}
+
+const char* elas_hip_last_error(const elas_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+void* elas_hip_stream(elas_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+int elas_hip_process(elas_ctx* c, const uint8_t* left, const uint8_t* right, float* D1, float* D2,
+                     const int32_t* dims) {
+    if (!c || !dims) return ELAS_ERR_INVALID_ARGUMENT;
+    ElasPairSpec s{dims[0], dims[1], c->prm.disp_max, left, right, D1, D2, false, false};
+    return c->run(&s, 1, nullptr);
+}
+
+int elas_hip_process_device(elas_ctx* c, const uint8_t* d_left, const uint8_t* d_right, float* d_D1,
+                            float* d_D2, int32_t width, int32_t height) {
+    if (!c) return ELAS_ERR_INVALID_ARGUMENT;
+    ElasPairSpec s{width, height, c->prm.disp_max, d_left, d_right, d_D1, d_D2, true, true};
+    return c->run(&s, 1, nullptr);
+}
+
+int elas_hip_process_batch(elas_ctx* c, int32_t n, const uint8_t* const* left, const uint8_t* const* right,
+                           float* const* D1, float* const* D2, const int32_t* dims, int32_t* statuses) {
+    if (!c || n <= 0 || !left || !right || !D1 || !D2 || !dims) return ELAS_ERR_INVALID_ARGUMENT;
+    std::vector<ElasPairSpec> v(n);
+    for (int i = 0; i < n; i++) {
+        const int dm = dims[3 * i + 2] < 0 ? c->prm.disp_max : dims[3 * i + 2];
+        v[i] = ElasPairSpec{dims[3 * i], dims[3 * i + 1], dm, left[i], right[i], D1[i], D2[i], false, false};
+    }
+    return c->run(v.data(), n, statuses);
+}
+
+int elas_hip_set_debug(elas_ctx* c, int keep) {
+    if (!c) return ELAS_ERR_INVALID_ARGUMENT;
+    c->debug = keep != 0;
+    return ELAS_OK;
+}
+
+int elas_hip_debug_fetch(elas_ctx* c, int stage, void* dst, size_t bytes, size_t* stage_bytes) {
+    if (!c) return ELAS_ERR_INVALID_ARGUMENT;
+    return c->fetch(stage, dst, bytes, stage_bytes);
+}
+
+int elas_hip_shape_info(const elas_ctx* c, elas_shape_info* info) {
+    if (!c || !info) return ELAS_ERR_INVALID_ARGUMENT;
+    memset(info, 0, sizeof(*info));
+    if (c->pairs.empty()) return ELAS_ERR_INVALID_ARGUMENT;
+    const ElasPair& P = c->pairs[0];
+    info->width = P.W;
+    info->height = P.H;
+    info->can_width = P.cw;
+    info->can_height = P.ch;
+    info->grid_width = P.gw;
+    info->grid_height = P.gh;
+    info->num_support = c->nsp[0];
+    info->num_tri0 = c->ntri0[0];
+    info->num_tri1 = c->ntri1[0];
+    info->status = c->status[0];
+    return ELAS_OK;
+}
+
+int elas_hip_stage_times(const elas_ctx* c, float* ms, int32_t n) {
+    if (!c || !ms) return ELAS_ERR_INVALID_ARGUMENT;
+    for (int i = 0; i < n && i < ELAS_T_COUNT; i++) ms[i] = c->times[i];
+    return ELAS_OK;
+}
+
+}  // extern "C"

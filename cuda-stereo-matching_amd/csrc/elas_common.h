@@ -1,0 +1,72 @@
+// elas_common.h -- data layout shared by the host engine and the gfx950 kernels.
+//
+// Every kernel works on a batch of stereo pairs: a device array of ElasPair
+// records (one per pair, pointing into the context's HBM arena) and, per
+// stage, an inclusive-exclusive prefix of tile counts.  A workgroup walks the
+// flattened tile list with a grid stride (a persistent kernel when the grid is
+// smaller than the tile count) and finds its pair by binary search on the
+// prefix.  A single Elas::process call is a batch of one.
+#pragma once
+#include <stdint.h>
+
+#define ELAS_MAX_P 16        // prior table entries: |d - d_plane| in [0, 15]
+#define ELAS_DENSE_ROWS 3    // image rows per dense tile (a wave = 20 px x 3 rows)
+#define ELAS_DENSE_CELLS 16  // grid cells per dense tile (one wave per cell)
+#define ELAS_MAX_DISP 511    // disp_max envelope (dense key packs d in 10 bits)
+
+// Per-triangle raster record (16 floats): plane and span edges, computed once.
+//   [0..2] plane a, b, c    [3..8] A_u, A_v, B_u, B_v, C_u, C_v (sorted by u)
+//   [9..14] AB_a, AB_b, AC_a, AC_b, BC_a, BC_b         [15] unused
+#define ELAS_TRIREC 16
+
+struct ElasPair {
+    int32_t W, H, dmin, dmax;
+    int32_t cw, ch, gw, gh;
+    int32_t words;           // 32-bit words of a cell's disparity bitmask: (dmax+32)/32
+    int32_t step, gsize;     // candidate_stepsize, grid_size
+    int32_t ntri[2];         // triangles of the left / right mesh (host-known)
+    int32_t nblk;            // 256-cell blocks of the support lattice
+    const uint8_t* img[2];   // [H][W]
+    uint8_t* desc[2];        // [H][W][16]
+    int32_t* can[4];         // [ch][cw]: left pass, LR pass, inconsistency, final
+    int32_t* blkcnt;         // [nblk] valid support points per 256-cell block
+    float* sp[2];            // [cw*ch][2] compacted support points (left, right)
+    int32_t* nsp;            // [1]
+    uint32_t* gflag[2];      // [gh*gw][words] scattered disparity flags
+    uint32_t* gmask[2];      // [gh*gw][words] 3x3-dilated flags (interior cells)
+    int32_t* glist[2];       // [gh*gw][dmax+2] reference list layout (debug only)
+    const int32_t* tri[2];   // [ntri][3]
+    float* trirec[2];        // [ntri][ELAS_TRIREC]
+    int32_t* idmap[2];       // [H][W] covering triangle (highest index), -1 if none
+    float* mean[2];          // [H][W] plane prior (debug only)
+    float* draw[2];          // [H][W] dense disparity before the LR check
+    float* dout[2];          // [H][W] final disparity
+};
+
+// Uniform parameters of a launch (from elas_params).
+struct ElasKParams {
+    float support_threshold;
+    int32_t support_texture;
+    int32_t incon_window, incon_threshold, incon_min_support;
+    int32_t lr_threshold;
+    int32_t plane_radius;
+    int32_t match_texture;
+    int32_t key_bias;        // added to dense costs so keys stay non-negative
+    int32_t all_p_negative;  // every prior P(k), k <= plane_radius, is < 0
+    int32_t P[ELAS_MAX_P];
+};
+
+// Launch a stage over a batch: pairs/prefix live in device memory, prefix has
+// npairs+1 entries, total = prefix[npairs].
+struct ElasLaunch {
+    const ElasPair* pairs;
+    const int32_t* prefix;
+    int32_t npairs;
+    int32_t total;
+};
+
+// Stage ids (one prefix table per stage).
+enum ElasStageId {
+    S_DESC = 0, S_SUPPORT, S_INCON, S_REDUN_H, S_REDUN_V, S_BLOCKS, S_GRID_CELLS,
+    S_PLANE, S_RASTER, S_DENSE, S_PIXELS, S_GRID_FILL, S_NUM_STAGES
+};

@@ -1,0 +1,46 @@
+// elas_device.h -- device helpers shared by the gfx950 kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "elas_common.h"
+
+// Pair owning flattened tile t: the last p with prefix[p] <= t.  t is
+// workgroup-uniform, so the search runs on the scalar unit.
+__device__ __forceinline__ int elas_find_pair(const int32_t* __restrict__ prefix, int np, int t) {
+    int lo = 0, hi = np - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (prefix[mid] <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Sum of |a_i - b_i| over 16 bytes, accumulated onto acc (4 x v_sad_u8).
+__device__ __forceinline__ uint32_t elas_sad16(const uint4 a, const uint4 b, uint32_t acc) {
+    acc = __builtin_amdgcn_sad_u8(a.x, b.x, acc);
+    acc = __builtin_amdgcn_sad_u8(a.y, b.y, acc);
+    acc = __builtin_amdgcn_sad_u8(a.z, b.z, acc);
+    acc = __builtin_amdgcn_sad_u8(a.w, b.w, acc);
+    return acc;
+}
+
+// float -> int32 / uint32 with saturation and NaN -> 0, written out so the
+// result does not depend on how the compiler treats out-of-range casts.
+__device__ __forceinline__ int32_t elas_f2i_sat(float x) {
+    if (x != x) return 0;
+    if (x >= 2147483648.0f) return INT32_MAX;
+    if (x <= -2147483648.0f) return INT32_MIN;
+    return (int32_t)x;
+}
+__device__ __forceinline__ uint32_t elas_f2u_sat(float x) {
+    if (x != x || x <= 0.0f) return 0u;
+    if (x >= 4294967296.0f) return 0xffffffffu;
+    return (uint32_t)x;
+}
+
+// floor(x / g) for g > 0 (the reference evaluates floor((float)x / g), equal
+// for |x| < 2^20).
+__device__ __forceinline__ int elas_floordiv(int x, int g) {
+    return x >= 0 ? x / g : -((-x + g - 1) / g);
+}

@@ -1,0 +1,23 @@
+// elas_launch.h -- host entry points of the gfx950 kernels (defined in the
+// .hip files, called by the host engine).
+#pragma once
+#include <hip/hip_runtime_api.h>
+#include <stddef.h>
+#include "elas_common.h"
+
+void elas_launch_descriptor(const ElasLaunch& L, int grid, hipStream_t s);
+void elas_launch_support(const ElasLaunch& L, const ElasKParams& kp, bool right, int grid, hipStream_t s);
+void elas_launch_incon(const ElasLaunch& L, const ElasKParams& kp, int grid, hipStream_t s);
+void elas_launch_redun(const ElasLaunch& L, bool vertical, int grid, hipStream_t s);
+void elas_launch_blkcount(const ElasLaunch& L, int grid, hipStream_t s);
+void elas_launch_compact(const ElasLaunch& L, int grid, hipStream_t s);
+void elas_launch_grid_fill(const ElasLaunch& L, int grid, hipStream_t s);
+void elas_launch_grid_scatter(const ElasLaunch& L, int grid, hipStream_t s);
+void elas_launch_grid_dilate(const ElasLaunch& L, int grid, hipStream_t s);
+void elas_launch_grid_list(const ElasLaunch& L, int grid, hipStream_t s);
+void elas_launch_plane(const ElasLaunch& L, int grid, hipStream_t s);
+void elas_launch_fill_idmap(const ElasLaunch& L, int grid, hipStream_t s);
+void elas_launch_raster(const ElasLaunch& L, int grid, hipStream_t s);
+void elas_launch_mean(const ElasLaunch& L, int grid, hipStream_t s);
+void elas_launch_dense(const ElasLaunch& L, const ElasKParams& kp, int grid, size_t lds, hipStream_t s);
+void elas_launch_lr(const ElasLaunch& L, const ElasKParams& kp, int grid, hipStream_t s);

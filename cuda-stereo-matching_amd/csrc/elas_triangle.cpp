@@ -1,0 +1,179 @@
+// elas_triangle.cpp -- host Delaunay bridge over the reference's Triangle.
+//
+// The reference triangulates on the host with TRILIB::triangulate("zQB")
+// (src/Elas.cu:282-328).  Triangle keeps its state in namespace-scope
+// globals (src/triangle.cpp:545-554, the RNG reset at :4034), so one instance
+// is not reentrant.  The build compiles the reference's triangle.cpp
+// unmodified ELAS_TRI_INSTANCES times with -DTRILIB=elas_triK: each copy gets
+// its own namespace and globals, and worker thread K of the pool below uses
+// only copy K.  Meshes are identical to a single serial Triangle because every
+// call starts from the same reset state.
+#include <condition_variable>
+#include <cstdlib>
+#include <cstring>
+#include <deque>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "elas_triangle.h"
+
+#define TRILIB elas_tri0
+#include "triangle.h"
+#undef TRILIB
+#define TRILIB elas_tri1
+#include "triangle.h"
+#undef TRILIB
+#define TRILIB elas_tri2
+#include "triangle.h"
+#undef TRILIB
+#define TRILIB elas_tri3
+#include "triangle.h"
+#undef TRILIB
+#define TRILIB elas_tri4
+#include "triangle.h"
+#undef TRILIB
+#define TRILIB elas_tri5
+#include "triangle.h"
+#undef TRILIB
+#define TRILIB elas_tri6
+#include "triangle.h"
+#undef TRILIB
+#define TRILIB elas_tri7
+#include "triangle.h"
+#undef TRILIB
+#define TRILIB elas_tri8
+#include "triangle.h"
+#undef TRILIB
+#define TRILIB elas_tri9
+#include "triangle.h"
+#undef TRILIB
+#define TRILIB elas_tri10
+#include "triangle.h"
+#undef TRILIB
+#define TRILIB elas_tri11
+#include "triangle.h"
+#undef TRILIB
+#define TRILIB elas_tri12
+#include "triangle.h"
+#undef TRILIB
+#define TRILIB elas_tri13
+#include "triangle.h"
+#undef TRILIB
+#define TRILIB elas_tri14
+#include "triangle.h"
+#undef TRILIB
+#define TRILIB elas_tri15
+#include "triangle.h"
+#undef TRILIB
+
+namespace {
+
+// "zQB": zero-based indices, quiet, no boundary markers (Elas.cu:315-317).
+#define ELAS_TRI_RUN(NS)                                                        \
+    int run_##NS(const float* pts, int n, std::vector<int32_t>& out) {          \
+        NS::triangulateio in, o;                                                \
+        memset(&in, 0, sizeof(in));                                             \
+        memset(&o, 0, sizeof(o));                                               \
+        float* copy = (float*)malloc(sizeof(float) * 2 * (size_t)n);            \
+        if (!copy) return -1;                                                   \
+        memcpy(copy, pts, sizeof(float) * 2 * (size_t)n);                       \
+        in.numberofpoints = n;                                                  \
+        in.pointlist = copy;                                                    \
+        char sw[] = "zQB";                                                      \
+        NS::triangulate(sw, &in, &o, nullptr);                                  \
+        const int nt = o.numberoftriangles;                                     \
+        out.assign(o.trianglelist, o.trianglelist + 3 * (size_t)nt);            \
+        free(copy);                                                             \
+        free(o.pointlist);                                                      \
+        free(o.trianglelist);                                                   \
+        return nt;                                                              \
+    }
+
+ELAS_TRI_RUN(elas_tri0) ELAS_TRI_RUN(elas_tri1) ELAS_TRI_RUN(elas_tri2) ELAS_TRI_RUN(elas_tri3)
+ELAS_TRI_RUN(elas_tri4) ELAS_TRI_RUN(elas_tri5) ELAS_TRI_RUN(elas_tri6) ELAS_TRI_RUN(elas_tri7)
+ELAS_TRI_RUN(elas_tri8) ELAS_TRI_RUN(elas_tri9) ELAS_TRI_RUN(elas_tri10) ELAS_TRI_RUN(elas_tri11)
+ELAS_TRI_RUN(elas_tri12) ELAS_TRI_RUN(elas_tri13) ELAS_TRI_RUN(elas_tri14) ELAS_TRI_RUN(elas_tri15)
+
+typedef int (*TriFn)(const float*, int, std::vector<int32_t>&);
+const TriFn kTri[ELAS_TRI_INSTANCES] = {
+    run_elas_tri0, run_elas_tri1, run_elas_tri2, run_elas_tri3, run_elas_tri4, run_elas_tri5,
+    run_elas_tri6, run_elas_tri7, run_elas_tri8, run_elas_tri9, run_elas_tri10, run_elas_tri11,
+    run_elas_tri12, run_elas_tri13, run_elas_tri14, run_elas_tri15};
+
+int run_instance(int k, const ElasTriJob& j) {
+    if (j.n < 3) {  // Triangle would exit(1) (src/triangle.cpp:7652-7654)
+        j.out->clear();
+        return -1;
+    }
+    return kTri[k](j.pts, j.n, *j.out);
+}
+
+// Worker K owns Triangle copy K.  Jobs of one submit() are spread over the
+// workers; submit() returns when all of them are done.
+class TriPool {
+public:
+    static TriPool& get() {
+        static TriPool* pool = new TriPool();  // lives until process exit
+        return *pool;
+    }
+    void run(ElasTriJob* jobs, int n, int max_workers) {
+        if (n <= 0) return;
+        if (n == 1 || max_workers <= 1) {
+            std::lock_guard<std::mutex> g(serial_);
+            for (int i = 0; i < n; i++) jobs[i].result = run_instance(0, jobs[i]);
+            return;
+        }
+        Batch b;
+        b.remaining = n;
+        {
+            std::lock_guard<std::mutex> g(m_);
+            ensure_workers();
+            for (int i = 0; i < n; i++) q_.push_back({&jobs[i], &b});
+        }
+        cv_.notify_all();
+        std::unique_lock<std::mutex> g(b.m);
+        b.cv.wait(g, [&] { return b.remaining == 0; });
+    }
+
+private:
+    struct Batch {
+        std::mutex m;
+        std::condition_variable cv;
+        int remaining = 0;
+    };
+    struct Item {
+        ElasTriJob* job;
+        Batch* batch;
+    };
+    void ensure_workers() {
+        if (!threads_.empty()) return;
+        for (int k = 1; k < ELAS_TRI_INSTANCES; k++) threads_.emplace_back([this, k] { loop(k); });
+    }
+    void loop(int k) {
+        for (;;) {
+            Item it;
+            {
+                std::unique_lock<std::mutex> g(m_);
+                cv_.wait(g, [&] { return !q_.empty(); });
+                it = q_.front();
+                q_.pop_front();
+            }
+            it.job->result = run_instance(k, *it.job);
+            {
+                std::lock_guard<std::mutex> g(it.batch->m);
+                if (--it.batch->remaining == 0) it.batch->cv.notify_all();
+            }
+        }
+    }
+    std::mutex m_, serial_;
+    std::condition_variable cv_;
+    std::deque<Item> q_;
+    std::vector<std::thread> threads_;
+};
+
+}  // namespace
+
+void elas_triangulate_jobs(ElasTriJob* jobs, int n, int max_workers) {
+    TriPool::get().run(jobs, n, max_workers);
+}

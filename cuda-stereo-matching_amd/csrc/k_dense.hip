@@ -1,0 +1,322 @@
+// k_dense.hip -- plane prior, triangle rasterisation, dense matching and the
+// left/right check (gfx950).  Compiled with -ffp-contract=off; the float
+// plane and span arithmetic also uses explicit _rn intrinsics so no FMA is
+// formed.
+#include <hip/hip_runtime.h>
+#include "elas_common.h"
+#include "elas_device.h"
+
+// ---- plane solve: kerSolve, KerComputeDisparity.cu:322-375 ---------------
+// Gauss-Jordan with full pivoting in double (eps 1e-20, ties on >= keep the
+// last pivot), the solution overwriting B.
+__device__ bool elas_solve3(double A[3][3], double B[3]) {
+    const double eps = 1e-20;
+    int ipiv[3] = {0, 0, 0};
+    int irow = 0, icol = 0;
+    for (int i = 0; i < 3; i++) {
+        double big = 0.0;
+        for (int j = 0; j < 3; j++)
+            if (ipiv[j] != 1)
+                for (int k = 0; k < 3; k++)
+                    if (ipiv[k] == 0)
+                        if (fabs(A[j][k]) >= big) { big = fabs(A[j][k]); irow = j; icol = k; }
+        ++(ipiv[icol]);
+        if (irow != icol) {
+            for (int l = 0; l < 3; l++) { const double tmp = A[irow][l]; A[irow][l] = A[icol][l]; A[icol][l] = tmp; }
+            const double tb = B[irow]; B[irow] = B[icol]; B[icol] = tb;
+        }
+        if (fabs(A[icol][icol]) < eps) return false;
+        const double pivinv = 1.0 / A[icol][icol];
+        A[icol][icol] = 1.0;
+        for (int l = 0; l < 3; l++) A[icol][l] = __dmul_rn(A[icol][l], pivinv);
+        B[icol] = __dmul_rn(B[icol], pivinv);
+        for (int ll = 0; ll < 3; ll++)
+            if (ll != icol) {
+                const double dum = A[ll][icol];
+                A[ll][icol] = 0.0;
+                for (int l = 0; l < 3; l++) A[ll][l] = __dsub_rn(A[ll][l], __dmul_rn(A[icol][l], dum));
+                B[ll] = __dsub_rn(B[ll], __dmul_rn(B[icol], dum));
+            }
+    }
+    return true;
+}
+
+// ---- per-triangle record: KerComputeDisparity.cu:10-124 -------------------
+// One lane per triangle of either mesh.  Corners come from the left support
+// points sp0 for both meshes; the right mesh shifts them by -d (Elas.cu:105,
+// KerComputeDisparity.cu:63-66,95-99).
+__global__ __launch_bounds__(256) void k_plane(ElasLaunch L) {
+    for (int t = blockIdx.x; t < L.total; t += gridDim.x) {
+        const int p = elas_find_pair(L.prefix, L.npairs, t);
+        const ElasPair& P = L.pairs[p];
+        const int idx = (t - L.prefix[p]) * 256 + threadIdx.x;
+        const int n0 = P.ntri[0], n1 = P.ntri[1];
+        if (idx >= n0 + n1) continue;
+        const int img = idx >= n0;
+        const int ti = img ? idx - n0 : idx;
+        const int32_t* tr = P.tri[img] + 3 * (size_t)ti;
+        const float* sp = P.sp[0];
+        int tu[3], tv[3], td[3];
+        double A[3][3], b[3];
+        for (int i = 0; i < 3; i++) {
+            tu[i] = (int)sp[2 * tr[i]];
+            tv[i] = (int)sp[2 * tr[i] + 1];
+            td[i] = P.can[3][(tv[i] * P.cw + tu[i]) / P.step];
+            A[i][0] = img ? (double)(tu[i] - td[i]) : (double)tu[i];
+            A[i][1] = (double)tv[i];
+            A[i][2] = 1.0;
+            b[i] = (double)td[i];
+        }
+        float pa = 0.f, pb = 0.f, pc = 0.f;
+        if (elas_solve3(A, b)) { pa = (float)b[0]; pb = (float)b[1]; pc = (float)b[2]; }
+        if (img) for (int i = 0; i < 3; i++) tu[i] -= td[i];
+        for (int j = 0; j < 3; j++)
+            for (int k = 0; k < j; k++)
+                if (tu[k] > tu[j]) {
+                    int x = tu[j]; tu[j] = tu[k]; tu[k] = x;
+                    x = tv[j]; tv[j] = tv[k]; tv[k] = x;
+                }
+        const float Au = (float)tu[0], Av = (float)tv[0];
+        const float Bu = (float)tu[1], Bv = (float)tv[1];
+        const float Cu = (float)tu[2], Cv = (float)tv[2];
+        float ABa = 0.f, ACa = 0.f, BCa = 0.f;
+        if (tu[0] != tu[1]) ABa = __fdiv_rn(__fsub_rn(Av, Bv), __fsub_rn(Au, Bu));
+        if (tu[0] != tu[2]) ACa = __fdiv_rn(__fsub_rn(Av, Cv), __fsub_rn(Au, Cu));
+        if (tu[1] != tu[2]) BCa = __fdiv_rn(__fsub_rn(Bv, Cv), __fsub_rn(Bu, Cu));
+        float* rec = P.trirec[img] + (size_t)ti * ELAS_TRIREC;
+        rec[0] = pa; rec[1] = pb; rec[2] = pc;
+        rec[3] = Au; rec[4] = Av; rec[5] = Bu; rec[6] = Bv; rec[7] = Cu; rec[8] = Cv;
+        rec[9] = ABa;  rec[10] = __fsub_rn(Av, __fmul_rn(ABa, Au));
+        rec[11] = ACa; rec[12] = __fsub_rn(Av, __fmul_rn(ACa, Au));
+        rec[13] = BCa; rec[14] = __fsub_rn(Bv, __fmul_rn(BCa, Bu));
+        rec[15] = 0.f;
+    }
+}
+
+// idmap := -1 for both images (a fill kernel, so the pipeline can be captured
+// in a graph without memset nodes).
+__global__ __launch_bounds__(256) void k_fill_idmap(ElasLaunch L) {
+    for (int t = blockIdx.x; t < L.total; t += gridDim.x) {
+        const int p = elas_find_pair(L.prefix, L.npairs, t);
+        const ElasPair& P = L.pairs[p];
+        const int i = (t - L.prefix[p]) * 256 + threadIdx.x;
+        if (i >= P.W * P.H) continue;
+        P.idmap[0][i] = -1;
+        P.idmap[1][i] = -1;
+    }
+}
+
+// ---- rasterisation: KerComputeDisparity.cu:126-150 ------------------------
+// 16 lanes per triangle walk its columns; each column covers the half-open
+// row span between the long edge AC and AB (u in [A_u, B_u)) or BC
+// (u in [B_u, C_u)).  Where spans of two triangles overlap (the reference
+// races there) the highest triangle index wins via atomicMax, the same
+// result as painting the triangles in index order.
+__global__ __launch_bounds__(256) void k_raster(ElasLaunch L) {
+    const int grp = threadIdx.x >> 4, gl = threadIdx.x & 15;
+    for (int t = blockIdx.x; t < L.total; t += gridDim.x) {
+        const int p = elas_find_pair(L.prefix, L.npairs, t);
+        const ElasPair& P = L.pairs[p];
+        const int idx = (t - L.prefix[p]) * 16 + grp;
+        const int n0 = P.ntri[0], n1 = P.ntri[1];
+        if (idx >= n0 + n1) continue;
+        const int img = idx >= n0;
+        const int ti = img ? idx - n0 : idx;
+        const float* rec = P.trirec[img] + (size_t)ti * ELAS_TRIREC;
+        const int W = P.W, H = P.H;
+        const int Au = (int)rec[3], Bu = (int)rec[5], Cu = (int)rec[7];
+        const float ACa = rec[11], ACb = rec[12];
+        int32_t* id = P.idmap[img];
+        for (int part = 0; part < 2; part++) {
+            const int lo = part ? Bu : Au, hi = part ? Cu : Bu;
+            if (lo == hi) continue;
+            const float ea = part ? rec[13] : rec[9], eb = part ? rec[14] : rec[10];
+            for (int u = max(lo, 0) + gl; u < min(hi, W); u += 16) {
+                const float fu = (float)u;
+                const int v1 = (int)elas_f2u_sat(__fadd_rn(__fmul_rn(ACa, fu), ACb));
+                const int v2 = (int)elas_f2u_sat(__fadd_rn(__fmul_rn(ea, fu), eb));
+                const int vlo = max(min(v1, v2), 0), vhi = min(max(v1, v2), H);
+                for (int v = vlo; v < vhi; v++) atomicMax(&id[(size_t)v * W + u], ti);
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ int elas_plane_d(const float* rec, int u, int v) {
+    return elas_f2i_sat(__fadd_rn(__fadd_rn(__fmul_rn(rec[0], (float)u), __fmul_rn(rec[1], (float)v)), rec[2]));
+}
+
+// Debug: the reference's mean map, (float)(int)plane(u,v) where covered.
+__global__ __launch_bounds__(256) void k_mean(ElasLaunch L) {
+    for (int t = blockIdx.x; t < L.total; t += gridDim.x) {
+        const int p = elas_find_pair(L.prefix, L.npairs, t);
+        const ElasPair& P = L.pairs[p];
+        const int i = (t - L.prefix[p]) * 256 + threadIdx.x;
+        if (i >= P.W * P.H) continue;
+        const int v = i / P.W, u = i - v * P.W;
+        for (int img = 0; img < 2; img++) {
+            const int id = P.idmap[img][i];
+            P.mean[img][i] = id >= 0 ? (float)elas_plane_d(P.trirec[img] + (size_t)id * ELAS_TRIREC, u, v) : -1.0f;
+        }
+    }
+}
+
+// ---- dense matching: KerComputeDisparity.cu:156-316,378-402 ---------------
+// Tile = R image rows (R = min(64/gs, gs), inside one grid-cell row) x 16
+// grid cells; wave w owns cell gx0+w, lane = (row, column) of that cell, so
+// the cell's candidate set is wave-uniform and is walked on the scalar unit
+// as the set bits of its bitmask (ascending d).  The other image's R rows are
+// staged in LDS for the tile's disparity reach; the pixel's own 16-byte block
+// stays in registers.
+//
+// Reference order and tie rule: grid candidates outside the plane window
+// [dp-r, dp+r] ascending, then the window ascending with the prior P added,
+// strict '<'.  This equals the minimum of the key
+//   ((cost + bias) << 11) | (phase << 10) | d,   phase 0 = grid, 1 = window.
+// When every P(k) < 0, a grid candidate inside the window is dominated by its
+// own window entry (same d, lower cost), so phase 0 need not exclude the
+// window; otherwise it does.
+template <bool PNEG>
+__global__ __launch_bounds__(1024) void k_dense(ElasLaunch L, ElasKParams kp) {
+    extern __shared__ uint4 s_i2[];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int t = blockIdx.x; t < L.total; t += gridDim.x) {
+        const int p = elas_find_pair(L.prefix, L.npairs, t);
+        const ElasPair& P = L.pairs[p];
+        const int W = P.W, H = P.H, gs = P.gsize, gw = P.gw;
+        const int R = max(1, min(64 / gs, gs));
+        const int sub = (gs + R - 1) / R;
+        const int segs = (gw + ELAS_DENSE_CELLS - 1) / ELAS_DENSE_CELLS;
+        const int per_img = P.gh * sub * segs;
+        int r = t - L.prefix[p];
+        const int img = r / per_img;
+        r -= img * per_img;
+        const int gy = r / (sub * segs);
+        r -= gy * sub * segs;
+        const int k = r / segs, s = r - k * segs;
+        const int v0 = gy * gs + k * R;
+        const int rows = max(0, min(min(R, gs - k * R), H - v0));
+        const int gx0 = s * ELAS_DENSE_CELLS;
+        const int u_lo = gx0 * gs, u_hi = min((gx0 + ELAS_DENSE_CELLS) * gs, W);
+        const int c_lo = img ? u_lo : max(u_lo - P.dmax, 0);
+        const int c_hi = img ? min(u_hi + P.dmax, W) : u_hi;
+        const int ncols = max(c_hi - c_lo, 0);
+        const uint8_t* I1 = P.desc[img];
+        const uint8_t* I2 = P.desc[1 - img];
+        for (int i = threadIdx.x; i < rows * ncols; i += 1024) {
+            const int rr = i / ncols, cc = i - rr * ncols;
+            s_i2[i] = *reinterpret_cast<const uint4*>(I2 + ((size_t)(v0 + rr) * W + c_lo + cc) * 16);
+        }
+        __syncthreads();
+        const int gx = gx0 + wave;
+        if (gx < gw) {
+            const int row = lane / gs, col = lane - row * gs;
+            const int u = gx * gs + col, v = v0 + row;
+            const bool active = lane < R * gs && row < rows && u < W;
+            bool valid = active && u >= 4 && u <= W - 4 && v >= 4 && v <= H - 4;
+            int dp = 0;
+            uint4 b1 = make_uint4(0u, 0u, 0u, 0u);
+            if (valid) {
+                const int id = P.idmap[img][(size_t)v * W + u];
+                if (id < 0) valid = false;
+                else {
+                    dp = elas_plane_d(P.trirec[img] + (size_t)id * ELAS_TRIREC, u, v);
+                    if (dp <= 0) valid = false;
+                }
+            }
+            if (valid) {
+                b1 = *reinterpret_cast<const uint4*>(I1 + ((size_t)v * W + u) * 16);
+                if (kp.match_texture > 0) {
+                    const uint32_t tex = elas_sad16(b1, make_uint4(0x80808080u, 0x80808080u, 0x80808080u, 0x80808080u), 0u);
+                    if ((int)tex < kp.match_texture) valid = false;
+                }
+            }
+            const int dpmin = max(dp - kp.plane_radius, 0), dpmax = min(dp + kp.plane_radius, P.dmax);
+            const int sgn = img ? 1 : -1;
+            const int lds_row = row * ncols - c_lo;
+            uint32_t kmin = 0xffffffffu;
+            const uint32_t* mask = P.gmask[img] + (size_t)(gy * gw + gx) * P.words;
+            for (int w = 0; w < P.words; w++) {
+                uint32_t bits = __builtin_amdgcn_readfirstlane(mask[w]);
+                while (bits) {
+                    const int d = w * 32 + __builtin_ctz(bits);
+                    bits &= bits - 1u;
+                    if (valid) {
+                        const int uw = u + sgn * d;
+                        bool ok = uw >= 3 && uw <= W - 4;
+                        if (!PNEG) ok = ok && (d < dpmin || d > dpmax);
+                        if (ok) {
+                            const uint32_t sad = elas_sad16(b1, s_i2[lds_row + uw], 0u);
+                            kmin = min(kmin, ((sad + (uint32_t)kp.key_bias) << 11) | (uint32_t)d);
+                        }
+                    }
+                }
+            }
+            if (valid) {
+                for (int d = dpmin; d <= dpmax; d++) {
+                    const int uw = u + sgn * d;
+                    if (uw < 3 || uw > W - 4) continue;
+                    const uint32_t sad = elas_sad16(b1, s_i2[lds_row + uw], 0u);
+                    const uint32_t cost = (uint32_t)((int)sad + kp.P[abs(d - dp)] + kp.key_bias);
+                    kmin = min(kmin, (cost << 11) | 1024u | (uint32_t)d);
+                }
+            }
+            if (active)
+                P.draw[img][(size_t)v * W + u] = (kmin != 0xffffffffu) ? (float)(kmin & 1023u) : -1.0f;
+        }
+        __syncthreads();
+    }
+}
+
+// ---- left/right check: KerLeftRightConsistencyCheck.cu:8-66 ---------------
+// Reads the dense results (the reference's copies, Elas.cu:541-558) and
+// writes the final maps.  |D_other(u -/+ d) - d| > lr_threshold -> -1; an
+// invalid partner (-1) passes for d in {0, 1}, as in the reference.
+__global__ __launch_bounds__(256) void k_lr(ElasLaunch L, ElasKParams kp) {
+    for (int t = blockIdx.x; t < L.total; t += gridDim.x) {
+        const int p = elas_find_pair(L.prefix, L.npairs, t);
+        const ElasPair& P = L.pairs[p];
+        const int i = (t - L.prefix[p]) * 256 + threadIdx.x;
+        const int W = P.W;
+        if (i >= W * P.H) continue;
+        const int v = i / W, u = i - v * W;
+        const float* c1 = P.draw[0];
+        const float* c2 = P.draw[1];
+        const float thr = (float)kp.lr_threshold;
+        const float d1 = c1[i], d2 = c2[i];
+        const float uw1 = __fsub_rn((float)u, d1), uw2 = __fadd_rn((float)u, d2);
+        float o1 = d1, o2 = d2;
+        if (d1 >= 0.f && uw1 >= 0.f && uw1 < (float)W) {
+            if (fabsf(__fsub_rn(c2[(size_t)v * W + (int)uw1], d1)) > thr) o1 = -1.f;
+        } else {
+            o1 = -1.f;
+        }
+        if (d2 >= 0.f && uw2 >= 0.f && uw2 < (float)W) {
+            if (fabsf(__fsub_rn(c1[(size_t)v * W + (int)uw2], d2)) > thr) o2 = -1.f;
+        } else {
+            o2 = -1.f;
+        }
+        P.dout[0][i] = o1;
+        P.dout[1][i] = o2;
+    }
+}
+
+void elas_launch_plane(const ElasLaunch& L, int grid, hipStream_t s) {
+    hipLaunchKernelGGL(k_plane, dim3(grid), dim3(256), 0, s, L);
+}
+void elas_launch_fill_idmap(const ElasLaunch& L, int grid, hipStream_t s) {
+    hipLaunchKernelGGL(k_fill_idmap, dim3(grid), dim3(256), 0, s, L);
+}
+void elas_launch_raster(const ElasLaunch& L, int grid, hipStream_t s) {
+    hipLaunchKernelGGL(k_raster, dim3(grid), dim3(256), 0, s, L);
+}
+void elas_launch_mean(const ElasLaunch& L, int grid, hipStream_t s) {
+    hipLaunchKernelGGL(k_mean, dim3(grid), dim3(256), 0, s, L);
+}
+void elas_launch_dense(const ElasLaunch& L, const ElasKParams& kp, int grid, size_t lds, hipStream_t s) {
+    if (kp.all_p_negative) hipLaunchKernelGGL(k_dense<true>, dim3(grid), dim3(1024), lds, s, L, kp);
+    else hipLaunchKernelGGL(k_dense<false>, dim3(grid), dim3(1024), lds, s, L, kp);
+}
+void elas_launch_lr(const ElasLaunch& L, const ElasKParams& kp, int grid, hipStream_t s) {
+    hipLaunchKernelGGL(k_lr, dim3(grid), dim3(256), 0, s, L, kp);
+}

@@ -1,0 +1,87 @@
+// k_desc.hip -- fused Sobel + 16-byte descriptor (gfx950).
+//
+// Restates src/Kernels/KerDescriptor.cu:9-100 (kerComputeSobel and
+// kerComputeDescriptor) as one kernel: a 64x16 output tile stages its
+// (64+6)x(16+6) image window in LDS, computes du/dv for the (64+4)x(16+4)
+// window in LDS, and writes one 16-byte descriptor per pixel with a
+// coalesced dwordx4 store (a wave writes 1 KiB contiguous).  du/dv never go
+// to HBM: algorithmic traffic is 1 B read + 16 B written per pixel.
+//
+// du = 128 + clamp(S_u >> 2, -128, 127) is the integer form of the
+// reference's (uint8_t)(min(max(-128.f, S/4), 127.f) + 128)
+// (KerDescriptor.cu:40-41): S/4 is exact in float and truncating the
+// non-negative sum is floor.  Descriptor bytes outside u in [3, W-4],
+// v in [3, H-4] are written as 0 (the reference leaves them unwritten).
+#include <hip/hip_runtime.h>
+#include "elas_common.h"
+#include "elas_device.h"
+
+#define DT_W 64
+#define DT_H 16
+
+__global__ __launch_bounds__(256) void k_descriptor(ElasLaunch L) {
+    __shared__ uint8_t s_img[DT_H + 6][DT_W + 8];
+    __shared__ uint8_t s_du[DT_H + 4][DT_W + 4];
+    __shared__ uint8_t s_dv[DT_H + 2][DT_W + 2];
+    const int tid = threadIdx.x;
+    for (int t = blockIdx.x; t < L.total; t += gridDim.x) {
+        const int p = elas_find_pair(L.prefix, L.npairs, t);
+        const ElasPair& P = L.pairs[p];
+        const int W = P.W, H = P.H;
+        const int local = t - L.prefix[p];
+        const int tx = (W + DT_W - 1) / DT_W, ty = (H + DT_H - 1) / DT_H;
+        const int img = local / (tx * ty);
+        const int r = local - img * tx * ty;
+        const int u0 = (r % tx) * DT_W, v0 = (r / tx) * DT_H;
+        const uint8_t* I = P.img[img];
+        // image window: rows v0-3 .. v0+DT_H+2, cols u0-3 .. u0+DT_W+2
+        for (int i = tid; i < (DT_H + 6) * (DT_W + 6); i += 256) {
+            const int rr = i / (DT_W + 6), cc = i - rr * (DT_W + 6);
+            const int u = u0 - 3 + cc, v = v0 - 3 + rr;
+            s_img[rr][cc] = (u >= 0 && u < W && v >= 0 && v < H) ? I[(size_t)v * W + u] : 0;
+        }
+        __syncthreads();
+        // du on rows v0-2.., cols u0-2..; dv on rows v0-1.., cols u0-1..
+        for (int i = tid; i < (DT_H + 4) * (DT_W + 4); i += 256) {
+            const int rr = i / (DT_W + 4), cc = i - rr * (DT_W + 4);
+            const int cr = rr + 1, c = cc + 1;  // centre in s_img
+            const int su = (int)s_img[cr - 1][c - 1] + 2 * (int)s_img[cr][c - 1] + (int)s_img[cr + 1][c - 1]
+                         - (int)s_img[cr - 1][c + 1] - 2 * (int)s_img[cr][c + 1] - (int)s_img[cr + 1][c + 1];
+            s_du[rr][cc] = (uint8_t)(128 + min(max(su >> 2, -128), 127));
+        }
+        for (int i = tid; i < (DT_H + 2) * (DT_W + 2); i += 256) {
+            const int rr = i / (DT_W + 2), cc = i - rr * (DT_W + 2);
+            const int cr = rr + 2, c = cc + 2;
+            const int sv = (int)s_img[cr - 1][c - 1] + 2 * (int)s_img[cr - 1][c] + (int)s_img[cr - 1][c + 1]
+                         - (int)s_img[cr + 1][c - 1] - 2 * (int)s_img[cr + 1][c] - (int)s_img[cr + 1][c + 1];
+            s_dv[rr][cc] = (uint8_t)(128 + min(max(sv >> 2, -128), 127));
+        }
+        __syncthreads();
+        uint8_t* D = P.desc[img];
+        for (int k = 0; k < (DT_W * DT_H) / 256; k++) {
+            const int i = tid + k * 256;
+            const int pr = i / DT_W, pc = i - pr * DT_W;
+            const int u = u0 + pc, v = v0 + pr;
+            if (u >= W || v >= H) continue;
+            uint4 q = make_uint4(0u, 0u, 0u, 0u);
+            if (u >= 3 && u <= W - 4 && v >= 3 && v <= H - 4) {
+                const int a = pr + 2, b = pc + 2;  // (u, v) in s_du
+                const int e = pr + 1, f = pc + 1;  // (u, v) in s_dv
+                q.x = (uint32_t)s_du[a - 2][b] | ((uint32_t)s_du[a - 1][b - 2] << 8)
+                    | ((uint32_t)s_du[a - 1][b] << 16) | ((uint32_t)s_du[a - 1][b + 2] << 24);
+                q.y = (uint32_t)s_du[a][b - 1] | ((uint32_t)s_du[a][b] << 8)
+                    | ((uint32_t)s_du[a][b] << 16) | ((uint32_t)s_du[a][b + 1] << 24);
+                q.z = (uint32_t)s_du[a + 1][b - 2] | ((uint32_t)s_du[a + 1][b] << 8)
+                    | ((uint32_t)s_du[a + 1][b + 2] << 16) | ((uint32_t)s_du[a + 2][b] << 24);
+                q.w = (uint32_t)s_dv[e - 1][f] | ((uint32_t)s_dv[e][f - 1] << 8)
+                    | ((uint32_t)s_dv[e][f + 1] << 16) | ((uint32_t)s_dv[e + 1][f] << 24);
+            }
+            *reinterpret_cast<uint4*>(D + ((size_t)v * W + u) * 16) = q;
+        }
+        __syncthreads();
+    }
+}
+
+void elas_launch_descriptor(const ElasLaunch& L, int grid, hipStream_t s) {
+    hipLaunchKernelGGL(k_descriptor, dim3(grid), dim3(256), 0, s, L);
+}

@@ -1,0 +1,332 @@
+// k_support.hip -- support-point matching, its filters, compaction and the
+// disparity grid (gfx950).
+//
+// Support matching restates src/Kernels/KerSupportMatches.cu:13-192.  One
+// wave per candidate, lane l evaluates d = dmin + l + 64k: E(d) is the
+// 64-byte SAD of four 16-byte blocks at (+-2, +-2) (16 x v_sad_u8).
+//
+// The reference reduces the (best, second best) of 32 threads in thread 0,
+// which starts from its own registers and then re-reads its own entry from
+// shared memory (:154-178), so thread 0's best is counted twice.  Thread t
+// holds d with (d - dmin) % 32 == t.  The resulting decision, in closed form:
+//   m      = min E,   (r*, d*) = lexicographic min of ((d - dmin) & 31, d)
+//            over the minimisers (the first entry of the serial scan);
+//   min2   = m if r* == 0 (thread 0 holds a minimiser: duplicated), else the
+//            second smallest E counted with multiplicity;
+//   accept = (float)m < support_threshold * (float)min2,   winner d*.
+// The wave computes it with one packed key (E << 15 | r << 10 | d) and a
+// (min, second-min) pair per lane, then a 6-step xor-shuffle reduction.
+#include <hip/hip_runtime.h>
+#include "elas_common.h"
+#include "elas_device.h"
+
+template <bool RIGHT>
+__global__ __launch_bounds__(256) void k_support(ElasLaunch L, ElasKParams kp) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int t = blockIdx.x; t < L.total; t += gridDim.x) {
+        const int p = elas_find_pair(L.prefix, L.npairs, t);
+        const ElasPair& P = L.pairs[p];
+        const int local = t - L.prefix[p];
+        const int per_row = (P.cw + 3) >> 2;
+        const int v_can = local / per_row;
+        const int u_can = (local - v_can * per_row) * 4 + wave;
+        if (u_can >= P.cw) continue;
+        const int W = P.W, H = P.H;
+        const int cell = v_can * P.cw + u_can;
+        int u = u_can * P.step;
+        const int v = v_can * P.step;
+        int result = -1;
+        bool ok = (u >= 5 && u <= W - 6 && v >= 5 && v <= H - 6);
+        int d_left = 0;
+        if (ok && RIGHT) {
+            d_left = P.can[0][cell];
+            if (d_left < 0) ok = false;
+            else u -= d_left;
+        }
+        if (ok) {
+            const uint8_t* I1 = P.desc[RIGHT ? 1 : 0] + (size_t)16 * W * v;
+            const uint8_t* I2 = P.desc[RIGHT ? 0 : 1] + (size_t)16 * W * v;
+            const uint4 c = *reinterpret_cast<const uint4*>(I1 + 16 * u);
+            const uint32_t tex = elas_sad16(c, make_uint4(0x80808080u, 0x80808080u, 0x80808080u, 0x80808080u), 0u);
+            const int dminv = max(P.dmin, 0);
+            const int dmaxv = RIGHT ? min(P.dmax, W - u - 5) : min(P.dmax, u - 5);
+            if ((int)tex >= kp.support_texture && dmaxv - dminv >= 10) {
+                const long rowoff = 32L * W;
+                const uint8_t* b1 = I1 + 16 * u;
+                const uint4 r0 = *reinterpret_cast<const uint4*>(b1 - 32 - rowoff);
+                const uint4 r1 = *reinterpret_cast<const uint4*>(b1 + 32 - rowoff);
+                const uint4 r2 = *reinterpret_cast<const uint4*>(b1 - 32 + rowoff);
+                const uint4 r3 = *reinterpret_cast<const uint4*>(b1 + 32 + rowoff);
+                uint32_t kmin = 0xffffffffu, m1 = 0xffffu, m2 = 0xffffu;
+                for (int d = dminv + lane; d <= dmaxv; d += 64) {
+                    const uint8_t* b2 = I2 + 16 * (RIGHT ? u + d : u - d);
+                    uint32_t E = elas_sad16(r0, *reinterpret_cast<const uint4*>(b2 - 32 - rowoff), 0u);
+                    E = elas_sad16(r1, *reinterpret_cast<const uint4*>(b2 + 32 - rowoff), E);
+                    E = elas_sad16(r2, *reinterpret_cast<const uint4*>(b2 - 32 + rowoff), E);
+                    E = elas_sad16(r3, *reinterpret_cast<const uint4*>(b2 + 32 + rowoff), E);
+                    const uint32_t key = (E << 15) | ((uint32_t)((d - dminv) & 31) << 10) | (uint32_t)d;
+                    kmin = min(kmin, key);
+                    m2 = min(m2, max(m1, E));
+                    m1 = min(m1, E);
+                }
+                #pragma unroll
+                for (int off = 32; off >= 1; off >>= 1) {
+                    const uint32_t ok2 = __shfl_xor(kmin, off, 64);
+                    const uint32_t o1 = __shfl_xor(m1, off, 64);
+                    const uint32_t o2 = __shfl_xor(m2, off, 64);
+                    kmin = min(kmin, ok2);
+                    const uint32_t n2 = min(max(m1, o1), min(m2, o2));
+                    m1 = min(m1, o1);
+                    m2 = n2;
+                }
+                const uint32_t m = kmin >> 15;
+                const uint32_t r = (kmin >> 10) & 31u;
+                const int dstar = (int)(kmin & 1023u);
+                const uint32_t min2 = (r == 0u) ? m : m2;
+                const float thr = kp.support_threshold * (float)min2;
+                if ((float)m < thr) {
+                    if (!RIGHT) result = dstar;
+                    else result = (abs(d_left - dstar) > kp.lr_threshold) ? -1 : d_left;
+                }
+            }
+        }
+        if (lane == 0) (RIGHT ? P.can[1] : P.can[0])[cell] = result;
+    }
+}
+
+// Inconsistency filter, KerSupportMatches.cu:196-238, read from can[1]
+// (snapshot) and written to can[2].  The reference updates in place and
+// races; reading a snapshot makes the result independent of scheduling.
+// The row guard y < can_width is the reference's (:208).
+__global__ __launch_bounds__(256) void k_incon(ElasLaunch L, ElasKParams kp) {
+    for (int t = blockIdx.x; t < L.total; t += gridDim.x) {
+        const int p = elas_find_pair(L.prefix, L.npairs, t);
+        const ElasPair& P = L.pairs[p];
+        const int idx = (t - L.prefix[p]) * 256 + threadIdx.x;
+        const int cw = P.cw, ch = P.ch;
+        if (idx >= cw * ch) continue;
+        const int y = idx / cw, x = idx - y * cw;
+        const int32_t* in = P.can[1];
+        const int d = in[idx];
+        int res = d;
+        if (d >= 0 && y < cw) {
+            const int R = kp.incon_window;
+            int support = 0;
+            const int x0 = max(x - R, 0), x1 = min(x + R, cw - 1);
+            const int y0 = max(y - R, 0), y1 = min(y + R, ch - 1);
+            for (int y2 = y0; y2 <= y1; y2++)
+                for (int x2 = x0; x2 <= x1; x2++) {
+                    const int d2 = in[y2 * cw + x2];
+                    support += (d2 >= 0 && abs(d - d2) <= kp.incon_threshold) ? 1 : 0;
+                }
+            if (support < kp.incon_min_support) res = -1;
+        }
+        P.can[2][idx] = res;
+    }
+}
+
+// Redundancy filters, KerSupportMatches.cu:242-356: REDUN_MAX_DIST 5,
+// REDUN_THRESHOLD 1.  One lane per row (horizontal, can[2] -> can[3]) or per
+// column (vertical, in place on can[3]), scanned in order as in the
+// reference: a point is removed when a valid point within 5 cells on BOTH
+// sides differs by at most 1.  The already-scanned side sees updated values:
+// they sit in a 5-register window; the other side is read ahead.
+__device__ __forceinline__ bool elas_near(int a, int d) { return a >= 0 && abs(a - d) <= 1; }
+
+template <bool VERTICAL>
+__global__ __launch_bounds__(64) void k_redun(ElasLaunch L) {
+    for (int t = blockIdx.x; t < L.total; t += gridDim.x) {
+        const int p = elas_find_pair(L.prefix, L.npairs, t);
+        const ElasPair& P = L.pairs[p];
+        const int line = (t - L.prefix[p]) * 64 + threadIdx.x;
+        const int cw = P.cw, ch = P.ch;
+        const int nlines = VERTICAL ? cw : ch, len = VERTICAL ? ch : cw;
+        if (line >= nlines) continue;
+        const int stride = VERTICAL ? cw : 1;
+        const int32_t* in = VERTICAL ? P.can[3] + line : P.can[2] + (size_t)line * cw;
+        int32_t* out = VERTICAL ? P.can[3] + line : P.can[3] + (size_t)line * cw;
+        int c1 = -1, c2 = -1, c3 = -1, c4 = -1, c5 = -1;  // updated values at i-1..i-5
+        int w0 = len > 0 ? in[0] : -1;
+        int w1 = len > 1 ? in[1 * stride] : -1, w2 = len > 2 ? in[2 * stride] : -1;
+        int w3 = len > 3 ? in[3 * stride] : -1, w4 = len > 4 ? in[4 * stride] : -1;
+        int w5 = len > 5 ? in[5 * stride] : -1;
+        for (int i = 0; i < len; i++) {
+            const int w6 = (i + 6 < len) ? in[(i + 6) * stride] : -1;
+            const int d = w0;
+            int res = d;
+            if (d >= 0) {
+                const bool ls = elas_near(c1, d) || elas_near(c2, d) || elas_near(c3, d) ||
+                                elas_near(c4, d) || elas_near(c5, d);
+                const bool rs = elas_near(w1, d) || elas_near(w2, d) || elas_near(w3, d) ||
+                                elas_near(w4, d) || elas_near(w5, d);
+                if (ls && rs) res = -1;
+            }
+            out[i * stride] = res;
+            c5 = c4; c4 = c3; c3 = c2; c2 = c1; c1 = res;
+            w0 = w1; w1 = w2; w2 = w3; w3 = w4; w4 = w5; w5 = w6;
+        }
+    }
+}
+
+// Compaction, KerSupportMatches.cu:361-423 + Elas.cu:338-394: row-major
+// order.  Pass 1 counts valid points per 256-cell block; pass 2 adds the
+// counts of earlier blocks and writes sp0 = (u*step, v*step) and
+// sp1 = (u*step - d, v*step); the last block stores the total.
+__global__ __launch_bounds__(256) void k_blkcount(ElasLaunch L) {
+    for (int t = blockIdx.x; t < L.total; t += gridDim.x) {
+        const int p = elas_find_pair(L.prefix, L.npairs, t);
+        const ElasPair& P = L.pairs[p];
+        const int b = t - L.prefix[p];
+        const int idx = b * 256 + threadIdx.x;
+        const bool valid = idx < P.cw * P.ch && P.can[3][idx] >= 0;
+        const int n = __syncthreads_count(valid);
+        if (threadIdx.x == 0) P.blkcnt[b] = n;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_compact(ElasLaunch L) {
+    __shared__ int s_red[256];
+    __shared__ int s_wave[4];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    for (int t = blockIdx.x; t < L.total; t += gridDim.x) {
+        const int p = elas_find_pair(L.prefix, L.npairs, t);
+        const ElasPair& P = L.pairs[p];
+        const int b = t - L.prefix[p];
+        int acc = 0;
+        for (int j = tid; j < b; j += 256) acc += P.blkcnt[j];
+        s_red[tid] = acc;
+        __syncthreads();
+        for (int s = 128; s > 0; s >>= 1) {
+            if (tid < s) s_red[tid] += s_red[tid + s];
+            __syncthreads();
+        }
+        const int base = s_red[0];
+        const int idx = b * 256 + tid;
+        const int d = idx < P.cw * P.ch ? P.can[3][idx] : -1;
+        const bool valid = d >= 0;
+        const uint64_t mask = __ballot(valid);
+        const int before = __popcll(mask & ((1ull << lane) - 1ull));
+        if (lane == 0) s_wave[wave] = __popcll(mask);
+        __syncthreads();
+        int off = base + before;
+        for (int w = 0; w < wave; w++) off += s_wave[w];
+        if (valid) {
+            const int vc = idx / P.cw, uc = idx - vc * P.cw;
+            const float pu = (float)(uc * P.step), pv = (float)(vc * P.step);
+            reinterpret_cast<float2*>(P.sp[0])[off] = make_float2(pu, pv);
+            reinterpret_cast<float2*>(P.sp[1])[off] = make_float2(pu - (float)d, pv);
+        }
+        if (b == P.nblk - 1 && tid == 0) P.nsp[0] = base + s_wave[0] + s_wave[1] + s_wave[2] + s_wave[3];
+        __syncthreads();
+    }
+}
+
+// Disparity grid, KerGrid.cu:9-126 + Elas.cu:403-464.  Flags are bitmasks
+// (words x 32 disparities per cell): fill, scatter d-1..d+1 of each support
+// point with atomicOr, then OR the 3x3 neighbourhood of interior cells.
+// The dense kernel walks the set bits in ascending order, which is the
+// reference's ascending candidate list; k_grid_list renders that list layout
+// ([0] = count, [1..] = d, zero tail) for parity dumps.
+__global__ __launch_bounds__(256) void k_grid_fill(ElasLaunch L) {
+    for (int t = blockIdx.x; t < L.total; t += gridDim.x) {
+        const int p = elas_find_pair(L.prefix, L.npairs, t);
+        const ElasPair& P = L.pairs[p];
+        const int n = P.gw * P.gh * P.words;
+        const int i = (t - L.prefix[p]) * 256 + threadIdx.x;
+        if (i < 2 * n) P.gflag[i >= n][i >= n ? i - n : i] = 0u;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_grid_scatter(ElasLaunch L) {
+    for (int t = blockIdx.x; t < L.total; t += gridDim.x) {
+        const int p = elas_find_pair(L.prefix, L.npairs, t);
+        const ElasPair& P = L.pairs[p];
+        const int idx = (t - L.prefix[p]) * 256 + threadIdx.x;
+        if (idx >= P.cw * P.ch) continue;
+        const int d = P.can[3][idx];
+        if (d < 0) continue;
+        const int vc = idx / P.cw, uc = idx - vc * P.cw;
+        const int u_img = uc * P.step, v_img = vc * P.step;
+        const int ug0 = elas_floordiv(u_img, P.gsize);
+        const int ug1 = elas_floordiv(u_img - d, P.gsize);
+        const int vg = elas_floordiv(v_img, P.gsize);
+        if (vg < 0 || vg >= P.gh) continue;
+        const int dlo = max(d - 1, 0), dhi = min(d + 1, P.dmax);
+        for (int dd = dlo; dd <= dhi; dd++) {
+            const uint32_t bit = 1u << (dd & 31);
+            if (ug0 >= 0 && ug0 < P.gw) atomicOr(&P.gflag[0][(vg * P.gw + ug0) * P.words + (dd >> 5)], bit);
+            if (ug1 >= 0 && ug1 < P.gw) atomicOr(&P.gflag[1][(vg * P.gw + ug1) * P.words + (dd >> 5)], bit);
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_grid_dilate(ElasLaunch L) {
+    for (int t = blockIdx.x; t < L.total; t += gridDim.x) {
+        const int p = elas_find_pair(L.prefix, L.npairs, t);
+        const ElasPair& P = L.pairs[p];
+        const int ncell = P.gw * P.gh;
+        const int i = (t - L.prefix[p]) * 256 + threadIdx.x;
+        if (i >= 2 * ncell) continue;
+        const int img = i >= ncell, c = img ? i - ncell : i;
+        const int gy = c / P.gw, gx = c - gy * P.gw;
+        const bool interior = gx > 0 && gx < P.gw - 1 && gy > 0 && gy < P.gh - 1;
+        const uint32_t* f = P.gflag[img];
+        uint32_t* m = P.gmask[img] + (size_t)c * P.words;
+        for (int w = 0; w < P.words; w++) {
+            uint32_t acc = 0u;
+            if (interior)
+                for (int dy = -1; dy <= 1; dy++)
+                    for (int dx = -1; dx <= 1; dx++)
+                        acc |= f[((gy + dy) * P.gw + (gx + dx)) * P.words + w];
+            m[w] = acc;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_grid_list(ElasLaunch L) {
+    for (int t = blockIdx.x; t < L.total; t += gridDim.x) {
+        const int p = elas_find_pair(L.prefix, L.npairs, t);
+        const ElasPair& P = L.pairs[p];
+        const int ncell = P.gw * P.gh;
+        const int i = (t - L.prefix[p]) * 256 + threadIdx.x;
+        if (i >= 2 * ncell) continue;
+        const int img = i >= ncell, c = img ? i - ncell : i;
+        const uint32_t* m = P.gmask[img] + (size_t)c * P.words;
+        int32_t* g = P.glist[img] + (size_t)c * (P.dmax + 2);
+        int n = 0;
+        for (int d = 0; d <= P.dmax; d++)
+            if ((m[d >> 5] >> (d & 31)) & 1u) g[1 + n++] = d;
+        for (int k = 1 + n; k < P.dmax + 2; k++) g[k] = 0;
+        g[0] = n;
+    }
+}
+
+void elas_launch_support(const ElasLaunch& L, const ElasKParams& kp, bool right, int grid, hipStream_t s) {
+    if (right) hipLaunchKernelGGL(k_support<true>, dim3(grid), dim3(256), 0, s, L, kp);
+    else hipLaunchKernelGGL(k_support<false>, dim3(grid), dim3(256), 0, s, L, kp);
+}
+void elas_launch_incon(const ElasLaunch& L, const ElasKParams& kp, int grid, hipStream_t s) {
+    hipLaunchKernelGGL(k_incon, dim3(grid), dim3(256), 0, s, L, kp);
+}
+void elas_launch_redun(const ElasLaunch& L, bool vertical, int grid, hipStream_t s) {
+    if (vertical) hipLaunchKernelGGL(k_redun<true>, dim3(grid), dim3(64), 0, s, L);
+    else hipLaunchKernelGGL(k_redun<false>, dim3(grid), dim3(64), 0, s, L);
+}
+void elas_launch_blkcount(const ElasLaunch& L, int grid, hipStream_t s) {
+    hipLaunchKernelGGL(k_blkcount, dim3(grid), dim3(256), 0, s, L);
+}
+void elas_launch_compact(const ElasLaunch& L, int grid, hipStream_t s) {
+    hipLaunchKernelGGL(k_compact, dim3(grid), dim3(256), 0, s, L);
+}
+void elas_launch_grid_fill(const ElasLaunch& L, int grid, hipStream_t s) {
+    hipLaunchKernelGGL(k_grid_fill, dim3(grid), dim3(256), 0, s, L);
+}
+void elas_launch_grid_scatter(const ElasLaunch& L, int grid, hipStream_t s) {
+    hipLaunchKernelGGL(k_grid_scatter, dim3(grid), dim3(256), 0, s, L);
+}
+void elas_launch_grid_dilate(const ElasLaunch& L, int grid, hipStream_t s) {
+    hipLaunchKernelGGL(k_grid_dilate, dim3(grid), dim3(256), 0, s, L);
+}
+void elas_launch_grid_list(const ElasLaunch& L, int grid, hipStream_t s) {
+    hipLaunchKernelGGL(k_grid_list, dim3(grid), dim3(256), 0, s, L);
+}

@@ -1,0 +1,164 @@
+/*
+ * elas_hip.h -- C ABI of the MI355X-native ELAS stereo path.
+ *
+ * Plain pointers and sizes only: a cgo / ctypes / JNI binding needs nothing
+ * else.  Every entry point returns an elas_status (0 = success); the message
+ * of the last failure on a context is available from elas_hip_last_error().
+ * The reference has no return codes (CUDA errors and Triangle's "fewer than
+ * three points" call exit(), src/Utils.h:11-38, src/triangle.cpp:7652-7654);
+ * here a non-zero status replaces every exit().
+ *
+ * Reference interface replaced (paths relative to the reference repository):
+ *   Elas::parameters                 src/Elas.h:14-67
+ *   Elas::Elas()                     src/Elas.h:70-72
+ *   Elas::process(l, r, D1, D2, dims) src/Elas.h:76-83, src/Elas.cu:19-120
+ */
+#ifndef ELAS_HIP_H
+#define ELAS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+#ifndef __cplusplus
+#include <stdbool.h>
+#endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Same fields, types and order as Elas::parameters (src/Elas.h:15-37), so a
+ * pointer to the reference struct can be passed unchanged.  The values the
+ * reference kernels actually use are compile-time macros
+ * (src/Kernels/KernelsCommon.h:8-27); elas_params_default() returns those
+ * values, with every post-processing switch off because the reference
+ * pipeline ends at the left/right check (src/Elas.cu:107-119). */
+typedef struct elas_params {
+    int32_t disp_min;              /* min disparity                              */
+    int32_t disp_max;              /* max disparity                              */
+    float   support_threshold;     /* max. uniqueness ratio (best vs. 2nd best)  */
+    int32_t support_texture;       /* min texture for support points             */
+    int32_t candidate_stepsize;    /* step of the support-point lattice          */
+    int32_t incon_window_size;     /* half window of the inconsistency check     */
+    int32_t incon_threshold;       /* disparity similarity for consistency       */
+    int32_t incon_min_support;     /* min number of consistent support points    */
+    bool    add_corners;           /* not implemented by the reference (ignored) */
+    int32_t grid_size;             /* cell size of the disparity grid            */
+    float   beta;                  /* image likelihood parameter                 */
+    float   gamma;                 /* prior constant                             */
+    float   sigma;                 /* prior sigma                                */
+    float   sradius;               /* prior sigma radius                         */
+    int32_t match_texture;         /* min texture for dense matching             */
+    int32_t lr_threshold;          /* left/right consistency threshold           */
+    float   speckle_sim_threshold; /* speckle segmentation similarity            */
+    int32_t speckle_size;          /* max speckle size removed (0 = off)         */
+    int32_t ipol_gap_width;        /* interpolated gap width (0 = off)           */
+    bool    filter_median;         /* optional median filter                     */
+    bool    filter_adaptive_mean;  /* optional adaptive mean filter              */
+    bool    postprocess_only_left; /* post-process D1 only                       */
+    bool    subsampling;           /* not implemented by the reference (ignored) */
+} elas_params;
+
+typedef enum elas_status {
+    ELAS_OK = 0,
+    ELAS_ERR_INVALID_ARGUMENT = 1, /* bad dims, null pointer, bad parameters     */
+    ELAS_ERR_HIP = 2,              /* a HIP runtime call or launch failed        */
+    ELAS_ERR_OUT_OF_MEMORY = 3,    /* device or host allocation failed           */
+    ELAS_ERR_TOO_FEW_POINTS = 4,   /* < 3 support points: D1/D2 are all -1       */
+    ELAS_ERR_NO_DEVICE = 5,        /* no usable gfx950 device                    */
+    ELAS_ERR_UNSUPPORTED = 6,      /* request outside the supported envelope     */
+    ELAS_ERR_BUSY = 7              /* serving queue full / budget exhausted      */
+} elas_status;
+
+/* Intermediates kept by the last process call, for parity dumps
+ * (elas_hip_debug_fetch).  Layouts follow the reference buffers. */
+typedef enum elas_stage {
+    ELAS_STAGE_DESC0 = 0,      /* u8  [H][W][16] left descriptor (0 where unwritten)  */
+    ELAS_STAGE_DESC1 = 1,      /* u8  [H][W][16] right descriptor                     */
+    ELAS_STAGE_CAN_LEFT = 2,   /* i32 [ch][cw] after the left->right support pass     */
+    ELAS_STAGE_CAN_LR = 3,     /* i32 [ch][cw] after the right->left pass (LR test)   */
+    ELAS_STAGE_CAN_INCON = 4,  /* i32 [ch][cw] after the inconsistency filter         */
+    ELAS_STAGE_CAN_FINAL = 5,  /* i32 [ch][cw] after the redundancy filters           */
+    ELAS_STAGE_SP0 = 6,        /* f32 [nsp][2] left support points (u, v)             */
+    ELAS_STAGE_SP1 = 7,        /* f32 [nsp][2] right support points (u - d, v)        */
+    ELAS_STAGE_GRID0 = 8,      /* i32 [gh][gw][disp_max+2] left disparity grid        */
+    ELAS_STAGE_GRID1 = 9,      /* i32 [gh][gw][disp_max+2] right disparity grid       */
+    ELAS_STAGE_TRI0 = 10,      /* i32 [ntri0][3] left Delaunay triangles              */
+    ELAS_STAGE_TRI1 = 11,      /* i32 [ntri1][3] right Delaunay triangles             */
+    ELAS_STAGE_MEAN0 = 12,     /* f32 [H][W] left plane prior (-1 = uncovered)        */
+    ELAS_STAGE_MEAN1 = 13,     /* f32 [H][W] right plane prior                        */
+    ELAS_STAGE_D1_RAW = 14,    /* f32 [H][W] left dense disparity before the LR check */
+    ELAS_STAGE_D2_RAW = 15,    /* f32 [H][W] right dense disparity before the LR check*/
+    ELAS_STAGE_D1 = 16,        /* f32 [H][W] final left disparity                     */
+    ELAS_STAGE_D2 = 17,        /* f32 [H][W] final right disparity                    */
+    ELAS_STAGE_COUNT = 18
+} elas_stage;
+
+/* Sizes of the last processed pair (valid after a successful process call). */
+typedef struct elas_shape_info {
+    int32_t width, height;        /* image                                  */
+    int32_t can_width, can_height;/* support lattice: W/step, H/step        */
+    int32_t grid_width, grid_height; /* disparity grid: ceil(W/gs), ceil(H/gs) */
+    int32_t num_support;          /* support points after compaction        */
+    int32_t num_tri0, num_tri1;   /* Delaunay triangles left / right        */
+    int32_t status;               /* elas_status of that call               */
+} elas_shape_info;
+
+/* Per-stage device time of the last call in milliseconds (HIP events). */
+typedef enum elas_timer {
+    ELAS_T_DESCRIPTOR = 0, ELAS_T_SUPPORT = 1, ELAS_T_FILTER = 2, ELAS_T_COMPACT = 3,
+    ELAS_T_GRID = 4, ELAS_T_TRIANGLE = 5, ELAS_T_PLANE = 6, ELAS_T_DENSE = 7,
+    ELAS_T_LR = 8, ELAS_T_POST = 9, ELAS_T_TOTAL = 10, ELAS_T_COUNT = 11
+} elas_timer;
+
+typedef struct elas_ctx elas_ctx;
+
+/* Effective reference defaults (src/Kernels/KernelsCommon.h:8-27). */
+void elas_params_default(elas_params* p);
+/* The dead initialisers of src/Elas.h:42-66 (post-processing on). */
+void elas_params_declared(elas_params* p);
+
+/* Create a context on HIP device `device` (replaces Elas::Elas(),
+ * src/Elas.h:70-72).  Device buffers are pooled in the context and grow on
+ * demand; nothing is allocated per call once a shape has been seen. */
+int elas_hip_create(const elas_params* params, int device, elas_ctx** out);
+int elas_hip_destroy(elas_ctx* ctx);
+int elas_hip_set_params(elas_ctx* ctx, const elas_params* params);
+const char* elas_hip_last_error(const elas_ctx* ctx);
+/* HIP stream the context enqueues on (a hipStream_t). */
+void* elas_hip_stream(elas_ctx* ctx);
+
+/* Elas::process (src/Elas.h:76-83, src/Elas.cu:19-120): host buffers in and
+ * out, synchronous.  dims[0] = width, dims[1] = height (dims[2], the
+ * bytes-per-line of upstream libelas, is not read).  Images are W*H u8
+ * row-major; D1/D2 are W*H f32 with -1 for invalid pixels. */
+int elas_hip_process(elas_ctx* ctx, const uint8_t* left, const uint8_t* right,
+                     float* D1, float* D2, const int32_t* dims);
+
+/* Same computation on device-resident buffers (hipMalloc'd by the caller):
+ * no H2D/D2H of images or disparities.  Synchronous. */
+int elas_hip_process_device(elas_ctx* ctx, const uint8_t* d_left, const uint8_t* d_right,
+                            float* d_D1, float* d_D2, int32_t width, int32_t height);
+
+/* Batched form: n pairs of possibly different sizes and disparity ranges,
+ * each stage one launch over all pairs.  Arrays of n host pointers; dims is
+ * n*3 int32: (W, H, disp_max), disp_max < 0 meaning the context's.
+ * statuses (optional) receives one elas_status per pair; the call returns
+ * ELAS_OK when every pair either succeeded or had too few support points. */
+int elas_hip_process_batch(elas_ctx* ctx, int32_t n, const uint8_t* const* left,
+                           const uint8_t* const* right, float* const* D1, float* const* D2,
+                           const int32_t* dims, int32_t* statuses);
+
+/* Intermediates of the last process call on this context (kept only when
+ * elas_hip_set_debug(ctx, 1)).  Copies min(bytes, stage size) bytes to dst
+ * and stores the stage size in *stage_bytes (either may be NULL/0 to query). */
+int elas_hip_set_debug(elas_ctx* ctx, int keep_intermediates);
+int elas_hip_debug_fetch(elas_ctx* ctx, int stage, void* dst, size_t bytes, size_t* stage_bytes);
+int elas_hip_shape_info(const elas_ctx* ctx, elas_shape_info* info);
+/* Per-stage device milliseconds of the last call (ELAS_T_COUNT floats). */
+int elas_hip_stage_times(const elas_ctx* ctx, float* ms, int32_t n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ELAS_HIP_H */

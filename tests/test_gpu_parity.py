@@ -1,0 +1,127 @@
+"""Parity of the HIP path with the oracle, through the C ABI.
+
+Every intermediate of the reference pipeline (descriptors, support lattice
+after each pass and filter, support points, disparity grids, Delaunay
+meshes, plane prior, dense disparities, final maps) must be bit-exact.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+STAGE_KEYS = ["desc0", "desc1", "can_left", "can_lr", "can_incon", "can_final", "sp0", "sp1",
+              "grid0", "grid1", "tri0", "tri1", "mean0", "mean1", "d1_raw", "d2_raw"]
+
+
+def _compare_all(el, ref, label):
+    for k in STAGE_KEYS:
+        got = el.fetch(k)
+        exp = ref[k]
+        assert got.shape == exp.shape, (label, k, got.shape, exp.shape)
+        if not np.array_equal(got, exp):
+            diff = np.argwhere(got != exp)
+            raise AssertionError("%s: stage %s differs at %d positions, first %s (gpu %s, oracle %s)" % (
+                label, k, len(diff), diff[0].tolist(), got[tuple(diff[0])], exp[tuple(diff[0])]))
+
+
+@pytest.mark.parametrize("name", ["c0", "c1"])
+def test_every_stage_bit_exact(pkg, oracle, pairs, name):
+    left, right, _, dmax = pairs(name)
+    print("case", name, left.shape, "dmax", dmax, flush=True)
+    el = pkg.Elas(pkg.default_params(disp_max=dmax))
+    el.set_debug(True)
+    D1, D2 = el.process(left, right)
+    ref = oracle.process(left, right, oracle.params(disp_max=dmax))
+    assert el.status == ref["status"] == 0
+    info = el.shape_info()
+    assert info["num_support"] == ref["nsp"]
+    assert (info["num_tri0"], info["num_tri1"]) == (ref["ntri0"], ref["ntri1"])
+    _compare_all(el, ref, name)
+    assert np.array_equal(D1, ref["D1"]) and np.array_equal(D2, ref["D2"])
+    assert np.array_equal(el.fetch("D1"), D1)
+
+
+def test_large_c2_bit_exact(pkg, oracle, pairs):
+    """2816x1880, disparities 0..255: final maps and key intermediates."""
+    left, right, gt, dmax = pairs("c2")
+    print("case c2", left.shape, flush=True)
+    el = pkg.Elas(pkg.default_params(disp_max=dmax))
+    el.set_debug(True)
+    D1, D2 = el.process(left, right)
+    ref = oracle.process(left, right, oracle.params(disp_max=dmax))
+    _compare_all(el, ref, "c2")
+    assert np.array_equal(D1, ref["D1"]) and np.array_equal(D2, ref["D2"])
+    v = D1 >= 0
+    assert v.mean() > 0.6 and (np.abs(D1[v] - gt[v]) <= 1).mean() > 0.9
+
+
+def test_repeat_runs_bitwise_equal(pkg, pairs):
+    left, right, _, dmax = pairs("c1")
+    el = pkg.Elas(pkg.default_params(disp_max=dmax))
+    outs = [el.process(left, right) for _ in range(3)]
+    for D1, D2 in outs[1:]:
+        assert np.array_equal(D1, outs[0][0]) and np.array_equal(D2, outs[0][1])
+
+
+def test_device_resident_equals_host(pkg, pairs):
+    import torch
+    left, right, _, dmax = pairs("c0")
+    el = pkg.Elas(pkg.default_params(disp_max=dmax))
+    D1, D2 = el.process(left, right)
+    l = torch.from_numpy(left).cuda()
+    r = torch.from_numpy(right).cuda()
+    d1 = torch.empty(left.shape, dtype=torch.float32, device="cuda")
+    d2 = torch.empty_like(d1)
+    torch.cuda.synchronize()
+    el.process_device(l, r, d1, d2)
+    assert np.array_equal(d1.cpu().numpy(), D1) and np.array_equal(d2.cpu().numpy(), D2)
+
+
+def test_batch_equals_plain(pkg, synth):
+    shapes = [(160, 120, 31), (200, 90, 63), (97, 131, 15), (240, 180, 63)]
+    pairs_ = []
+    for i, (W, H, dm) in enumerate(shapes):
+        l, r, _ = synth.make_pair(W, H, dm, regions=4, seed=100 + i)
+        pairs_.append((l, r, dm))
+    el = pkg.Elas(pkg.default_params(disp_max=63))
+    batch = el.process_batch(pairs_)
+    for (l, r, dm), (b1, b2, st) in zip(pairs_, batch):
+        one = pkg.Elas(pkg.default_params(disp_max=dm))
+        D1, D2 = one.process(l, r)
+        assert st == one.status
+        assert np.array_equal(b1, D1) and np.array_equal(b2, D2)
+
+
+def test_textureless_pair_reports_too_few_points(pkg, oracle):
+    img = np.full((120, 160), 77, np.uint8)
+    el = pkg.Elas(pkg.default_params(disp_max=31))
+    D1, D2 = el.process(img, img)
+    assert el.status == pkg.ELAS_ERR_TOO_FEW_POINTS
+    assert (D1 == -1).all() and (D2 == -1).all()
+    ref = oracle.process(img, img, oracle.params(disp_max=31))
+    assert ref["status"] == pkg.ELAS_ERR_TOO_FEW_POINTS
+
+
+@pytest.mark.parametrize("W,H", [(1, 1), (11, 9), (37, 23), (64, 64)])
+def test_tiny_images(pkg, oracle, synth, W, H):
+    l, r, _ = synth.make_pair(W, H, 15, regions=2, seed=W * 1000 + H) if W > 8 and H > 8 else (
+        np.zeros((H, W), np.uint8), np.zeros((H, W), np.uint8), None)
+    el = pkg.Elas(pkg.default_params(disp_max=15))
+    D1, D2 = el.process(l, r)
+    ref = oracle.process(l, r, oracle.params(disp_max=15), dump=False)
+    assert el.status == ref["status"]
+    assert np.array_equal(D1, ref["D1"]) and np.array_equal(D2, ref["D2"])
+
+
+def test_non_default_parameters(pkg, oracle, synth):
+    """Every parameter the kernels read is honoured (the reference hardwires macros)."""
+    l, r, _ = synth.make_pair(300, 200, 47, regions=5, seed=11)
+    kw = dict(disp_max=47, support_threshold=0.9, support_texture=6, candidate_stepsize=4,
+              incon_window_size=4, incon_threshold=4, incon_min_support=4, grid_size=16,
+              gamma=4.0, sradius=3.0, lr_threshold=1, disp_min=2)
+    el = pkg.Elas(pkg.default_params(**kw))
+    el.set_debug(True)
+    D1, D2 = el.process(l, r)
+    ref = oracle.process(l, r, oracle.params(**kw))
+    _compare_all(el, ref, "params")
+    assert np.array_equal(D1, ref["D1"]) and np.array_equal(D2, ref["D2"])
