@@ -80,6 +80,14 @@ def _img(a):
     return a
 
 
+def prior(prm, delta):
+    return lib().eo_prior(ctypes.byref(prm), int(delta))
+
+
+def plane_radius(prm):
+    return lib().eo_plane_radius(ctypes.byref(prm))
+
+
 def sobel(img):
     img = _img(img)
     H, W = img.shape

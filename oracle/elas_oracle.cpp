@@ -75,6 +75,10 @@ inline int32_t prior_P(const elas_params* p, int32_t delta_d) {
 
 extern "C" {
 
+/* prior P(|d - d_plane|) and plane radius, exposed for tests */
+int32_t eo_prior(const elas_params* p, int32_t delta) { return prior_P(p, delta); }
+int32_t eo_plane_radius(const elas_params* p) { return plane_radius(p); }
+
 /* ---- Sobel, KerDescriptor.cu:9-49 (kerComputeSobel) ---------------------- */
 void eo_sobel(const uint8_t* img, int W, int H, uint8_t* du, uint8_t* dv) {
     for (int v = 1; v < H - 1; v++)
