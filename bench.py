@@ -65,6 +65,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched comparison")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--workload", default="c1", choices=["c0", "c1", "c2"],
+                    help="c1 (1242x375, d 0..127) is the headline; c0/c2 for profiling")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (pairs would shard across GPUs with no "
@@ -76,7 +78,7 @@ def main():
     synth = __graft_entry__.load_synth()
     import oracle
 
-    left, right, _, dmax = synth.config_pair("c1")
+    left, right, _, dmax = synth.config_pair(args.workload)
     H, W = left.shape
     mpxd = W * H * (dmax + 1) / 1e6
     el = pkg.Elas(pkg.default_params(disp_max=dmax), device=0)
@@ -125,7 +127,7 @@ def main():
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(wall / args.steps * 1e3, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u8",
         "data": "synthetic",
-        "config": {"workload": "elas_plain_1242x375_d128_device_resident", "width": W, "height": H,
+        "config": {"workload": "elas_plain_%dx%d_d%d_device_resident" % (W, H, dmax + 1), "width": W, "height": H,
                    "disp_max": dmax, "pairs_per_step": 1, "host_delaunay": "included"},
         "pairs_per_s": round(args.steps / wall, 2),
         "stages_ms": {k: round(v, 5) for k, v in stages.items()},
@@ -139,8 +141,8 @@ def main():
     if not args.no_cpu:
         n, el_s = cpu_baseline(oracle, left, right, dmax, args.cpu_seconds)
         res["cpu_baseline"] = {"value": round(n * mpxd / el_s, 3), "unit": "Mpx*disp/s", "cores": 1,
-                               "kind": "port", "sample": "%d full pipelines of the same 1242x375 d128 pair "
-                               "(oracle, 1 thread, %.1f s)" % (n, el_s)}
+                               "kind": "port", "sample": "%d full pipelines of the same %dx%d d%d pair "
+                               "(oracle, 1 thread, %.1f s)" % (n, W, H, dmax + 1, el_s)}
     print(json.dumps(res), flush=True)
 
 

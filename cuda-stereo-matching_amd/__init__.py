@@ -78,12 +78,22 @@ _lib = None
 
 
 def load_library(path=LIB_PATH):
-    """Load libelas_hip.so (raises if it has not been built)."""
+    """Load libelas_hip.so (raises if it has not been built).
+
+    torch ships its own HIP runtime under the same soname (libamdhip64.so.7).
+    Importing torch first makes the library bind to that runtime, so a
+    process holds one HIP runtime and torch tensors can be passed to
+    process_device; loading the library first would leave torch a second
+    runtime that cannot open the device."""
     global _lib
     if _lib is not None:
         return _lib
     if not os.path.isfile(path):
         raise RuntimeError("libelas_hip.so is not built (%s): run __graft_entry__.build()" % path)
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = ctypes.CDLL(path)
     for name, (res, args) in SIGNATURES.items():
         f = getattr(lib, name)

@@ -247,7 +247,7 @@ int elas_ctx::run(const ElasPairSpec* sp, int n, int32_t* statuses) {
         switch (st) {
             case S_DESC: return cdiv(s.W, 64) * cdiv(s.H, 16) * 2;
             case S_SUPPORT: return d.ch * cdiv(d.cw, 4);
-            case S_INCON: return d.nblk;
+            case S_INCON: return cdiv(d.cw, 16) * cdiv(d.ch, 16);
             case S_REDUN_H: return cdiv(d.ch, 64);
             case S_REDUN_V: return cdiv(d.cw, 64);
             case S_BLOCKS: return d.nblk;

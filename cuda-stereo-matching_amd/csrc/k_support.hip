@@ -97,31 +97,59 @@ __global__ __launch_bounds__(256) void k_support(ElasLaunch L, ElasKParams kp) {
 // Inconsistency filter, KerSupportMatches.cu:196-238, read from can[1]
 // (snapshot) and written to can[2].  The reference updates in place and
 // races; reading a snapshot makes the result independent of scheduling.
-// The row guard y < can_width is the reference's (:208).
+// The row guard y < can_width is the reference's (:208).  A 16x16-cell tile
+// stages its (16+2R)^2 window in LDS (R <= 8; wider windows read L2).
+#define INC_T 16
+#define INC_RMAX 8
 __global__ __launch_bounds__(256) void k_incon(ElasLaunch L, ElasKParams kp) {
+    __shared__ int s_win[INC_T + 2 * INC_RMAX][INC_T + 2 * INC_RMAX + 1];
+    const int tid = threadIdx.x;
     for (int t = blockIdx.x; t < L.total; t += gridDim.x) {
         const int p = elas_find_pair(L.prefix, L.npairs, t);
         const ElasPair& P = L.pairs[p];
-        const int idx = (t - L.prefix[p]) * 256 + threadIdx.x;
         const int cw = P.cw, ch = P.ch;
-        if (idx >= cw * ch) continue;
-        const int y = idx / cw, x = idx - y * cw;
+        const int local = t - L.prefix[p];
+        const int tx = (cw + INC_T - 1) / INC_T;
+        const int x0 = (local % tx) * INC_T, y0 = (local / tx) * INC_T;
+        const int R = kp.incon_window;
+        const bool lds = R <= INC_RMAX;
         const int32_t* in = P.can[1];
-        const int d = in[idx];
-        int res = d;
-        if (d >= 0 && y < cw) {
-            const int R = kp.incon_window;
-            int support = 0;
-            const int x0 = max(x - R, 0), x1 = min(x + R, cw - 1);
-            const int y0 = max(y - R, 0), y1 = min(y + R, ch - 1);
-            for (int y2 = y0; y2 <= y1; y2++)
-                for (int x2 = x0; x2 <= x1; x2++) {
-                    const int d2 = in[y2 * cw + x2];
-                    support += (d2 >= 0 && abs(d - d2) <= kp.incon_threshold) ? 1 : 0;
-                }
-            if (support < kp.incon_min_support) res = -1;
+        if (lds) {
+            const int side = INC_T + 2 * R;
+            for (int i = tid; i < side * side; i += 256) {
+                const int yy = i / side, xx = i - yy * side;
+                const int gx = x0 - R + xx, gy = y0 - R + yy;
+                s_win[yy][xx] = (gx >= 0 && gx < cw && gy >= 0 && gy < ch) ? in[gy * cw + gx] : -1;
+            }
         }
-        P.can[2][idx] = res;
+        __syncthreads();
+        const int x = x0 + (tid & (INC_T - 1)), y = y0 + (tid / INC_T);
+        if (x < cw && y < ch) {
+            const int d = in[y * cw + x];
+            int res = d;
+            if (d >= 0 && y < cw) {
+                int support = 0;
+                if (lds) {
+                    const int lx = x - x0, ly = y - y0;
+                    for (int dy = 0; dy <= 2 * R; dy++)
+                        for (int dx = 0; dx <= 2 * R; dx++) {
+                            const int d2 = s_win[ly + dy][lx + dx];
+                            support += (d2 >= 0 && abs(d - d2) <= kp.incon_threshold) ? 1 : 0;
+                        }
+                } else {
+                    const int xa = max(x - R, 0), xb = min(x + R, cw - 1);
+                    const int ya = max(y - R, 0), yb = min(y + R, ch - 1);
+                    for (int y2 = ya; y2 <= yb; y2++)
+                        for (int x2 = xa; x2 <= xb; x2++) {
+                            const int d2 = in[y2 * cw + x2];
+                            support += (d2 >= 0 && abs(d - d2) <= kp.incon_threshold) ? 1 : 0;
+                        }
+                }
+                if (support < kp.incon_min_support) res = -1;
+            }
+            P.can[2][y * cw + x] = res;
+        }
+        __syncthreads();
     }
 }
 
@@ -129,41 +157,79 @@ __global__ __launch_bounds__(256) void k_incon(ElasLaunch L, ElasKParams kp) {
 // REDUN_THRESHOLD 1.  One lane per row (horizontal, can[2] -> can[3]) or per
 // column (vertical, in place on can[3]), scanned in order as in the
 // reference: a point is removed when a valid point within 5 cells on BOTH
-// sides differs by at most 1.  The already-scanned side sees updated values:
-// they sit in a 5-register window; the other side is read ahead.
+// sides differs by at most 1.  The already-scanned side sees updated values
+// (a 5-register window carried along the line); the other side sees the
+// values before this scan.  A 64-line group stages 64-cell tiles plus 5
+// cells of look-ahead in LDS with coalesced loads, each lane pulls its 69
+// values into registers and scans them without touching memory, and the
+// results leave through LDS with coalesced stores.
+#define RD_T 64
+#define RD_LA 5
 __device__ __forceinline__ bool elas_near(int a, int d) { return a >= 0 && abs(a - d) <= 1; }
 
 template <bool VERTICAL>
 __global__ __launch_bounds__(64) void k_redun(ElasLaunch L) {
+    // H: element (line l, pos j) at [l][j]; V: at [j][l] -- both conflict-free
+    __shared__ int s_t[VERTICAL ? (RD_T + RD_LA) * 64 : 64 * (RD_T + RD_LA)];
+    const int lane = threadIdx.x;
     for (int t = blockIdx.x; t < L.total; t += gridDim.x) {
         const int p = elas_find_pair(L.prefix, L.npairs, t);
         const ElasPair& P = L.pairs[p];
-        const int line = (t - L.prefix[p]) * 64 + threadIdx.x;
         const int cw = P.cw, ch = P.ch;
+        const int line0 = (t - L.prefix[p]) * 64;
         const int nlines = VERTICAL ? cw : ch, len = VERTICAL ? ch : cw;
-        if (line >= nlines) continue;
-        const int stride = VERTICAL ? cw : 1;
-        const int32_t* in = VERTICAL ? P.can[3] + line : P.can[2] + (size_t)line * cw;
-        int32_t* out = VERTICAL ? P.can[3] + line : P.can[3] + (size_t)line * cw;
-        int c1 = -1, c2 = -1, c3 = -1, c4 = -1, c5 = -1;  // updated values at i-1..i-5
-        int w0 = len > 0 ? in[0] : -1;
-        int w1 = len > 1 ? in[1 * stride] : -1, w2 = len > 2 ? in[2 * stride] : -1;
-        int w3 = len > 3 ? in[3 * stride] : -1, w4 = len > 4 ? in[4 * stride] : -1;
-        int w5 = len > 5 ? in[5 * stride] : -1;
-        for (int i = 0; i < len; i++) {
-            const int w6 = (i + 6 < len) ? in[(i + 6) * stride] : -1;
-            const int d = w0;
-            int res = d;
-            if (d >= 0) {
-                const bool ls = elas_near(c1, d) || elas_near(c2, d) || elas_near(c3, d) ||
-                                elas_near(c4, d) || elas_near(c5, d);
-                const bool rs = elas_near(w1, d) || elas_near(w2, d) || elas_near(w3, d) ||
-                                elas_near(w4, d) || elas_near(w5, d);
-                if (ls && rs) res = -1;
+        const int32_t* in = VERTICAL ? P.can[3] : P.can[2];
+        int32_t* out = P.can[3];
+        // element (line, pos) of the lattice
+        auto gidx = [&](int line, int pos) { return VERTICAL ? pos * cw + line : line * cw + pos; };
+        auto sidx = [&](int l, int j) { return VERTICAL ? j * 64 + l : l * (RD_T + RD_LA) + j; };
+        int c1 = -1, c2 = -1, c3 = -1, c4 = -1, c5 = -1;
+        for (int pos0 = 0; pos0 < len; pos0 += RD_T) {
+            __syncthreads();
+            if (VERTICAL) {
+                for (int j = 0; j < RD_T + RD_LA; j++) {
+                    const int line = line0 + lane, pos = pos0 + j;
+                    s_t[sidx(lane, j)] = (line < nlines && pos < len) ? in[gidx(line, pos)] : -1;
+                }
+            } else {
+                for (int l = 0; l < 64; l++) {
+                    const int line = line0 + l;
+                    for (int j = lane; j < RD_T + RD_LA; j += 64) {
+                        const int pos = pos0 + j;
+                        s_t[sidx(l, j)] = (line < nlines && pos < len) ? in[gidx(line, pos)] : -1;
+                    }
+                }
             }
-            out[i * stride] = res;
-            c5 = c4; c4 = c3; c3 = c2; c2 = c1; c1 = res;
-            w0 = w1; w1 = w2; w2 = w3; w3 = w4; w4 = w5; w5 = w6;
+            __syncthreads();
+            int w[RD_T + RD_LA];
+            #pragma unroll
+            for (int j = 0; j < RD_T + RD_LA; j++) w[j] = s_t[sidx(lane, j)];
+            #pragma unroll
+            for (int i = 0; i < RD_T; i++) {
+                const int d = w[i];
+                int res = d;
+                if (d >= 0) {
+                    const bool ls = elas_near(c1, d) || elas_near(c2, d) || elas_near(c3, d) ||
+                                    elas_near(c4, d) || elas_near(c5, d);
+                    const bool rs = elas_near(w[i + 1], d) || elas_near(w[i + 2], d) || elas_near(w[i + 3], d) ||
+                                    elas_near(w[i + 4], d) || elas_near(w[i + 5], d);
+                    if (ls && rs) res = -1;
+                }
+                s_t[sidx(lane, i)] = res;
+                c5 = c4; c4 = c3; c3 = c2; c2 = c1; c1 = res;
+            }
+            __syncthreads();
+            if (VERTICAL) {
+                for (int j = 0; j < RD_T; j++) {
+                    const int line = line0 + lane, pos = pos0 + j;
+                    if (line < nlines && pos < len) out[gidx(line, pos)] = s_t[sidx(lane, j)];
+                }
+            } else {
+                for (int l = 0; l < 64; l++) {
+                    const int line = line0 + l, pos = pos0 + lane;
+                    if (line < nlines && pos < len) out[gidx(line, pos)] = s_t[sidx(l, lane)];
+                }
+            }
         }
     }
 }
