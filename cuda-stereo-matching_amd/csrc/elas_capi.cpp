@@ -122,6 +122,8 @@ int elas_hip_destroy(elas_ctx* c) {
     if (c->arena) (void)hipFree(c->arena);
     if (c->table) (void)hipFree(c->table);
     if (c->pin) (void)hipHostFree(c->pin);
+    if (c->blist) (void)hipFree(c->blist);
+    if (c->blist_pin) (void)hipHostFree(c->blist_pin);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     if (c->stream2) (void)hipStreamDestroy(c->stream2);
     delete c;

@@ -11,7 +11,8 @@
 
 #define ELAS_MAX_P 16        // prior table entries: |d - d_plane| in [0, 15]
 #define ELAS_DENSE_ROWS 3    // image rows per dense tile (a wave = 20 px x 3 rows)
-#define ELAS_DENSE_CELLS 16  // grid cells per dense tile (one wave per cell)
+#define ELAS_DENSE_WAVES 16  // waves per dense tile; a wave covers dense_cpw cells
+#define ELAS_RASTER_T 32     // raster tile edge in pixels
 #define ELAS_MAX_DISP 511    // disp_max envelope (dense key packs d in 10 bits)
 
 // Per-triangle raster record (16 floats): plane and span edges, computed once.
@@ -37,8 +38,12 @@ struct ElasPair {
     int32_t* glist[2];       // [gh*gw][dmax+2] reference list layout (debug only)
     const int32_t* tri[2];   // [ntri][3]
     float* trirec[2];        // [ntri][ELAS_TRIREC]
-    int32_t* idmap[2];       // [H][W] covering triangle (highest index), -1 if none
-    float* mean[2];          // [H][W] plane prior (debug only)
+    int32_t ntx, nty;        // raster tiles (ELAS_RASTER_TILE px) across / down
+    const int32_t* toff[2];  // [ntx*nty+1] tile -> range of tlist
+    const int32_t* tlist[2]; // triangles meeting each tile, ascending
+    int32_t* prior[2];       // [H][W] (int)plane(u,v) of the covering triangle
+                             //        with the highest index, -1 if uncovered
+    float* mean[2];          // [H][W] (float)prior: the reference's map (debug only)
     float* draw[2];          // [H][W] dense disparity before the LR check
     float* dout[2];          // [H][W] final disparity
 };
@@ -53,6 +58,7 @@ struct ElasKParams {
     int32_t match_texture;
     int32_t key_bias;        // added to dense costs so keys stay non-negative
     int32_t all_p_negative;  // every prior P(k), k <= plane_radius, is < 0
+    int32_t dense_cpw;       // grid cells per wave in the dense tile (1 or 2)
     int32_t P[ELAS_MAX_P];
 };
 

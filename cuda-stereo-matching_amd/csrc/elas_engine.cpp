@@ -56,8 +56,9 @@ Dims dims_of(const elas_params& p, int W, int H, int dmax) {
 
 int dense_rows(int gs) { return std::max(1, std::min(64 / gs, gs)); }
 
-size_t dense_lds(int gs, int dmax) {
-    return (size_t)dense_rows(gs) * (size_t)(ELAS_DENSE_CELLS * gs + dmax) * 16;
+// LDS of a dense tile: R rows x (16*cpw cells x gs + dmax) columns x 16 B
+size_t dense_lds(int gs, int dmax, int cpw) {
+    return (size_t)dense_rows(gs) * (size_t)(ELAS_DENSE_WAVES * cpw * gs + dmax) * 16;
 }
 
 int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
@@ -126,8 +127,10 @@ int elas_ctx::run(const ElasPairSpec* sp, int n, int32_t* statuses) {
             return fail(ELAS_ERR_INVALID_ARGUMENT, "pair " + std::to_string(i) + ": null buffer");
         if (s.dmax < std::max(prm.disp_min, 0) || s.dmax > ELAS_MAX_DISP)
             return fail(ELAS_ERR_INVALID_ARGUMENT, "pair " + std::to_string(i) + ": disp_max out of range");
-        if (dense_lds(prm.grid_size, s.dmax) > 65536)
+        if (dense_lds(prm.grid_size, s.dmax, 1) > 65536)
             return fail(ELAS_ERR_UNSUPPORTED, "grid_size/disp_max exceed the dense LDS tile");
+        if (2 * 16 * (size_t)(63 * prm.candidate_stepsize + 2 * s.dmax + 5) > 65536)
+            return fail(ELAS_ERR_UNSUPPORTED, "candidate_stepsize/disp_max exceed the support LDS rows");
         Dims d = dims_of(prm, s.W, s.H, s.dmax);
         dd[i] = d;
         size_t b = 0;
@@ -137,13 +140,18 @@ int elas_ctx::run(const ElasPairSpec* sp, int n, int32_t* statuses) {
         b += 4 * al(4 * d.ncell * d.words);
         if (debug) b += 2 * al(4 * d.ncell * (size_t)(s.dmax + 2)) + 2 * al(4 * d.npx);
         b += 2 * al(12 * (size_t)d.tricap) + 2 * al(4 * ELAS_TRIREC * (size_t)d.tricap);
-        b += 2 * al(4 * d.npx) + 2 * al(4 * d.npx);  // idmap, draw
+        b += 2 * al(4 * d.npx) + 2 * al(4 * d.npx);  // prior, draw
         if (!s.dev_out) b += 2 * al(4 * d.npx);
         base[i] = arena_need;
         arena_need += b;
         pin_sp += 2 * 8 * d.ncan;
         pin_tri += 2 * 12 * (size_t)d.tricap;
     }
+    int maxd = 0;
+    for (int i = 0; i < n; i++) maxd = std::max(maxd, sp[i].dmax);
+    const int cpw = dense_lds(prm.grid_size, maxd, 2) <= 65536 ? 2 : 1;
+    ElasKParams kpl = kp;
+    kpl.dense_cpw = cpw;
     if (arena_need > arena_bytes) {
         if (arena) (void)hipFree(arena);
         arena = nullptr;
@@ -231,7 +239,9 @@ int elas_ctx::run(const ElasPairSpec* sp, int n, int32_t* statuses) {
         }
         for (int k = 0; k < 2; k++) P.tri[k] = (int32_t*)take(12 * (size_t)d.tricap);
         for (int k = 0; k < 2; k++) P.trirec[k] = (float*)take(4 * ELAS_TRIREC * (size_t)d.tricap);
-        for (int k = 0; k < 2; k++) P.idmap[k] = (int32_t*)take(4 * d.npx);
+        for (int k = 0; k < 2; k++) P.prior[k] = (int32_t*)take(4 * d.npx);
+        P.ntx = (s.W + ELAS_RASTER_T - 1) / ELAS_RASTER_T;
+        P.nty = (s.H + ELAS_RASTER_T - 1) / ELAS_RASTER_T;
         for (int k = 0; k < 2; k++) P.draw[k] = (float*)take(4 * d.npx);
         if (!s.dev_out) { P.dout[0] = (float*)take(4 * d.npx); P.dout[1] = (float*)take(4 * d.npx); }
         else { P.dout[0] = s.D1; P.dout[1] = s.D2; }
@@ -246,7 +256,7 @@ int elas_ctx::run(const ElasPairSpec* sp, int n, int32_t* statuses) {
         const int gs = prm.grid_size, R = dense_rows(gs);
         switch (st) {
             case S_DESC: return cdiv(s.W, 64) * cdiv(s.H, 16) * 2;
-            case S_SUPPORT: return d.ch * cdiv(d.cw, 4);
+            case S_SUPPORT: return d.ch * cdiv(d.cw, 64);
             case S_INCON: return cdiv(d.cw, 16) * cdiv(d.ch, 16);
             case S_REDUN_H: return cdiv(d.ch, 64);
             case S_REDUN_V: return cdiv(d.cw, 64);
@@ -254,8 +264,8 @@ int elas_ctx::run(const ElasPairSpec* sp, int n, int32_t* statuses) {
             case S_GRID_CELLS: return cdiv(2 * (long)d.ncell, 256);
             case S_GRID_FILL: return cdiv(2 * (long)d.ncell * d.words, 256);
             case S_PLANE: return cdiv((long)pairs[i].ntri[0] + pairs[i].ntri[1], 256);
-            case S_RASTER: return cdiv((long)pairs[i].ntri[0] + pairs[i].ntri[1], 16);
-            case S_DENSE: return 2 * d.gh * cdiv(gs, R) * cdiv(d.gw, ELAS_DENSE_CELLS);
+            case S_RASTER: return 2 * pairs[i].ntx * pairs[i].nty;
+            case S_DENSE: return 2 * d.gh * cdiv(gs, R) * cdiv(d.gw, ELAS_DENSE_WAVES * cpw);
             case S_PIXELS: return cdiv((long)d.npx, 256);
         }
         return 0;
@@ -293,8 +303,10 @@ int elas_ctx::run(const ElasPairSpec* sp, int n, int32_t* statuses) {
     { ElasLaunch L = launch(0, S_DESC); if (L.total) elas_launch_descriptor(L, g(L, cap256), stream); }
     HIPCHECK(hipEventRecord(ev[1], stream));
     { ElasLaunch L = launch(0, S_SUPPORT);
-      if (L.total) { elas_launch_support(L, kp, false, g(L, cap256), stream);
-                     elas_launch_support(L, kp, true, g(L, cap256), stream); } }
+      // LDS: rows v-2 and v+2 over 63*step + 2*dmax + 5 columns
+      const size_t lds = 2 * 16 * (size_t)(63 * prm.candidate_stepsize + 2 * maxd + 5);
+      if (L.total) { elas_launch_support(L, kp, false, g(L, cap256 * 4), lds, stream);
+                     elas_launch_support(L, kp, true, g(L, cap256 * 4), lds, stream); } }
     HIPCHECK(hipEventRecord(ev[2], stream));
     { ElasLaunch L = launch(0, S_INCON); if (L.total) elas_launch_incon(L, kp, g(L, cap256), stream); }
     { ElasLaunch L = launch(0, S_REDUN_H); if (L.total) elas_launch_redun(L, false, L.total, stream); }
@@ -329,14 +341,23 @@ int elas_ctx::run(const ElasPairSpec* sp, int n, int32_t* statuses) {
         }
     }
     HIPCHECK(hipStreamSynchronize(stream));
-    std::vector<std::vector<int32_t>> tri_out(2 * (size_t)n);
+    std::vector<std::vector<int32_t>> tri_out(2 * (size_t)n), toff(2 * (size_t)n), tlist(2 * (size_t)n);
     std::vector<ElasTriJob> jobs;
     std::vector<int> job_of(2 * (size_t)n, -1);
     for (int i = 0; i < n; i++) {
         if (nsp[i] < 3) { status[i] = ELAS_ERR_TOO_FEW_POINTS; continue; }
         for (int k = 0; k < 2; k++) {
             job_of[2 * i + k] = (int)jobs.size();
-            jobs.push_back({hsp + sp_off[i] + (size_t)k * 2 * dd[i].ncan, nsp[i], &tri_out[2 * i + k], 0});
+            ElasTriJob j;
+            j.pts = hsp + sp_off[i] + (size_t)k * 2 * dd[i].ncan;
+            j.n = nsp[i];
+            j.out = &tri_out[2 * i + k];
+            j.result = 0;
+            j.W = sp[i].W;
+            j.H = sp[i].H;
+            j.tile_off = &toff[2 * i + k];
+            j.tile_list = &tlist[2 * i + k];
+            jobs.push_back(j);
         }
     }
     elas_triangulate_jobs(jobs.data(), (int)jobs.size(), tri_workers);
@@ -351,7 +372,15 @@ int elas_ctx::run(const ElasPairSpec* sp, int n, int32_t* statuses) {
             if (nt[0] > dd[i].tricap || nt[1] > dd[i].tricap)
                 return fail(ELAS_ERR_HIP, "triangle count exceeds capacity");
         }
-        if (status[i] != ELAS_OK) nt[0] = nt[1] = 0;
+        if (status[i] != ELAS_OK) {
+            nt[0] = nt[1] = 0;
+            // a pair without a mesh rasterises nothing: every tile's list is empty
+            for (int k = 0; k < 2; k++)
+                if (job_of[2 * i + k] >= 0) {
+                    std::fill(toff[2 * i + k].begin(), toff[2 * i + k].end(), 0);
+                    tlist[2 * i + k].clear();
+                }
+        }
         ntri0[i] = nt[0];
         ntri1[i] = nt[1];
         pairs[i].ntri[0] = nt[0];
@@ -363,6 +392,60 @@ int elas_ctx::run(const ElasPairSpec* sp, int n, int32_t* statuses) {
                 HIPCHECK(hipMemcpyAsync((void*)pairs[i].tri[k], dst, 12 * (size_t)nt[k], hipMemcpyHostToDevice, stream));
             }
     }
+    // raster bins: [toff0 | toff1 | tlist0 | tlist1] per pair in one upload
+    size_t bl_ints = 0;
+    std::vector<size_t> bl_off(n);
+    for (int i = 0; i < n; i++) {
+        const size_t nt = (size_t)pairs[i].ntx * pairs[i].nty + 1;
+        bl_off[i] = bl_ints;
+        bl_ints += 2 * nt;
+        for (int k = 0; k < 2; k++) bl_ints += (job_of[2 * i + k] >= 0) ? tlist[2 * i + k].size() : 0;
+        bl_ints = (bl_ints + 63) & ~(size_t)63;
+    }
+    if (4 * bl_ints > blist_bytes) {
+        if (blist) (void)hipFree(blist);
+        blist = nullptr;
+        blist_bytes = 0;
+        if (hipMalloc(&blist, 4 * bl_ints + 4 * bl_ints / 4 + 256) != hipSuccess) {
+            blist = nullptr;
+            (void)hipGetLastError();
+            return fail(ELAS_ERR_OUT_OF_MEMORY, "raster bin allocation failed");
+        }
+        blist_bytes = 4 * bl_ints + 4 * bl_ints / 4 + 256;
+    }
+    if (4 * bl_ints > blist_pin_bytes) {
+        if (blist_pin) (void)hipHostFree(blist_pin);
+        blist_pin = nullptr;
+        blist_pin_bytes = 0;
+        if (hipHostMalloc(&blist_pin, 4 * bl_ints + 4 * bl_ints / 4 + 256, hipHostMallocDefault) != hipSuccess) {
+            blist_pin = nullptr;
+            (void)hipGetLastError();
+            return fail(ELAS_ERR_OUT_OF_MEMORY, "raster bin staging allocation failed");
+        }
+        blist_pin_bytes = 4 * bl_ints + 4 * bl_ints / 4 + 256;
+    }
+    {
+        int32_t* hb = (int32_t*)blist_pin;
+        int32_t* db = (int32_t*)blist;
+        for (int i = 0; i < n; i++) {
+            const size_t nt = (size_t)pairs[i].ntx * pairs[i].nty + 1;
+            size_t o = bl_off[i];
+            for (int k = 0; k < 2; k++) {
+                int32_t* dst = hb + o;
+                if (job_of[2 * i + k] >= 0) memcpy(dst, toff[2 * i + k].data(), 4 * nt);
+                else memset(dst, 0, 4 * nt);
+                pairs[i].toff[k] = db + o;
+                o += nt;
+            }
+            for (int k = 0; k < 2; k++) {
+                const size_t nl = (job_of[2 * i + k] >= 0) ? tlist[2 * i + k].size() : 0;
+                if (nl) memcpy(hb + o, tlist[2 * i + k].data(), 4 * nl);
+                pairs[i].tlist[k] = db + o;
+                o += nl;
+            }
+        }
+        if (bl_ints) HIPCHECK(hipMemcpyAsync(blist, blist_pin, 4 * bl_ints, hipMemcpyHostToDevice, stream));
+    }
     build_prefix(prefix.data());
     memcpy(hpairs_b, pairs.data(), sizeof(ElasPair) * n);
     memcpy(hprefix_b, prefix.data(), sizeof(int32_t) * prefix.size());
@@ -373,15 +456,12 @@ int elas_ctx::run(const ElasPairSpec* sp, int n, int32_t* statuses) {
 
     // ---- phase B: plane prior, dense matching, left/right check -------------
     HIPCHECK(hipEventRecord(ev[6], stream));
-    { ElasLaunch L = launch(1, S_PIXELS); if (L.total) elas_launch_fill_idmap(L, g(L, cap256), stream); }
     { ElasLaunch L = launch(1, S_PLANE); if (L.total) elas_launch_plane(L, g(L, cap256), stream); }
     { ElasLaunch L = launch(1, S_RASTER); if (L.total) elas_launch_raster(L, g(L, cap256), stream); }
     HIPCHECK(hipEventRecord(ev[7], stream));
     {
         ElasLaunch L = launch(1, S_DENSE);
-        int maxd = 0;
-        for (int i = 0; i < n; i++) maxd = std::max(maxd, sp[i].dmax);
-        if (L.total) elas_launch_dense(L, kp, g(L, cap1024), dense_lds(prm.grid_size, maxd), stream);
+        if (L.total) elas_launch_dense(L, kpl, g(L, cap1024), dense_lds(prm.grid_size, maxd, cpw), stream);
     }
     HIPCHECK(hipEventRecord(ev[8], stream));
     { ElasLaunch L = launch(1, S_PIXELS); if (L.total) elas_launch_lr(L, kp, g(L, cap256), stream); }

@@ -40,6 +40,11 @@ struct elas_ctx {
     // pinned host staging
     void* pin = nullptr;
     size_t pin_bytes = 0;
+    // raster tile offsets + lists (sizes known only after Triangle)
+    void* blist = nullptr;
+    size_t blist_bytes = 0;
+    void* blist_pin = nullptr;
+    size_t blist_pin_bytes = 0;
 
     std::vector<ElasPair> pairs;                    // host copy of the device table
     std::vector<int32_t> prefix;                    // [S_NUM_STAGES][n+1]

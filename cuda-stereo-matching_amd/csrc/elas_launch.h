@@ -6,7 +6,8 @@
 #include "elas_common.h"
 
 void elas_launch_descriptor(const ElasLaunch& L, int grid, hipStream_t s);
-void elas_launch_support(const ElasLaunch& L, const ElasKParams& kp, bool right, int grid, hipStream_t s);
+void elas_launch_support(const ElasLaunch& L, const ElasKParams& kp, bool right, int grid, size_t lds,
+                         hipStream_t s);
 void elas_launch_incon(const ElasLaunch& L, const ElasKParams& kp, int grid, hipStream_t s);
 void elas_launch_redun(const ElasLaunch& L, bool vertical, int grid, hipStream_t s);
 void elas_launch_blkcount(const ElasLaunch& L, int grid, hipStream_t s);
@@ -16,7 +17,6 @@ void elas_launch_grid_scatter(const ElasLaunch& L, int grid, hipStream_t s);
 void elas_launch_grid_dilate(const ElasLaunch& L, int grid, hipStream_t s);
 void elas_launch_grid_list(const ElasLaunch& L, int grid, hipStream_t s);
 void elas_launch_plane(const ElasLaunch& L, int grid, hipStream_t s);
-void elas_launch_fill_idmap(const ElasLaunch& L, int grid, hipStream_t s);
 void elas_launch_raster(const ElasLaunch& L, int grid, hipStream_t s);
 void elas_launch_mean(const ElasLaunch& L, int grid, hipStream_t s);
 void elas_launch_dense(const ElasLaunch& L, const ElasKParams& kp, int grid, size_t lds, hipStream_t s);

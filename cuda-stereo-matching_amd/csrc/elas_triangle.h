@@ -5,12 +5,24 @@
 
 #define ELAS_TRI_INSTANCES 16
 
+#define ELAS_RASTER_TILE 32
+
 struct ElasTriJob {
     const float* pts;            // [n][2] support points
     int n;
     std::vector<int32_t>* out;   // [ntri][3] zero-based triangles
     int result;                  // ntri, or -1 for fewer than 3 points
+    // optional tile binning of the mesh (done by the same worker): for every
+    // ELAS_RASTER_TILE^2 pixel tile of a W x H image, the ascending list of
+    // triangles whose pixel bounding box meets it
+    int W, H;
+    std::vector<int32_t>* tile_off;   // [ntiles + 1]
+    std::vector<int32_t>* tile_list;  // [tile_off[ntiles]]
 };
+
+// Tile binning of triangles with corners pts (pixel coordinates).
+void elas_bin_triangles(const float* pts, const int32_t* tri, int ntri, int W, int H,
+                        std::vector<int32_t>& off, std::vector<int32_t>& list);
 
 // Runs every job (each a "zQB" Triangle call) on up to max_workers of the
 // process-wide Triangle workers; returns when all are done.

@@ -93,77 +93,81 @@ __global__ __launch_bounds__(256) void k_plane(ElasLaunch L) {
     }
 }
 
-// idmap := -1 for both images (a fill kernel, so the pipeline can be captured
-// in a graph without memset nodes).
-__global__ __launch_bounds__(256) void k_fill_idmap(ElasLaunch L) {
-    for (int t = blockIdx.x; t < L.total; t += gridDim.x) {
-        const int p = elas_find_pair(L.prefix, L.npairs, t);
-        const ElasPair& P = L.pairs[p];
-        const int i = (t - L.prefix[p]) * 256 + threadIdx.x;
-        if (i >= P.W * P.H) continue;
-        P.idmap[0][i] = -1;
-        P.idmap[1][i] = -1;
-    }
+__device__ __forceinline__ int elas_plane_d(const float* rec, int u, int v) {
+    return elas_f2i_sat(__fadd_rn(__fadd_rn(__fmul_rn(rec[0], (float)u), __fmul_rn(rec[1], (float)v)), rec[2]));
 }
 
 // ---- rasterisation: KerComputeDisparity.cu:126-150 ------------------------
-// 16 lanes per triangle walk its columns; each column covers the half-open
-// row span between the long edge AC and AB (u in [A_u, B_u)) or BC
-// (u in [B_u, C_u)).  Where spans of two triangles overlap (the reference
-// races there) the highest triangle index wins via atomicMax, the same
-// result as painting the triangles in index order.
+// One workgroup per 32x32 pixel tile; thread (column c, row group g) owns the
+// 4 pixels (c, g + 8k).  The tile's triangles arrive in ascending index
+// order; for each, the owner computes its column's half-open row span
+// between the long edge AC and AB (u in [A_u, B_u)) or BC (u in [B_u, C_u))
+// and records the triangle for the pixels inside it.  A later triangle
+// overwrites an earlier one, so where spans overlap (the reference races
+// there) the highest index wins -- the result of painting the triangles in
+// index order -- with no atomics.  The owner then stores the plane prior
+// (int)(a*u + b*v + c) of its covering triangle, or -1.
 __global__ __launch_bounds__(256) void k_raster(ElasLaunch L) {
-    const int grp = threadIdx.x >> 4, gl = threadIdx.x & 15;
+    const int c = threadIdx.x & 31, g = threadIdx.x >> 5;
     for (int t = blockIdx.x; t < L.total; t += gridDim.x) {
         const int p = elas_find_pair(L.prefix, L.npairs, t);
         const ElasPair& P = L.pairs[p];
-        const int idx = (t - L.prefix[p]) * 16 + grp;
-        const int n0 = P.ntri[0], n1 = P.ntri[1];
-        if (idx >= n0 + n1) continue;
-        const int img = idx >= n0;
-        const int ti = img ? idx - n0 : idx;
-        const float* rec = P.trirec[img] + (size_t)ti * ELAS_TRIREC;
         const int W = P.W, H = P.H;
-        const int Au = (int)rec[3], Bu = (int)rec[5], Cu = (int)rec[7];
-        const float ACa = rec[11], ACb = rec[12];
-        int32_t* id = P.idmap[img];
-        for (int part = 0; part < 2; part++) {
-            const int lo = part ? Bu : Au, hi = part ? Cu : Bu;
-            if (lo == hi) continue;
-            const float ea = part ? rec[13] : rec[9], eb = part ? rec[14] : rec[10];
-            for (int u = max(lo, 0) + gl; u < min(hi, W); u += 16) {
-                const float fu = (float)u;
-                const int v1 = (int)elas_f2u_sat(__fadd_rn(__fmul_rn(ACa, fu), ACb));
-                const int v2 = (int)elas_f2u_sat(__fadd_rn(__fmul_rn(ea, fu), eb));
-                const int vlo = max(min(v1, v2), 0), vhi = min(max(v1, v2), H);
-                for (int v = vlo; v < vhi; v++) atomicMax(&id[(size_t)v * W + u], ti);
+        const int ntiles = P.ntx * P.nty;
+        int local = t - L.prefix[p];
+        const int img = local >= ntiles;
+        if (img) local -= ntiles;
+        const int u = (local % P.ntx) * ELAS_RASTER_T + c;
+        const int v0 = (local / P.ntx) * ELAS_RASTER_T + g;
+        const float* recs = P.trirec[img];
+        const int* lst = P.tlist[img];
+        const int k0 = P.toff[img][local], k1 = P.toff[img][local + 1];
+        int id0 = -1, id1 = -1, id2 = -1, id3 = -1;
+        for (int k = k0; k < k1; k++) {
+            const int ti = lst[k];
+            const float* rec = recs + (size_t)ti * ELAS_TRIREC;
+            const int Au = (int)rec[3], Bu = (int)rec[5], Cu = (int)rec[7];
+            if (u < Au || u >= Cu) continue;
+            const bool first = u < Bu;
+            const float ea = first ? rec[9] : rec[13], eb = first ? rec[10] : rec[14];
+            const float fu = (float)u;
+            const int v1 = (int)elas_f2u_sat(__fadd_rn(__fmul_rn(rec[11], fu), rec[12]));
+            const int v2 = (int)elas_f2u_sat(__fadd_rn(__fmul_rn(ea, fu), eb));
+            const int vlo = max(min(v1, v2), 0), vhi = min(max(v1, v2), H);
+            if (v0 >= vlo && v0 < vhi) id0 = ti;
+            if (v0 + 8 >= vlo && v0 + 8 < vhi) id1 = ti;
+            if (v0 + 16 >= vlo && v0 + 16 < vhi) id2 = ti;
+            if (v0 + 24 >= vlo && v0 + 24 < vhi) id3 = ti;
+        }
+        if (u < W) {
+            const int ids[4] = {id0, id1, id2, id3};
+            #pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int v = v0 + 8 * r;
+                if (v < H)
+                    P.prior[img][(size_t)v * W + u] =
+                        ids[r] >= 0 ? elas_plane_d(recs + (size_t)ids[r] * ELAS_TRIREC, u, v) : -1;
             }
         }
     }
 }
 
-__device__ __forceinline__ int elas_plane_d(const float* rec, int u, int v) {
-    return elas_f2i_sat(__fadd_rn(__fadd_rn(__fmul_rn(rec[0], (float)u), __fmul_rn(rec[1], (float)v)), rec[2]));
-}
-
-// Debug: the reference's mean map, (float)(int)plane(u,v) where covered.
+// Debug: the reference's mean map, (float)(int)plane(u,v) where covered, -1.
 __global__ __launch_bounds__(256) void k_mean(ElasLaunch L) {
     for (int t = blockIdx.x; t < L.total; t += gridDim.x) {
         const int p = elas_find_pair(L.prefix, L.npairs, t);
         const ElasPair& P = L.pairs[p];
         const int i = (t - L.prefix[p]) * 256 + threadIdx.x;
         if (i >= P.W * P.H) continue;
-        const int v = i / P.W, u = i - v * P.W;
-        for (int img = 0; img < 2; img++) {
-            const int id = P.idmap[img][i];
-            P.mean[img][i] = id >= 0 ? (float)elas_plane_d(P.trirec[img] + (size_t)id * ELAS_TRIREC, u, v) : -1.0f;
-        }
+        P.mean[0][i] = (float)P.prior[0][i];
+        P.mean[1][i] = (float)P.prior[1][i];
     }
 }
 
 // ---- dense matching: KerComputeDisparity.cu:156-316,378-402 ---------------
-// Tile = R image rows (R = min(64/gs, gs), inside one grid-cell row) x 16
-// grid cells; wave w owns cell gx0+w, lane = (row, column) of that cell, so
+// Tile = R image rows (R = min(64/gs, gs), inside one grid-cell row) x
+// 16*cpw grid cells; wave w owns cells gx0+w+16j, lane = (row, column) of a
+// cell, so
 // the cell's candidate set is wave-uniform and is walked on the scalar unit
 // as the set bits of its bitmask (ascending d).  The other image's R rows are
 // staged in LDS for the tile's disparity reach; the pixel's own 16-byte block
@@ -180,13 +184,15 @@ template <bool PNEG>
 __global__ __launch_bounds__(1024) void k_dense(ElasLaunch L, ElasKParams kp) {
     extern __shared__ uint4 s_i2[];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int cpw = kp.dense_cpw;
+    const int tcells = ELAS_DENSE_WAVES * cpw;
     for (int t = blockIdx.x; t < L.total; t += gridDim.x) {
         const int p = elas_find_pair(L.prefix, L.npairs, t);
         const ElasPair& P = L.pairs[p];
         const int W = P.W, H = P.H, gs = P.gsize, gw = P.gw;
         const int R = max(1, min(64 / gs, gs));
         const int sub = (gs + R - 1) / R;
-        const int segs = (gw + ELAS_DENSE_CELLS - 1) / ELAS_DENSE_CELLS;
+        const int segs = (gw + tcells - 1) / tcells;
         const int per_img = P.gh * sub * segs;
         int r = t - L.prefix[p];
         const int img = r / per_img;
@@ -196,8 +202,8 @@ __global__ __launch_bounds__(1024) void k_dense(ElasLaunch L, ElasKParams kp) {
         const int k = r / segs, s = r - k * segs;
         const int v0 = gy * gs + k * R;
         const int rows = max(0, min(min(R, gs - k * R), H - v0));
-        const int gx0 = s * ELAS_DENSE_CELLS;
-        const int u_lo = gx0 * gs, u_hi = min((gx0 + ELAS_DENSE_CELLS) * gs, W);
+        const int gx0 = s * tcells;
+        const int u_lo = gx0 * gs, u_hi = min((gx0 + tcells) * gs, W);
         const int c_lo = img ? u_lo : max(u_lo - P.dmax, 0);
         const int c_hi = img ? min(u_hi + P.dmax, W) : u_hi;
         const int ncols = max(c_hi - c_lo, 0);
@@ -207,29 +213,32 @@ __global__ __launch_bounds__(1024) void k_dense(ElasLaunch L, ElasKParams kp) {
             const int rr = i / ncols, cc = i - rr * ncols;
             s_i2[i] = *reinterpret_cast<const uint4*>(I2 + ((size_t)(v0 + rr) * W + c_lo + cc) * 16);
         }
+        // this lane's pixel in each of its cells: prior and own descriptor
+        const int row = lane / gs, col = lane - row * gs;
+        int dpv[2] = {0, 0};
+        uint4 b1v[2] = {make_uint4(0u, 0u, 0u, 0u), make_uint4(0u, 0u, 0u, 0u)};
+        for (int ci = 0; ci < cpw; ci++) {
+            const int gx = gx0 + wave + ci * ELAS_DENSE_WAVES;
+            const int u = gx * gs + col, v = v0 + row;
+            const bool valid = gx < gw && lane < R * gs && row < rows && u < W &&
+                               u >= 4 && u <= W - 4 && v >= 4 && v <= H - 4;
+            if (valid) {
+                dpv[ci] = P.prior[img][(size_t)v * W + u];
+                b1v[ci] = *reinterpret_cast<const uint4*>(I1 + ((size_t)v * W + u) * 16);
+            }
+        }
         __syncthreads();
-        const int gx = gx0 + wave;
-        if (gx < gw) {
-            const int row = lane / gs, col = lane - row * gs;
+        for (int ci = 0; ci < cpw; ci++) {
+            const int gx = gx0 + wave + ci * ELAS_DENSE_WAVES;
+            if (gx >= gw) continue;
             const int u = gx * gs + col, v = v0 + row;
             const bool active = lane < R * gs && row < rows && u < W;
-            bool valid = active && u >= 4 && u <= W - 4 && v >= 4 && v <= H - 4;
-            int dp = 0;
-            uint4 b1 = make_uint4(0u, 0u, 0u, 0u);
-            if (valid) {
-                const int id = P.idmap[img][(size_t)v * W + u];
-                if (id < 0) valid = false;
-                else {
-                    dp = elas_plane_d(P.trirec[img] + (size_t)id * ELAS_TRIREC, u, v);
-                    if (dp <= 0) valid = false;
-                }
-            }
-            if (valid) {
-                b1 = *reinterpret_cast<const uint4*>(I1 + ((size_t)v * W + u) * 16);
-                if (kp.match_texture > 0) {
-                    const uint32_t tex = elas_sad16(b1, make_uint4(0x80808080u, 0x80808080u, 0x80808080u, 0x80808080u), 0u);
-                    if ((int)tex < kp.match_texture) valid = false;
-                }
+            const int dp = dpv[ci];
+            const uint4 b1 = b1v[ci];
+            bool valid = active && u >= 4 && u <= W - 4 && v >= 4 && v <= H - 4 && dp > 0;
+            if (valid && kp.match_texture > 0) {
+                const uint32_t tex = elas_sad16(b1, make_uint4(0x80808080u, 0x80808080u, 0x80808080u, 0x80808080u), 0u);
+                if ((int)tex < kp.match_texture) valid = false;
             }
             const int dpmin = max(dp - kp.plane_radius, 0), dpmax = min(dp + kp.plane_radius, P.dmax);
             const int sgn = img ? 1 : -1;
@@ -303,9 +312,6 @@ __global__ __launch_bounds__(256) void k_lr(ElasLaunch L, ElasKParams kp) {
 
 void elas_launch_plane(const ElasLaunch& L, int grid, hipStream_t s) {
     hipLaunchKernelGGL(k_plane, dim3(grid), dim3(256), 0, s, L);
-}
-void elas_launch_fill_idmap(const ElasLaunch& L, int grid, hipStream_t s) {
-    hipLaunchKernelGGL(k_fill_idmap, dim3(grid), dim3(256), 0, s, L);
 }
 void elas_launch_raster(const ElasLaunch& L, int grid, hipStream_t s) {
     hipLaunchKernelGGL(k_raster, dim3(grid), dim3(256), 0, s, L);

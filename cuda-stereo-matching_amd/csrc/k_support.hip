@@ -1,9 +1,9 @@
 // k_support.hip -- support-point matching, its filters, compaction and the
 // disparity grid (gfx950).
 //
-// Support matching restates src/Kernels/KerSupportMatches.cu:13-192.  One
-// wave per candidate, lane l evaluates d = dmin + l + 64k: E(d) is the
-// 64-byte SAD of four 16-byte blocks at (+-2, +-2) (16 x v_sad_u8).
+// Support matching restates src/Kernels/KerSupportMatches.cu:13-192.
+// E(d) is the 64-byte SAD of four 16-byte blocks at (+-2, +-2) around the
+// candidate (16 x v_sad_u8).
 //
 // The reference reduces the (best, second best) of 32 threads in thread 0,
 // which starts from its own registers and then re-reads its own entry from
@@ -14,83 +14,96 @@
 //   min2   = m if r* == 0 (thread 0 holds a minimiser: duplicated), else the
 //            second smallest E counted with multiplicity;
 //   accept = (float)m < support_threshold * (float)min2,   winner d*.
-// The wave computes it with one packed key (E << 15 | r << 10 | d) and a
-// (min, second-min) pair per lane, then a 6-step xor-shuffle reduction.
+// With key(d) = E << 15 | r << 10 | d (distinct per d), m, r*, d* come from
+// the smallest key and the second smallest E from the second smallest key.
+//
+// Mapping: one workgroup (one wave) = 64 consecutive candidates of a lattice
+// row, lane = candidate.  The other image's descriptor rows v-2 and v+2 are
+// staged in LDS over the columns the 64 candidates can reach; each lane keeps
+// its four reference blocks in registers and walks its own disparity range
+// keeping the two smallest keys, so no cross-lane reduction is needed.  Lanes
+// read LDS 16*step bytes apart (80 B for step 5: no bank conflicts).
 #include <hip/hip_runtime.h>
 #include "elas_common.h"
 #include "elas_device.h"
 
 template <bool RIGHT>
-__global__ __launch_bounds__(256) void k_support(ElasLaunch L, ElasKParams kp) {
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+__global__ __launch_bounds__(64) void k_support(ElasLaunch L, ElasKParams kp) {
+    extern __shared__ uint4 s_rows[];  // [2][ncols]: row v-2, then row v+2
+    const int lane = threadIdx.x;
     for (int t = blockIdx.x; t < L.total; t += gridDim.x) {
         const int p = elas_find_pair(L.prefix, L.npairs, t);
         const ElasPair& P = L.pairs[p];
+        const int W = P.W, H = P.H, cw = P.cw, step = P.step;
         const int local = t - L.prefix[p];
-        const int per_row = (P.cw + 3) >> 2;
+        const int per_row = (cw + 63) >> 6;
         const int v_can = local / per_row;
-        const int u_can = (local - v_can * per_row) * 4 + wave;
-        if (u_can >= P.cw) continue;
-        const int W = P.W, H = P.H;
-        const int cell = v_can * P.cw + u_can;
-        int u = u_can * P.step;
-        const int v = v_can * P.step;
+        const int k0 = (local - v_can * per_row) * 64;
+        const int u_can = k0 + lane;
+        const int v = v_can * step;
+        // columns reachable by this wave's candidates
+        const int u_first = k0 * step, u_last = min(k0 + 63, cw - 1) * step;
+        const int c_lo = max(u_first - P.dmax - 2, 0);
+        const int c_hi = RIGHT ? min(u_last + P.dmax + 3, W) : min(u_last + 3, W);
+        const int ncols = max(c_hi - c_lo, 0);
+        const bool row_ok = v >= 5 && v <= H - 6;
+        const uint8_t* I1 = P.desc[RIGHT ? 1 : 0] + (size_t)16 * W * v;
+        const uint8_t* I2 = P.desc[RIGHT ? 0 : 1] + (size_t)16 * W * v;
+        if (row_ok) {
+            const uint4* top = reinterpret_cast<const uint4*>(I2 - 32 * (size_t)W) + c_lo;
+            const uint4* bot = reinterpret_cast<const uint4*>(I2 + 32 * (size_t)W) + c_lo;
+            for (int i = lane; i < ncols; i += 64) {
+                s_rows[i] = top[i];
+                s_rows[ncols + i] = bot[i];
+            }
+        }
+        __syncthreads();
         int result = -1;
-        bool ok = (u >= 5 && u <= W - 6 && v >= 5 && v <= H - 6);
+        int u = u_can * step;
+        bool ok = row_ok && u_can < cw && u >= 5 && u <= W - 6;
         int d_left = 0;
         if (ok && RIGHT) {
-            d_left = P.can[0][cell];
+            d_left = P.can[0][v_can * cw + u_can];
             if (d_left < 0) ok = false;
             else u -= d_left;
         }
+        const int dminv = max(P.dmin, 0);
+        const int dmaxv = RIGHT ? min(P.dmax, W - u - 5) : min(P.dmax, u - 5);
         if (ok) {
-            const uint8_t* I1 = P.desc[RIGHT ? 1 : 0] + (size_t)16 * W * v;
-            const uint8_t* I2 = P.desc[RIGHT ? 0 : 1] + (size_t)16 * W * v;
-            const uint4 c = *reinterpret_cast<const uint4*>(I1 + 16 * u);
+            const uint4 c = *reinterpret_cast<const uint4*>(I1 + 16 * (size_t)u);
             const uint32_t tex = elas_sad16(c, make_uint4(0x80808080u, 0x80808080u, 0x80808080u, 0x80808080u), 0u);
-            const int dminv = max(P.dmin, 0);
-            const int dmaxv = RIGHT ? min(P.dmax, W - u - 5) : min(P.dmax, u - 5);
-            if ((int)tex >= kp.support_texture && dmaxv - dminv >= 10) {
-                const long rowoff = 32L * W;
-                const uint8_t* b1 = I1 + 16 * u;
-                const uint4 r0 = *reinterpret_cast<const uint4*>(b1 - 32 - rowoff);
-                const uint4 r1 = *reinterpret_cast<const uint4*>(b1 + 32 - rowoff);
-                const uint4 r2 = *reinterpret_cast<const uint4*>(b1 - 32 + rowoff);
-                const uint4 r3 = *reinterpret_cast<const uint4*>(b1 + 32 + rowoff);
-                uint32_t kmin = 0xffffffffu, m1 = 0xffffu, m2 = 0xffffu;
-                for (int d = dminv + lane; d <= dmaxv; d += 64) {
-                    const uint8_t* b2 = I2 + 16 * (RIGHT ? u + d : u - d);
-                    uint32_t E = elas_sad16(r0, *reinterpret_cast<const uint4*>(b2 - 32 - rowoff), 0u);
-                    E = elas_sad16(r1, *reinterpret_cast<const uint4*>(b2 + 32 - rowoff), E);
-                    E = elas_sad16(r2, *reinterpret_cast<const uint4*>(b2 - 32 + rowoff), E);
-                    E = elas_sad16(r3, *reinterpret_cast<const uint4*>(b2 + 32 + rowoff), E);
-                    const uint32_t key = (E << 15) | ((uint32_t)((d - dminv) & 31) << 10) | (uint32_t)d;
-                    kmin = min(kmin, key);
-                    m2 = min(m2, max(m1, E));
-                    m1 = min(m1, E);
-                }
-                #pragma unroll
-                for (int off = 32; off >= 1; off >>= 1) {
-                    const uint32_t ok2 = __shfl_xor(kmin, off, 64);
-                    const uint32_t o1 = __shfl_xor(m1, off, 64);
-                    const uint32_t o2 = __shfl_xor(m2, off, 64);
-                    kmin = min(kmin, ok2);
-                    const uint32_t n2 = min(max(m1, o1), min(m2, o2));
-                    m1 = min(m1, o1);
-                    m2 = n2;
-                }
-                const uint32_t m = kmin >> 15;
-                const uint32_t r = (kmin >> 10) & 31u;
-                const int dstar = (int)(kmin & 1023u);
-                const uint32_t min2 = (r == 0u) ? m : m2;
-                const float thr = kp.support_threshold * (float)min2;
-                if ((float)m < thr) {
-                    if (!RIGHT) result = dstar;
-                    else result = (abs(d_left - dstar) > kp.lr_threshold) ? -1 : d_left;
-                }
+            ok = (int)tex >= kp.support_texture && dmaxv - dminv >= 10;
+        }
+        if (ok) {
+            const long rowoff = 32L * W;
+            const uint8_t* b1 = I1 + 16 * (size_t)u;
+            const uint4 rTL = *reinterpret_cast<const uint4*>(b1 - 32 - rowoff);
+            const uint4 rTR = *reinterpret_cast<const uint4*>(b1 + 32 - rowoff);
+            const uint4 rBL = *reinterpret_cast<const uint4*>(b1 - 32 + rowoff);
+            const uint4 rBR = *reinterpret_cast<const uint4*>(b1 + 32 + rowoff);
+            const uint4* top = s_rows - c_lo;
+            const uint4* bot = s_rows + ncols - c_lo;
+            uint32_t k1 = 0xffffffffu, k2 = 0xffffffffu;
+            for (int d = dminv; d <= dmaxv; d++) {
+                const int uw = RIGHT ? u + d : u - d;
+                uint32_t E = elas_sad16(rTL, top[uw - 2], 0u);
+                E = elas_sad16(rTR, top[uw + 2], E);
+                E = elas_sad16(rBL, bot[uw - 2], E);
+                E = elas_sad16(rBR, bot[uw + 2], E);
+                const uint32_t key = (E << 15) | ((uint32_t)((d - dminv) & 31) << 10) | (uint32_t)d;
+                k2 = min(k2, max(k1, key));
+                k1 = min(k1, key);
+            }
+            const uint32_t m = k1 >> 15;
+            const uint32_t min2 = (((k1 >> 10) & 31u) == 0u) ? m : (k2 >> 15);
+            const int dstar = (int)(k1 & 1023u);
+            if ((float)m < kp.support_threshold * (float)min2) {
+                if (!RIGHT) result = dstar;
+                else result = (abs(d_left - dstar) > kp.lr_threshold) ? -1 : d_left;
             }
         }
-        if (lane == 0) (RIGHT ? P.can[1] : P.can[0])[cell] = result;
+        if (u_can < cw) (RIGHT ? P.can[1] : P.can[0])[v_can * cw + u_can] = result;
+        __syncthreads();
     }
 }
 
@@ -159,19 +172,21 @@ __global__ __launch_bounds__(256) void k_incon(ElasLaunch L, ElasKParams kp) {
 // reference: a point is removed when a valid point within 5 cells on BOTH
 // sides differs by at most 1.  The already-scanned side sees updated values
 // (a 5-register window carried along the line); the other side sees the
-// values before this scan.  A 64-line group stages 64-cell tiles plus 5
-// cells of look-ahead in LDS with coalesced loads, each lane pulls its 69
-// values into registers and scans them without touching memory, and the
-// results leave through LDS with coalesced stores.
+// values before this scan.  A 64-line group moves through the lattice in
+// 64-cell tiles (+5 cells of look-ahead): all four waves load a tile into
+// LDS with independent coalesced loads, wave 0 pulls each lane's 69 values
+// into registers and scans them, and all waves store the results.
 #define RD_T 64
 #define RD_LA 5
+#define RD_N (64 * (RD_T + RD_LA))
+#define RD_PER ((RD_N + 255) / 256)
 __device__ __forceinline__ bool elas_near(int a, int d) { return a >= 0 && abs(a - d) <= 1; }
 
 template <bool VERTICAL>
-__global__ __launch_bounds__(64) void k_redun(ElasLaunch L) {
+__global__ __launch_bounds__(256) void k_redun(ElasLaunch L) {
     // H: element (line l, pos j) at [l][j]; V: at [j][l] -- both conflict-free
-    __shared__ int s_t[VERTICAL ? (RD_T + RD_LA) * 64 : 64 * (RD_T + RD_LA)];
-    const int lane = threadIdx.x;
+    __shared__ int s_t[RD_N];
+    const int tid = threadIdx.x;
     for (int t = blockIdx.x; t < L.total; t += gridDim.x) {
         const int p = elas_find_pair(L.prefix, L.npairs, t);
         const ElasPair& P = L.pairs[p];
@@ -180,56 +195,59 @@ __global__ __launch_bounds__(64) void k_redun(ElasLaunch L) {
         const int nlines = VERTICAL ? cw : ch, len = VERTICAL ? ch : cw;
         const int32_t* in = VERTICAL ? P.can[3] : P.can[2];
         int32_t* out = P.can[3];
-        // element (line, pos) of the lattice
-        auto gidx = [&](int line, int pos) { return VERTICAL ? pos * cw + line : line * cw + pos; };
-        auto sidx = [&](int l, int j) { return VERTICAL ? j * 64 + l : l * (RD_T + RD_LA) + j; };
         int c1 = -1, c2 = -1, c3 = -1, c4 = -1, c5 = -1;
         for (int pos0 = 0; pos0 < len; pos0 += RD_T) {
+            int v[RD_PER];
+            #pragma unroll
+            for (int k = 0; k < RD_PER; k++) {
+                const int i = tid + 256 * k;
+                int l, j;
+                if (VERTICAL) { j = i >> 6; l = i & 63; }
+                else { l = i / (RD_T + RD_LA); j = i - l * (RD_T + RD_LA); }
+                const int line = line0 + l, pos = pos0 + j;
+                v[k] = (i < RD_N && line < nlines && pos < len)
+                           ? in[VERTICAL ? pos * cw + line : line * cw + pos] : -1;
+            }
+            #pragma unroll
+            for (int k = 0; k < RD_PER; k++) {
+                const int i = tid + 256 * k;
+                if (i < RD_N) s_t[i] = v[k];
+            }
             __syncthreads();
-            if (VERTICAL) {
-                for (int j = 0; j < RD_T + RD_LA; j++) {
-                    const int line = line0 + lane, pos = pos0 + j;
-                    s_t[sidx(lane, j)] = (line < nlines && pos < len) ? in[gidx(line, pos)] : -1;
-                }
-            } else {
-                for (int l = 0; l < 64; l++) {
-                    const int line = line0 + l;
-                    for (int j = lane; j < RD_T + RD_LA; j += 64) {
-                        const int pos = pos0 + j;
-                        s_t[sidx(l, j)] = (line < nlines && pos < len) ? in[gidx(line, pos)] : -1;
+            if (tid < 64) {
+                const int lane = tid;
+                auto sidx = [&](int j) { return VERTICAL ? j * 64 + lane : lane * (RD_T + RD_LA) + j; };
+                int w[RD_T + RD_LA];
+                #pragma unroll
+                for (int j = 0; j < RD_T + RD_LA; j++) w[j] = s_t[sidx(j)];
+                #pragma unroll
+                for (int i = 0; i < RD_T; i++) {
+                    const int d = w[i];
+                    int res = d;
+                    if (d >= 0) {
+                        const bool ls = elas_near(c1, d) || elas_near(c2, d) || elas_near(c3, d) ||
+                                        elas_near(c4, d) || elas_near(c5, d);
+                        const bool rs = elas_near(w[i + 1], d) || elas_near(w[i + 2], d) ||
+                                        elas_near(w[i + 3], d) || elas_near(w[i + 4], d) || elas_near(w[i + 5], d);
+                        if (ls && rs) res = -1;
                     }
+                    s_t[sidx(i)] = res;
+                    c5 = c4; c4 = c3; c3 = c2; c2 = c1; c1 = res;
                 }
             }
             __syncthreads();
-            int w[RD_T + RD_LA];
             #pragma unroll
-            for (int j = 0; j < RD_T + RD_LA; j++) w[j] = s_t[sidx(lane, j)];
-            #pragma unroll
-            for (int i = 0; i < RD_T; i++) {
-                const int d = w[i];
-                int res = d;
-                if (d >= 0) {
-                    const bool ls = elas_near(c1, d) || elas_near(c2, d) || elas_near(c3, d) ||
-                                    elas_near(c4, d) || elas_near(c5, d);
-                    const bool rs = elas_near(w[i + 1], d) || elas_near(w[i + 2], d) || elas_near(w[i + 3], d) ||
-                                    elas_near(w[i + 4], d) || elas_near(w[i + 5], d);
-                    if (ls && rs) res = -1;
-                }
-                s_t[sidx(lane, i)] = res;
-                c5 = c4; c4 = c3; c3 = c2; c2 = c1; c1 = res;
+            for (int k = 0; k < (64 * RD_T) / 256; k++) {
+                const int i = tid + 256 * k;
+                int l, j;
+                if (VERTICAL) { j = i >> 6; l = i & 63; }
+                else { l = i >> 6; j = i & 63; }
+                const int line = line0 + l, pos = pos0 + j;
+                if (line < nlines && pos < len)
+                    out[VERTICAL ? pos * cw + line : line * cw + pos] =
+                        s_t[VERTICAL ? j * 64 + l : l * (RD_T + RD_LA) + j];
             }
             __syncthreads();
-            if (VERTICAL) {
-                for (int j = 0; j < RD_T; j++) {
-                    const int line = line0 + lane, pos = pos0 + j;
-                    if (line < nlines && pos < len) out[gidx(line, pos)] = s_t[sidx(lane, j)];
-                }
-            } else {
-                for (int l = 0; l < 64; l++) {
-                    const int line = line0 + l, pos = pos0 + lane;
-                    if (line < nlines && pos < len) out[gidx(line, pos)] = s_t[sidx(l, lane)];
-                }
-            }
         }
     }
 }
@@ -367,16 +385,17 @@ __global__ __launch_bounds__(256) void k_grid_list(ElasLaunch L) {
     }
 }
 
-void elas_launch_support(const ElasLaunch& L, const ElasKParams& kp, bool right, int grid, hipStream_t s) {
-    if (right) hipLaunchKernelGGL(k_support<true>, dim3(grid), dim3(256), 0, s, L, kp);
-    else hipLaunchKernelGGL(k_support<false>, dim3(grid), dim3(256), 0, s, L, kp);
+void elas_launch_support(const ElasLaunch& L, const ElasKParams& kp, bool right, int grid, size_t lds,
+                         hipStream_t s) {
+    if (right) hipLaunchKernelGGL(k_support<true>, dim3(grid), dim3(64), lds, s, L, kp);
+    else hipLaunchKernelGGL(k_support<false>, dim3(grid), dim3(64), lds, s, L, kp);
 }
 void elas_launch_incon(const ElasLaunch& L, const ElasKParams& kp, int grid, hipStream_t s) {
     hipLaunchKernelGGL(k_incon, dim3(grid), dim3(256), 0, s, L, kp);
 }
 void elas_launch_redun(const ElasLaunch& L, bool vertical, int grid, hipStream_t s) {
-    if (vertical) hipLaunchKernelGGL(k_redun<true>, dim3(grid), dim3(64), 0, s, L);
-    else hipLaunchKernelGGL(k_redun<false>, dim3(grid), dim3(64), 0, s, L);
+    if (vertical) hipLaunchKernelGGL(k_redun<true>, dim3(grid), dim3(256), 0, s, L);
+    else hipLaunchKernelGGL(k_redun<false>, dim3(grid), dim3(256), 0, s, L);
 }
 void elas_launch_blkcount(const ElasLaunch& L, int grid, hipStream_t s) {
     hipLaunchKernelGGL(k_blkcount, dim3(grid), dim3(256), 0, s, L);
