@@ -107,7 +107,10 @@ __device__ __forceinline__ int elas_plane_d(const float* rec, int u, int v) {
 // there) the highest index wins -- the result of painting the triangles in
 // index order -- with no atomics.  The owner then stores the plane prior
 // (int)(a*u + b*v + c) of its covering triangle, or -1.
+#define RS_CHUNK 128
 __global__ __launch_bounds__(256) void k_raster(ElasLaunch L) {
+    __shared__ float s_rec[RS_CHUNK][ELAS_TRIREC];
+    __shared__ int s_id[RS_CHUNK];
     const int c = threadIdx.x & 31, g = threadIdx.x >> 5;
     for (int t = blockIdx.x; t < L.total; t += gridDim.x) {
         const int p = elas_find_pair(L.prefix, L.npairs, t);
@@ -123,21 +126,33 @@ __global__ __launch_bounds__(256) void k_raster(ElasLaunch L) {
         const int* lst = P.tlist[img];
         const int k0 = P.toff[img][local], k1 = P.toff[img][local + 1];
         int id0 = -1, id1 = -1, id2 = -1, id3 = -1;
-        for (int k = k0; k < k1; k++) {
-            const int ti = lst[k];
-            const float* rec = recs + (size_t)ti * ELAS_TRIREC;
-            const int Au = (int)rec[3], Bu = (int)rec[5], Cu = (int)rec[7];
-            if (u < Au || u >= Cu) continue;
-            const bool first = u < Bu;
-            const float ea = first ? rec[9] : rec[13], eb = first ? rec[10] : rec[14];
-            const float fu = (float)u;
-            const int v1 = (int)elas_f2u_sat(__fadd_rn(__fmul_rn(rec[11], fu), rec[12]));
-            const int v2 = (int)elas_f2u_sat(__fadd_rn(__fmul_rn(ea, fu), eb));
-            const int vlo = max(min(v1, v2), 0), vhi = min(max(v1, v2), H);
-            if (v0 >= vlo && v0 < vhi) id0 = ti;
-            if (v0 + 8 >= vlo && v0 + 8 < vhi) id1 = ti;
-            if (v0 + 16 >= vlo && v0 + 16 < vhi) id2 = ti;
-            if (v0 + 24 >= vlo && v0 + 24 < vhi) id3 = ti;
+        for (int kc = k0; kc < k1; kc += RS_CHUNK) {
+            const int n = min(RS_CHUNK, k1 - kc);
+            // stage the chunk's records: 16 floats each, 16 threads per record
+            for (int i = threadIdx.x; i < n * ELAS_TRIREC; i += 256) {
+                const int j = i / ELAS_TRIREC, f = i - j * ELAS_TRIREC;
+                const int ti = lst[kc + j];
+                s_rec[j][f] = recs[(size_t)ti * ELAS_TRIREC + f];
+                if (f == 0) s_id[j] = ti;
+            }
+            __syncthreads();
+            for (int j = 0; j < n; j++) {
+                const float* rec = s_rec[j];
+                const int Au = (int)rec[3], Bu = (int)rec[5], Cu = (int)rec[7];
+                if (u < Au || u >= Cu) continue;
+                const bool first = u < Bu;
+                const float ea = first ? rec[9] : rec[13], eb = first ? rec[10] : rec[14];
+                const float fu = (float)u;
+                const int v1 = (int)elas_f2u_sat(__fadd_rn(__fmul_rn(rec[11], fu), rec[12]));
+                const int v2 = (int)elas_f2u_sat(__fadd_rn(__fmul_rn(ea, fu), eb));
+                const int vlo = max(min(v1, v2), 0), vhi = min(max(v1, v2), H);
+                const int ti = s_id[j];
+                if (v0 >= vlo && v0 < vhi) id0 = ti;
+                if (v0 + 8 >= vlo && v0 + 8 < vhi) id1 = ti;
+                if (v0 + 16 >= vlo && v0 + 16 < vhi) id2 = ti;
+                if (v0 + 24 >= vlo && v0 + 24 < vhi) id3 = ti;
+            }
+            __syncthreads();
         }
         if (u < W) {
             const int ids[4] = {id0, id1, id2, id3};
@@ -213,8 +228,15 @@ __global__ __launch_bounds__(1024) void k_dense(ElasLaunch L, ElasKParams kp) {
             const int rr = i / ncols, cc = i - rr * ncols;
             s_i2[i] = *reinterpret_cast<const uint4*>(I2 + ((size_t)(v0 + rr) * W + c_lo + cc) * 16);
         }
-        // this lane's pixel in each of its cells: prior and own descriptor
+        // this lane's pixel in each of its cells: prior and own descriptor; lane
+        // 32*ci + w holds word w of cell ci's candidate mask
         const int row = lane / gs, col = lane - row * gs;
+        uint32_t maskv = 0u;
+        {
+            const int ci = lane >> 5, w = lane & 31;
+            const int gx = gx0 + wave + ci * ELAS_DENSE_WAVES;
+            if (ci < cpw && w < P.words && gx < gw) maskv = P.gmask[img][(size_t)(gy * gw + gx) * P.words + w];
+        }
         int dpv[2] = {0, 0};
         uint4 b1v[2] = {make_uint4(0u, 0u, 0u, 0u), make_uint4(0u, 0u, 0u, 0u)};
         for (int ci = 0; ci < cpw; ci++) {
@@ -244,9 +266,8 @@ __global__ __launch_bounds__(1024) void k_dense(ElasLaunch L, ElasKParams kp) {
             const int sgn = img ? 1 : -1;
             const int lds_row = row * ncols - c_lo;
             uint32_t kmin = 0xffffffffu;
-            const uint32_t* mask = P.gmask[img] + (size_t)(gy * gw + gx) * P.words;
             for (int w = 0; w < P.words; w++) {
-                uint32_t bits = __builtin_amdgcn_readfirstlane(mask[w]);
+                uint32_t bits = __builtin_amdgcn_readlane(maskv, 32 * ci + w);
                 while (bits) {
                     const int d = w * 32 + __builtin_ctz(bits);
                     bits &= bits - 1u;
@@ -262,6 +283,7 @@ __global__ __launch_bounds__(1024) void k_dense(ElasLaunch L, ElasKParams kp) {
                 }
             }
             if (valid) {
+                #pragma unroll 5
                 for (int d = dpmin; d <= dpmax; d++) {
                     const int uw = u + sgn * d;
                     if (uw < 3 || uw > W - 4) continue;

@@ -84,6 +84,7 @@ __global__ __launch_bounds__(64) void k_support(ElasLaunch L, ElasKParams kp) {
             const uint4* top = s_rows - c_lo;
             const uint4* bot = s_rows + ncols - c_lo;
             uint32_t k1 = 0xffffffffu, k2 = 0xffffffffu;
+            #pragma unroll 4
             for (int d = dminv; d <= dmaxv; d++) {
                 const int uw = RIGHT ? u + d : u - d;
                 uint32_t E = elas_sad16(rTL, top[uw - 2], 0u);
@@ -180,7 +181,9 @@ __global__ __launch_bounds__(256) void k_incon(ElasLaunch L, ElasKParams kp) {
 #define RD_LA 5
 #define RD_N (64 * (RD_T + RD_LA))
 #define RD_PER ((RD_N + 255) / 256)
-__device__ __forceinline__ bool elas_near(int a, int d) { return a >= 0 && abs(a - d) <= 1; }
+#define RD_INVALID (-(1 << 20))
+// a valid (a >= 0) and |a - d| <= 1, for d >= 0 and invalid a = RD_INVALID
+__device__ __forceinline__ bool rd_near(int a, int d) { return (unsigned)(a - d + 1) <= 2u; }
 
 template <bool VERTICAL>
 __global__ __launch_bounds__(256) void k_redun(ElasLaunch L) {
@@ -195,7 +198,7 @@ __global__ __launch_bounds__(256) void k_redun(ElasLaunch L) {
         const int nlines = VERTICAL ? cw : ch, len = VERTICAL ? ch : cw;
         const int32_t* in = VERTICAL ? P.can[3] : P.can[2];
         int32_t* out = P.can[3];
-        int c1 = -1, c2 = -1, c3 = -1, c4 = -1, c5 = -1;
+        int c1 = RD_INVALID, c2 = RD_INVALID, c3 = RD_INVALID, c4 = RD_INVALID, c5 = RD_INVALID;
         for (int pos0 = 0; pos0 < len; pos0 += RD_T) {
             int v[RD_PER];
             #pragma unroll
@@ -215,23 +218,24 @@ __global__ __launch_bounds__(256) void k_redun(ElasLaunch L) {
             }
             __syncthreads();
             if (tid < 64) {
+                // Branchless scan: invalid cells hold a large negative sentinel, so
+                // "valid and within 1 of d" (d >= 0) is one unsigned compare.
                 const int lane = tid;
                 auto sidx = [&](int j) { return VERTICAL ? j * 64 + lane : lane * (RD_T + RD_LA) + j; };
                 int w[RD_T + RD_LA];
                 #pragma unroll
-                for (int j = 0; j < RD_T + RD_LA; j++) w[j] = s_t[sidx(j)];
+                for (int j = 0; j < RD_T + RD_LA; j++) {
+                    const int x = s_t[sidx(j)];
+                    w[j] = x < 0 ? RD_INVALID : x;
+                }
                 #pragma unroll
                 for (int i = 0; i < RD_T; i++) {
                     const int d = w[i];
-                    int res = d;
-                    if (d >= 0) {
-                        const bool ls = elas_near(c1, d) || elas_near(c2, d) || elas_near(c3, d) ||
-                                        elas_near(c4, d) || elas_near(c5, d);
-                        const bool rs = elas_near(w[i + 1], d) || elas_near(w[i + 2], d) ||
-                                        elas_near(w[i + 3], d) || elas_near(w[i + 4], d) || elas_near(w[i + 5], d);
-                        if (ls && rs) res = -1;
-                    }
-                    s_t[sidx(i)] = res;
+                    const bool ls = rd_near(c1, d) | rd_near(c2, d) | rd_near(c3, d) | rd_near(c4, d) | rd_near(c5, d);
+                    const bool rs = rd_near(w[i + 1], d) | rd_near(w[i + 2], d) | rd_near(w[i + 3], d) |
+                                    rd_near(w[i + 4], d) | rd_near(w[i + 5], d);
+                    const int res = ((d >= 0) & ls & rs) ? RD_INVALID : d;
+                    s_t[sidx(i)] = res < 0 ? -1 : res;
                     c5 = c4; c4 = c3; c3 = c2; c2 = c1; c1 = res;
                 }
             }
