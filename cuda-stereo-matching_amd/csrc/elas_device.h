@@ -16,6 +16,29 @@ __device__ __forceinline__ int elas_find_pair(const int32_t* __restrict__ prefix
     return lo;
 }
 
+// Loads through pointers known to address global memory (the ElasPair
+// pointers are generic; without the cast the compiler emits flat loads that
+// also wait on the LDS counter).
+#define ELAS_AS_GLOBAL __attribute__((address_space(1)))
+template <typename T>
+__device__ __forceinline__ T elas_ld(const T* p) {
+    return *(const ELAS_AS_GLOBAL T*)p;
+}
+template <typename T>
+__device__ __forceinline__ void elas_st(T* p, const T& v) {
+    *(ELAS_AS_GLOBAL T*)p = v;
+}
+typedef uint32_t elas_u32x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ uint4 elas_ld(const uint4* p) {
+    const elas_u32x4 v = *(const ELAS_AS_GLOBAL elas_u32x4*)p;
+    return make_uint4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ void elas_st(uint4* p, const uint4& v) {
+    elas_u32x4 x;
+    x.x = v.x; x.y = v.y; x.z = v.z; x.w = v.w;
+    *(ELAS_AS_GLOBAL elas_u32x4*)p = x;
+}
+
 // Sum of |a_i - b_i| over 16 bytes, accumulated onto acc (4 x v_sad_u8).
 __device__ __forceinline__ uint32_t elas_sad16(const uint4 a, const uint4 b, uint32_t acc) {
     acc = __builtin_amdgcn_sad_u8(a.x, b.x, acc);

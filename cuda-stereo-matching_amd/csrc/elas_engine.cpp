@@ -303,10 +303,11 @@ int elas_ctx::run(const ElasPairSpec* sp, int n, int32_t* statuses) {
     { ElasLaunch L = launch(0, S_DESC); if (L.total) elas_launch_descriptor(L, g(L, cap256), stream); }
     HIPCHECK(hipEventRecord(ev[1], stream));
     { ElasLaunch L = launch(0, S_SUPPORT);
-      // LDS: rows v-2 and v+2 over 63*step + 2*dmax + 5 columns
-      const size_t lds = 2 * 16 * (size_t)(63 * prm.candidate_stepsize + 2 * maxd + 5);
-      if (L.total) { elas_launch_support(L, kp, false, g(L, cap256 * 4), lds, stream);
-                     elas_launch_support(L, kp, true, g(L, cap256 * 4), lds, stream); } }
+      // LDS: rows v-2 and v+2 over 63*step + dmax + 5 (left) or + 2*dmax + 5 (right) columns
+      const size_t lds_l = 2 * 16 * (size_t)(63 * prm.candidate_stepsize + maxd + 5);
+      const size_t lds_r = 2 * 16 * (size_t)(63 * prm.candidate_stepsize + 2 * maxd + 5);
+      if (L.total) { elas_launch_support(L, kp, false, g(L, cap256), lds_l, stream);
+                     elas_launch_support(L, kp, true, g(L, cap256), lds_r, stream); } }
     HIPCHECK(hipEventRecord(ev[2], stream));
     { ElasLaunch L = launch(0, S_INCON); if (L.total) elas_launch_incon(L, kp, g(L, cap256), stream); }
     { ElasLaunch L = launch(0, S_REDUN_H); if (L.total) elas_launch_redun(L, false, L.total, stream); }

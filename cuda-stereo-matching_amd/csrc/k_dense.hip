@@ -226,7 +226,7 @@ __global__ __launch_bounds__(1024) void k_dense(ElasLaunch L, ElasKParams kp) {
         const uint8_t* I2 = P.desc[1 - img];
         for (int i = threadIdx.x; i < rows * ncols; i += 1024) {
             const int rr = i / ncols, cc = i - rr * ncols;
-            s_i2[i] = *reinterpret_cast<const uint4*>(I2 + ((size_t)(v0 + rr) * W + c_lo + cc) * 16);
+            s_i2[i] = elas_ld(reinterpret_cast<const uint4*>(I2 + ((size_t)(v0 + rr) * W + c_lo + cc) * 16));
         }
         // this lane's pixel in each of its cells: prior and own descriptor; lane
         // 32*ci + w holds word w of cell ci's candidate mask
@@ -235,24 +235,26 @@ __global__ __launch_bounds__(1024) void k_dense(ElasLaunch L, ElasKParams kp) {
         {
             const int ci = lane >> 5, w = lane & 31;
             const int gx = gx0 + wave + ci * ELAS_DENSE_WAVES;
-            if (ci < cpw && w < P.words && gx < gw) maskv = P.gmask[img][(size_t)(gy * gw + gx) * P.words + w];
+            if (ci < cpw && w < P.words && gx < gw) maskv = elas_ld(P.gmask[img] + (size_t)(gy * gw + gx) * P.words + w);
         }
         int dpv[2] = {0, 0};
         uint4 b1v[2] = {make_uint4(0u, 0u, 0u, 0u), make_uint4(0u, 0u, 0u, 0u)};
-        for (int ci = 0; ci < cpw; ci++) {
+        #pragma unroll
+        for (int ci = 0; ci < 2; ci++) {
             const int gx = gx0 + wave + ci * ELAS_DENSE_WAVES;
             const int u = gx * gs + col, v = v0 + row;
-            const bool valid = gx < gw && lane < R * gs && row < rows && u < W &&
+            const bool valid = ci < cpw && gx < gw && lane < R * gs && row < rows && u < W &&
                                u >= 4 && u <= W - 4 && v >= 4 && v <= H - 4;
             if (valid) {
-                dpv[ci] = P.prior[img][(size_t)v * W + u];
-                b1v[ci] = *reinterpret_cast<const uint4*>(I1 + ((size_t)v * W + u) * 16);
+                dpv[ci] = elas_ld(P.prior[img] + (size_t)v * W + u);
+                b1v[ci] = elas_ld(reinterpret_cast<const uint4*>(I1 + ((size_t)v * W + u) * 16));
             }
         }
         __syncthreads();
-        for (int ci = 0; ci < cpw; ci++) {
+        #pragma unroll
+        for (int ci = 0; ci < 2; ci++) {
             const int gx = gx0 + wave + ci * ELAS_DENSE_WAVES;
-            if (gx >= gw) continue;
+            if (ci >= cpw || gx >= gw) continue;
             const int u = gx * gs + col, v = v0 + row;
             const bool active = lane < R * gs && row < rows && u < W;
             const int dp = dpv[ci];
@@ -293,7 +295,7 @@ __global__ __launch_bounds__(1024) void k_dense(ElasLaunch L, ElasKParams kp) {
                 }
             }
             if (active)
-                P.draw[img][(size_t)v * W + u] = (kmin != 0xffffffffu) ? (float)(kmin & 1023u) : -1.0f;
+                elas_st(P.draw[img] + (size_t)v * W + u, (kmin != 0xffffffffu) ? (float)(kmin & 1023u) : -1.0f);
         }
         __syncthreads();
     }

@@ -1,5 +1,9 @@
 // k_support.hip -- support-point matching, its filters, compaction and the
 // disparity grid (gfx950).
+#include <hip/hip_runtime.h>
+#include "elas_common.h"
+#include "elas_device.h"
+
 //
 // Support matching restates src/Kernels/KerSupportMatches.cu:13-192.
 // E(d) is the 64-byte SAD of four 16-byte blocks at (+-2, +-2) around the
@@ -17,20 +21,23 @@
 // With key(d) = E << 15 | r << 10 | d (distinct per d), m, r*, d* come from
 // the smallest key and the second smallest E from the second smallest key.
 //
-// Mapping: one workgroup (one wave) = 64 consecutive candidates of a lattice
-// row, lane = candidate.  The other image's descriptor rows v-2 and v+2 are
-// staged in LDS over the columns the 64 candidates can reach; each lane keeps
-// its four reference blocks in registers and walks its own disparity range
-// keeping the two smallest keys, so no cross-lane reduction is needed.  Lanes
-// read LDS 16*step bytes apart (80 B for step 5: no bank conflicts).
-#include <hip/hip_runtime.h>
-#include "elas_common.h"
-#include "elas_device.h"
-
+// Mapping: one workgroup (4 waves) = 64 consecutive candidates of a lattice
+// row.  Lane (j, c) of wave w works on candidate 16w + c and the disparities
+// d = dmin + j + 4m (j = lane / 16).  The other image's descriptor rows v-2
+// and v+2 are staged in LDS over the columns the 64 candidates can reach.
+// E(d) pairs the reference blocks with the block pair X(d) at column
+// u -/+ d -/+ 2 and with X(d -/+ 4) at the column 4 further in: stepping d by
+// 4, each lane loads one new block pair and reuses the previous one, so LDS
+// supplies 32 bytes per (candidate, d).  Within a 16-lane group the
+// candidates' columns are 16*step bytes apart (no bank conflicts for step 5).
+// The four lanes of a candidate merge their two smallest keys with two
+// shuffles.
 template <bool RIGHT>
-__global__ __launch_bounds__(64) void k_support(ElasLaunch L, ElasKParams kp) {
+__global__ __launch_bounds__(256) void k_support(ElasLaunch L, ElasKParams kp) {
     extern __shared__ uint4 s_rows[];  // [2][ncols]: row v-2, then row v+2
-    const int lane = threadIdx.x;
+    const int tid = threadIdx.x;
+    const int wave = tid >> 6, lane = tid & 63;
+    const int j = lane >> 4, cl = lane & 15;
     for (int t = blockIdx.x; t < L.total; t += gridDim.x) {
         const int p = elas_find_pair(L.prefix, L.npairs, t);
         const ElasPair& P = L.pairs[p];
@@ -39,9 +46,8 @@ __global__ __launch_bounds__(64) void k_support(ElasLaunch L, ElasKParams kp) {
         const int per_row = (cw + 63) >> 6;
         const int v_can = local / per_row;
         const int k0 = (local - v_can * per_row) * 64;
-        const int u_can = k0 + lane;
+        const int u_can = k0 + 16 * wave + cl;
         const int v = v_can * step;
-        // columns reachable by this wave's candidates
         const int u_first = k0 * step, u_last = min(k0 + 63, cw - 1) * step;
         const int c_lo = max(u_first - P.dmax - 2, 0);
         const int c_hi = RIGHT ? min(u_last + P.dmax + 3, W) : min(u_last + 3, W);
@@ -52,58 +58,78 @@ __global__ __launch_bounds__(64) void k_support(ElasLaunch L, ElasKParams kp) {
         if (row_ok) {
             const uint4* top = reinterpret_cast<const uint4*>(I2 - 32 * (size_t)W) + c_lo;
             const uint4* bot = reinterpret_cast<const uint4*>(I2 + 32 * (size_t)W) + c_lo;
-            for (int i = lane; i < ncols; i += 64) {
-                s_rows[i] = top[i];
-                s_rows[ncols + i] = bot[i];
-            }
+            for (int i = tid; i < 2 * ncols; i += 256)
+                s_rows[i] = elas_ld(i < ncols ? top + i : bot + (i - ncols));
         }
-        __syncthreads();
-        int result = -1;
         int u = u_can * step;
         bool ok = row_ok && u_can < cw && u >= 5 && u <= W - 6;
         int d_left = 0;
         if (ok && RIGHT) {
-            d_left = P.can[0][v_can * cw + u_can];
+            d_left = elas_ld(P.can[0] + v_can * cw + u_can);
             if (d_left < 0) ok = false;
             else u -= d_left;
         }
         const int dminv = max(P.dmin, 0);
         const int dmaxv = RIGHT ? min(P.dmax, W - u - 5) : min(P.dmax, u - 5);
+        uint4 rTL, rTR, rBL, rBR;
         if (ok) {
-            const uint4 c = *reinterpret_cast<const uint4*>(I1 + 16 * (size_t)u);
+            const uint8_t* b1 = I1 + 16 * (size_t)u;
+            const uint4 c = elas_ld(reinterpret_cast<const uint4*>(b1));
             const uint32_t tex = elas_sad16(c, make_uint4(0x80808080u, 0x80808080u, 0x80808080u, 0x80808080u), 0u);
             ok = (int)tex >= kp.support_texture && dmaxv - dminv >= 10;
-        }
-        if (ok) {
             const long rowoff = 32L * W;
-            const uint8_t* b1 = I1 + 16 * (size_t)u;
-            const uint4 rTL = *reinterpret_cast<const uint4*>(b1 - 32 - rowoff);
-            const uint4 rTR = *reinterpret_cast<const uint4*>(b1 + 32 - rowoff);
-            const uint4 rBL = *reinterpret_cast<const uint4*>(b1 - 32 + rowoff);
-            const uint4 rBR = *reinterpret_cast<const uint4*>(b1 + 32 + rowoff);
+            rTL = elas_ld(reinterpret_cast<const uint4*>(b1 - 32 - rowoff));
+            rTR = elas_ld(reinterpret_cast<const uint4*>(b1 + 32 - rowoff));
+            rBL = elas_ld(reinterpret_cast<const uint4*>(b1 - 32 + rowoff));
+            rBR = elas_ld(reinterpret_cast<const uint4*>(b1 + 32 + rowoff));
+        }
+        __syncthreads();
+        uint32_t k1 = 0xffffffffu, k2 = 0xffffffffu;
+        if (ok) {
+            // new pair at column cx(d) = u - d - 2 (left) / u + d + 2 (right),
+            // reused pair at cx(d) + 4 / cx(d) - 4
+            const uint4 rXt = RIGHT ? rTR : rTL, rXb = RIGHT ? rBR : rBL;
+            const uint4 rPt = RIGHT ? rTL : rTR, rPb = RIGHT ? rBL : rBR;
             const uint4* top = s_rows - c_lo;
             const uint4* bot = s_rows + ncols - c_lo;
-            uint32_t k1 = 0xffffffffu, k2 = 0xffffffffu;
-            #pragma unroll 4
-            for (int d = dminv; d <= dmaxv; d++) {
-                const int uw = RIGHT ? u + d : u - d;
-                uint32_t E = elas_sad16(rTL, top[uw - 2], 0u);
-                E = elas_sad16(rTR, top[uw + 2], E);
-                E = elas_sad16(rBL, bot[uw - 2], E);
-                E = elas_sad16(rBR, bot[uw + 2], E);
+            int d = dminv + j;
+            int cx = RIGHT ? u + d + 2 : u - d - 2;
+            const int dstep = RIGHT ? 4 : -4;
+            uint4 pt = top[cx - dstep], pb = bot[cx - dstep];
+            #pragma unroll 2
+            for (; d <= dmaxv; d += 4, cx += dstep) {
+                const uint4 xt = top[cx], xb = bot[cx];
+                uint32_t E = elas_sad16(rXt, xt, 0u);
+                E = elas_sad16(rXb, xb, E);
+                E = elas_sad16(rPt, pt, E);
+                E = elas_sad16(rPb, pb, E);
                 const uint32_t key = (E << 15) | ((uint32_t)((d - dminv) & 31) << 10) | (uint32_t)d;
                 k2 = min(k2, max(k1, key));
                 k1 = min(k1, key);
-            }
-            const uint32_t m = k1 >> 15;
-            const uint32_t min2 = (((k1 >> 10) & 31u) == 0u) ? m : (k2 >> 15);
-            const int dstar = (int)(k1 & 1023u);
-            if ((float)m < kp.support_threshold * (float)min2) {
-                if (!RIGHT) result = dstar;
-                else result = (abs(d_left - dstar) > kp.lr_threshold) ? -1 : d_left;
+                pt = xt;
+                pb = xb;
             }
         }
-        if (u_can < cw) (RIGHT ? P.can[1] : P.can[0])[v_can * cw + u_can] = result;
+        #pragma unroll
+        for (int off = 16; off <= 32; off <<= 1) {
+            const uint32_t o1 = __shfl_xor(k1, off, 64);
+            const uint32_t o2 = __shfl_xor(k2, off, 64);
+            k2 = min(max(k1, o1), min(k2, o2));
+            k1 = min(k1, o1);
+        }
+        if (j == 0 && u_can < cw) {
+            int result = -1;
+            if (ok) {
+                const uint32_t m = k1 >> 15;
+                const uint32_t min2 = (((k1 >> 10) & 31u) == 0u) ? m : (k2 >> 15);
+                const int dstar = (int)(k1 & 1023u);
+                if ((float)m < kp.support_threshold * (float)min2) {
+                    if (!RIGHT) result = dstar;
+                    else result = (abs(d_left - dstar) > kp.lr_threshold) ? -1 : d_left;
+                }
+            }
+            elas_st((RIGHT ? P.can[1] : P.can[0]) + v_can * cw + u_can, result);
+        }
         __syncthreads();
     }
 }
@@ -391,8 +417,8 @@ __global__ __launch_bounds__(256) void k_grid_list(ElasLaunch L) {
 
 void elas_launch_support(const ElasLaunch& L, const ElasKParams& kp, bool right, int grid, size_t lds,
                          hipStream_t s) {
-    if (right) hipLaunchKernelGGL(k_support<true>, dim3(grid), dim3(64), lds, s, L, kp);
-    else hipLaunchKernelGGL(k_support<false>, dim3(grid), dim3(64), lds, s, L, kp);
+    if (right) hipLaunchKernelGGL(k_support<true>, dim3(grid), dim3(256), lds, s, L, kp);
+    else hipLaunchKernelGGL(k_support<false>, dim3(grid), dim3(256), lds, s, L, kp);
 }
 void elas_launch_incon(const ElasLaunch& L, const ElasKParams& kp, int grid, hipStream_t s) {
     hipLaunchKernelGGL(k_incon, dim3(grid), dim3(256), 0, s, L, kp);
