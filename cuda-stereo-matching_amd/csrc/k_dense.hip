@@ -128,12 +128,15 @@ __global__ __launch_bounds__(256) void k_raster(ElasLaunch L) {
         int id0 = -1, id1 = -1, id2 = -1, id3 = -1;
         for (int kc = k0; kc < k1; kc += RS_CHUNK) {
             const int n = min(RS_CHUNK, k1 - kc);
-            // stage the chunk's records: 16 floats each, 16 threads per record
-            for (int i = threadIdx.x; i < n * ELAS_TRIREC; i += 256) {
-                const int j = i / ELAS_TRIREC, f = i - j * ELAS_TRIREC;
-                const int ti = lst[kc + j];
-                s_rec[j][f] = recs[(size_t)ti * ELAS_TRIREC + f];
-                if (f == 0) s_id[j] = ti;
+            // stage the chunk's records: thread j < n fetches triangle j's id and
+            // its 64-byte record with independent loads
+            if ((int)threadIdx.x < n) {
+                const int ti = elas_ld(lst + kc + threadIdx.x);
+                const uint4* src = reinterpret_cast<const uint4*>(recs + (size_t)ti * ELAS_TRIREC);
+                const uint4 r0 = elas_ld(src), r1 = elas_ld(src + 1), r2 = elas_ld(src + 2), r3 = elas_ld(src + 3);
+                uint4* dst = reinterpret_cast<uint4*>(s_rec[threadIdx.x]);
+                dst[0] = r0; dst[1] = r1; dst[2] = r2; dst[3] = r3;
+                s_id[threadIdx.x] = ti;
             }
             __syncthreads();
             for (int j = 0; j < n; j++) {
@@ -224,9 +227,25 @@ __global__ __launch_bounds__(1024) void k_dense(ElasLaunch L, ElasKParams kp) {
         const int ncols = max(c_hi - c_lo, 0);
         const uint8_t* I1 = P.desc[img];
         const uint8_t* I2 = P.desc[1 - img];
-        for (int i = threadIdx.x; i < rows * ncols; i += 1024) {
-            const int rr = i / ncols, cc = i - rr * ncols;
-            s_i2[i] = elas_ld(reinterpret_cast<const uint4*>(I2 + ((size_t)(v0 + rr) * W + c_lo + cc) * 16));
+        // staging: every thread issues all of its loads before storing any
+        {
+            const int n = rows * ncols;
+            for (int base = 0; base < n; base += 4 * 1024) {
+                uint4 q[4];
+                #pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const int i = base + threadIdx.x + k * 1024;
+                    if (i < n) {
+                        const int rr = i / ncols, cc = i - rr * ncols;
+                        q[k] = elas_ld(reinterpret_cast<const uint4*>(I2 + ((size_t)(v0 + rr) * W + c_lo + cc) * 16));
+                    }
+                }
+                #pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const int i = base + threadIdx.x + k * 1024;
+                    if (i < n) s_i2[i] = q[k];
+                }
+            }
         }
         // this lane's pixel in each of its cells: prior and own descriptor; lane
         // 32*ci + w holds word w of cell ci's candidate mask

@@ -35,10 +35,23 @@ __global__ __launch_bounds__(256) void k_descriptor(ElasLaunch L) {
         const int u0 = (r % tx) * DT_W, v0 = (r / tx) * DT_H;
         const uint8_t* I = P.img[img];
         // image window: rows v0-3 .. v0+DT_H+2, cols u0-3 .. u0+DT_W+2
-        for (int i = tid; i < (DT_H + 6) * (DT_W + 6); i += 256) {
-            const int rr = i / (DT_W + 6), cc = i - rr * (DT_W + 6);
-            const int u = u0 - 3 + cc, v = v0 - 3 + rr;
-            s_img[rr][cc] = (u >= 0 && u < W && v >= 0 && v < H) ? I[(size_t)v * W + u] : 0;
+        {
+            // every thread issues all of its loads before storing any
+            constexpr int N = (DT_H + 6) * (DT_W + 6), K = (N + 255) / 256;
+            uint8_t q[K];
+            #pragma unroll
+            for (int k = 0; k < K; k++) {
+                const int i = tid + 256 * k;
+                const int rr = i / (DT_W + 6), cc = i - rr * (DT_W + 6);
+                const int u = u0 - 3 + cc, v = v0 - 3 + rr;
+                q[k] = (i < N && u >= 0 && u < W && v >= 0 && v < H) ? elas_ld(I + (size_t)v * W + u) : (uint8_t)0;
+            }
+            #pragma unroll
+            for (int k = 0; k < K; k++) {
+                const int i = tid + 256 * k;
+                const int rr = i / (DT_W + 6), cc = i - rr * (DT_W + 6);
+                if (i < N) s_img[rr][cc] = q[k];
+            }
         }
         __syncthreads();
         // du on rows v0-2.., cols u0-2..; dv on rows v0-1.., cols u0-1..
@@ -76,7 +89,7 @@ __global__ __launch_bounds__(256) void k_descriptor(ElasLaunch L) {
                 q.w = (uint32_t)s_dv[e - 1][f] | ((uint32_t)s_dv[e][f - 1] << 8)
                     | ((uint32_t)s_dv[e][f + 1] << 16) | ((uint32_t)s_dv[e + 1][f] << 24);
             }
-            *reinterpret_cast<uint4*>(D + ((size_t)v * W + u) * 16) = q;
+            elas_st(reinterpret_cast<uint4*>(D + ((size_t)v * W + u) * 16), q);
         }
         __syncthreads();
     }

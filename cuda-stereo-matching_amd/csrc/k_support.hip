@@ -56,10 +56,22 @@ __global__ __launch_bounds__(256) void k_support(ElasLaunch L, ElasKParams kp) {
         const uint8_t* I1 = P.desc[RIGHT ? 1 : 0] + (size_t)16 * W * v;
         const uint8_t* I2 = P.desc[RIGHT ? 0 : 1] + (size_t)16 * W * v;
         if (row_ok) {
+            // every thread issues all of its loads before storing any
             const uint4* top = reinterpret_cast<const uint4*>(I2 - 32 * (size_t)W) + c_lo;
             const uint4* bot = reinterpret_cast<const uint4*>(I2 + 32 * (size_t)W) + c_lo;
-            for (int i = tid; i < 2 * ncols; i += 256)
-                s_rows[i] = elas_ld(i < ncols ? top + i : bot + (i - ncols));
+            for (int base = 0; base < 2 * ncols; base += 8 * 256) {
+                uint4 q[8];
+                #pragma unroll
+                for (int k = 0; k < 8; k++) {
+                    const int i = base + tid + k * 256;
+                    if (i < 2 * ncols) q[k] = elas_ld(i < ncols ? top + i : bot + (i - ncols));
+                }
+                #pragma unroll
+                for (int k = 0; k < 8; k++) {
+                    const int i = base + tid + k * 256;
+                    if (i < 2 * ncols) s_rows[i] = q[k];
+                }
+            }
         }
         int u = u_can * step;
         bool ok = row_ok && u_can < cw && u >= 5 && u <= W - 6;
