@@ -67,6 +67,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--workload", default="c1", choices=["c0", "c1", "c2"],
                     help="c1 (1242x375, d 0..127) is the headline; c0/c2 for profiling")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only copy a 256 MiB device buffer 5 times (calibrates HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (pairs would shard across GPUs with no "
@@ -74,6 +76,16 @@ def main():
         sys.exit(2)
 
     import torch
+    if args.copy_probe:
+        src = torch.empty(256 << 20, dtype=torch.uint8, device="cuda:0")
+        dst = torch.empty_like(src)
+        src.fill_(1)
+        torch.cuda.synchronize()
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": 256 << 20, "copies": 5}))
+        return
     pkg = __graft_entry__.load_package()
     synth = __graft_entry__.load_synth()
     import oracle
