@@ -98,20 +98,21 @@ __device__ __forceinline__ int elas_plane_d(const float* rec, int u, int v) {
 }
 
 // ---- rasterisation: KerComputeDisparity.cu:126-150 ------------------------
-// One workgroup per 32x32 pixel tile; thread (column c, row group g) owns the
-// 4 pixels (c, g + 8k).  The tile's triangles arrive in ascending index
-// order; for each, the owner computes its column's half-open row span
-// between the long edge AC and AB (u in [A_u, B_u)) or BC (u in [B_u, C_u))
-// and records the triangle for the pixels inside it.  A later triangle
-// overwrites an earlier one, so where spans overlap (the reference races
-// there) the highest index wins -- the result of painting the triangles in
-// index order -- with no atomics.  The owner then stores the plane prior
-// (int)(a*u + b*v + c) of its covering triangle, or -1.
+// One workgroup per 32x32 pixel tile.  The tile's triangles (ascending index,
+// binned on the host) are staged in LDS in chunks; every (triangle, column)
+// pair is one work item: it computes the column's half-open row span between
+// the long edge AC and AB (u in [A_u, B_u)) or BC (u in [B_u, C_u)) and
+// records the triangle id for the span's pixels in an LDS tile with
+// atomicMax.  Where spans overlap (the reference races there) the highest
+// triangle index wins -- the result of painting the triangles in index order.
+// Each pixel's owner then stores the plane prior (int)(a*u + b*v + c) of its
+// triangle, or -1.
 #define RS_CHUNK 128
 __global__ __launch_bounds__(256) void k_raster(ElasLaunch L) {
     __shared__ float s_rec[RS_CHUNK][ELAS_TRIREC];
     __shared__ int s_id[RS_CHUNK];
-    const int c = threadIdx.x & 31, g = threadIdx.x >> 5;
+    __shared__ int s_tile[ELAS_RASTER_T][ELAS_RASTER_T];
+    const int tid = threadIdx.x;
     for (int t = blockIdx.x; t < L.total; t += gridDim.x) {
         const int p = elas_find_pair(L.prefix, L.npairs, t);
         const ElasPair& P = L.pairs[p];
@@ -120,53 +121,56 @@ __global__ __launch_bounds__(256) void k_raster(ElasLaunch L) {
         int local = t - L.prefix[p];
         const int img = local >= ntiles;
         if (img) local -= ntiles;
-        const int u = (local % P.ntx) * ELAS_RASTER_T + c;
-        const int v0 = (local / P.ntx) * ELAS_RASTER_T + g;
+        const int tx0 = (local % P.ntx) * ELAS_RASTER_T, ty0 = (local / P.ntx) * ELAS_RASTER_T;
         const float* recs = P.trirec[img];
         const int* lst = P.tlist[img];
-        const int k0 = P.toff[img][local], k1 = P.toff[img][local + 1];
-        int id0 = -1, id1 = -1, id2 = -1, id3 = -1;
+        const int k0 = elas_ld(P.toff[img] + local), k1 = elas_ld(P.toff[img] + local + 1);
+        #pragma unroll
+        for (int r = 0; r < 4; r++) s_tile[(tid >> 5) + 8 * r][tid & 31] = -1;
+        __syncthreads();
         for (int kc = k0; kc < k1; kc += RS_CHUNK) {
             const int n = min(RS_CHUNK, k1 - kc);
-            // stage the chunk's records: thread j < n fetches triangle j's id and
-            // its 64-byte record with independent loads
-            if ((int)threadIdx.x < n) {
-                const int ti = elas_ld(lst + kc + threadIdx.x);
+            if (tid < n) {
+                const int ti = elas_ld(lst + kc + tid);
                 const uint4* src = reinterpret_cast<const uint4*>(recs + (size_t)ti * ELAS_TRIREC);
                 const uint4 r0 = elas_ld(src), r1 = elas_ld(src + 1), r2 = elas_ld(src + 2), r3 = elas_ld(src + 3);
-                uint4* dst = reinterpret_cast<uint4*>(s_rec[threadIdx.x]);
+                uint4* dst = reinterpret_cast<uint4*>(s_rec[tid]);
                 dst[0] = r0; dst[1] = r1; dst[2] = r2; dst[3] = r3;
-                s_id[threadIdx.x] = ti;
+                s_id[tid] = ti;
             }
             __syncthreads();
-            for (int j = 0; j < n; j++) {
+            for (int item = tid; item < n * ELAS_RASTER_T; item += 256) {
+                const int j = item >> 5, c = item & 31;
+                const int u = tx0 + c;
                 const float* rec = s_rec[j];
                 const int Au = (int)rec[3], Bu = (int)rec[5], Cu = (int)rec[7];
-                if (u < Au || u >= Cu) continue;
+                if (u < Au || u >= Cu || u >= W) continue;
                 const bool first = u < Bu;
                 const float ea = first ? rec[9] : rec[13], eb = first ? rec[10] : rec[14];
                 const float fu = (float)u;
                 const int v1 = (int)elas_f2u_sat(__fadd_rn(__fmul_rn(rec[11], fu), rec[12]));
                 const int v2 = (int)elas_f2u_sat(__fadd_rn(__fmul_rn(ea, fu), eb));
-                const int vlo = max(min(v1, v2), 0), vhi = min(max(v1, v2), H);
+                const int vlo = max(max(min(v1, v2), 0), ty0);
+                const int vhi = min(min(max(v1, v2), H), ty0 + ELAS_RASTER_T);
                 const int ti = s_id[j];
-                if (v0 >= vlo && v0 < vhi) id0 = ti;
-                if (v0 + 8 >= vlo && v0 + 8 < vhi) id1 = ti;
-                if (v0 + 16 >= vlo && v0 + 16 < vhi) id2 = ti;
-                if (v0 + 24 >= vlo && v0 + 24 < vhi) id3 = ti;
+                for (int v = vlo; v < vhi; v++) atomicMax(&s_tile[v - ty0][c], ti);
             }
             __syncthreads();
         }
+        const int c = tid & 31;
+        const int u = tx0 + c;
         if (u < W) {
-            const int ids[4] = {id0, id1, id2, id3};
             #pragma unroll
             for (int r = 0; r < 4; r++) {
-                const int v = v0 + 8 * r;
-                if (v < H)
-                    P.prior[img][(size_t)v * W + u] =
-                        ids[r] >= 0 ? elas_plane_d(recs + (size_t)ids[r] * ELAS_TRIREC, u, v) : -1;
+                const int rr = (tid >> 5) + 8 * r, v = ty0 + rr;
+                if (v < H) {
+                    const int id = s_tile[rr][c];
+                    elas_st(P.prior[img] + (size_t)v * W + u,
+                            id >= 0 ? elas_plane_d(recs + (size_t)id * ELAS_TRIREC, u, v) : -1);
+                }
             }
         }
+        __syncthreads();
     }
 }
 
@@ -270,6 +274,9 @@ __global__ __launch_bounds__(1024) void k_dense(ElasLaunch L, ElasKParams kp) {
             }
         }
         __syncthreads();
+        // Branchless candidate evaluation: a lane that is invalid for a
+        // candidate reads a safe in-tile address and gets the all-ones key;
+        // candidates are taken four at a time so their LDS reads overlap.
         #pragma unroll
         for (int ci = 0; ci < 2; ci++) {
             const int gx = gx0 + wave + ci * ELAS_DENSE_WAVES;
@@ -279,38 +286,66 @@ __global__ __launch_bounds__(1024) void k_dense(ElasLaunch L, ElasKParams kp) {
             const int dp = dpv[ci];
             const uint4 b1 = b1v[ci];
             bool valid = active && u >= 4 && u <= W - 4 && v >= 4 && v <= H - 4 && dp > 0;
-            if (valid && kp.match_texture > 0) {
+            if (kp.match_texture > 0) {
                 const uint32_t tex = elas_sad16(b1, make_uint4(0x80808080u, 0x80808080u, 0x80808080u, 0x80808080u), 0u);
                 if ((int)tex < kp.match_texture) valid = false;
             }
             const int dpmin = max(dp - kp.plane_radius, 0), dpmax = min(dp + kp.plane_radius, P.dmax);
             const int sgn = img ? 1 : -1;
-            const int lds_row = row * ncols - c_lo;
+            const uint4* rowp = s_i2 + (row < rows ? row : 0) * ncols - c_lo;
+            const int usafe = min(max(u, c_lo), c_hi - 1);
+            const uint32_t bias = (uint32_t)kp.key_bias;
             uint32_t kmin = 0xffffffffu;
+            auto grid_key = [&](int d, const uint4 b2, bool ok) {
+                const uint32_t sad = elas_sad16(b1, b2, 0u);
+                return ok ? (((sad + bias) << 11) | (uint32_t)d) : 0xffffffffu;
+            };
+            auto cand_ok = [&](int d, int uw) {
+                bool ok = valid && uw >= 3 && uw <= W - 4;
+                if (!PNEG) ok = ok && (d < dpmin || d > dpmax);
+                return ok;
+            };
             for (int w = 0; w < P.words; w++) {
                 uint32_t bits = __builtin_amdgcn_readlane(maskv, 32 * ci + w);
                 while (bits) {
-                    const int d = w * 32 + __builtin_ctz(bits);
-                    bits &= bits - 1u;
-                    if (valid) {
-                        const int uw = u + sgn * d;
-                        bool ok = uw >= 3 && uw <= W - 4;
-                        if (!PNEG) ok = ok && (d < dpmin || d > dpmax);
-                        if (ok) {
-                            const uint32_t sad = elas_sad16(b1, s_i2[lds_row + uw], 0u);
-                            kmin = min(kmin, ((sad + (uint32_t)kp.key_bias) << 11) | (uint32_t)d);
-                        }
+                    int dq[4];
+                    #pragma unroll
+                    for (int q = 0; q < 4; q++) {
+                        dq[q] = bits ? w * 32 + __builtin_ctz(bits) : dq[0];
+                        bits &= bits - 1u;
                     }
+                    int uw[4];
+                    bool ok[4];
+                    uint4 b2[4];
+                    #pragma unroll
+                    for (int q = 0; q < 4; q++) {
+                        uw[q] = u + sgn * dq[q];
+                        ok[q] = cand_ok(dq[q], uw[q]);
+                        b2[q] = rowp[ok[q] ? uw[q] : usafe];
+                    }
+                    #pragma unroll
+                    for (int q = 0; q < 4; q++) kmin = min(kmin, grid_key(dq[q], b2[q], ok[q]));
                 }
             }
-            if (valid) {
-                #pragma unroll 5
-                for (int d = dpmin; d <= dpmax; d++) {
-                    const int uw = u + sgn * d;
-                    if (uw < 3 || uw > W - 4) continue;
-                    const uint32_t sad = elas_sad16(b1, s_i2[lds_row + uw], 0u);
-                    const uint32_t cost = (uint32_t)((int)sad + kp.P[abs(d - dp)] + kp.key_bias);
-                    kmin = min(kmin, (cost << 11) | 1024u | (uint32_t)d);
+            // plane window d = dp - r .. dp + r with the prior added
+            const int pr = kp.plane_radius;
+            for (int k0 = 0; k0 <= 2 * pr; k0 += 4) {
+                int dq[4], uw[4];
+                bool ok[4];
+                uint4 b2[4];
+                #pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    dq[q] = dp - pr + k0 + q;
+                    uw[q] = u + sgn * dq[q];
+                    ok[q] = valid && k0 + q <= 2 * pr && dq[q] >= dpmin && dq[q] <= dpmax && uw[q] >= 3 && uw[q] <= W - 4;
+                    b2[q] = rowp[ok[q] ? uw[q] : usafe];
+                }
+                #pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const uint32_t sad = elas_sad16(b1, b2[q], 0u);
+                    const int pk = kp.P[min(abs(dq[q] - dp), ELAS_MAX_P - 1)];
+                    const uint32_t key = (((uint32_t)((int)sad + pk) + bias) << 11) | 1024u | (uint32_t)(dq[q] & 1023);
+                    kmin = min(kmin, ok[q] ? key : 0xffffffffu);
                 }
             }
             if (active)
