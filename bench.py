@@ -45,6 +45,37 @@ def work_counts(oracle, left, right, dmax):
     return dict(support_left=w_left, support_right=w_right, dense=out["work_dense"])
 
 
+def batch_vs_plain(pkg, synth, n=64, reps=2):
+    """The mixed batch: n pairs drawn 3:2:1 from 640x480/d64, 1242x375/d128,
+    1920x1080/d256 (seeds 3000+i), host buffers in and out.  One
+    elas_hip_process_batch call (one launch per stage over all pairs, Triangle
+    spread over the host workers) against n plain process calls; outputs must
+    be identical."""
+    pairs = synth.batch_pairs(n)
+    work = sum(l.shape[0] * l.shape[1] * (dm + 1) for l, _, dm in pairs) / 1e6
+    eb = pkg.Elas(pkg.default_params(disp_max=255))
+    ctxs = {}
+    for _, _, dm in pairs:
+        if dm not in ctxs:
+            ctxs[dm] = pkg.Elas(pkg.default_params(disp_max=dm))
+    eb.process_batch(pairs)                       # warm-up (allocations)
+    for l, r, dm in pairs:
+        ctxs[dm].process(l, r)
+    tb = tp = 1e9
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        out_b = eb.process_batch(pairs)
+        tb = min(tb, time.perf_counter() - t0)
+        t0 = time.perf_counter()
+        out_p = [ctxs[dm].process(l, r) for l, r, dm in pairs]
+        tp = min(tp, time.perf_counter() - t0)
+    same = all(np.array_equal(b[0], p[0]) and np.array_equal(b[1], p[1]) for b, p in zip(out_b, out_p))
+    return {"pairs": n, "mpx_disp": round(work, 1), "batched_pairs_per_s": round(n / tb, 2),
+            "plain_pairs_per_s": round(n / tp, 2), "batched_mpx_disp_per_s": round(work / tb, 1),
+            "plain_mpx_disp_per_s": round(work / tp, 1), "speedup": round(tp / tb, 3),
+            "identical_outputs": bool(same), "batch_ms": round(tb * 1e3, 2), "plain_ms": round(tp * 1e3, 2)}
+
+
 def cpu_baseline(oracle, left, right, dmax, budget_s=12.0):
     prm = oracle.params(disp_max=dmax)
     n, t0 = 0, time.perf_counter()
@@ -150,8 +181,16 @@ def main():
                     for k, v in kernels.items()},
         "work": wc,
     }
+    if not args.no_batch:
+        res["batch_vs_plain"] = batch_vs_plain(pkg, synth)
     if not args.no_cpu:
         n, el_s = cpu_baseline(oracle, left, right, dmax, args.cpu_seconds)
+        model = ""
+        try:
+            model = [l.split(":", 1)[1].strip() for l in open("/proc/cpuinfo") if l.startswith("model name")][0]
+        except Exception:
+            pass
+        res["host_cpu"] = {"model": model, "logical_cpus": os.cpu_count()}
         res["cpu_baseline"] = {"value": round(n * mpxd / el_s, 3), "unit": "Mpx*disp/s", "cores": 1,
                                "kind": "port", "sample": "%d full pipelines of the same %dx%d d%d pair "
                                "(oracle, 1 thread, %.1f s)" % (n, W, H, dmax + 1, el_s)}
