@@ -67,6 +67,8 @@ SIGNATURES = {
     "elas_hip_process": (ctypes.c_int, [_P, _P, _P, _P, _P, _P]),
     "elas_hip_process_device": (ctypes.c_int, [_P, _P, _P, _P, _P, ctypes.c_int32, ctypes.c_int32]),
     "elas_hip_process_batch": (ctypes.c_int, [_P, ctypes.c_int32, _P, _P, _P, _P, _P, _P]),
+    "elas_hip_process_batch_device": (ctypes.c_int, [_P, ctypes.c_int32, _P, _P, _P, _P, _P, _P]),
+    "elas_hip_set_pipeline": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32]),
     "elas_hip_set_debug": (ctypes.c_int, [_P, ctypes.c_int]),
     "elas_hip_debug_fetch": (ctypes.c_int, [_P, ctypes.c_int, _P, ctypes.c_size_t,
                                             ctypes.POINTER(ctypes.c_size_t)]),
@@ -101,6 +103,13 @@ def load_library(path=LIB_PATH):
         f.argtypes = args
     _lib = lib
     return lib
+
+
+def _torch_sync(t):
+    """The library enqueues on its own streams: work torch queued on the
+    tensors' current stream has to finish first."""
+    import torch
+    torch.cuda.current_stream(t.device).synchronize()
 
 
 def default_params(**overrides):
@@ -192,9 +201,41 @@ class Elas:
         for t, dt in ((left, "uint8"), (right, "uint8"), (D1, "float32"), (D2, "float32")):
             if not t.is_cuda or not t.is_contiguous() or str(t.dtype) != "torch." + dt or tuple(t.shape) != (H, W):
                 raise ValueError("process_device expects contiguous cuda tensors of shape [H, W]")
+        _torch_sync(left)
         st = self._lib.elas_hip_process_device(self._ctx, left.data_ptr(), right.data_ptr(), D1.data_ptr(),
                                                D2.data_ptr(), W, H)
         return self._check(st, allow_few=True)
+
+    def set_pipeline(self, slots=3, chunk=0):
+        """Batches are cut into chunks of `chunk` pairs (0: one chunk) that
+        move through `slots` pipeline slots, overlapping the GPU stages of one
+        chunk with the host Delaunay step of another."""
+        self._check(self._lib.elas_hip_set_pipeline(self._ctx, int(slots), int(chunk)))
+
+    def process_batch_device(self, lefts, rights, D1s, D2s, disp_max=None):
+        """Device-resident batch: lists of torch cuda tensors (uint8 [H,W] in,
+        float32 [H,W] out), shapes may differ per pair; disp_max is None (the
+        context's) or a per-pair list.  Returns the per-pair statuses."""
+        n = len(lefts)
+        if not (len(rights) == len(D1s) == len(D2s) == n) or n == 0:
+            raise ValueError("need n > 0 matching lists")
+        dims = (ctypes.c_int32 * (3 * n))()
+        for i in range(n):
+            H, W = int(lefts[i].shape[0]), int(lefts[i].shape[1])
+            for t, dt in ((lefts[i], "uint8"), (rights[i], "uint8"), (D1s[i], "float32"), (D2s[i], "float32")):
+                if not t.is_cuda or not t.is_contiguous() or str(t.dtype) != "torch." + dt or tuple(t.shape) != (H, W):
+                    raise ValueError("pair %d: expected contiguous cuda tensors of shape [H, W]" % i)
+            dims[3 * i], dims[3 * i + 1] = W, H
+            dims[3 * i + 2] = -1 if disp_max is None or disp_max[i] is None else int(disp_max[i])
+        arr = lambda xs: (ctypes.c_void_p * n)(*[x.data_ptr() for x in xs])
+        stat = (ctypes.c_int32 * n)()
+        _torch_sync(lefts[0])
+        st = self._lib.elas_hip_process_batch_device(self._ctx, n, ctypes.cast(arr(lefts), _P),
+                                                     ctypes.cast(arr(rights), _P), ctypes.cast(arr(D1s), _P),
+                                                     ctypes.cast(arr(D2s), _P), ctypes.cast(dims, _P),
+                                                     ctypes.cast(stat, _P))
+        self._check(st)
+        return [int(stat[i]) for i in range(n)]
 
     def process_batch(self, pairs):
         """pairs: list of (left, right, disp_max or None).  One launch per stage

@@ -4,6 +4,7 @@
 #include <cstring>
 #include <new>
 #include <string>
+#include <vector>
 
 #include "elas_engine.h"
 #include "elas_hip.h"
@@ -95,38 +96,52 @@ int elas_hip_create(const elas_params* params, int device, elas_ctx** out) {
         return st;
     }
     c->prm = p;
-    if (hipSetDevice(device) != hipSuccess ||
-        hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking) != hipSuccess) {
+    if (hipSetDevice(device) != hipSuccess) {
         (void)hipGetLastError();
         delete c;
         return ELAS_ERR_HIP;
     }
-    for (auto& e : c->ev)
-        if (hipEventCreate(&e) != hipSuccess) {
-            (void)hipGetLastError();
-            delete c;
-            return ELAS_ERR_HIP;
-        }
+    st = elas_hip_set_pipeline(c, 3, 0);
+    if (st != ELAS_OK) {
+        elas_hip_destroy(c);
+        return st;
+    }
     *out = c;
     return ELAS_OK;
 }
 
 int elas_hip_destroy(elas_ctx* c) {
     if (!c) return ELAS_ERR_INVALID_ARGUMENT;
-    (void)hipSetDevice(c->device);
-    if (c->stream) (void)hipStreamSynchronize(c->stream);
-    if (c->stream2) (void)hipStreamSynchronize(c->stream2);
-    for (auto& e : c->ev)
-        if (e) (void)hipEventDestroy(e);
-    if (c->arena) (void)hipFree(c->arena);
-    if (c->table) (void)hipFree(c->table);
-    if (c->pin) (void)hipHostFree(c->pin);
-    if (c->blist) (void)hipFree(c->blist);
-    if (c->blist_pin) (void)hipHostFree(c->blist_pin);
-    if (c->stream) (void)hipStreamDestroy(c->stream);
-    if (c->stream2) (void)hipStreamDestroy(c->stream2);
+    for (ElasSlot* S : c->slots) {
+        c->slot_free(*S);
+        delete S;
+    }
+    c->slots.clear();
     delete c;
+    return ELAS_OK;
+}
+
+int elas_hip_set_pipeline(elas_ctx* c, int32_t slots, int32_t chunk) {
+    if (!c) return ELAS_ERR_INVALID_ARGUMENT;
+    if (slots < 1 || slots > 8 || chunk < 0) {
+        c->err = "need 1 <= slots <= 8 and chunk >= 0";
+        return ELAS_ERR_INVALID_ARGUMENT;
+    }
+    while ((int)c->slots.size() > slots) {
+        ElasSlot* S = c->slots.back();
+        if (c->last == S) c->last = nullptr;
+        c->slot_free(*S);
+        delete S;
+        c->slots.pop_back();
+    }
+    while ((int)c->slots.size() < slots) {
+        ElasSlot* S = new (std::nothrow) ElasSlot();
+        if (!S) return ELAS_ERR_OUT_OF_MEMORY;
+        c->slots.push_back(S);
+        const int st = c->slot_init(*S);
+        if (st != ELAS_OK) return st;
+    }
+    c->chunk = chunk;
     return ELAS_OK;
 }
 
@@ -142,31 +157,42 @@ int elas_hip_set_params(elas_ctx* c, const elas_params* p) {
 
 const char* elas_hip_last_error(const elas_ctx* c) { return c ? c->err.c_str() : "null context"; }
 
-void* elas_hip_stream(elas_ctx* c) { return c ? (void*)c->stream : nullptr; }
+void* elas_hip_stream(elas_ctx* c) { return c && !c->slots.empty() ? (void*)c->slots[0]->stream : nullptr; }
 
 int elas_hip_process(elas_ctx* c, const uint8_t* left, const uint8_t* right, float* D1, float* D2,
                      const int32_t* dims) {
     if (!c || !dims) return ELAS_ERR_INVALID_ARGUMENT;
-    ElasPairSpec s{dims[0], dims[1], c->prm.disp_max, left, right, D1, D2, false, false};
+    ElasPairSpec s{dims[0], dims[1], c->prm.disp_max, left, right, D1, D2, false, false, -1};
     return c->run(&s, 1, nullptr);
 }
 
 int elas_hip_process_device(elas_ctx* c, const uint8_t* d_left, const uint8_t* d_right, float* d_D1,
                             float* d_D2, int32_t width, int32_t height) {
     if (!c) return ELAS_ERR_INVALID_ARGUMENT;
-    ElasPairSpec s{width, height, c->prm.disp_max, d_left, d_right, d_D1, d_D2, true, true};
+    ElasPairSpec s{width, height, c->prm.disp_max, d_left, d_right, d_D1, d_D2, true, true, -1};
     return c->run(&s, 1, nullptr);
 }
 
-int elas_hip_process_batch(elas_ctx* c, int32_t n, const uint8_t* const* left, const uint8_t* const* right,
-                           float* const* D1, float* const* D2, const int32_t* dims, int32_t* statuses) {
+static int batch(elas_ctx* c, int32_t n, const uint8_t* const* left, const uint8_t* const* right,
+                 float* const* D1, float* const* D2, const int32_t* dims, int32_t* statuses, bool dev) {
     if (!c || n <= 0 || !left || !right || !D1 || !D2 || !dims) return ELAS_ERR_INVALID_ARGUMENT;
     std::vector<ElasPairSpec> v(n);
     for (int i = 0; i < n; i++) {
         const int dm = dims[3 * i + 2] < 0 ? c->prm.disp_max : dims[3 * i + 2];
-        v[i] = ElasPairSpec{dims[3 * i], dims[3 * i + 1], dm, left[i], right[i], D1[i], D2[i], false, false};
+        v[i] = ElasPairSpec{dims[3 * i], dims[3 * i + 1], dm, left[i], right[i], D1[i], D2[i], dev, dev, -1};
     }
-    return c->run(v.data(), n, statuses);
+    return c->run_batch(v.data(), n, statuses, c->chunk);
+}
+
+int elas_hip_process_batch(elas_ctx* c, int32_t n, const uint8_t* const* left, const uint8_t* const* right,
+                           float* const* D1, float* const* D2, const int32_t* dims, int32_t* statuses) {
+    return batch(c, n, left, right, D1, D2, dims, statuses, false);
+}
+
+int elas_hip_process_batch_device(elas_ctx* c, int32_t n, const uint8_t* const* d_left,
+                                  const uint8_t* const* d_right, float* const* d_D1, float* const* d_D2,
+                                  const int32_t* dims, int32_t* statuses) {
+    return batch(c, n, d_left, d_right, d_D1, d_D2, dims, statuses, true);
 }
 
 int elas_hip_set_debug(elas_ctx* c, int keep) {
@@ -183,24 +209,25 @@ int elas_hip_debug_fetch(elas_ctx* c, int stage, void* dst, size_t bytes, size_t
 int elas_hip_shape_info(const elas_ctx* c, elas_shape_info* info) {
     if (!c || !info) return ELAS_ERR_INVALID_ARGUMENT;
     memset(info, 0, sizeof(*info));
-    if (c->pairs.empty()) return ELAS_ERR_INVALID_ARGUMENT;
-    const ElasPair& P = c->pairs[0];
+    if (!c->last || c->last->pairs.empty()) return ELAS_ERR_INVALID_ARGUMENT;
+    const ElasSlot& S = *c->last;
+    const ElasPair& P = S.pairs[0];
     info->width = P.W;
     info->height = P.H;
     info->can_width = P.cw;
     info->can_height = P.ch;
     info->grid_width = P.gw;
     info->grid_height = P.gh;
-    info->num_support = c->nsp[0];
-    info->num_tri0 = c->ntri0[0];
-    info->num_tri1 = c->ntri1[0];
-    info->status = c->status[0];
+    info->num_support = S.nsp[0];
+    info->num_tri0 = S.ntri0[0];
+    info->num_tri1 = S.ntri1[0];
+    info->status = S.status[0];
     return ELAS_OK;
 }
 
 int elas_hip_stage_times(const elas_ctx* c, float* ms, int32_t n) {
     if (!c || !ms) return ELAS_ERR_INVALID_ARGUMENT;
-    for (int i = 0; i < n && i < ELAS_T_COUNT; i++) ms[i] = c->times[i];
+    for (int i = 0; i < n && i < ELAS_T_COUNT; i++) ms[i] = c->last ? c->last->times[i] : 0.0f;
     return ELAS_OK;
 }
 

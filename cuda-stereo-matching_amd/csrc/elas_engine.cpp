@@ -3,14 +3,15 @@
 // Stage order, buffer initial values and the host Delaunay excursion follow
 // Elas::process (src/Elas.cu:19-120).  Differences of structure, none of
 // arithmetic:
-//   * every stage is one launch over a whole batch of pairs (a single process
-//     call is a batch of one);
-//   * device buffers live in one arena that grows on demand and is reused
-//     across calls (the reference allocates ~20 buffers per call and leaks
-//     its scan buffers, Elas.cu:349-371);
+//   * every stage is one launch over a whole chunk of pairs (a single process
+//     call is a chunk of one);
+//   * device buffers live in per-slot arenas that grow on demand and are
+//     reused across calls (the reference allocates ~20 buffers per call and
+//     leaks its scan buffers, Elas.cu:349-371);
 //   * the disparity grid (Elas.cu:65) runs on a second stream while the host
-//     triangulates, and the left and right meshes are triangulated in
-//     parallel by independent Triangle instances;
+//     triangulates, the left and right meshes are triangulated in parallel by
+//     independent Triangle instances, and a batch keeps several chunks in
+//     flight so that the GPU phases of one overlap the host step of another;
 //   * buffers the reference initialises with thrust::fill (-1 for D1/D2 and
 //     the plane map, 0 for the grid, Elas.cu:94-95,415-420,488) are written in
 //     full by the kernels themselves, or cleared by fill kernels.
@@ -20,6 +21,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <thread>
 
 #include "elas_launch.h"
 #include "elas_triangle.h"
@@ -32,15 +34,48 @@
 
 namespace {
 
+using clk = std::chrono::steady_clock;
+
 size_t al(size_t x) { return (x + 255) & ~size_t(255); }
 
-struct Dims {
-    int32_t cw, ch, gw, gh, words, nblk, tricap;
-    size_t npx, ncan, ncell;
-};
+int dense_rows(int gs) { return std::max(1, std::min(64 / gs, gs)); }
 
-Dims dims_of(const elas_params& p, int W, int H, int dmax) {
-    Dims d;
+// LDS of a dense tile: R rows x (16*cpw cells x gs + dmax) columns x 16 B
+size_t dense_lds(int gs, int dmax, int cpw) {
+    return (size_t)dense_rows(gs) * (size_t)(ELAS_DENSE_WAVES * cpw * gs + dmax) * 16;
+}
+
+// LDS of a support workgroup: rows v-2 and v+2 over 63*step + dmax + 5
+// (left pass) or 63*step + 2*dmax + 5 (right pass) columns
+size_t support_lds(int step, int dmax, bool right) {
+    return 2 * 16 * (size_t)(63 * step + (right ? 2 : 1) * dmax + 5);
+}
+
+int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
+
+// device bytes of one pair in a slot arena
+size_t pair_arena(const ElasPairSpec& s, const ElasDims& d, bool debug) {
+    size_t b = 0;
+    if (!s.dev_in) b += 2 * al(d.npx);
+    b += 2 * al(16 * d.npx);
+    b += 4 * al(4 * d.ncan) + al(4 * (size_t)d.nblk) + 2 * al(8 * d.ncan);
+    b += 4 * al(4 * d.ncell * d.words);
+    if (debug) b += 2 * al(4 * d.ncell * (size_t)(s.dmax + 2)) + 2 * al(4 * d.npx);
+    b += 2 * al(12 * (size_t)d.tricap) + 2 * al(4 * ELAS_TRIREC * (size_t)d.tricap);
+    b += 2 * al(4 * d.npx) + 2 * al(4 * d.npx);  // prior, draw
+    if (!s.dev_out) b += 2 * al(4 * d.npx);
+    return b;
+}
+
+// one device table: pairs, per-stage tile prefixes, support point counts
+size_t table_one(int n) {
+    return al(sizeof(ElasPair) * n) + al(sizeof(int32_t) * S_NUM_STAGES * (n + 1)) + al(4 * (size_t)n);
+}
+
+}  // namespace
+
+ElasDims elas_dims(const elas_params& p, int W, int H, int dmax) {
+    ElasDims d;
     d.cw = W / p.candidate_stepsize;
     d.ch = H / p.candidate_stepsize;
     d.gw = (int32_t)ceil((float)W / (float)p.grid_size);  // Elas.cu:411-412
@@ -54,16 +89,9 @@ Dims dims_of(const elas_params& p, int W, int H, int dmax) {
     return d;
 }
 
-int dense_rows(int gs) { return std::max(1, std::min(64 / gs, gs)); }
-
-// LDS of a dense tile: R rows x (16*cpw cells x gs + dmax) columns x 16 B
-size_t dense_lds(int gs, int dmax, int cpw) {
-    return (size_t)dense_rows(gs) * (size_t)(ELAS_DENSE_WAVES * cpw * gs + dmax) * 16;
+size_t elas_pair_device_bytes(const elas_params& p, const ElasPairSpec& s, bool debug) {
+    return pair_arena(s, elas_dims(p, s.W, s.H, s.dmax), debug);
 }
-
-int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
-
-}  // namespace
 
 int elas_make_kparams(const elas_params* p, ElasKParams* kp, std::string* err) {
     auto bad = [&](const char* m) {
@@ -105,20 +133,43 @@ int elas_make_kparams(const elas_params* p, ElasKParams* kp, std::string* err) {
     if (pmin < -(1 << 18)) return bad("prior too strong (beta too small)");
     kp->key_bias = -pmin;
     kp->all_p_negative = allneg ? 1 : 0;
+    kp->dense_cpw = 1;
     return ELAS_OK;
 }
 
-int elas_ctx::run(const ElasPairSpec* sp, int n, int32_t* statuses) {
-    using clk = std::chrono::steady_clock;
-    const auto t_start = clk::now();
+// ---- slot lifetime ---------------------------------------------------------
+
+int elas_ctx::slot_init(ElasSlot& S) {
+    HIPCHECK(hipSetDevice(device));
+    HIPCHECK(hipStreamCreateWithFlags(&S.stream, hipStreamNonBlocking));
+    HIPCHECK(hipStreamCreateWithFlags(&S.stream2, hipStreamNonBlocking));
+    for (auto& e : S.ev) HIPCHECK(hipEventCreate(&e));
+    return ELAS_OK;
+}
+
+void elas_ctx::slot_free(ElasSlot& S) {
+    (void)hipSetDevice(device);
+    if (S.stream) (void)hipStreamSynchronize(S.stream);
+    if (S.stream2) (void)hipStreamSynchronize(S.stream2);
+    if (S.ticket) elas_tri_release(S.ticket);
+    S.ticket = nullptr;
+    for (auto& e : S.ev)
+        if (e) (void)hipEventDestroy(e);
+    if (S.arena) (void)hipFree(S.arena);
+    if (S.table) (void)hipFree(S.table);
+    if (S.blist) (void)hipFree(S.blist);
+    if (S.pin) (void)hipHostFree(S.pin);
+    if (S.blist_pin) (void)hipHostFree(S.blist_pin);
+    if (S.stream) (void)hipStreamDestroy(S.stream);
+    if (S.stream2) (void)hipStreamDestroy(S.stream2);
+}
+
+// Validates a chunk; returns the arena and pinned bytes it needs.
+int elas_ctx::slot_need(const ElasPairSpec* sp, int n, size_t* arena_need, size_t* pin_need) {
     if (n <= 0 || !sp) return fail(ELAS_ERR_INVALID_ARGUMENT, "empty batch");
     if (prm.speckle_size > 0 || prm.ipol_gap_width > 0 || prm.filter_median || prm.filter_adaptive_mean)
         return fail(ELAS_ERR_UNSUPPORTED, "post-processing filters are not available on the GPU path yet");
-    HIPCHECK(hipSetDevice(device));
-    specs.assign(sp, sp + n);
-    std::vector<Dims> dd(n);
-    size_t arena_need = 0, pin_sp = 0, pin_tri = 0;
-    std::vector<size_t> base(n);
+    size_t a = 0, psp = 0, ptri = 0;
     for (int i = 0; i < n; i++) {
         const ElasPairSpec& s = sp[i];
         if (s.W < 1 || s.H < 1 || s.W > 16384 || s.H > 16384 || (size_t)s.W * s.H > (size_t)1 << 27)
@@ -129,100 +180,128 @@ int elas_ctx::run(const ElasPairSpec* sp, int n, int32_t* statuses) {
             return fail(ELAS_ERR_INVALID_ARGUMENT, "pair " + std::to_string(i) + ": disp_max out of range");
         if (dense_lds(prm.grid_size, s.dmax, 1) > 65536)
             return fail(ELAS_ERR_UNSUPPORTED, "grid_size/disp_max exceed the dense LDS tile");
-        if (2 * 16 * (size_t)(63 * prm.candidate_stepsize + 2 * s.dmax + 5) > 65536)
+        if (support_lds(prm.candidate_stepsize, s.dmax, true) > 65536)
             return fail(ELAS_ERR_UNSUPPORTED, "candidate_stepsize/disp_max exceed the support LDS rows");
-        Dims d = dims_of(prm, s.W, s.H, s.dmax);
-        dd[i] = d;
-        size_t b = 0;
-        if (!s.dev_in) b += 2 * al(d.npx);
-        b += 2 * al(16 * d.npx);
-        b += 4 * al(4 * d.ncan) + al(4 * (size_t)d.nblk) + 2 * al(8 * d.ncan);
-        b += 4 * al(4 * d.ncell * d.words);
-        if (debug) b += 2 * al(4 * d.ncell * (size_t)(s.dmax + 2)) + 2 * al(4 * d.npx);
-        b += 2 * al(12 * (size_t)d.tricap) + 2 * al(4 * ELAS_TRIREC * (size_t)d.tricap);
-        b += 2 * al(4 * d.npx) + 2 * al(4 * d.npx);  // prior, draw
-        if (!s.dev_out) b += 2 * al(4 * d.npx);
-        base[i] = arena_need;
-        arena_need += b;
-        pin_sp += 2 * 8 * d.ncan;
-        pin_tri += 2 * 12 * (size_t)d.tricap;
+        const ElasDims d = elas_dims(prm, s.W, s.H, s.dmax);
+        a += pair_arena(s, d, debug);
+        psp += 2 * 8 * d.ncan;
+        ptri += 2 * 12 * (size_t)d.tricap;
     }
-    int maxd = 0;
-    for (int i = 0; i < n; i++) maxd = std::max(maxd, sp[i].dmax);
-    const int cpw = dense_lds(prm.grid_size, maxd, 2) <= 65536 ? 2 : 1;
-    ElasKParams kpl = kp;
-    kpl.dense_cpw = cpw;
-    if (arena_need > arena_bytes) {
-        if (arena) (void)hipFree(arena);
-        arena = nullptr;
-        arena_bytes = 0;
+    *arena_need = a;
+    *pin_need = 2 * table_one(n) + al(psp) + al(ptri);
+    return ELAS_OK;
+}
+
+// Grows a slot's buffers; the slot must be idle.
+int elas_ctx::slot_reserve(ElasSlot& S, size_t arena_need, size_t pin_need, int n) {
+    HIPCHECK(hipSetDevice(device));
+    if (arena_need > S.arena_bytes) {
+        if (S.arena) (void)hipFree(S.arena);
+        S.arena = nullptr;
+        S.arena_bytes = 0;
         const size_t want = arena_need + arena_need / 4;
-        if (hipMalloc(&arena, want) != hipSuccess) {
-            arena = nullptr;
+        if (hipMalloc(&S.arena, want) != hipSuccess) {
+            S.arena = nullptr;
             (void)hipGetLastError();
             return fail(ELAS_ERR_OUT_OF_MEMORY, "arena allocation of " + std::to_string(want) + " bytes failed");
         }
-        arena_bytes = want;
+        S.arena_bytes = want;
     }
-    // two device tables (phase A / phase B): pairs, prefixes, nsp
-    const size_t tab_pairs = al(sizeof(ElasPair) * n);
-    const size_t tab_prefix = al(sizeof(int32_t) * S_NUM_STAGES * (n + 1));
-    const size_t tab_one = tab_pairs + tab_prefix + al(4 * (size_t)n);
-    if (2 * tab_one > table_bytes) {
-        if (table) (void)hipFree(table);
-        table = nullptr;
-        table_bytes = 0;
-        if (hipMalloc(&table, 2 * tab_one) != hipSuccess) {
-            table = nullptr;
+    const size_t tab = 2 * table_one(n);
+    if (tab > S.table_bytes) {
+        if (S.table) (void)hipFree(S.table);
+        S.table = nullptr;
+        S.table_bytes = 0;
+        if (hipMalloc(&S.table, tab + tab / 4) != hipSuccess) {
+            S.table = nullptr;
             (void)hipGetLastError();
             return fail(ELAS_ERR_OUT_OF_MEMORY, "table allocation failed");
         }
-        table_bytes = 2 * tab_one;
+        S.table_bytes = tab + tab / 4;
     }
-    const size_t pin_need = 2 * tab_one + al(pin_sp) + al(pin_tri);
-    if (pin_need > pin_bytes) {
-        if (pin) (void)hipHostFree(pin);
-        pin = nullptr;
-        pin_bytes = 0;
-        if (hipHostMalloc(&pin, pin_need + pin_need / 4, hipHostMallocDefault) != hipSuccess) {
-            pin = nullptr;
+    if (pin_need > S.pin_bytes) {
+        if (S.pin) (void)hipHostFree(S.pin);
+        S.pin = nullptr;
+        S.pin_bytes = 0;
+        if (hipHostMalloc(&S.pin, pin_need + pin_need / 4, hipHostMallocDefault) != hipSuccess) {
+            S.pin = nullptr;
             (void)hipGetLastError();
             return fail(ELAS_ERR_OUT_OF_MEMORY, "pinned staging allocation failed");
         }
-        pin_bytes = pin_need + pin_need / 4;
+        S.pin_bytes = pin_need + pin_need / 4;
     }
-    char* A = (char*)arena;
-    char* T = (char*)table;
-    char* PIN = (char*)pin;
-    // device table layout
-    ElasPair* dpairs[2] = {(ElasPair*)T, (ElasPair*)(T + tab_one)};
-    int32_t* dprefix[2] = {(int32_t*)(T + tab_pairs), (int32_t*)(T + tab_one + tab_pairs)};
-    int32_t* dnsp = (int32_t*)(T + tab_pairs + tab_prefix);
-    ElasPair* hpairs = (ElasPair*)PIN;
-    int32_t* hprefix = (int32_t*)(PIN + tab_pairs);
-    int32_t* hnsp = (int32_t*)(PIN + tab_pairs + tab_prefix);
-    ElasPair* hpairs_b = (ElasPair*)(PIN + tab_one);
-    int32_t* hprefix_b = (int32_t*)(PIN + tab_one + tab_pairs);
-    float* hsp = (float*)(PIN + 2 * tab_one);
-    int32_t* htri = (int32_t*)(PIN + 2 * tab_one + al(pin_sp));
+    return ELAS_OK;
+}
 
-    pairs.assign(n, ElasPair());
-    prefix.assign((size_t)S_NUM_STAGES * (n + 1), 0);
-    nsp.assign(n, 0);
-    ntri0.assign(n, 0);
-    ntri1.assign(n, 0);
-    status.assign(n, ELAS_OK);
-    std::vector<size_t> sp_off(n), tri_off(n);
-    size_t so = 0, to = 0;
+// ---- chunk layout ------------------------------------------------------------
+
+int elas_ctx::slot_prepare(ElasSlot& S, const ElasPairSpec* sp, int n) {
+    S.t_start = clk::now();
+    size_t arena_need = 0, pin_need = 0;
+    int st = slot_need(sp, n, &arena_need, &pin_need);
+    if (st != ELAS_OK) return st;
+    st = slot_reserve(S, arena_need, pin_need, n);
+    if (st != ELAS_OK) return st;
+    S.n = n;
+    S.specs.assign(sp, sp + n);
+    S.dd.resize(n);
+    S.base.resize(n);
+    S.sp_off.resize(n);
+    S.tri_off.resize(n);
+    S.maxd = 0;
+    size_t a = 0, so = 0, to = 0, psp = 0, ptri = 0;
+    for (int i = 0; i < n; i++) {
+        S.dd[i] = elas_dims(prm, sp[i].W, sp[i].H, sp[i].dmax);
+        S.base[i] = a;
+        a += pair_arena(sp[i], S.dd[i], debug);
+        S.sp_off[i] = so;
+        so += 2 * 2 * S.dd[i].ncan;
+        S.tri_off[i] = to;
+        to += 2 * 3 * (size_t)S.dd[i].tricap;
+        psp += 2 * 8 * S.dd[i].ncan;
+        ptri += 2 * 12 * (size_t)S.dd[i].tricap;
+        S.maxd = std::max(S.maxd, sp[i].dmax);
+    }
+    S.cpw = dense_lds(prm.grid_size, S.maxd, 2) <= 65536 ? 2 : 1;
+    S.kpl = kp;
+    S.kpl.dense_cpw = S.cpw;
+    S.pin_sp = psp;
+    S.pin_tri = ptri;
+    // device tables (phase A / phase B) and their pinned mirrors
+    S.tab_pairs = al(sizeof(ElasPair) * n);
+    S.tab_prefix = al(sizeof(int32_t) * S_NUM_STAGES * (n + 1));
+    S.tab_one = S.tab_pairs + S.tab_prefix + al(4 * (size_t)n);
+    char* A = (char*)S.arena;
+    char* T = (char*)S.table;
+    char* PIN = (char*)S.pin;
+    S.dpairs[0] = (ElasPair*)T;
+    S.dpairs[1] = (ElasPair*)(T + S.tab_one);
+    S.dprefix[0] = (int32_t*)(T + S.tab_pairs);
+    S.dprefix[1] = (int32_t*)(T + S.tab_one + S.tab_pairs);
+    S.dnsp = (int32_t*)(T + S.tab_pairs + S.tab_prefix);
+    S.hpairs[0] = (ElasPair*)PIN;
+    S.hprefix[0] = (int32_t*)(PIN + S.tab_pairs);
+    S.hnsp = (int32_t*)(PIN + S.tab_pairs + S.tab_prefix);
+    S.hpairs[1] = (ElasPair*)(PIN + S.tab_one);
+    S.hprefix[1] = (int32_t*)(PIN + S.tab_one + S.tab_pairs);
+    S.hsp = (float*)(PIN + 2 * S.tab_one);
+    S.htri = (int32_t*)(PIN + 2 * S.tab_one + al(psp));
+
+    S.pairs.assign(n, ElasPair());
+    S.prefix.assign((size_t)S_NUM_STAGES * (n + 1), 0);
+    S.nsp.assign(n, 0);
+    S.ntri0.assign(n, 0);
+    S.ntri1.assign(n, 0);
+    S.status.assign(n, ELAS_OK);
     for (int i = 0; i < n; i++) {
         const ElasPairSpec& s = sp[i];
-        const Dims& d = dd[i];
-        ElasPair& P = pairs[i];
+        const ElasDims& d = S.dd[i];
+        ElasPair& P = S.pairs[i];
         memset(&P, 0, sizeof(P));
         P.W = s.W; P.H = s.H; P.dmin = prm.disp_min; P.dmax = s.dmax;
         P.cw = d.cw; P.ch = d.ch; P.gw = d.gw; P.gh = d.gh; P.words = d.words;
         P.step = prm.candidate_stepsize; P.gsize = prm.grid_size; P.nblk = d.nblk;
-        char* c = A + base[i];
+        char* c = A + S.base[i];
         auto take = [&](size_t bytes) { char* r = c; c += al(bytes); return r; };
         if (!s.dev_in) { P.img[0] = (uint8_t*)take(d.npx); P.img[1] = (uint8_t*)take(d.npx); }
         else { P.img[0] = s.left; P.img[1] = s.right; }
@@ -230,7 +309,7 @@ int elas_ctx::run(const ElasPairSpec* sp, int n, int32_t* statuses) {
         for (int k = 0; k < 4; k++) P.can[k] = (int32_t*)take(4 * d.ncan);
         P.blkcnt = (int32_t*)take(4 * (size_t)d.nblk);
         for (int k = 0; k < 2; k++) P.sp[k] = (float*)take(8 * d.ncan);
-        P.nsp = dnsp + i;
+        P.nsp = S.dnsp + i;
         for (int k = 0; k < 2; k++) P.gflag[k] = (uint32_t*)take(4 * d.ncell * d.words);
         for (int k = 0; k < 2; k++) P.gmask[k] = (uint32_t*)take(4 * d.ncell * d.words);
         if (debug) {
@@ -245,269 +324,426 @@ int elas_ctx::run(const ElasPairSpec* sp, int n, int32_t* statuses) {
         for (int k = 0; k < 2; k++) P.draw[k] = (float*)take(4 * d.npx);
         if (!s.dev_out) { P.dout[0] = (float*)take(4 * d.npx); P.dout[1] = (float*)take(4 * d.npx); }
         else { P.dout[0] = s.D1; P.dout[1] = s.D2; }
-        sp_off[i] = so;
-        so += 2 * 2 * d.ncan;
-        tri_off[i] = to;
-        to += 2 * 3 * (size_t)d.tricap;
     }
-    auto tiles = [&](int i, int st) -> int {
-        const ElasPairSpec& s = sp[i];
-        const Dims& d = dd[i];
-        const int gs = prm.grid_size, R = dense_rows(gs);
-        switch (st) {
-            case S_DESC: return cdiv(s.W, 64) * cdiv(s.H, 16) * 2;
-            case S_SUPPORT: return d.ch * cdiv(d.cw, 64);
-            case S_INCON: return cdiv(d.cw, 16) * cdiv(d.ch, 16);
-            case S_REDUN_H: return cdiv(d.ch, 64);
-            case S_REDUN_V: return cdiv(d.cw, 64);
-            case S_BLOCKS: return d.nblk;
-            case S_GRID_CELLS: return cdiv(2 * (long)d.ncell, 256);
-            case S_GRID_FILL: return cdiv(2 * (long)d.ncell * d.words, 256);
-            case S_PLANE: return cdiv((long)pairs[i].ntri[0] + pairs[i].ntri[1], 256);
-            case S_RASTER: return 2 * pairs[i].ntx * pairs[i].nty;
-            case S_DENSE: return 2 * d.gh * cdiv(gs, R) * cdiv(d.gw, ELAS_DENSE_WAVES * cpw);
-            case S_PIXELS: return cdiv((long)d.npx, 256);
-        }
-        return 0;
-    };
-    auto build_prefix = [&](int32_t* out) {
-        for (int st = 0; st < S_NUM_STAGES; st++) {
-            int32_t* pf = out + (size_t)st * (n + 1);
-            pf[0] = 0;
-            for (int i = 0; i < n; i++) pf[i + 1] = pf[i] + tiles(i, st);
-        }
-    };
-    build_prefix(prefix.data());
-    memcpy(hpairs, pairs.data(), sizeof(ElasPair) * n);
-    memcpy(hprefix, prefix.data(), sizeof(int32_t) * prefix.size());
-    HIPCHECK(hipMemcpyAsync(dpairs[0], hpairs, sizeof(ElasPair) * n, hipMemcpyHostToDevice, stream));
-    HIPCHECK(hipMemcpyAsync(dprefix[0], hprefix, sizeof(int32_t) * prefix.size(), hipMemcpyHostToDevice, stream));
+    return ELAS_OK;
+}
+
+namespace {
+
+// workgroup tiles of stage st for one pair
+int stage_tiles(const elas_params& prm, const ElasPairSpec& s, const ElasDims& d, const ElasPair& P, int cpw,
+                int st) {
+    const int gs = prm.grid_size, R = dense_rows(gs);
+    switch (st) {
+        case S_DESC: return cdiv(s.W, 64) * cdiv(s.H, 16) * 2;
+        case S_SUPPORT: return d.ch * cdiv(d.cw, 64);
+        case S_INCON: return cdiv(d.cw, 16) * cdiv(d.ch, 16);
+        case S_REDUN_H: return cdiv(d.ch, 64);
+        case S_REDUN_V: return cdiv(d.cw, 64);
+        case S_BLOCKS: return d.nblk;
+        case S_GRID_CELLS: return cdiv(2 * (long)d.ncell, 256);
+        case S_GRID_FILL: return cdiv(2 * (long)d.ncell * d.words, 256);
+        case S_PLANE: return cdiv((long)P.ntri[0] + P.ntri[1], 256);
+        case S_RASTER: return 2 * P.ntx * P.nty;
+        case S_DENSE: return 2 * d.gh * cdiv(gs, R) * cdiv(d.gw, ELAS_DENSE_WAVES * cpw);
+        case S_PIXELS: return cdiv((long)d.npx, 256);
+    }
+    return 0;
+}
+
+void build_prefix(const elas_params& prm, ElasSlot& S) {
+    const int n = S.n;
+    for (int st = 0; st < S_NUM_STAGES; st++) {
+        int32_t* pf = S.prefix.data() + (size_t)st * (n + 1);
+        pf[0] = 0;
+        for (int i = 0; i < n; i++) pf[i + 1] = pf[i] + stage_tiles(prm, S.specs[i], S.dd[i], S.pairs[i], S.cpw, st);
+    }
+}
+
+ElasLaunch launch_of(const ElasSlot& S, int table, int st) {
+    ElasLaunch L;
+    L.pairs = S.dpairs[table];
+    L.prefix = S.dprefix[table] + (size_t)st * (S.n + 1);
+    L.npairs = S.n;
+    L.total = S.hprefix[table][(size_t)st * (S.n + 1) + S.n];
+    return L;
+}
+
+}  // namespace
+
+// ---- phase A: descriptors, support points, filters, compaction, grid -------
+
+int elas_ctx::slot_phase_a(ElasSlot& S) {
+    const int n = S.n;
+    HIPCHECK(hipSetDevice(device));
+    build_prefix(prm, S);
+    memcpy(S.hpairs[0], S.pairs.data(), sizeof(ElasPair) * n);
+    memcpy(S.hprefix[0], S.prefix.data(), sizeof(int32_t) * S.prefix.size());
+    hipStream_t s1 = S.stream, s2 = S.stream2;
+    HIPCHECK(hipMemcpyAsync(S.dpairs[0], S.hpairs[0], sizeof(ElasPair) * n, hipMemcpyHostToDevice, s1));
+    HIPCHECK(hipMemcpyAsync(S.dprefix[0], S.hprefix[0], sizeof(int32_t) * S.prefix.size(), hipMemcpyHostToDevice, s1));
     for (int i = 0; i < n; i++)
-        if (!sp[i].dev_in) {
-            HIPCHECK(hipMemcpyAsync((void*)pairs[i].img[0], sp[i].left, dd[i].npx, hipMemcpyHostToDevice, stream));
-            HIPCHECK(hipMemcpyAsync((void*)pairs[i].img[1], sp[i].right, dd[i].npx, hipMemcpyHostToDevice, stream));
+        if (!S.specs[i].dev_in) {
+            HIPCHECK(hipMemcpyAsync((void*)S.pairs[i].img[0], S.specs[i].left, S.dd[i].npx, hipMemcpyHostToDevice, s1));
+            HIPCHECK(hipMemcpyAsync((void*)S.pairs[i].img[1], S.specs[i].right, S.dd[i].npx, hipMemcpyHostToDevice, s1));
         }
-    auto launch = [&](int table, int st) {
-        ElasLaunch L;
-        L.pairs = dpairs[table];
-        L.prefix = dprefix[table] + (size_t)st * (n + 1);
-        L.npairs = n;
-        L.total = (table ? hprefix_b : hprefix)[(size_t)st * (n + 1) + n];
-        return L;
-    };
-    const int cap256 = std::max(1, num_cu) * 8, cap1024 = std::max(1, num_cu) * 2;
+    const int cap256 = std::max(1, num_cu) * 8;
     auto g = [&](const ElasLaunch& L, int cap) { return std::min(L.total, cap); };
-
-    // ---- phase A: descriptors, support points, filters, compaction ----------
-    HIPCHECK(hipEventRecord(ev[0], stream));
-    { ElasLaunch L = launch(0, S_DESC); if (L.total) elas_launch_descriptor(L, g(L, cap256), stream); }
-    HIPCHECK(hipEventRecord(ev[1], stream));
-    { ElasLaunch L = launch(0, S_SUPPORT);
-      // LDS: rows v-2 and v+2 over 63*step + dmax + 5 (left) or + 2*dmax + 5 (right) columns
-      const size_t lds_l = 2 * 16 * (size_t)(63 * prm.candidate_stepsize + maxd + 5);
-      const size_t lds_r = 2 * 16 * (size_t)(63 * prm.candidate_stepsize + 2 * maxd + 5);
-      if (L.total) { elas_launch_support(L, kp, false, g(L, cap256), lds_l, stream);
-                     elas_launch_support(L, kp, true, g(L, cap256), lds_r, stream); } }
-    HIPCHECK(hipEventRecord(ev[2], stream));
-    { ElasLaunch L = launch(0, S_INCON); if (L.total) elas_launch_incon(L, kp, g(L, cap256), stream); }
-    { ElasLaunch L = launch(0, S_REDUN_H); if (L.total) elas_launch_redun(L, false, L.total, stream); }
-    { ElasLaunch L = launch(0, S_REDUN_V); if (L.total) elas_launch_redun(L, true, L.total, stream); }
-    HIPCHECK(hipEventRecord(ev[3], stream));
-    { ElasLaunch L = launch(0, S_BLOCKS);
-      if (L.total) { elas_launch_blkcount(L, g(L, cap256), stream); elas_launch_compact(L, g(L, cap256), stream); } }
+    HIPCHECK(hipEventRecord(S.ev[0], s1));
+    { ElasLaunch L = launch_of(S, 0, S_DESC); if (L.total) elas_launch_descriptor(L, g(L, cap256), s1); }
+    HIPCHECK(hipEventRecord(S.ev[1], s1));
+    { ElasLaunch L = launch_of(S, 0, S_SUPPORT);
+      if (L.total) {
+          elas_launch_support(L, kp, false, g(L, cap256), support_lds(prm.candidate_stepsize, S.maxd, false), s1);
+          elas_launch_support(L, kp, true, g(L, cap256), support_lds(prm.candidate_stepsize, S.maxd, true), s1);
+      } }
+    HIPCHECK(hipEventRecord(S.ev[2], s1));
+    { ElasLaunch L = launch_of(S, 0, S_INCON); if (L.total) elas_launch_incon(L, kp, g(L, cap256), s1); }
+    { ElasLaunch L = launch_of(S, 0, S_REDUN_H); if (L.total) elas_launch_redun(L, false, L.total, s1); }
+    { ElasLaunch L = launch_of(S, 0, S_REDUN_V); if (L.total) elas_launch_redun(L, true, L.total, s1); }
+    HIPCHECK(hipEventRecord(S.ev[3], s1));
+    { ElasLaunch L = launch_of(S, 0, S_BLOCKS);
+      if (L.total) { elas_launch_blkcount(L, g(L, cap256), s1); elas_launch_compact(L, g(L, cap256), s1); } }
     HIPCHECK(hipGetLastError());
-    HIPCHECK(hipEventRecord(ev[4], stream));
-    HIPCHECK(hipMemcpyAsync(hnsp, dnsp, 4 * (size_t)n, hipMemcpyDeviceToHost, stream));
-    HIPCHECK(hipEventRecord(ev[5], stream));
+    HIPCHECK(hipEventRecord(S.ev[4], s1));
+    HIPCHECK(hipMemcpyAsync(S.hnsp, S.dnsp, 4 * (size_t)n, hipMemcpyDeviceToHost, s1));
+    HIPCHECK(hipEventRecord(S.ev[5], s1));
     // disparity grid on the second stream, overlapping the host Triangle step
-    HIPCHECK(hipStreamWaitEvent(stream2, ev[4], 0));
-    HIPCHECK(hipEventRecord(ev[11], stream2));
-    { ElasLaunch L = launch(0, S_GRID_FILL); if (L.total) elas_launch_grid_fill(L, g(L, cap256), stream2); }
-    { ElasLaunch L = launch(0, S_BLOCKS); if (L.total) elas_launch_grid_scatter(L, g(L, cap256), stream2); }
-    { ElasLaunch L = launch(0, S_GRID_CELLS);
-      if (L.total) { elas_launch_grid_dilate(L, g(L, cap256), stream2);
-                     if (debug) elas_launch_grid_list(L, g(L, cap256), stream2); } }
+    HIPCHECK(hipStreamWaitEvent(s2, S.ev[4], 0));
+    HIPCHECK(hipEventRecord(S.ev[11], s2));
+    { ElasLaunch L = launch_of(S, 0, S_GRID_FILL); if (L.total) elas_launch_grid_fill(L, g(L, cap256), s2); }
+    { ElasLaunch L = launch_of(S, 0, S_BLOCKS); if (L.total) elas_launch_grid_scatter(L, g(L, cap256), s2); }
+    { ElasLaunch L = launch_of(S, 0, S_GRID_CELLS);
+      if (L.total) { elas_launch_grid_dilate(L, g(L, cap256), s2);
+                     if (debug) elas_launch_grid_list(L, g(L, cap256), s2); } }
     HIPCHECK(hipGetLastError());
-    HIPCHECK(hipEventRecord(ev[12], stream2));
+    HIPCHECK(hipEventRecord(S.ev[12], s2));
+    S.phase = SLOT_A;
+    return ELAS_OK;
+}
 
-    // ---- host: support points down, Triangle, triangles up ------------------
-    HIPCHECK(hipEventSynchronize(ev[5]));
-    const auto t_tri0 = clk::now();
+bool elas_ctx::slot_a_ready(ElasSlot& S) { return hipEventQuery(S.ev[5]) == hipSuccess; }
+
+// ---- host step: support points down, Triangle jobs queued --------------------
+
+int elas_ctx::slot_start_tri(ElasSlot& S) {
+    const int n = S.n;
+    HIPCHECK(hipSetDevice(device));
+    HIPCHECK(hipEventSynchronize(S.ev[5]));
+    S.t_tri0 = clk::now();
     for (int i = 0; i < n; i++) {
-        nsp[i] = dd[i].nblk > 0 ? hnsp[i] : 0;
-        if (nsp[i] > 0) {
-            HIPCHECK(hipMemcpyAsync(hsp + sp_off[i], pairs[i].sp[0], 8 * (size_t)nsp[i], hipMemcpyDeviceToHost, stream));
-            HIPCHECK(hipMemcpyAsync(hsp + sp_off[i] + 2 * dd[i].ncan, pairs[i].sp[1], 8 * (size_t)nsp[i],
-                                    hipMemcpyDeviceToHost, stream));
+        S.nsp[i] = S.dd[i].nblk > 0 ? S.hnsp[i] : 0;
+        if (S.nsp[i] > 0) {
+            HIPCHECK(hipMemcpyAsync(S.hsp + S.sp_off[i], S.pairs[i].sp[0], 8 * (size_t)S.nsp[i],
+                                    hipMemcpyDeviceToHost, S.stream));
+            HIPCHECK(hipMemcpyAsync(S.hsp + S.sp_off[i] + 2 * S.dd[i].ncan, S.pairs[i].sp[1], 8 * (size_t)S.nsp[i],
+                                    hipMemcpyDeviceToHost, S.stream));
         }
     }
-    HIPCHECK(hipStreamSynchronize(stream));
-    std::vector<std::vector<int32_t>> tri_out(2 * (size_t)n), toff(2 * (size_t)n), tlist(2 * (size_t)n);
-    std::vector<ElasTriJob> jobs;
-    std::vector<int> job_of(2 * (size_t)n, -1);
+    HIPCHECK(hipStreamSynchronize(S.stream));
+    S.tri_out.assign(2 * (size_t)n, {});
+    S.toff.assign(2 * (size_t)n, {});
+    S.tlist.assign(2 * (size_t)n, {});
+    S.jobs.clear();
+    S.job_of.assign(2 * (size_t)n, -1);
     for (int i = 0; i < n; i++) {
-        if (nsp[i] < 3) { status[i] = ELAS_ERR_TOO_FEW_POINTS; continue; }
+        if (S.nsp[i] < 3) { S.status[i] = ELAS_ERR_TOO_FEW_POINTS; continue; }
         for (int k = 0; k < 2; k++) {
-            job_of[2 * i + k] = (int)jobs.size();
+            S.job_of[2 * i + k] = (int)S.jobs.size();
             ElasTriJob j;
-            j.pts = hsp + sp_off[i] + (size_t)k * 2 * dd[i].ncan;
-            j.n = nsp[i];
-            j.out = &tri_out[2 * i + k];
+            j.pts = S.hsp + S.sp_off[i] + (size_t)k * 2 * S.dd[i].ncan;
+            j.n = S.nsp[i];
+            j.out = &S.tri_out[2 * i + k];
             j.result = 0;
-            j.W = sp[i].W;
-            j.H = sp[i].H;
-            j.tile_off = &toff[2 * i + k];
-            j.tile_list = &tlist[2 * i + k];
-            jobs.push_back(j);
+            j.W = S.specs[i].W;
+            j.H = S.specs[i].H;
+            j.tile_off = &S.toff[2 * i + k];
+            j.tile_list = &S.tlist[2 * i + k];
+            S.jobs.push_back(j);
         }
     }
-    elas_triangulate_jobs(jobs.data(), (int)jobs.size(), tri_workers);
+    if (tri_workers <= 1) {
+        elas_triangulate_jobs(S.jobs.data(), (int)S.jobs.size(), 1);
+        S.ticket = elas_tri_submit(nullptr, 0);
+    } else {
+        S.ticket = elas_tri_submit(S.jobs.data(), (int)S.jobs.size());
+    }
+    S.phase = SLOT_TRI;
+    return ELAS_OK;
+}
+
+bool elas_ctx::slot_tri_ready(ElasSlot& S) { return elas_tri_done(S.ticket); }
+
+// ---- phase B: plane prior, dense matching, left/right check ------------------
+
+int elas_ctx::slot_phase_b(ElasSlot& S) {
+    const int n = S.n;
+    HIPCHECK(hipSetDevice(device));
+    elas_tri_release(S.ticket);  // waits for the meshes
+    S.ticket = nullptr;
     for (int i = 0; i < n; i++) {
         int nt[2] = {0, 0};
-        if (status[i] == ELAS_OK) {
+        if (S.status[i] == ELAS_OK) {
             for (int k = 0; k < 2; k++) {
-                const int r = jobs[job_of[2 * i + k]].result;
+                const int r = S.jobs[S.job_of[2 * i + k]].result;
                 nt[k] = r < 0 ? 0 : r;
-                if (r < 0) status[i] = ELAS_ERR_TOO_FEW_POINTS;
+                if (r < 0) S.status[i] = ELAS_ERR_TOO_FEW_POINTS;
             }
-            if (nt[0] > dd[i].tricap || nt[1] > dd[i].tricap)
+            if (nt[0] > S.dd[i].tricap || nt[1] > S.dd[i].tricap)
                 return fail(ELAS_ERR_HIP, "triangle count exceeds capacity");
         }
-        if (status[i] != ELAS_OK) {
+        if (S.status[i] != ELAS_OK) {
             nt[0] = nt[1] = 0;
             // a pair without a mesh rasterises nothing: every tile's list is empty
             for (int k = 0; k < 2; k++)
-                if (job_of[2 * i + k] >= 0) {
-                    std::fill(toff[2 * i + k].begin(), toff[2 * i + k].end(), 0);
-                    tlist[2 * i + k].clear();
+                if (S.job_of[2 * i + k] >= 0) {
+                    std::fill(S.toff[2 * i + k].begin(), S.toff[2 * i + k].end(), 0);
+                    S.tlist[2 * i + k].clear();
                 }
         }
-        ntri0[i] = nt[0];
-        ntri1[i] = nt[1];
-        pairs[i].ntri[0] = nt[0];
-        pairs[i].ntri[1] = nt[1];
+        S.ntri0[i] = nt[0];
+        S.ntri1[i] = nt[1];
+        S.pairs[i].ntri[0] = nt[0];
+        S.pairs[i].ntri[1] = nt[1];
         for (int k = 0; k < 2; k++)
             if (nt[k] > 0) {
-                int32_t* dst = htri + tri_off[i] + (size_t)k * 3 * dd[i].tricap;
-                memcpy(dst, tri_out[2 * i + k].data(), 12 * (size_t)nt[k]);
-                HIPCHECK(hipMemcpyAsync((void*)pairs[i].tri[k], dst, 12 * (size_t)nt[k], hipMemcpyHostToDevice, stream));
+                int32_t* dst = S.htri + S.tri_off[i] + (size_t)k * 3 * S.dd[i].tricap;
+                memcpy(dst, S.tri_out[2 * i + k].data(), 12 * (size_t)nt[k]);
+                HIPCHECK(hipMemcpyAsync((void*)S.pairs[i].tri[k], dst, 12 * (size_t)nt[k], hipMemcpyHostToDevice,
+                                        S.stream));
             }
     }
     // raster bins: [toff0 | toff1 | tlist0 | tlist1] per pair in one upload
     size_t bl_ints = 0;
     std::vector<size_t> bl_off(n);
     for (int i = 0; i < n; i++) {
-        const size_t nt = (size_t)pairs[i].ntx * pairs[i].nty + 1;
+        const size_t nt = (size_t)S.pairs[i].ntx * S.pairs[i].nty + 1;
         bl_off[i] = bl_ints;
         bl_ints += 2 * nt;
-        for (int k = 0; k < 2; k++) bl_ints += (job_of[2 * i + k] >= 0) ? tlist[2 * i + k].size() : 0;
+        for (int k = 0; k < 2; k++) bl_ints += (S.job_of[2 * i + k] >= 0) ? S.tlist[2 * i + k].size() : 0;
         bl_ints = (bl_ints + 63) & ~(size_t)63;
     }
-    if (4 * bl_ints > blist_bytes) {
-        if (blist) (void)hipFree(blist);
-        blist = nullptr;
-        blist_bytes = 0;
-        if (hipMalloc(&blist, 4 * bl_ints + 4 * bl_ints / 4 + 256) != hipSuccess) {
-            blist = nullptr;
+    const size_t bl_want = 4 * bl_ints + 4 * bl_ints / 4 + 256;
+    if (4 * bl_ints > S.blist_bytes) {
+        if (S.blist) (void)hipFree(S.blist);
+        S.blist = nullptr;
+        S.blist_bytes = 0;
+        if (hipMalloc(&S.blist, bl_want) != hipSuccess) {
+            S.blist = nullptr;
             (void)hipGetLastError();
             return fail(ELAS_ERR_OUT_OF_MEMORY, "raster bin allocation failed");
         }
-        blist_bytes = 4 * bl_ints + 4 * bl_ints / 4 + 256;
+        S.blist_bytes = bl_want;
     }
-    if (4 * bl_ints > blist_pin_bytes) {
-        if (blist_pin) (void)hipHostFree(blist_pin);
-        blist_pin = nullptr;
-        blist_pin_bytes = 0;
-        if (hipHostMalloc(&blist_pin, 4 * bl_ints + 4 * bl_ints / 4 + 256, hipHostMallocDefault) != hipSuccess) {
-            blist_pin = nullptr;
+    if (4 * bl_ints > S.blist_pin_bytes) {
+        if (S.blist_pin) (void)hipHostFree(S.blist_pin);
+        S.blist_pin = nullptr;
+        S.blist_pin_bytes = 0;
+        if (hipHostMalloc(&S.blist_pin, bl_want, hipHostMallocDefault) != hipSuccess) {
+            S.blist_pin = nullptr;
             (void)hipGetLastError();
             return fail(ELAS_ERR_OUT_OF_MEMORY, "raster bin staging allocation failed");
         }
-        blist_pin_bytes = 4 * bl_ints + 4 * bl_ints / 4 + 256;
+        S.blist_pin_bytes = bl_want;
     }
     {
-        int32_t* hb = (int32_t*)blist_pin;
-        int32_t* db = (int32_t*)blist;
+        int32_t* hb = (int32_t*)S.blist_pin;
+        int32_t* db = (int32_t*)S.blist;
         for (int i = 0; i < n; i++) {
-            const size_t nt = (size_t)pairs[i].ntx * pairs[i].nty + 1;
+            const size_t nt = (size_t)S.pairs[i].ntx * S.pairs[i].nty + 1;
             size_t o = bl_off[i];
             for (int k = 0; k < 2; k++) {
                 int32_t* dst = hb + o;
-                if (job_of[2 * i + k] >= 0) memcpy(dst, toff[2 * i + k].data(), 4 * nt);
+                if (S.job_of[2 * i + k] >= 0) memcpy(dst, S.toff[2 * i + k].data(), 4 * nt);
                 else memset(dst, 0, 4 * nt);
-                pairs[i].toff[k] = db + o;
+                S.pairs[i].toff[k] = db + o;
                 o += nt;
             }
             for (int k = 0; k < 2; k++) {
-                const size_t nl = (job_of[2 * i + k] >= 0) ? tlist[2 * i + k].size() : 0;
-                if (nl) memcpy(hb + o, tlist[2 * i + k].data(), 4 * nl);
-                pairs[i].tlist[k] = db + o;
+                const size_t nl = (S.job_of[2 * i + k] >= 0) ? S.tlist[2 * i + k].size() : 0;
+                if (nl) memcpy(hb + o, S.tlist[2 * i + k].data(), 4 * nl);
+                S.pairs[i].tlist[k] = db + o;
                 o += nl;
             }
         }
-        if (bl_ints) HIPCHECK(hipMemcpyAsync(blist, blist_pin, 4 * bl_ints, hipMemcpyHostToDevice, stream));
+        if (bl_ints) HIPCHECK(hipMemcpyAsync(S.blist, S.blist_pin, 4 * bl_ints, hipMemcpyHostToDevice, S.stream));
     }
-    build_prefix(prefix.data());
-    memcpy(hpairs_b, pairs.data(), sizeof(ElasPair) * n);
-    memcpy(hprefix_b, prefix.data(), sizeof(int32_t) * prefix.size());
-    HIPCHECK(hipMemcpyAsync(dpairs[1], hpairs_b, sizeof(ElasPair) * n, hipMemcpyHostToDevice, stream));
-    HIPCHECK(hipMemcpyAsync(dprefix[1], hprefix_b, sizeof(int32_t) * prefix.size(), hipMemcpyHostToDevice, stream));
-    const auto t_tri1 = clk::now();
-    HIPCHECK(hipStreamWaitEvent(stream, ev[12], 0));
+    build_prefix(prm, S);
+    memcpy(S.hpairs[1], S.pairs.data(), sizeof(ElasPair) * n);
+    memcpy(S.hprefix[1], S.prefix.data(), sizeof(int32_t) * S.prefix.size());
+    hipStream_t s1 = S.stream;
+    HIPCHECK(hipMemcpyAsync(S.dpairs[1], S.hpairs[1], sizeof(ElasPair) * n, hipMemcpyHostToDevice, s1));
+    HIPCHECK(hipMemcpyAsync(S.dprefix[1], S.hprefix[1], sizeof(int32_t) * S.prefix.size(), hipMemcpyHostToDevice, s1));
+    S.t_tri1 = clk::now();
+    HIPCHECK(hipStreamWaitEvent(s1, S.ev[12], 0));
 
-    // ---- phase B: plane prior, dense matching, left/right check -------------
-    HIPCHECK(hipEventRecord(ev[6], stream));
-    { ElasLaunch L = launch(1, S_PLANE); if (L.total) elas_launch_plane(L, g(L, cap256), stream); }
-    { ElasLaunch L = launch(1, S_RASTER); if (L.total) elas_launch_raster(L, g(L, cap256), stream); }
-    HIPCHECK(hipEventRecord(ev[7], stream));
-    {
-        ElasLaunch L = launch(1, S_DENSE);
-        if (L.total) elas_launch_dense(L, kpl, g(L, cap1024), dense_lds(prm.grid_size, maxd, cpw), stream);
-    }
-    HIPCHECK(hipEventRecord(ev[8], stream));
-    { ElasLaunch L = launch(1, S_PIXELS); if (L.total) elas_launch_lr(L, kp, g(L, cap256), stream); }
-    HIPCHECK(hipEventRecord(ev[9], stream));
-    HIPCHECK(hipEventRecord(ev[10], stream));
-    if (debug) { ElasLaunch L = launch(1, S_PIXELS); if (L.total) elas_launch_mean(L, g(L, cap256), stream); }
+    const int cap256 = std::max(1, num_cu) * 8, cap1024 = std::max(1, num_cu) * 2;
+    auto g = [&](const ElasLaunch& L, int cap) { return std::min(L.total, cap); };
+    HIPCHECK(hipEventRecord(S.ev[6], s1));
+    { ElasLaunch L = launch_of(S, 1, S_PLANE); if (L.total) elas_launch_plane(L, g(L, cap256), s1); }
+    { ElasLaunch L = launch_of(S, 1, S_RASTER); if (L.total) elas_launch_raster(L, g(L, cap256), s1); }
+    HIPCHECK(hipEventRecord(S.ev[7], s1));
+    { ElasLaunch L = launch_of(S, 1, S_DENSE);
+      if (L.total) elas_launch_dense(L, S.kpl, g(L, cap1024), dense_lds(prm.grid_size, S.maxd, S.cpw), s1); }
+    HIPCHECK(hipEventRecord(S.ev[8], s1));
+    { ElasLaunch L = launch_of(S, 1, S_PIXELS); if (L.total) elas_launch_lr(L, kp, g(L, cap256), s1); }
+    HIPCHECK(hipEventRecord(S.ev[9], s1));
+    HIPCHECK(hipEventRecord(S.ev[10], s1));
+    if (debug) { ElasLaunch L = launch_of(S, 1, S_PIXELS); if (L.total) elas_launch_mean(L, g(L, cap256), s1); }
     HIPCHECK(hipGetLastError());
     for (int i = 0; i < n; i++)
-        if (!sp[i].dev_out) {
-            HIPCHECK(hipMemcpyAsync(sp[i].D1, pairs[i].dout[0], 4 * dd[i].npx, hipMemcpyDeviceToHost, stream));
-            HIPCHECK(hipMemcpyAsync(sp[i].D2, pairs[i].dout[1], 4 * dd[i].npx, hipMemcpyDeviceToHost, stream));
+        if (!S.specs[i].dev_out) {
+            HIPCHECK(hipMemcpyAsync(S.specs[i].D1, S.pairs[i].dout[0], 4 * S.dd[i].npx, hipMemcpyDeviceToHost, s1));
+            HIPCHECK(hipMemcpyAsync(S.specs[i].D2, S.pairs[i].dout[1], 4 * S.dd[i].npx, hipMemcpyDeviceToHost, s1));
         }
-    HIPCHECK(hipStreamSynchronize(stream));
-    const auto t_end = clk::now();
+    HIPCHECK(hipEventRecord(S.ev[13], s1));
+    S.phase = SLOT_B;
+    return ELAS_OK;
+}
+
+bool elas_ctx::slot_b_ready(ElasSlot& S) { return hipEventQuery(S.ev[13]) == hipSuccess; }
+
+int elas_ctx::slot_finish(ElasSlot& S, int32_t* statuses) {
+    HIPCHECK(hipSetDevice(device));
+    HIPCHECK(hipEventSynchronize(S.ev[13]));
+    S.t_end = clk::now();
     auto el = [&](int a, int b) {
         float ms = 0.f;
-        (void)hipEventElapsedTime(&ms, ev[a], ev[b]);
+        (void)hipEventElapsedTime(&ms, S.ev[a], S.ev[b]);
         return ms;
     };
-    times[ELAS_T_DESCRIPTOR] = el(0, 1);
-    times[ELAS_T_SUPPORT] = el(1, 2);
-    times[ELAS_T_FILTER] = el(2, 3);
-    times[ELAS_T_COMPACT] = el(3, 4);
-    times[ELAS_T_GRID] = el(11, 12);
-    times[ELAS_T_TRIANGLE] = std::chrono::duration<float, std::milli>(t_tri1 - t_tri0).count();
-    times[ELAS_T_PLANE] = el(6, 7);
-    times[ELAS_T_DENSE] = el(7, 8);
-    times[ELAS_T_LR] = el(8, 9);
-    times[ELAS_T_POST] = el(9, 10);
-    times[ELAS_T_TOTAL] = std::chrono::duration<float, std::milli>(t_end - t_start).count();
+    S.times[ELAS_T_DESCRIPTOR] = el(0, 1);
+    S.times[ELAS_T_SUPPORT] = el(1, 2);
+    S.times[ELAS_T_FILTER] = el(2, 3);
+    S.times[ELAS_T_COMPACT] = el(3, 4);
+    S.times[ELAS_T_GRID] = el(11, 12);
+    S.times[ELAS_T_TRIANGLE] = std::chrono::duration<float, std::milli>(S.t_tri1 - S.t_tri0).count();
+    S.times[ELAS_T_PLANE] = el(6, 7);
+    S.times[ELAS_T_DENSE] = el(7, 8);
+    S.times[ELAS_T_LR] = el(8, 9);
+    S.times[ELAS_T_POST] = el(9, 10);
+    S.times[ELAS_T_TOTAL] = std::chrono::duration<float, std::milli>(S.t_end - S.t_start).count();
+    if (statuses)
+        for (int i = 0; i < S.n; i++) statuses[i] = S.status[i];
+    S.phase = SLOT_IDLE;
+    last = &S;
+    return ELAS_OK;
+}
+
+// Returns a slot to idle after an error: waits for its queued GPU work and
+// its Triangle jobs (both write into the slot's buffers).
+void elas_ctx::slot_abort(ElasSlot& S) {
+    (void)hipSetDevice(device);
+    if (S.ticket) elas_tri_release(S.ticket);
+    S.ticket = nullptr;
+    if (S.stream) (void)hipStreamSynchronize(S.stream);
+    if (S.stream2) (void)hipStreamSynchronize(S.stream2);
+    S.phase = SLOT_IDLE;
+}
+
+// ---- drivers -----------------------------------------------------------------
+
+int elas_ctx::run(const ElasPairSpec* sp, int n, int32_t* statuses) {
+    if (slots.empty()) return fail(ELAS_ERR_INVALID_ARGUMENT, "context has no slot");
+    ElasSlot& S = *slots[0];
+    std::vector<int32_t> st(n > 0 ? n : 1, ELAS_OK);
+    int r = slot_prepare(S, sp, n);
+    if (r == ELAS_OK) r = slot_phase_a(S);
+    if (r == ELAS_OK) r = slot_start_tri(S);
+    if (r == ELAS_OK) r = slot_phase_b(S);
+    if (r == ELAS_OK) r = slot_finish(S, st.data());
+    if (r != ELAS_OK) {
+        slot_abort(S);
+        return r;
+    }
     int ret = ELAS_OK;
     for (int i = 0; i < n; i++) {
-        if (statuses) statuses[i] = status[i];
-        if (status[i] != ELAS_OK && status[i] != ELAS_ERR_TOO_FEW_POINTS) ret = status[i];
+        if (statuses) statuses[i] = st[i];
+        if (st[i] != ELAS_OK && st[i] != ELAS_ERR_TOO_FEW_POINTS) ret = st[i];
     }
-    if (n == 1 && status[0] != ELAS_OK) {
-        ret = status[0];
+    if (n == 1 && st[0] != ELAS_OK) {
+        ret = st[0];
         err = "fewer than 3 support points (or a degenerate mesh): D1/D2 are all -1";
     }
     return ret;
 }
 
+// Chunks of the batch move through the slots: a slot whose phase A is done
+// hands its support points to the Triangle workers, a slot whose meshes are
+// done enqueues phase B, a finished slot takes the next chunk.  Each slot has
+// its own streams, so the GPU phases of different chunks overlap each other
+// and the host step.
+int elas_ctx::run_batch(const ElasPairSpec* sp, int n, int32_t* statuses, int chunk) {
+    if (n <= 0 || !sp) return fail(ELAS_ERR_INVALID_ARGUMENT, "empty batch");
+    if (chunk <= 0 || chunk > n) chunk = n;
+    const int nchunks = (n + chunk - 1) / chunk;
+    if (nchunks == 1 || slots.size() < 2) {
+        // one slot: chunks in sequence
+        for (int c = 0; c < nchunks; c++) {
+            const int c0 = c * chunk, cn = std::min(chunk, n - c0);
+            int r = run(sp + c0, cn, statuses ? statuses + c0 : nullptr);
+            if (r != ELAS_OK && !(cn == 1 && r == ELAS_ERR_TOO_FEW_POINTS)) return r;
+        }
+        return ELAS_OK;
+    }
+    // reserve every slot for the largest chunk before anything is in flight
+    size_t a_max = 0, p_max = 0;
+    for (int c = 0; c < nchunks; c++) {
+        size_t a = 0, p = 0;
+        const int c0 = c * chunk, cn = std::min(chunk, n - c0);
+        int r = slot_need(sp + c0, cn, &a, &p);
+        if (r != ELAS_OK) return r;
+        a_max = std::max(a_max, a);
+        p_max = std::max(p_max, p);
+    }
+    for (ElasSlot* S : slots) {
+        int r = slot_reserve(*S, a_max, p_max, chunk);
+        if (r != ELAS_OK) return r;
+    }
+    std::vector<int> chunk_of(slots.size(), -1);
+    int next = 0, done = 0, ret = ELAS_OK;
+    while (done < nchunks && ret == ELAS_OK) {
+        bool progressed = false;
+        for (size_t k = 0; k < slots.size() && ret == ELAS_OK; k++) {
+            ElasSlot& S = *slots[k];
+            int r = ELAS_OK;
+            if (S.phase == SLOT_IDLE && next < nchunks) {
+                const int c0 = next * chunk, cn = std::min(chunk, n - c0);
+                chunk_of[k] = next++;
+                r = slot_prepare(S, sp + c0, cn);
+                if (r == ELAS_OK) r = slot_phase_a(S);
+                progressed = true;
+            } else if (S.phase == SLOT_A && slot_a_ready(S)) {
+                r = slot_start_tri(S);
+                progressed = true;
+            } else if (S.phase == SLOT_TRI && slot_tri_ready(S)) {
+                r = slot_phase_b(S);
+                progressed = true;
+            } else if (S.phase == SLOT_B && slot_b_ready(S)) {
+                r = slot_finish(S, statuses ? statuses + (size_t)chunk_of[k] * chunk : nullptr);
+                done++;
+                progressed = true;
+            }
+            if (r != ELAS_OK) ret = r;
+        }
+        if (!progressed) std::this_thread::sleep_for(std::chrono::microseconds(20));
+    }
+    if (ret != ELAS_OK) {
+        const std::string msg = err;
+        for (ElasSlot* S : slots) slot_abort(*S);
+        err = msg;
+        return ret;
+    }
+    if (statuses)
+        for (int i = 0; i < n; i++)
+            if (statuses[i] != ELAS_OK && statuses[i] != ELAS_ERR_TOO_FEW_POINTS) return statuses[i];
+    return ELAS_OK;
+}
+
 int elas_ctx::fetch(int stage, void* dst, size_t bytes, size_t* stage_bytes) {
-    if (pairs.empty()) return fail(ELAS_ERR_INVALID_ARGUMENT, "no process call yet");
-    const ElasPair& P = pairs[0];
+    if (!last || last->pairs.empty()) return fail(ELAS_ERR_INVALID_ARGUMENT, "no process call yet");
+    const ElasSlot& S = *last;
+    const ElasPair& P = S.pairs[0];
     const size_t npx = (size_t)P.W * P.H, ncan = (size_t)P.cw * P.ch;
     const void* src = nullptr;
     size_t sz = 0;
@@ -518,12 +754,12 @@ int elas_ctx::fetch(int stage, void* dst, size_t bytes, size_t* stage_bytes) {
         case ELAS_STAGE_CAN_LR: src = P.can[1]; sz = 4 * ncan; break;
         case ELAS_STAGE_CAN_INCON: src = P.can[2]; sz = 4 * ncan; break;
         case ELAS_STAGE_CAN_FINAL: src = P.can[3]; sz = 4 * ncan; break;
-        case ELAS_STAGE_SP0: src = P.sp[0]; sz = 8 * (size_t)nsp[0]; break;
-        case ELAS_STAGE_SP1: src = P.sp[1]; sz = 8 * (size_t)nsp[0]; break;
+        case ELAS_STAGE_SP0: src = P.sp[0]; sz = 8 * (size_t)S.nsp[0]; break;
+        case ELAS_STAGE_SP1: src = P.sp[1]; sz = 8 * (size_t)S.nsp[0]; break;
         case ELAS_STAGE_GRID0: src = P.glist[0]; sz = 4 * (size_t)P.gw * P.gh * (P.dmax + 2); break;
         case ELAS_STAGE_GRID1: src = P.glist[1]; sz = 4 * (size_t)P.gw * P.gh * (P.dmax + 2); break;
-        case ELAS_STAGE_TRI0: src = P.tri[0]; sz = 12 * (size_t)ntri0[0]; break;
-        case ELAS_STAGE_TRI1: src = P.tri[1]; sz = 12 * (size_t)ntri1[0]; break;
+        case ELAS_STAGE_TRI0: src = P.tri[0]; sz = 12 * (size_t)S.ntri0[0]; break;
+        case ELAS_STAGE_TRI1: src = P.tri[1]; sz = 12 * (size_t)S.ntri1[0]; break;
         case ELAS_STAGE_MEAN0: src = P.mean[0]; sz = 4 * npx; break;
         case ELAS_STAGE_MEAN1: src = P.mean[1]; sz = 4 * npx; break;
         case ELAS_STAGE_D1_RAW: src = P.draw[0]; sz = 4 * npx; break;

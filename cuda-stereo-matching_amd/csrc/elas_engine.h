@@ -1,9 +1,18 @@
-// elas_engine.h -- host engine behind the C ABI: one context = one device,
-// one pair of HIP streams, one HBM arena, pinned staging, stage timers.
+// elas_engine.h -- host engine behind the C ABI.
+//
+// A context owns one device and a set of pipeline slots.  A slot carries one
+// chunk of pairs through the three phases of the path:
+//   A  (GPU)  descriptors, support matching, filters, compaction, grid;
+//   T  (host) Triangle on the worker pool (non-blocking);
+//   B  (GPU)  plane prior, dense matching, left/right check;
+// with its own streams, HBM arena, device tables and pinned staging, so that
+// several chunks can be in different phases at once.  A plain call uses
+// slot 0 start to finish; a batch or the serving loop drives several slots.
 #pragma once
 #include <hip/hip_runtime_api.h>
 #include <stdint.h>
 
+#include <chrono>
 #include <string>
 #include <vector>
 
@@ -19,45 +28,103 @@ struct ElasPairSpec {
     float* D1;
     float* D2;
     bool dev_in, dev_out;
+    int64_t tag;  // serving request id (-1 otherwise)
 };
+
+// Derived sizes of one pair.
+struct ElasDims {
+    int32_t cw, ch, gw, gh, words, nblk, tricap;
+    size_t npx, ncan, ncell;
+};
+
+enum ElasSlotPhase { SLOT_IDLE = 0, SLOT_A = 1, SLOT_TRI = 2, SLOT_B = 3 };
+
+struct ElasSlot {
+    hipStream_t stream = nullptr;   // phases A and B
+    hipStream_t stream2 = nullptr;  // disparity grid, overlapped with the host step
+    hipEvent_t ev[16] = {};
+    // device memory
+    void* arena = nullptr;
+    size_t arena_bytes = 0;
+    void* table = nullptr;  // 2 x (ElasPair[n] + prefix[S_NUM_STAGES][n+1] + nsp[n])
+    size_t table_bytes = 0;
+    void* blist = nullptr;  // raster tile offsets + lists (known after Triangle)
+    size_t blist_bytes = 0;
+    // pinned host staging
+    void* pin = nullptr;
+    size_t pin_bytes = 0;
+    void* blist_pin = nullptr;
+    size_t blist_pin_bytes = 0;
+
+    // the chunk in flight
+    int phase = SLOT_IDLE;
+    int n = 0;
+    std::vector<ElasPairSpec> specs;
+    std::vector<ElasDims> dd;
+    std::vector<size_t> base, sp_off, tri_off;
+    std::vector<ElasPair> pairs;
+    std::vector<int32_t> prefix;
+    std::vector<int32_t> nsp, ntri0, ntri1, status;
+    int maxd = 0, cpw = 1;
+    ElasKParams kpl;
+    size_t tab_pairs = 0, tab_prefix = 0, tab_one = 0, pin_sp = 0, pin_tri = 0;
+    ElasPair* dpairs[2] = {nullptr, nullptr};
+    int32_t* dprefix[2] = {nullptr, nullptr};
+    int32_t* dnsp = nullptr;
+    ElasPair* hpairs[2] = {nullptr, nullptr};
+    int32_t* hprefix[2] = {nullptr, nullptr};
+    int32_t* hnsp = nullptr;
+    float* hsp = nullptr;
+    int32_t* htri = nullptr;
+    std::vector<std::vector<int32_t>> tri_out, toff, tlist;
+    std::vector<ElasTriJob> jobs;
+    std::vector<int> job_of;
+    ElasTriTicket* ticket = nullptr;
+    std::chrono::steady_clock::time_point t_start, t_tri0, t_tri1, t_end;
+    float times[ELAS_T_COUNT] = {};
+};
+
+struct ElasServer;
 
 struct elas_ctx {
     int device = 0;
     int num_cu = 0;
-    hipStream_t stream = nullptr;   // main pipeline
-    hipStream_t stream2 = nullptr;  // disparity grid, overlapped with host Triangle
     elas_params prm;
     ElasKParams kp;
     bool debug = false;
     int tri_workers = ELAS_TRI_INSTANCES;
+    int chunk = 0;  // pairs per pipeline chunk of a batch (0: whole batch)
     std::string err;
+    std::vector<ElasSlot*> slots;
+    ElasSlot* last = nullptr;  // slot of the last finished call (fetch, shape info, times)
+    ElasServer* server = nullptr;
 
-    // device memory
-    void* arena = nullptr;
-    size_t arena_bytes = 0;
-    void* table = nullptr;  // ElasPair[n] + prefix[S_NUM_STAGES][n+1] + nsp[n]
-    size_t table_bytes = 0;
-    // pinned host staging
-    void* pin = nullptr;
-    size_t pin_bytes = 0;
-    // raster tile offsets + lists (sizes known only after Triangle)
-    void* blist = nullptr;
-    size_t blist_bytes = 0;
-    void* blist_pin = nullptr;
-    size_t blist_pin_bytes = 0;
-
-    std::vector<ElasPair> pairs;                    // host copy of the device table
-    std::vector<int32_t> prefix;                    // [S_NUM_STAGES][n+1]
-    std::vector<ElasPairSpec> specs;                // last call
-    std::vector<int32_t> nsp, ntri0, ntri1, status; // last call, per pair
-    hipEvent_t ev[16] = {};
-    hipEvent_t ev_grid = nullptr;
-    float times[ELAS_T_COUNT] = {};
-
+    // blocking single-chunk run on slot 0 (Elas::process and the device form)
     int run(const ElasPairSpec* specs, int n, int32_t* statuses);
+    // pipelined run of n pairs in chunks of up to `chunk` pairs over the slots
+    int run_batch(const ElasPairSpec* specs, int n, int32_t* statuses, int chunk);
     int fetch(int stage, void* dst, size_t bytes, size_t* stage_bytes);
     int fail(int code, const std::string& msg) { err = msg; return code; }
+
+    // phases of one slot
+    int slot_init(ElasSlot& S);
+    void slot_free(ElasSlot& S);
+    int slot_need(const ElasPairSpec* sp, int n, size_t* arena_need, size_t* pin_need);
+    int slot_reserve(ElasSlot& S, size_t arena_need, size_t pin_need, int n);
+    int slot_prepare(ElasSlot& S, const ElasPairSpec* sp, int n);
+    int slot_phase_a(ElasSlot& S);
+    bool slot_a_ready(ElasSlot& S);
+    int slot_start_tri(ElasSlot& S);
+    bool slot_tri_ready(ElasSlot& S);
+    int slot_phase_b(ElasSlot& S);
+    bool slot_b_ready(ElasSlot& S);
+    int slot_finish(ElasSlot& S, int32_t* statuses);
+    void slot_abort(ElasSlot& S);
 };
+
+// Derived sizes of a pair and its device footprint in a slot arena.
+ElasDims elas_dims(const elas_params& p, int W, int H, int dmax);
+size_t elas_pair_device_bytes(const elas_params& p, const ElasPairSpec& s, bool debug);
 
 // Parameter checks and derived kernel parameters (prior table, key bias).
 int elas_make_kparams(const elas_params* p, ElasKParams* kp, std::string* err);

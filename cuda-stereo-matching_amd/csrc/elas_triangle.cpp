@@ -122,6 +122,11 @@ public:
         static TriPool* pool = new TriPool();  // lives until process exit
         return *pool;
     }
+    struct Batch {
+        std::mutex m;
+        std::condition_variable cv;
+        int remaining = 0;
+    };
     void run(ElasTriJob* jobs, int n, int max_workers) {
         if (n <= 0) return;
         if (n == 1 || max_workers <= 1) {
@@ -130,23 +135,21 @@ public:
             return;
         }
         Batch b;
-        b.remaining = n;
-        {
-            std::lock_guard<std::mutex> g(m_);
-            ensure_workers();
-            for (int i = 0; i < n; i++) q_.push_back({&jobs[i], &b});
-        }
-        cv_.notify_all();
+        enqueue(jobs, n, &b);
         std::unique_lock<std::mutex> g(b.m);
         b.cv.wait(g, [&] { return b.remaining == 0; });
     }
+    void enqueue(ElasTriJob* jobs, int n, Batch* b) {
+        b->remaining = n;
+        {
+            std::lock_guard<std::mutex> g(m_);
+            ensure_workers();
+            for (int i = 0; i < n; i++) q_.push_back({&jobs[i], b});
+        }
+        cv_.notify_all();
+    }
 
 private:
-    struct Batch {
-        std::mutex m;
-        std::condition_variable cv;
-        int remaining = 0;
-    };
     struct Item {
         ElasTriJob* job;
         Batch* batch;
@@ -228,4 +231,34 @@ void elas_bin_triangles(const float* pts, const int32_t* tri, int ntri, int W, i
 
 void elas_triangulate_jobs(ElasTriJob* jobs, int n, int max_workers) {
     TriPool::get().run(jobs, n, max_workers);
+}
+
+struct ElasTriTicket {
+    TriPool::Batch batch;
+};
+
+ElasTriTicket* elas_tri_submit(ElasTriJob* jobs, int n) {
+    ElasTriTicket* t = new ElasTriTicket();
+    if (n <= 0) {
+        t->batch.remaining = 0;
+        return t;
+    }
+    TriPool::get().enqueue(jobs, n, &t->batch);
+    return t;
+}
+
+bool elas_tri_done(ElasTriTicket* t) {
+    std::lock_guard<std::mutex> g(t->batch.m);
+    return t->batch.remaining == 0;
+}
+
+void elas_tri_wait(ElasTriTicket* t) {
+    std::unique_lock<std::mutex> g(t->batch.m);
+    t->batch.cv.wait(g, [&] { return t->batch.remaining == 0; });
+}
+
+void elas_tri_release(ElasTriTicket* t) {
+    if (!t) return;
+    elas_tri_wait(t);
+    delete t;
 }

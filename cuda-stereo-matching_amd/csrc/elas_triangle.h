@@ -27,3 +27,13 @@ void elas_bin_triangles(const float* pts, const int32_t* tri, int ntri, int W, i
 // Runs every job (each a "zQB" Triangle call) on up to max_workers of the
 // process-wide Triangle workers; returns when all are done.
 void elas_triangulate_jobs(ElasTriJob* jobs, int n, int max_workers);
+
+// Non-blocking form: queues the jobs and returns a ticket; elas_tri_done()
+// tells whether they have all finished, elas_tri_wait() blocks until then.
+// The jobs must stay alive until done; elas_tri_release() frees a finished
+// ticket.
+typedef struct ElasTriTicket ElasTriTicket;
+ElasTriTicket* elas_tri_submit(ElasTriJob* jobs, int n);
+bool elas_tri_done(ElasTriTicket* t);
+void elas_tri_wait(ElasTriTicket* t);
+void elas_tri_release(ElasTriTicket* t);

@@ -148,6 +148,19 @@ int elas_hip_process_batch(elas_ctx* ctx, int32_t n, const uint8_t* const* left,
                            const uint8_t* const* right, float* const* D1, float* const* D2,
                            const int32_t* dims, int32_t* statuses);
 
+/* Batched form on device-resident buffers (arrays of n device pointers). */
+int elas_hip_process_batch_device(elas_ctx* ctx, int32_t n, const uint8_t* const* d_left,
+                                  const uint8_t* const* d_right, float* const* d_D1,
+                                  float* const* d_D2, const int32_t* dims, int32_t* statuses);
+
+/* Pipelining of the batched forms: a batch is cut into chunks of up to
+ * `chunk` pairs (0: the whole batch is one chunk) that move through `slots`
+ * pipeline slots (1..8), so the GPU stages of one chunk overlap the host
+ * Delaunay step of another.  Each slot keeps its own device arena sized for
+ * the largest chunk.  Defaults: 3 slots, chunk 0.  No reference counterpart
+ * (the reference processes one pair per call). */
+int elas_hip_set_pipeline(elas_ctx* ctx, int32_t slots, int32_t chunk);
+
 /* Intermediates of the last process call on this context (kept only when
  * elas_hip_set_debug(ctx, 1)).  Copies min(bytes, stage size) bytes to dst
  * and stores the stage size in *stage_bytes (either may be NULL/0 to query). */

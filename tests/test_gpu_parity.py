@@ -92,6 +92,48 @@ def test_batch_equals_plain(pkg, synth):
         assert np.array_equal(b1, D1) and np.array_equal(b2, D2)
 
 
+@pytest.mark.parametrize("slots,chunk", [(1, 2), (3, 1), (3, 2), (2, 3)])
+def test_pipelined_batch_equals_plain(pkg, synth, slots, chunk):
+    """Chunks moving through pipeline slots give each pair its own result,
+    including a textureless pair (status 4) in the middle of the batch."""
+    shapes = [(160, 120, 31), (200, 90, 63), (97, 131, 15), (240, 180, 63), (128, 96, 31), (180, 140, 47)]
+    pairs_ = []
+    for i, (W, H, dm) in enumerate(shapes):
+        l, r, _ = synth.make_pair(W, H, dm, regions=4, seed=200 + i)
+        pairs_.append((l, r, dm))
+    flat = np.full((100, 140), 60, np.uint8)
+    pairs_.insert(3, (flat, flat, 31))
+    print("pipeline slots", slots, "chunk", chunk, flush=True)
+    el = pkg.Elas(pkg.default_params(disp_max=63))
+    el.set_pipeline(slots, chunk)
+    for rep in range(2):
+        batch = el.process_batch(pairs_)
+        for (l, r, dm), (b1, b2, st) in zip(pairs_, batch):
+            one = pkg.Elas(pkg.default_params(disp_max=dm))
+            D1, D2 = one.process(l, r)
+            assert st == one.status
+            assert np.array_equal(b1, D1) and np.array_equal(b2, D2)
+
+
+def test_device_batch_equals_plain(pkg, synth):
+    import torch
+    shapes = [(160, 120, 31), (200, 90, 63), (97, 131, 15), (240, 180, 63), (128, 96, 31)]
+    pairs_ = [synth.make_pair(W, H, dm, regions=4, seed=300 + i)[:2] + (dm,) for i, (W, H, dm) in enumerate(shapes)]
+    el = pkg.Elas(pkg.default_params(disp_max=63))
+    el.set_pipeline(3, 2)
+    dev = torch.device("cuda:0")
+    L = [torch.from_numpy(l).to(dev) for l, _, _ in pairs_]
+    R = [torch.from_numpy(r).to(dev) for _, r, _ in pairs_]
+    D1 = [torch.empty(l.shape, dtype=torch.float32, device=dev) for l in L]
+    D2 = [torch.empty(l.shape, dtype=torch.float32, device=dev) for l in L]
+    st = el.process_batch_device(L, R, D1, D2, [dm for _, _, dm in pairs_])
+    for i, (l, r, dm) in enumerate(pairs_):
+        one = pkg.Elas(pkg.default_params(disp_max=dm))
+        a1, a2 = one.process(l, r)
+        assert st[i] == one.status
+        assert np.array_equal(D1[i].cpu().numpy(), a1) and np.array_equal(D2[i].cpu().numpy(), a2)
+
+
 def test_textureless_pair_reports_too_few_points(pkg, oracle):
     img = np.full((120, 160), 77, np.uint8)
     el = pkg.Elas(pkg.default_params(disp_max=31))
