@@ -45,7 +45,7 @@ def work_counts(oracle, left, right, dmax):
     return dict(support_left=w_left, support_right=w_right, dense=out["work_dense"])
 
 
-def batch_vs_plain(pkg, synth, n=64, reps=3):
+def batch_vs_plain(pkg, synth, n=64, reps=3, pool=None):
     """The mixed batch: n pairs drawn 3:2:1 from 640x480/d64, 1242x375/d128,
     1920x1080/d256 (seeds 3000+i), images and disparity maps resident in HBM.
     Plain: n elas_hip_process_device calls, one pair at a time.  Batched: one
@@ -56,7 +56,7 @@ def batch_vs_plain(pkg, synth, n=64, reps=3):
     to the plain ones.  The host-buffer form (PCIe transfers included) is
     timed once with the chosen setting."""
     import torch
-    pairs = synth.batch_pairs(n)
+    pairs = pool[:n] if pool else synth.batch_pairs(n)
     work = sum(l.shape[0] * l.shape[1] * (dm + 1) for l, _, dm in pairs) / 1e6
     dev = torch.device("cuda:0")
     L = [torch.from_numpy(l).to(dev) for l, _, _ in pairs]
@@ -119,6 +119,61 @@ def batch_vs_plain(pkg, synth, n=64, reps=3):
             "host_buffers_batched_pairs_per_s": round(n / th, 2), "buffers": "device-resident"}
 
 
+def serving(pkg, synth, rates=(100, 400, 800), n_req=400, pool=None, seed=4000):
+    """Serving sweep: Poisson arrivals at each rate of pairs drawn from the
+    mixed-batch pool (request i uses pool pair i mod len(pool)), host buffers
+    in pinned memory, submitted to the continuous-batching loop
+    (elas_hip_submit / elas_hip_poll).  Latency = arrival -> D1/D2 in host
+    memory (the feeder's own lateness included); sustained pairs/s = requests
+    / (first arrival -> last completion)."""
+    import torch
+    pool = pool or synth.batch_pairs(64)
+    pin = lambda a: torch.from_numpy(a).pin_memory().numpy()
+    P = [(pin(l), pin(r), dm, torch.empty(l.shape, dtype=torch.float32, pin_memory=True).numpy(),
+          torch.empty(l.shape, dtype=torch.float32, pin_memory=True).numpy()) for l, r, dm in pool]
+    el = pkg.Elas(pkg.default_params(disp_max=255))
+    out = {}
+    rng = np.random.default_rng(seed)
+    for lam in rates:
+        el.serve_start()
+        gaps = rng.exponential(1.0 / lam, n_req)
+        lat, late, pending = [], [], {}
+        t0 = time.perf_counter() + 0.01
+        arr = t0 + np.cumsum(gaps)
+        t_last = t0
+        i = 0
+        while i < n_req or pending:
+            now = time.perf_counter()
+            if i < n_req and now >= arr[i]:
+                l, r, dm, d1, d2 = P[i % len(P)]
+                pending[el.submit(l, r, d1, d2, dm)] = arr[i]
+                late.append(time.perf_counter() - arr[i])
+                i += 1
+                continue
+            for t in list(pending):
+                res = el.poll(t)
+                if res is not None:
+                    a = pending.pop(t)
+                    done = time.perf_counter()
+                    t_last = max(t_last, done)
+                    lat.append((done - a) * 1e3)
+            if i < n_req and not pending:
+                time.sleep(max(0.0, min(arr[i] - time.perf_counter() - 2e-4, 0.002)))
+        st = el.serve_stats()
+        el.serve_stop()
+        lat = np.array(lat)
+        out[str(lam)] = {"requests": n_req, "p50_ms": round(float(np.percentile(lat, 50)), 3),
+                         "p99_ms": round(float(np.percentile(lat, 99)), 3),
+                         "mean_ms": round(float(lat.mean()), 3),
+                         "sustained_pairs_per_s": round(n_req / (t_last - t0), 1),
+                         "feeder_late_p99_ms": round(float(np.percentile(late, 99)) * 1e3, 3),
+                         "chunks": int(st["chunks"]), "max_chunk_pairs": int(st["max_chunk_pairs"]),
+                         "max_in_flight_pairs": int(st["max_in_flight_pairs"]),
+                         "peak_in_flight_MB": round(st["peak_in_flight_bytes"] / 1e6, 1)}
+    el.close()
+    return {"rates": out, "pool_pairs": len(P), "hbm_budget_GiB": 8, "buffers": "pinned host"}
+
+
 def cpu_baseline(oracle, left, right, dmax, budget_s=12.0):
     prm = oracle.params(disp_max=dmax)
     n, t0 = 0, time.perf_counter()
@@ -139,6 +194,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched comparison")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving sweep")
+    ap.add_argument("--serve-full", action="store_true",
+                    help="serving sweep at 50..800 pairs/s with 2000 requests per rate")
     ap.add_argument("--workload", default="c1", choices=["c0", "c1", "c2"],
                     help="c1 (1242x375, d 0..127) is the headline; c0/c2 for profiling")
     ap.add_argument("--copy-probe", action="store_true",
@@ -224,8 +282,16 @@ def main():
                     for k, v in kernels.items()},
         "work": wc,
     }
+    pool = None
+    if not args.no_batch or not args.no_serve:
+        pool = synth.batch_pairs(64)
     if not args.no_batch:
-        res["batch_vs_plain"] = batch_vs_plain(pkg, synth)
+        res["batch_vs_plain"] = batch_vs_plain(pkg, synth, pool=pool)
+    if not args.no_serve:
+        if args.serve_full:
+            res["serving"] = serving(pkg, synth, rates=(50, 100, 200, 400, 800), n_req=2000, pool=pool)
+        else:
+            res["serving"] = serving(pkg, synth, pool=pool)
     if not args.no_cpu:
         n, el_s = cpu_baseline(oracle, left, right, dmax, args.cpu_seconds)
         model = ""

@@ -20,7 +20,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libelas_hip.so")
 
 STATUS = {0: "ok", 1: "invalid argument", 2: "HIP error", 3: "out of memory",
           4: "too few support points", 5: "no gfx950 device", 6: "unsupported", 7: "busy"}
-ELAS_OK, ELAS_ERR_TOO_FEW_POINTS = 0, 4
+ELAS_OK, ELAS_ERR_TOO_FEW_POINTS, ELAS_ERR_BUSY = 0, 4, 7
 
 STAGES = ["desc0", "desc1", "can_left", "can_lr", "can_incon", "can_final", "sp0", "sp1", "grid0",
           "grid1", "tri0", "tri1", "mean0", "mean1", "d1_raw", "d2_raw", "D1", "D2"]
@@ -56,6 +56,19 @@ class ShapeInfo(ctypes.Structure):
 
 # Exported symbols and their signatures (include/elas_hip.h).
 _P = ctypes.c_void_p
+class ServeConfig(ctypes.Structure):
+    """elas_serve_config (include/elas_hip.h)."""
+    _fields_ = [("hbm_budget_bytes", ctypes.c_uint64), ("max_chunk_pairs", ctypes.c_int32),
+                ("slots", ctypes.c_int32)]
+
+
+class ServeStats(ctypes.Structure):
+    """elas_serve_stats (include/elas_hip.h)."""
+    _fields_ = [("submitted", ctypes.c_int64), ("completed", ctypes.c_int64), ("chunks", ctypes.c_int64),
+                ("max_chunk_pairs", ctypes.c_int64), ("max_in_flight_pairs", ctypes.c_int64),
+                ("slot_capacity_bytes", ctypes.c_uint64), ("peak_in_flight_bytes", ctypes.c_uint64)]
+
+
 SIGNATURES = {
     "elas_params_default": (None, [ctypes.POINTER(ElasParams)]),
     "elas_params_declared": (None, [ctypes.POINTER(ElasParams)]),
@@ -69,6 +82,13 @@ SIGNATURES = {
     "elas_hip_process_batch": (ctypes.c_int, [_P, ctypes.c_int32, _P, _P, _P, _P, _P, _P]),
     "elas_hip_process_batch_device": (ctypes.c_int, [_P, ctypes.c_int32, _P, _P, _P, _P, _P, _P]),
     "elas_hip_set_pipeline": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32]),
+    "elas_hip_serve_start": (ctypes.c_int, [_P, _P]),
+    "elas_hip_submit": (ctypes.c_int, [_P, _P, _P, _P, _P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                       ctypes.c_int32, ctypes.POINTER(ctypes.c_int64)]),
+    "elas_hip_poll": (ctypes.c_int, [_P, ctypes.c_int64, ctypes.POINTER(ctypes.c_int32),
+                                     ctypes.POINTER(ctypes.c_float)]),
+    "elas_hip_serve_stop": (ctypes.c_int, [_P]),
+    "elas_hip_serve_stats": (ctypes.c_int, [_P, _P]),
     "elas_hip_set_debug": (ctypes.c_int, [_P, ctypes.c_int]),
     "elas_hip_debug_fetch": (ctypes.c_int, [_P, ctypes.c_int, _P, ctypes.c_size_t,
                                             ctypes.POINTER(ctypes.c_size_t)]),
@@ -205,6 +225,56 @@ class Elas:
         st = self._lib.elas_hip_process_device(self._ctx, left.data_ptr(), right.data_ptr(), D1.data_ptr(),
                                                D2.data_ptr(), W, H)
         return self._check(st, allow_few=True)
+
+    # ---- serving -----------------------------------------------------------
+    def serve_start(self, hbm_budget_bytes=0, max_chunk_pairs=0, slots=0):
+        """Start the context's scheduler thread (continuous batching over the
+        pipeline slots, admission by HBM budget).  0 = library default."""
+        cfg = ServeConfig(int(hbm_budget_bytes), int(max_chunk_pairs), int(slots))
+        self._check(self._lib.elas_hip_serve_start(self._ctx, ctypes.byref(cfg)))
+        self._inflight = {}
+
+    def submit(self, left, right, D1, D2, disp_max=None):
+        """Queue one pair; returns a ticket.  numpy (host) or torch cuda
+        (device-resident) buffers; they are kept referenced until polled done.
+        Host buffers should be pinned for the copies to overlap."""
+        dev = hasattr(left, "is_cuda") and left.is_cuda
+        if dev:
+            H, W = int(left.shape[0]), int(left.shape[1])
+            ptr = lambda t: t.data_ptr()
+            _torch_sync(left)
+        else:
+            H, W = left.shape
+            for a, dt in ((left, np.uint8), (right, np.uint8), (D1, np.float32), (D2, np.float32)):
+                if a.dtype != dt or not a.flags.c_contiguous or a.shape != (H, W):
+                    raise ValueError("submit expects contiguous [H, W] uint8 images and float32 maps")
+            ptr = lambda a: a.ctypes.data
+        t = ctypes.c_int64(0)
+        dm = -1 if disp_max is None else int(disp_max)
+        self._check(self._lib.elas_hip_submit(self._ctx, ptr(left), ptr(right), ptr(D1), ptr(D2), W, H, dm,
+                                              1 if dev else 0, ctypes.byref(t)))
+        self._inflight[t.value] = (left, right, D1, D2)
+        return t.value
+
+    def poll(self, ticket):
+        """None while queued or in flight, else (status, latency_ms)."""
+        st = ctypes.c_int32(0)
+        lat = ctypes.c_float(0)
+        r = self._lib.elas_hip_poll(self._ctx, int(ticket), ctypes.byref(st), ctypes.byref(lat))
+        if r == ELAS_ERR_BUSY:
+            return None
+        self._check(r)
+        self._inflight.pop(int(ticket), None)
+        return int(st.value), float(lat.value)
+
+    def serve_stats(self):
+        s = ServeStats()
+        self._check(self._lib.elas_hip_serve_stats(self._ctx, ctypes.byref(s)))
+        return {k: getattr(s, k) for k, _ in s._fields_}
+
+    def serve_stop(self):
+        """Drain the queue and stop the scheduler thread."""
+        self._check(self._lib.elas_hip_serve_stop(self._ctx))
 
     def set_pipeline(self, slots=3, chunk=0):
         """Batches are cut into chunks of `chunk` pairs (0: one chunk) that

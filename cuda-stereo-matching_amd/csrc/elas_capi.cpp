@@ -110,8 +110,14 @@ int elas_hip_create(const elas_params* params, int device, elas_ctx** out) {
     return ELAS_OK;
 }
 
+#define ELAS_NOT_SERVING(c)                                                           \
+    do {                                                                              \
+        if ((c)->server) return (c)->fail(ELAS_ERR_BUSY, "context is serving requests"); \
+    } while (0)
+
 int elas_hip_destroy(elas_ctx* c) {
     if (!c) return ELAS_ERR_INVALID_ARGUMENT;
+    elas_srv_stop(c);
     for (ElasSlot* S : c->slots) {
         c->slot_free(*S);
         delete S;
@@ -123,6 +129,7 @@ int elas_hip_destroy(elas_ctx* c) {
 
 int elas_hip_set_pipeline(elas_ctx* c, int32_t slots, int32_t chunk) {
     if (!c) return ELAS_ERR_INVALID_ARGUMENT;
+    ELAS_NOT_SERVING(c);
     if (slots < 1 || slots > 8 || chunk < 0) {
         c->err = "need 1 <= slots <= 8 and chunk >= 0";
         return ELAS_ERR_INVALID_ARGUMENT;
@@ -147,6 +154,7 @@ int elas_hip_set_pipeline(elas_ctx* c, int32_t slots, int32_t chunk) {
 
 int elas_hip_set_params(elas_ctx* c, const elas_params* p) {
     if (!c || !p) return ELAS_ERR_INVALID_ARGUMENT;
+    ELAS_NOT_SERVING(c);
     ElasKParams kp;
     int st = elas_make_kparams(p, &kp, &c->err);
     if (st != ELAS_OK) return st;
@@ -155,13 +163,20 @@ int elas_hip_set_params(elas_ctx* c, const elas_params* p) {
     return ELAS_OK;
 }
 
-const char* elas_hip_last_error(const elas_ctx* c) { return c ? c->err.c_str() : "null context"; }
+const char* elas_hip_last_error(const elas_ctx* c) {
+    if (!c) return "null context";
+    static thread_local std::string copy;
+    std::lock_guard<std::mutex> g(const_cast<elas_ctx*>(c)->err_m);
+    copy = c->err;
+    return copy.c_str();
+}
 
 void* elas_hip_stream(elas_ctx* c) { return c && !c->slots.empty() ? (void*)c->slots[0]->stream : nullptr; }
 
 int elas_hip_process(elas_ctx* c, const uint8_t* left, const uint8_t* right, float* D1, float* D2,
                      const int32_t* dims) {
     if (!c || !dims) return ELAS_ERR_INVALID_ARGUMENT;
+    ELAS_NOT_SERVING(c);
     ElasPairSpec s{dims[0], dims[1], c->prm.disp_max, left, right, D1, D2, false, false, -1};
     return c->run(&s, 1, nullptr);
 }
@@ -169,6 +184,7 @@ int elas_hip_process(elas_ctx* c, const uint8_t* left, const uint8_t* right, flo
 int elas_hip_process_device(elas_ctx* c, const uint8_t* d_left, const uint8_t* d_right, float* d_D1,
                             float* d_D2, int32_t width, int32_t height) {
     if (!c) return ELAS_ERR_INVALID_ARGUMENT;
+    ELAS_NOT_SERVING(c);
     ElasPairSpec s{width, height, c->prm.disp_max, d_left, d_right, d_D1, d_D2, true, true, -1};
     return c->run(&s, 1, nullptr);
 }
@@ -176,6 +192,7 @@ int elas_hip_process_device(elas_ctx* c, const uint8_t* d_left, const uint8_t* d
 static int batch(elas_ctx* c, int32_t n, const uint8_t* const* left, const uint8_t* const* right,
                  float* const* D1, float* const* D2, const int32_t* dims, int32_t* statuses, bool dev) {
     if (!c || n <= 0 || !left || !right || !D1 || !D2 || !dims) return ELAS_ERR_INVALID_ARGUMENT;
+    ELAS_NOT_SERVING(c);
     std::vector<ElasPairSpec> v(n);
     for (int i = 0; i < n; i++) {
         const int dm = dims[3 * i + 2] < 0 ? c->prm.disp_max : dims[3 * i + 2];
@@ -197,19 +214,21 @@ int elas_hip_process_batch_device(elas_ctx* c, int32_t n, const uint8_t* const* 
 
 int elas_hip_set_debug(elas_ctx* c, int keep) {
     if (!c) return ELAS_ERR_INVALID_ARGUMENT;
+    ELAS_NOT_SERVING(c);
     c->debug = keep != 0;
     return ELAS_OK;
 }
 
 int elas_hip_debug_fetch(elas_ctx* c, int stage, void* dst, size_t bytes, size_t* stage_bytes) {
     if (!c) return ELAS_ERR_INVALID_ARGUMENT;
+    ELAS_NOT_SERVING(c);
     return c->fetch(stage, dst, bytes, stage_bytes);
 }
 
 int elas_hip_shape_info(const elas_ctx* c, elas_shape_info* info) {
     if (!c || !info) return ELAS_ERR_INVALID_ARGUMENT;
     memset(info, 0, sizeof(*info));
-    if (!c->last || c->last->pairs.empty()) return ELAS_ERR_INVALID_ARGUMENT;
+    if (c->server || !c->last || c->last->pairs.empty()) return ELAS_ERR_INVALID_ARGUMENT;
     const ElasSlot& S = *c->last;
     const ElasPair& P = S.pairs[0];
     info->width = P.W;
@@ -226,9 +245,39 @@ int elas_hip_shape_info(const elas_ctx* c, elas_shape_info* info) {
 }
 
 int elas_hip_stage_times(const elas_ctx* c, float* ms, int32_t n) {
-    if (!c || !ms) return ELAS_ERR_INVALID_ARGUMENT;
+    if (!c || !ms || c->server) return ELAS_ERR_INVALID_ARGUMENT;
     for (int i = 0; i < n && i < ELAS_T_COUNT; i++) ms[i] = c->last ? c->last->times[i] : 0.0f;
     return ELAS_OK;
+}
+
+int elas_hip_serve_start(elas_ctx* c, const elas_serve_config* cfg) {
+    if (!c) return ELAS_ERR_INVALID_ARGUMENT;
+    elas_serve_config d{0, 0, 0};
+    if (cfg) d = *cfg;
+    return elas_srv_start(c, d.hbm_budget_bytes, d.max_chunk_pairs, d.slots);
+}
+
+int elas_hip_submit(elas_ctx* c, const uint8_t* left, const uint8_t* right, float* D1, float* D2, int32_t width,
+                    int32_t height, int32_t disp_max, int32_t flags, int64_t* ticket) {
+    if (!c) return ELAS_ERR_INVALID_ARGUMENT;
+    const bool dev = (flags & 1) != 0;
+    ElasPairSpec s{width, height, disp_max < 0 ? c->prm.disp_max : disp_max, left, right, D1, D2, dev, dev, -1};
+    return elas_srv_submit(c, s, ticket);
+}
+
+int elas_hip_poll(elas_ctx* c, int64_t ticket, int32_t* status, float* latency_ms) {
+    if (!c) return ELAS_ERR_INVALID_ARGUMENT;
+    return elas_srv_poll(c, ticket, status, latency_ms);
+}
+
+int elas_hip_serve_stop(elas_ctx* c) {
+    if (!c) return ELAS_ERR_INVALID_ARGUMENT;
+    return elas_srv_stop(c);
+}
+
+int elas_hip_serve_stats(elas_ctx* c, elas_serve_stats* st) {
+    if (!c) return ELAS_ERR_INVALID_ARGUMENT;
+    return elas_srv_stats(c, st);
 }
 
 }  // extern "C"

@@ -13,6 +13,7 @@
 #include <stdint.h>
 
 #include <chrono>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -104,7 +105,12 @@ struct elas_ctx {
     // pipelined run of n pairs in chunks of up to `chunk` pairs over the slots
     int run_batch(const ElasPairSpec* specs, int n, int32_t* statuses, int chunk);
     int fetch(int stage, void* dst, size_t bytes, size_t* stage_bytes);
-    int fail(int code, const std::string& msg) { err = msg; return code; }
+    std::mutex err_m;  // the serving thread reports errors too
+    int fail(int code, const std::string& msg) {
+        std::lock_guard<std::mutex> g(err_m);
+        err = msg;
+        return code;
+    }
 
     // phases of one slot
     int slot_init(ElasSlot& S);
@@ -121,6 +127,13 @@ struct elas_ctx {
     int slot_finish(ElasSlot& S, int32_t* statuses);
     void slot_abort(ElasSlot& S);
 };
+
+// Serving loop (elas_serve.cpp).
+int elas_srv_start(elas_ctx* c, uint64_t budget, int max_chunk, int slots);
+int elas_srv_submit(elas_ctx* c, const ElasPairSpec& s, int64_t* ticket);
+int elas_srv_poll(elas_ctx* c, int64_t ticket, int32_t* status, float* latency_ms);
+int elas_srv_stop(elas_ctx* c);
+int elas_srv_stats(elas_ctx* c, elas_serve_stats* st);
 
 // Derived sizes of a pair and its device footprint in a slot arena.
 ElasDims elas_dims(const elas_params& p, int W, int H, int dmax);

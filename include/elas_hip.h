@@ -170,6 +170,43 @@ int elas_hip_shape_info(const elas_ctx* ctx, elas_shape_info* info);
 /* Per-stage device milliseconds of the last call (ELAS_T_COUNT floats). */
 int elas_hip_stage_times(const elas_ctx* ctx, float* ms, int32_t n);
 
+/* ---- Serving (no reference counterpart: the reference has one blocking
+ * process call).  A scheduler thread owned by the context admits queued
+ * requests into idle pipeline slots as chunks, FIFO, as long as the chunk's
+ * device footprint fits one slot's share of the HBM budget; requests that
+ * arrive while others are on the GPU join the next chunk, so the GPU stages
+ * of several chunks and the host Delaunay step of another overlap.  While
+ * serving, the blocking calls on this context return ELAS_ERR_BUSY. */
+typedef struct elas_serve_config {
+    uint64_t hbm_budget_bytes; /* device bytes for all slot arenas (0: 8 GiB)  */
+    int32_t max_chunk_pairs;   /* cap on pairs per chunk (0: 32)                */
+    int32_t slots;             /* pipeline slots (0: the context's, default 3)  */
+} elas_serve_config;
+
+typedef struct elas_serve_stats {
+    int64_t submitted, completed, chunks;
+    int64_t max_chunk_pairs;       /* largest chunk formed                     */
+    int64_t max_in_flight_pairs;   /* most pairs on the device at once          */
+    uint64_t slot_capacity_bytes;  /* arena bytes per slot (budget / slots)     */
+    uint64_t peak_in_flight_bytes; /* largest sum of admitted chunk footprints  */
+} elas_serve_stats;
+
+int elas_hip_serve_start(elas_ctx* ctx, const elas_serve_config* cfg);
+/* Queue one pair.  Host buffers unless flags & 1 (device-resident buffers).
+ * Buffers must stay valid until the request completes; pinned host buffers
+ * (hipHostMalloc) let the copies overlap the GPU work.  disp_max < 0: the
+ * context's.  Returns ELAS_ERR_OUT_OF_MEMORY at once for a pair larger than
+ * a slot's share of the budget. */
+int elas_hip_submit(elas_ctx* ctx, const uint8_t* left, const uint8_t* right, float* D1, float* D2,
+                    int32_t width, int32_t height, int32_t disp_max, int32_t flags, int64_t* ticket);
+/* ELAS_OK once the request has finished (its status in *status, its
+ * submit-to-completion time in *latency_ms; the ticket is then retired),
+ * ELAS_ERR_BUSY while it is queued or in flight. */
+int elas_hip_poll(elas_ctx* ctx, int64_t ticket, int32_t* status, float* latency_ms);
+/* Waits for every queued request, then stops the scheduler thread. */
+int elas_hip_serve_stop(elas_ctx* ctx);
+int elas_hip_serve_stats(elas_ctx* ctx, elas_serve_stats* stats);
+
 #ifdef __cplusplus
 }
 #endif
