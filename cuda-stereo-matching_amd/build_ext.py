@@ -25,7 +25,7 @@ REF_SRC = "/root/reference/src"
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 TRI_INSTANCES = 16
 
-HIP_SOURCES = ["k_desc.hip", "k_support.hip", "k_dense.hip"]
+HIP_SOURCES = ["k_desc.hip", "k_support.hip", "k_dense.hip", "k_post.hip"]
 HOST_SOURCES = ["elas_engine.cpp", "elas_capi.cpp", "elas_serve.cpp", "Elas.cpp"]
 HIPFLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

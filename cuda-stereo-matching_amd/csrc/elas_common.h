@@ -20,6 +20,10 @@
 //   [9..14] AB_a, AB_b, AC_a, AC_b, BC_a, BC_b         [15] unused
 #define ELAS_TRIREC 16
 
+#define ELAS_POST_TW 64      // post-filter tile: 64 x 16 output pixels
+#define ELAS_POST_TH 16
+#define ELAS_GAP_MAX_LINE 15872  // gap interpolation keeps a line in LDS (<= 62 KiB)
+
 struct ElasPair {
     int32_t W, H, dmin, dmax;
     int32_t cw, ch, gw, gh;
@@ -46,6 +50,7 @@ struct ElasPair {
     float* mean[2];          // [H][W] (float)prior: the reference's map (debug only)
     float* draw[2];          // [H][W] dense disparity before the LR check
     float* dout[2];          // [H][W] final disparity
+    int32_t* pbuf[2];        // [2][H][W] post-filter scratch (labels, sizes / ping-pong map)
 };
 
 // Uniform parameters of a launch (from elas_params).
@@ -60,6 +65,9 @@ struct ElasKParams {
     int32_t all_p_negative;  // every prior P(k), k <= plane_radius, is < 0
     int32_t dense_cpw;       // grid cells per wave in the dense tile (1 or 2)
     int32_t P[ELAS_MAX_P];
+    // post-processing (off unless enabled)
+    float speckle_sim;
+    int32_t speckle_size, ipol_gap, adaptive_mean, median, post_images;  // post_images: 1 (left) or 2
 };
 
 // Launch a stage over a batch: pairs/prefix live in device memory, prefix has
@@ -74,5 +82,6 @@ struct ElasLaunch {
 // Stage ids (one prefix table per stage).
 enum ElasStageId {
     S_DESC = 0, S_SUPPORT, S_INCON, S_REDUN_H, S_REDUN_V, S_BLOCKS, S_GRID_CELLS,
-    S_PLANE, S_RASTER, S_DENSE, S_PIXELS, S_GRID_FILL, S_NUM_STAGES
+    S_PLANE, S_RASTER, S_DENSE, S_PIXELS, S_GRID_FILL, S_POST_TILES, S_POST_ROWS, S_POST_COLS,
+    S_NUM_STAGES
 };

@@ -53,8 +53,10 @@ size_t support_lds(int step, int dmax, bool right) {
 
 int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
 
+bool post_scratch(const elas_params& p) { return p.speckle_size > 0 || p.filter_adaptive_mean || p.filter_median; }
+
 // device bytes of one pair in a slot arena
-size_t pair_arena(const ElasPairSpec& s, const ElasDims& d, bool debug) {
+size_t pair_arena(const ElasPairSpec& s, const ElasDims& d, bool debug, bool post) {
     size_t b = 0;
     if (!s.dev_in) b += 2 * al(d.npx);
     b += 2 * al(16 * d.npx);
@@ -64,6 +66,7 @@ size_t pair_arena(const ElasPairSpec& s, const ElasDims& d, bool debug) {
     b += 2 * al(12 * (size_t)d.tricap) + 2 * al(4 * ELAS_TRIREC * (size_t)d.tricap);
     b += 2 * al(4 * d.npx) + 2 * al(4 * d.npx);  // prior, draw
     if (!s.dev_out) b += 2 * al(4 * d.npx);
+    if (post) b += 2 * al(8 * d.npx);  // post-filter scratch
     return b;
 }
 
@@ -90,7 +93,7 @@ ElasDims elas_dims(const elas_params& p, int W, int H, int dmax) {
 }
 
 size_t elas_pair_device_bytes(const elas_params& p, const ElasPairSpec& s, bool debug) {
-    return pair_arena(s, elas_dims(p, s.W, s.H, s.dmax), debug);
+    return pair_arena(s, elas_dims(p, s.W, s.H, s.dmax), debug, post_scratch(p));
 }
 
 int elas_make_kparams(const elas_params* p, ElasKParams* kp, std::string* err) {
@@ -106,6 +109,8 @@ int elas_make_kparams(const elas_params* p, ElasKParams* kp, std::string* err) {
     if (!(p->beta > 0.0f) || !(p->gamma > 0.0f) || !(p->sigma > 0.0f) || !(p->sradius >= 0.0f))
         return bad("beta, gamma, sigma must be > 0 and sradius >= 0");
     if (!(p->support_threshold == p->support_threshold)) return bad("support_threshold is NaN");
+    if (p->speckle_size < 0 || p->ipol_gap_width < 0 || !(p->speckle_sim_threshold >= 0.0f))
+        return bad("speckle_size, ipol_gap_width and speckle_sim_threshold must be >= 0");
     memset(kp, 0, sizeof(*kp));
     kp->support_threshold = p->support_threshold;
     kp->support_texture = p->support_texture;
@@ -134,6 +139,12 @@ int elas_make_kparams(const elas_params* p, ElasKParams* kp, std::string* err) {
     kp->key_bias = -pmin;
     kp->all_p_negative = allneg ? 1 : 0;
     kp->dense_cpw = 1;
+    kp->speckle_sim = p->speckle_sim_threshold;
+    kp->speckle_size = p->speckle_size;
+    kp->ipol_gap = p->ipol_gap_width;
+    kp->adaptive_mean = p->filter_adaptive_mean ? 1 : 0;
+    kp->median = p->filter_median ? 1 : 0;
+    kp->post_images = p->postprocess_only_left ? 1 : 2;
     return ELAS_OK;
 }
 
@@ -167,8 +178,6 @@ void elas_ctx::slot_free(ElasSlot& S) {
 // Validates a chunk; returns the arena and pinned bytes it needs.
 int elas_ctx::slot_need(const ElasPairSpec* sp, int n, size_t* arena_need, size_t* pin_need) {
     if (n <= 0 || !sp) return fail(ELAS_ERR_INVALID_ARGUMENT, "empty batch");
-    if (prm.speckle_size > 0 || prm.ipol_gap_width > 0 || prm.filter_median || prm.filter_adaptive_mean)
-        return fail(ELAS_ERR_UNSUPPORTED, "post-processing filters are not available on the GPU path yet");
     size_t a = 0, psp = 0, ptri = 0;
     for (int i = 0; i < n; i++) {
         const ElasPairSpec& s = sp[i];
@@ -182,8 +191,10 @@ int elas_ctx::slot_need(const ElasPairSpec* sp, int n, size_t* arena_need, size_
             return fail(ELAS_ERR_UNSUPPORTED, "grid_size/disp_max exceed the dense LDS tile");
         if (support_lds(prm.candidate_stepsize, s.dmax, true) > 65536)
             return fail(ELAS_ERR_UNSUPPORTED, "candidate_stepsize/disp_max exceed the support LDS rows");
+        if (prm.ipol_gap_width > 0 && std::max(s.W, s.H) > ELAS_GAP_MAX_LINE)
+            return fail(ELAS_ERR_UNSUPPORTED, "gap interpolation supports lines up to 15872 pixels");
         const ElasDims d = elas_dims(prm, s.W, s.H, s.dmax);
-        a += pair_arena(s, d, debug);
+        a += pair_arena(s, d, debug, post_scratch(prm));
         psp += 2 * 8 * d.ncan;
         ptri += 2 * 12 * (size_t)d.tricap;
     }
@@ -253,7 +264,7 @@ int elas_ctx::slot_prepare(ElasSlot& S, const ElasPairSpec* sp, int n) {
     for (int i = 0; i < n; i++) {
         S.dd[i] = elas_dims(prm, sp[i].W, sp[i].H, sp[i].dmax);
         S.base[i] = a;
-        a += pair_arena(sp[i], S.dd[i], debug);
+        a += pair_arena(sp[i], S.dd[i], debug, post_scratch(prm));
         S.sp_off[i] = so;
         so += 2 * 2 * S.dd[i].ncan;
         S.tri_off[i] = to;
@@ -324,6 +335,8 @@ int elas_ctx::slot_prepare(ElasSlot& S, const ElasPairSpec* sp, int n) {
         for (int k = 0; k < 2; k++) P.draw[k] = (float*)take(4 * d.npx);
         if (!s.dev_out) { P.dout[0] = (float*)take(4 * d.npx); P.dout[1] = (float*)take(4 * d.npx); }
         else { P.dout[0] = s.D1; P.dout[1] = s.D2; }
+        if (post_scratch(prm))
+            for (int k = 0; k < 2; k++) P.pbuf[k] = (int32_t*)take(8 * d.npx);
     }
     return ELAS_OK;
 }
@@ -347,6 +360,9 @@ int stage_tiles(const elas_params& prm, const ElasPairSpec& s, const ElasDims& d
         case S_RASTER: return 2 * P.ntx * P.nty;
         case S_DENSE: return 2 * d.gh * cdiv(gs, R) * cdiv(d.gw, ELAS_DENSE_WAVES * cpw);
         case S_PIXELS: return cdiv((long)d.npx, 256);
+        case S_POST_TILES: return 2 * cdiv(s.W, ELAS_POST_TW) * cdiv(s.H, ELAS_POST_TH);
+        case S_POST_ROWS: return 2 * s.H;
+        case S_POST_COLS: return 2 * s.W;
     }
     return 0;
 }
@@ -587,9 +603,24 @@ int elas_ctx::slot_phase_b(ElasSlot& S) {
       if (L.total) elas_launch_dense(L, S.kpl, g(L, cap1024), dense_lds(prm.grid_size, S.maxd, S.cpw), s1); }
     HIPCHECK(hipEventRecord(S.ev[8], s1));
     { ElasLaunch L = launch_of(S, 1, S_PIXELS); if (L.total) elas_launch_lr(L, kp, g(L, cap256), s1); }
-    HIPCHECK(hipEventRecord(S.ev[9], s1));
-    HIPCHECK(hipEventRecord(S.ev[10], s1));
     if (debug) { ElasLaunch L = launch_of(S, 1, S_PIXELS); if (L.total) elas_launch_mean(L, g(L, cap256), s1); }
+    HIPCHECK(hipEventRecord(S.ev[9], s1));
+    // post-processing on D1 (and D2): speckle, gap interpolation, adaptive mean, median
+    {
+        const ElasLaunch Lt = launch_of(S, 1, S_POST_TILES), Lp = launch_of(S, 1, S_PIXELS);
+        if (kp.speckle_size > 0 && Lt.total) elas_launch_speckle(Lt, Lp, kp, cap256, s1);
+        if (kp.ipol_gap > 0 && Lt.total) {
+            int mw = 1, mh = 1;
+            for (int i = 0; i < n; i++) { mw = std::max(mw, S.specs[i].W); mh = std::max(mh, S.specs[i].H); }
+            elas_launch_gap(launch_of(S, 1, S_POST_ROWS), launch_of(S, 1, S_POST_COLS), kp, cap256,
+                            4 * (size_t)mw, 4 * (size_t)mh, s1);
+        }
+        int in_tmp = 0;
+        if (kp.adaptive_mean && Lt.total) { elas_launch_sep(Lt, kp, false, in_tmp, cap256, s1); in_tmp ^= 1; }
+        if (kp.median && Lt.total) { elas_launch_sep(Lt, kp, true, in_tmp, cap256, s1); in_tmp ^= 1; }
+        if (in_tmp && Lp.total) elas_launch_post_copy(Lp, kp, cap256, s1);
+    }
+    HIPCHECK(hipEventRecord(S.ev[10], s1));
     HIPCHECK(hipGetLastError());
     for (int i = 0; i < n; i++)
         if (!S.specs[i].dev_out) {

@@ -21,3 +21,10 @@ void elas_launch_raster(const ElasLaunch& L, int grid, hipStream_t s);
 void elas_launch_mean(const ElasLaunch& L, int grid, hipStream_t s);
 void elas_launch_dense(const ElasLaunch& L, const ElasKParams& kp, int grid, size_t lds, hipStream_t s);
 void elas_launch_lr(const ElasLaunch& L, const ElasKParams& kp, int grid, hipStream_t s);
+void elas_launch_speckle(const ElasLaunch& Ltiles, const ElasLaunch& Lpix, const ElasKParams& kp, int grid,
+                         hipStream_t s);
+void elas_launch_gap(const ElasLaunch& Lrows, const ElasLaunch& Lcols, const ElasKParams& kp, int grid,
+                     size_t lds_rows, size_t lds_cols, hipStream_t s);
+void elas_launch_sep(const ElasLaunch& Ltiles, const ElasKParams& kp, bool median, int from_tmp, int grid,
+                     hipStream_t s);
+void elas_launch_post_copy(const ElasLaunch& Lpix, const ElasKParams& kp, int grid, hipStream_t s);
