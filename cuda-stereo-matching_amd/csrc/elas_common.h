@@ -11,7 +11,8 @@
 
 #define ELAS_MAX_P 16        // prior table entries: |d - d_plane| in [0, 15]
 #define ELAS_DENSE_ROWS 3    // image rows per dense tile (a wave = 20 px x 3 rows)
-#define ELAS_DENSE_WAVES 16  // waves per dense tile; a wave covers dense_cpw cells
+#define ELAS_DENSE_WAVES 8   // waves per dense tile; a wave covers dense_cpw cells
+#define ELAS_DENSE_THREADS (64 * ELAS_DENSE_WAVES)
 #define ELAS_RASTER_T 32     // raster tile edge in pixels
 #define ELAS_MAX_DISP 511    // disp_max envelope (dense key packs d in 10 bits)
 

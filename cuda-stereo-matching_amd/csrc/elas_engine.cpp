@@ -40,9 +40,10 @@ size_t al(size_t x) { return (x + 255) & ~size_t(255); }
 
 int dense_rows(int gs) { return std::max(1, std::min(64 / gs, gs)); }
 
-// LDS of a dense tile: R rows x (16*cpw cells x gs + dmax) columns x 16 B
+// LDS of a dense tile: R rows x (waves*cpw cells x gs + dmax columns, padded
+// by up to 15 to the bank-friendly stride) x 16 B
 size_t dense_lds(int gs, int dmax, int cpw) {
-    return (size_t)dense_rows(gs) * (size_t)(ELAS_DENSE_WAVES * cpw * gs + dmax) * 16;
+    return (size_t)dense_rows(gs) * (size_t)(ELAS_DENSE_WAVES * cpw * gs + dmax + 15) * 16;
 }
 
 // LDS of a support workgroup: rows v-2 and v+2 over 63*step + dmax + 5
@@ -593,14 +594,14 @@ int elas_ctx::slot_phase_b(ElasSlot& S) {
     S.t_tri1 = clk::now();
     HIPCHECK(hipStreamWaitEvent(s1, S.ev[12], 0));
 
-    const int cap256 = std::max(1, num_cu) * 8, cap1024 = std::max(1, num_cu) * 2;
+    const int cap256 = std::max(1, num_cu) * 8, cap_dense = std::max(1, num_cu) * (2048 / ELAS_DENSE_THREADS);
     auto g = [&](const ElasLaunch& L, int cap) { return std::min(L.total, cap); };
     HIPCHECK(hipEventRecord(S.ev[6], s1));
     { ElasLaunch L = launch_of(S, 1, S_PLANE); if (L.total) elas_launch_plane(L, g(L, cap256), s1); }
     { ElasLaunch L = launch_of(S, 1, S_RASTER); if (L.total) elas_launch_raster(L, g(L, cap256), s1); }
     HIPCHECK(hipEventRecord(S.ev[7], s1));
     { ElasLaunch L = launch_of(S, 1, S_DENSE);
-      if (L.total) elas_launch_dense(L, S.kpl, g(L, cap1024), dense_lds(prm.grid_size, S.maxd, S.cpw), s1); }
+      if (L.total) elas_launch_dense(L, S.kpl, g(L, cap_dense), dense_lds(prm.grid_size, S.maxd, S.cpw), s1); }
     HIPCHECK(hipEventRecord(S.ev[8], s1));
     { ElasLaunch L = launch_of(S, 1, S_PIXELS); if (L.total) elas_launch_lr(L, kp, g(L, cap256), s1); }
     if (debug) { ElasLaunch L = launch_of(S, 1, S_PIXELS); if (L.total) elas_launch_mean(L, g(L, cap256), s1); }

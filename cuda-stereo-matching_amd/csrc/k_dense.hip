@@ -188,12 +188,14 @@ __global__ __launch_bounds__(256) void k_mean(ElasLaunch L) {
 
 // ---- dense matching: KerComputeDisparity.cu:156-316,378-402 ---------------
 // Tile = R image rows (R = min(64/gs, gs), inside one grid-cell row) x
-// 16*cpw grid cells; wave w owns cells gx0+w+16j, lane = (row, column) of a
-// cell, so
-// the cell's candidate set is wave-uniform and is walked on the scalar unit
-// as the set bits of its bitmask (ascending d).  The other image's R rows are
-// staged in LDS for the tile's disparity reach; the pixel's own 16-byte block
-// stays in registers.
+// ELAS_DENSE_WAVES*cpw grid cells; wave w owns cells gx0 + w + WAVES*j, lane =
+// (row, column) of a cell, so the cell's candidate set is wave-uniform and is
+// walked on the scalar unit as the set bits of its bitmask.  The other image's
+// R rows are staged in LDS for the tile's disparity reach, with a row stride
+// of 4 (mod 16) descriptors so that a wave's three rows hit disjoint banks;
+// the pixel's own 16-byte block stays in registers.  512-thread workgroups
+// capped at 64 VGPRs keep two workgroups per CU, so one stages while the
+// other matches.
 //
 // Reference order and tie rule: grid candidates outside the plane window
 // [dp-r, dp+r] ascending, then the window ascending with the prior P added,
@@ -203,7 +205,7 @@ __global__ __launch_bounds__(256) void k_mean(ElasLaunch L) {
 // own window entry (same d, lower cost), so phase 0 need not exclude the
 // window; otherwise it does.
 template <bool PNEG>
-__global__ __launch_bounds__(1024) void k_dense(ElasLaunch L, ElasKParams kp) {
+__global__ __launch_bounds__(ELAS_DENSE_THREADS, 8) void k_dense(ElasLaunch L, ElasKParams kp) {
     extern __shared__ uint4 s_i2[];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int cpw = kp.dense_cpw;
@@ -229,16 +231,19 @@ __global__ __launch_bounds__(1024) void k_dense(ElasLaunch L, ElasKParams kp) {
         const int c_lo = img ? u_lo : max(u_lo - P.dmax, 0);
         const int c_hi = img ? min(u_hi + P.dmax, W) : u_hi;
         const int ncols = max(c_hi - c_lo, 0);
+        // LDS row stride = 4 (mod 16) descriptors: the three rows a wave reads
+        // for one candidate fall on disjoint banks
+        const int rs = ncols + ((20 - (ncols & 15)) & 15);
         const uint8_t* I1 = P.desc[img];
         const uint8_t* I2 = P.desc[1 - img];
         // staging: every thread issues all of its loads before storing any
         {
             const int n = rows * ncols;
-            for (int base = 0; base < n; base += 4 * 1024) {
+            for (int base = 0; base < n; base += 4 * ELAS_DENSE_THREADS) {
                 uint4 q[4];
                 #pragma unroll
                 for (int k = 0; k < 4; k++) {
-                    const int i = base + threadIdx.x + k * 1024;
+                    const int i = base + threadIdx.x + k * ELAS_DENSE_THREADS;
                     if (i < n) {
                         const int rr = i / ncols, cc = i - rr * ncols;
                         q[k] = elas_ld(reinterpret_cast<const uint4*>(I2 + ((size_t)(v0 + rr) * W + c_lo + cc) * 16));
@@ -246,8 +251,11 @@ __global__ __launch_bounds__(1024) void k_dense(ElasLaunch L, ElasKParams kp) {
                 }
                 #pragma unroll
                 for (int k = 0; k < 4; k++) {
-                    const int i = base + threadIdx.x + k * 1024;
-                    if (i < n) s_i2[i] = q[k];
+                    const int i = base + threadIdx.x + k * ELAS_DENSE_THREADS;
+                    if (i < n) {
+                        const int rr = i / ncols, cc = i - rr * ncols;
+                        s_i2[rr * rs + cc] = q[k];
+                    }
                 }
             }
         }
@@ -292,7 +300,7 @@ __global__ __launch_bounds__(1024) void k_dense(ElasLaunch L, ElasKParams kp) {
             }
             const int dpmin = max(dp - kp.plane_radius, 0), dpmax = min(dp + kp.plane_radius, P.dmax);
             const int sgn = img ? 1 : -1;
-            const uint4* rowp = s_i2 + (row < rows ? row : 0) * ncols - c_lo;
+            const uint4* rowp = s_i2 + (row < rows ? row : 0) * rs - c_lo;
             const int usafe = min(max(u, c_lo), c_hi - 1);
             const uint32_t bias = (uint32_t)kp.key_bias;
             uint32_t kmin = 0xffffffffu;
@@ -307,11 +315,13 @@ __global__ __launch_bounds__(1024) void k_dense(ElasLaunch L, ElasKParams kp) {
             };
             for (int w = 0; w < P.words; w++) {
                 uint32_t bits = __builtin_amdgcn_readlane(maskv, 32 * ci + w);
-                while (bits) {
+                // four candidates at a time while four remain, so their LDS
+                // reads overlap; then one at a time (the set bits are uniform)
+                while (__builtin_popcount(bits) >= 4) {
                     int dq[4];
                     #pragma unroll
                     for (int q = 0; q < 4; q++) {
-                        dq[q] = bits ? w * 32 + __builtin_ctz(bits) : dq[0];
+                        dq[q] = w * 32 + __builtin_ctz(bits);
                         bits &= bits - 1u;
                     }
                     int uw[4];
@@ -326,27 +336,45 @@ __global__ __launch_bounds__(1024) void k_dense(ElasLaunch L, ElasKParams kp) {
                     #pragma unroll
                     for (int q = 0; q < 4; q++) kmin = min(kmin, grid_key(dq[q], b2[q], ok[q]));
                 }
+                while (bits) {
+                    const int d = w * 32 + __builtin_ctz(bits);
+                    bits &= bits - 1u;
+                    const int uw = u + sgn * d;
+                    const bool ok = cand_ok(d, uw);
+                    kmin = min(kmin, grid_key(d, rowp[ok ? uw : usafe], ok));
+                }
             }
-            // plane window d = dp - r .. dp + r with the prior added
-            const int pr = kp.plane_radius;
-            for (int k0 = 0; k0 <= 2 * pr; k0 += 4) {
-                int dq[4], uw[4];
+            // plane window d = dp - r .. dp + r with the prior P(|d - dp|)
+            // added; |d - dp| is the same for every lane of a step
+            const int pr = kp.plane_radius, nwin = 2 * pr + 1;
+            auto win_key = [&](int k, const uint4 b2, bool ok) {
+                const uint32_t sad = elas_sad16(b1, b2, 0u);
+                const int d = dp - pr + k;
+                const uint32_t key = (((uint32_t)((int)sad + kp.P[abs(k - pr)]) + bias) << 11) | 1024u |
+                                     (uint32_t)(d & 1023);
+                return ok ? key : 0xffffffffu;
+            };
+            auto win_ok = [&](int d, int uw) { return valid && d >= dpmin && d <= dpmax && uw >= 3 && uw <= W - 4; };
+            int k0 = 0;
+            for (; k0 + 4 <= nwin; k0 += 4) {
+                int uw[4];
                 bool ok[4];
                 uint4 b2[4];
                 #pragma unroll
                 for (int q = 0; q < 4; q++) {
-                    dq[q] = dp - pr + k0 + q;
-                    uw[q] = u + sgn * dq[q];
-                    ok[q] = valid && k0 + q <= 2 * pr && dq[q] >= dpmin && dq[q] <= dpmax && uw[q] >= 3 && uw[q] <= W - 4;
+                    const int d = dp - pr + k0 + q;
+                    uw[q] = u + sgn * d;
+                    ok[q] = win_ok(d, uw[q]);
                     b2[q] = rowp[ok[q] ? uw[q] : usafe];
                 }
                 #pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    const uint32_t sad = elas_sad16(b1, b2[q], 0u);
-                    const int pk = kp.P[min(abs(dq[q] - dp), ELAS_MAX_P - 1)];
-                    const uint32_t key = (((uint32_t)((int)sad + pk) + bias) << 11) | 1024u | (uint32_t)(dq[q] & 1023);
-                    kmin = min(kmin, ok[q] ? key : 0xffffffffu);
-                }
+                for (int q = 0; q < 4; q++) kmin = min(kmin, win_key(k0 + q, b2[q], ok[q]));
+            }
+            for (; k0 < nwin; k0++) {
+                const int d = dp - pr + k0;
+                const int uw = u + sgn * d;
+                const bool ok = win_ok(d, uw);
+                kmin = min(kmin, win_key(k0, rowp[ok ? uw : usafe], ok));
             }
             if (active)
                 elas_st(P.draw[img] + (size_t)v * W + u, (kmin != 0xffffffffu) ? (float)(kmin & 1023u) : -1.0f);
@@ -398,8 +426,8 @@ void elas_launch_mean(const ElasLaunch& L, int grid, hipStream_t s) {
     hipLaunchKernelGGL(k_mean, dim3(grid), dim3(256), 0, s, L);
 }
 void elas_launch_dense(const ElasLaunch& L, const ElasKParams& kp, int grid, size_t lds, hipStream_t s) {
-    if (kp.all_p_negative) hipLaunchKernelGGL(k_dense<true>, dim3(grid), dim3(1024), lds, s, L, kp);
-    else hipLaunchKernelGGL(k_dense<false>, dim3(grid), dim3(1024), lds, s, L, kp);
+    if (kp.all_p_negative) hipLaunchKernelGGL(k_dense<true>, dim3(grid), dim3(ELAS_DENSE_THREADS), lds, s, L, kp);
+    else hipLaunchKernelGGL(k_dense<false>, dim3(grid), dim3(ELAS_DENSE_THREADS), lds, s, L, kp);
 }
 void elas_launch_lr(const ElasLaunch& L, const ElasKParams& kp, int grid, hipStream_t s) {
     hipLaunchKernelGGL(k_lr, dim3(grid), dim3(256), 0, s, L, kp);
