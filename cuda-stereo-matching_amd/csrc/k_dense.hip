@@ -210,10 +210,13 @@ __global__ __launch_bounds__(ELAS_DENSE_THREADS, 8) void k_dense(ElasLaunch L, E
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int cpw = kp.dense_cpw;
     const int tcells = ELAS_DENSE_WAVES * cpw;
+    // a candidate outside the image (or of an excluded kind) starts its SAD
+    // from KEY_BAD: its key exceeds every valid key and stays below 2^32
+    constexpr uint32_t KEY_BAD = 0x1ff000u;
     for (int t = blockIdx.x; t < L.total; t += gridDim.x) {
         const int p = elas_find_pair(L.prefix, L.npairs, t);
         const ElasPair& P = L.pairs[p];
-        const int W = P.W, H = P.H, gs = P.gsize, gw = P.gw;
+        const int W = P.W, H = P.H, gs = P.gsize, gw = P.gw, dmax = P.dmax;
         const int R = max(1, min(64 / gs, gs));
         const int sub = (gs + R - 1) / R;
         const int segs = (gw + tcells - 1) / tcells;
@@ -227,36 +230,39 @@ __global__ __launch_bounds__(ELAS_DENSE_THREADS, 8) void k_dense(ElasLaunch L, E
         const int v0 = gy * gs + k * R;
         const int rows = max(0, min(min(R, gs - k * R), H - v0));
         const int gx0 = s * tcells;
-        const int u_lo = gx0 * gs, u_hi = min((gx0 + tcells) * gs, W);
-        const int c_lo = img ? u_lo : max(u_lo - P.dmax, 0);
-        const int c_hi = img ? min(u_hi + P.dmax, W) : u_hi;
-        const int ncols = max(c_hi - c_lo, 0);
-        // LDS row stride = 4 (mod 16) descriptors: the three rows a wave reads
-        // for one candidate fall on disjoint banks
-        const int rs = ncols + ((20 - (ncols & 15)) & 15);
+        // staged columns: the whole reach of the tile's lanes, [u_lo - dmax,
+        // u_lo + tcells*gs) or [u_lo, u_lo + tcells*gs + dmax); columns
+        // outside the image are zero and never selected by a valid candidate
+        const int u_lo = gx0 * gs;
+        const int ncols = tcells * gs + dmax;
+        const int c_lo = img ? u_lo : u_lo - dmax;
+        const int rs = ncols + ((20 - (ncols & 15)) & 15);  // 4 (mod 16): disjoint banks per row
         const uint8_t* I1 = P.desc[img];
         const uint8_t* I2 = P.desc[1 - img];
-        // staging: every thread issues all of its loads before storing any
         {
+            // every thread issues all of its loads before storing any
             const int n = rows * ncols;
             for (int base = 0; base < n; base += 4 * ELAS_DENSE_THREADS) {
                 uint4 q[4];
+                int li[4];
+                int rr = (base + (int)threadIdx.x) / ncols, cc = base + threadIdx.x - rr * ncols;
                 #pragma unroll
                 for (int k = 0; k < 4; k++) {
                     const int i = base + threadIdx.x + k * ELAS_DENSE_THREADS;
-                    if (i < n) {
-                        const int rr = i / ncols, cc = i - rr * ncols;
-                        q[k] = elas_ld(reinterpret_cast<const uint4*>(I2 + ((size_t)(v0 + rr) * W + c_lo + cc) * 16));
+                    const int c = c_lo + cc;
+                    q[k] = make_uint4(0u, 0u, 0u, 0u);
+                    if (i < n && c >= 0 && c < W)
+                        q[k] = elas_ld(reinterpret_cast<const uint4*>(I2 + ((size_t)(v0 + rr) * W + c) * 16));
+                    li[k] = i < n ? rr * rs + cc : -1;
+                    cc += ELAS_DENSE_THREADS;
+                    while (cc >= ncols) {
+                        cc -= ncols;
+                        rr++;
                     }
                 }
                 #pragma unroll
-                for (int k = 0; k < 4; k++) {
-                    const int i = base + threadIdx.x + k * ELAS_DENSE_THREADS;
-                    if (i < n) {
-                        const int rr = i / ncols, cc = i - rr * ncols;
-                        s_i2[rr * rs + cc] = q[k];
-                    }
-                }
+                for (int k = 0; k < 4; k++)
+                    if (li[k] >= 0) s_i2[li[k]] = q[k];
             }
         }
         // this lane's pixel in each of its cells: prior and own descriptor; lane
@@ -282,9 +288,6 @@ __global__ __launch_bounds__(ELAS_DENSE_THREADS, 8) void k_dense(ElasLaunch L, E
             }
         }
         __syncthreads();
-        // Branchless candidate evaluation: a lane that is invalid for a
-        // candidate reads a safe in-tile address and gets the all-ones key;
-        // candidates are taken four at a time so their LDS reads overlap.
         #pragma unroll
         for (int ci = 0; ci < 2; ci++) {
             const int gx = gx0 + wave + ci * ELAS_DENSE_WAVES;
@@ -298,86 +301,66 @@ __global__ __launch_bounds__(ELAS_DENSE_THREADS, 8) void k_dense(ElasLaunch L, E
                 const uint32_t tex = elas_sad16(b1, make_uint4(0x80808080u, 0x80808080u, 0x80808080u, 0x80808080u), 0u);
                 if ((int)tex < kp.match_texture) valid = false;
             }
-            const int dpmin = max(dp - kp.plane_radius, 0), dpmax = min(dp + kp.plane_radius, P.dmax);
+            // u_warp = u -/+ d must lie in [3, W-4] (KerComputeDisparity.cu:259,267): d <= dlim
+            const int dlim = img ? W - 4 - u : u - 3;
             const int sgn = img ? 1 : -1;
-            const uint4* rowp = s_i2 + (row < rows ? row : 0) * rs - c_lo;
-            const int usafe = min(max(u, c_lo), c_hi - 1);
+            // this lane's descriptor at disparity 0 in the staged row
+            const uint4* lp = s_i2 + (row < rows ? row : 0) * rs + (u - c_lo);
             const uint32_t bias = (uint32_t)kp.key_bias;
+            const int dpmin = max(dp - kp.plane_radius, 0), dpmax = min(dp + kp.plane_radius, dmax);
             uint32_t kmin = 0xffffffffu;
-            auto grid_key = [&](int d, const uint4 b2, bool ok) {
-                const uint32_t sad = elas_sad16(b1, b2, 0u);
-                return ok ? (((sad + bias) << 11) | (uint32_t)d) : 0xffffffffu;
-            };
-            auto cand_ok = [&](int d, int uw) {
-                bool ok = valid && uw >= 3 && uw <= W - 4;
+            // grid candidates (wave-uniform d): key = (sad + bias) << 11 | d
+            auto grid_acc = [&](int d) {
+                bool ok = d <= dlim;
                 if (!PNEG) ok = ok && (d < dpmin || d > dpmax);
-                return ok;
+                return ok ? bias : KEY_BAD;
             };
             for (int w = 0; w < P.words; w++) {
                 uint32_t bits = __builtin_amdgcn_readlane(maskv, 32 * ci + w);
                 // four candidates at a time while four remain, so their LDS
-                // reads overlap; then one at a time (the set bits are uniform)
+                // reads overlap; then one at a time
                 while (__builtin_popcount(bits) >= 4) {
                     int dq[4];
+                    uint4 b2[4];
                     #pragma unroll
                     for (int q = 0; q < 4; q++) {
                         dq[q] = w * 32 + __builtin_ctz(bits);
                         bits &= bits - 1u;
-                    }
-                    int uw[4];
-                    bool ok[4];
-                    uint4 b2[4];
-                    #pragma unroll
-                    for (int q = 0; q < 4; q++) {
-                        uw[q] = u + sgn * dq[q];
-                        ok[q] = cand_ok(dq[q], uw[q]);
-                        b2[q] = rowp[ok[q] ? uw[q] : usafe];
+                        b2[q] = lp[sgn * dq[q]];
                     }
                     #pragma unroll
-                    for (int q = 0; q < 4; q++) kmin = min(kmin, grid_key(dq[q], b2[q], ok[q]));
+                    for (int q = 0; q < 4; q++)
+                        kmin = min(kmin, (elas_sad16(b1, b2[q], grid_acc(dq[q])) << 11) | (uint32_t)dq[q]);
                 }
                 while (bits) {
                     const int d = w * 32 + __builtin_ctz(bits);
                     bits &= bits - 1u;
-                    const int uw = u + sgn * d;
-                    const bool ok = cand_ok(d, uw);
-                    kmin = min(kmin, grid_key(d, rowp[ok ? uw : usafe], ok));
+                    kmin = min(kmin, (elas_sad16(b1, lp[sgn * d], grid_acc(d)) << 11) | (uint32_t)d);
                 }
             }
-            // plane window d = dp - r .. dp + r with the prior P(|d - dp|)
-            // added; |d - dp| is the same for every lane of a step
+            // plane window d = dp - r + k, k = 0..2r, with the prior P(|k - r|)
+            // added: key = (sad + P + bias) << 11 | 1024 | d
             const int pr = kp.plane_radius, nwin = 2 * pr + 1;
-            auto win_key = [&](int k, const uint4 b2, bool ok) {
-                const uint32_t sad = elas_sad16(b1, b2, 0u);
-                const int d = dp - pr + k;
-                const uint32_t key = (((uint32_t)((int)sad + kp.P[abs(k - pr)]) + bias) << 11) | 1024u |
-                                     (uint32_t)(d & 1023);
-                return ok ? key : 0xffffffffu;
+            const int wlim = min(dmax, dlim);
+            const int d0 = dp - pr;
+            auto win_key = [&](int k, const uint4 b2) {
+                const int d = d0 + k;
+                const uint32_t acc = (uint32_t)d <= (uint32_t)wlim ? (uint32_t)((int)bias + kp.P[abs(k - pr)]) : KEY_BAD;
+                return (elas_sad16(b1, b2, acc) << 11) | (1024u | ((uint32_t)d & 1023u));
             };
-            auto win_ok = [&](int d, int uw) { return valid && d >= dpmin && d <= dpmax && uw >= 3 && uw <= W - 4; };
+            auto win_lds = [&](int k) { return lp[sgn * min(max(d0 + k, 0), dmax)]; };
             int k0 = 0;
             for (; k0 + 4 <= nwin; k0 += 4) {
-                int uw[4];
-                bool ok[4];
                 uint4 b2[4];
                 #pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    const int d = dp - pr + k0 + q;
-                    uw[q] = u + sgn * d;
-                    ok[q] = win_ok(d, uw[q]);
-                    b2[q] = rowp[ok[q] ? uw[q] : usafe];
-                }
+                for (int q = 0; q < 4; q++) b2[q] = win_lds(k0 + q);
                 #pragma unroll
-                for (int q = 0; q < 4; q++) kmin = min(kmin, win_key(k0 + q, b2[q], ok[q]));
+                for (int q = 0; q < 4; q++) kmin = min(kmin, win_key(k0 + q, b2[q]));
             }
-            for (; k0 < nwin; k0++) {
-                const int d = dp - pr + k0;
-                const int uw = u + sgn * d;
-                const bool ok = win_ok(d, uw);
-                kmin = min(kmin, win_key(k0, rowp[ok ? uw : usafe], ok));
-            }
+            for (; k0 < nwin; k0++) kmin = min(kmin, win_key(k0, win_lds(k0)));
             if (active)
-                elas_st(P.draw[img] + (size_t)v * W + u, (kmin != 0xffffffffu) ? (float)(kmin & 1023u) : -1.0f);
+                elas_st(P.draw[img] + (size_t)v * W + u,
+                        (valid && kmin < (KEY_BAD << 11)) ? (float)(kmin & 1023u) : -1.0f);
         }
         __syncthreads();
     }
