@@ -25,6 +25,18 @@
 #define ELAS_POST_TH 16
 #define ELAS_GAP_MAX_LINE 15872  // gap interpolation keeps a line in LDS (<= 62 KiB)
 
+// Lines per workgroup of the redundancy filters: a line of len cells takes
+// 3 bytes per cell of LDS (int16 value + flag byte); at most 4 lines (more
+// workgroups for the parallel phases), 48 KiB (plus alignment of the flags).
+constexpr int elas_redun_lines(int len) {
+    int l = 4;
+    while (l > 1 && l * len * 3 > 49152) l >>= 1;
+    return l;
+}
+constexpr int elas_redun_lds(int len) {
+    return ((2 * elas_redun_lines(len) * len + 15) & ~15) + elas_redun_lines(len) * ((len + 15) & ~15);
+}
+
 struct ElasPair {
     int32_t W, H, dmin, dmax;
     int32_t cw, ch, gw, gh;

@@ -352,12 +352,12 @@ int stage_tiles(const elas_params& prm, const ElasPairSpec& s, const ElasDims& d
         case S_DESC: return cdiv(s.W, 64) * cdiv(s.H, 16) * 2;
         case S_SUPPORT: return d.ch * cdiv(d.cw, 64);
         case S_INCON: return cdiv(d.cw, 16) * cdiv(d.ch, 16);
-        case S_REDUN_H: return cdiv(d.ch, 64);
-        case S_REDUN_V: return cdiv(d.cw, 64);
+        case S_REDUN_H: return cdiv(d.ch, elas_redun_lines(d.cw));
+        case S_REDUN_V: return cdiv(d.cw, elas_redun_lines(d.ch));
         case S_BLOCKS: return d.nblk;
         case S_GRID_CELLS: return cdiv(2 * (long)d.ncell, 256);
         case S_GRID_FILL: return cdiv(2 * (long)d.ncell * d.words, 256);
-        case S_PLANE: return cdiv((long)P.ntri[0] + P.ntri[1], 256);
+        case S_PLANE: return cdiv((long)P.ntri[0] + P.ntri[1], 64);
         case S_RASTER: return 2 * P.ntx * P.nty;
         case S_DENSE: return 2 * d.gh * cdiv(gs, R) * cdiv(d.gw, ELAS_DENSE_WAVES * cpw);
         case S_PIXELS: return cdiv((long)d.npx, 256);
@@ -416,8 +416,17 @@ int elas_ctx::slot_phase_a(ElasSlot& S) {
       } }
     HIPCHECK(hipEventRecord(S.ev[2], s1));
     { ElasLaunch L = launch_of(S, 0, S_INCON); if (L.total) elas_launch_incon(L, kp, g(L, cap256), s1); }
-    { ElasLaunch L = launch_of(S, 0, S_REDUN_H); if (L.total) elas_launch_redun(L, false, L.total, s1); }
-    { ElasLaunch L = launch_of(S, 0, S_REDUN_V); if (L.total) elas_launch_redun(L, true, L.total, s1); }
+    {
+        size_t lh = 4, lv = 4;
+        for (int i = 0; i < n; i++) {
+            lh = std::max(lh, (size_t)elas_redun_lds(S.dd[i].cw));
+            lv = std::max(lv, (size_t)elas_redun_lds(S.dd[i].ch));
+        }
+        ElasLaunch L = launch_of(S, 0, S_REDUN_H);
+        if (L.total) elas_launch_redun(L, false, g(L, cap256), lh, s1);
+        L = launch_of(S, 0, S_REDUN_V);
+        if (L.total) elas_launch_redun(L, true, g(L, cap256), lv, s1);
+    }
     HIPCHECK(hipEventRecord(S.ev[3], s1));
     { ElasLaunch L = launch_of(S, 0, S_BLOCKS);
       if (L.total) { elas_launch_blkcount(L, g(L, cap256), s1); elas_launch_compact(L, g(L, cap256), s1); } }
@@ -597,7 +606,7 @@ int elas_ctx::slot_phase_b(ElasSlot& S) {
     const int cap256 = std::max(1, num_cu) * 8, cap_dense = std::max(1, num_cu) * (2048 / ELAS_DENSE_THREADS);
     auto g = [&](const ElasLaunch& L, int cap) { return std::min(L.total, cap); };
     HIPCHECK(hipEventRecord(S.ev[6], s1));
-    { ElasLaunch L = launch_of(S, 1, S_PLANE); if (L.total) elas_launch_plane(L, g(L, cap256), s1); }
+    { ElasLaunch L = launch_of(S, 1, S_PLANE); if (L.total) elas_launch_plane(L, g(L, 4 * cap256), s1); }
     { ElasLaunch L = launch_of(S, 1, S_RASTER); if (L.total) elas_launch_raster(L, g(L, cap256), s1); }
     HIPCHECK(hipEventRecord(S.ev[7], s1));
     { ElasLaunch L = launch_of(S, 1, S_DENSE);

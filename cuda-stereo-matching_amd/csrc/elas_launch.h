@@ -9,7 +9,7 @@ void elas_launch_descriptor(const ElasLaunch& L, int grid, hipStream_t s);
 void elas_launch_support(const ElasLaunch& L, const ElasKParams& kp, bool right, int grid, size_t lds,
                          hipStream_t s);
 void elas_launch_incon(const ElasLaunch& L, const ElasKParams& kp, int grid, hipStream_t s);
-void elas_launch_redun(const ElasLaunch& L, bool vertical, int grid, hipStream_t s);
+void elas_launch_redun(const ElasLaunch& L, bool vertical, int grid, size_t lds, hipStream_t s);
 void elas_launch_blkcount(const ElasLaunch& L, int grid, hipStream_t s);
 void elas_launch_compact(const ElasLaunch& L, int grid, hipStream_t s);
 void elas_launch_grid_fill(const ElasLaunch& L, int grid, hipStream_t s);

@@ -42,26 +42,31 @@ __device__ bool elas_solve3(double A[3][3], double B[3]) {
 }
 
 // ---- per-triangle record: KerComputeDisparity.cu:10-124 -------------------
-// One lane per triangle of either mesh.  Corners come from the left support
+// One lane per triangle of either mesh, 64-lane workgroups so that a few
+// thousand triangles spread over many CUs.  Corners come from the left support
 // points sp0 for both meshes; the right mesh shifts them by -d (Elas.cu:105,
-// KerComputeDisparity.cu:63-66,95-99).
-__global__ __launch_bounds__(256) void k_plane(ElasLaunch L) {
+// KerComputeDisparity.cu:63-66,95-99).  A corner's d is sp0.u - sp1.u (exact
+// in float), which saves a dependent load of the lattice.
+__global__ __launch_bounds__(64) void k_plane(ElasLaunch L) {
     for (int t = blockIdx.x; t < L.total; t += gridDim.x) {
         const int p = elas_find_pair(L.prefix, L.npairs, t);
         const ElasPair& P = L.pairs[p];
-        const int idx = (t - L.prefix[p]) * 256 + threadIdx.x;
+        const int idx = (t - L.prefix[p]) * 64 + threadIdx.x;
         const int n0 = P.ntri[0], n1 = P.ntri[1];
         if (idx >= n0 + n1) continue;
         const int img = idx >= n0;
         const int ti = img ? idx - n0 : idx;
         const int32_t* tr = P.tri[img] + 3 * (size_t)ti;
         const float* sp = P.sp[0];
+        const float* sq = P.sp[1];
         int tu[3], tv[3], td[3];
         double A[3][3], b[3];
+        int ci[3];
+        for (int i = 0; i < 3; i++) ci[i] = tr[i];
         for (int i = 0; i < 3; i++) {
-            tu[i] = (int)sp[2 * tr[i]];
-            tv[i] = (int)sp[2 * tr[i] + 1];
-            td[i] = P.can[3][(tv[i] * P.cw + tu[i]) / P.step];
+            tu[i] = (int)sp[2 * ci[i]];
+            tv[i] = (int)sp[2 * ci[i] + 1];
+            td[i] = tu[i] - (int)sq[2 * ci[i]];
             A[i][0] = img ? (double)(tu[i] - td[i]) : (double)tu[i];
             A[i][1] = (double)tv[i];
             A[i][2] = 1.0;
@@ -400,7 +405,7 @@ __global__ __launch_bounds__(256) void k_lr(ElasLaunch L, ElasKParams kp) {
 }
 
 void elas_launch_plane(const ElasLaunch& L, int grid, hipStream_t s) {
-    hipLaunchKernelGGL(k_plane, dim3(grid), dim3(256), 0, s, L);
+    hipLaunchKernelGGL(k_plane, dim3(grid), dim3(64), 0, s, L);
 }
 void elas_launch_raster(const ElasLaunch& L, int grid, hipStream_t s) {
     hipLaunchKernelGGL(k_raster, dim3(grid), dim3(256), 0, s, L);

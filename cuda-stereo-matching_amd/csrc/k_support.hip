@@ -167,22 +167,50 @@ __global__ __launch_bounds__(256) void k_incon(ElasLaunch L, ElasKParams kp) {
         const bool lds = R <= INC_RMAX;
         const int32_t* in = P.can[1];
         if (lds) {
+            // all loads of a thread in flight before its stores; rows of the
+            // window by incremental index (no division)
             const int side = INC_T + 2 * R;
-            for (int i = tid; i < side * side; i += 256) {
-                const int yy = i / side, xx = i - yy * side;
+            constexpr int K = ((INC_T + 2 * INC_RMAX) * (INC_T + 2 * INC_RMAX) + 255) / 256;
+            int q[K];
+            int yy = tid / side, xx = tid - yy * side;
+            const int qy = 256 / side, rx = 256 - qy * side;
+            #pragma unroll
+            for (int k = 0; k < K; k++) {
                 const int gx = x0 - R + xx, gy = y0 - R + yy;
-                s_win[yy][xx] = (gx >= 0 && gx < cw && gy >= 0 && gy < ch) ? in[gy * cw + gx] : -1;
+                q[k] = (yy < side && gx >= 0 && gx < cw && gy >= 0 && gy < ch) ? elas_ld(in + gy * cw + gx) : -1;
+                yy += qy;
+                xx += rx;
+                if (xx >= side) { xx -= side; yy++; }
+            }
+            yy = tid / side;
+            xx = tid - yy * side;
+            #pragma unroll
+            for (int k = 0; k < K; k++) {
+                if (yy < side) s_win[yy][xx] = q[k];
+                yy += qy;
+                xx += rx;
+                if (xx >= side) { xx -= side; yy++; }
             }
         }
         __syncthreads();
         const int x = x0 + (tid & (INC_T - 1)), y = y0 + (tid / INC_T);
         if (x < cw && y < ch) {
-            const int d = in[y * cw + x];
+            const int lx = x - x0, ly = y - y0;
+            const int d = lds ? s_win[ly + R][lx + R] : in[y * cw + x];
             int res = d;
             if (d >= 0 && y < cw) {
                 int support = 0;
-                if (lds) {
-                    const int lx = x - x0, ly = y - y0;
+                // |d - d2| <= thr  <=>  (unsigned)(d2 - d + thr) <= 2 thr, for thr >= 0
+                const unsigned thr2 = 2u * (unsigned)kp.incon_threshold;
+                if (lds && R == 5 && kp.incon_threshold >= 0) {
+                    #pragma unroll
+                    for (int dy = 0; dy <= 10; dy++)
+                        #pragma unroll
+                        for (int dx = 0; dx <= 10; dx++) {
+                            const int d2 = s_win[ly + dy][lx + dx];
+                            support += (d2 >= 0 && (unsigned)(d2 - d + kp.incon_threshold) <= thr2) ? 1 : 0;
+                        }
+                } else if (lds) {
                     for (int dy = 0; dy <= 2 * R; dy++)
                         for (int dx = 0; dx <= 2 * R; dx++) {
                             const int d2 = s_win[ly + dy][lx + dx];
@@ -206,91 +234,154 @@ __global__ __launch_bounds__(256) void k_incon(ElasLaunch L, ElasKParams kp) {
 }
 
 // Redundancy filters, KerSupportMatches.cu:242-356: REDUN_MAX_DIST 5,
-// REDUN_THRESHOLD 1.  One lane per row (horizontal, can[2] -> can[3]) or per
-// column (vertical, in place on can[3]), scanned in order as in the
-// reference: a point is removed when a valid point within 5 cells on BOTH
-// sides differs by at most 1.  The already-scanned side sees updated values
-// (a 5-register window carried along the line); the other side sees the
-// values before this scan.  A 64-line group moves through the lattice in
-// 64-cell tiles (+5 cells of look-ahead): all four waves load a tile into
-// LDS with independent coalesced loads, wave 0 pulls each lane's 69 values
-// into registers and scans them, and all waves store the results.
-#define RD_T 64
-#define RD_LA 5
-#define RD_N (64 * (RD_T + RD_LA))
-#define RD_PER ((RD_N + 255) / 256)
-#define RD_INVALID (-(1 << 20))
-// a valid (a >= 0) and |a - d| <= 1, for d >= 0 and invalid a = RD_INVALID
+// REDUN_THRESHOLD 1, horizontal (can[2] -> can[3]) then vertical (in place on
+// can[3]).  The reference scans each line in order and removes a point when a
+// valid point within 5 cells on BOTH sides differs by at most 1.  In that
+// sequential scan the side ahead still holds the input values and the side
+// behind holds the results, so for cell x
+//   ahead(x)  = some x+j, j in 1..5, is valid and within 1     (input only)
+//   M(x)      = bits j-1 for which x-j is valid and within 1   (input only)
+//   removed(x)= valid(x) && ahead(x) && (M(x) & ~H(x)) != 0,
+// where H(x) holds the removed flags of x-1..x-5.  A workgroup takes lpw lines
+// (elas_redun_lines) into LDS with coalesced loads, computes ahead/M for every
+// cell in parallel, then one wave per line runs the 5-bit recurrence on the
+// scalar unit, and all threads write the lines back.
+#define RD_INV (-30000)  // invalid cell in the int16 strip: never within 1 of a valid d
 __device__ __forceinline__ bool rd_near(int a, int d) { return (unsigned)(a - d + 1) <= 2u; }
 
 template <bool VERTICAL>
 __global__ __launch_bounds__(256) void k_redun(ElasLaunch L) {
-    // H: element (line l, pos j) at [l][j]; V: at [j][l] -- both conflict-free
-    __shared__ int s_t[RD_N];
+    extern __shared__ int16_t s_rd[];  // [lpw][len] values, then [lpw][lenp] flag bytes
     const int tid = threadIdx.x;
     for (int t = blockIdx.x; t < L.total; t += gridDim.x) {
         const int p = elas_find_pair(L.prefix, L.npairs, t);
         const ElasPair& P = L.pairs[p];
         const int cw = P.cw, ch = P.ch;
-        const int line0 = (t - L.prefix[p]) * 64;
         const int nlines = VERTICAL ? cw : ch, len = VERTICAL ? ch : cw;
+        const int lpw = elas_redun_lines(len);
+        const int lenp = (len + 15) & ~15;
+        const int line0 = (t - L.prefix[p]) * lpw;
+        const int nl = min(lpw, nlines - line0);
         const int32_t* in = VERTICAL ? P.can[3] : P.can[2];
         int32_t* out = P.can[3];
-        int c1 = RD_INVALID, c2 = RD_INVALID, c3 = RD_INVALID, c4 = RD_INVALID, c5 = RD_INVALID;
-        for (int pos0 = 0; pos0 < len; pos0 += RD_T) {
-            int v[RD_PER];
+        int16_t* s_val = s_rd;
+        uint8_t* s_flag = reinterpret_cast<uint8_t*>(s_rd) + ((2 * lpw * len + 15) & ~15);
+        // cell e of the strip: H -> (line e / len, pos e % len), V -> (pos e / nl, line e % nl)
+        const int n = nl * len;
+        const int inner = VERTICAL ? nl : len;
+        auto gaddr = [&](int a, int b) -> size_t {  // (outer, inner) -> lattice index
+            return VERTICAL ? (size_t)a * cw + line0 + b : (size_t)(line0 + a) * cw + b;
+        };
+        auto laddr = [&](int a, int b) { return VERTICAL ? b * len + a : a * len + b; };
+        const int qa = 256 / inner, rb = 256 - qa * inner;  // a step of 256 cells
+        for (int base = 0; base < n; base += 16 * 256) {
+            int q[16], li[16];
+            int a = (base + tid) / inner, b = base + tid - a * inner;
             #pragma unroll
-            for (int k = 0; k < RD_PER; k++) {
-                const int i = tid + 256 * k;
-                int l, j;
-                if (VERTICAL) { j = i >> 6; l = i & 63; }
-                else { l = i / (RD_T + RD_LA); j = i - l * (RD_T + RD_LA); }
-                const int line = line0 + l, pos = pos0 + j;
-                v[k] = (i < RD_N && line < nlines && pos < len)
-                           ? in[VERTICAL ? pos * cw + line : line * cw + pos] : -1;
-            }
-            #pragma unroll
-            for (int k = 0; k < RD_PER; k++) {
-                const int i = tid + 256 * k;
-                if (i < RD_N) s_t[i] = v[k];
-            }
-            __syncthreads();
-            if (tid < 64) {
-                // Branchless scan: invalid cells hold a large negative sentinel, so
-                // "valid and within 1 of d" (d >= 0) is one unsigned compare.
-                const int lane = tid;
-                auto sidx = [&](int j) { return VERTICAL ? j * 64 + lane : lane * (RD_T + RD_LA) + j; };
-                int w[RD_T + RD_LA];
-                #pragma unroll
-                for (int j = 0; j < RD_T + RD_LA; j++) {
-                    const int x = s_t[sidx(j)];
-                    w[j] = x < 0 ? RD_INVALID : x;
-                }
-                #pragma unroll
-                for (int i = 0; i < RD_T; i++) {
-                    const int d = w[i];
-                    const bool ls = rd_near(c1, d) | rd_near(c2, d) | rd_near(c3, d) | rd_near(c4, d) | rd_near(c5, d);
-                    const bool rs = rd_near(w[i + 1], d) | rd_near(w[i + 2], d) | rd_near(w[i + 3], d) |
-                                    rd_near(w[i + 4], d) | rd_near(w[i + 5], d);
-                    const int res = ((d >= 0) & ls & rs) ? RD_INVALID : d;
-                    s_t[sidx(i)] = res < 0 ? -1 : res;
-                    c5 = c4; c4 = c3; c3 = c2; c2 = c1; c1 = res;
+            for (int k = 0; k < 16; k++) {
+                const bool ok = base + tid + 256 * k < n;
+                q[k] = ok ? elas_ld(in + gaddr(a, b)) : -1;
+                li[k] = ok ? laddr(a, b) : -1;
+                a += qa;
+                b += rb;
+                if (b >= inner) {
+                    b -= inner;
+                    a++;
                 }
             }
-            __syncthreads();
             #pragma unroll
-            for (int k = 0; k < (64 * RD_T) / 256; k++) {
-                const int i = tid + 256 * k;
-                int l, j;
-                if (VERTICAL) { j = i >> 6; l = i & 63; }
-                else { l = i >> 6; j = i & 63; }
-                const int line = line0 + l, pos = pos0 + j;
-                if (line < nlines && pos < len)
-                    out[VERTICAL ? pos * cw + line : line * cw + pos] =
-                        s_t[VERTICAL ? j * 64 + l : l * (RD_T + RD_LA) + j];
-            }
-            __syncthreads();
+            for (int k = 0; k < 16; k++)
+                if (li[k] >= 0) s_val[li[k]] = (int16_t)(q[k] < 0 ? RD_INV : q[k]);
         }
+        __syncthreads();
+        // flag byte: M(x) if valid(x) && ahead(x), else 0 (x is then never
+        // removed); a lane takes 2 cells (consecutive lanes read consecutive
+        // words: no bank conflicts)
+        const int runs = lenp >> 1;  // flags cover the padding too (as 0)
+        for (int e = tid; e < nl * runs; e += 256) {
+            const int l = e / runs, x0 = (e - l * runs) * 2;
+            const int16_t* row = s_val + l * len;
+            int w[12];
+            #pragma unroll
+            for (int j = 0; j < 12; j++) {
+                const int x = x0 - 5 + j;
+                w[j] = (x >= 0 && x < len) ? row[x] : RD_INV;
+            }
+            #pragma unroll
+            for (int i = 0; i < 2; i++) {
+                const int d = w[i + 5];
+                uint32_t f = 0;
+                bool ahead = false;
+                #pragma unroll
+                for (int j = 1; j <= 5; j++) {
+                    f |= rd_near(w[i + 5 - j], d) ? (1u << (j - 1)) : 0u;
+                    ahead |= rd_near(w[i + 5 + j], d);
+                }
+                s_flag[l * lenp + x0 + i] = (uint8_t)((d >= 0 && ahead) ? f : 0u);
+            }
+        }
+        __syncthreads();
+        {
+            // The recurrence runs on the scalar unit, one line per wave: nh has
+            // bit j-1 set when x-j was kept, and x is removed when its flag
+            // meets a kept neighbour.  Each lane holds 4 flag bytes of a
+            // 256-cell chunk; only the non-zero words are visited (a ballot
+            // lists them): across a run of zero flags every cell is kept, so the
+            // state just shifts in ones.  Lane k keeps the removal nibble of
+            // its 4 cells for a parallel apply.
+            const int wv = tid >> 6, lane = tid & 63;
+            if (wv < nl) {
+                const uint32_t* fl = reinterpret_cast<const uint32_t*>(s_flag + wv * lenp);
+                int16_t* row = s_val + wv * len;
+                uint32_t nh = 0;
+                auto skip = [&](int cells) {
+                    const int g = min(cells, 5);
+                    nh = ((nh << g) | ((1u << g) - 1u)) & 31u;
+                };
+                for (int c0 = 0; c0 < lenp; c0 += 256) {
+                    const int nd = min(64, (lenp - c0) >> 2);
+                    const uint32_t fv = lane < nd ? fl[(c0 >> 2) + lane] : 0u;
+                    uint64_t nz = __ballot(fv != 0u);
+                    uint32_t rmv = 0;
+                    int kprev = -1;
+                    while (nz) {
+                        const int k = __builtin_ctzll(nz);
+                        nz &= nz - 1u;
+                        skip(4 * (k - kprev - 1));
+                        const uint32_t f4 = __builtin_amdgcn_readlane(fv, k);
+                        uint32_t r4 = 0;
+                        #pragma unroll
+                        for (int i = 0; i < 4; i++) {
+                            const uint32_t m = (f4 >> (8 * i)) & 31u;
+                            const uint32_t r = ((m & nh) + 31u) >> 5;  // (m & nh) != 0, without a branch
+                            nh = ((nh << 1) & 30u) | (r ^ 1u);
+                            r4 |= r << i;
+                        }
+                        rmv = lane == k ? r4 : rmv;
+                        kprev = k;
+                    }
+                    skip(4 * (nd - kprev - 1));
+                    #pragma unroll
+                    for (int i = 0; i < 4; i++)
+                        if ((rmv >> i) & 1u) row[c0 + 4 * lane + i] = (int16_t)RD_INV;
+                }
+            }
+        }
+        __syncthreads();
+        {
+            int a = tid / inner, b = tid - a * inner;
+            for (int e = tid; e < n; e += 256) {
+                const int v = s_val[laddr(a, b)];
+                out[gaddr(a, b)] = v < 0 ? -1 : v;
+                a += qa;
+                b += rb;
+                if (b >= inner) {
+                    b -= inner;
+                    a++;
+                }
+            }
+        }
+        __syncthreads();
     }
 }
 
@@ -435,9 +526,9 @@ void elas_launch_support(const ElasLaunch& L, const ElasKParams& kp, bool right,
 void elas_launch_incon(const ElasLaunch& L, const ElasKParams& kp, int grid, hipStream_t s) {
     hipLaunchKernelGGL(k_incon, dim3(grid), dim3(256), 0, s, L, kp);
 }
-void elas_launch_redun(const ElasLaunch& L, bool vertical, int grid, hipStream_t s) {
-    if (vertical) hipLaunchKernelGGL(k_redun<true>, dim3(grid), dim3(256), 0, s, L);
-    else hipLaunchKernelGGL(k_redun<false>, dim3(grid), dim3(256), 0, s, L);
+void elas_launch_redun(const ElasLaunch& L, bool vertical, int grid, size_t lds, hipStream_t s) {
+    if (vertical) hipLaunchKernelGGL(k_redun<true>, dim3(grid), dim3(256), lds, s, L);
+    else hipLaunchKernelGGL(k_redun<false>, dim3(grid), dim3(256), lds, s, L);
 }
 void elas_launch_blkcount(const ElasLaunch& L, int grid, hipStream_t s) {
     hipLaunchKernelGGL(k_blkcount, dim3(grid), dim3(256), 0, s, L);
