@@ -21,6 +21,7 @@
 //   [9..14] AB_a, AB_b, AC_a, AC_b, BC_a, BC_b         [15] unused
 #define ELAS_TRIREC 16
 
+#define ELAS_DESC_TW 64      // descriptor tile: ELAS_DESC_TW x (1024 / ELAS_DESC_TW) pixels
 #define ELAS_POST_TW 64      // post-filter tile: 64 x 16 output pixels
 #define ELAS_POST_TH 16
 #define ELAS_GAP_MAX_LINE 15872  // gap interpolation keeps a line in LDS (<= 62 KiB)
@@ -84,10 +85,18 @@ struct ElasKParams {
 };
 
 // Launch a stage over a batch: pairs/prefix live in device memory, prefix has
-// npairs+1 entries, total = prefix[npairs].
+// npairs+1 entries, total = prefix[npairs].  Both are read-only for the
+// duration of a launch; device code reads them through the constant address
+// space, so they come in through scalar loads that do not wait on the
+// kernel's own vector loads and stores (vector loads of them would have to).
+#if defined(__HIP_DEVICE_COMPILE__)
+#define ELAS_CONST_AS __attribute__((address_space(4)))
+#else
+#define ELAS_CONST_AS
+#endif
 struct ElasLaunch {
-    const ElasPair* pairs;
-    const int32_t* prefix;
+    const ELAS_CONST_AS ElasPair* pairs;
+    const ELAS_CONST_AS int32_t* prefix;
     int32_t npairs;
     int32_t total;
 };

@@ -6,7 +6,7 @@
 
 // Pair owning flattened tile t: the last p with prefix[p] <= t.  t is
 // workgroup-uniform, so the search runs on the scalar unit.
-__device__ __forceinline__ int elas_find_pair(const int32_t* __restrict__ prefix, int np, int t) {
+__device__ __forceinline__ int elas_find_pair(const ELAS_CONST_AS int32_t* prefix, int np, int t) {
     int lo = 0, hi = np - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
@@ -37,6 +37,15 @@ __device__ __forceinline__ void elas_st(uint4* p, const uint4& v) {
     elas_u32x4 x;
     x.x = v.x; x.y = v.y; x.z = v.z; x.w = v.w;
     *(ELAS_AS_GLOBAL elas_u32x4*)p = x;
+}
+
+// Workgroup barrier that orders LDS only.  __syncthreads() is also a
+// workgroup memory fence, which waits for every global load and store the
+// wave has in flight (vmcnt 0); here a wave's prefetched loads and finished
+// tile's stores stay in flight across the barrier.  The asm's memory clobber
+// keeps the compiler from moving memory accesses across it.
+__device__ __forceinline__ void elas_lds_barrier() {
+    __asm__ __volatile__("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
 }
 
 // Sum of |a_i - b_i| over 16 bytes, accumulated onto acc (4 x v_sad_u8).

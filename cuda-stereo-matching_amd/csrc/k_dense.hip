@@ -50,7 +50,7 @@ __device__ bool elas_solve3(double A[3][3], double B[3]) {
 __global__ __launch_bounds__(64) void k_plane(ElasLaunch L) {
     for (int t = blockIdx.x; t < L.total; t += gridDim.x) {
         const int p = elas_find_pair(L.prefix, L.npairs, t);
-        const ElasPair& P = L.pairs[p];
+        const auto& P = L.pairs[p];
         const int idx = (t - L.prefix[p]) * 64 + threadIdx.x;
         const int n0 = P.ntri[0], n1 = P.ntri[1];
         if (idx >= n0 + n1) continue;
@@ -120,7 +120,7 @@ __global__ __launch_bounds__(256) void k_raster(ElasLaunch L) {
     const int tid = threadIdx.x;
     for (int t = blockIdx.x; t < L.total; t += gridDim.x) {
         const int p = elas_find_pair(L.prefix, L.npairs, t);
-        const ElasPair& P = L.pairs[p];
+        const auto& P = L.pairs[p];
         const int W = P.W, H = P.H;
         const int ntiles = P.ntx * P.nty;
         int local = t - L.prefix[p];
@@ -183,7 +183,7 @@ __global__ __launch_bounds__(256) void k_raster(ElasLaunch L) {
 __global__ __launch_bounds__(256) void k_mean(ElasLaunch L) {
     for (int t = blockIdx.x; t < L.total; t += gridDim.x) {
         const int p = elas_find_pair(L.prefix, L.npairs, t);
-        const ElasPair& P = L.pairs[p];
+        const auto& P = L.pairs[p];
         const int i = (t - L.prefix[p]) * 256 + threadIdx.x;
         if (i >= P.W * P.H) continue;
         P.mean[0][i] = (float)P.prior[0][i];
@@ -220,7 +220,7 @@ __global__ __launch_bounds__(ELAS_DENSE_THREADS, 8) void k_dense(ElasLaunch L, E
     constexpr uint32_t KEY_BAD = 0x1ff000u;
     for (int t = blockIdx.x; t < L.total; t += gridDim.x) {
         const int p = elas_find_pair(L.prefix, L.npairs, t);
-        const ElasPair& P = L.pairs[p];
+        const auto& P = L.pairs[p];
         const int W = P.W, H = P.H, gs = P.gsize, gw = P.gw, dmax = P.dmax;
         const int R = max(1, min(64 / gs, gs));
         const int sub = (gs + R - 1) / R;
@@ -378,7 +378,7 @@ __global__ __launch_bounds__(ELAS_DENSE_THREADS, 8) void k_dense(ElasLaunch L, E
 __global__ __launch_bounds__(256) void k_lr(ElasLaunch L, ElasKParams kp) {
     for (int t = blockIdx.x; t < L.total; t += gridDim.x) {
         const int p = elas_find_pair(L.prefix, L.npairs, t);
-        const ElasPair& P = L.pairs[p];
+        const auto& P = L.pairs[p];
         const int i = (t - L.prefix[p]) * 256 + threadIdx.x;
         const int W = P.W;
         if (i >= W * P.H) continue;

@@ -47,7 +47,7 @@ struct PostTile {
 __device__ __forceinline__ PostTile post_tile(const ElasLaunch& L, int t) {
     PostTile r;
     r.p = elas_find_pair(L.prefix, L.npairs, t);
-    const ElasPair& P = L.pairs[r.p];
+    const auto& P = L.pairs[r.p];
     const int tx = (P.W + PT_W - 1) / PT_W, ty = (P.H + PT_H - 1) / PT_H;
     int local = t - L.prefix[r.p];
     r.img = local / (tx * ty);
@@ -129,7 +129,7 @@ __global__ __launch_bounds__(256) void k_speckle_local(ElasLaunch L, ElasKParams
     for (int t = blockIdx.x; t < L.total; t += gridDim.x) {
         const PostTile T = post_tile(L, t);
         if (T.img >= kp.post_images) continue;
-        const ElasPair& P = L.pairs[T.p];
+        const auto& P = L.pairs[T.p];
         const int W = P.W, H = P.H;
         const float* D = P.dout[T.img];
         float q[PT_H * PT_W / 256];
@@ -183,7 +183,7 @@ __global__ __launch_bounds__(256) void k_speckle_merge(ElasLaunch L, ElasKParams
     for (int t = blockIdx.x; t < L.total; t += gridDim.x) {
         const PostTile T = post_tile(L, t);
         if (T.img >= kp.post_images) continue;
-        const ElasPair& P = L.pairs[T.p];
+        const auto& P = L.pairs[T.p];
         const int W = P.W, H = P.H;
         const float* D = P.dout[T.img];
         int32_t* lab = P.pbuf[T.img];
@@ -205,7 +205,7 @@ __global__ __launch_bounds__(256) void k_speckle_merge(ElasLaunch L, ElasKParams
 __global__ __launch_bounds__(256) void k_speckle_count(ElasLaunch L, ElasKParams kp) {
     for (int t = blockIdx.x; t < L.total; t += gridDim.x) {
         const int p = elas_find_pair(L.prefix, L.npairs, t);
-        const ElasPair& P = L.pairs[p];
+        const auto& P = L.pairs[p];
         const int n = P.W * P.H;
         const int i = (t - L.prefix[p]) * 256 + threadIdx.x;
         for (int k = 0; k < kp.post_images; k++) {
@@ -234,7 +234,7 @@ __global__ __launch_bounds__(256) void k_speckle_count(ElasLaunch L, ElasKParams
 __global__ __launch_bounds__(256) void k_speckle_apply(ElasLaunch L, ElasKParams kp) {
     for (int t = blockIdx.x; t < L.total; t += gridDim.x) {
         const int p = elas_find_pair(L.prefix, L.npairs, t);
-        const ElasPair& P = L.pairs[p];
+        const auto& P = L.pairs[p];
         const int n = P.W * P.H;
         const int i = (t - L.prefix[p]) * 256 + threadIdx.x;
         if (i >= n) continue;
@@ -256,7 +256,7 @@ __global__ __launch_bounds__(256) void k_gap(ElasLaunch L, ElasKParams kp) {
     const int gap = kp.ipol_gap;
     for (int t = blockIdx.x; t < L.total; t += gridDim.x) {
         const int p = elas_find_pair(L.prefix, L.npairs, t);
-        const ElasPair& P = L.pairs[p];
+        const auto& P = L.pairs[p];
         const int W = P.W, H = P.H;
         const int nl = COLS ? W : H, len = COLS ? H : W;
         const int local = t - L.prefix[p];
@@ -374,7 +374,7 @@ __global__ __launch_bounds__(256) void k_sep(ElasLaunch L, ElasKParams kp, int f
     for (int t = blockIdx.x; t < L.total; t += gridDim.x) {
         const PostTile T = post_tile(L, t);
         if (T.img >= kp.post_images) continue;
-        const ElasPair& P = L.pairs[T.p];
+        const auto& P = L.pairs[T.p];
         const int W = P.W, H = P.H;
         float* mapA = P.dout[T.img];
         float* mapB = reinterpret_cast<float*>(P.pbuf[T.img] + (size_t)W * H);
@@ -424,7 +424,7 @@ __global__ __launch_bounds__(256) void k_sep(ElasLaunch L, ElasKParams kp, int f
 __global__ __launch_bounds__(256) void k_post_copy(ElasLaunch L, ElasKParams kp) {
     for (int t = blockIdx.x; t < L.total; t += gridDim.x) {
         const int p = elas_find_pair(L.prefix, L.npairs, t);
-        const ElasPair& P = L.pairs[p];
+        const auto& P = L.pairs[p];
         const int n = P.W * P.H;
         const int i = (t - L.prefix[p]) * 256 + threadIdx.x;
         if (i >= n) continue;

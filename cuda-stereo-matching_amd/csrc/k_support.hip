@@ -40,7 +40,7 @@ __global__ __launch_bounds__(256) void k_support(ElasLaunch L, ElasKParams kp) {
     const int j = lane >> 4, cl = lane & 15;
     for (int t = blockIdx.x; t < L.total; t += gridDim.x) {
         const int p = elas_find_pair(L.prefix, L.npairs, t);
-        const ElasPair& P = L.pairs[p];
+        const auto& P = L.pairs[p];
         const int W = P.W, H = P.H, cw = P.cw, step = P.step;
         const int local = t - L.prefix[p];
         const int per_row = (cw + 63) >> 6;
@@ -158,7 +158,7 @@ __global__ __launch_bounds__(256) void k_incon(ElasLaunch L, ElasKParams kp) {
     const int tid = threadIdx.x;
     for (int t = blockIdx.x; t < L.total; t += gridDim.x) {
         const int p = elas_find_pair(L.prefix, L.npairs, t);
-        const ElasPair& P = L.pairs[p];
+        const auto& P = L.pairs[p];
         const int cw = P.cw, ch = P.ch;
         const int local = t - L.prefix[p];
         const int tx = (cw + INC_T - 1) / INC_T;
@@ -255,7 +255,7 @@ __global__ __launch_bounds__(256) void k_redun(ElasLaunch L) {
     const int tid = threadIdx.x;
     for (int t = blockIdx.x; t < L.total; t += gridDim.x) {
         const int p = elas_find_pair(L.prefix, L.npairs, t);
-        const ElasPair& P = L.pairs[p];
+        const auto& P = L.pairs[p];
         const int cw = P.cw, ch = P.ch;
         const int nlines = VERTICAL ? cw : ch, len = VERTICAL ? ch : cw;
         const int lpw = elas_redun_lines(len);
@@ -392,7 +392,7 @@ __global__ __launch_bounds__(256) void k_redun(ElasLaunch L) {
 __global__ __launch_bounds__(256) void k_blkcount(ElasLaunch L) {
     for (int t = blockIdx.x; t < L.total; t += gridDim.x) {
         const int p = elas_find_pair(L.prefix, L.npairs, t);
-        const ElasPair& P = L.pairs[p];
+        const auto& P = L.pairs[p];
         const int b = t - L.prefix[p];
         const int idx = b * 256 + threadIdx.x;
         const bool valid = idx < P.cw * P.ch && P.can[3][idx] >= 0;
@@ -407,7 +407,7 @@ __global__ __launch_bounds__(256) void k_compact(ElasLaunch L) {
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     for (int t = blockIdx.x; t < L.total; t += gridDim.x) {
         const int p = elas_find_pair(L.prefix, L.npairs, t);
-        const ElasPair& P = L.pairs[p];
+        const auto& P = L.pairs[p];
         const int b = t - L.prefix[p];
         int acc = 0;
         for (int j = tid; j < b; j += 256) acc += P.blkcnt[j];
@@ -447,7 +447,7 @@ __global__ __launch_bounds__(256) void k_compact(ElasLaunch L) {
 __global__ __launch_bounds__(256) void k_grid_fill(ElasLaunch L) {
     for (int t = blockIdx.x; t < L.total; t += gridDim.x) {
         const int p = elas_find_pair(L.prefix, L.npairs, t);
-        const ElasPair& P = L.pairs[p];
+        const auto& P = L.pairs[p];
         const int n = P.gw * P.gh * P.words;
         const int i = (t - L.prefix[p]) * 256 + threadIdx.x;
         if (i < 2 * n) P.gflag[i >= n][i >= n ? i - n : i] = 0u;
@@ -457,7 +457,7 @@ __global__ __launch_bounds__(256) void k_grid_fill(ElasLaunch L) {
 __global__ __launch_bounds__(256) void k_grid_scatter(ElasLaunch L) {
     for (int t = blockIdx.x; t < L.total; t += gridDim.x) {
         const int p = elas_find_pair(L.prefix, L.npairs, t);
-        const ElasPair& P = L.pairs[p];
+        const auto& P = L.pairs[p];
         const int idx = (t - L.prefix[p]) * 256 + threadIdx.x;
         if (idx >= P.cw * P.ch) continue;
         const int d = P.can[3][idx];
@@ -480,7 +480,7 @@ __global__ __launch_bounds__(256) void k_grid_scatter(ElasLaunch L) {
 __global__ __launch_bounds__(256) void k_grid_dilate(ElasLaunch L) {
     for (int t = blockIdx.x; t < L.total; t += gridDim.x) {
         const int p = elas_find_pair(L.prefix, L.npairs, t);
-        const ElasPair& P = L.pairs[p];
+        const auto& P = L.pairs[p];
         const int ncell = P.gw * P.gh;
         const int i = (t - L.prefix[p]) * 256 + threadIdx.x;
         if (i >= 2 * ncell) continue;
@@ -503,7 +503,7 @@ __global__ __launch_bounds__(256) void k_grid_dilate(ElasLaunch L) {
 __global__ __launch_bounds__(256) void k_grid_list(ElasLaunch L) {
     for (int t = blockIdx.x; t < L.total; t += gridDim.x) {
         const int p = elas_find_pair(L.prefix, L.npairs, t);
-        const ElasPair& P = L.pairs[p];
+        const auto& P = L.pairs[p];
         const int ncell = P.gw * P.gh;
         const int i = (t - L.prefix[p]) * 256 + threadIdx.x;
         if (i >= 2 * ncell) continue;
