@@ -174,6 +174,66 @@ def serving(pkg, synth, rates=(100, 400, 800), n_req=400, pool=None, seed=4000):
     return {"rates": out, "pool_pairs": len(P), "hbm_budget_GiB": 8, "buffers": "pinned host"}
 
 
+def kernel_rooflines(el, wc, W, H, dmax, reps=50):
+    """Per-kernel roofline of the last call.  Each kernel's average launch
+    time comes from HIP events around `reps` back-to-back re-launches on the
+    pipeline stream (elas_hip_time_kernel).  Algorithmic work per launch:
+      descriptor  17 B per pixel and image (1 B image read, 16 B written)
+      support     64 byte-|diffs| per evaluated (candidate, d) (oracle count);
+                  HBM: descriptor rows v-2/v+2 of both images, 64 B per
+                  lattice-row pixel
+      dense       16 byte-|diffs| per evaluated (pixel, d) (oracle count);
+                  HBM: own + other descriptor, prior, output: 40 B per pixel
+                  and image
+      lr          16 B per pixel (read D1, D2; write D1, D2)
+    The bound is the one with the longer time at its peak (HBM 8 TB/s,
+    v_sad_u8 157.3 T byte-|diff|/s)."""
+    cw, ch = W // 5, H // 5
+    work = {
+        "descriptor": (0.0, 2 * 17.0 * W * H),
+        "support_left": (64.0 * wc["support_left"], 64.0 * W * ch * 2),
+        "support_right": (64.0 * wc["support_right"], 64.0 * W * ch * 2),
+        "dense": (16.0 * wc["dense"], 2 * 40.0 * W * H),
+        "lr": (0.0, 16.0 * W * H),
+    }
+    out = {}
+    for k, (ops, byts) in work.items():
+        ms = el.time_kernel(k, reps)
+        t_sad = ops / (SAD_PEAK_TBYTE * 1e12)
+        t_hbm = byts / (HBM_PEAK_GBS * 1e9)
+        if t_sad >= t_hbm:
+            out[k] = dict(bound="valu_sad", achieved=ops / (ms * 1e-3) / 1e12, peak=SAD_PEAK_TBYTE,
+                          unit="Tbyte-diff/s", ms=ms, algorithmic=ops)
+        else:
+            out[k] = dict(bound="hbm", achieved=byts / (ms * 1e-3) / 1e9, peak=HBM_PEAK_GBS, unit="GB/s",
+                          ms=ms, algorithmic=byts)
+        out[k]["frac"] = out[k]["achieved"] / out[k]["peak"]
+        out[k]["sad_frac"] = (ops / (ms * 1e-3) / 1e12) / SAD_PEAK_TBYTE
+        out[k]["hbm_frac"] = (byts / (ms * 1e-3) / 1e9) / HBM_PEAK_GBS
+    return out
+
+
+KERNEL_SYMBOL = {"descriptor": "k_descriptor", "support_left": "k_support<false>",
+                 "support_right": "k_support<true>", "dense": "k_dense<true>", "lr": "k_lr"}
+
+
+def traffic_of(kernel, workload, W, H, dmax):
+    """HBM bytes per launch of `kernel` from the committed PMC summary
+    profiles/hbm_traffic_<workload>_<W>x<H>_d<dmax+1>.txt (FETCH_SIZE x2 +
+    WRITE_SIZE x1, scaled by a plain device copy), or None."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_%s_%dx%d_d%d.txt" % (workload, W, H, dmax + 1))
+    if not os.path.isfile(path):
+        return None
+    for line in open(path):
+        parts = line.split()
+        if len(parts) >= 4 and parts[0] == KERNEL_SYMBOL.get(kernel):
+            try:
+                return int(float(parts[3]))
+            except ValueError:
+                return None
+    return None
+
+
 def cpu_baseline(oracle, left, right, dmax, budget_s=12.0):
     prm = oracle.params(disp_max=dmax)
     n, t0 = 0, time.perf_counter()
@@ -247,24 +307,12 @@ def main():
     value = args.steps * mpxd / wall
 
     wc = work_counts(oracle, left, right, dmax)
-    sup_ms = stages["support"]
-    sup_ops = 64.0 * (wc["support_left"] + wc["support_right"])
-    den_ops = 16.0 * wc["dense"]
-    desc_bytes = 2 * 17.0 * W * H
-    kernels = {
-        "support": dict(bound="valu_sad", achieved=sup_ops / (sup_ms * 1e-3) / 1e12, peak=SAD_PEAK_TBYTE,
-                        unit="Tbyte-diff/s", ms=sup_ms),
-        "dense": dict(bound="valu_sad", achieved=den_ops / (stages["dense"] * 1e-3) / 1e12,
-                      peak=SAD_PEAK_TBYTE, unit="Tbyte-diff/s", ms=stages["dense"]),
-        "descriptor": dict(bound="hbm", achieved=desc_bytes / (stages["descriptor"] * 1e-3) / 1e9,
-                           peak=HBM_PEAK_GBS, unit="GB/s", ms=stages["descriptor"]),
-    }
-    for k in kernels.values():
-        k["frac"] = k["achieved"] / k["peak"]
-    dom = max(("support", "dense", "descriptor"), key=lambda k: kernels[k]["ms"])
+    kernels = kernel_rooflines(el, wc, W, H, dmax)
+    dom = max(kernels, key=lambda k: kernels[k]["ms"])
     d = kernels[dom]
     roofline = {"bound": d["bound"], "achieved": round(d["achieved"], 4), "peak": round(d["peak"], 3),
-                "unit": d["unit"], "frac": round(d["frac"], 5), "traffic": None, "kernel": dom}
+                "unit": d["unit"], "frac": round(d["frac"], 5), "traffic": traffic_of(dom, args.workload, W, H, dmax),
+                "kernel": dom, "ms_per_launch": round(d["ms"], 5)}
 
     res = {
         "metric": "Mpx*disp/s", "value": round(value, 2), "unit": "Mpx*disp/s", "n_gpus": 1,
@@ -275,8 +323,8 @@ def main():
                    "disp_max": dmax, "pairs_per_step": 1, "host_delaunay": "included"},
         "pairs_per_s": round(args.steps / wall, 2),
         "stages_ms": {k: round(v, 5) for k, v in stages.items()},
-        "gpu_ms_per_pair": round(sum(stages[k] for k in ("descriptor", "support", "filter", "compact",
-                                                         "plane", "dense", "lr")), 5),
+        "gpu_stage_ms_per_pair": round(sum(stages[k] for k in ("descriptor", "support", "filter", "compact",
+                                                               "plane", "dense", "lr", "post")), 5),
         "roofline": roofline,
         "kernels": {k: {kk: (round(vv, 5) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                     for k, v in kernels.items()},

@@ -24,6 +24,7 @@ ELAS_OK, ELAS_ERR_TOO_FEW_POINTS, ELAS_ERR_BUSY = 0, 4, 7
 
 STAGES = ["desc0", "desc1", "can_left", "can_lr", "can_incon", "can_final", "sp0", "sp1", "grid0",
           "grid1", "tri0", "tri1", "mean0", "mean1", "d1_raw", "d2_raw", "D1", "D2"]
+KERNELS = ["descriptor", "support_left", "support_right", "dense", "lr"]  # elas_kernel order
 TIMERS = ["descriptor", "support", "filter", "compact", "grid", "triangle", "plane", "dense", "lr",
           "post", "total"]
 
@@ -82,6 +83,7 @@ SIGNATURES = {
     "elas_hip_process_batch": (ctypes.c_int, [_P, ctypes.c_int32, _P, _P, _P, _P, _P, _P]),
     "elas_hip_process_batch_device": (ctypes.c_int, [_P, ctypes.c_int32, _P, _P, _P, _P, _P, _P]),
     "elas_hip_set_pipeline": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32]),
+    "elas_hip_time_kernel": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(ctypes.c_float)]),
     "elas_hip_serve_start": (ctypes.c_int, [_P, _P]),
     "elas_hip_submit": (ctypes.c_int, [_P, _P, _P, _P, _P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                        ctypes.c_int32, ctypes.POINTER(ctypes.c_int64)]),
@@ -225,6 +227,13 @@ class Elas:
         st = self._lib.elas_hip_process_device(self._ctx, left.data_ptr(), right.data_ptr(), D1.data_ptr(),
                                                D2.data_ptr(), W, H)
         return self._check(st, allow_few=True)
+
+    def time_kernel(self, kernel, reps=20):
+        """Average ms per launch of one kernel of the last call, re-launched
+        `reps` times back to back (kernel: one of KERNELS)."""
+        ms = ctypes.c_float(0)
+        self._check(self._lib.elas_hip_time_kernel(self._ctx, KERNELS.index(kernel), int(reps), ctypes.byref(ms)))
+        return float(ms.value)
 
     # ---- serving -----------------------------------------------------------
     def serve_start(self, hbm_budget_bytes=0, max_chunk_pairs=0, slots=0):

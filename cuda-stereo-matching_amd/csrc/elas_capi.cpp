@@ -250,6 +250,12 @@ int elas_hip_stage_times(const elas_ctx* c, float* ms, int32_t n) {
     return ELAS_OK;
 }
 
+int elas_hip_time_kernel(elas_ctx* c, int32_t kernel, int32_t reps, float* avg_ms) {
+    if (!c) return ELAS_ERR_INVALID_ARGUMENT;
+    ELAS_NOT_SERVING(c);
+    return c->time_kernel(kernel, reps, avg_ms);
+}
+
 int elas_hip_serve_start(elas_ctx* c, const elas_serve_config* cfg) {
     if (!c) return ELAS_ERR_INVALID_ARGUMENT;
     elas_serve_config d{0, 0, 0};

@@ -816,3 +816,48 @@ int elas_ctx::fetch(int stage, void* dst, size_t bytes, size_t* stage_bytes) {
     HIPCHECK(hipMemcpy(dst, src, std::min(bytes, sz), hipMemcpyDeviceToHost));
     return ELAS_OK;
 }
+
+int elas_ctx::time_kernel(int kernel, int reps, float* avg_ms) {
+    if (!last || last->pairs.empty()) return fail(ELAS_ERR_INVALID_ARGUMENT, "no process call yet");
+    if (reps < 1 || reps > 100000 || !avg_ms) return fail(ELAS_ERR_INVALID_ARGUMENT, "reps must be in [1, 100000]");
+    ElasSlot& S = *last;
+    HIPCHECK(hipSetDevice(device));
+    const int cap256 = std::max(1, num_cu) * 8, cap_dense = std::max(1, num_cu) * (2048 / ELAS_DENSE_THREADS);
+    auto g = [&](const ElasLaunch& L, int cap) { return std::min(L.total, cap); };
+    ElasLaunch L;
+    switch (kernel) {
+        case ELAS_K_DESCRIPTOR: L = launch_of(S, 0, S_DESC); break;
+        case ELAS_K_SUPPORT_LEFT:
+        case ELAS_K_SUPPORT_RIGHT: L = launch_of(S, 0, S_SUPPORT); break;
+        case ELAS_K_DENSE: L = launch_of(S, 1, S_DENSE); break;
+        case ELAS_K_LR: L = launch_of(S, 1, S_PIXELS); break;
+        default: return fail(ELAS_ERR_INVALID_ARGUMENT, "unknown kernel");
+    }
+    if (L.total == 0) return fail(ELAS_ERR_INVALID_ARGUMENT, "kernel has no work in the last call");
+    hipStream_t s1 = S.stream;
+    auto launch = [&]() {
+        switch (kernel) {
+            case ELAS_K_DESCRIPTOR: elas_launch_descriptor(L, g(L, cap256), s1); break;
+            case ELAS_K_SUPPORT_LEFT:
+                elas_launch_support(L, kp, false, g(L, cap256), support_lds(prm.candidate_stepsize, S.maxd, false), s1);
+                break;
+            case ELAS_K_SUPPORT_RIGHT:
+                elas_launch_support(L, kp, true, g(L, cap256), support_lds(prm.candidate_stepsize, S.maxd, true), s1);
+                break;
+            case ELAS_K_DENSE:
+                elas_launch_dense(L, S.kpl, g(L, cap_dense), dense_lds(prm.grid_size, S.maxd, S.cpw), s1);
+                break;
+            case ELAS_K_LR: elas_launch_lr(L, kp, g(L, cap256), s1); break;
+        }
+    };
+    launch();  // warm
+    HIPCHECK(hipEventRecord(S.ev[14], s1));
+    for (int i = 0; i < reps; i++) launch();
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipEventRecord(S.ev[15], s1));
+    HIPCHECK(hipEventSynchronize(S.ev[15]));
+    float ms = 0.f;
+    HIPCHECK(hipEventElapsedTime(&ms, S.ev[14], S.ev[15]));
+    *avg_ms = ms / reps;
+    return ELAS_OK;
+}

@@ -105,6 +105,7 @@ struct elas_ctx {
     // pipelined run of n pairs in chunks of up to `chunk` pairs over the slots
     int run_batch(const ElasPairSpec* specs, int n, int32_t* statuses, int chunk);
     int fetch(int stage, void* dst, size_t bytes, size_t* stage_bytes);
+    int time_kernel(int kernel, int reps, float* avg_ms);
     std::mutex err_m;  // the serving thread reports errors too
     int fail(int code, const std::string& msg) {
         std::lock_guard<std::mutex> g(err_m);

@@ -170,6 +170,17 @@ int elas_hip_shape_info(const elas_ctx* ctx, elas_shape_info* info);
 /* Per-stage device milliseconds of the last call (ELAS_T_COUNT floats). */
 int elas_hip_stage_times(const elas_ctx* ctx, float* ms, int32_t n);
 
+/* Kernel timing (no reference counterpart): re-launches one kernel of the
+ * last call `reps` times back to back on the context's stream and returns the
+ * average milliseconds per launch between HIP events.  Every kernel listed
+ * here recomputes the same outputs from the same inputs, so the results of
+ * the last call are unchanged. */
+typedef enum elas_kernel {
+    ELAS_K_DESCRIPTOR = 0, ELAS_K_SUPPORT_LEFT = 1, ELAS_K_SUPPORT_RIGHT = 2,
+    ELAS_K_DENSE = 3, ELAS_K_LR = 4, ELAS_K_COUNT = 5
+} elas_kernel;
+int elas_hip_time_kernel(elas_ctx* ctx, int32_t kernel, int32_t reps, float* avg_ms);
+
 /* ---- Serving (no reference counterpart: the reference has one blocking
  * process call).  A scheduler thread owned by the context admits queued
  * requests into idle pipeline slots as chunks, FIFO, as long as the chunk's
