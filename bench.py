@@ -323,8 +323,8 @@ def main():
                    "disp_max": dmax, "pairs_per_step": 1, "host_delaunay": "included"},
         "pairs_per_s": round(args.steps / wall, 2),
         "stages_ms": {k: round(v, 5) for k, v in stages.items()},
-        "gpu_stage_ms_per_pair": round(sum(stages[k] for k in ("descriptor", "support", "filter", "compact",
-                                                               "plane", "dense", "lr", "post")), 5),
+        "gpu_phase_ms_per_pair": round(stages["phase_a"] + stages["phase_b"], 5),
+        "launch_mode": "hip graphs per shape bucket (stages inside a graph read -1)",
         "roofline": roofline,
         "kernels": {k: {kk: (round(vv, 5) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                     for k, v in kernels.items()},

@@ -26,7 +26,7 @@ STAGES = ["desc0", "desc1", "can_left", "can_lr", "can_incon", "can_final", "sp0
           "grid1", "tri0", "tri1", "mean0", "mean1", "d1_raw", "d2_raw", "D1", "D2"]
 KERNELS = ["descriptor", "support_left", "support_right", "dense", "lr"]  # elas_kernel order
 TIMERS = ["descriptor", "support", "filter", "compact", "grid", "triangle", "plane", "dense", "lr",
-          "post", "total"]
+          "post", "total", "phase_a", "phase_b"]
 
 
 class ElasParams(ctypes.Structure):
@@ -83,6 +83,7 @@ SIGNATURES = {
     "elas_hip_process_batch": (ctypes.c_int, [_P, ctypes.c_int32, _P, _P, _P, _P, _P, _P]),
     "elas_hip_process_batch_device": (ctypes.c_int, [_P, ctypes.c_int32, _P, _P, _P, _P, _P, _P]),
     "elas_hip_set_pipeline": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32]),
+    "elas_hip_set_graphs": (ctypes.c_int, [_P, ctypes.c_int32]),
     "elas_hip_time_kernel": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(ctypes.c_float)]),
     "elas_hip_serve_start": (ctypes.c_int, [_P, _P]),
     "elas_hip_submit": (ctypes.c_int, [_P, _P, _P, _P, _P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
@@ -284,6 +285,11 @@ class Elas:
     def serve_stop(self):
         """Drain the queue and stop the scheduler thread."""
         self._check(self._lib.elas_hip_serve_stop(self._ctx))
+
+    def set_graphs(self, on=True):
+        """Replay phases A and B as HIP graphs per shape bucket (default), or
+        launch kernel by kernel with per-stage timing events."""
+        self._check(self._lib.elas_hip_set_graphs(self._ctx, 1 if on else 0))
 
     def set_pipeline(self, slots=3, chunk=0):
         """Batches are cut into chunks of `chunk` pairs (0: one chunk) that

@@ -127,6 +127,13 @@ int elas_hip_destroy(elas_ctx* c) {
     return ELAS_OK;
 }
 
+int elas_hip_set_graphs(elas_ctx* c, int32_t on) {
+    if (!c) return ELAS_ERR_INVALID_ARGUMENT;
+    ELAS_NOT_SERVING(c);
+    c->use_graphs = on != 0;
+    return ELAS_OK;
+}
+
 int elas_hip_set_pipeline(elas_ctx* c, int32_t slots, int32_t chunk) {
     if (!c) return ELAS_ERR_INVALID_ARGUMENT;
     ELAS_NOT_SERVING(c);
@@ -160,6 +167,7 @@ int elas_hip_set_params(elas_ctx* c, const elas_params* p) {
     if (st != ELAS_OK) return st;
     c->prm = *p;
     c->kp = kp;
+    c->param_gen++;
     return ELAS_OK;
 }
 
@@ -216,6 +224,7 @@ int elas_hip_set_debug(elas_ctx* c, int keep) {
     if (!c) return ELAS_ERR_INVALID_ARGUMENT;
     ELAS_NOT_SERVING(c);
     c->debug = keep != 0;
+    c->param_gen++;
     return ELAS_OK;
 }
 

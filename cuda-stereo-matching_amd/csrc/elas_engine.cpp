@@ -163,6 +163,7 @@ void elas_ctx::slot_free(ElasSlot& S) {
     (void)hipSetDevice(device);
     if (S.stream) (void)hipStreamSynchronize(S.stream);
     if (S.stream2) (void)hipStreamSynchronize(S.stream2);
+    slot_drop_graphs(S);
     if (S.ticket) elas_tri_release(S.ticket);
     S.ticket = nullptr;
     for (auto& e : S.ev)
@@ -208,6 +209,7 @@ int elas_ctx::slot_need(const ElasPairSpec* sp, int n, size_t* arena_need, size_
 int elas_ctx::slot_reserve(ElasSlot& S, size_t arena_need, size_t pin_need, int n) {
     HIPCHECK(hipSetDevice(device));
     if (arena_need > S.arena_bytes) {
+        slot_drop_graphs(S);
         if (S.arena) (void)hipFree(S.arena);
         S.arena = nullptr;
         S.arena_bytes = 0;
@@ -221,6 +223,7 @@ int elas_ctx::slot_reserve(ElasSlot& S, size_t arena_need, size_t pin_need, int 
     }
     const size_t tab = 2 * table_one(n);
     if (tab > S.table_bytes) {
+        slot_drop_graphs(S);
         if (S.table) (void)hipFree(S.table);
         S.table = nullptr;
         S.table_bytes = 0;
@@ -232,6 +235,7 @@ int elas_ctx::slot_reserve(ElasSlot& S, size_t arena_need, size_t pin_need, int 
         S.table_bytes = tab + tab / 4;
     }
     if (pin_need > S.pin_bytes) {
+        slot_drop_graphs(S);
         if (S.pin) (void)hipHostFree(S.pin);
         S.pin = nullptr;
         S.pin_bytes = 0;
@@ -390,31 +394,26 @@ ElasLaunch launch_of(const ElasSlot& S, int table, int st) {
 
 // ---- phase A: descriptors, support points, filters, compaction, grid -------
 
-int elas_ctx::slot_phase_a(ElasSlot& S) {
+// Phase A on the pipeline stream: table upload, descriptors .. compaction,
+// support point count back to pinned memory.  With `timed`, stage events are
+// recorded between the kernels (not possible inside a captured graph).
+int elas_ctx::enqueue_a(ElasSlot& S, bool timed) {
     const int n = S.n;
-    HIPCHECK(hipSetDevice(device));
-    build_prefix(prm, S);
-    memcpy(S.hpairs[0], S.pairs.data(), sizeof(ElasPair) * n);
-    memcpy(S.hprefix[0], S.prefix.data(), sizeof(int32_t) * S.prefix.size());
-    hipStream_t s1 = S.stream, s2 = S.stream2;
-    HIPCHECK(hipMemcpyAsync(S.dpairs[0], S.hpairs[0], sizeof(ElasPair) * n, hipMemcpyHostToDevice, s1));
-    HIPCHECK(hipMemcpyAsync(S.dprefix[0], S.hprefix[0], sizeof(int32_t) * S.prefix.size(), hipMemcpyHostToDevice, s1));
-    for (int i = 0; i < n; i++)
-        if (!S.specs[i].dev_in) {
-            HIPCHECK(hipMemcpyAsync((void*)S.pairs[i].img[0], S.specs[i].left, S.dd[i].npx, hipMemcpyHostToDevice, s1));
-            HIPCHECK(hipMemcpyAsync((void*)S.pairs[i].img[1], S.specs[i].right, S.dd[i].npx, hipMemcpyHostToDevice, s1));
-        }
+    hipStream_t s1 = S.stream;
     const int cap256 = std::max(1, num_cu) * 8;
     auto g = [&](const ElasLaunch& L, int cap) { return std::min(L.total, cap); };
-    HIPCHECK(hipEventRecord(S.ev[0], s1));
+    auto ev = [&](int i) { return timed ? hipEventRecord(S.ev[i], s1) : hipSuccess; };
+    HIPCHECK(hipMemcpyAsync(S.dpairs[0], S.hpairs[0], sizeof(ElasPair) * n, hipMemcpyHostToDevice, s1));
+    HIPCHECK(hipMemcpyAsync(S.dprefix[0], S.hprefix[0], sizeof(int32_t) * S.prefix.size(), hipMemcpyHostToDevice, s1));
+    HIPCHECK(ev(1));
     { ElasLaunch L = launch_of(S, 0, S_DESC); if (L.total) elas_launch_descriptor(L, g(L, cap256), s1); }
-    HIPCHECK(hipEventRecord(S.ev[1], s1));
+    HIPCHECK(ev(2));
     { ElasLaunch L = launch_of(S, 0, S_SUPPORT);
       if (L.total) {
           elas_launch_support(L, kp, false, g(L, cap256), support_lds(prm.candidate_stepsize, S.maxd, false), s1);
           elas_launch_support(L, kp, true, g(L, cap256), support_lds(prm.candidate_stepsize, S.maxd, true), s1);
       } }
-    HIPCHECK(hipEventRecord(S.ev[2], s1));
+    HIPCHECK(ev(3));
     { ElasLaunch L = launch_of(S, 0, S_INCON); if (L.total) elas_launch_incon(L, kp, g(L, cap256), s1); }
     {
         size_t lh = 4, lv = 4;
@@ -427,15 +426,43 @@ int elas_ctx::slot_phase_a(ElasSlot& S) {
         L = launch_of(S, 0, S_REDUN_V);
         if (L.total) elas_launch_redun(L, true, g(L, cap256), lv, s1);
     }
-    HIPCHECK(hipEventRecord(S.ev[3], s1));
+    HIPCHECK(ev(4));
     { ElasLaunch L = launch_of(S, 0, S_BLOCKS);
       if (L.total) { elas_launch_blkcount(L, g(L, cap256), s1); elas_launch_compact(L, g(L, cap256), s1); } }
     HIPCHECK(hipGetLastError());
-    HIPCHECK(hipEventRecord(S.ev[4], s1));
     HIPCHECK(hipMemcpyAsync(S.hnsp, S.dnsp, 4 * (size_t)n, hipMemcpyDeviceToHost, s1));
+    return ELAS_OK;
+}
+
+int elas_ctx::slot_phase_a(ElasSlot& S) {
+    const int n = S.n;
+    HIPCHECK(hipSetDevice(device));
+    build_prefix(prm, S);
+    memcpy(S.hpairs[0], S.pairs.data(), sizeof(ElasPair) * n);
+    memcpy(S.hprefix[0], S.prefix.data(), sizeof(int32_t) * S.prefix.size());
+    hipStream_t s1 = S.stream, s2 = S.stream2;
+    for (int i = 0; i < n; i++)
+        if (!S.specs[i].dev_in) {
+            HIPCHECK(hipMemcpyAsync((void*)S.pairs[i].img[0], S.specs[i].left, S.dd[i].npx, hipMemcpyHostToDevice, s1));
+            HIPCHECK(hipMemcpyAsync((void*)S.pairs[i].img[1], S.specs[i].right, S.dd[i].npx, hipMemcpyHostToDevice, s1));
+        }
+    const int cap256 = std::max(1, num_cu) * 8;
+    auto g = [&](const ElasLaunch& L, int cap) { return std::min(L.total, cap); };
+    HIPCHECK(hipEventRecord(S.ev[0], s1));
+    S.graphed = false;
+    if (use_graphs) {
+        hipGraphExec_t ex = nullptr;
+        int r = slot_graph(S, 0, &ex);
+        if (r != ELAS_OK) return r;
+        HIPCHECK(hipGraphLaunch(ex, s1));
+        S.graphed = true;
+    } else {
+        int r = enqueue_a(S, true);
+        if (r != ELAS_OK) return r;
+    }
     HIPCHECK(hipEventRecord(S.ev[5], s1));
     // disparity grid on the second stream, overlapping the host Triangle step
-    HIPCHECK(hipStreamWaitEvent(s2, S.ev[4], 0));
+    HIPCHECK(hipStreamWaitEvent(s2, S.ev[5], 0));
     HIPCHECK(hipEventRecord(S.ev[11], s2));
     { ElasLaunch L = launch_of(S, 0, S_GRID_FILL); if (L.total) elas_launch_grid_fill(L, g(L, cap256), s2); }
     { ElasLaunch L = launch_of(S, 0, S_BLOCKS); if (L.total) elas_launch_grid_scatter(L, g(L, cap256), s2); }
@@ -598,23 +625,57 @@ int elas_ctx::slot_phase_b(ElasSlot& S) {
     memcpy(S.hpairs[1], S.pairs.data(), sizeof(ElasPair) * n);
     memcpy(S.hprefix[1], S.prefix.data(), sizeof(int32_t) * S.prefix.size());
     hipStream_t s1 = S.stream;
-    HIPCHECK(hipMemcpyAsync(S.dpairs[1], S.hpairs[1], sizeof(ElasPair) * n, hipMemcpyHostToDevice, s1));
-    HIPCHECK(hipMemcpyAsync(S.dprefix[1], S.hprefix[1], sizeof(int32_t) * S.prefix.size(), hipMemcpyHostToDevice, s1));
     S.t_tri1 = clk::now();
     HIPCHECK(hipStreamWaitEvent(s1, S.ev[12], 0));
+    HIPCHECK(hipEventRecord(S.ev[6], s1));
+    if (use_graphs) {
+        hipGraphExec_t ex = nullptr;
+        int r = slot_graph(S, 1, &ex);
+        if (r != ELAS_OK) return r;
+        HIPCHECK(hipGraphLaunch(ex, s1));
+    } else {
+        int r = enqueue_b(S, true);
+        if (r != ELAS_OK) return r;
+    }
+    HIPCHECK(hipEventRecord(S.ev[10], s1));
+    for (int i = 0; i < n; i++)
+        if (!S.specs[i].dev_out) {
+            HIPCHECK(hipMemcpyAsync(S.specs[i].D1, S.pairs[i].dout[0], 4 * S.dd[i].npx, hipMemcpyDeviceToHost, s1));
+            HIPCHECK(hipMemcpyAsync(S.specs[i].D2, S.pairs[i].dout[1], 4 * S.dd[i].npx, hipMemcpyDeviceToHost, s1));
+        }
+    HIPCHECK(hipEventRecord(S.ev[13], s1));
+    S.phase = SLOT_B;
+    return ELAS_OK;
+}
 
+// Phase B on the pipeline stream: table upload, plane records, raster, dense
+// matching, LR check, post filters.  Every launch depends only on the chunk's
+// shapes and parameters (the plane kernel reads its triangle count from the
+// table), so the sequence can be captured once per shape bucket.
+int elas_ctx::enqueue_b(ElasSlot& S, bool timed) {
+    const int n = S.n;
+    hipStream_t s1 = S.stream;
+    auto ev = [&](int i) { return timed ? hipEventRecord(S.ev[i], s1) : hipSuccess; };
+    HIPCHECK(hipMemcpyAsync(S.dpairs[1], S.hpairs[1], sizeof(ElasPair) * n, hipMemcpyHostToDevice, s1));
+    HIPCHECK(hipMemcpyAsync(S.dprefix[1], S.hprefix[1], sizeof(int32_t) * S.prefix.size(), hipMemcpyHostToDevice, s1));
     const int cap256 = std::max(1, num_cu) * 8, cap_dense = std::max(1, num_cu) * (2048 / ELAS_DENSE_THREADS);
     auto g = [&](const ElasLaunch& L, int cap) { return std::min(L.total, cap); };
-    HIPCHECK(hipEventRecord(S.ev[6], s1));
-    { ElasLaunch L = launch_of(S, 1, S_PLANE); if (L.total) elas_launch_plane(L, g(L, 4 * cap256), s1); }
+    HIPCHECK(ev(15));
+    {
+        // grid from the largest possible triangle count (a shape property)
+        long maxtri = 0;
+        for (int i = 0; i < n; i++) maxtri += 2L * S.dd[i].tricap;
+        ElasLaunch L = launch_of(S, 1, S_PLANE);
+        elas_launch_plane(L, std::max(1, std::min(4 * cap256, cdiv(maxtri, 64))), s1);
+    }
     { ElasLaunch L = launch_of(S, 1, S_RASTER); if (L.total) elas_launch_raster(L, g(L, cap256), s1); }
-    HIPCHECK(hipEventRecord(S.ev[7], s1));
+    HIPCHECK(ev(7));
     { ElasLaunch L = launch_of(S, 1, S_DENSE);
       if (L.total) elas_launch_dense(L, S.kpl, g(L, cap_dense), dense_lds(prm.grid_size, S.maxd, S.cpw), s1); }
-    HIPCHECK(hipEventRecord(S.ev[8], s1));
+    HIPCHECK(ev(8));
     { ElasLaunch L = launch_of(S, 1, S_PIXELS); if (L.total) elas_launch_lr(L, kp, g(L, cap256), s1); }
     if (debug) { ElasLaunch L = launch_of(S, 1, S_PIXELS); if (L.total) elas_launch_mean(L, g(L, cap256), s1); }
-    HIPCHECK(hipEventRecord(S.ev[9], s1));
+    HIPCHECK(ev(9));
     // post-processing on D1 (and D2): speckle, gap interpolation, adaptive mean, median
     {
         const ElasLaunch Lt = launch_of(S, 1, S_POST_TILES), Lp = launch_of(S, 1, S_PIXELS);
@@ -630,16 +691,73 @@ int elas_ctx::slot_phase_b(ElasSlot& S) {
         if (kp.median && Lt.total) { elas_launch_sep(Lt, kp, true, in_tmp, cap256, s1); in_tmp ^= 1; }
         if (in_tmp && Lp.total) elas_launch_post_copy(Lp, kp, cap256, s1);
     }
-    HIPCHECK(hipEventRecord(S.ev[10], s1));
     HIPCHECK(hipGetLastError());
-    for (int i = 0; i < n; i++)
-        if (!S.specs[i].dev_out) {
-            HIPCHECK(hipMemcpyAsync(S.specs[i].D1, S.pairs[i].dout[0], 4 * S.dd[i].npx, hipMemcpyDeviceToHost, s1));
-            HIPCHECK(hipMemcpyAsync(S.specs[i].D2, S.pairs[i].dout[1], 4 * S.dd[i].npx, hipMemcpyDeviceToHost, s1));
-        }
-    HIPCHECK(hipEventRecord(S.ev[13], s1));
-    S.phase = SLOT_B;
     return ELAS_OK;
+}
+
+// The executable graph of phase `which` (0: A, 1: B) for the chunk's shape
+// bucket, captured from enqueue_a/enqueue_b on first use.  The key holds
+// everything a launch depends on: pair shapes, where buffers live, the
+// parameter generation and the debug flag; the tables and pinned buffers
+// the graph reads are the slot's own, at fixed addresses.
+int elas_ctx::slot_graph(ElasSlot& S, int which, hipGraphExec_t* out) {
+    std::vector<int64_t> key;
+    key.reserve(6 + 5 * (size_t)S.n);
+    key.push_back(S.n);
+    key.push_back((int64_t)param_gen);
+    key.push_back(debug ? 1 : 0);
+    key.push_back((int64_t)(uintptr_t)S.arena);
+    key.push_back((int64_t)(uintptr_t)S.table);
+    for (int i = 0; i < S.n; i++) {
+        key.push_back(S.specs[i].W);
+        key.push_back(S.specs[i].H);
+        key.push_back(S.specs[i].dmax);
+        key.push_back((S.specs[i].dev_in ? 1 : 0) | (S.specs[i].dev_out ? 2 : 0));
+    }
+    ElasGraphEntry* e = nullptr;
+    for (auto& x : S.graphs)
+        if (x.key == key) e = &x;
+    if (!e) {
+        if (S.graphs.size() >= ELAS_GRAPHS_PER_SLOT) {
+            auto it = std::min_element(S.graphs.begin(), S.graphs.end(),
+                                       [](const ElasGraphEntry& a, const ElasGraphEntry& b) { return a.use < b.use; });
+            for (auto& x : it->exec)
+                if (x) (void)hipGraphExecDestroy(x);
+            S.graphs.erase(it);
+        }
+        S.graphs.emplace_back();
+        e = &S.graphs.back();
+        e->key = key;
+    }
+    e->use = ++S.graph_clock;
+    if (!e->exec[which]) {
+        hipStream_t s1 = S.stream;
+        HIPCHECK(hipStreamBeginCapture(s1, hipStreamCaptureModeThreadLocal));
+        int r = which == 0 ? enqueue_a(S, false) : enqueue_b(S, false);
+        hipGraph_t graph = nullptr;
+        const hipError_t ec = hipStreamEndCapture(s1, &graph);
+        if (r != ELAS_OK) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return r;
+        }
+        if (ec != hipSuccess || !graph) return fail(ELAS_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(ec));
+        const hipError_t ei = hipGraphInstantiate(&e->exec[which], graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (ei != hipSuccess) {
+            e->exec[which] = nullptr;
+            return fail(ELAS_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(ei));
+        }
+        S.graph_captures++;
+    }
+    *out = e->exec[which];
+    return ELAS_OK;
+}
+
+void elas_ctx::slot_drop_graphs(ElasSlot& S) {
+    for (auto& x : S.graphs)
+        for (auto& g : x.exec)
+            if (g) (void)hipGraphExecDestroy(g);
+    S.graphs.clear();
 }
 
 bool elas_ctx::slot_b_ready(ElasSlot& S) { return hipEventQuery(S.ev[13]) == hipSuccess; }
@@ -653,17 +771,22 @@ int elas_ctx::slot_finish(ElasSlot& S, int32_t* statuses) {
         (void)hipEventElapsedTime(&ms, S.ev[a], S.ev[b]);
         return ms;
     };
-    S.times[ELAS_T_DESCRIPTOR] = el(0, 1);
-    S.times[ELAS_T_SUPPORT] = el(1, 2);
-    S.times[ELAS_T_FILTER] = el(2, 3);
-    S.times[ELAS_T_COMPACT] = el(3, 4);
+    // per-stage times exist only for eagerly launched phases (a captured
+    // graph has no events between its kernels): -1 otherwise
+    const bool st = !S.graphed;
+    S.times[ELAS_T_DESCRIPTOR] = st ? el(1, 2) : -1.0f;
+    S.times[ELAS_T_SUPPORT] = st ? el(2, 3) : -1.0f;
+    S.times[ELAS_T_FILTER] = st ? el(3, 4) : -1.0f;
+    S.times[ELAS_T_COMPACT] = st ? el(4, 5) : -1.0f;
     S.times[ELAS_T_GRID] = el(11, 12);
     S.times[ELAS_T_TRIANGLE] = std::chrono::duration<float, std::milli>(S.t_tri1 - S.t_tri0).count();
-    S.times[ELAS_T_PLANE] = el(6, 7);
-    S.times[ELAS_T_DENSE] = el(7, 8);
-    S.times[ELAS_T_LR] = el(8, 9);
-    S.times[ELAS_T_POST] = el(9, 10);
+    S.times[ELAS_T_PLANE] = st ? el(15, 7) : -1.0f;
+    S.times[ELAS_T_DENSE] = st ? el(7, 8) : -1.0f;
+    S.times[ELAS_T_LR] = st ? el(8, 9) : -1.0f;
+    S.times[ELAS_T_POST] = st ? el(9, 10) : -1.0f;
     S.times[ELAS_T_TOTAL] = std::chrono::duration<float, std::milli>(S.t_end - S.t_start).count();
+    S.times[ELAS_T_PHASE_A] = el(0, 5);
+    S.times[ELAS_T_PHASE_B] = el(6, 10);
     if (statuses)
         for (int i = 0; i < S.n; i++) statuses[i] = S.status[i];
     S.phase = SLOT_IDLE;
@@ -851,13 +974,13 @@ int elas_ctx::time_kernel(int kernel, int reps, float* avg_ms) {
         }
     };
     launch();  // warm
-    HIPCHECK(hipEventRecord(S.ev[14], s1));
+    HIPCHECK(hipEventRecord(S.ev[16], s1));
     for (int i = 0; i < reps; i++) launch();
     HIPCHECK(hipGetLastError());
-    HIPCHECK(hipEventRecord(S.ev[15], s1));
-    HIPCHECK(hipEventSynchronize(S.ev[15]));
+    HIPCHECK(hipEventRecord(S.ev[17], s1));
+    HIPCHECK(hipEventSynchronize(S.ev[17]));
     float ms = 0.f;
-    HIPCHECK(hipEventElapsedTime(&ms, S.ev[14], S.ev[15]));
+    HIPCHECK(hipEventElapsedTime(&ms, S.ev[16], S.ev[17]));
     *avg_ms = ms / reps;
     return ELAS_OK;
 }

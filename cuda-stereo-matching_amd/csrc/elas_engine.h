@@ -40,10 +40,19 @@ struct ElasDims {
 
 enum ElasSlotPhase { SLOT_IDLE = 0, SLOT_A = 1, SLOT_TRI = 2, SLOT_B = 3 };
 
+// Captured phases of one shape bucket (key: pair shapes, buffer placement,
+// parameter generation, debug flag, arena/table addresses).
+#define ELAS_GRAPHS_PER_SLOT 8
+struct ElasGraphEntry {
+    std::vector<int64_t> key;
+    hipGraphExec_t exec[2] = {nullptr, nullptr};  // phase A, phase B
+    uint64_t use = 0;
+};
+
 struct ElasSlot {
     hipStream_t stream = nullptr;   // phases A and B
     hipStream_t stream2 = nullptr;  // disparity grid, overlapped with the host step
-    hipEvent_t ev[16] = {};
+    hipEvent_t ev[18] = {};
     // device memory
     void* arena = nullptr;
     size_t arena_bytes = 0;
@@ -83,6 +92,10 @@ struct ElasSlot {
     ElasTriTicket* ticket = nullptr;
     std::chrono::steady_clock::time_point t_start, t_tri0, t_tri1, t_end;
     float times[ELAS_T_COUNT] = {};
+    // graphs of this slot's shape buckets
+    std::vector<ElasGraphEntry> graphs;
+    uint64_t graph_clock = 0, graph_captures = 0;
+    bool graphed = false;  // the chunk in flight runs its phases as graphs
 };
 
 struct ElasServer;
@@ -95,6 +108,8 @@ struct elas_ctx {
     bool debug = false;
     int tri_workers = ELAS_TRI_INSTANCES;
     int chunk = 0;  // pairs per pipeline chunk of a batch (0: whole batch)
+    bool use_graphs = true;   // phases A and B replayed as graphs per shape bucket
+    uint64_t param_gen = 0;   // bumped when parameters or the debug flag change
     std::string err;
     std::vector<ElasSlot*> slots;
     ElasSlot* last = nullptr;  // slot of the last finished call (fetch, shape info, times)
@@ -127,6 +142,10 @@ struct elas_ctx {
     bool slot_b_ready(ElasSlot& S);
     int slot_finish(ElasSlot& S, int32_t* statuses);
     void slot_abort(ElasSlot& S);
+    int enqueue_a(ElasSlot& S, bool timed);
+    int enqueue_b(ElasSlot& S, bool timed);
+    int slot_graph(ElasSlot& S, int which, hipGraphExec_t* out);
+    void slot_drop_graphs(ElasSlot& S);
 };
 
 // Serving loop (elas_serve.cpp).

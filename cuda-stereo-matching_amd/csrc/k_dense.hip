@@ -48,7 +48,10 @@ __device__ bool elas_solve3(double A[3][3], double B[3]) {
 // KerComputeDisparity.cu:63-66,95-99).  A corner's d is sp0.u - sp1.u (exact
 // in float), which saves a dependent load of the lattice.
 __global__ __launch_bounds__(64) void k_plane(ElasLaunch L) {
-    for (int t = blockIdx.x; t < L.total; t += gridDim.x) {
+    // the triangle count is data: read the total from the table (a captured
+    // launch keeps the grid of its shape bucket)
+    const int total = L.prefix[L.npairs];
+    for (int t = blockIdx.x; t < total; t += gridDim.x) {
         const int p = elas_find_pair(L.prefix, L.npairs, t);
         const auto& P = L.pairs[p];
         const int idx = (t - L.prefix[p]) * 64 + threadIdx.x;

@@ -103,11 +103,14 @@ typedef struct elas_shape_info {
     int32_t status;               /* elas_status of that call               */
 } elas_shape_info;
 
-/* Per-stage device time of the last call in milliseconds (HIP events). */
+/* Per-stage device time of the last call in milliseconds (HIP events).  The
+ * stages inside phases A and B read -1 when the phases ran as graphs
+ * (elas_hip_set_graphs); PHASE_A / PHASE_B cover them in either mode. */
 typedef enum elas_timer {
     ELAS_T_DESCRIPTOR = 0, ELAS_T_SUPPORT = 1, ELAS_T_FILTER = 2, ELAS_T_COMPACT = 3,
     ELAS_T_GRID = 4, ELAS_T_TRIANGLE = 5, ELAS_T_PLANE = 6, ELAS_T_DENSE = 7,
-    ELAS_T_LR = 8, ELAS_T_POST = 9, ELAS_T_TOTAL = 10, ELAS_T_COUNT = 11
+    ELAS_T_LR = 8, ELAS_T_POST = 9, ELAS_T_TOTAL = 10, ELAS_T_PHASE_A = 11,
+    ELAS_T_PHASE_B = 12, ELAS_T_COUNT = 13
 } elas_timer;
 
 typedef struct elas_ctx elas_ctx;
@@ -152,6 +155,12 @@ int elas_hip_process_batch(elas_ctx* ctx, int32_t n, const uint8_t* const* left,
 int elas_hip_process_batch_device(elas_ctx* ctx, int32_t n, const uint8_t* const* d_left,
                                   const uint8_t* const* d_right, float* const* d_D1,
                                   float* const* d_D2, const int32_t* dims, int32_t* statuses);
+
+/* Launch mode (no reference counterpart): phases A (descriptor .. support
+ * point count) and B (plane .. post-processing) are captured as HIP graphs
+ * once per shape bucket and replayed (default), or launched kernel by kernel
+ * with per-stage timing events (on = 0).  Results are identical. */
+int elas_hip_set_graphs(elas_ctx* ctx, int32_t on);
 
 /* Pipelining of the batched forms: a batch is cut into chunks of up to
  * `chunk` pairs (0: the whole batch is one chunk) that move through `slots`

@@ -134,6 +134,31 @@ def test_device_batch_equals_plain(pkg, synth):
         assert np.array_equal(D1[i].cpu().numpy(), a1) and np.array_equal(D2[i].cpu().numpy(), a2)
 
 
+def test_graph_replays_equal_eager(pkg, synth):
+    """Phases captured as graphs per shape bucket and replayed with new data,
+    interleaved across buckets and with eager runs, equal the eager launches."""
+    shapes = [(160, 120), (200, 90), (97, 131)]
+    pairs_ = [synth.make_pair(W, H, 63, regions=4, seed=600 + i)[:2] for i, (W, H) in enumerate(shapes * 3)]
+    eager = pkg.Elas(pkg.default_params(disp_max=63))
+    eager.set_graphs(False)
+    graph = pkg.Elas(pkg.default_params(disp_max=63))
+    for rep in range(3):
+        for i, (l, r) in enumerate(pairs_):
+            print("graph replay", rep, i, l.shape, flush=True)
+            e1, e2 = eager.process(l, r)
+            g1, g2 = graph.process(l, r)
+            assert graph.status == eager.status
+            assert np.array_equal(e1, g1) and np.array_equal(e2, g2)
+    # a parameter change re-captures
+    kw = dict(disp_max=63, support_threshold=0.9, lr_threshold=1)
+    eager.set_params(pkg.default_params(**kw))
+    graph.set_params(pkg.default_params(**kw))
+    for l, r in pairs_[:3]:
+        e1, e2 = eager.process(l, r)
+        g1, g2 = graph.process(l, r)
+        assert np.array_equal(e1, g1) and np.array_equal(e2, g2)
+
+
 def test_textureless_pair_reports_too_few_points(pkg, oracle):
     img = np.full((120, 160), 77, np.uint8)
     el = pkg.Elas(pkg.default_params(disp_max=31))
