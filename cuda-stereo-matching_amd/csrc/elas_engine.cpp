@@ -494,6 +494,7 @@ int elas_ctx::slot_start_tri(ElasSlot& S) {
         }
     }
     HIPCHECK(hipStreamSynchronize(S.stream));
+    S.t_sp = clk::now();
     S.tri_out.assign(2 * (size_t)n, {});
     S.toff.assign(2 * (size_t)n, {});
     S.tlist.assign(2 * (size_t)n, {});
@@ -515,6 +516,7 @@ int elas_ctx::slot_start_tri(ElasSlot& S) {
             S.jobs.push_back(j);
         }
     }
+    S.t_sub = clk::now();
     if (tri_workers <= 1) {
         elas_triangulate_jobs(S.jobs.data(), (int)S.jobs.size(), 1);
         S.ticket = elas_tri_submit(nullptr, 0);
@@ -534,6 +536,7 @@ int elas_ctx::slot_phase_b(ElasSlot& S) {
     HIPCHECK(hipSetDevice(device));
     elas_tri_release(S.ticket);  // waits for the meshes
     S.ticket = nullptr;
+    S.t_done = clk::now();
     for (int i = 0; i < n; i++) {
         int nt[2] = {0, 0};
         if (S.status[i] == ELAS_OK) {
@@ -787,6 +790,16 @@ int elas_ctx::slot_finish(ElasSlot& S, int32_t* statuses) {
     S.times[ELAS_T_TOTAL] = std::chrono::duration<float, std::milli>(S.t_end - S.t_start).count();
     S.times[ELAS_T_PHASE_A] = el(0, 5);
     S.times[ELAS_T_PHASE_B] = el(6, 10);
+    auto msd = [](clk::time_point a, clk::time_point b) { return std::chrono::duration<float, std::milli>(b - a).count(); };
+    S.times[ELAS_T_SP_DOWNLOAD] = msd(S.t_tri0, S.t_sp);
+    S.times[ELAS_T_TRI_WALL] = msd(S.t_sub, S.t_done);
+    float tj = 0.f, bj = 0.f;
+    for (const ElasTriJob& j : S.jobs) {
+        tj = std::max(tj, (float)((j.t_tri - j.t_begin) * 1e3));
+        bj = std::max(bj, (float)((j.t_end - j.t_tri) * 1e3));
+    }
+    S.times[ELAS_T_TRI_JOB] = tj;
+    S.times[ELAS_T_BIN_JOB] = bj;
     if (statuses)
         for (int i = 0; i < S.n; i++) statuses[i] = S.status[i];
     S.phase = SLOT_IDLE;

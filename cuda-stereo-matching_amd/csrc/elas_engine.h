@@ -90,7 +90,7 @@ struct ElasSlot {
     std::vector<ElasTriJob> jobs;
     std::vector<int> job_of;
     ElasTriTicket* ticket = nullptr;
-    std::chrono::steady_clock::time_point t_start, t_tri0, t_tri1, t_end;
+    std::chrono::steady_clock::time_point t_start, t_tri0, t_sp, t_sub, t_done, t_tri1, t_end;
     float times[ELAS_T_COUNT] = {};
     // graphs of this slot's shape buckets
     std::vector<ElasGraphEntry> graphs;

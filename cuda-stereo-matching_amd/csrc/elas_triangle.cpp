@@ -9,6 +9,9 @@
 // only copy K.  Meshes are identical to a single serial Triangle because every
 // call starts from the same reset state.
 #include <algorithm>
+#include <atomic>
+#include <immintrin.h>
+#include <chrono>
 #include <cmath>
 #include <condition_variable>
 #include <cstdlib>
@@ -103,19 +106,32 @@ const TriFn kTri[ELAS_TRI_INSTANCES] = {
     run_elas_tri6, run_elas_tri7, run_elas_tri8, run_elas_tri9, run_elas_tri10, run_elas_tri11,
     run_elas_tri12, run_elas_tri13, run_elas_tri14, run_elas_tri15};
 
-int run_instance(int k, const ElasTriJob& j) {
+double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+int run_instance(int k, ElasTriJob& j) {
+    j.t_begin = now_s();
     if (j.n < 3) {  // Triangle would exit(1) (src/triangle.cpp:7652-7654)
         j.out->clear();
+        j.t_tri = now_s();
         if (j.tile_off) elas_bin_triangles(j.pts, nullptr, 0, j.W, j.H, *j.tile_off, *j.tile_list);
+        j.t_end = now_s();
         return -1;
     }
     const int nt = kTri[k](j.pts, j.n, *j.out);
+    j.t_tri = now_s();
     if (j.tile_off) elas_bin_triangles(j.pts, j.out->data(), nt < 0 ? 0 : nt, j.W, j.H, *j.tile_off, *j.tile_list);
+    j.t_end = now_s();
     return nt;
 }
 
 // Worker K owns Triangle copy K.  Jobs of one submit() are spread over the
-// workers; submit() returns when all of them are done.
+// workers; submit() returns when all of them are done.  A worker that just
+// finished a job polls the queue for up to ELAS_TRI_SPIN_S before sleeping: a
+// core that went idle between calls runs Triangle up to ~20% slower for its
+// first few hundred microseconds, which is the whole job at these sizes.
+#define ELAS_TRI_SPIN_S 0.003
 class TriPool {
 public:
     static TriPool& get() {
@@ -145,8 +161,9 @@ public:
             std::lock_guard<std::mutex> g(m_);
             ensure_workers();
             for (int i = 0; i < n; i++) q_.push_back({&jobs[i], b});
+            pending_.fetch_add(n, std::memory_order_release);
         }
-        cv_.notify_all();
+        for (int i = 0; i < n; i++) cv_.notify_one();
     }
 
 private:
@@ -159,21 +176,26 @@ private:
         for (int k = 1; k < ELAS_TRI_INSTANCES; k++) threads_.emplace_back([this, k] { loop(k); });
     }
     void loop(int k) {
+        double last = 0.0;  // end of this worker's last job
         for (;;) {
             Item it;
             {
+                while (pending_.load(std::memory_order_acquire) == 0 && now_s() - last < ELAS_TRI_SPIN_S) _mm_pause();
                 std::unique_lock<std::mutex> g(m_);
                 cv_.wait(g, [&] { return !q_.empty(); });
                 it = q_.front();
                 q_.pop_front();
+                pending_.fetch_sub(1, std::memory_order_relaxed);
             }
             it.job->result = run_instance(k, *it.job);
             {
                 std::lock_guard<std::mutex> g(it.batch->m);
                 if (--it.batch->remaining == 0) it.batch->cv.notify_all();
             }
+            last = now_s();
         }
     }
+    std::atomic<int> pending_{0};  // queued jobs, read without the lock while polling
     std::mutex m_, serial_;
     std::condition_variable cv_;
     std::deque<Item> q_;
@@ -195,7 +217,8 @@ void elas_bin_triangles(const float* pts, const int32_t* tri, int ntri, int W, i
     const int T = ELAS_RASTER_TILE;
     const int ntx = (W + T - 1) / T, nty = (H + T - 1) / T;
     off.assign((size_t)ntx * nty + 1, 0);
-    std::vector<int32_t> box(4 * (size_t)ntri);
+    thread_local std::vector<int32_t> box;  // per worker, reused across calls
+    box.resize(4 * (size_t)ntri);
     for (int t = 0; t < ntri; t++) {
         int cu[3], cv[3];
         for (int k = 0; k < 3; k++) {
@@ -205,12 +228,25 @@ void elas_bin_triangles(const float* pts, const int32_t* tri, int ntri, int W, i
         }
         const int umin = std::min(cu[0], std::min(cu[1], cu[2])), umax = std::max(cu[0], std::max(cu[1], cu[2]));
         const int vmin = std::min(cv[0], std::min(cv[1], cv[2])), vmax = std::max(cv[0], std::max(cv[1], cv[2]));
-        double smax = 0.0;
+        // margin >= 2 + ceil(x), x = 8 (3 smax U + 3 V) / 2^24, evaluated in
+        // float: x is at most a few units here, so 1e-3 covers the rounding of
+        // x and floor(x + 1e-3) + 3 >= ceil(x) + 2; large x takes doubles
+        float smax = 0.0f;
         for (int a = 0; a < 3; a++)
             for (int b = a + 1; b < 3; b++)
-                if (cu[a] != cu[b]) smax = std::max(smax, std::fabs((double)(cv[a] - cv[b]) / (double)(cu[a] - cu[b])));
-        const double U = std::max(std::abs(umin), std::abs(umax)) + 1.0, V = std::max(std::abs(vmin), std::abs(vmax)) + 1.0;
-        const int margin = 2 + (int)std::ceil(8.0 * (3.0 * smax * U + 3.0 * V) / 16777216.0);
+                if (cu[a] != cu[b]) smax = std::max(smax, std::fabs((float)(cv[a] - cv[b]) / (float)(cu[a] - cu[b])));
+        const int Ui = std::max(std::abs(umin), std::abs(umax)) + 1, Vi = std::max(std::abs(vmin), std::abs(vmax)) + 1;
+        const float xf = 8.0f * (3.0f * smax * (float)Ui + 3.0f * (float)Vi) * (1.0f / 16777216.0f);
+        int margin;
+        if (xf < 100.0f) {
+            margin = 3 + (int)(xf + 1e-3f);
+        } else {
+            double sd = 0.0;
+            for (int a = 0; a < 3; a++)
+                for (int b = a + 1; b < 3; b++)
+                    if (cu[a] != cu[b]) sd = std::max(sd, std::fabs((double)(cv[a] - cv[b]) / (double)(cu[a] - cu[b])));
+            margin = 2 + (int)std::ceil(8.0 * (3.0 * sd * Ui + 3.0 * Vi) / 16777216.0);
+        }
         const int u0 = std::max(umin, 0), u1 = std::min(umax, W);                      // [u0, u1)
         const int v0 = std::max(vmin - margin, 0), v1 = std::min(vmax + margin + 1, H);  // [v0, v1)
         int* b = &box[4 * (size_t)t];
@@ -220,13 +256,16 @@ void elas_bin_triangles(const float* pts, const int32_t* tri, int ntri, int W, i
             for (int tx = b[0]; tx <= b[1]; tx++) off[(size_t)ty * ntx + tx + 1]++;
     }
     for (size_t i = 1; i < off.size(); i++) off[i] += off[i - 1];
-    list.assign((size_t)off.back(), 0);
-    std::vector<int32_t> cur(off.begin(), off.end() - 1);
+    list.resize((size_t)off.back());
+    // fill with running cursors kept in off[1..] (restored afterwards)
     for (int t = 0; t < ntri; t++) {
         const int* b = &box[4 * (size_t)t];
         for (int ty = b[2]; ty <= b[3]; ty++)
-            for (int tx = b[0]; tx <= b[1]; tx++) list[(size_t)cur[(size_t)ty * ntx + tx]++] = t;
+            for (int tx = b[0]; tx <= b[1]; tx++) list[(size_t)off[(size_t)ty * ntx + tx]++] = t;
     }
+    // off[i] now holds the end of tile i = the start of tile i + 1
+    for (size_t i = off.size() - 1; i > 0; i--) off[i] = off[i - 1];
+    off[0] = 0;
 }
 
 void elas_triangulate_jobs(ElasTriJob* jobs, int n, int max_workers) {

@@ -18,6 +18,8 @@ struct ElasTriJob {
     int W, H;
     std::vector<int32_t>* tile_off;   // [ntiles + 1]
     std::vector<int32_t>* tile_list;  // [tile_off[ntiles]]
+    // filled by the worker: seconds (steady clock) around Triangle and binning
+    double t_begin = 0, t_tri = 0, t_end = 0;
 };
 
 // Tile binning of triangles with corners pts (pixel coordinates).

@@ -110,7 +110,11 @@ typedef enum elas_timer {
     ELAS_T_DESCRIPTOR = 0, ELAS_T_SUPPORT = 1, ELAS_T_FILTER = 2, ELAS_T_COMPACT = 3,
     ELAS_T_GRID = 4, ELAS_T_TRIANGLE = 5, ELAS_T_PLANE = 6, ELAS_T_DENSE = 7,
     ELAS_T_LR = 8, ELAS_T_POST = 9, ELAS_T_TOTAL = 10, ELAS_T_PHASE_A = 11,
-    ELAS_T_PHASE_B = 12, ELAS_T_COUNT = 13
+    ELAS_T_PHASE_B = 12,
+    /* host step (wall clock): support points down, Triangle jobs from submit
+     * to the last done, the longest job's Triangle and binning parts */
+    ELAS_T_SP_DOWNLOAD = 13, ELAS_T_TRI_WALL = 14, ELAS_T_TRI_JOB = 15, ELAS_T_BIN_JOB = 16,
+    ELAS_T_COUNT = 17
 } elas_timer;
 
 typedef struct elas_ctx elas_ctx;
