@@ -342,8 +342,10 @@ class Elas:
         self._check(st)
         return [(d1[i], d2[i], int(stat[i])) for i in range(n)]
 
-    def set_debug(self, on=True):
-        self._check(self._lib.elas_hip_set_debug(self._ctx, 1 if on else 0))
+    def set_debug(self, on=True, bin_fallback=False):
+        """Keep every intermediate for fetch(); bin_fallback forces the host
+        raster-binning path (a test of the GPU bins' overflow fallback)."""
+        self._check(self._lib.elas_hip_set_debug(self._ctx, (1 if on else 0) | (2 if bin_fallback else 0)))
 
     def shape_info(self):
         info = ShapeInfo()

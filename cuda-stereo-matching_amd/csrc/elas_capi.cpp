@@ -223,7 +223,8 @@ int elas_hip_process_batch_device(elas_ctx* c, int32_t n, const uint8_t* const* 
 int elas_hip_set_debug(elas_ctx* c, int keep) {
     if (!c) return ELAS_ERR_INVALID_ARGUMENT;
     ELAS_NOT_SERVING(c);
-    c->debug = keep != 0;
+    c->debug = (keep & 1) != 0;
+    c->bin_test = (keep & 2) != 0;
     c->param_gen++;
     return ELAS_OK;
 }

@@ -58,7 +58,14 @@ struct ElasPair {
     float* trirec[2];        // [ntri][ELAS_TRIREC]
     int32_t ntx, nty;        // raster tiles (ELAS_RASTER_TILE px) across / down
     const int32_t* toff[2];  // [ntx*nty+1] tile -> range of tlist
-    const int32_t* tlist[2]; // triangles meeting each tile, ascending
+    const int32_t* tlist[2]; // triangles meeting each tile (any order)
+    // raster bins built on the GPU (toff/tlist point here when gpu_bins)
+    int32_t* bcnt[2];        // [ntx*nty+1] triangles per tile, then list offsets
+    int32_t* bcur[2];        // [ntx*nty] fill cursors
+    int32_t* blist[2];       // [bcap] tile lists
+    int32_t bcap;            // list capacity per image
+    int32_t gpu_bins;        // 1: bins built on the GPU; 0: uploaded from the host
+    int32_t* bovf;           // set when the lists outgrow bcap (the host then bins)
     int32_t* prior[2];       // [H][W] (int)plane(u,v) of the covering triangle
                              //        with the highest index, -1 if uncovered
     float* mean[2];          // [H][W] (float)prior: the reference's map (debug only)
@@ -105,5 +112,5 @@ struct ElasLaunch {
 enum ElasStageId {
     S_DESC = 0, S_SUPPORT, S_INCON, S_REDUN_H, S_REDUN_V, S_BLOCKS, S_GRID_CELLS,
     S_PLANE, S_RASTER, S_DENSE, S_PIXELS, S_GRID_FILL, S_POST_TILES, S_POST_ROWS, S_POST_COLS,
-    S_NUM_STAGES
+    S_BIN_CLEAR, S_BIN_SCAN, S_NUM_STAGES
 };

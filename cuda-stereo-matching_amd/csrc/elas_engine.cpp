@@ -56,8 +56,19 @@ int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
 
 bool post_scratch(const elas_params& p) { return p.speckle_size > 0 || p.filter_adaptive_mean || p.filter_median; }
 
+size_t bin_tiles(int W, int H) {
+    return (size_t)((W + ELAS_RASTER_T - 1) / ELAS_RASTER_T) * ((H + ELAS_RASTER_T - 1) / ELAS_RASTER_T);
+}
+
+// capacity of one image's GPU raster bins: room for every triangle in 4 tiles
+// plus 4 per tile; a mesh that needs more is binned on the host instead.
+// `test` shrinks it to 1 so that tests exercise that fallback.
+int32_t bin_cap(const ElasDims& d, size_t ntiles, bool test) {
+    return test ? 1 : (int32_t)(4 * (size_t)d.tricap + 4 * ntiles);
+}
+
 // device bytes of one pair in a slot arena
-size_t pair_arena(const ElasPairSpec& s, const ElasDims& d, bool debug, bool post) {
+size_t pair_arena(const ElasPairSpec& s, const ElasDims& d, bool debug, bool post, bool bin_test) {
     size_t b = 0;
     if (!s.dev_in) b += 2 * al(d.npx);
     b += 2 * al(16 * d.npx);
@@ -68,6 +79,8 @@ size_t pair_arena(const ElasPairSpec& s, const ElasDims& d, bool debug, bool pos
     b += 2 * al(4 * d.npx) + 2 * al(4 * d.npx);  // prior, draw
     if (!s.dev_out) b += 2 * al(4 * d.npx);
     if (post) b += 2 * al(8 * d.npx);  // post-filter scratch
+    const size_t nt = bin_tiles(s.W, s.H);
+    b += 2 * al(4 * (nt + 1)) + 2 * al(4 * nt) + 2 * al(4 * (size_t)bin_cap(d, nt, bin_test));
     return b;
 }
 
@@ -94,7 +107,7 @@ ElasDims elas_dims(const elas_params& p, int W, int H, int dmax) {
 }
 
 size_t elas_pair_device_bytes(const elas_params& p, const ElasPairSpec& s, bool debug) {
-    return pair_arena(s, elas_dims(p, s.W, s.H, s.dmax), debug, post_scratch(p));
+    return pair_arena(s, elas_dims(p, s.W, s.H, s.dmax), debug, post_scratch(p), false);
 }
 
 int elas_make_kparams(const elas_params* p, ElasKParams* kp, std::string* err) {
@@ -196,7 +209,7 @@ int elas_ctx::slot_need(const ElasPairSpec* sp, int n, size_t* arena_need, size_
         if (prm.ipol_gap_width > 0 && std::max(s.W, s.H) > ELAS_GAP_MAX_LINE)
             return fail(ELAS_ERR_UNSUPPORTED, "gap interpolation supports lines up to 15872 pixels");
         const ElasDims d = elas_dims(prm, s.W, s.H, s.dmax);
-        a += pair_arena(s, d, debug, post_scratch(prm));
+        a += pair_arena(s, d, debug, post_scratch(prm), bin_test);
         psp += 2 * 8 * d.ncan;
         ptri += 2 * 12 * (size_t)d.tricap;
     }
@@ -269,7 +282,7 @@ int elas_ctx::slot_prepare(ElasSlot& S, const ElasPairSpec* sp, int n) {
     for (int i = 0; i < n; i++) {
         S.dd[i] = elas_dims(prm, sp[i].W, sp[i].H, sp[i].dmax);
         S.base[i] = a;
-        a += pair_arena(sp[i], S.dd[i], debug, post_scratch(prm));
+        a += pair_arena(sp[i], S.dd[i], debug, post_scratch(prm), bin_test);
         S.sp_off[i] = so;
         so += 2 * 2 * S.dd[i].ncan;
         S.tri_off[i] = to;
@@ -295,11 +308,13 @@ int elas_ctx::slot_prepare(ElasSlot& S, const ElasPairSpec* sp, int n) {
     S.dprefix[0] = (int32_t*)(T + S.tab_pairs);
     S.dprefix[1] = (int32_t*)(T + S.tab_one + S.tab_pairs);
     S.dnsp = (int32_t*)(T + S.tab_pairs + S.tab_prefix);
+    S.dovf = (int32_t*)(T + S.tab_one + S.tab_pairs + S.tab_prefix);
     S.hpairs[0] = (ElasPair*)PIN;
     S.hprefix[0] = (int32_t*)(PIN + S.tab_pairs);
     S.hnsp = (int32_t*)(PIN + S.tab_pairs + S.tab_prefix);
     S.hpairs[1] = (ElasPair*)(PIN + S.tab_one);
     S.hprefix[1] = (int32_t*)(PIN + S.tab_one + S.tab_pairs);
+    S.hovf = (int32_t*)(PIN + S.tab_one + S.tab_pairs + S.tab_prefix);
     S.hsp = (float*)(PIN + 2 * S.tab_one);
     S.htri = (int32_t*)(PIN + 2 * S.tab_one + al(psp));
 
@@ -342,6 +357,17 @@ int elas_ctx::slot_prepare(ElasSlot& S, const ElasPairSpec* sp, int n) {
         else { P.dout[0] = s.D1; P.dout[1] = s.D2; }
         if (post_scratch(prm))
             for (int k = 0; k < 2; k++) P.pbuf[k] = (int32_t*)take(8 * d.npx);
+        const size_t nt = bin_tiles(s.W, s.H);
+        P.bcap = bin_cap(d, nt, bin_test);
+        for (int k = 0; k < 2; k++) P.bcnt[k] = (int32_t*)take(4 * (nt + 1));
+        for (int k = 0; k < 2; k++) P.bcur[k] = (int32_t*)take(4 * nt);
+        for (int k = 0; k < 2; k++) P.blist[k] = (int32_t*)take(4 * (size_t)P.bcap);
+        for (int k = 0; k < 2; k++) {
+            P.toff[k] = P.bcnt[k];
+            P.tlist[k] = P.blist[k];
+        }
+        P.gpu_bins = 1;
+        P.bovf = S.dovf + i;
     }
     return ELAS_OK;
 }
@@ -368,6 +394,8 @@ int stage_tiles(const elas_params& prm, const ElasPairSpec& s, const ElasDims& d
         case S_POST_TILES: return 2 * cdiv(s.W, ELAS_POST_TW) * cdiv(s.H, ELAS_POST_TH);
         case S_POST_ROWS: return 2 * s.H;
         case S_POST_COLS: return 2 * s.W;
+        case S_BIN_CLEAR: return cdiv(2 * ((long)bin_tiles(s.W, s.H) + 1) + 1, 256);
+        case S_BIN_SCAN: return 2;
     }
     return 0;
 }
@@ -511,8 +539,8 @@ int elas_ctx::slot_start_tri(ElasSlot& S) {
             j.result = 0;
             j.W = S.specs[i].W;
             j.H = S.specs[i].H;
-            j.tile_off = &S.toff[2 * i + k];
-            j.tile_list = &S.tlist[2 * i + k];
+            j.tile_off = nullptr;  // raster bins are built on the GPU
+            j.tile_list = nullptr;
             S.jobs.push_back(j);
         }
     }
@@ -569,7 +597,103 @@ int elas_ctx::slot_phase_b(ElasSlot& S) {
                                         S.stream));
             }
     }
-    // raster bins: [toff0 | toff1 | tlist0 | tlist1] per pair in one upload
+    build_prefix(prm, S);
+    memcpy(S.hpairs[1], S.pairs.data(), sizeof(ElasPair) * n);
+    memcpy(S.hprefix[1], S.prefix.data(), sizeof(int32_t) * S.prefix.size());
+    hipStream_t s1 = S.stream;
+    S.t_tri1 = clk::now();
+    HIPCHECK(hipStreamWaitEvent(s1, S.ev[12], 0));
+    HIPCHECK(hipEventRecord(S.ev[6], s1));
+    if (use_graphs) {
+        hipGraphExec_t ex = nullptr;
+        int r = slot_graph(S, 1, &ex);
+        if (r != ELAS_OK) return r;
+        HIPCHECK(hipGraphLaunch(ex, s1));
+    } else {
+        int r = enqueue_b(S, true);
+        if (r != ELAS_OK) return r;
+    }
+    HIPCHECK(hipEventRecord(S.ev[10], s1));
+    for (int i = 0; i < n; i++)
+        if (!S.specs[i].dev_out) {
+            HIPCHECK(hipMemcpyAsync(S.specs[i].D1, S.pairs[i].dout[0], 4 * S.dd[i].npx, hipMemcpyDeviceToHost, s1));
+            HIPCHECK(hipMemcpyAsync(S.specs[i].D2, S.pairs[i].dout[1], 4 * S.dd[i].npx, hipMemcpyDeviceToHost, s1));
+        }
+    HIPCHECK(hipEventRecord(S.ev[13], s1));
+    S.phase = SLOT_B;
+    return ELAS_OK;
+}
+
+// Phase B on the pipeline stream: table upload, plane records, raster, dense
+// matching, LR check, post filters.  Every launch depends only on the chunk's
+// shapes and parameters (the plane kernel reads its triangle count from the
+// table), so the sequence can be captured once per shape bucket.
+int elas_ctx::enqueue_b(ElasSlot& S, bool timed) {
+    const int n = S.n;
+    hipStream_t s1 = S.stream;
+    auto ev = [&](int i) { return timed ? hipEventRecord(S.ev[i], s1) : hipSuccess; };
+    HIPCHECK(hipMemcpyAsync(S.dpairs[1], S.hpairs[1], sizeof(ElasPair) * n, hipMemcpyHostToDevice, s1));
+    HIPCHECK(hipMemcpyAsync(S.dprefix[1], S.hprefix[1], sizeof(int32_t) * S.prefix.size(), hipMemcpyHostToDevice, s1));
+    const int cap256 = std::max(1, num_cu) * 8, cap_dense = std::max(1, num_cu) * (2048 / ELAS_DENSE_THREADS);
+    auto g = [&](const ElasLaunch& L, int cap) { return std::min(L.total, cap); };
+    HIPCHECK(ev(15));
+    {
+        // plane records + bin counts, bin offsets, bin lists; the triangle
+        // lanes' grid comes from the largest possible triangle count (a shape
+        // property; the kernels read the actual count from the table)
+        long maxtri = 0;
+        for (int i = 0; i < n; i++) maxtri += 2L * S.dd[i].tricap;
+        const int tgrid = std::max(1, std::min(4 * cap256, cdiv(maxtri, 64)));
+        ElasLaunch Lc = launch_of(S, 1, S_BIN_CLEAR), Lt = launch_of(S, 1, S_PLANE), Ls = launch_of(S, 1, S_BIN_SCAN);
+        if (Lc.total) elas_launch_bin_clear(Lc, g(Lc, cap256), s1);
+        elas_launch_plane(Lt, tgrid, s1);
+        if (Ls.total) elas_launch_bin_scan(Ls, g(Ls, std::max(1, num_cu) * 2), s1);
+        elas_launch_bin_fill(Lt, tgrid, s1);
+    }
+    { ElasLaunch L = launch_of(S, 1, S_RASTER); if (L.total) elas_launch_raster(L, g(L, cap256), s1); }
+    HIPCHECK(ev(7));
+    { ElasLaunch L = launch_of(S, 1, S_DENSE);
+      if (L.total) elas_launch_dense(L, S.kpl, g(L, cap_dense), dense_lds(prm.grid_size, S.maxd, S.cpw), s1); }
+    HIPCHECK(ev(8));
+    { ElasLaunch L = launch_of(S, 1, S_PIXELS); if (L.total) elas_launch_lr(L, kp, g(L, cap256), s1); }
+    if (debug) { ElasLaunch L = launch_of(S, 1, S_PIXELS); if (L.total) elas_launch_mean(L, g(L, cap256), s1); }
+    HIPCHECK(ev(9));
+    // post-processing on D1 (and D2): speckle, gap interpolation, adaptive mean, median
+    {
+        const ElasLaunch Lt = launch_of(S, 1, S_POST_TILES), Lp = launch_of(S, 1, S_PIXELS);
+        if (kp.speckle_size > 0 && Lt.total) elas_launch_speckle(Lt, Lp, kp, cap256, s1);
+        if (kp.ipol_gap > 0 && Lt.total) {
+            int mw = 1, mh = 1;
+            for (int i = 0; i < n; i++) { mw = std::max(mw, S.specs[i].W); mh = std::max(mh, S.specs[i].H); }
+            elas_launch_gap(launch_of(S, 1, S_POST_ROWS), launch_of(S, 1, S_POST_COLS), kp, cap256,
+                            4 * (size_t)mw, 4 * (size_t)mh, s1);
+        }
+        int in_tmp = 0;
+        if (kp.adaptive_mean && Lt.total) { elas_launch_sep(Lt, kp, false, in_tmp, cap256, s1); in_tmp ^= 1; }
+        if (kp.median && Lt.total) { elas_launch_sep(Lt, kp, true, in_tmp, cap256, s1); in_tmp ^= 1; }
+        if (in_tmp && Lp.total) elas_launch_post_copy(Lp, kp, cap256, s1);
+    }
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipMemcpyAsync(S.hovf, S.dovf, 4 * (size_t)n, hipMemcpyDeviceToHost, s1));
+    return ELAS_OK;
+}
+
+// Raster bins on the host, for a chunk whose GPU bins outgrew their capacity
+// (a mesh of unusually large triangles): bin every mesh, upload
+// [toff0 | toff1 | tlist0 | tlist1] per pair, and re-run phase B eagerly.
+// Synchronous; the results are those of the GPU-binned path.
+int elas_ctx::slot_host_bins(ElasSlot& S) {
+    const int n = S.n;
+    S.toff.assign(2 * (size_t)n, {});
+    S.tlist.assign(2 * (size_t)n, {});
+    for (int i = 0; i < n; i++)
+        for (int k = 0; k < 2; k++) {
+            const int nt = k ? S.ntri1[i] : S.ntri0[i];
+            const float* pts = S.hsp + S.sp_off[i] + (size_t)k * 2 * S.dd[i].ncan;
+            elas_bin_triangles(pts, nt > 0 ? S.tri_out[2 * i + k].data() : nullptr, nt, S.specs[i].W, S.specs[i].H,
+                               S.toff[2 * i + k], S.tlist[2 * i + k]);
+            S.job_of[2 * i + k] = 0;  // every mesh has bins now
+        }
     size_t bl_ints = 0;
     std::vector<size_t> bl_off(n);
     for (int i = 0; i < n; i++) {
@@ -624,77 +748,17 @@ int elas_ctx::slot_phase_b(ElasSlot& S) {
         }
         if (bl_ints) HIPCHECK(hipMemcpyAsync(S.blist, S.blist_pin, 4 * bl_ints, hipMemcpyHostToDevice, S.stream));
     }
-    build_prefix(prm, S);
+    for (int i = 0; i < n; i++) S.pairs[i].gpu_bins = 0;
     memcpy(S.hpairs[1], S.pairs.data(), sizeof(ElasPair) * n);
-    memcpy(S.hprefix[1], S.prefix.data(), sizeof(int32_t) * S.prefix.size());
-    hipStream_t s1 = S.stream;
-    S.t_tri1 = clk::now();
-    HIPCHECK(hipStreamWaitEvent(s1, S.ev[12], 0));
-    HIPCHECK(hipEventRecord(S.ev[6], s1));
-    if (use_graphs) {
-        hipGraphExec_t ex = nullptr;
-        int r = slot_graph(S, 1, &ex);
-        if (r != ELAS_OK) return r;
-        HIPCHECK(hipGraphLaunch(ex, s1));
-    } else {
-        int r = enqueue_b(S, true);
-        if (r != ELAS_OK) return r;
-    }
-    HIPCHECK(hipEventRecord(S.ev[10], s1));
+    int r = enqueue_b(S, false);
+    if (r != ELAS_OK) return r;
     for (int i = 0; i < n; i++)
         if (!S.specs[i].dev_out) {
-            HIPCHECK(hipMemcpyAsync(S.specs[i].D1, S.pairs[i].dout[0], 4 * S.dd[i].npx, hipMemcpyDeviceToHost, s1));
-            HIPCHECK(hipMemcpyAsync(S.specs[i].D2, S.pairs[i].dout[1], 4 * S.dd[i].npx, hipMemcpyDeviceToHost, s1));
+            HIPCHECK(hipMemcpyAsync(S.specs[i].D1, S.pairs[i].dout[0], 4 * S.dd[i].npx, hipMemcpyDeviceToHost, S.stream));
+            HIPCHECK(hipMemcpyAsync(S.specs[i].D2, S.pairs[i].dout[1], 4 * S.dd[i].npx, hipMemcpyDeviceToHost, S.stream));
         }
-    HIPCHECK(hipEventRecord(S.ev[13], s1));
-    S.phase = SLOT_B;
-    return ELAS_OK;
-}
-
-// Phase B on the pipeline stream: table upload, plane records, raster, dense
-// matching, LR check, post filters.  Every launch depends only on the chunk's
-// shapes and parameters (the plane kernel reads its triangle count from the
-// table), so the sequence can be captured once per shape bucket.
-int elas_ctx::enqueue_b(ElasSlot& S, bool timed) {
-    const int n = S.n;
-    hipStream_t s1 = S.stream;
-    auto ev = [&](int i) { return timed ? hipEventRecord(S.ev[i], s1) : hipSuccess; };
-    HIPCHECK(hipMemcpyAsync(S.dpairs[1], S.hpairs[1], sizeof(ElasPair) * n, hipMemcpyHostToDevice, s1));
-    HIPCHECK(hipMemcpyAsync(S.dprefix[1], S.hprefix[1], sizeof(int32_t) * S.prefix.size(), hipMemcpyHostToDevice, s1));
-    const int cap256 = std::max(1, num_cu) * 8, cap_dense = std::max(1, num_cu) * (2048 / ELAS_DENSE_THREADS);
-    auto g = [&](const ElasLaunch& L, int cap) { return std::min(L.total, cap); };
-    HIPCHECK(ev(15));
-    {
-        // grid from the largest possible triangle count (a shape property)
-        long maxtri = 0;
-        for (int i = 0; i < n; i++) maxtri += 2L * S.dd[i].tricap;
-        ElasLaunch L = launch_of(S, 1, S_PLANE);
-        elas_launch_plane(L, std::max(1, std::min(4 * cap256, cdiv(maxtri, 64))), s1);
-    }
-    { ElasLaunch L = launch_of(S, 1, S_RASTER); if (L.total) elas_launch_raster(L, g(L, cap256), s1); }
-    HIPCHECK(ev(7));
-    { ElasLaunch L = launch_of(S, 1, S_DENSE);
-      if (L.total) elas_launch_dense(L, S.kpl, g(L, cap_dense), dense_lds(prm.grid_size, S.maxd, S.cpw), s1); }
-    HIPCHECK(ev(8));
-    { ElasLaunch L = launch_of(S, 1, S_PIXELS); if (L.total) elas_launch_lr(L, kp, g(L, cap256), s1); }
-    if (debug) { ElasLaunch L = launch_of(S, 1, S_PIXELS); if (L.total) elas_launch_mean(L, g(L, cap256), s1); }
-    HIPCHECK(ev(9));
-    // post-processing on D1 (and D2): speckle, gap interpolation, adaptive mean, median
-    {
-        const ElasLaunch Lt = launch_of(S, 1, S_POST_TILES), Lp = launch_of(S, 1, S_PIXELS);
-        if (kp.speckle_size > 0 && Lt.total) elas_launch_speckle(Lt, Lp, kp, cap256, s1);
-        if (kp.ipol_gap > 0 && Lt.total) {
-            int mw = 1, mh = 1;
-            for (int i = 0; i < n; i++) { mw = std::max(mw, S.specs[i].W); mh = std::max(mh, S.specs[i].H); }
-            elas_launch_gap(launch_of(S, 1, S_POST_ROWS), launch_of(S, 1, S_POST_COLS), kp, cap256,
-                            4 * (size_t)mw, 4 * (size_t)mh, s1);
-        }
-        int in_tmp = 0;
-        if (kp.adaptive_mean && Lt.total) { elas_launch_sep(Lt, kp, false, in_tmp, cap256, s1); in_tmp ^= 1; }
-        if (kp.median && Lt.total) { elas_launch_sep(Lt, kp, true, in_tmp, cap256, s1); in_tmp ^= 1; }
-        if (in_tmp && Lp.total) elas_launch_post_copy(Lp, kp, cap256, s1);
-    }
-    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(S.stream));
+    S.bin_fallbacks++;
     return ELAS_OK;
 }
 
@@ -768,6 +832,12 @@ bool elas_ctx::slot_b_ready(ElasSlot& S) { return hipEventQuery(S.ev[13]) == hip
 int elas_ctx::slot_finish(ElasSlot& S, int32_t* statuses) {
     HIPCHECK(hipSetDevice(device));
     HIPCHECK(hipEventSynchronize(S.ev[13]));
+    bool ovf = false;
+    for (int i = 0; i < S.n; i++) ovf = ovf || (S.pairs[i].gpu_bins && S.hovf[i] != 0);
+    if (ovf) {
+        const int r = slot_host_bins(S);
+        if (r != ELAS_OK) return r;
+    }
     S.t_end = clk::now();
     auto el = [&](int a, int b) {
         float ms = 0.f;

@@ -81,9 +81,11 @@ struct ElasSlot {
     ElasPair* dpairs[2] = {nullptr, nullptr};
     int32_t* dprefix[2] = {nullptr, nullptr};
     int32_t* dnsp = nullptr;
+    int32_t* dovf = nullptr;  // per pair: GPU raster bins outgrew their capacity
     ElasPair* hpairs[2] = {nullptr, nullptr};
     int32_t* hprefix[2] = {nullptr, nullptr};
     int32_t* hnsp = nullptr;
+    int32_t* hovf = nullptr;
     float* hsp = nullptr;
     int32_t* htri = nullptr;
     std::vector<std::vector<int32_t>> tri_out, toff, tlist;
@@ -94,7 +96,7 @@ struct ElasSlot {
     float times[ELAS_T_COUNT] = {};
     // graphs of this slot's shape buckets
     std::vector<ElasGraphEntry> graphs;
-    uint64_t graph_clock = 0, graph_captures = 0;
+    uint64_t graph_clock = 0, graph_captures = 0, bin_fallbacks = 0;
     bool graphed = false;  // the chunk in flight runs its phases as graphs
 };
 
@@ -106,6 +108,7 @@ struct elas_ctx {
     elas_params prm;
     ElasKParams kp;
     bool debug = false;
+    bool bin_test = false;  // raster bin capacity 1 (tests the host-binning fallback)
     int tri_workers = ELAS_TRI_INSTANCES;
     int chunk = 0;  // pairs per pipeline chunk of a batch (0: whole batch)
     bool use_graphs = true;   // phases A and B replayed as graphs per shape bucket
@@ -146,6 +149,7 @@ struct elas_ctx {
     int enqueue_b(ElasSlot& S, bool timed);
     int slot_graph(ElasSlot& S, int which, hipGraphExec_t* out);
     void slot_drop_graphs(ElasSlot& S);
+    int slot_host_bins(ElasSlot& S);
 };
 
 // Serving loop (elas_serve.cpp).

@@ -17,6 +17,9 @@ void elas_launch_grid_scatter(const ElasLaunch& L, int grid, hipStream_t s);
 void elas_launch_grid_dilate(const ElasLaunch& L, int grid, hipStream_t s);
 void elas_launch_grid_list(const ElasLaunch& L, int grid, hipStream_t s);
 void elas_launch_plane(const ElasLaunch& L, int grid, hipStream_t s);
+void elas_launch_bin_clear(const ElasLaunch& L, int grid, hipStream_t s);
+void elas_launch_bin_scan(const ElasLaunch& L, int grid, hipStream_t s);
+void elas_launch_bin_fill(const ElasLaunch& L, int grid, hipStream_t s);
 void elas_launch_raster(const ElasLaunch& L, int grid, hipStream_t s);
 void elas_launch_mean(const ElasLaunch& L, int grid, hipStream_t s);
 void elas_launch_dense(const ElasLaunch& L, const ElasKParams& kp, int grid, size_t lds, hipStream_t s);

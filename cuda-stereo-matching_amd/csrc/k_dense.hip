@@ -41,6 +41,204 @@ __device__ bool elas_solve3(double A[3][3], double B[3]) {
     return true;
 }
 
+// ---- raster bins ------------------------------------------------------------
+// Tiles (ELAS_RASTER_T px) a triangle's covered pixels can fall in.  Covered
+// pixels lie in columns [min u, max u) (the reference's spans are half-open
+// between corner columns, KerComputeDisparity.cu:128-150) and in rows that
+// are truncated float evaluations of the edge lines, within the corner rows
+// up to rounding: with slopes s, |u| <= U and rows <= V the float error of
+// s*u + (v0 - s*u0) is below 8 * 2^-24 * (3 s U + 3 V) rows.  The rows are
+// widened by 3 + that bound (rounded up with room for the float evaluation
+// of the bound itself), so the tile range is a superset; rasterisation
+// tests every pixel exactly.
+__device__ __forceinline__ bool elas_bin_box(const float* rec, int W, int H, int& tx0, int& tx1, int& ty0,
+                                             int& ty1) {
+    const int cu0 = (int)rec[3], cv0 = (int)rec[4], cu1 = (int)rec[5], cv1 = (int)rec[6];
+    const int cu2 = (int)rec[7], cv2 = (int)rec[8];  // corners sorted by u
+    const int vmin = min(cv0, min(cv1, cv2)), vmax = max(cv0, max(cv1, cv2));
+    float smax = 0.0f;
+    if (cu0 != cu1) smax = fmaxf(smax, fabsf((float)(cv0 - cv1) / (float)(cu0 - cu1)));
+    if (cu0 != cu2) smax = fmaxf(smax, fabsf((float)(cv0 - cv2) / (float)(cu0 - cu2)));
+    if (cu1 != cu2) smax = fmaxf(smax, fabsf((float)(cv1 - cv2) / (float)(cu1 - cu2)));
+    const int Ui = max(abs(cu0), abs(cu2)) + 1, Vi = max(abs(vmin), abs(vmax)) + 1;
+    const float xf = 8.0f * (3.0f * smax * (float)Ui + 3.0f * (float)Vi) * (1.0f / 16777216.0f);
+    const int margin = xf < 100.0f ? 3 + (int)(xf + 1e-3f) : 4 + (int)(xf * 1.001f);
+    const int u0 = max(cu0, 0), u1 = min(cu2, W);
+    const int v0 = max(vmin - margin, 0), v1 = min(vmax + margin + 1, H);
+    if (u0 >= u1 || v0 >= v1) return false;
+    tx0 = u0 / ELAS_RASTER_T;
+    tx1 = (u1 - 1) / ELAS_RASTER_T;
+    ty0 = v0 / ELAS_RASTER_T;
+    ty1 = (v1 - 1) / ELAS_RASTER_T;
+    return true;
+}
+
+// The (triangle, tile) items of a 64-lane workgroup's triangles, spread evenly
+// over the lanes: a triangle's box is a few tiles on average but up to a few
+// hundred for slivers along the hull, and a lane walking its own box would
+// hold the whole wave.  Lane i's triangle owns items [incl[i-1], incl[i]); item
+// k is tile (ty0 + k / w, tx0 + k % w) of its box.  Calls f(tag, tile row,
+// tile column) per item.  s holds 5 * 64 ints of LDS.
+template <typename F>
+__device__ __forceinline__ void elas_bin_items(int* s, bool ok, int tx0, int tx1, int ty0, int ty1, int tag, F f) {
+    const int lane = threadIdx.x;
+    const int w = ok ? tx1 - tx0 + 1 : 1;
+    const int nt = ok ? w * (ty1 - ty0 + 1) : 0;
+    int incl = nt;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int x = __shfl_up(incl, o);
+        if (lane >= o) incl += x;
+    }
+    const int total = __shfl(incl, 63);
+    int* s_incl = s;
+    int* s_tx = s + 64;
+    int* s_ty = s + 128;
+    int* s_w = s + 192;
+    int* s_tag = s + 256;
+    s_incl[lane] = incl; s_tx[lane] = tx0; s_ty[lane] = ty0; s_w[lane] = w; s_tag[lane] = tag;
+    elas_lds_barrier();
+    for (int item = lane; item < total; item += 64) {
+        // owner: the number of lanes whose inclusive count is <= item (lane 63's
+        // count, the total, is above every item)
+        int j = 0;
+        #pragma unroll
+        for (int step = 32; step > 0; step >>= 1)
+            if (s_incl[j + step - 1] <= item) j += step;
+        const int k = item - (j ? s_incl[j - 1] : 0);
+        const int ww = s_w[j];
+        const int q = k / ww;
+        f(s_tag[j], s_ty[j] + q, s_tx[j] + (k - q * ww));
+    }
+    elas_lds_barrier();  // the arrays are rewritten by the next grid-stride step
+}
+
+// Clears the per-tile counts of both images and the overflow flag.
+__global__ __launch_bounds__(256) void k_bin_clear(ElasLaunch L) {
+    for (int t = blockIdx.x; t < L.total; t += gridDim.x) {
+        const int p = elas_find_pair(L.prefix, L.npairs, t);
+        const auto& P = L.pairs[p];
+        if (!P.gpu_bins) continue;
+        const int nt1 = P.ntx * P.nty + 1;
+        const int i = (t - L.prefix[p]) * 256 + threadIdx.x;
+        if (i < nt1) elas_st(P.bcnt[0] + i, 0);
+        else if (i < 2 * nt1) elas_st(P.bcnt[1] + i - nt1, 0);
+        else if (i == 2 * nt1) elas_st(P.bovf, 0);
+    }
+}
+
+// Exclusive scan of one image's tile counts into list offsets (and fill
+// cursors); one workgroup per image.  Lists that would outgrow the capacity
+// set the overflow flag and are left empty.
+__global__ __launch_bounds__(1024) void k_bin_scan(ElasLaunch L) {
+    __shared__ int s_sum[1024];
+    const int tid = threadIdx.x;
+    for (int t = blockIdx.x; t < L.total; t += gridDim.x) {
+        const int p = elas_find_pair(L.prefix, L.npairs, t);
+        const auto& P = L.pairs[p];
+        if (!P.gpu_bins) continue;
+        const int img = t - L.prefix[p];
+        const int n = P.ntx * P.nty;
+        int32_t* cnt = P.bcnt[img];
+        int32_t* cur = P.bcur[img];
+        const int C = (n + 1023) / 1024;
+        const int a = min(n, tid * C), b = min(n, a + C);
+        int sum = 0;
+        for (int i = a; i < b; i++) sum += cnt[i];
+        s_sum[tid] = sum;
+        __syncthreads();
+        for (int off = 1; off < 1024; off <<= 1) {
+            const int x = tid >= off ? s_sum[tid - off] : 0;
+            __syncthreads();
+            s_sum[tid] += x;
+            __syncthreads();
+        }
+        const int total = s_sum[1023];
+        const bool over = total > P.bcap;
+        int run = tid > 0 ? s_sum[tid - 1] : 0;
+        for (int i = a; i < b; i++) {
+            const int c = cnt[i];
+            cnt[i] = over ? 0 : run;
+            cur[i] = over ? 0 : run;
+            run += c;
+        }
+        if (tid == 0) {
+            cnt[n] = over ? 0 : total;
+            if (over) elas_st(P.bovf, 1);
+        }
+        __syncthreads();
+    }
+}
+
+// Fills the tile lists from the plane records (which hold the sorted corners),
+// a workgroup's (triangle, tile) items spread over its lanes; the order within
+// a list does not matter to the raster.
+__global__ __launch_bounds__(64) void k_bin_fill(ElasLaunch L) {
+    __shared__ int s_items[5 * 64];
+    const int total = L.prefix[L.npairs];
+    for (int t = blockIdx.x; t < total; t += gridDim.x) {
+        const int p = elas_find_pair(L.prefix, L.npairs, t);
+        const auto& P = L.pairs[p];
+        if (!P.gpu_bins) continue;
+        const int idx = (t - L.prefix[p]) * 64 + threadIdx.x;
+        const int n0 = P.ntri[0], n1 = P.ntri[1];
+        // an image whose lists overflowed has all offsets 0 (k_bin_scan): its
+        // writes below stay within the capacity and are never read
+        const bool live = idx < n0 + n1;
+        const int img = live && idx >= n0;
+        const int ti = img ? idx - n0 : idx;
+        int tx0 = 0, tx1 = 0, ty0 = 0, ty1 = 0;
+        const bool ok = live && elas_bin_box(P.trirec[img] + (size_t)ti * ELAS_TRIREC, P.W, P.H, tx0, tx1, ty0, ty1);
+        const int cap = P.bcap, ntx = P.ntx;
+        elas_bin_items(s_items, ok, tx0, tx1, ty0, ty1, idx, [&](int tag, int ty, int tx) {
+            const int im = tag >= n0;
+            const int pos = atomicAdd(P.bcur[im] + ty * ntx + tx, 1);
+            if (pos < cap) P.blist[im][pos] = tag - (im ? n0 : 0);
+        });
+    }
+}
+
+// Plane coefficients and sorted-corner edge lines of triangle ti of image img.
+__device__ __forceinline__ void elas_plane_rec(const ElasPair& P, int img, int ti, float* rec) {
+    const int32_t* tr = P.tri[img] + 3 * (size_t)ti;
+    const float* sp = P.sp[0];
+    const float* sq = P.sp[1];
+    int tu[3], tv[3], td[3];
+    double A[3][3], b[3];
+    int ci[3];
+    for (int i = 0; i < 3; i++) ci[i] = tr[i];
+    for (int i = 0; i < 3; i++) {
+        tu[i] = (int)sp[2 * ci[i]];
+        tv[i] = (int)sp[2 * ci[i] + 1];
+        td[i] = tu[i] - (int)sq[2 * ci[i]];
+        A[i][0] = img ? (double)(tu[i] - td[i]) : (double)tu[i];
+        A[i][1] = (double)tv[i];
+        A[i][2] = 1.0;
+        b[i] = (double)td[i];
+    }
+    float pa = 0.f, pb = 0.f, pc = 0.f;
+    if (elas_solve3(A, b)) { pa = (float)b[0]; pb = (float)b[1]; pc = (float)b[2]; }
+    if (img) for (int i = 0; i < 3; i++) tu[i] -= td[i];
+    for (int j = 0; j < 3; j++)
+        for (int k = 0; k < j; k++)
+            if (tu[k] > tu[j]) {
+                int x = tu[j]; tu[j] = tu[k]; tu[k] = x;
+                x = tv[j]; tv[j] = tv[k]; tv[k] = x;
+            }
+    const float Au = (float)tu[0], Av = (float)tv[0];
+    const float Bu = (float)tu[1], Bv = (float)tv[1];
+    const float Cu = (float)tu[2], Cv = (float)tv[2];
+    float ABa = 0.f, ACa = 0.f, BCa = 0.f;
+    if (tu[0] != tu[1]) ABa = __fdiv_rn(__fsub_rn(Av, Bv), __fsub_rn(Au, Bu));
+    if (tu[0] != tu[2]) ACa = __fdiv_rn(__fsub_rn(Av, Cv), __fsub_rn(Au, Cu));
+    if (tu[1] != tu[2]) BCa = __fdiv_rn(__fsub_rn(Bv, Cv), __fsub_rn(Bu, Cu));
+    rec[0] = pa; rec[1] = pb; rec[2] = pc;
+    rec[3] = Au; rec[4] = Av; rec[5] = Bu; rec[6] = Bv; rec[7] = Cu; rec[8] = Cv;
+    rec[9] = ABa;  rec[10] = __fsub_rn(Av, __fmul_rn(ABa, Au));
+    rec[11] = ACa; rec[12] = __fsub_rn(Av, __fmul_rn(ACa, Au));
+    rec[13] = BCa; rec[14] = __fsub_rn(Bv, __fmul_rn(BCa, Bu));
+    rec[15] = 0.f;
+}
+
 // ---- per-triangle record: KerComputeDisparity.cu:10-124 -------------------
 // One lane per triangle of either mesh, 64-lane workgroups so that a few
 // thousand triangles spread over many CUs.  Corners come from the left support
@@ -48,6 +246,7 @@ __device__ bool elas_solve3(double A[3][3], double B[3]) {
 // KerComputeDisparity.cu:63-66,95-99).  A corner's d is sp0.u - sp1.u (exact
 // in float), which saves a dependent load of the lattice.
 __global__ __launch_bounds__(64) void k_plane(ElasLaunch L) {
+    __shared__ int s_items[5 * 64];
     // the triangle count is data: read the total from the table (a captured
     // launch keeps the grid of its shape bucket)
     const int total = L.prefix[L.npairs];
@@ -56,48 +255,21 @@ __global__ __launch_bounds__(64) void k_plane(ElasLaunch L) {
         const auto& P = L.pairs[p];
         const int idx = (t - L.prefix[p]) * 64 + threadIdx.x;
         const int n0 = P.ntri[0], n1 = P.ntri[1];
-        if (idx >= n0 + n1) continue;
-        const int img = idx >= n0;
+        const bool live = idx < n0 + n1;
+        const int img = live && idx >= n0;
         const int ti = img ? idx - n0 : idx;
-        const int32_t* tr = P.tri[img] + 3 * (size_t)ti;
-        const float* sp = P.sp[0];
-        const float* sq = P.sp[1];
-        int tu[3], tv[3], td[3];
-        double A[3][3], b[3];
-        int ci[3];
-        for (int i = 0; i < 3; i++) ci[i] = tr[i];
-        for (int i = 0; i < 3; i++) {
-            tu[i] = (int)sp[2 * ci[i]];
-            tv[i] = (int)sp[2 * ci[i] + 1];
-            td[i] = tu[i] - (int)sq[2 * ci[i]];
-            A[i][0] = img ? (double)(tu[i] - td[i]) : (double)tu[i];
-            A[i][1] = (double)tv[i];
-            A[i][2] = 1.0;
-            b[i] = (double)td[i];
-        }
-        float pa = 0.f, pb = 0.f, pc = 0.f;
-        if (elas_solve3(A, b)) { pa = (float)b[0]; pb = (float)b[1]; pc = (float)b[2]; }
-        if (img) for (int i = 0; i < 3; i++) tu[i] -= td[i];
-        for (int j = 0; j < 3; j++)
-            for (int k = 0; k < j; k++)
-                if (tu[k] > tu[j]) {
-                    int x = tu[j]; tu[j] = tu[k]; tu[k] = x;
-                    x = tv[j]; tv[j] = tv[k]; tv[k] = x;
-                }
-        const float Au = (float)tu[0], Av = (float)tv[0];
-        const float Bu = (float)tu[1], Bv = (float)tv[1];
-        const float Cu = (float)tu[2], Cv = (float)tv[2];
-        float ABa = 0.f, ACa = 0.f, BCa = 0.f;
-        if (tu[0] != tu[1]) ABa = __fdiv_rn(__fsub_rn(Av, Bv), __fsub_rn(Au, Bu));
-        if (tu[0] != tu[2]) ACa = __fdiv_rn(__fsub_rn(Av, Cv), __fsub_rn(Au, Cu));
-        if (tu[1] != tu[2]) BCa = __fdiv_rn(__fsub_rn(Bv, Cv), __fsub_rn(Bu, Cu));
         float* rec = P.trirec[img] + (size_t)ti * ELAS_TRIREC;
-        rec[0] = pa; rec[1] = pb; rec[2] = pc;
-        rec[3] = Au; rec[4] = Av; rec[5] = Bu; rec[6] = Bv; rec[7] = Cu; rec[8] = Cv;
-        rec[9] = ABa;  rec[10] = __fsub_rn(Av, __fmul_rn(ABa, Au));
-        rec[11] = ACa; rec[12] = __fsub_rn(Av, __fmul_rn(ACa, Au));
-        rec[13] = BCa; rec[14] = __fsub_rn(Bv, __fmul_rn(BCa, Bu));
-        rec[15] = 0.f;
+        if (live) elas_plane_rec(P, img, ti, rec);
+        if (P.gpu_bins) {
+            // count the triangle in every tile of its box (k_bin_scan and
+            // k_bin_fill follow); the whole wave takes part
+            int tx0 = 0, tx1 = 0, ty0 = 0, ty1 = 0;
+            const bool ok = live && elas_bin_box(rec, P.W, P.H, tx0, tx1, ty0, ty1);
+            const int ntx = P.ntx;
+            elas_bin_items(s_items, ok, tx0, tx1, ty0, ty1, idx, [&](int tag, int ty, int tx) {
+                atomicAdd(P.bcnt[tag >= n0] + ty * ntx + tx, 1);
+            });
+        }
     }
 }
 
@@ -106,8 +278,8 @@ __device__ __forceinline__ int elas_plane_d(const float* rec, int u, int v) {
 }
 
 // ---- rasterisation: KerComputeDisparity.cu:126-150 ------------------------
-// One workgroup per 32x32 pixel tile.  The tile's triangles (ascending index,
-// binned on the host) are staged in LDS in chunks; every (triangle, column)
+// One workgroup per 32x32 pixel tile.  The tile's triangles (binned on the
+// GPU, or on the host as a fallback) are staged in LDS in chunks; every (triangle, column)
 // pair is one work item: it computes the column's half-open row span between
 // the long edge AC and AB (u in [A_u, B_u)) or BC (u in [B_u, C_u)) and
 // records the triangle id for the span's pixels in an LDS tile with
@@ -407,6 +579,15 @@ __global__ __launch_bounds__(256) void k_lr(ElasLaunch L, ElasKParams kp) {
     }
 }
 
+void elas_launch_bin_clear(const ElasLaunch& L, int grid, hipStream_t s) {
+    hipLaunchKernelGGL(k_bin_clear, dim3(grid), dim3(256), 0, s, L);
+}
+void elas_launch_bin_scan(const ElasLaunch& L, int grid, hipStream_t s) {
+    hipLaunchKernelGGL(k_bin_scan, dim3(grid), dim3(1024), 0, s, L);
+}
+void elas_launch_bin_fill(const ElasLaunch& L, int grid, hipStream_t s) {
+    hipLaunchKernelGGL(k_bin_fill, dim3(grid), dim3(64), 0, s, L);
+}
 void elas_launch_plane(const ElasLaunch& L, int grid, hipStream_t s) {
     hipLaunchKernelGGL(k_plane, dim3(grid), dim3(64), 0, s, L);
 }

@@ -175,9 +175,11 @@ int elas_hip_set_graphs(elas_ctx* ctx, int32_t on);
 int elas_hip_set_pipeline(elas_ctx* ctx, int32_t slots, int32_t chunk);
 
 /* Intermediates of the last process call on this context (kept only when
- * elas_hip_set_debug(ctx, 1)).  Copies min(bytes, stage size) bytes to dst
+ * elas_hip_set_debug(ctx, 1)).  Bit 1 of the flags (value 2) shrinks the GPU
+ * raster-bin capacity to 1 so that every mesh takes the host-binning
+ * fallback (for tests; results are unchanged).  Copies min(bytes, stage size) bytes to dst
  * and stores the stage size in *stage_bytes (either may be NULL/0 to query). */
-int elas_hip_set_debug(elas_ctx* ctx, int keep_intermediates);
+int elas_hip_set_debug(elas_ctx* ctx, int flags);
 int elas_hip_debug_fetch(elas_ctx* ctx, int stage, void* dst, size_t bytes, size_t* stage_bytes);
 int elas_hip_shape_info(const elas_ctx* ctx, elas_shape_info* info);
 /* Per-stage device milliseconds of the last call (ELAS_T_COUNT floats). */
