@@ -97,6 +97,8 @@ SIGNATURES = {
                                             ctypes.POINTER(ctypes.c_size_t)]),
     "elas_hip_shape_info": (ctypes.c_int, [_P, ctypes.POINTER(ShapeInfo)]),
     "elas_hip_stage_times": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_float), ctypes.c_int32]),
+    "elas_hip_triangulate": (ctypes.c_int, [_P, ctypes.c_int32, _P, ctypes.c_int64,
+                                            ctypes.POINTER(ctypes.c_int64)]),
 }
 
 _lib = None
@@ -157,6 +159,22 @@ class ElasError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__("%s (%d): %s" % (STATUS.get(code, "error"), code, msg))
         self.code = code
+
+
+def triangulate(points):
+    """Delaunay triangles of [n][2] float points exactly as the pipeline's host
+    step computes them (the reference's Triangle, switches "zQB",
+    src/Elas.cu:278-328): an [ntri][3] int32 array of zero-based indices.
+    Host only; no device needed."""
+    lib = load_library()
+    pts = np.ascontiguousarray(points, dtype=np.float32).reshape(-1, 2)
+    cap = 2 * len(pts) + 8
+    tri = np.zeros((cap, 3), np.int32)
+    n = ctypes.c_int64()
+    st = lib.elas_hip_triangulate(pts.ctypes.data, len(pts), tri.ctypes.data, cap, ctypes.byref(n))
+    if st != ELAS_OK:
+        raise ElasError(st, "triangulate: status %d" % st)
+    return tri[:min(n.value, cap)].copy()
 
 
 def _u8(a):

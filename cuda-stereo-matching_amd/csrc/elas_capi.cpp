@@ -1,6 +1,7 @@
 // elas_capi.cpp -- extern "C" entry points declared in include/elas_hip.h.
 #include <hip/hip_runtime_api.h>
 
+#include <algorithm>
 #include <cstring>
 #include <new>
 #include <string>
@@ -8,6 +9,7 @@
 
 #include "elas_engine.h"
 #include "elas_hip.h"
+#include "elas_triangle.h"
 
 extern "C" {
 
@@ -289,6 +291,23 @@ int elas_hip_poll(elas_ctx* c, int64_t ticket, int32_t* status, float* latency_m
 int elas_hip_serve_stop(elas_ctx* c) {
     if (!c) return ELAS_ERR_INVALID_ARGUMENT;
     return elas_srv_stop(c);
+}
+
+int elas_hip_triangulate(const float* pts, int32_t n, int32_t* tri, int64_t cap, int64_t* ntri) {
+    if ((!pts && n > 0) || n < 0 || cap < 0 || (cap > 0 && !tri) || !ntri) return ELAS_ERR_INVALID_ARGUMENT;
+    std::vector<int32_t> out;
+    ElasTriJob job{};
+    job.pts = pts;
+    job.n = n;
+    job.out = &out;
+    elas_triangulate_jobs(&job, 1, 1);
+    if (job.result < 0) {
+        *ntri = 0;
+        return n < 3 ? ELAS_ERR_TOO_FEW_POINTS : ELAS_ERR_OUT_OF_MEMORY;
+    }
+    *ntri = job.result;
+    memcpy(tri, out.data(), sizeof(int32_t) * 3 * (size_t)std::min<int64_t>(cap, job.result));
+    return ELAS_OK;
 }
 
 int elas_hip_serve_stats(elas_ctx* c, elas_serve_stats* st) {

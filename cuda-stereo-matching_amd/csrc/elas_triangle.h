@@ -26,6 +26,11 @@ struct ElasTriJob {
 void elas_bin_triangles(const float* pts, const int32_t* tri, int ntri, int W, int H,
                         std::vector<int32_t>& off, std::vector<int32_t>& list);
 
+// The (x, y) order of a point set when it can be handed to Triangle in place
+// of the caller's order with the same mesh: integer x, no duplicates.  order
+// maps sorted position to input index.  False (order unused) otherwise.
+bool elas_tri_sorted_order(const float* p, int n, std::vector<int32_t>& order, std::vector<float>& sorted);
+
 // Runs every job (each a "zQB" Triangle call) on up to max_workers of the
 // process-wide Triangle workers; returns when all are done.
 void elas_triangulate_jobs(ElasTriJob* jobs, int n, int max_workers);

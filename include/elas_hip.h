@@ -196,6 +196,14 @@ typedef enum elas_kernel {
 } elas_kernel;
 int elas_hip_time_kernel(elas_ctx* ctx, int32_t kernel, int32_t reps, float* avg_ms);
 
+/* Host Delaunay triangulation as the pipeline does it: the reference's
+ * Triangle with switches "zQB" on n points [x0, y0, x1, y1, ...]
+ * (replaces Elas::computeDelaunyTrianguation, src/Elas.cu:278-328, for tests
+ * and host tools).  Writes min(count, cap) triangles of three zero-based point
+ * indices to tri and the count to *ntri.  Needs no device.  Fewer than 3 points
+ * returns ELAS_ERR_TOO_FEW_POINTS (the reference's Triangle exits there). */
+int elas_hip_triangulate(const float* pts, int32_t n, int32_t* tri, int64_t cap, int64_t* ntri);
+
 /* ---- Serving (no reference counterpart: the reference has one blocking
  * process call).  A scheduler thread owned by the context admits queued
  * requests into idle pipeline slots as chunks, FIFO, as long as the chunk's

@@ -1,0 +1,74 @@
+"""Host Delaunay step: the library hands Triangle the support points in
+(x, y) order when that leaves the mesh unchanged, and maps the corners back.
+
+Checked against the oracle's own build of the reference's triangle.cpp, which
+always triangulates the points in the caller's (row-major) order, as the
+reference does (src/Elas.cu:278-328).  No device needed.
+"""
+import numpy as np
+import pytest
+
+
+def _lattice(rng, cw, ch, step, keep):
+    cu, cv = np.meshgrid(np.arange(cw), np.arange(ch))
+    m = rng.random((ch, cw)) < keep
+    return np.stack([cu[m] * step, cv[m] * step], 1).astype(np.float32)
+
+
+def test_golden_mesh(pkg):
+    z = np.load(__file__.rsplit("/", 1)[0] + "/golden/small_pair.npz")
+    assert np.array_equal(pkg.triangulate(z["sp0"]), z["tri0"])
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_lattice_sets_equal_reference_order(pkg, oracle, seed):
+    """Left-image support points: lattice cells in row order (highly cocircular)."""
+    rng = np.random.default_rng(seed)
+    pts = _lattice(rng, 40 + 17 * seed, 20 + 9 * seed, 5, 0.3 + 0.1 * seed)
+    assert np.array_equal(pkg.triangulate(pts), oracle.triangulate(pts))
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_shifted_sets_with_duplicates_equal_reference_order(pkg, oracle, seed):
+    """Right-image support points (u - d, v): duplicates occur where two cells of
+    a row differ in d by their u distance; the library then keeps the caller's
+    order, since which duplicate Triangle drops depends on it."""
+    rng = np.random.default_rng(100 + seed)
+    pts = _lattice(rng, 60, 30, 5, 0.6)
+    d = rng.integers(0, 12, len(pts)).astype(np.float32)
+    pts[:, 0] -= d
+    if seed % 2 == 0:  # force duplicates
+        pts[1::7] = pts[0:-1:7][: len(pts[1::7])]
+    assert np.array_equal(pkg.triangulate(pts), oracle.triangulate(pts))
+
+
+def test_order_independence_without_duplicates(pkg, oracle):
+    """The property the sorted hand-over relies on: for distinct points,
+    Triangle's mesh (corners numbered by input position) is the same for any
+    input order."""
+    rng = np.random.default_rng(7)
+    pts = _lattice(rng, 70, 40, 5, 0.5)
+    ref = oracle.triangulate(pts)
+    for rep in range(4):
+        perm = rng.permutation(len(pts))
+        got = pkg.triangulate(pts[perm])
+        assert np.array_equal(perm[got], ref)
+
+
+def test_non_integer_points_keep_caller_order(pkg, oracle):
+    rng = np.random.default_rng(3)
+    pts = (rng.random((500, 2)) * [300, 200]).astype(np.float32)
+    assert np.array_equal(pkg.triangulate(pts), oracle.triangulate(pts))
+
+
+def test_pipeline_meshes_of_c0(pkg, oracle, pairs):
+    left, right, _, dmax = pairs("c0")
+    ref = oracle.process(left, right, oracle.params(disp_max=dmax))
+    assert np.array_equal(pkg.triangulate(ref["sp0"]), ref["tri0"])
+    assert np.array_equal(pkg.triangulate(ref["sp1"]), ref["tri1"])
+
+
+def test_too_few_points(pkg):
+    with pytest.raises(pkg.ElasError) as e:
+        pkg.triangulate(np.zeros((2, 2), np.float32))
+    assert e.value.code == pkg.ELAS_ERR_TOO_FEW_POINTS
