@@ -21,7 +21,8 @@
 //   [9..14] AB_a, AB_b, AC_a, AC_b, BC_a, BC_b         [15] unused
 #define ELAS_TRIREC 16
 
-#define ELAS_DESC_TW 64      // descriptor tile: ELAS_DESC_TW x (1024 / ELAS_DESC_TW) pixels
+#define ELAS_DESC_SW 58      // descriptor strip: output columns of a 64-lane wave
+#define ELAS_DESC_SH 34      // descriptor strip: output rows (34 + 6 rows walked)
 #define ELAS_POST_TW 64      // post-filter tile: 64 x 16 output pixels
 #define ELAS_POST_TH 16
 #define ELAS_GAP_MAX_LINE 15872  // gap interpolation keeps a line in LDS (<= 62 KiB)

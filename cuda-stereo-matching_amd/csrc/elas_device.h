@@ -44,6 +44,24 @@ __device__ __forceinline__ void elas_st(uint4* p, const uint4& v) {
 // wave has in flight (vmcnt 0); here a wave's prefetched loads and finished
 // tile's stores stay in flight across the barrier.  The asm's memory clobber
 // keeps the compiler from moving memory accesses across it.
+typedef uint32_t elas_u32x4 __attribute__((ext_vector_type(4)));
+
+// Buffer resource over [base, base + bytes) for raw buffer stores (stride 0,
+// the gfx9 default format word).  bytes < 2^32.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t elas_rsrc(void* base, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(base, (short)0, (int)bytes, 0x00020000);
+}
+
+// 16-byte store at byte offset off of a buffer resource.  An access at or
+// beyond the resource's range is dropped by the hardware, so a lane with
+// nothing to store passes the range size as its offset instead of branching
+// around the store: the store then always issues, and the compiler's count
+// of outstanding memory operations stays exact across it.
+__device__ __forceinline__ void elas_st16_ranged(__amdgpu_buffer_rsrc_t r, uint32_t off, uint4 q) {
+    elas_u32x4 v = {q.x, q.y, q.z, q.w};
+    __builtin_amdgcn_raw_buffer_store_b128(v, r, (int)off, 0, 0);
+}
+
 __device__ __forceinline__ void elas_lds_barrier() {
     __asm__ __volatile__("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
 }

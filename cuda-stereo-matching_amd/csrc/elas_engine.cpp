@@ -379,7 +379,7 @@ int stage_tiles(const elas_params& prm, const ElasPairSpec& s, const ElasDims& d
                 int st) {
     const int gs = prm.grid_size, R = dense_rows(gs);
     switch (st) {
-        case S_DESC: return cdiv(s.W, ELAS_DESC_TW) * cdiv(s.H, 1024 / ELAS_DESC_TW) * 2;
+        case S_DESC: return cdiv(s.W, ELAS_DESC_SW) * cdiv(s.H, ELAS_DESC_SH) * 2;
         case S_SUPPORT: return d.ch * cdiv(d.cw, 64);
         case S_INCON: return cdiv(d.cw, 16) * cdiv(d.ch, 16);
         case S_REDUN_H: return cdiv(d.ch, elas_redun_lines(d.cw));

@@ -1,47 +1,54 @@
 // k_desc.hip -- fused Sobel + 16-byte descriptor (gfx950).
 //
 // Restates src/Kernels/KerDescriptor.cu:9-100 (kerComputeSobel and
-// kerComputeDescriptor) as one kernel.  du/dv never go to HBM: algorithmic
-// traffic is 1 B read + 16 B written per pixel.
+// kerComputeDescriptor) as one kernel.  du/dv never leave registers:
+// algorithmic traffic is 1 B read + 16 B written per pixel.
 //
 // du = 128 + clamp(S_u >> 2, -128, 127) is the integer form of the
 // reference's (uint8_t)(min(max(-128.f, S/4), 127.f) + 128)
 // (KerDescriptor.cu:40-41): S/4 is exact in float and truncating the
-// non-negative sum is floor.  Descriptor bytes outside u in [3, W-4],
-// v in [3, H-4] are written as 0 (the reference leaves them unwritten).
+// non-negative sum is floor.  S_u(u) = V(u-1) - V(u+1) with the vertical
+// weighted sum V = I(v-1) + 2 I(v) + I(v+1); S_v(v) = H(v-1) - H(v+1) with the
+// horizontal one H = I(u-1) + 2 I(u) + I(u+1).  Descriptor bytes outside
+// u in [3, W-4], v in [3, H-4] are written as 0 (the reference leaves them
+// unwritten).  Byte order (KerDescriptor.cu:76-92), for row y, column u:
+//   du(y-2,u) du(y-1,u-2) du(y-1,u) du(y-1,u+2) | du(y,u-1) du(y,u) du(y,u) du(y,u+1) |
+//   du(y+1,u-2) du(y+1,u) du(y+1,u+2) du(y+2,u) | dv(y-1,u) dv(y,u-1) dv(y,u+1) dv(y+1,u)
 //
-// Work is done on packed bytes, four columns per dword:
-//   * a 64x16 output tile stages image rows v0-3 .. v0+18, columns
-//     u0-8 .. u0+71, in LDS as dwords (column c of a row = u - u0 + 8);
-//   * du and dv for four columns at once: with the vertical (horizontal)
-//     weighted sums V = I(r-1) + 2 I(r) + I(r+1) of columns (c, c+2) and
-//     (c+1, c+3) held as 16-bit halves of a dword (each <= 1020, so plain
-//     32-bit adds never carry between halves), S_u(c) = V(c-1) - V(c+1) and
-//     S_v = H(v-1) - H(v+1) are two packed 16-bit subtractions per four
-//     columns, then >> 2, clamp and + 128 in packed 16-bit lanes, and one
-//     byte permute packs the four results;
-//   * each thread writes the descriptors of four adjacent pixels: their 64
-//     bytes are gathered with v_perm_b32 (compile-time selectors) from 16
-//     dword reads of the du/dv rows around them.
+// One pixel column per lane.  A wave owns a strip of 64 consecutive columns
+// u = x0 - 3 + lane and walks down the image rows y0-3 .. y0+SH+2 of a band;
+// lanes 3 .. 60 are the strip's ELAS_DESC_SW output columns, the three lanes
+// on either side only feed their neighbours.  Per image row the wave computes
+// H of that row and V, du, dv of the row above it, for its own columns;
+// values of neighbouring columns come from cross-lane permutes.  The last
+// five du rows and three dv rows stay in registers, so every pixel's
+// descriptor is assembled with a handful of byte permutes, and the wave's
+// 16-byte stores of one row cover consecutive pixels (928 contiguous bytes
+// per store instruction).  No LDS, no barriers: the four waves of a
+// workgroup work on independent strips.
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include "elas_common.h"
 #include "elas_device.h"
 
-#define DT_W ELAS_DESC_TW        // tile width (a multiple of 16); DT_W * DT_H = 1024
-#define DT_H (1024 / DT_W)
-#define DT_GP (DT_W / 4)         // four-pixel groups per output row
-#define DT_NG (DT_GP + 2)        // du/dv groups per row (1 .. DT_GP + 2)
-#define DT_DW (DT_GP + 4)        // dwords per staged row (columns u0-8 .. u0+DT_W+7)
-#define DT_IR (DT_H + 6)         // image rows v0-3 .. v0+DT_H+2
-#define DT_UR (DT_H + 4)         // du rows v0-2 .. v0+DT_H+1
-#define DT_VR (DT_H + 2)         // dv rows v0-1 .. v0+DT_H
+#define DS_STEPS (ELAS_DESC_SH + 6)  // image rows walked per band
+static_assert(DS_STEPS % 8 == 0, "rows are loaded in blocks of 8");
+static_assert(ELAS_DESC_SW == 58, "lanes 3 .. 60 are the output columns");
 
 typedef int16_t elas_s16x2 __attribute__((ext_vector_type(2)));
 
 namespace {
 
-__device__ __forceinline__ uint32_t lo_bytes(uint32_t x) { return x & 0x00ff00ffu; }
-__device__ __forceinline__ uint32_t hi_bytes(uint32_t x) { return (x >> 8) & 0x00ff00ffu; }
+// value of the lane to the left (u - 1) / right (u + 1) through the LDS
+// crossbar (ds_bpermute, no LDS memory); lanes 0 / 63 read lanes 63 / 0,
+// values no output lane uses.  ixl / ixr: byte addresses 4 (lane -/+ 1).
+struct Xlane {
+    int ixl, ixr;
+    __device__ __forceinline__ int left(int x) const { return __builtin_amdgcn_ds_bpermute(ixl, x); }
+    __device__ __forceinline__ int right(int x) const { return __builtin_amdgcn_ds_bpermute(ixr, x); }
+    __device__ __forceinline__ uint32_t left(uint32_t x) const { return (uint32_t)left((int)x); }
+    __device__ __forceinline__ uint32_t right(uint32_t x) const { return (uint32_t)right((int)x); }
+};
 
 // 128 + clamp(s >> 2, -128, 127) in both 16-bit halves
 __device__ __forceinline__ uint32_t sobel_byte2(uint32_t s) {
@@ -53,175 +60,109 @@ __device__ __forceinline__ uint32_t sobel_byte2(uint32_t s) {
     return __builtin_bit_cast(uint32_t, v);
 }
 
-__device__ __forceinline__ uint32_t sub16x2(uint32_t a, uint32_t b) {
-    return __builtin_bit_cast(uint32_t, __builtin_bit_cast(elas_s16x2, a) - __builtin_bit_cast(elas_s16x2, b));
-}
-
-// bytes of (even, odd) halves -> [e.b0, o.b0, e.b2, o.b2]
-__device__ __forceinline__ uint32_t pack_eo(uint32_t e, uint32_t o) { return __builtin_amdgcn_perm(o, e, 0x06020400u); }
-
-constexpr uint32_t sel4(int b0, int b1, int b2, int b3) {
-    return (uint32_t)b0 | ((uint32_t)b1 << 8) | ((uint32_t)b2 << 16) | ((uint32_t)b3 << 24);
+__device__ __forceinline__ uint32_t perm(uint32_t hi, uint32_t lo, uint32_t sel) {
+    return __builtin_amdgcn_perm(hi, lo, sel);
 }
 
 }  // namespace
 
-__global__ __launch_bounds__(256) void k_descriptor(ElasLaunch L) {
-    __shared__ uint32_t s_img[DT_IR * DT_DW];
-    __shared__ uint32_t s_du[DT_UR * DT_DW];
-    __shared__ uint32_t s_dv[DT_VR * DT_DW];
-    const int tid = threadIdx.x;
-    constexpr int N = DT_IR * DT_DW, K = (N + 255) / 256;
-    // tile t -> pair, image, origin
-    struct Tile {
-        const ELAS_CONST_AS ElasPair* P;
-        int W, H, img, u0, v0;
+// One strip: columns u = x0 - 3 + lane, output rows [y0, y1).  EDGE: the
+// strip holds pixels whose descriptor is 0 (image sides, top or bottom rows);
+// interior strips skip that test.
+template <bool EDGE>
+__device__ __forceinline__ void desc_strip(const ELAS_CONST_AS ElasPair& P, int img, int x0, int y0, int y1) {
+    const int lane = threadIdx.x & 63;
+    const Xlane X{((lane + 63) & 63) * 4, ((lane + 1) & 63) * 4};
+    const int W = P.W, H = P.H;
+    const int u = x0 - 3 + lane;
+    // image bytes through a buffer resource: the row offset is scalar, the
+    // column a per-lane constant
+    const __amdgpu_buffer_rsrc_t I = elas_rsrc((void*)P.img[img], (uint32_t)W * (uint32_t)H);
+    const int uc = min(max(u, 0), W - 1);
+    auto load = [&](int R) -> uint32_t {
+        return __builtin_amdgcn_raw_buffer_load_b8(I, uc, min(max(R, 0), H - 1) * W, 0);
     };
-    auto tile_of = [&](int t) {
-        Tile T;
-        const int p = elas_find_pair(L.prefix, L.npairs, t);
-        T.P = &L.pairs[p];
-        T.W = T.P->W;
-        T.H = T.P->H;
-        const int local = t - L.prefix[p];
-        const int tx = (T.W + DT_W - 1) / DT_W, ty = (T.H + DT_H - 1) / DT_H;
-        T.img = local / (tx * ty);
-        const int r = local - T.img * tx * ty;
-        T.u0 = (r % tx) * DT_W;
-        T.v0 = (r / tx) * DT_H;
-        return T;
-    };
-    // Image window of a tile: every byte load is issued unconditionally from
-    // a clamped address (so they are all in flight together); out-of-image
-    // bytes are zeroed when the bytes are packed into dwords for LDS.
-    auto load_window = [&](const Tile& T, uint32_t (&b)[K][4]) {
-        const uint8_t* I = T.P->img[T.img];
+    // stores: byte offset 16 (y W + u); lanes and rows with nothing to store
+    // take an offset past the buffer (saturating add), which the hardware
+    // drops, so every store issues without a branch
+    const uint32_t range = (uint32_t)W * (uint32_t)H * 16u;  // W * H <= 2^27
+    const __amdgpu_buffer_rsrc_t D = elas_rsrc(P.desc[img], range);
+    const uint32_t col = (lane >= 3 && lane < 3 + ELAS_DESC_SW && u < W) ? 16u * (uint32_t)u : 0xfffffff0u;
+    const bool zcol = u < 3 || u > W - 4;
+
+    uint32_t q[8], nq[8];
+    #pragma unroll
+    for (int k = 0; k < 8; k++) q[k] = load(y0 - 3 + k);
+    // state: image bytes and H of the two previous rows; packed du/dv (du in
+    // byte 0, dv in byte 2) of the last four du rows, and the neighbourhood
+    // words A (du at u-2, u, u+2), B (du at u-1, u, u, u+1) and D (dv at u-1,
+    // u+1) of the last rows
+    int i1 = 0, i2 = 0, h1 = 0, h2 = 0;
+    uint32_t c1 = 0, c2 = 0, c3 = 0, c4 = 0, A1 = 0, A2 = 0, A3 = 0, B1 = 0, B2 = 0, D1 = 0, D2 = 0;
+    for (int jb = 0; jb < DS_STEPS; jb += 8) {
+        // the next block of rows (past the band: clamped rows nobody reads)
         #pragma unroll
-        for (int k = 0; k < K; k++) {
-            const int i = tid + 256 * k;
-            const int rr = i / DT_DW, dw = i - rr * DT_DW;
-            const int v = min(max(T.v0 - 3 + rr, 0), T.H - 1), ub = T.u0 - 8 + 4 * dw;
-            #pragma unroll
-            for (int j = 0; j < 4; j++) {
-                const int u = min(max(ub + j, 0), T.W - 1);
-                b[k][j] = elas_ld(I + (size_t)v * T.W + u);
-            }
-        }
-    };
-    auto store_window = [&](const Tile& T, const uint32_t (&b)[K][4]) {
+        for (int k = 0; k < 8; k++) nq[k] = load(y0 + 5 + jb + k);
         #pragma unroll
-        for (int k = 0; k < K; k++) {
-            const int i = tid + 256 * k;
-            const int rr = i / DT_DW, dw = i - rr * DT_DW;
-            const int v = T.v0 - 3 + rr, ub = T.u0 - 8 + 4 * dw;
-            const bool rowok = i < N && v >= 0 && v < T.H;
-            uint32_t x = 0;
-            #pragma unroll
-            for (int j = 0; j < 4; j++) {
-                const int u = ub + j;
-                x |= ((rowok && u >= 0 && u < T.W) ? b[k][j] : 0u) << (8 * j);
-            }
-            if (i < N) s_img[i] = x;
+        for (int k = 0; k < 8; k++) {
+            // image row R = y0 - 3 + j gives du/dv of row R - 1 and the
+            // descriptor of row y = R - 3 (du rows y-2 .. y+2 = c4 .. c,
+            // dv rows y-1 .. y+1 = c3 .. c1)
+            const int j = jb + k;
+            const int iR = (int)q[k];
+            const int hR = X.left(iR) + X.right(iR) + 2 * iR;
+            const int V = i2 + 2 * i1 + iR;
+            const int du = X.left(V) - X.right(V);
+            const int dv = h2 - hR;
+            const uint32_t c = sobel_byte2(perm((uint32_t)dv, (uint32_t)du, 0x05040100u));
+            const uint32_t cl = X.left(c), cr = X.right(c);
+            const uint32_t cl2 = X.left(cl), cr2 = X.right(cr);
+            const uint32_t A = (cr2 << 16) | perm(c, cl2, 0x0c0c0400u);  // du u-2, u, u+2
+            const uint32_t B = (cr << 24) | perm(c, cl, 0x0c040400u);    // du u-1, u, u, u+1
+            const uint32_t Dn = perm(cr, cl, 0x0c0c0602u);               // dv u-1, u+1
+            const int y = y0 - 6 + j;
+            uint4 o;
+            o.x = perm(A3, c4, 0x06050400u);
+            o.y = B2;
+            o.z = (c << 24) | A1;
+            o.w = perm(c1, perm(D2, c3, 0x0c050402u), 0x06020100u);
+            if (EDGE && (zcol || y < 3 || y > H - 4)) o = make_uint4(0u, 0u, 0u, 0u);
+            const uint32_t row = (j >= 6 && y < y1) ? (uint32_t)y * (uint32_t)W * 16u : 0xfffffff0u;
+            elas_st16_ranged(D, __builtin_elementwise_add_sat(row, col), o);
+            i2 = i1; i1 = iR; h2 = h1; h1 = hR;
+            c4 = c3; c3 = c2; c2 = c1; c1 = c;
+            A3 = A2; A2 = A1; A1 = A;
+            B2 = B1; B1 = B;
+            D2 = D1; D1 = Dn;
         }
-    };
-    // The next tile's window is loaded while this tile is computed and
-    // stored, so the load latency (long under the descriptor write stream)
-    // overlaps the work.
-    uint32_t bn[K][4];
-    int t = blockIdx.x;
-    Tile Tn;
-    if (t < L.total) {
-        Tn = tile_of(t);
-        load_window(Tn, bn);
+        #pragma unroll
+        for (int k = 0; k < 8; k++) q[k] = nq[k];
     }
-    for (; t < L.total; t += gridDim.x) {
-        const Tile T = Tn;
-        const auto& P = *T.P;
-        const int W = T.W, H = T.H, img = T.img, u0 = T.u0, v0 = T.v0;
-        store_window(T, bn);
-        if (t + (int)gridDim.x < L.total) {
-            Tn = tile_of(t + gridDim.x);
-            load_window(Tn, bn);
-        }
-        elas_lds_barrier();
-        // du (groups of four columns 4g..4g+3, g = 1..DT_NG) and dv
-        for (int i = tid; i < (DT_UR + DT_VR) * DT_NG; i += 256) {
-            const bool is_du = i < DT_UR * DT_NG;
-            const int j = is_du ? i : i - DT_UR * DT_NG;
-            const int row = j / DT_NG, g = j - row * DT_NG + 1;
-            if (is_du) {
-                // du row `row` (image row v) takes window rows row .. row + 2 (v-1 .. v+1)
-                const uint32_t* a = s_img + row * DT_DW + g;
-                const uint32_t* b = a + DT_DW;
-                const uint32_t* c = b + DT_DW;
-                const uint32_t Pa = a[-1], Oa = a[0], Na = a[1];
-                const uint32_t Pb = b[-1], Ob = b[0], Nb = b[1];
-                const uint32_t Pc = c[-1], Oc = c[0], Nc = c[1];
-                const uint32_t Vlo = lo_bytes(Oa) + lo_bytes(Oc) + (lo_bytes(Ob) << 1);  // cols c, c+2
-                const uint32_t Vhi = hi_bytes(Oa) + hi_bytes(Oc) + (hi_bytes(Ob) << 1);  // cols c+1, c+3
-                const uint32_t Vm1 = (Pa >> 24) + (Pc >> 24) + ((Pb >> 24) << 1);         // col c-1
-                const uint32_t Vp4 = (Na & 0xffu) + (Nc & 0xffu) + ((Nb & 0xffu) << 1);   // col c+4
-                const uint32_t E = sub16x2(Vm1 | (Vhi << 16), Vhi);                       // S(c), S(c+2)
-                const uint32_t O = sub16x2(Vlo, (Vlo >> 16) | (Vp4 << 16));               // S(c+1), S(c+3)
-                s_du[row * DT_DW + g] = pack_eo(sobel_byte2(E), sobel_byte2(O));
-            } else {
-                // dv row `row` (image row v) takes window rows row + 1 (v-1) and row + 3 (v+1)
-                uint32_t He[2], Ho[2];
-                #pragma unroll
-                for (int h = 0; h < 2; h++) {
-                    const uint32_t* a = s_img + (row + 1 + 2 * h) * DT_DW + g;
-                    const uint32_t Pw = a[-1], Ow = a[0], Nw = a[1];
-                    const uint32_t lo = lo_bytes(Ow), hi = hi_bytes(Ow);
-                    const uint32_t le = lo_bytes(__builtin_amdgcn_alignbyte(Ow, Pw, 3));  // cols c-1, c+1
-                    const uint32_t ro = lo_bytes(__builtin_amdgcn_alignbyte(Nw, Ow, 2));  // cols c+2, c+4
-                    He[h] = le + hi + (lo << 1);
-                    Ho[h] = lo + ro + (hi << 1);
-                }
-                s_dv[row * DT_DW + g] = pack_eo(sobel_byte2(sub16x2(He[0], He[1])), sobel_byte2(sub16x2(Ho[0], Ho[1])));
-            }
-        }
-        elas_lds_barrier();
-        // four adjacent pixels per thread: row pr, columns 4 pg .. 4 pg + 3
-        {
-            const int pr = tid / DT_GP, pg = tid % DT_GP;
-            const int G = pg + 2;
-            const int v = v0 + pr;
-            const uint32_t* du = s_du + pr * DT_DW + G;  // du rows v-2 .. v+2 = pr .. pr+4
-            const uint32_t* dv = s_dv + pr * DT_DW + G;  // dv rows v-1 .. v+1 = pr .. pr+2
-            const uint32_t um2 = du[0];
-            const uint32_t um1p = du[DT_DW - 1], um1 = du[DT_DW], um1n = du[DT_DW + 1];
-            const uint32_t u0p = du[2 * DT_DW - 1], u00 = du[2 * DT_DW], u0n = du[2 * DT_DW + 1];
-            const uint32_t up1p = du[3 * DT_DW - 1], up1 = du[3 * DT_DW], up1n = du[3 * DT_DW + 1];
-            const uint32_t up2 = du[4 * DT_DW];
-            const uint32_t vm1 = dv[0];
-            const uint32_t v0p = dv[DT_DW - 1], v00 = dv[DT_DW], v0n = dv[DT_DW + 1];
-            const uint32_t vp1 = dv[2 * DT_DW];
-            uint8_t* D = P.desc[img];
-            #pragma unroll
-            for (int k = 0; k < 4; k++) {
-                // a pixel's columns k-2 .. k+2 lie in (prev, own) for k < 2, in (own, next) otherwise
-                const int b = k < 2 ? 4 + k : k;
-                const uint32_t wl_m1 = k < 2 ? um1p : um1, wh_m1 = k < 2 ? um1 : um1n;
-                const uint32_t wl_0 = k < 2 ? u0p : u00, wh_0 = k < 2 ? u00 : u0n;
-                const uint32_t wl_p1 = k < 2 ? up1p : up1, wh_p1 = k < 2 ? up1 : up1n;
-                const uint32_t wl_v = k < 2 ? v0p : v00, wh_v = k < 2 ? v00 : v0n;
-                uint4 q;
-                q.x = __builtin_amdgcn_perm(um2, __builtin_amdgcn_perm(wh_m1, wl_m1, sel4(12, b - 2, b, b + 2)),
-                                            sel4(4 + k, 1, 2, 3));
-                q.y = __builtin_amdgcn_perm(wh_0, wl_0, sel4(b - 1, b, b, b + 1));
-                q.z = __builtin_amdgcn_perm(up2, __builtin_amdgcn_perm(wh_p1, wl_p1, sel4(b - 2, b, b + 2, 12)),
-                                            sel4(0, 1, 2, 4 + k));
-                const uint32_t w = __builtin_amdgcn_perm(wh_v, wl_v, sel4(12, b - 1, b + 1, 12));
-                q.w = __builtin_amdgcn_perm(vp1, __builtin_amdgcn_perm(vm1, w, sel4(4 + k, 1, 2, 3)), sel4(0, 1, 2, 4 + k));
-                const int u = u0 + 4 * pg + k;
-                if (!(u >= 3 && u <= W - 4 && v >= 3 && v <= H - 4)) q = make_uint4(0u, 0u, 0u, 0u);
-                if (u < W && v < H) elas_st(reinterpret_cast<uint4*>(D + ((size_t)v * W + u) * 16), q);
-            }
-        }
-        elas_lds_barrier();
+}
+
+__global__ __launch_bounds__(256) void k_descriptor(ElasLaunch L) {
+    const int stride = gridDim.x * 4;
+    for (int t0 = blockIdx.x * 4 + (threadIdx.x >> 6); t0 < L.total; t0 += stride) {
+        const int t = __builtin_amdgcn_readfirstlane(t0);  // one strip per wave
+        const int p = elas_find_pair(L.prefix, L.npairs, t);
+        const auto& P = L.pairs[p];
+        const int W = P.W, H = P.H;
+        const int ns = (W + ELAS_DESC_SW - 1) / ELAS_DESC_SW, nb = (H + ELAS_DESC_SH - 1) / ELAS_DESC_SH;
+        const int local = t - L.prefix[p];
+        const int img = local / (ns * nb);
+        const int r = local - img * ns * nb;
+        const int x0 = (r % ns) * ELAS_DESC_SW, y0 = (r / ns) * ELAS_DESC_SH;
+        const int y1 = min(y0 + ELAS_DESC_SH, H);
+        // output columns [x0, x0 + 58) meet u < 3 or u > W - 4, or rows
+        // [y0, y1) meet y < 3 or y > H - 4
+        if (x0 < 3 || x0 + ELAS_DESC_SW > W - 3 || y0 < 3 || y1 > H - 3)
+            desc_strip<true>(P, img, x0, y0, y1);
+        else
+            desc_strip<false>(P, img, x0, y0, y1);
     }
 }
 
 void elas_launch_descriptor(const ElasLaunch& L, int grid, hipStream_t s) {
+    grid = std::min(grid, (L.total + 3) / 4);  // four strips per workgroup
     hipLaunchKernelGGL(k_descriptor, dim3(grid), dim3(256), 0, s, L);
 }
