@@ -22,7 +22,7 @@
 #define ELAS_TRIREC 16
 
 #define ELAS_DESC_SW 58      // descriptor strip: output columns of a 64-lane wave
-#define ELAS_DESC_SH 34      // descriptor strip: output rows (34 + 6 rows walked)
+#define ELAS_DESC_SH 18      // descriptor strip: output rows (18 + 6 rows walked)
 #define ELAS_POST_TW 64      // post-filter tile: 64 x 16 output pixels
 #define ELAS_POST_TH 16
 #define ELAS_GAP_MAX_LINE 15872  // gap interpolation keeps a line in LDS (<= 62 KiB)

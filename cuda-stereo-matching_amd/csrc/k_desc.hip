@@ -32,7 +32,6 @@
 #include "elas_device.h"
 
 #define DS_STEPS (ELAS_DESC_SH + 6)  // image rows walked per band
-static_assert(DS_STEPS % 8 == 0, "rows are loaded in blocks of 8");
 static_assert(ELAS_DESC_SW == 58, "lanes 3 .. 60 are the output columns");
 
 typedef int16_t elas_s16x2 __attribute__((ext_vector_type(2)));
@@ -90,53 +89,50 @@ __device__ __forceinline__ void desc_strip(const ELAS_CONST_AS ElasPair& P, int 
     const uint32_t col = (lane >= 3 && lane < 3 + ELAS_DESC_SW && u < W) ? 16u * (uint32_t)u : 0xfffffff0u;
     const bool zcol = u < 3 || u > W - 4;
 
-    uint32_t q[8], nq[8];
+    // every image byte of the band is loaded before the first store: the
+    // vector memory counter completes in issue order, so a load issued after
+    // stores could only be waited for together with their write-back
+    uint32_t q[DS_STEPS];
     #pragma unroll
-    for (int k = 0; k < 8; k++) q[k] = load(y0 - 3 + k);
+    for (int k = 0; k < DS_STEPS; k++) q[k] = load(y0 - 3 + k);
     // state: image bytes and H of the two previous rows; packed du/dv (du in
     // byte 0, dv in byte 2) of the last four du rows, and the neighbourhood
     // words A (du at u-2, u, u+2), B (du at u-1, u, u, u+1) and D (dv at u-1,
     // u+1) of the last rows
     int i1 = 0, i2 = 0, h1 = 0, h2 = 0;
     uint32_t c1 = 0, c2 = 0, c3 = 0, c4 = 0, A1 = 0, A2 = 0, A3 = 0, B1 = 0, B2 = 0, D1 = 0, D2 = 0;
-    for (int jb = 0; jb < DS_STEPS; jb += 8) {
-        // the next block of rows (past the band: clamped rows nobody reads)
-        #pragma unroll
-        for (int k = 0; k < 8; k++) nq[k] = load(y0 + 5 + jb + k);
-        #pragma unroll
-        for (int k = 0; k < 8; k++) {
-            // image row R = y0 - 3 + j gives du/dv of row R - 1 and the
-            // descriptor of row y = R - 3 (du rows y-2 .. y+2 = c4 .. c,
-            // dv rows y-1 .. y+1 = c3 .. c1)
-            const int j = jb + k;
-            const int iR = (int)q[k];
-            const int hR = X.left(iR) + X.right(iR) + 2 * iR;
-            const int V = i2 + 2 * i1 + iR;
-            const int du = X.left(V) - X.right(V);
-            const int dv = h2 - hR;
-            const uint32_t c = sobel_byte2(perm((uint32_t)dv, (uint32_t)du, 0x05040100u));
-            const uint32_t cl = X.left(c), cr = X.right(c);
-            const uint32_t cl2 = X.left(cl), cr2 = X.right(cr);
-            const uint32_t A = (cr2 << 16) | perm(c, cl2, 0x0c0c0400u);  // du u-2, u, u+2
-            const uint32_t B = (cr << 24) | perm(c, cl, 0x0c040400u);    // du u-1, u, u, u+1
-            const uint32_t Dn = perm(cr, cl, 0x0c0c0602u);               // dv u-1, u+1
-            const int y = y0 - 6 + j;
-            uint4 o;
-            o.x = perm(A3, c4, 0x06050400u);
-            o.y = B2;
-            o.z = (c << 24) | A1;
-            o.w = perm(c1, perm(D2, c3, 0x0c050402u), 0x06020100u);
-            if (EDGE && (zcol || y < 3 || y > H - 4)) o = make_uint4(0u, 0u, 0u, 0u);
-            const uint32_t row = (j >= 6 && y < y1) ? (uint32_t)y * (uint32_t)W * 16u : 0xfffffff0u;
+    #pragma unroll
+    for (int j = 0; j < DS_STEPS; j++) {
+        // image row R = y0 - 3 + j gives du/dv of row R - 1 and the
+        // descriptor of row y = R - 3 (du rows y-2 .. y+2 = c4 .. c,
+        // dv rows y-1 .. y+1 = c3 .. c1)
+        const int iR = (int)q[j];
+        const int hR = X.left(iR) + X.right(iR) + 2 * iR;
+        const int V = i2 + 2 * i1 + iR;
+        const int du = X.left(V) - X.right(V);
+        const int dv = h2 - hR;
+        const uint32_t c = sobel_byte2(perm((uint32_t)dv, (uint32_t)du, 0x05040100u));
+        const uint32_t cl = X.left(c), cr = X.right(c);
+        const uint32_t cl2 = X.left(cl), cr2 = X.right(cr);
+        const uint32_t A = (cr2 << 16) | perm(c, cl2, 0x0c0c0400u);  // du u-2, u, u+2
+        const uint32_t B = (cr << 24) | perm(c, cl, 0x0c040400u);    // du u-1, u, u, u+1
+        const uint32_t Dn = perm(cr, cl, 0x0c0c0602u);               // dv u-1, u+1
+        const int y = y0 - 6 + j;
+        uint4 o;
+        o.x = perm(A3, c4, 0x06050400u);
+        o.y = B2;
+        o.z = (c << 24) | A1;
+        o.w = perm(c1, perm(D2, c3, 0x0c050402u), 0x06020100u);
+        if (EDGE && (zcol || y < 3 || y > H - 4)) o = make_uint4(0u, 0u, 0u, 0u);
+        if (j >= 6) {  // the first six rows only fill the registers
+            const uint32_t row = y < y1 ? (uint32_t)y * (uint32_t)W * 16u : 0xfffffff0u;
             elas_st16_ranged(D, __builtin_elementwise_add_sat(row, col), o);
-            i2 = i1; i1 = iR; h2 = h1; h1 = hR;
-            c4 = c3; c3 = c2; c2 = c1; c1 = c;
-            A3 = A2; A2 = A1; A1 = A;
-            B2 = B1; B1 = B;
-            D2 = D1; D1 = Dn;
         }
-        #pragma unroll
-        for (int k = 0; k < 8; k++) q[k] = nq[k];
+        i2 = i1; i1 = iR; h2 = h1; h1 = hR;
+        c4 = c3; c3 = c2; c2 = c1; c1 = c;
+        A3 = A2; A2 = A1; A1 = A;
+        B2 = B1; B1 = B;
+        D2 = D1; D1 = Dn;
     }
 }
 
