@@ -39,11 +39,6 @@ __device__ __forceinline__ void elas_st(uint4* p, const uint4& v) {
     *(ELAS_AS_GLOBAL elas_u32x4*)p = x;
 }
 
-// Workgroup barrier that orders LDS only.  __syncthreads() is also a
-// workgroup memory fence, which waits for every global load and store the
-// wave has in flight (vmcnt 0); here a wave's prefetched loads and finished
-// tile's stores stay in flight across the barrier.  The asm's memory clobber
-// keeps the compiler from moving memory accesses across it.
 typedef uint32_t elas_u32x4 __attribute__((ext_vector_type(4)));
 
 // Buffer resource over [base, base + bytes) for raw buffer stores (stride 0,
@@ -62,6 +57,15 @@ __device__ __forceinline__ void elas_st16_ranged(__amdgpu_buffer_rsrc_t r, uint3
     __builtin_amdgcn_raw_buffer_store_b128(v, r, (int)off, 0, 0);
 }
 
+// Workgroup barrier that orders LDS only.  __syncthreads() is also a
+// workgroup memory fence, which waits for every global load and store the
+// wave has in flight (vmcnt 0): a wave would stall at each tile boundary until
+// its result stores are written back.  The kernels of this library exchange
+// data between the threads of a workgroup only through LDS (global data a
+// workgroup reads is its own or was written by an earlier launch), so here a
+// wave's prefetched loads and finished tile's stores stay in flight across
+// the barrier.  The asm's memory clobber keeps the compiler from moving
+// memory accesses across it.
 __device__ __forceinline__ void elas_lds_barrier() {
     __asm__ __volatile__("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
 }

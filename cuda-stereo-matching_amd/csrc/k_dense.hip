@@ -145,12 +145,12 @@ __global__ __launch_bounds__(1024) void k_bin_scan(ElasLaunch L) {
         int sum = 0;
         for (int i = a; i < b; i++) sum += cnt[i];
         s_sum[tid] = sum;
-        __syncthreads();
+        elas_lds_barrier();
         for (int off = 1; off < 1024; off <<= 1) {
             const int x = tid >= off ? s_sum[tid - off] : 0;
-            __syncthreads();
+            elas_lds_barrier();
             s_sum[tid] += x;
-            __syncthreads();
+            elas_lds_barrier();
         }
         const int total = s_sum[1023];
         const bool over = total > P.bcap;
@@ -165,7 +165,7 @@ __global__ __launch_bounds__(1024) void k_bin_scan(ElasLaunch L) {
             cnt[n] = over ? 0 : total;
             if (over) elas_st(P.bovf, 1);
         }
-        __syncthreads();
+        elas_lds_barrier();
     }
 }
 
@@ -307,7 +307,7 @@ __global__ __launch_bounds__(256) void k_raster(ElasLaunch L) {
         const int k0 = elas_ld(P.toff[img] + local), k1 = elas_ld(P.toff[img] + local + 1);
         #pragma unroll
         for (int r = 0; r < 4; r++) s_tile[(tid >> 5) + 8 * r][tid & 31] = -1;
-        __syncthreads();
+        elas_lds_barrier();
         for (int kc = k0; kc < k1; kc += RS_CHUNK) {
             const int n = min(RS_CHUNK, k1 - kc);
             if (tid < n) {
@@ -318,7 +318,7 @@ __global__ __launch_bounds__(256) void k_raster(ElasLaunch L) {
                 dst[0] = r0; dst[1] = r1; dst[2] = r2; dst[3] = r3;
                 s_id[tid] = ti;
             }
-            __syncthreads();
+            elas_lds_barrier();
             for (int item = tid; item < n * ELAS_RASTER_T; item += 256) {
                 const int j = item >> 5, c = item & 31;
                 const int u = tx0 + c;
@@ -335,7 +335,7 @@ __global__ __launch_bounds__(256) void k_raster(ElasLaunch L) {
                 const int ti = s_id[j];
                 for (int v = vlo; v < vhi; v++) atomicMax(&s_tile[v - ty0][c], ti);
             }
-            __syncthreads();
+            elas_lds_barrier();
         }
         const int c = tid & 31;
         const int u = tx0 + c;
@@ -350,7 +350,7 @@ __global__ __launch_bounds__(256) void k_raster(ElasLaunch L) {
                 }
             }
         }
-        __syncthreads();
+        elas_lds_barrier();
     }
 }
 
@@ -467,7 +467,7 @@ __global__ __launch_bounds__(ELAS_DENSE_THREADS, 8) void k_dense(ElasLaunch L, E
                 b1v[ci] = elas_ld(reinterpret_cast<const uint4*>(I1 + ((size_t)v * W + u) * 16));
             }
         }
-        __syncthreads();
+        elas_lds_barrier();
         #pragma unroll
         for (int ci = 0; ci < 2; ci++) {
             const int gx = gx0 + wave + ci * ELAS_DENSE_WAVES;
@@ -542,7 +542,7 @@ __global__ __launch_bounds__(ELAS_DENSE_THREADS, 8) void k_dense(ElasLaunch L, E
                 elas_st(P.draw[img] + (size_t)v * W + u,
                         (valid && kmin < (KEY_BAD << 11)) ? (float)(kmin & 1023u) : -1.0f);
         }
-        __syncthreads();
+        elas_lds_barrier();
     }
 }
 

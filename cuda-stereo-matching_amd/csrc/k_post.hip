@@ -145,7 +145,7 @@ __global__ __launch_bounds__(256) void k_speckle_local(ElasLaunch L, ElasKParams
             s_d[i / PT_W][i % PT_W] = q[k];
             s_lab[i] = q[k] >= 0.0f ? i : -1;
         }
-        __syncthreads();
+        elas_lds_barrier();
         #pragma unroll
         for (int k = 0; k < PT_H * PT_W / 256; k++) {
             const int i = tid + 256 * k, r = i / PT_W, c = i % PT_W;
@@ -156,7 +156,7 @@ __global__ __launch_bounds__(256) void k_speckle_local(ElasLaunch L, ElasKParams
             if (r + 1 < PT_H && s_d[r + 1][c] >= 0.0f && fabsf(d - s_d[r + 1][c]) <= sim)
                 uf_union_lds(s_lab, i, i + PT_W);
         }
-        __syncthreads();
+        elas_lds_barrier();
         int32_t* lab = P.pbuf[T.img];
         int32_t* sz = lab + (size_t)W * H;
         #pragma unroll
@@ -172,7 +172,7 @@ __global__ __launch_bounds__(256) void k_speckle_local(ElasLaunch L, ElasKParams
             elas_st(lab + (size_t)v * W + u, g);
             elas_st(sz + (size_t)v * W + u, 0);
         }
-        __syncthreads();
+        elas_lds_barrier();
     }
 }
 
@@ -265,7 +265,7 @@ __global__ __launch_bounds__(256) void k_gap(ElasLaunch L, ElasKParams kp) {
         float* D = P.dout[img];
         const size_t es = COLS ? (size_t)W : 1, base = COLS ? (size_t)l : (size_t)l * W;
         for (int i = tid; i < len; i += 256) s_line[i] = D[base + i * es];
-        __syncthreads();
+        elas_lds_barrier();
         const int C = (len + 255) / 256;
         const int c0 = min(len, tid * C), c1 = min(len, c0 + C);
         int lastv = -1, firstv = 0x7fffffff;
@@ -276,15 +276,15 @@ __global__ __launch_bounds__(256) void k_gap(ElasLaunch L, ElasKParams kp) {
             }
         s_last[tid] = lastv;
         s_first[tid] = firstv;
-        __syncthreads();
+        elas_lds_barrier();
         // inclusive prefix max of s_last, inclusive suffix min of s_first
         for (int off = 1; off < 256; off <<= 1) {
             const int a = tid >= off ? s_last[tid - off] : -1;
             const int b = tid + off < 256 ? s_first[tid + off] : 0x7fffffff;
-            __syncthreads();
+            elas_lds_barrier();
             s_last[tid] = max(s_last[tid], a);
             s_first[tid] = min(s_first[tid], b);
-            __syncthreads();
+            elas_lds_barrier();
         }
         int Lv = tid > 0 ? s_last[tid - 1] : -1;
         const int Rin = tid < 255 ? s_first[tid + 1] : 0x7fffffff;
@@ -304,9 +304,9 @@ __global__ __launch_bounds__(256) void k_gap(ElasLaunch L, ElasKParams kp) {
             }
             q = e;
         }
-        __syncthreads();
+        elas_lds_barrier();
         for (int i = tid; i < len; i += 256) D[base + i * es] = s_line[i];
-        __syncthreads();
+        elas_lds_barrier();
     }
 }
 
@@ -395,7 +395,7 @@ __global__ __launch_bounds__(256) void k_sep(ElasLaunch L, ElasKParams kp, int f
                 if (i < N) s_in[r][c] = q[k];
             }
         }
-        __syncthreads();
+        elas_lds_barrier();
         // row pass over all staged rows (rows outside the image stay absent)
         for (int i = tid; i < RH * PT_W; i += 256) {
             const int r = i / PT_W, c = i - r * PT_W;
@@ -407,7 +407,7 @@ __global__ __launch_bounds__(256) void k_sep(ElasLaunch L, ElasKParams kp, int f
             if (v >= 0 && v < H) o = MEDIAN ? median_1d(w) : adaptive_1d(w);
             s_row[r][c] = o;
         }
-        __syncthreads();
+        elas_lds_barrier();
         for (int i = tid; i < PT_H * PT_W; i += 256) {
             const int r = i / PT_W, c = i - r * PT_W;
             const int u = T.u0 + c, v = T.v0 + r;
@@ -417,7 +417,7 @@ __global__ __launch_bounds__(256) void k_sep(ElasLaunch L, ElasKParams kp, int f
             for (int k = 0; k < 2 * PT_R + 1; k++) w[k] = s_row[r + k][c];
             elas_st(dst + (size_t)v * W + u, MEDIAN ? median_1d(w) : adaptive_1d(w));
         }
-        __syncthreads();
+        elas_lds_barrier();
     }
 }
 

@@ -95,7 +95,7 @@ __global__ __launch_bounds__(256) void k_support(ElasLaunch L, ElasKParams kp) {
             rBL = elas_ld(reinterpret_cast<const uint4*>(b1 - 32 + rowoff));
             rBR = elas_ld(reinterpret_cast<const uint4*>(b1 + 32 + rowoff));
         }
-        __syncthreads();
+        elas_lds_barrier();
         uint32_t k1 = 0xffffffffu, k2 = 0xffffffffu;
         if (ok) {
             // new pair at column cx(d) = u - d - 2 (left) / u + d + 2 (right),
@@ -142,7 +142,7 @@ __global__ __launch_bounds__(256) void k_support(ElasLaunch L, ElasKParams kp) {
             }
             elas_st((RIGHT ? P.can[1] : P.can[0]) + v_can * cw + u_can, result);
         }
-        __syncthreads();
+        elas_lds_barrier();
     }
 }
 
@@ -192,7 +192,7 @@ __global__ __launch_bounds__(256) void k_incon(ElasLaunch L, ElasKParams kp) {
                 if (xx >= side) { xx -= side; yy++; }
             }
         }
-        __syncthreads();
+        elas_lds_barrier();
         const int x = x0 + (tid & (INC_T - 1)), y = y0 + (tid / INC_T);
         if (x < cw && y < ch) {
             const int lx = x - x0, ly = y - y0;
@@ -229,7 +229,7 @@ __global__ __launch_bounds__(256) void k_incon(ElasLaunch L, ElasKParams kp) {
             }
             P.can[2][y * cw + x] = res;
         }
-        __syncthreads();
+        elas_lds_barrier();
     }
 }
 
@@ -293,7 +293,7 @@ __global__ __launch_bounds__(256) void k_redun(ElasLaunch L) {
             for (int k = 0; k < 16; k++)
                 if (li[k] >= 0) s_val[li[k]] = (int16_t)(q[k] < 0 ? RD_INV : q[k]);
         }
-        __syncthreads();
+        elas_lds_barrier();
         // flag byte: M(x) if valid(x) && ahead(x), else 0 (x is then never
         // removed); a lane takes 2 cells (consecutive lanes read consecutive
         // words: no bank conflicts)
@@ -320,7 +320,7 @@ __global__ __launch_bounds__(256) void k_redun(ElasLaunch L) {
                 s_flag[l * lenp + x0 + i] = (uint8_t)((d >= 0 && ahead) ? f : 0u);
             }
         }
-        __syncthreads();
+        elas_lds_barrier();
         {
             // The recurrence runs on the scalar unit, one line per wave: nh has
             // bit j-1 set when x-j was kept, and x is removed when its flag
@@ -367,7 +367,7 @@ __global__ __launch_bounds__(256) void k_redun(ElasLaunch L) {
                 }
             }
         }
-        __syncthreads();
+        elas_lds_barrier();
         {
             int a = tid / inner, b = tid - a * inner;
             for (int e = tid; e < n; e += 256) {
@@ -381,7 +381,7 @@ __global__ __launch_bounds__(256) void k_redun(ElasLaunch L) {
                 }
             }
         }
-        __syncthreads();
+        elas_lds_barrier();
     }
 }
 
@@ -412,10 +412,10 @@ __global__ __launch_bounds__(256) void k_compact(ElasLaunch L) {
         int acc = 0;
         for (int j = tid; j < b; j += 256) acc += P.blkcnt[j];
         s_red[tid] = acc;
-        __syncthreads();
+        elas_lds_barrier();
         for (int s = 128; s > 0; s >>= 1) {
             if (tid < s) s_red[tid] += s_red[tid + s];
-            __syncthreads();
+            elas_lds_barrier();
         }
         const int base = s_red[0];
         const int idx = b * 256 + tid;
@@ -424,7 +424,7 @@ __global__ __launch_bounds__(256) void k_compact(ElasLaunch L) {
         const uint64_t mask = __ballot(valid);
         const int before = __popcll(mask & ((1ull << lane) - 1ull));
         if (lane == 0) s_wave[wave] = __popcll(mask);
-        __syncthreads();
+        elas_lds_barrier();
         int off = base + before;
         for (int w = 0; w < wave; w++) off += s_wave[w];
         if (valid) {
@@ -434,7 +434,7 @@ __global__ __launch_bounds__(256) void k_compact(ElasLaunch L) {
             reinterpret_cast<float2*>(P.sp[1])[off] = make_float2(pu - (float)d, pv);
         }
         if (b == P.nblk - 1 && tid == 0) P.nsp[0] = base + s_wave[0] + s_wave[1] + s_wave[2] + s_wave[3];
-        __syncthreads();
+        elas_lds_barrier();
     }
 }
 
