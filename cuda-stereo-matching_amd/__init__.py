@@ -99,6 +99,11 @@ SIGNATURES = {
     "elas_hip_stage_times": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_float), ctypes.c_int32]),
     "elas_hip_triangulate": (ctypes.c_int, [_P, ctypes.c_int32, _P, ctypes.c_int64,
                                             ctypes.POINTER(ctypes.c_int64)]),
+    "elas_hip_set_autotune": (ctypes.c_int, [_P, ctypes.c_int32]),
+    "elas_hip_set_dense_tiles": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32]),
+    "elas_hip_dense_tiles": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                            ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
+                                            ctypes.POINTER(ctypes.c_float)]),
 }
 
 _lib = None
@@ -308,6 +313,24 @@ class Elas:
         """Replay phases A and B as HIP graphs per shape bucket (default), or
         launch kernel by kernel with per-stage timing events."""
         self._check(self._lib.elas_hip_set_graphs(self._ctx, 1 if on else 0))
+
+    def set_autotune(self, on=True):
+        """Time every dense tiling on the first pair of each new (W, H,
+        disp_max) and keep the fastest (default), or use the heuristic."""
+        self._check(self._lib.elas_hip_set_autotune(self._ctx, 1 if on else 0))
+
+    def set_dense_tiles(self, cells_per_wave=0, steps_per_chunk=0):
+        """Force the dense tiling (cells per wave 1 or 2, steps per chunk;
+        0 = the heuristic's); (0, 0) restores automatic tiling."""
+        self._check(self._lib.elas_hip_set_dense_tiles(self._ctx, int(cells_per_wave), int(steps_per_chunk)))
+
+    def dense_tiles(self, W, H, disp_max):
+        """(cells per wave, steps per chunk, tuned microseconds or -1) a
+        shape would be matched with now."""
+        c, s, us = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_float()
+        self._check(self._lib.elas_hip_dense_tiles(self._ctx, int(W), int(H), int(disp_max), ctypes.byref(c),
+                                                   ctypes.byref(s), ctypes.byref(us)))
+        return c.value, s.value, us.value
 
     def set_pipeline(self, slots=3, chunk=0):
         """Batches are cut into chunks of `chunk` pairs (0: one chunk) that

@@ -9,6 +9,7 @@
 
 #include "elas_engine.h"
 #include "elas_hip.h"
+#include "elas_launch.h"
 #include "elas_triangle.h"
 
 extern "C" {
@@ -89,6 +90,7 @@ int elas_hip_create(const elas_params* params, int device, elas_ctx** out) {
     if (!c) return ELAS_ERR_OUT_OF_MEMORY;
     c->device = device;
     c->num_cu = prop.multiProcessorCount;
+    c->max_lds = std::min<size_t>(prop.sharedMemPerBlock, 163840);
     elas_params p;
     if (params) p = *params;
     else elas_params_default(&p);
@@ -103,6 +105,7 @@ int elas_hip_create(const elas_params* params, int device, elas_ctx** out) {
         delete c;
         return ELAS_ERR_HIP;
     }
+    if (elas_dense_allow_lds(c->max_lds) != 0) c->max_lds = 65536;
     st = elas_hip_set_pipeline(c, 3, 0);
     if (st != ELAS_OK) {
         elas_hip_destroy(c);
@@ -170,6 +173,7 @@ int elas_hip_set_params(elas_ctx* c, const elas_params* p) {
     c->prm = *p;
     c->kp = kp;
     c->param_gen++;
+    c->dense_tuned.clear();  // tilings depend on grid_size
     return ELAS_OK;
 }
 
@@ -259,6 +263,37 @@ int elas_hip_shape_info(const elas_ctx* c, elas_shape_info* info) {
 int elas_hip_stage_times(const elas_ctx* c, float* ms, int32_t n) {
     if (!c || !ms || c->server) return ELAS_ERR_INVALID_ARGUMENT;
     for (int i = 0; i < n && i < ELAS_T_COUNT; i++) ms[i] = c->last ? c->last->times[i] : 0.0f;
+    return ELAS_OK;
+}
+
+int elas_hip_set_autotune(elas_ctx* c, int32_t on) {
+    if (!c) return ELAS_ERR_INVALID_ARGUMENT;
+    ELAS_NOT_SERVING(c);
+    c->autotune = on != 0;
+    return ELAS_OK;
+}
+
+int elas_hip_set_dense_tiles(elas_ctx* c, int32_t cells_per_wave, int32_t steps_per_chunk) {
+    if (!c) return ELAS_ERR_INVALID_ARGUMENT;
+    ELAS_NOT_SERVING(c);
+    if (cells_per_wave < 0 || cells_per_wave > 2 || steps_per_chunk < 0 ||
+        (cells_per_wave == 0 && steps_per_chunk != 0)) {
+        c->err = "need cells_per_wave in {0, 1, 2}, steps_per_chunk >= 0, and both 0 for automatic tiling";
+        return ELAS_ERR_INVALID_ARGUMENT;
+    }
+    c->force_cpw = cells_per_wave;
+    c->force_spc = steps_per_chunk;
+    return ELAS_OK;
+}
+
+int elas_hip_dense_tiles(elas_ctx* c, int32_t W, int32_t H, int32_t disp_max, int32_t* cells_per_wave,
+                         int32_t* steps_per_chunk, float* tuned_us) {
+    if (!c || W < 1 || H < 1 || disp_max < 0) return ELAS_ERR_INVALID_ARGUMENT;
+    ELAS_NOT_SERVING(c);
+    const ElasDenseTiles t = c->dense_tiles(elas_dims(c->prm, W, H, disp_max), W, H, disp_max);
+    if (cells_per_wave) *cells_per_wave = t.cpw;
+    if (steps_per_chunk) *steps_per_chunk = t.spc;
+    if (tuned_us) *tuned_us = t.us;
     return ELAS_OK;
 }
 

@@ -11,7 +11,7 @@
 
 #define ELAS_MAX_P 16        // prior table entries: |d - d_plane| in [0, 15]
 #define ELAS_DENSE_ROWS 3    // image rows per dense tile (a wave = 20 px x 3 rows)
-#define ELAS_DENSE_WAVES 8   // waves per dense tile; a wave covers dense_cpw cells
+#define ELAS_DENSE_WAVES 8   // waves per dense workgroup; a wave covers dcpw cells per step
 #define ELAS_DENSE_THREADS (64 * ELAS_DENSE_WAVES)
 #define ELAS_RASTER_T 32     // raster tile edge in pixels
 #define ELAS_MAX_DISP 511    // disp_max envelope (dense key packs d in 10 bits)
@@ -44,6 +44,7 @@ struct ElasPair {
     int32_t cw, ch, gw, gh;
     int32_t words;           // 32-bit words of a cell's disparity bitmask: (dmax+32)/32
     int32_t step, gsize;     // candidate_stepsize, grid_size
+    int32_t dcpw, dspc;      // dense tiling: grid cells per wave per step (1, 2), steps per chunk
     int32_t ntri[2];         // triangles of the left / right mesh (host-known)
     int32_t nblk;            // 256-cell blocks of the support lattice
     const uint8_t* img[2];   // [H][W]
@@ -85,7 +86,6 @@ struct ElasKParams {
     int32_t match_texture;
     int32_t key_bias;        // added to dense costs so keys stay non-negative
     int32_t all_p_negative;  // every prior P(k), k <= plane_radius, is < 0
-    int32_t dense_cpw;       // grid cells per wave in the dense tile (1 or 2)
     int32_t P[ELAS_MAX_P];
     // post-processing (off unless enabled)
     float speckle_sim;
@@ -109,9 +109,10 @@ struct ElasLaunch {
     int32_t total;
 };
 
-// Stage ids (one prefix table per stage).
+// Stage ids (one prefix table per stage).  S_DENSE holds the dense units of
+// the pairs tiled with one cell per wave, S_DENSE2 those with two.
 enum ElasStageId {
     S_DESC = 0, S_SUPPORT, S_INCON, S_REDUN_H, S_REDUN_V, S_BLOCKS, S_GRID_CELLS,
     S_PLANE, S_RASTER, S_DENSE, S_PIXELS, S_GRID_FILL, S_POST_TILES, S_POST_ROWS, S_POST_COLS,
-    S_BIN_CLEAR, S_BIN_SCAN, S_NUM_STAGES
+    S_BIN_CLEAR, S_BIN_SCAN, S_DENSE2, S_NUM_STAGES
 };

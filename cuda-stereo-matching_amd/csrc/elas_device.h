@@ -57,6 +57,25 @@ __device__ __forceinline__ void elas_st16_ranged(__amdgpu_buffer_rsrc_t r, uint3
     __builtin_amdgcn_raw_buffer_store_b128(v, r, (int)off, 0, 0);
 }
 
+// Offset past any buffer of this library (every range is below 2^31 bytes):
+// a load there returns 0 and a store there is dropped.
+#define ELAS_OOR 0xfffffff0u
+
+// 16- and 4-byte buffer loads at byte offset off (ELAS_OOR: 0, no access),
+// and a 4-byte buffer store (ELAS_OOR: dropped).  Issued without a branch,
+// they keep the compiler's count of outstanding memory operations exact, so
+// a wait for one load need not wait for everything issued before it.
+__device__ __forceinline__ uint4 elas_ld16_ranged(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    const elas_u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, 0, 0);
+    return make_uint4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ uint32_t elas_ld4_ranged(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    return __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0);
+}
+__device__ __forceinline__ void elas_st4_ranged(__amdgpu_buffer_rsrc_t r, uint32_t off, uint32_t v) {
+    __builtin_amdgcn_raw_buffer_store_b32(v, r, (int)off, 0, 0);
+}
+
 // Workgroup barrier that orders LDS only.  __syncthreads() is also a
 // workgroup memory fence, which waits for every global load and store the
 // wave has in flight (vmcnt 0): a wave would stall at each tile boundary until

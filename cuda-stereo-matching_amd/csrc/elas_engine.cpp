@@ -38,12 +38,30 @@ using clk = std::chrono::steady_clock;
 
 size_t al(size_t x) { return (x + 255) & ~size_t(255); }
 
+int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
+
 int dense_rows(int gs) { return std::max(1, std::min(64 / gs, gs)); }
 
-// LDS of a dense tile: R rows x (waves*cpw cells x gs + dmax columns, padded
-// by up to 15 to the bank-friendly stride) x 16 B
+// LDS of a dense unit: two rings (desc0, desc1) of R rows, each row the
+// smallest power of two >= waves*cpw*gs + dmax slots of 16 B
 size_t dense_lds(int gs, int dmax, int cpw) {
-    return (size_t)dense_rows(gs) * (size_t)(ELAS_DENSE_WAVES * cpw * gs + dmax + 15) * 16;
+    size_t rw = 1;
+    while (rw < (size_t)(ELAS_DENSE_WAVES * cpw * gs + dmax)) rw <<= 1;
+    return 2 * (size_t)dense_rows(gs) * rw * 16;
+}
+
+// steps of a band and work units (band x chunk) of one pair's dense launch
+int dense_segs(int gw, int cpw) { return cdiv(gw, ELAS_DENSE_WAVES * cpw); }
+int dense_units(const ElasDims& d, int gs, int cpw, int spc) {
+    return d.gh * cdiv(gs, dense_rows(gs)) * cdiv(dense_segs(d.gw, cpw), spc);
+}
+
+// Grid of a dense launch: the units rounded up to a multiple of 8 (so that a
+// workgroup's XCD is its index mod 8 in every grid-stride step), at most 4
+// workgroups of 512 lanes per CU.
+int dense_grid(int units, int num_cu) {
+    const int cap = std::max(1, num_cu) * 4;
+    return std::min((units + 7) & ~7, (cap + 7) & ~7);
 }
 
 // LDS of a support workgroup: rows v-2 and v+2 over 63*step + dmax + 5
@@ -51,8 +69,6 @@ size_t dense_lds(int gs, int dmax, int cpw) {
 size_t support_lds(int step, int dmax, bool right) {
     return 2 * 16 * (size_t)(63 * step + (right ? 2 : 1) * dmax + 5);
 }
-
-int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
 
 bool post_scratch(const elas_params& p) { return p.speckle_size > 0 || p.filter_adaptive_mean || p.filter_median; }
 
@@ -152,7 +168,6 @@ int elas_make_kparams(const elas_params* p, ElasKParams* kp, std::string* err) {
     if (pmin < -(1 << 18)) return bad("prior too strong (beta too small)");
     kp->key_bias = -pmin;
     kp->all_p_negative = allneg ? 1 : 0;
-    kp->dense_cpw = 1;
     kp->speckle_sim = p->speckle_sim_threshold;
     kp->speckle_size = p->speckle_size;
     kp->ipol_gap = p->ipol_gap_width;
@@ -186,6 +201,7 @@ void elas_ctx::slot_free(ElasSlot& S) {
     if (S.blist) (void)hipFree(S.blist);
     if (S.pin) (void)hipHostFree(S.pin);
     if (S.blist_pin) (void)hipHostFree(S.blist_pin);
+    if (S.tune) (void)hipFree(S.tune);
     if (S.stream) (void)hipStreamDestroy(S.stream);
     if (S.stream2) (void)hipStreamDestroy(S.stream2);
 }
@@ -202,8 +218,8 @@ int elas_ctx::slot_need(const ElasPairSpec* sp, int n, size_t* arena_need, size_
             return fail(ELAS_ERR_INVALID_ARGUMENT, "pair " + std::to_string(i) + ": null buffer");
         if (s.dmax < std::max(prm.disp_min, 0) || s.dmax > ELAS_MAX_DISP)
             return fail(ELAS_ERR_INVALID_ARGUMENT, "pair " + std::to_string(i) + ": disp_max out of range");
-        if (dense_lds(prm.grid_size, s.dmax, 1) > 65536)
-            return fail(ELAS_ERR_UNSUPPORTED, "grid_size/disp_max exceed the dense LDS tile");
+        if (dense_lds(prm.grid_size, s.dmax, 1) > max_lds)
+            return fail(ELAS_ERR_UNSUPPORTED, "grid_size/disp_max exceed the dense LDS rings");
         if (support_lds(prm.candidate_stepsize, s.dmax, true) > 65536)
             return fail(ELAS_ERR_UNSUPPORTED, "candidate_stepsize/disp_max exceed the support LDS rows");
         if (prm.ipol_gap_width > 0 && std::max(s.W, s.H) > ELAS_GAP_MAX_LINE)
@@ -291,9 +307,7 @@ int elas_ctx::slot_prepare(ElasSlot& S, const ElasPairSpec* sp, int n) {
         ptri += 2 * 12 * (size_t)S.dd[i].tricap;
         S.maxd = std::max(S.maxd, sp[i].dmax);
     }
-    S.cpw = dense_lds(prm.grid_size, S.maxd, 2) <= 65536 ? 2 : 1;
     S.kpl = kp;
-    S.kpl.dense_cpw = S.cpw;
     S.pin_sp = psp;
     S.pin_tri = ptri;
     // device tables (phase A / phase B) and their pinned mirrors
@@ -319,6 +333,7 @@ int elas_ctx::slot_prepare(ElasSlot& S, const ElasPairSpec* sp, int n) {
     S.htri = (int32_t*)(PIN + 2 * S.tab_one + al(psp));
 
     S.pairs.assign(n, ElasPair());
+    S.dense_lds[0] = S.dense_lds[1] = 0;
     S.prefix.assign((size_t)S_NUM_STAGES * (n + 1), 0);
     S.nsp.assign(n, 0);
     S.ntri0.assign(n, 0);
@@ -332,6 +347,10 @@ int elas_ctx::slot_prepare(ElasSlot& S, const ElasPairSpec* sp, int n) {
         P.W = s.W; P.H = s.H; P.dmin = prm.disp_min; P.dmax = s.dmax;
         P.cw = d.cw; P.ch = d.ch; P.gw = d.gw; P.gh = d.gh; P.words = d.words;
         P.step = prm.candidate_stepsize; P.gsize = prm.grid_size; P.nblk = d.nblk;
+        const ElasDenseTiles dt = dense_tiles(d, s.W, s.H, s.dmax);
+        P.dcpw = dt.cpw;
+        P.dspc = dt.spc;
+        S.dense_lds[dt.cpw - 1] = std::max(S.dense_lds[dt.cpw - 1], dense_lds(prm.grid_size, s.dmax, dt.cpw));
         char* c = A + S.base[i];
         auto take = [&](size_t bytes) { char* r = c; c += al(bytes); return r; };
         if (!s.dev_in) { P.img[0] = (uint8_t*)take(d.npx); P.img[1] = (uint8_t*)take(d.npx); }
@@ -375,8 +394,7 @@ int elas_ctx::slot_prepare(ElasSlot& S, const ElasPairSpec* sp, int n) {
 namespace {
 
 // workgroup tiles of stage st for one pair
-int stage_tiles(const elas_params& prm, const ElasPairSpec& s, const ElasDims& d, const ElasPair& P, int cpw,
-                int st) {
+int stage_tiles(const elas_params& prm, const ElasPairSpec& s, const ElasDims& d, const ElasPair& P, int st) {
     const int gs = prm.grid_size, R = dense_rows(gs);
     switch (st) {
         case S_DESC: return cdiv(s.W, ELAS_DESC_SW) * cdiv(s.H, ELAS_DESC_SH) * 2;
@@ -389,7 +407,8 @@ int stage_tiles(const elas_params& prm, const ElasPairSpec& s, const ElasDims& d
         case S_GRID_FILL: return cdiv(2 * (long)d.ncell * d.words, 256);
         case S_PLANE: return cdiv((long)P.ntri[0] + P.ntri[1], 64);
         case S_RASTER: return 2 * P.ntx * P.nty;
-        case S_DENSE: return 2 * d.gh * cdiv(gs, R) * cdiv(d.gw, ELAS_DENSE_WAVES * cpw);
+        case S_DENSE: return P.dcpw == 1 ? dense_units(d, gs, 1, P.dspc) : 0;
+        case S_DENSE2: return P.dcpw == 2 ? dense_units(d, gs, 2, P.dspc) : 0;
         case S_PIXELS: return cdiv((long)d.npx, 256);
         case S_POST_TILES: return 2 * cdiv(s.W, ELAS_POST_TW) * cdiv(s.H, ELAS_POST_TH);
         case S_POST_ROWS: return 2 * s.H;
@@ -405,7 +424,7 @@ void build_prefix(const elas_params& prm, ElasSlot& S) {
     for (int st = 0; st < S_NUM_STAGES; st++) {
         int32_t* pf = S.prefix.data() + (size_t)st * (n + 1);
         pf[0] = 0;
-        for (int i = 0; i < n; i++) pf[i + 1] = pf[i] + stage_tiles(prm, S.specs[i], S.dd[i], S.pairs[i], S.cpw, st);
+        for (int i = 0; i < n; i++) pf[i + 1] = pf[i] + stage_tiles(prm, S.specs[i], S.dd[i], S.pairs[i], st);
     }
 }
 
@@ -634,7 +653,7 @@ int elas_ctx::enqueue_b(ElasSlot& S, bool timed) {
     auto ev = [&](int i) { return timed ? hipEventRecord(S.ev[i], s1) : hipSuccess; };
     HIPCHECK(hipMemcpyAsync(S.dpairs[1], S.hpairs[1], sizeof(ElasPair) * n, hipMemcpyHostToDevice, s1));
     HIPCHECK(hipMemcpyAsync(S.dprefix[1], S.hprefix[1], sizeof(int32_t) * S.prefix.size(), hipMemcpyHostToDevice, s1));
-    const int cap256 = std::max(1, num_cu) * 8, cap_dense = std::max(1, num_cu) * (2048 / ELAS_DENSE_THREADS);
+    const int cap256 = std::max(1, num_cu) * 8;
     auto g = [&](const ElasLaunch& L, int cap) { return std::min(L.total, cap); };
     HIPCHECK(ev(15));
     {
@@ -652,8 +671,10 @@ int elas_ctx::enqueue_b(ElasSlot& S, bool timed) {
     }
     { ElasLaunch L = launch_of(S, 1, S_RASTER); if (L.total) elas_launch_raster(L, g(L, cap256), s1); }
     HIPCHECK(ev(7));
-    { ElasLaunch L = launch_of(S, 1, S_DENSE);
-      if (L.total) elas_launch_dense(L, S.kpl, g(L, cap_dense), dense_lds(prm.grid_size, S.maxd, S.cpw), s1); }
+    for (int c = 1; c <= 2; c++) {
+        ElasLaunch L = launch_of(S, 1, c == 1 ? S_DENSE : S_DENSE2);
+        if (L.total) elas_launch_dense(L, S.kpl, c, dense_grid(L.total, num_cu), S.dense_lds[c - 1], s1);
+    }
     HIPCHECK(ev(8));
     { ElasLaunch L = launch_of(S, 1, S_PIXELS); if (L.total) elas_launch_lr(L, kp, g(L, cap256), s1); }
     if (debug) { ElasLaunch L = launch_of(S, 1, S_PIXELS); if (L.total) elas_launch_mean(L, g(L, cap256), s1); }
@@ -780,6 +801,7 @@ int elas_ctx::slot_graph(ElasSlot& S, int which, hipGraphExec_t* out) {
         key.push_back(S.specs[i].H);
         key.push_back(S.specs[i].dmax);
         key.push_back((S.specs[i].dev_in ? 1 : 0) | (S.specs[i].dev_out ? 2 : 0));
+        key.push_back(S.pairs[i].dcpw * 65536 + S.pairs[i].dspc);
     }
     ElasGraphEntry* e = nullptr;
     for (auto& x : S.graphs)
@@ -872,8 +894,119 @@ int elas_ctx::slot_finish(ElasSlot& S, int32_t* statuses) {
     S.times[ELAS_T_BIN_JOB] = bj;
     if (statuses)
         for (int i = 0; i < S.n; i++) statuses[i] = S.status[i];
+    if (autotune && force_cpw == 0)
+        for (int i = 0; i < S.n; i++)
+            if (S.status[i] == ELAS_OK && !dense_tuned.count({S.specs[i].W, S.specs[i].H, S.specs[i].dmax})) {
+                const int r = tune_dense(S, i);
+                if (r != ELAS_OK) return r;
+            }
     S.phase = SLOT_IDLE;
     last = &S;
+    return ELAS_OK;
+}
+
+// ---- dense tiling ------------------------------------------------------------
+
+// Tiling of a pair shape: the tuned one; else two cells per wave where the
+// rings fit in 64 KiB, with bands cut into chunks so that the units fill about
+// four workgroups per CU.  force_cpw / force_spc (> 0) override (two cells
+// per wave still only where they fit).
+ElasDenseTiles elas_ctx::dense_tiles(const ElasDims& d, int W, int H, int dmax) const {
+    const int gs = prm.grid_size;
+    if (force_cpw == 0) {
+        auto it = dense_tuned.find({W, H, dmax});
+        if (it != dense_tuned.end()) return it->second;
+    }
+    ElasDenseTiles t;
+    const size_t lds2 = dense_lds(gs, dmax, 2);
+    t.cpw = (force_cpw == 0 || force_cpw == 2) && lds2 <= std::min<size_t>(max_lds, 65536) ? 2 : 1;
+    const int segs = dense_segs(d.gw, t.cpw);
+    if (force_spc > 0) {
+        t.spc = std::min(force_spc, segs);
+    } else {
+        const long bands = (long)d.gh * cdiv(gs, dense_rows(gs));
+        const long want = 4L * std::max(1, num_cu);
+        const int chunks = (int)std::max(1L, std::min<long>(segs, (want + bands - 1) / bands));
+        t.spc = cdiv(segs, chunks);
+    }
+    return t;
+}
+
+// Times every candidate tiling of pair i's dense launch and keeps the fastest
+// for the pair's shape.  The inputs of the call that just finished (its
+// descriptors, priors and candidate masks) are still in the arena, and every
+// tiling recomputes the same disparities from them, so the results of the
+// call are unchanged.  Candidates: 1 or 2 cells per wave (where the rings
+// fit) x every distinct number of chunks per band.
+int elas_ctx::tune_dense(ElasSlot& S, int i) {
+    const ElasPairSpec& sp = S.specs[i];
+    const ElasDims& d = S.dd[i];
+    const int gs = prm.grid_size;
+    std::vector<ElasDenseTiles> cand;
+    for (int cpw = 1; cpw <= 2; cpw++) {
+        // two cells per wave only within 64 KiB (one cell always runs)
+        if (dense_lds(gs, sp.dmax, cpw) > (cpw == 1 ? max_lds : std::min<size_t>(max_lds, 65536))) continue;
+        const int segs = dense_segs(d.gw, cpw);
+        int prev = -1;
+        for (int spc = 1; spc <= segs; spc++) {
+            const int chunks = cdiv(segs, spc);
+            if (chunks == prev) continue;  // same units as a shorter chunk
+            prev = chunks;
+            ElasDenseTiles t;
+            t.cpw = cpw;
+            t.spc = spc;
+            cand.push_back(t);
+        }
+    }
+    if (cand.empty()) return ELAS_OK;
+    const size_t nc = cand.size();
+    const size_t pre_off = al(sizeof(ElasPair) * nc);
+    const size_t need = pre_off + al(8 * nc);
+    HIPCHECK(hipSetDevice(device));
+    if (S.tune_bytes < need) {
+        if (S.tune) (void)hipFree(S.tune);
+        S.tune = nullptr;
+        S.tune_bytes = 0;
+        HIPCHECK(hipMalloc(&S.tune, need));
+        S.tune_bytes = need;
+    }
+    std::vector<char> h(need, 0);
+    std::vector<int> units(nc);
+    for (size_t k = 0; k < nc; k++) {
+        ElasPair P = S.pairs[i];
+        P.dcpw = cand[k].cpw;
+        P.dspc = cand[k].spc;
+        memcpy(h.data() + k * sizeof(ElasPair), &P, sizeof(P));
+        units[k] = dense_units(d, gs, P.dcpw, P.dspc);
+        int32_t* pf = (int32_t*)(h.data() + pre_off) + 2 * k;
+        pf[0] = 0;
+        pf[1] = units[k];
+    }
+    hipStream_t s1 = S.stream;
+    HIPCHECK(hipMemcpyAsync(S.tune, h.data(), need, hipMemcpyHostToDevice, s1));
+    const int reps = 3;
+    for (size_t k = 0; k < nc; k++) {
+        ElasLaunch L;
+        L.pairs = (const ElasPair*)S.tune + k;
+        L.prefix = (const int32_t*)((const char*)S.tune + pre_off) + 2 * k;
+        L.npairs = 1;
+        L.total = units[k];
+        const size_t lds = dense_lds(gs, sp.dmax, cand[k].cpw);
+        elas_launch_dense(L, S.kpl, cand[k].cpw, dense_grid(L.total, num_cu), lds, s1);  // warm
+        HIPCHECK(hipEventRecord(S.ev[16], s1));
+        for (int r = 0; r < reps; r++) elas_launch_dense(L, S.kpl, cand[k].cpw, dense_grid(L.total, num_cu), lds, s1);
+        HIPCHECK(hipGetLastError());
+        HIPCHECK(hipEventRecord(S.ev[17], s1));
+        HIPCHECK(hipEventSynchronize(S.ev[17]));
+        float ms = 0.f;
+        HIPCHECK(hipEventElapsedTime(&ms, S.ev[16], S.ev[17]));
+        cand[k].us = ms * 1e3f / reps;
+    }
+    size_t best = 0;
+    for (size_t k = 1; k < nc; k++)
+        if (cand[k].us < cand[best].us) best = k;
+    dense_tuned[{sp.W, sp.H, sp.dmax}] = cand[best];
+    dense_tunings++;
     return ELAS_OK;
 }
 
@@ -1028,14 +1161,22 @@ int elas_ctx::time_kernel(int kernel, int reps, float* avg_ms) {
     if (reps < 1 || reps > 100000 || !avg_ms) return fail(ELAS_ERR_INVALID_ARGUMENT, "reps must be in [1, 100000]");
     ElasSlot& S = *last;
     HIPCHECK(hipSetDevice(device));
-    const int cap256 = std::max(1, num_cu) * 8, cap_dense = std::max(1, num_cu) * (2048 / ELAS_DENSE_THREADS);
+    const int cap256 = std::max(1, num_cu) * 8;
     auto g = [&](const ElasLaunch& L, int cap) { return std::min(L.total, cap); };
     ElasLaunch L;
+    int dense_cpw = 1;
     switch (kernel) {
         case ELAS_K_DESCRIPTOR: L = launch_of(S, 0, S_DESC); break;
         case ELAS_K_SUPPORT_LEFT:
         case ELAS_K_SUPPORT_RIGHT: L = launch_of(S, 0, S_SUPPORT); break;
-        case ELAS_K_DENSE: L = launch_of(S, 1, S_DENSE); break;
+        case ELAS_K_DENSE:
+            // the launch of the pairs tiled with one cell per wave, else two
+            L = launch_of(S, 1, S_DENSE);
+            if (L.total == 0) {
+                L = launch_of(S, 1, S_DENSE2);
+                dense_cpw = 2;
+            }
+            break;
         case ELAS_K_LR: L = launch_of(S, 1, S_PIXELS); break;
         default: return fail(ELAS_ERR_INVALID_ARGUMENT, "unknown kernel");
     }
@@ -1051,7 +1192,7 @@ int elas_ctx::time_kernel(int kernel, int reps, float* avg_ms) {
                 elas_launch_support(L, kp, true, g(L, cap256), support_lds(prm.candidate_stepsize, S.maxd, true), s1);
                 break;
             case ELAS_K_DENSE:
-                elas_launch_dense(L, S.kpl, g(L, cap_dense), dense_lds(prm.grid_size, S.maxd, S.cpw), s1);
+                elas_launch_dense(L, S.kpl, dense_cpw, dense_grid(L.total, num_cu), S.dense_lds[dense_cpw - 1], s1);
                 break;
             case ELAS_K_LR: elas_launch_lr(L, kp, g(L, cap256), s1); break;
         }

@@ -12,7 +12,9 @@
 #include <hip/hip_runtime_api.h>
 #include <stdint.h>
 
+#include <array>
 #include <chrono>
+#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -65,6 +67,8 @@ struct ElasSlot {
     size_t pin_bytes = 0;
     void* blist_pin = nullptr;
     size_t blist_pin_bytes = 0;
+    void* tune = nullptr;  // dense tiling candidates: pair records + prefixes
+    size_t tune_bytes = 0;
 
     // the chunk in flight
     int phase = SLOT_IDLE;
@@ -75,7 +79,8 @@ struct ElasSlot {
     std::vector<ElasPair> pairs;
     std::vector<int32_t> prefix;
     std::vector<int32_t> nsp, ntri0, ntri1, status;
-    int maxd = 0, cpw = 1;
+    int maxd = 0;
+    size_t dense_lds[2] = {0, 0};  // dynamic LDS of the dense launches (1, 2 cells per wave)
     ElasKParams kpl;
     size_t tab_pairs = 0, tab_prefix = 0, tab_one = 0, pin_sp = 0, pin_tri = 0;
     ElasPair* dpairs[2] = {nullptr, nullptr};
@@ -100,6 +105,13 @@ struct ElasSlot {
     bool graphed = false;  // the chunk in flight runs its phases as graphs
 };
 
+// Dense tiling of one pair shape: grid cells per wave per step, steps per
+// chunk, and the measured microseconds of a launch (tuned shapes; -1 otherwise).
+struct ElasDenseTiles {
+    int cpw = 1, spc = 1;
+    float us = -1.0f;
+};
+
 struct ElasServer;
 
 struct elas_ctx {
@@ -117,6 +129,17 @@ struct elas_ctx {
     std::vector<ElasSlot*> slots;
     ElasSlot* last = nullptr;  // slot of the last finished call (fetch, shape info, times)
     ElasServer* server = nullptr;
+    size_t max_lds = 65536;   // dynamic LDS a workgroup may allocate on this device
+    // Dense tiling per (W, H, disp_max): with autotune on, the first call with
+    // a new shape times every candidate tiling on that pair after the call
+    // and keeps the fastest; untuned shapes use the heuristic of
+    // dense_tiles(); force_cpw > 0 overrides both.
+    bool autotune = true;
+    int force_cpw = 0, force_spc = 0;
+    std::map<std::array<int, 3>, ElasDenseTiles> dense_tuned;
+    uint64_t dense_tunings = 0;
+    ElasDenseTiles dense_tiles(const ElasDims& d, int W, int H, int dmax) const;
+    int tune_dense(ElasSlot& S, int i);
 
     // blocking single-chunk run on slot 0 (Elas::process and the device form)
     int run(const ElasPairSpec* specs, int n, int32_t* statuses);

@@ -22,7 +22,9 @@ void elas_launch_bin_scan(const ElasLaunch& L, int grid, hipStream_t s);
 void elas_launch_bin_fill(const ElasLaunch& L, int grid, hipStream_t s);
 void elas_launch_raster(const ElasLaunch& L, int grid, hipStream_t s);
 void elas_launch_mean(const ElasLaunch& L, int grid, hipStream_t s);
-void elas_launch_dense(const ElasLaunch& L, const ElasKParams& kp, int grid, size_t lds, hipStream_t s);
+void elas_launch_dense(const ElasLaunch& L, const ElasKParams& kp, int cpw, int grid, size_t lds, hipStream_t s);
+// Lets the dense kernels allocate up to `bytes` of dynamic LDS; 0 on success.
+int elas_dense_allow_lds(size_t bytes);
 void elas_launch_lr(const ElasLaunch& L, const ElasKParams& kp, int grid, hipStream_t s);
 void elas_launch_speckle(const ElasLaunch& Ltiles, const ElasLaunch& Lpix, const ElasKParams& kp, int grid,
                          hipStream_t s);

@@ -367,15 +367,27 @@ __global__ __launch_bounds__(256) void k_mean(ElasLaunch L) {
 }
 
 // ---- dense matching: KerComputeDisparity.cu:156-316,378-402 ---------------
-// Tile = R image rows (R = min(64/gs, gs), inside one grid-cell row) x
-// ELAS_DENSE_WAVES*cpw grid cells; wave w owns cells gx0 + w + WAVES*j, lane =
-// (row, column) of a cell, so the cell's candidate set is wave-uniform and is
-// walked on the scalar unit as the set bits of its bitmask.  The other image's
-// R rows are staged in LDS for the tile's disparity reach, with a row stride
-// of 4 (mod 16) descriptors so that a wave's three rows hit disjoint banks;
-// the pixel's own 16-byte block stays in registers.  512-thread workgroups
-// capped at 64 VGPRs keep two workgroups per CU, so one stages while the
-// other matches.
+// Work unit = (pair, band, chunk).  A band is R = min(64/gs, gs) image rows
+// inside one grid-cell row, matched for both images at once; a chunk is a run
+// of up to P.dspc steps along the band, a step covering tw = WAVES*P.dcpw*gs
+// columns.  In a step, wave w owns cells s*WAVES*dcpw + w + WAVES*j and a lane
+// is a (row, column) of a cell, so the cell's candidate set is wave-uniform
+// and is walked on the scalar unit as the set bits of its bitmask.
+//
+// LDS holds two rings of R descriptor rows:
+//   ring 0: desc0 over [u_lo, u_hi + dmax): the left pixels' own blocks and
+//           the right pixels' matches at u + d;
+//   ring 1: desc1 over [u_lo - dmax, u_hi): the right pixels' own blocks and
+//           the left pixels' matches at u - d.
+// A ring row has RW slots, the smallest power of two >= tw + dmax.  Column c
+// of step s sits at logical slot (ph + c - base) mod RW, with
+// ph = (s - s0)*tw mod RW and base = u_lo or u_lo - dmax, so the tw columns a
+// step brings in take the slots of columns that have left, and each
+// descriptor row of a chunk is read from HBM once.  With RW a power of two a
+// slot address is one subtraction and one and-or.  Row l of the rings is
+// rotated by 4 l slots so that a wave's rows start on different banks.  The
+// next step's columns, priors and candidate masks are loaded into registers
+// while the current step matches and are written after it.
 //
 // Reference order and tie rule: grid candidates outside the plane window
 // [dp-r, dp+r] ascending, then the window ascending with the prior P added,
@@ -384,165 +396,283 @@ __global__ __launch_bounds__(256) void k_mean(ElasLaunch L) {
 // When every P(k) < 0, a grid candidate inside the window is dominated by its
 // own window entry (same d, lower cost), so phase 0 need not exclude the
 // window; otherwise it does.
-template <bool PNEG>
-__global__ __launch_bounds__(ELAS_DENSE_THREADS, 8) void k_dense(ElasLaunch L, ElasKParams kp) {
-    extern __shared__ uint4 s_i2[];
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int cpw = kp.dense_cpw;
-    const int tcells = ELAS_DENSE_WAVES * cpw;
+
+// smallest power of two >= x (x >= 1)
+__device__ __forceinline__ uint32_t elas_pow2_at_least(uint32_t x) { return x <= 1u ? 1u : 1u << (32 - __clz(x - 1u)); }
+
+// Workgroups are dispatched to the 8 XCDs round robin (workgroup b runs on XCD
+// b mod 8).  With a grid that is a multiple of 8, work item t of a grid-stride
+// loop runs on XCD t mod 8; this order gives each XCD a contiguous run of the
+// work list, so neighbouring units (which share descriptor columns) meet in
+// the same L2.  A bijection of [0, total).
+__device__ __forceinline__ int elas_xcd_order(int t, int total, int grid) {
+    if (grid & 7) return t;
+    const int x = t & 7, i = t >> 3, q = total >> 3, r = total & 7;
+    return x * q + min(x, r) + i;
+}
+
+// CPW = P.dcpw: cells per wave per step (a launch holds the pairs of one CPW).
+// Every global load and store is a buffer access issued without a branch
+// (lanes with nothing to move take an out-of-range offset), so the waits
+// below are for exactly the loads they need.
+template <int CPW, bool PNEG>
+__global__ __launch_bounds__(ELAS_DENSE_THREADS, CPW == 1 ? 6 : 4) void k_dense(ElasLaunch L, ElasKParams kp) {
+    extern __shared__ uint4 s_ring[];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform: scalar registers
     // a candidate outside the image (or of an excluded kind) starts its SAD
     // from KEY_BAD: its key exceeds every valid key and stays below 2^32
     constexpr uint32_t KEY_BAD = 0x1ff000u;
-    for (int t = blockIdx.x; t < L.total; t += gridDim.x) {
+    // ring reads by LDS byte address: the dynamic LDS (the rings) starts
+    // right after the kernel's static LDS, of which it has none, so at 0
+    typedef const __attribute__((address_space(3))) elas_u32x4 lds_u4;
+    constexpr uint32_t lds0 = 0u;
+    for (int t0 = blockIdx.x; t0 < L.total; t0 += gridDim.x) {
+        const int t = elas_xcd_order(t0, L.total, gridDim.x);
         const int p = elas_find_pair(L.prefix, L.npairs, t);
         const auto& P = L.pairs[p];
-        const int W = P.W, H = P.H, gs = P.gsize, gw = P.gw, dmax = P.dmax;
+        const int W = P.W, H = P.H, gs = P.gsize, gw = P.gw, dmax = P.dmax, words = P.words;
         const int R = max(1, min(64 / gs, gs));
         const int sub = (gs + R - 1) / R;
+        const int tcells = ELAS_DENSE_WAVES * CPW, tw = tcells * gs;
         const int segs = (gw + tcells - 1) / tcells;
-        const int per_img = P.gh * sub * segs;
-        int r = t - L.prefix[p];
-        const int img = r / per_img;
-        r -= img * per_img;
-        const int gy = r / (sub * segs);
-        r -= gy * sub * segs;
-        const int k = r / segs, s = r - k * segs;
-        const int v0 = gy * gs + k * R;
-        const int rows = max(0, min(min(R, gs - k * R), H - v0));
-        const int gx0 = s * tcells;
-        // staged columns: the whole reach of the tile's lanes, [u_lo - dmax,
-        // u_lo + tcells*gs) or [u_lo, u_lo + tcells*gs + dmax); columns
-        // outside the image are zero and never selected by a valid candidate
-        const int u_lo = gx0 * gs;
-        const int ncols = tcells * gs + dmax;
-        const int c_lo = img ? u_lo : u_lo - dmax;
-        const int rs = ncols + ((20 - (ncols & 15)) & 15);  // 4 (mod 16): disjoint banks per row
-        const uint8_t* I1 = P.desc[img];
-        const uint8_t* I2 = P.desc[1 - img];
+        const int chunks = (segs + P.dspc - 1) / P.dspc;
+        const int r = t - L.prefix[p];
+        const int band = r / chunks, chunk = r - band * chunks;
+        const int gy = band / sub, kb = band - gy * sub;
+        const int v0 = gy * gs + kb * R;
+        const int rows = max(0, min(min(R, gs - kb * R), H - v0));
+        if (rows == 0) continue;  // workgroup-uniform
+        const int s0 = chunk * P.dspc, s1 = min(segs, s0 + P.dspc);
+        // ring rows of RW = 2^k >= tw + dmax slots; line l (ring 0 rows, then
+        // ring 1 rows) is rotated by 4 l slots so that a wave's rows start
+        // on different banks: logical slot q of line l sits at
+        // l*RW + ((q + 4 l) mod RW)
+        const int RW = (int)elas_pow2_at_least((uint32_t)(tw + dmax));
+        const uint32_t M16 = 16u * (uint32_t)RW - 1u;
+        auto phys = [&](int line, int q) { return line * RW + ((q + 4 * line) & (RW - 1)); };
+        const uint32_t npx = (uint32_t)W * (uint32_t)H;  // <= 2^27
+        const __amdgpu_buffer_rsrc_t rd0 = elas_rsrc(P.desc[0], 16u * npx);
+        const __amdgpu_buffer_rsrc_t rd1 = elas_rsrc(P.desc[1], 16u * npx);
+        // descriptor of image k, band row rr, column c (0 off the image); k is
+        // a constant at every call, so the buffer resource stays scalar
+        auto dload = [&](int k, int rr, int c, bool ok) {
+            const uint32_t off = ok && c >= 0 && c < W ? ((uint32_t)(v0 + rr) * (uint32_t)W + (uint32_t)c) * 16u : ELAS_OOR;
+            return elas_ld16_ranged(k ? rd1 : rd0, off);
+        };
+        // prologue: step s0's windows (slot = column - base), both rings at
+        // the same (row, slot), every load of a batch issued before its stores
         {
-            // every thread issues all of its loads before storing any
-            const int n = rows * ncols;
-            for (int base = 0; base < n; base += 4 * ELAS_DENSE_THREADS) {
-                uint4 q[4];
-                int li[4];
-                int rr = (base + (int)threadIdx.x) / ncols, cc = base + threadIdx.x - rr * ncols;
+            const int span = tw + dmax, n = rows * span, u_lo = s0 * tw;
+            int rr = tid / span, j = tid - (tid / span) * span;
+            for (int base = 0; base < n; base += 2 * ELAS_DENSE_THREADS) {
+                uint4 q[2][2];
+                int li[2];
                 #pragma unroll
-                for (int k = 0; k < 4; k++) {
-                    const int i = base + threadIdx.x + k * ELAS_DENSE_THREADS;
-                    const int c = c_lo + cc;
-                    q[k] = make_uint4(0u, 0u, 0u, 0u);
-                    if (i < n && c >= 0 && c < W)
-                        q[k] = elas_ld(reinterpret_cast<const uint4*>(I2 + ((size_t)(v0 + rr) * W + c) * 16));
-                    li[k] = i < n ? rr * rs + cc : -1;
-                    cc += ELAS_DENSE_THREADS;
-                    while (cc >= ncols) {
-                        cc -= ncols;
+                for (int e = 0; e < 2; e++) {
+                    const bool ok = base + tid + e * ELAS_DENSE_THREADS < n;
+                    q[0][e] = dload(0, rr, u_lo + j, ok);
+                    q[1][e] = dload(1, rr, u_lo - dmax + j, ok);
+                    li[e] = ok ? (rr << 16) | j : -1;
+                    j += ELAS_DENSE_THREADS;
+                    while (j >= span) {
+                        j -= span;
                         rr++;
                     }
                 }
                 #pragma unroll
-                for (int k = 0; k < 4; k++)
-                    if (li[k] >= 0) s_i2[li[k]] = q[k];
-            }
-        }
-        // this lane's pixel in each of its cells: prior and own descriptor; lane
-        // 32*ci + w holds word w of cell ci's candidate mask
-        const int row = lane / gs, col = lane - row * gs;
-        uint32_t maskv = 0u;
-        {
-            const int ci = lane >> 5, w = lane & 31;
-            const int gx = gx0 + wave + ci * ELAS_DENSE_WAVES;
-            if (ci < cpw && w < P.words && gx < gw) maskv = elas_ld(P.gmask[img] + (size_t)(gy * gw + gx) * P.words + w);
-        }
-        int dpv[2] = {0, 0};
-        uint4 b1v[2] = {make_uint4(0u, 0u, 0u, 0u), make_uint4(0u, 0u, 0u, 0u)};
-        #pragma unroll
-        for (int ci = 0; ci < 2; ci++) {
-            const int gx = gx0 + wave + ci * ELAS_DENSE_WAVES;
-            const int u = gx * gs + col, v = v0 + row;
-            const bool valid = ci < cpw && gx < gw && lane < R * gs && row < rows && u < W &&
-                               u >= 4 && u <= W - 4 && v >= 4 && v <= H - 4;
-            if (valid) {
-                dpv[ci] = elas_ld(P.prior[img] + (size_t)v * W + u);
-                b1v[ci] = elas_ld(reinterpret_cast<const uint4*>(I1 + ((size_t)v * W + u) * 16));
-            }
-        }
-        elas_lds_barrier();
-        #pragma unroll
-        for (int ci = 0; ci < 2; ci++) {
-            const int gx = gx0 + wave + ci * ELAS_DENSE_WAVES;
-            if (ci >= cpw || gx >= gw) continue;
-            const int u = gx * gs + col, v = v0 + row;
-            const bool active = lane < R * gs && row < rows && u < W;
-            const int dp = dpv[ci];
-            const uint4 b1 = b1v[ci];
-            bool valid = active && u >= 4 && u <= W - 4 && v >= 4 && v <= H - 4 && dp > 0;
-            if (kp.match_texture > 0) {
-                const uint32_t tex = elas_sad16(b1, make_uint4(0x80808080u, 0x80808080u, 0x80808080u, 0x80808080u), 0u);
-                if ((int)tex < kp.match_texture) valid = false;
-            }
-            // u_warp = u -/+ d must lie in [3, W-4] (KerComputeDisparity.cu:259,267): d <= dlim
-            const int dlim = img ? W - 4 - u : u - 3;
-            const int sgn = img ? 1 : -1;
-            // this lane's descriptor at disparity 0 in the staged row
-            const uint4* lp = s_i2 + (row < rows ? row : 0) * rs + (u - c_lo);
-            const uint32_t bias = (uint32_t)kp.key_bias;
-            const int dpmin = max(dp - kp.plane_radius, 0), dpmax = min(dp + kp.plane_radius, dmax);
-            uint32_t kmin = 0xffffffffu;
-            // grid candidates (wave-uniform d): key = (sad + bias) << 11 | d
-            auto grid_acc = [&](int d) {
-                bool ok = d <= dlim;
-                if (!PNEG) ok = ok && (d < dpmin || d > dpmax);
-                return ok ? bias : KEY_BAD;
-            };
-            for (int w = 0; w < P.words; w++) {
-                uint32_t bits = __builtin_amdgcn_readlane(maskv, 32 * ci + w);
-                // four candidates at a time while four remain, so their LDS
-                // reads overlap; then one at a time
-                while (__builtin_popcount(bits) >= 4) {
-                    int dq[4];
-                    uint4 b2[4];
-                    #pragma unroll
-                    for (int q = 0; q < 4; q++) {
-                        dq[q] = w * 32 + __builtin_ctz(bits);
-                        bits &= bits - 1u;
-                        b2[q] = lp[sgn * dq[q]];
+                for (int e = 0; e < 2; e++)
+                    if (li[e] >= 0) {
+                        s_ring[phys(li[e] >> 16, li[e] & 0xffff)] = q[0][e];
+                        s_ring[phys(rows + (li[e] >> 16), li[e] & 0xffff)] = q[1][e];
                     }
-                    #pragma unroll
-                    for (int q = 0; q < 4; q++)
-                        kmin = min(kmin, (elas_sad16(b1, b2[q], grid_acc(dq[q])) << 11) | (uint32_t)dq[q]);
-                }
-                while (bits) {
-                    const int d = w * 32 + __builtin_ctz(bits);
-                    bits &= bits - 1u;
-                    kmin = min(kmin, (elas_sad16(b1, lp[sgn * d], grid_acc(d)) << 11) | (uint32_t)d);
-                }
             }
-            // plane window d = dp - r + k, k = 0..2r, with the prior P(|k - r|)
-            // added: key = (sad + P + bias) << 11 | 1024 | d
-            const int pr = kp.plane_radius, nwin = 2 * pr + 1;
-            const int wlim = min(dmax, dlim);
-            const int d0 = dp - pr;
-            auto win_key = [&](int k, const uint4 b2) {
-                const int d = d0 + k;
-                const uint32_t acc = (uint32_t)d <= (uint32_t)wlim ? (uint32_t)((int)bias + kp.P[abs(k - pr)]) : KEY_BAD;
-                return (elas_sad16(b1, b2, acc) << 11) | (1024u | ((uint32_t)d & 1023u));
-            };
-            auto win_lds = [&](int k) { return lp[sgn * min(max(d0 + k, 0), dmax)]; };
-            int k0 = 0;
-            for (; k0 + 4 <= nwin; k0 += 4) {
-                uint4 b2[4];
-                #pragma unroll
-                for (int q = 0; q < 4; q++) b2[q] = win_lds(k0 + q);
-                #pragma unroll
-                for (int q = 0; q < 4; q++) kmin = min(kmin, win_key(k0 + q, b2[q]));
-            }
-            for (; k0 < nwin; k0++) kmin = min(kmin, win_key(k0, win_lds(k0)));
-            if (active)
-                elas_st(P.draw[img] + (size_t)v * W + u,
-                        (valid && kmin < (KEY_BAD << 11)) ? (float)(kmin & 1023u) : -1.0f);
         }
+        // this thread's share of a step's new columns, the same (row, column)
+        // in both rings (rows x tw <= 64 * 8 * CPW per ring):
+        // (row << 16) | column, or -1
+        int pf_at[CPW];
+        {
+            const int n = rows * tw;
+            #pragma unroll
+            for (int e = 0; e < CPW; e++) {
+                const int i = tid + e * ELAS_DENSE_THREADS;
+                pf_at[e] = i < n ? ((i / tw) << 16) | (i - (i / tw) * tw) : -1;
+            }
+        }
+        const int row = lane / gs, col = lane - row * gs;
+        const int lr = row < rows ? row : 0;
+        const int v = v0 + row;
+        const bool lane_on = lane < R * gs && row < rows;
+        const __amdgpu_buffer_rsrc_t rp[2] = {elas_rsrc(P.prior[0], 4u * npx), elas_rsrc(P.prior[1], 4u * npx)};
+        const uint32_t mbytes = 4u * (uint32_t)(P.gh * gw * words);
+        const __amdgpu_buffer_rsrc_t rm[2] = {elas_rsrc(P.gmask[0], mbytes), elas_rsrc(P.gmask[1], mbytes)};
+        const __amdgpu_buffer_rsrc_t ro[2] = {elas_rsrc(P.draw[0], 4u * npx), elas_rsrc(P.draw[1], 4u * npx)};
+        // priors of this lane's pixel in each (image, cell) of step s (0 off
+        // the matching domain), and the cells' candidate masks: lane 32*ci + w
+        // holds word w of cell ci
+        auto load_px = [&](int s, int (&pr)[2][CPW], uint32_t (&mk)[2]) {
+            #pragma unroll
+            for (int img = 0; img < 2; img++) {
+                const int ci = lane >> 5, w = lane & 31;
+                const int gx = s * tcells + wave + ci * ELAS_DENSE_WAVES;
+                const bool okm = ci < CPW && w < words && gx < gw && s < s1;
+                mk[img] = elas_ld4_ranged(rm[img], okm ? ((uint32_t)(gy * gw + gx) * (uint32_t)words + (uint32_t)w) * 4u
+                                                       : ELAS_OOR);
+                #pragma unroll
+                for (int c = 0; c < CPW; c++) {
+                    const int gxc = s * tcells + wave + c * ELAS_DENSE_WAVES;
+                    const int u = gxc * gs + col;
+                    const bool ok = s < s1 && gxc < gw && lane_on && u >= 4 && u <= W - 4 && v >= 4 && v <= H - 4;
+                    pr[img][c] = (int)elas_ld4_ranged(rp[img], ok ? ((uint32_t)v * (uint32_t)W + (uint32_t)u) * 4u : ELAS_OOR);
+                }
+            }
+        };
+        int pr_cur[2][CPW];
+        uint32_t mk_cur[2];
+        load_px(s0, pr_cur, mk_cur);
         elas_lds_barrier();
+        int ph = 0;
+        for (int s = s0; s < s1; s++) {
+            // next step's new columns (ring 0: [u + dmax, u + dmax + tw),
+            // ring 1: [u, u + tw) with u = (s + 1) tw), priors and masks; past
+            // the chunk's last step these loads read nothing
+            const bool more = s + 1 < s1;
+            const int u_nx = (s + 1) * tw;
+            uint4 pf[2][CPW];
+            #pragma unroll
+            for (int e = 0; e < CPW; e++) {
+                const int rr = pf_at[e] >> 16, j = pf_at[e] & 0xffff;
+                pf[0][e] = dload(0, rr, u_nx + dmax + j, more && pf_at[e] >= 0);
+                pf[1][e] = dload(1, rr, u_nx + j, more && pf_at[e] >= 0);
+            }
+            int pr_nx[2][CPW];
+            uint32_t mk_nx[2];
+            load_px(s + 1, pr_nx, mk_nx);
+            #pragma unroll
+            for (int img = 0; img < 2; img++) {
+                const int sgn = img ? 1 : -1;
+                // own blocks in ring img, matches in ring 1 - img: byte address of
+                // logical slot q of line l = ((16 (q + 4 l)) & M16) | 16 RW l
+                const int own_line = img * rows + lr, oth_line = (1 - img) * rows + lr;
+                const uint32_t own_base = 16u * (uint32_t)(RW * own_line), oth_base = 16u * (uint32_t)(RW * oth_line);
+                auto ld = [&](uint32_t a) {
+                    const elas_u32x4 x = *(lds_u4*)(uintptr_t)(lds0 + a);
+                    return make_uint4(x.x, x.y, x.z, x.w);
+                };
+                #pragma unroll
+                for (int ci = 0; ci < CPW; ci++) {
+                    const int gx = s * tcells + wave + ci * ELAS_DENSE_WAVES;
+                    const int u = gx * gs + col;
+                    const bool active = lane_on && gx < gw && u < W;
+                    float res = -1.0f;
+                    if (gx < gw) {  // wave-uniform
+                        const int xo = (gx - s * tcells) * gs + col;  // u - u_lo
+                        const int dp = pr_cur[img][ci];
+                        const uint4 b1 = ld((16u * (uint32_t)(ph + 4 * own_line + xo + (img ? dmax : 0)) & M16) | own_base);
+                        bool valid = active && u >= 4 && u <= W - 4 && v >= 4 && v <= H - 4 && dp > 0;
+                        if (kp.match_texture > 0) {
+                            const uint32_t tex =
+                                elas_sad16(b1, make_uint4(0x80808080u, 0x80808080u, 0x80808080u, 0x80808080u), 0u);
+                            if ((int)tex < kp.match_texture) valid = false;
+                        }
+                        // u_warp = u -/+ d must lie in [3, W-4] (KerComputeDisparity.cu:259,267): d <= dlim
+                        const int dlim = img ? W - 4 - u : u - 3;
+                        // the match at disparity d: logical slot bm + sgn*d
+                        const uint32_t bm = 16u * (uint32_t)(ph + 4 * oth_line + xo + (img ? 0 : dmax));
+                        auto maddr = [&](int d) { return ((bm + (uint32_t)(sgn * 16 * d)) & M16) | oth_base; };
+                        // every column of the cell takes d <= dlim_min
+                        const int dlim_min = img ? W - 4 - (gx * gs + gs - 1) : gx * gs - 3;
+                        const uint32_t bias = (uint32_t)kp.key_bias;
+                        const int dpmin = max(dp - kp.plane_radius, 0), dpmax = min(dp + kp.plane_radius, dmax);
+                        uint32_t kmin = 0xffffffffu;
+                        // grid candidates (wave-uniform d): key = (sad + bias) << 11 | d
+                        auto grid_acc = [&](int d) {
+                            bool ok = d <= dlim;
+                            if (!PNEG) ok = ok && (d < dpmin || d > dpmax);
+                            return ok ? bias : KEY_BAD;
+                        };
+                        const uint32_t mkv = mk_cur[img];
+                        for (int w = 0; w < words; w++) {
+                            uint32_t bits = __builtin_amdgcn_readlane(mkv, 32 * ci + w);
+                            // four candidates at a time while four remain, so their LDS
+                            // reads overlap; then one at a time
+                            while (__builtin_popcount(bits) >= 4) {
+                                int dq[4];
+                                uint4 b2[4];
+                                #pragma unroll
+                                for (int q = 0; q < 4; q++) {
+                                    dq[q] = w * 32 + __builtin_ctz(bits);
+                                    bits &= bits - 1u;
+                                    b2[q] = ld(maddr(dq[q]));
+                                }
+                                if (PNEG && dq[3] <= dlim_min) {  // wave-uniform: no per-lane test
+                                    #pragma unroll
+                                    for (int q = 0; q < 4; q++)
+                                        kmin = min(kmin, (elas_sad16(b1, b2[q], bias) << 11) | (uint32_t)dq[q]);
+                                } else {
+                                    #pragma unroll
+                                    for (int q = 0; q < 4; q++)
+                                        kmin = min(kmin, (elas_sad16(b1, b2[q], grid_acc(dq[q])) << 11) | (uint32_t)dq[q]);
+                                }
+                            }
+                            while (bits) {
+                                const int d = w * 32 + __builtin_ctz(bits);
+                                bits &= bits - 1u;
+                                kmin = min(kmin, (elas_sad16(b1, ld(maddr(d)), grid_acc(d)) << 11) | (uint32_t)d);
+                            }
+                        }
+                        // plane window d = dp - r + k, k = 0..2r, with the prior P(|k - r|)
+                        // added: key = (sad + P + bias) << 11 | 1024 | d
+                        const int pr = kp.plane_radius, nwin = 2 * pr + 1;
+                        const int wlim = min(dmax, dlim);
+                        const int d0 = dp - pr;
+                        auto win_key = [&](int k, const uint4 b2) {
+                            const int d = d0 + k;
+                            const uint32_t acc =
+                                (uint32_t)d <= (uint32_t)wlim ? (uint32_t)((int)bias + kp.P[abs(k - pr)]) : KEY_BAD;
+                            return (elas_sad16(b1, b2, acc) << 11) | (1024u | ((uint32_t)d & 1023u));
+                        };
+                        // a window entry off [0, wlim] starts from KEY_BAD, so its
+                        // (wrapped, in-bounds) read need not be clamped
+                        const uint32_t aw = bm + (uint32_t)(sgn * 16 * d0);
+                        auto win_lds = [&](int k) { return ld(((aw + (uint32_t)(sgn * 16 * k)) & M16) | oth_base); };
+                        int k0 = 0;
+                        for (; k0 + 4 <= nwin; k0 += 4) {
+                            uint4 b2[4];
+                            #pragma unroll
+                            for (int q = 0; q < 4; q++) b2[q] = win_lds(k0 + q);
+                            #pragma unroll
+                            for (int q = 0; q < 4; q++) kmin = min(kmin, win_key(k0 + q, b2[q]));
+                        }
+                        for (; k0 < nwin; k0++) kmin = min(kmin, win_key(k0, win_lds(k0)));
+                        if (valid && kmin < (KEY_BAD << 11)) res = (float)(kmin & 1023u);
+                    }
+                    elas_st4_ranged(ro[img], active ? ((uint32_t)v * (uint32_t)W + (uint32_t)u) * 4u : ELAS_OOR,
+                                    __float_as_uint(res));
+                }
+            }
+            const int phn = (ph + tw) & (RW - 1);
+            if (more) {
+                elas_lds_barrier();  // every wave is done with step s's slots
+                #pragma unroll
+                for (int e = 0; e < CPW; e++)
+                    if (pf_at[e] >= 0) {
+                        const int rr = pf_at[e] >> 16, q = phn + dmax + (pf_at[e] & 0xffff);
+                        s_ring[phys(rr, q)] = pf[0][e];
+                        s_ring[phys(rows + rr, q)] = pf[1][e];
+                    }
+                elas_lds_barrier();
+            }
+            #pragma unroll
+            for (int img = 0; img < 2; img++) {
+                mk_cur[img] = mk_nx[img];
+                #pragma unroll
+                for (int c = 0; c < CPW; c++) pr_cur[img][c] = pr_nx[img][c];
+            }
+            ph = phn;
+        }
+        elas_lds_barrier();  // the next unit's prologue rewrites the rings
     }
 }
 
@@ -597,9 +727,25 @@ void elas_launch_raster(const ElasLaunch& L, int grid, hipStream_t s) {
 void elas_launch_mean(const ElasLaunch& L, int grid, hipStream_t s) {
     hipLaunchKernelGGL(k_mean, dim3(grid), dim3(256), 0, s, L);
 }
-void elas_launch_dense(const ElasLaunch& L, const ElasKParams& kp, int grid, size_t lds, hipStream_t s) {
-    if (kp.all_p_negative) hipLaunchKernelGGL(k_dense<true>, dim3(grid), dim3(ELAS_DENSE_THREADS), lds, s, L, kp);
-    else hipLaunchKernelGGL(k_dense<false>, dim3(grid), dim3(ELAS_DENSE_THREADS), lds, s, L, kp);
+void elas_launch_dense(const ElasLaunch& L, const ElasKParams& kp, int cpw, int grid, size_t lds, hipStream_t s) {
+    const dim3 b(ELAS_DENSE_THREADS);
+    if (cpw == 2) {
+        if (kp.all_p_negative) hipLaunchKernelGGL((k_dense<2, true>), dim3(grid), b, lds, s, L, kp);
+        else hipLaunchKernelGGL((k_dense<2, false>), dim3(grid), b, lds, s, L, kp);
+    } else {
+        if (kp.all_p_negative) hipLaunchKernelGGL((k_dense<1, true>), dim3(grid), b, lds, s, L, kp);
+        else hipLaunchKernelGGL((k_dense<1, false>), dim3(grid), b, lds, s, L, kp);
+    }
+}
+int elas_dense_allow_lds(size_t bytes) {
+    const void* fn[4] = {reinterpret_cast<const void*>(&k_dense<1, true>), reinterpret_cast<const void*>(&k_dense<1, false>),
+                         reinterpret_cast<const void*>(&k_dense<2, true>), reinterpret_cast<const void*>(&k_dense<2, false>)};
+    for (const void* f : fn)
+        if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            return 1;
+        }
+    return 0;
 }
 void elas_launch_lr(const ElasLaunch& L, const ElasKParams& kp, int grid, hipStream_t s) {
     hipLaunchKernelGGL(k_lr, dim3(grid), dim3(256), 0, s, L, kp);

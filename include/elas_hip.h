@@ -196,6 +196,23 @@ typedef enum elas_kernel {
 } elas_kernel;
 int elas_hip_time_kernel(elas_ctx* ctx, int32_t kernel, int32_t reps, float* avg_ms);
 
+/* Dense-matching tile autotuner (no reference counterpart: the reference
+ * runs one block per image row, KerComputeDisparity.cu:156-316).  The dense
+ * kernel works on bands of image rows cut into chunks of steps; a step is
+ * 8 waves x cells_per_wave grid cells wide.  With autotune on (default), the
+ * first successful call with a new (W, H, disp_max) times every candidate
+ * tiling on that pair after its results are complete and keeps the fastest
+ * for later calls (results are identical under every tiling).  Off: a
+ * built-in heuristic.  elas_hip_set_dense_tiles forces a tiling
+ * (cells_per_wave 1 or 2, steps_per_chunk >= 1 or 0 for the heuristic's;
+ * 0, 0 restores automatic tiling).  elas_hip_dense_tiles reports the tiling
+ * a shape would use now and its tuned microseconds per launch (-1 if not
+ * tuned). */
+int elas_hip_set_autotune(elas_ctx* ctx, int32_t on);
+int elas_hip_set_dense_tiles(elas_ctx* ctx, int32_t cells_per_wave, int32_t steps_per_chunk);
+int elas_hip_dense_tiles(elas_ctx* ctx, int32_t W, int32_t H, int32_t disp_max, int32_t* cells_per_wave,
+                         int32_t* steps_per_chunk, float* tuned_us);
+
 /* Host Delaunay triangulation as the pipeline does it: the reference's
  * Triangle with switches "zQB" on n points [x0, y0, x1, y1, ...]
  * (replaces Elas::computeDelaunyTrianguation, src/Elas.cu:278-328, for tests
