@@ -593,34 +593,48 @@ __global__ __launch_bounds__(ELAS_DENSE_THREADS, CPW == 1 ? 6 : 4) void k_dense(
                             return ok ? bias : KEY_BAD;
                         };
                         const uint32_t mkv = mk_cur[img];
-                        for (int w = 0; w < words; w++) {
-                            uint32_t bits = __builtin_amdgcn_readlane(mkv, 32 * ci + w);
-                            // four candidates at a time while four remain, so their LDS
-                            // reads overlap; then one at a time
-                            while (__builtin_popcount(bits) >= 4) {
-                                int dq[4];
-                                uint4 b2[4];
+                        // candidates in ascending d across the cell's non-empty mask
+                        // words (one ballot finds them), four at a time so that their
+                        // LDS reads overlap, then the last one to three singly
+                        uint32_t wleft = (uint32_t)(__ballot(mkv != 0u) >> (32 * ci));
+                        if (words < 32) wleft &= (1u << words) - 1u;
+                        int left = 0;
+                        for (uint32_t m = wleft; m; m &= m - 1u)
+                            left += __builtin_popcount(__builtin_amdgcn_readlane(mkv, 32 * ci + __builtin_ctz(m)));
+                        uint32_t bits = 0u;
+                        int wbase = 0;
+                        auto next = [&]() {
+                            while (!bits) {
+                                const int w = __builtin_ctz(wleft);
+                                wleft &= wleft - 1u;
+                                bits = __builtin_amdgcn_readlane(mkv, 32 * ci + w);
+                                wbase = 32 * w;
+                            }
+                            const int d = wbase + __builtin_ctz(bits);
+                            bits &= bits - 1u;
+                            return d;
+                        };
+                        for (; left >= 4; left -= 4) {
+                            int dq[4];
+                            uint4 b2[4];
+                            #pragma unroll
+                            for (int q = 0; q < 4; q++) {
+                                dq[q] = next();
+                                b2[q] = ld(maddr(dq[q]));
+                            }
+                            if (PNEG && dq[3] <= dlim_min) {  // wave-uniform: no per-lane test
                                 #pragma unroll
-                                for (int q = 0; q < 4; q++) {
-                                    dq[q] = w * 32 + __builtin_ctz(bits);
-                                    bits &= bits - 1u;
-                                    b2[q] = ld(maddr(dq[q]));
-                                }
-                                if (PNEG && dq[3] <= dlim_min) {  // wave-uniform: no per-lane test
-                                    #pragma unroll
-                                    for (int q = 0; q < 4; q++)
-                                        kmin = min(kmin, (elas_sad16(b1, b2[q], bias) << 11) | (uint32_t)dq[q]);
-                                } else {
-                                    #pragma unroll
-                                    for (int q = 0; q < 4; q++)
-                                        kmin = min(kmin, (elas_sad16(b1, b2[q], grid_acc(dq[q])) << 11) | (uint32_t)dq[q]);
-                                }
+                                for (int q = 0; q < 4; q++)
+                                    kmin = min(kmin, (elas_sad16(b1, b2[q], bias) << 11) | (uint32_t)dq[q]);
+                            } else {
+                                #pragma unroll
+                                for (int q = 0; q < 4; q++)
+                                    kmin = min(kmin, (elas_sad16(b1, b2[q], grid_acc(dq[q])) << 11) | (uint32_t)dq[q]);
                             }
-                            while (bits) {
-                                const int d = w * 32 + __builtin_ctz(bits);
-                                bits &= bits - 1u;
-                                kmin = min(kmin, (elas_sad16(b1, ld(maddr(d)), grid_acc(d)) << 11) | (uint32_t)d);
-                            }
+                        }
+                        for (; left > 0; left--) {
+                            const int d = next();
+                            kmin = min(kmin, (elas_sad16(b1, ld(maddr(d)), grid_acc(d)) << 11) | (uint32_t)d);
                         }
                         // plane window d = dp - r + k, k = 0..2r, with the prior P(|k - r|)
                         // added: key = (sad + P + bias) << 11 | 1024 | d
@@ -638,6 +652,14 @@ __global__ __launch_bounds__(ELAS_DENSE_THREADS, CPW == 1 ? 6 : 4) void k_dense(
                         const uint32_t aw = bm + (uint32_t)(sgn * 16 * d0);
                         auto win_lds = [&](int k) { return ld(((aw + (uint32_t)(sgn * 16 * k)) & M16) | oth_base); };
                         int k0 = 0;
+                        if (nwin == 5) {  // the effective parameters: one batch of five
+                            uint4 b2[5];
+                            #pragma unroll
+                            for (int q = 0; q < 5; q++) b2[q] = win_lds(q);
+                            #pragma unroll
+                            for (int q = 0; q < 5; q++) kmin = min(kmin, win_key(q, b2[q]));
+                            k0 = 5;
+                        }
                         for (; k0 + 4 <= nwin; k0 += 4) {
                             uint4 b2[4];
                             #pragma unroll
