@@ -17,10 +17,11 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "lib", "libelas_hip.so")
+CLI_PATH = os.path.join(HERE, "lib", "elas_cli")  # command-line tool (src/Main.cu counterpart)
 
 STATUS = {0: "ok", 1: "invalid argument", 2: "HIP error", 3: "out of memory",
-          4: "too few support points", 5: "no gfx950 device", 6: "unsupported", 7: "busy"}
-ELAS_OK, ELAS_ERR_TOO_FEW_POINTS, ELAS_ERR_BUSY = 0, 4, 7
+          4: "too few support points", 5: "no gfx950 device", 6: "unsupported", 7: "busy", 8: "I/O error"}
+ELAS_OK, ELAS_ERR_TOO_FEW_POINTS, ELAS_ERR_BUSY, ELAS_ERR_IO = 0, 4, 7, 8
 
 STAGES = ["desc0", "desc1", "can_left", "can_lr", "can_incon", "can_final", "sp0", "sp1", "grid0",
           "grid1", "tri0", "tri1", "mean0", "mean1", "d1_raw", "d2_raw", "D1", "D2"]
@@ -100,6 +101,11 @@ SIGNATURES = {
     "elas_hip_triangulate": (ctypes.c_int, [_P, ctypes.c_int32, _P, ctypes.c_int64,
                                             ctypes.POINTER(ctypes.c_int64)]),
     "elas_hip_set_autotune": (ctypes.c_int, [_P, ctypes.c_int32]),
+    "elas_io_read_pgm": (ctypes.c_int, [ctypes.c_char_p, _P, ctypes.c_int64, ctypes.POINTER(ctypes.c_int32),
+                                        ctypes.POINTER(ctypes.c_int32)]),
+    "elas_io_write_pgm": (ctypes.c_int, [ctypes.c_char_p, _P, ctypes.c_int32, ctypes.c_int32]),
+    "elas_io_write_pfm": (ctypes.c_int, [ctypes.c_char_p, _P, ctypes.c_int32, ctypes.c_int32]),
+    "elas_io_disparity_to_u8": (ctypes.c_int, [_P, _P, ctypes.c_int64]),
     "elas_hip_set_dense_tiles": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32]),
     "elas_hip_dense_tiles": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                             ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
@@ -180,6 +186,49 @@ def triangulate(points):
     if st != ELAS_OK:
         raise ElasError(st, "triangulate: status %d" % st)
     return tri[:min(n.value, cap)].copy()
+
+
+def read_pgm(path):
+    """uint8 [H, W] image of a binary PGM (the reference's loadPGM,
+    src/Image.h:129-160); raises ElasError(ELAS_ERR_IO) on a malformed file."""
+    lib = load_library()
+    w, h = ctypes.c_int32(), ctypes.c_int32()
+    p = os.fsencode(path)
+    st = lib.elas_io_read_pgm(p, None, 0, ctypes.byref(w), ctypes.byref(h))
+    if st != ELAS_OK:
+        raise ElasError(st, "read_pgm: %s" % path)
+    img = np.empty((h.value, w.value), np.uint8)
+    st = lib.elas_io_read_pgm(p, img.ctypes.data, img.size, ctypes.byref(w), ctypes.byref(h))
+    if st != ELAS_OK:
+        raise ElasError(st, "read_pgm: %s" % path)
+    return img
+
+
+def write_pgm(path, img):
+    """Binary PGM of a uint8 [H, W] image (savePGM, src/Image.h:162-169)."""
+    img = _u8(img)
+    st = load_library().elas_io_write_pgm(os.fsencode(path), img.ctypes.data, img.shape[1], img.shape[0])
+    if st != ELAS_OK:
+        raise ElasError(st, "write_pgm: %s" % path)
+
+
+def write_pfm(path, disp):
+    """Float map as PFM (scale -1: little endian, bottom row first)."""
+    d = np.ascontiguousarray(disp, dtype=np.float32)
+    st = load_library().elas_io_write_pfm(os.fsencode(path), d.ctypes.data, d.shape[1], d.shape[0])
+    if st != ELAS_OK:
+        raise ElasError(st, "write_pfm: %s" % path)
+
+
+def disparity_to_u8(disp):
+    """The 8-bit view of a disparity map that the reference displays
+    (src/Main.cu:14-27): (uint8)max(255 * d / max(d), 0)."""
+    d = np.ascontiguousarray(disp, dtype=np.float32)
+    out = np.empty(d.shape, np.uint8)
+    st = load_library().elas_io_disparity_to_u8(d.ctypes.data, out.ctypes.data, d.size)
+    if st != ELAS_OK:
+        raise ElasError(st, "disparity_to_u8")
+    return out
 
 
 def _u8(a):

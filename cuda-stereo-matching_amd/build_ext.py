@@ -7,7 +7,8 @@
                     (never copied), ELAS_TRI_INSTANCES times, each under its
                     own namespace elas_triK (-DTRILIB=elas_triK) so that each
                     host worker owns an independent, non-shared Triangle state;
-linked by hipcc into lib/libelas_hip.so (in-tree, loaded by ctypes).
+linked by hipcc into lib/libelas_hip.so (in-tree, loaded by ctypes), and
+  * csrc/elas_cli.cpp, the command-line tool, linked against it as lib/elas_cli.
 Without the reference checkout an existing library is kept as it is.
 """
 import concurrent.futures as cf
@@ -26,7 +27,8 @@ ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 TRI_INSTANCES = 16
 
 HIP_SOURCES = ["k_desc.hip", "k_support.hip", "k_dense.hip", "k_post.hip"]
-HOST_SOURCES = ["elas_engine.cpp", "elas_capi.cpp", "elas_serve.cpp", "Elas.cpp"]
+HOST_SOURCES = ["elas_engine.cpp", "elas_capi.cpp", "elas_serve.cpp", "Elas.cpp", "elas_io.cpp"]
+CLI = os.path.join(LIBDIR, "elas_cli")
 HIPFLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
             "-I", CSRC, "-I", os.path.join(ROOT, "include")]
@@ -88,6 +90,13 @@ def build(verbose=False, jobs=None):
     if rc != 0:
         raise RuntimeError("link failed: %s\n%s" % (" ".join(cmd), out))
     os.replace(LIB + ".tmp", LIB)
+    cli = ["g++"] + HOSTFLAGS + [os.path.join(CSRC, "elas_cli.cpp"), "-o", CLI + ".tmp", "-L", LIBDIR, "-lelas_hip",
+                                 "-L", os.path.join(ROCM, "lib"), "-lamdhip64", "-Wl,-rpath,$ORIGIN",
+                                 "-Wl,-rpath," + os.path.join(ROCM, "lib")]
+    cmd, rc, out = _run(cli)
+    if rc != 0:
+        raise RuntimeError("cli link failed: %s\n%s" % (" ".join(cmd), out))
+    os.replace(CLI + ".tmp", CLI)
     return LIB
 
 

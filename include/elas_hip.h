@@ -66,7 +66,8 @@ typedef enum elas_status {
     ELAS_ERR_TOO_FEW_POINTS = 4,   /* < 3 support points: D1/D2 are all -1       */
     ELAS_ERR_NO_DEVICE = 5,        /* no usable gfx950 device                    */
     ELAS_ERR_UNSUPPORTED = 6,      /* request outside the supported envelope     */
-    ELAS_ERR_BUSY = 7              /* serving queue full / budget exhausted      */
+    ELAS_ERR_BUSY = 7,             /* serving queue full / budget exhausted      */
+    ELAS_ERR_IO = 8                /* unreadable, malformed or unwritable file   */
 } elas_status;
 
 /* Intermediates kept by the last process call, for parity dumps
@@ -212,6 +213,21 @@ int elas_hip_set_autotune(elas_ctx* ctx, int32_t on);
 int elas_hip_set_dense_tiles(elas_ctx* ctx, int32_t cells_per_wave, int32_t steps_per_chunk);
 int elas_hip_dense_tiles(elas_ctx* ctx, int32_t W, int32_t H, int32_t disp_max, int32_t* cells_per_wave,
                          int32_t* steps_per_chunk, float* tuned_us);
+
+/* ---- Caller-side image I/O (host only; no device needed) ------------------
+ * elas_io_read_pgm replaces loadPGM (src/Image.h:129-160): binary "P5" with
+ * '#' comment lines between header tokens and maxval <= 255; dst == NULL
+ * only reports width/height; a malformed or truncated file returns
+ * ELAS_ERR_IO (the reference throws on a bad magic or maxval and reads short
+ * files silently).  elas_io_write_pgm replaces savePGM (src/Image.h:162-169).
+ * elas_io_write_pfm writes a float map as PFM ("Pf", scale -1, bottom row
+ * first).  elas_io_disparity_to_u8 is the 8-bit view the reference displays
+ * (src/Main.cu:14-27): (uint8)max(255.0 * d / max(d), 0.0); all zeros when no
+ * value is positive (the reference divides by zero there). */
+int elas_io_read_pgm(const char* path, uint8_t* dst, int64_t cap, int32_t* width, int32_t* height);
+int elas_io_write_pgm(const char* path, const uint8_t* src, int32_t width, int32_t height);
+int elas_io_write_pfm(const char* path, const float* src, int32_t width, int32_t height);
+int elas_io_disparity_to_u8(const float* d, uint8_t* out, int64_t n);
 
 /* Host Delaunay triangulation as the pipeline does it: the reference's
  * Triangle with switches "zQB" on n points [x0, y0, x1, y1, ...]
