@@ -460,6 +460,39 @@ __global__ __launch_bounds__(ELAS_DENSE_THREADS, CPW == 1 ? 6 : 4) void k_dense(
             const uint32_t off = ok && c >= 0 && c < W ? ((uint32_t)(v0 + rr) * (uint32_t)W + (uint32_t)c) * 16u : ELAS_OOR;
             return elas_ld16_ranged(k ? rd1 : rd0, off);
         };
+        const int row = lane / gs, col = lane - row * gs;
+        const int lr = row < rows ? row : 0;
+        const int v = v0 + row;
+        const bool lane_on = lane < R * gs && row < rows;
+        const uint32_t vrow4 = 4u * (uint32_t)v * (uint32_t)W;  // byte offset of this lane's row in a float/int map
+        const __amdgpu_buffer_rsrc_t rp[2] = {elas_rsrc(P.prior[0], 4u * npx), elas_rsrc(P.prior[1], 4u * npx)};
+        const uint32_t mbytes = 4u * (uint32_t)(P.gh * gw * words);
+        const __amdgpu_buffer_rsrc_t rm[2] = {elas_rsrc(P.gmask[0], mbytes), elas_rsrc(P.gmask[1], mbytes)};
+        const __amdgpu_buffer_rsrc_t ro[2] = {elas_rsrc(P.draw[0], 4u * npx), elas_rsrc(P.draw[1], 4u * npx)};
+        // priors of this lane's pixel in each (image, cell) of step s (0 off
+        // the matching domain), and the cells' candidate masks: lane 32*ci + w
+        // holds word w of cell ci
+        auto load_px = [&](int s, int (&pr)[2][CPW], uint32_t (&mk)[2]) {
+            #pragma unroll
+            for (int img = 0; img < 2; img++) {
+                const int ci = lane >> 5, w = lane & 31;
+                const int gx = s * tcells + wave + ci * ELAS_DENSE_WAVES;
+                const bool okm = ci < CPW && w < words && gx < gw && s < s1;
+                mk[img] = elas_ld4_ranged(rm[img], okm ? ((uint32_t)(gy * gw + gx) * (uint32_t)words + (uint32_t)w) * 4u
+                                                       : ELAS_OOR);
+                #pragma unroll
+                for (int c = 0; c < CPW; c++) {
+                    const int gxc = s * tcells + wave + c * ELAS_DENSE_WAVES;
+                    const int u = gxc * gs + col;
+                    const bool ok = s < s1 && gxc < gw && lane_on && u >= 4 && u <= W - 4 && v >= 4 && v <= H - 4;
+                    pr[img][c] = (int)elas_ld4_ranged(rp[img], ok ? vrow4 + 4u * (uint32_t)u : ELAS_OOR);
+                }
+            }
+        };
+        // this step's priors and masks are in flight with the ring loads below
+        int pr_cur[2][CPW];
+        uint32_t mk_cur[2];
+        load_px(s0, pr_cur, mk_cur);
         // prologue: step s0's windows (slot = column - base), both rings at
         // the same (row, slot), every load of a batch issued before its stores
         {
@@ -500,37 +533,6 @@ __global__ __launch_bounds__(ELAS_DENSE_THREADS, CPW == 1 ? 6 : 4) void k_dense(
                 pf_at[e] = i < n ? ((i / tw) << 16) | (i - (i / tw) * tw) : -1;
             }
         }
-        const int row = lane / gs, col = lane - row * gs;
-        const int lr = row < rows ? row : 0;
-        const int v = v0 + row;
-        const bool lane_on = lane < R * gs && row < rows;
-        const __amdgpu_buffer_rsrc_t rp[2] = {elas_rsrc(P.prior[0], 4u * npx), elas_rsrc(P.prior[1], 4u * npx)};
-        const uint32_t mbytes = 4u * (uint32_t)(P.gh * gw * words);
-        const __amdgpu_buffer_rsrc_t rm[2] = {elas_rsrc(P.gmask[0], mbytes), elas_rsrc(P.gmask[1], mbytes)};
-        const __amdgpu_buffer_rsrc_t ro[2] = {elas_rsrc(P.draw[0], 4u * npx), elas_rsrc(P.draw[1], 4u * npx)};
-        // priors of this lane's pixel in each (image, cell) of step s (0 off
-        // the matching domain), and the cells' candidate masks: lane 32*ci + w
-        // holds word w of cell ci
-        auto load_px = [&](int s, int (&pr)[2][CPW], uint32_t (&mk)[2]) {
-            #pragma unroll
-            for (int img = 0; img < 2; img++) {
-                const int ci = lane >> 5, w = lane & 31;
-                const int gx = s * tcells + wave + ci * ELAS_DENSE_WAVES;
-                const bool okm = ci < CPW && w < words && gx < gw && s < s1;
-                mk[img] = elas_ld4_ranged(rm[img], okm ? ((uint32_t)(gy * gw + gx) * (uint32_t)words + (uint32_t)w) * 4u
-                                                       : ELAS_OOR);
-                #pragma unroll
-                for (int c = 0; c < CPW; c++) {
-                    const int gxc = s * tcells + wave + c * ELAS_DENSE_WAVES;
-                    const int u = gxc * gs + col;
-                    const bool ok = s < s1 && gxc < gw && lane_on && u >= 4 && u <= W - 4 && v >= 4 && v <= H - 4;
-                    pr[img][c] = (int)elas_ld4_ranged(rp[img], ok ? ((uint32_t)v * (uint32_t)W + (uint32_t)u) * 4u : ELAS_OOR);
-                }
-            }
-        };
-        int pr_cur[2][CPW];
-        uint32_t mk_cur[2];
-        load_px(s0, pr_cur, mk_cur);
         elas_lds_barrier();
         int ph = 0;
         for (int s = s0; s < s1; s++) {
@@ -652,12 +654,34 @@ __global__ __launch_bounds__(ELAS_DENSE_THREADS, CPW == 1 ? 6 : 4) void k_dense(
                         const uint32_t aw = bm + (uint32_t)(sgn * 16 * d0);
                         auto win_lds = [&](int k) { return ld(((aw + (uint32_t)(sgn * 16 * k)) & M16) | oth_base); };
                         int k0 = 0;
-                        if (nwin == 5) {  // the effective parameters: one batch of five
+                        if (nwin == 5) {
+                            // the effective parameters: one batch of five, the priors
+                            // P(2), P(1), P(0), P(1), P(2) indexed by constants (a
+                            // runtime index into the kernel arguments is a scalar
+                            // memory load per entry)
+                            const int pw[5] = {kp.P[2], kp.P[1], kp.P[0], kp.P[1], kp.P[2]};
                             uint4 b2[5];
                             #pragma unroll
                             for (int q = 0; q < 5; q++) b2[q] = win_lds(q);
-                            #pragma unroll
-                            for (int q = 0; q < 5; q++) kmin = min(kmin, win_key(q, b2[q]));
+                            // when every valid lane's window lies in [0, wlim] (the usual
+                            // case) the prior is added without a per-lane test; other
+                            // lanes' results are discarded
+                            const bool inside = d0 >= 0 && d0 + 4 <= wlim;
+                            if (__ballot(valid && !inside) == 0) {
+                                const uint32_t kd = 1024u | (uint32_t)d0;
+                                #pragma unroll
+                                for (int q = 0; q < 5; q++)
+                                    kmin = min(kmin, (elas_sad16(b1, b2[q], (uint32_t)((int)bias + pw[q])) << 11) |
+                                                         ((kd + (uint32_t)q) & 2047u));
+                            } else {
+                                #pragma unroll
+                                for (int q = 0; q < 5; q++) {
+                                    const int d = d0 + q;
+                                    const uint32_t acc =
+                                        (uint32_t)d <= (uint32_t)wlim ? (uint32_t)((int)bias + pw[q]) : KEY_BAD;
+                                    kmin = min(kmin, (elas_sad16(b1, b2[q], acc) << 11) | (1024u | ((uint32_t)d & 1023u)));
+                                }
+                            }
                             k0 = 5;
                         }
                         for (; k0 + 4 <= nwin; k0 += 4) {
@@ -670,8 +694,7 @@ __global__ __launch_bounds__(ELAS_DENSE_THREADS, CPW == 1 ? 6 : 4) void k_dense(
                         for (; k0 < nwin; k0++) kmin = min(kmin, win_key(k0, win_lds(k0)));
                         if (valid && kmin < (KEY_BAD << 11)) res = (float)(kmin & 1023u);
                     }
-                    elas_st4_ranged(ro[img], active ? ((uint32_t)v * (uint32_t)W + (uint32_t)u) * 4u : ELAS_OOR,
-                                    __float_as_uint(res));
+                    elas_st4_ranged(ro[img], active ? vrow4 + 4u * (uint32_t)u : ELAS_OOR, __float_as_uint(res));
                 }
             }
             const int phn = (ph + tw) & (RW - 1);
