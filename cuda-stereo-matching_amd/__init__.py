@@ -107,6 +107,9 @@ SIGNATURES = {
     "elas_io_write_pfm": (ctypes.c_int, [ctypes.c_char_p, _P, ctypes.c_int32, ctypes.c_int32]),
     "elas_io_disparity_to_u8": (ctypes.c_int, [_P, _P, ctypes.c_int64]),
     "elas_hip_set_dense_tiles": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32]),
+    "elas_hip_set_desc_rows": (ctypes.c_int, [_P, ctypes.c_int32]),
+    "elas_hip_desc_rows": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                          ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_float)]),
     "elas_hip_dense_tiles": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                             ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
                                             ctypes.POINTER(ctypes.c_float)]),
@@ -380,6 +383,17 @@ class Elas:
         self._check(self._lib.elas_hip_dense_tiles(self._ctx, int(W), int(H), int(disp_max), ctypes.byref(c),
                                                    ctypes.byref(s), ctypes.byref(us)))
         return c.value, s.value, us.value
+
+    def set_desc_rows(self, rows=0):
+        """Force the descriptor strip height (6, 12 or 18 rows; 0 = automatic)."""
+        self._check(self._lib.elas_hip_set_desc_rows(self._ctx, int(rows)))
+
+    def desc_rows(self, W, H, disp_max):
+        """(strip rows, tuned microseconds or -1) a shape's descriptor would use now."""
+        r, us = ctypes.c_int32(), ctypes.c_float()
+        self._check(self._lib.elas_hip_desc_rows(self._ctx, int(W), int(H), int(disp_max), ctypes.byref(r),
+                                                 ctypes.byref(us)))
+        return r.value, us.value
 
     def set_pipeline(self, slots=3, chunk=0):
         """Batches are cut into chunks of `chunk` pairs (0: one chunk) that

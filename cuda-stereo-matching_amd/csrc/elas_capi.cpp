@@ -173,7 +173,7 @@ int elas_hip_set_params(elas_ctx* c, const elas_params* p) {
     c->prm = *p;
     c->kp = kp;
     c->param_gen++;
-    c->dense_tuned.clear();  // tilings depend on grid_size
+    c->tuned.clear();  // tilings depend on grid_size
     return ELAS_OK;
 }
 
@@ -290,10 +290,30 @@ int elas_hip_dense_tiles(elas_ctx* c, int32_t W, int32_t H, int32_t disp_max, in
                          int32_t* steps_per_chunk, float* tuned_us) {
     if (!c || W < 1 || H < 1 || disp_max < 0) return ELAS_ERR_INVALID_ARGUMENT;
     ELAS_NOT_SERVING(c);
-    const ElasDenseTiles t = c->dense_tiles(elas_dims(c->prm, W, H, disp_max), W, H, disp_max);
+    const ElasTiles t = c->tiles(elas_dims(c->prm, W, H, disp_max), W, H, disp_max);
     if (cells_per_wave) *cells_per_wave = t.cpw;
     if (steps_per_chunk) *steps_per_chunk = t.spc;
     if (tuned_us) *tuned_us = t.us;
+    return ELAS_OK;
+}
+
+int elas_hip_set_desc_rows(elas_ctx* c, int32_t rows) {
+    if (!c) return ELAS_ERR_INVALID_ARGUMENT;
+    ELAS_NOT_SERVING(c);
+    if (rows != 0 && rows != 6 && rows != 12 && rows != 18) {
+        c->err = "descriptor strip rows must be 6, 12, 18 or 0 (automatic)";
+        return ELAS_ERR_INVALID_ARGUMENT;
+    }
+    c->force_dsh = rows;
+    return ELAS_OK;
+}
+
+int elas_hip_desc_rows(elas_ctx* c, int32_t W, int32_t H, int32_t disp_max, int32_t* rows, float* tuned_us) {
+    if (!c || W < 1 || H < 1 || disp_max < 0) return ELAS_ERR_INVALID_ARGUMENT;
+    ELAS_NOT_SERVING(c);
+    const ElasTiles t = c->tiles(elas_dims(c->prm, W, H, disp_max), W, H, disp_max);
+    if (rows) *rows = t.dsh;
+    if (tuned_us) *tuned_us = t.desc_us;
     return ELAS_OK;
 }
 

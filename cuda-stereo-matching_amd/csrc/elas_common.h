@@ -22,7 +22,8 @@
 #define ELAS_TRIREC 16
 
 #define ELAS_DESC_SW 58      // descriptor strip: output columns of a 64-lane wave
-#define ELAS_DESC_SH 18      // descriptor strip: output rows (18 + 6 rows walked)
+#define ELAS_DESC_SH 18      // descriptor strip: output rows by default (18 + 6 rows walked);
+                             // a pair's P.dsh is one of 6, 12, 18
 #define ELAS_POST_TW 64      // post-filter tile: 64 x 16 output pixels
 #define ELAS_POST_TH 16
 #define ELAS_GAP_MAX_LINE 15872  // gap interpolation keeps a line in LDS (<= 62 KiB)
@@ -45,6 +46,7 @@ struct ElasPair {
     int32_t words;           // 32-bit words of a cell's disparity bitmask: (dmax+32)/32
     int32_t step, gsize;     // candidate_stepsize, grid_size
     int32_t dcpw, dspc;      // dense tiling: grid cells per wave per step (1, 2), steps per chunk
+    int32_t dsh;             // descriptor strip rows (6, 12 or 18)
     int32_t ntri[2];         // triangles of the left / right mesh (host-known)
     int32_t nblk;            // 256-cell blocks of the support lattice
     const uint8_t* img[2];   // [H][W]
@@ -110,9 +112,11 @@ struct ElasLaunch {
 };
 
 // Stage ids (one prefix table per stage).  S_DENSE holds the dense units of
-// the pairs tiled with one cell per wave, S_DENSE2 those with two.
+// the pairs tiled with one cell per wave, S_DENSE2 those with two; S_DESC,
+// S_DESC12 and S_DESC6 the descriptor strips of the pairs with 18-, 12- and
+// 6-row strips.
 enum ElasStageId {
     S_DESC = 0, S_SUPPORT, S_INCON, S_REDUN_H, S_REDUN_V, S_BLOCKS, S_GRID_CELLS,
     S_PLANE, S_RASTER, S_DENSE, S_PIXELS, S_GRID_FILL, S_POST_TILES, S_POST_ROWS, S_POST_COLS,
-    S_BIN_CLEAR, S_BIN_SCAN, S_DENSE2, S_NUM_STAGES
+    S_BIN_CLEAR, S_BIN_SCAN, S_DENSE2, S_DESC12, S_DESC6, S_NUM_STAGES
 };

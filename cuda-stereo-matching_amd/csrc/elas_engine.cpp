@@ -56,6 +56,10 @@ int dense_units(const ElasDims& d, int gs, int cpw, int spc) {
     return d.gh * cdiv(gs, dense_rows(gs)) * cdiv(dense_segs(d.gw, cpw), spc);
 }
 
+// descriptor strip heights and the strips of one pair (both images)
+const int kDescRows[3] = {6, 12, 18};
+int desc_units(int W, int H, int sh) { return 2 * cdiv(W, ELAS_DESC_SW) * cdiv(H, sh); }
+
 // Grid of a dense launch: the units rounded up to a multiple of 8 (so that a
 // workgroup's XCD is its index mod 8 in every grid-stride step), at most 4
 // workgroups of 512 lanes per CU.
@@ -347,9 +351,10 @@ int elas_ctx::slot_prepare(ElasSlot& S, const ElasPairSpec* sp, int n) {
         P.W = s.W; P.H = s.H; P.dmin = prm.disp_min; P.dmax = s.dmax;
         P.cw = d.cw; P.ch = d.ch; P.gw = d.gw; P.gh = d.gh; P.words = d.words;
         P.step = prm.candidate_stepsize; P.gsize = prm.grid_size; P.nblk = d.nblk;
-        const ElasDenseTiles dt = dense_tiles(d, s.W, s.H, s.dmax);
+        const ElasTiles dt = tiles(d, s.W, s.H, s.dmax);
         P.dcpw = dt.cpw;
         P.dspc = dt.spc;
+        P.dsh = dt.dsh;
         S.dense_lds[dt.cpw - 1] = std::max(S.dense_lds[dt.cpw - 1], dense_lds(prm.grid_size, s.dmax, dt.cpw));
         char* c = A + S.base[i];
         auto take = [&](size_t bytes) { char* r = c; c += al(bytes); return r; };
@@ -397,7 +402,9 @@ namespace {
 int stage_tiles(const elas_params& prm, const ElasPairSpec& s, const ElasDims& d, const ElasPair& P, int st) {
     const int gs = prm.grid_size, R = dense_rows(gs);
     switch (st) {
-        case S_DESC: return cdiv(s.W, ELAS_DESC_SW) * cdiv(s.H, ELAS_DESC_SH) * 2;
+        case S_DESC: return P.dsh == 18 ? desc_units(s.W, s.H, 18) : 0;
+        case S_DESC12: return P.dsh == 12 ? desc_units(s.W, s.H, 12) : 0;
+        case S_DESC6: return P.dsh == 6 ? desc_units(s.W, s.H, 6) : 0;
         case S_SUPPORT: return d.ch * cdiv(d.cw, 64);
         case S_INCON: return cdiv(d.cw, 16) * cdiv(d.ch, 16);
         case S_REDUN_H: return cdiv(d.ch, elas_redun_lines(d.cw));
@@ -453,7 +460,10 @@ int elas_ctx::enqueue_a(ElasSlot& S, bool timed) {
     HIPCHECK(hipMemcpyAsync(S.dpairs[0], S.hpairs[0], sizeof(ElasPair) * n, hipMemcpyHostToDevice, s1));
     HIPCHECK(hipMemcpyAsync(S.dprefix[0], S.hprefix[0], sizeof(int32_t) * S.prefix.size(), hipMemcpyHostToDevice, s1));
     HIPCHECK(ev(1));
-    { ElasLaunch L = launch_of(S, 0, S_DESC); if (L.total) elas_launch_descriptor(L, g(L, cap256), s1); }
+    for (int sh : {18, 12, 6}) {
+        const ElasLaunch L = launch_of(S, 0, sh == 18 ? S_DESC : sh == 12 ? S_DESC12 : S_DESC6);
+        if (L.total) elas_launch_descriptor(L, sh, g(L, cap256), s1);
+    }
     HIPCHECK(ev(2));
     { ElasLaunch L = launch_of(S, 0, S_SUPPORT);
       if (L.total) {
@@ -801,7 +811,7 @@ int elas_ctx::slot_graph(ElasSlot& S, int which, hipGraphExec_t* out) {
         key.push_back(S.specs[i].H);
         key.push_back(S.specs[i].dmax);
         key.push_back((S.specs[i].dev_in ? 1 : 0) | (S.specs[i].dev_out ? 2 : 0));
-        key.push_back(S.pairs[i].dcpw * 65536 + S.pairs[i].dspc);
+        key.push_back(((int64_t)S.pairs[i].dsh << 32) | (S.pairs[i].dcpw * 65536 + S.pairs[i].dspc));
     }
     ElasGraphEntry* e = nullptr;
     for (auto& x : S.graphs)
@@ -894,10 +904,10 @@ int elas_ctx::slot_finish(ElasSlot& S, int32_t* statuses) {
     S.times[ELAS_T_BIN_JOB] = bj;
     if (statuses)
         for (int i = 0; i < S.n; i++) statuses[i] = S.status[i];
-    if (autotune && force_cpw == 0)
+    if (autotune && force_cpw == 0 && force_dsh == 0)
         for (int i = 0; i < S.n; i++)
-            if (S.status[i] == ELAS_OK && !dense_tuned.count({S.specs[i].W, S.specs[i].H, S.specs[i].dmax})) {
-                const int r = tune_dense(S, i);
+            if (S.status[i] == ELAS_OK && !tuned.count({S.specs[i].W, S.specs[i].H, S.specs[i].dmax})) {
+                const int r = tune(S, i);
                 if (r != ELAS_OK) return r;
             }
     S.phase = SLOT_IDLE;
@@ -905,44 +915,59 @@ int elas_ctx::slot_finish(ElasSlot& S, int32_t* statuses) {
     return ELAS_OK;
 }
 
-// ---- dense tiling ------------------------------------------------------------
+// ---- tilings and the autotuner --------------------------------------------------
 
-// Tiling of a pair shape: the tuned one; else two cells per wave where the
-// rings fit in 64 KiB, with bands cut into chunks so that the units fill about
-// four workgroups per CU.  force_cpw / force_spc (> 0) override (two cells
-// per wave still only where they fit).
-ElasDenseTiles elas_ctx::dense_tiles(const ElasDims& d, int W, int H, int dmax) const {
+// Tiling of a pair shape: the tuned one; else, for dense, two cells per wave
+// where the rings fit in 64 KiB, with bands cut into chunks so that the units
+// fill about four workgroups per CU, and for the descriptor the tallest strip
+// that still gives every SIMD two waves.  force_cpw / force_spc / force_dsh
+// (> 0) override (two cells per wave still only where they fit).
+ElasTiles elas_ctx::tiles(const ElasDims& d, int W, int H, int dmax) const {
     const int gs = prm.grid_size;
-    if (force_cpw == 0) {
-        auto it = dense_tuned.find({W, H, dmax});
-        if (it != dense_tuned.end()) return it->second;
+    ElasTiles t;
+    auto it = tuned.find({W, H, dmax});
+    if (it != tuned.end()) t = it->second;
+    if (it == tuned.end() || force_cpw > 0) {
+        const size_t lds2 = dense_lds(gs, dmax, 2);
+        t.cpw = (force_cpw == 0 || force_cpw == 2) && lds2 <= std::min<size_t>(max_lds, 65536) ? 2 : 1;
+        const int segs = dense_segs(d.gw, t.cpw);
+        if (force_spc > 0) {
+            t.spc = std::min(force_spc, segs);
+        } else {
+            const long bands = (long)d.gh * cdiv(gs, dense_rows(gs));
+            const long want = 4L * std::max(1, num_cu);
+            const int chunks = (int)std::max(1L, std::min<long>(segs, (want + bands - 1) / bands));
+            t.spc = cdiv(segs, chunks);
+        }
+        t.us = -1.0f;
     }
-    ElasDenseTiles t;
-    const size_t lds2 = dense_lds(gs, dmax, 2);
-    t.cpw = (force_cpw == 0 || force_cpw == 2) && lds2 <= std::min<size_t>(max_lds, 65536) ? 2 : 1;
-    const int segs = dense_segs(d.gw, t.cpw);
-    if (force_spc > 0) {
-        t.spc = std::min(force_spc, segs);
-    } else {
-        const long bands = (long)d.gh * cdiv(gs, dense_rows(gs));
-        const long want = 4L * std::max(1, num_cu);
-        const int chunks = (int)std::max(1L, std::min<long>(segs, (want + bands - 1) / bands));
-        t.spc = cdiv(segs, chunks);
+    if (force_dsh > 0) {
+        t.dsh = force_dsh;
+        t.desc_us = -1.0f;
+    } else if (it == tuned.end()) {
+        t.dsh = 6;
+        for (int sh : {18, 12})
+            if (desc_units(W, H, sh) >= 8 * std::max(1, num_cu)) {
+                t.dsh = sh;
+                break;
+            }
+        t.desc_us = -1.0f;
     }
     return t;
 }
 
-// Times every candidate tiling of pair i's dense launch and keeps the fastest
-// for the pair's shape.  The inputs of the call that just finished (its
-// descriptors, priors and candidate masks) are still in the arena, and every
-// tiling recomputes the same disparities from them, so the results of the
-// call are unchanged.  Candidates: 1 or 2 cells per wave (where the rings
+// Times every candidate tiling of pair i's dense launch and every descriptor
+// strip height on that pair, and keeps the fastest of each for the pair's
+// shape.  The inputs of the call that just finished (images, descriptors,
+// priors, candidate masks) are still in place, and every tiling recomputes
+// the same descriptors and disparities from them, so the results of the call
+// are unchanged.  Dense candidates: 1 or 2 cells per wave (where the rings
 // fit) x every distinct number of chunks per band.
-int elas_ctx::tune_dense(ElasSlot& S, int i) {
+int elas_ctx::tune(ElasSlot& S, int i) {
     const ElasPairSpec& sp = S.specs[i];
     const ElasDims& d = S.dd[i];
     const int gs = prm.grid_size;
-    std::vector<ElasDenseTiles> cand;
+    std::vector<ElasTiles> cand;
     for (int cpw = 1; cpw <= 2; cpw++) {
         // two cells per wave only within 64 KiB (one cell always runs)
         if (dense_lds(gs, sp.dmax, cpw) > (cpw == 1 ? max_lds : std::min<size_t>(max_lds, 65536))) continue;
@@ -952,16 +977,18 @@ int elas_ctx::tune_dense(ElasSlot& S, int i) {
             const int chunks = cdiv(segs, spc);
             if (chunks == prev) continue;  // same units as a shorter chunk
             prev = chunks;
-            ElasDenseTiles t;
+            ElasTiles t;
             t.cpw = cpw;
             t.spc = spc;
             cand.push_back(t);
         }
     }
-    if (cand.empty()) return ELAS_OK;
-    const size_t nc = cand.size();
-    const size_t pre_off = al(sizeof(ElasPair) * nc);
-    const size_t need = pre_off + al(8 * nc);
+    // device records: the dense candidates, then the pair as it is (for the
+    // descriptor); prefixes [0, units] of the dense candidates, then of the
+    // descriptor strip heights
+    const size_t nc = cand.size(), nrec = nc + 1, npre = nc + 3;
+    const size_t pre_off = al(sizeof(ElasPair) * nrec);
+    const size_t need = pre_off + al(8 * npre);
     HIPCHECK(hipSetDevice(device));
     if (S.tune_bytes < need) {
         if (S.tune) (void)hipFree(S.tune);
@@ -971,13 +998,18 @@ int elas_ctx::tune_dense(ElasSlot& S, int i) {
         S.tune_bytes = need;
     }
     std::vector<char> h(need, 0);
-    std::vector<int> units(nc);
-    for (size_t k = 0; k < nc; k++) {
+    std::vector<int> units(npre);
+    for (size_t k = 0; k < nrec; k++) {
         ElasPair P = S.pairs[i];
-        P.dcpw = cand[k].cpw;
-        P.dspc = cand[k].spc;
+        if (k < nc) {
+            P.dcpw = cand[k].cpw;
+            P.dspc = cand[k].spc;
+            units[k] = dense_units(d, gs, P.dcpw, P.dspc);
+        }
         memcpy(h.data() + k * sizeof(ElasPair), &P, sizeof(P));
-        units[k] = dense_units(d, gs, P.dcpw, P.dspc);
+    }
+    for (int j = 0; j < 3; j++) units[nc + j] = desc_units(sp.W, sp.H, kDescRows[j]);
+    for (size_t k = 0; k < npre; k++) {
         int32_t* pf = (int32_t*)(h.data() + pre_off) + 2 * k;
         pf[0] = 0;
         pf[1] = units[k];
@@ -985,28 +1017,51 @@ int elas_ctx::tune_dense(ElasSlot& S, int i) {
     hipStream_t s1 = S.stream;
     HIPCHECK(hipMemcpyAsync(S.tune, h.data(), need, hipMemcpyHostToDevice, s1));
     const int reps = 3;
-    for (size_t k = 0; k < nc; k++) {
-        ElasLaunch L;
-        L.pairs = (const ElasPair*)S.tune + k;
-        L.prefix = (const int32_t*)((const char*)S.tune + pre_off) + 2 * k;
-        L.npairs = 1;
-        L.total = units[k];
-        const size_t lds = dense_lds(gs, sp.dmax, cand[k].cpw);
-        elas_launch_dense(L, S.kpl, cand[k].cpw, dense_grid(L.total, num_cu), lds, s1);  // warm
+    // average microseconds of `reps` launches after a warm one
+    auto time_us = [&](auto&& launch, float* us) -> int {
+        launch();
         HIPCHECK(hipEventRecord(S.ev[16], s1));
-        for (int r = 0; r < reps; r++) elas_launch_dense(L, S.kpl, cand[k].cpw, dense_grid(L.total, num_cu), lds, s1);
+        for (int r = 0; r < reps; r++) launch();
         HIPCHECK(hipGetLastError());
         HIPCHECK(hipEventRecord(S.ev[17], s1));
         HIPCHECK(hipEventSynchronize(S.ev[17]));
         float ms = 0.f;
         HIPCHECK(hipEventElapsedTime(&ms, S.ev[16], S.ev[17]));
-        cand[k].us = ms * 1e3f / reps;
+        *us = ms * 1e3f / reps;
+        return ELAS_OK;
+    };
+    auto launch_of_k = [&](size_t rec, size_t pre) {
+        ElasLaunch L;
+        L.pairs = (const ElasPair*)S.tune + rec;
+        L.prefix = (const int32_t*)((const char*)S.tune + pre_off) + 2 * pre;
+        L.npairs = 1;
+        L.total = units[pre];
+        return L;
+    };
+    for (size_t k = 0; k < nc; k++) {
+        const ElasLaunch L = launch_of_k(k, k);
+        const size_t lds = dense_lds(gs, sp.dmax, cand[k].cpw);
+        const int r = time_us([&] { elas_launch_dense(L, S.kpl, cand[k].cpw, dense_grid(L.total, num_cu), lds, s1); },
+                              &cand[k].us);
+        if (r != ELAS_OK) return r;
     }
     size_t best = 0;
     for (size_t k = 1; k < nc; k++)
         if (cand[k].us < cand[best].us) best = k;
-    dense_tuned[{sp.W, sp.H, sp.dmax}] = cand[best];
-    dense_tunings++;
+    ElasTiles t = nc ? cand[best] : tiles(d, sp.W, sp.H, sp.dmax);
+    const int cap256 = std::max(1, num_cu) * 8;
+    for (int j = 0; j < 3; j++) {
+        const ElasLaunch L = launch_of_k(nc, nc + j);
+        float us = 0.f;
+        const int r = time_us([&] { elas_launch_descriptor(L, kDescRows[j], std::min(L.total, cap256), s1); }, &us);
+        if (r != ELAS_OK) return r;
+        if (j == 0 || us < t.desc_us) {
+            t.dsh = kDescRows[j];
+            t.desc_us = us;
+        }
+    }
+    tuned[{sp.W, sp.H, sp.dmax}] = t;
+    tunings++;
     return ELAS_OK;
 }
 
@@ -1164,9 +1219,16 @@ int elas_ctx::time_kernel(int kernel, int reps, float* avg_ms) {
     const int cap256 = std::max(1, num_cu) * 8;
     auto g = [&](const ElasLaunch& L, int cap) { return std::min(L.total, cap); };
     ElasLaunch L;
-    int dense_cpw = 1;
+    int dense_cpw = 1, desc_sh = 18;
     switch (kernel) {
-        case ELAS_K_DESCRIPTOR: L = launch_of(S, 0, S_DESC); break;
+        case ELAS_K_DESCRIPTOR:
+            // the launch of the first strip height any pair uses
+            for (int sh : {18, 12, 6}) {
+                L = launch_of(S, 0, sh == 18 ? S_DESC : sh == 12 ? S_DESC12 : S_DESC6);
+                desc_sh = sh;
+                if (L.total) break;
+            }
+            break;
         case ELAS_K_SUPPORT_LEFT:
         case ELAS_K_SUPPORT_RIGHT: L = launch_of(S, 0, S_SUPPORT); break;
         case ELAS_K_DENSE:
@@ -1184,7 +1246,7 @@ int elas_ctx::time_kernel(int kernel, int reps, float* avg_ms) {
     hipStream_t s1 = S.stream;
     auto launch = [&]() {
         switch (kernel) {
-            case ELAS_K_DESCRIPTOR: elas_launch_descriptor(L, g(L, cap256), s1); break;
+            case ELAS_K_DESCRIPTOR: elas_launch_descriptor(L, desc_sh, g(L, cap256), s1); break;
             case ELAS_K_SUPPORT_LEFT:
                 elas_launch_support(L, kp, false, g(L, cap256), support_lds(prm.candidate_stepsize, S.maxd, false), s1);
                 break;

@@ -105,11 +105,12 @@ struct ElasSlot {
     bool graphed = false;  // the chunk in flight runs its phases as graphs
 };
 
-// Dense tiling of one pair shape: grid cells per wave per step, steps per
-// chunk, and the measured microseconds of a launch (tuned shapes; -1 otherwise).
-struct ElasDenseTiles {
-    int cpw = 1, spc = 1;
-    float us = -1.0f;
+// Tiling of one pair shape: dense grid cells per wave per step and steps per
+// chunk, descriptor strip rows, and the measured microseconds of a launch of
+// each (tuned shapes; -1 otherwise).
+struct ElasTiles {
+    int cpw = 1, spc = 1, dsh = ELAS_DESC_SH;
+    float us = -1.0f, desc_us = -1.0f;
 };
 
 struct ElasServer;
@@ -130,16 +131,17 @@ struct elas_ctx {
     ElasSlot* last = nullptr;  // slot of the last finished call (fetch, shape info, times)
     ElasServer* server = nullptr;
     size_t max_lds = 65536;   // dynamic LDS a workgroup may allocate on this device
-    // Dense tiling per (W, H, disp_max): with autotune on, the first call with
-    // a new shape times every candidate tiling on that pair after the call
-    // and keeps the fastest; untuned shapes use the heuristic of
-    // dense_tiles(); force_cpw > 0 overrides both.
+    // Tilings per (W, H, disp_max): with autotune on, the first call with a
+    // new shape times every candidate dense tiling and descriptor strip
+    // height on that pair after the call and keeps the fastest; untuned
+    // shapes use the heuristics of tiles(); force_cpw > 0 / force_dsh > 0
+    // override both.
     bool autotune = true;
-    int force_cpw = 0, force_spc = 0;
-    std::map<std::array<int, 3>, ElasDenseTiles> dense_tuned;
-    uint64_t dense_tunings = 0;
-    ElasDenseTiles dense_tiles(const ElasDims& d, int W, int H, int dmax) const;
-    int tune_dense(ElasSlot& S, int i);
+    int force_cpw = 0, force_spc = 0, force_dsh = 0;
+    std::map<std::array<int, 3>, ElasTiles> tuned;
+    uint64_t tunings = 0;
+    ElasTiles tiles(const ElasDims& d, int W, int H, int dmax) const;
+    int tune(ElasSlot& S, int i);
 
     // blocking single-chunk run on slot 0 (Elas::process and the device form)
     int run(const ElasPairSpec* specs, int n, int32_t* statuses);

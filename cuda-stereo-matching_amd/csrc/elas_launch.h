@@ -5,7 +5,7 @@
 #include <stddef.h>
 #include "elas_common.h"
 
-void elas_launch_descriptor(const ElasLaunch& L, int grid, hipStream_t s);
+void elas_launch_descriptor(const ElasLaunch& L, int sh, int grid, hipStream_t s);
 void elas_launch_support(const ElasLaunch& L, const ElasKParams& kp, bool right, int grid, size_t lds,
                          hipStream_t s);
 void elas_launch_incon(const ElasLaunch& L, const ElasKParams& kp, int grid, hipStream_t s);

@@ -16,7 +16,9 @@
 //   du(y+1,u-2) du(y+1,u) du(y+1,u+2) du(y+2,u) | dv(y-1,u) dv(y,u-1) dv(y,u+1) dv(y+1,u)
 //
 // One pixel column per lane.  A wave owns a strip of 64 consecutive columns
-// u = x0 - 3 + lane and walks down the image rows y0-3 .. y0+SH+2 of a band;
+// u = x0 - 3 + lane and walks down the image rows y0-3 .. y0+SH+2 of a band
+// (SH = 6, 12 or 18 output rows, per pair: short strips give a small image
+// more waves);
 // lanes 3 .. 60 are the strip's ELAS_DESC_SW output columns, the three lanes
 // on either side only feed their neighbours.  Per image row the wave computes
 // H of that row and V, du, dv of the row above it, for its own columns;
@@ -31,7 +33,6 @@
 #include "elas_common.h"
 #include "elas_device.h"
 
-#define DS_STEPS (ELAS_DESC_SH + 6)  // image rows walked per band
 static_assert(ELAS_DESC_SW == 58, "lanes 3 .. 60 are the output columns");
 
 typedef int16_t elas_s16x2 __attribute__((ext_vector_type(2)));
@@ -68,7 +69,7 @@ __device__ __forceinline__ uint32_t perm(uint32_t hi, uint32_t lo, uint32_t sel)
 // One strip: columns u = x0 - 3 + lane, output rows [y0, y1).  EDGE: the
 // strip holds pixels whose descriptor is 0 (image sides, top or bottom rows);
 // interior strips skip that test.
-template <bool EDGE>
+template <int SH, bool EDGE>
 __device__ __forceinline__ void desc_strip(const ELAS_CONST_AS ElasPair& P, int img, int x0, int y0, int y1) {
     const int lane = threadIdx.x & 63;
     const Xlane X{((lane + 63) & 63) * 4, ((lane + 1) & 63) * 4};
@@ -92,6 +93,7 @@ __device__ __forceinline__ void desc_strip(const ELAS_CONST_AS ElasPair& P, int 
     // every image byte of the band is loaded before the first store: the
     // vector memory counter completes in issue order, so a load issued after
     // stores could only be waited for together with their write-back
+    constexpr int DS_STEPS = SH + 6;  // image rows walked per band
     uint32_t q[DS_STEPS];
     #pragma unroll
     for (int k = 0; k < DS_STEPS; k++) q[k] = load(y0 - 3 + k);
@@ -136,6 +138,7 @@ __device__ __forceinline__ void desc_strip(const ELAS_CONST_AS ElasPair& P, int 
     }
 }
 
+template <int SH>
 __global__ __launch_bounds__(256) void k_descriptor(ElasLaunch L) {
     const int stride = gridDim.x * 4;
     for (int t0 = blockIdx.x * 4 + (threadIdx.x >> 6); t0 < L.total; t0 += stride) {
@@ -143,22 +146,24 @@ __global__ __launch_bounds__(256) void k_descriptor(ElasLaunch L) {
         const int p = elas_find_pair(L.prefix, L.npairs, t);
         const auto& P = L.pairs[p];
         const int W = P.W, H = P.H;
-        const int ns = (W + ELAS_DESC_SW - 1) / ELAS_DESC_SW, nb = (H + ELAS_DESC_SH - 1) / ELAS_DESC_SH;
+        const int ns = (W + ELAS_DESC_SW - 1) / ELAS_DESC_SW, nb = (H + SH - 1) / SH;
         const int local = t - L.prefix[p];
         const int img = local / (ns * nb);
         const int r = local - img * ns * nb;
-        const int x0 = (r % ns) * ELAS_DESC_SW, y0 = (r / ns) * ELAS_DESC_SH;
-        const int y1 = min(y0 + ELAS_DESC_SH, H);
+        const int x0 = (r % ns) * ELAS_DESC_SW, y0 = (r / ns) * SH;
+        const int y1 = min(y0 + SH, H);
         // output columns [x0, x0 + 58) meet u < 3 or u > W - 4, or rows
         // [y0, y1) meet y < 3 or y > H - 4
         if (x0 < 3 || x0 + ELAS_DESC_SW > W - 3 || y0 < 3 || y1 > H - 3)
-            desc_strip<true>(P, img, x0, y0, y1);
+            desc_strip<SH, true>(P, img, x0, y0, y1);
         else
-            desc_strip<false>(P, img, x0, y0, y1);
+            desc_strip<SH, false>(P, img, x0, y0, y1);
     }
 }
 
-void elas_launch_descriptor(const ElasLaunch& L, int grid, hipStream_t s) {
+void elas_launch_descriptor(const ElasLaunch& L, int sh, int grid, hipStream_t s) {
     grid = std::min(grid, (L.total + 3) / 4);  // four strips per workgroup
-    hipLaunchKernelGGL(k_descriptor, dim3(grid), dim3(256), 0, s, L);
+    if (sh == 6) hipLaunchKernelGGL(k_descriptor<6>, dim3(grid), dim3(256), 0, s, L);
+    else if (sh == 12) hipLaunchKernelGGL(k_descriptor<12>, dim3(grid), dim3(256), 0, s, L);
+    else hipLaunchKernelGGL(k_descriptor<18>, dim3(grid), dim3(256), 0, s, L);
 }

@@ -213,6 +213,13 @@ int elas_hip_set_autotune(elas_ctx* ctx, int32_t on);
 int elas_hip_set_dense_tiles(elas_ctx* ctx, int32_t cells_per_wave, int32_t steps_per_chunk);
 int elas_hip_dense_tiles(elas_ctx* ctx, int32_t W, int32_t H, int32_t disp_max, int32_t* cells_per_wave,
                          int32_t* steps_per_chunk, float* tuned_us);
+/* The same for the descriptor tile (replaces the fixed 32x32 blocks of
+ * src/Elas.cu:131-154): a wave computes a strip of 58 columns x rows rows
+ * (6, 12 or 18; short strips give a small image more waves).  The autotuner
+ * times the three on the first pair of a shape; elas_hip_set_desc_rows
+ * forces one (0: automatic). */
+int elas_hip_set_desc_rows(elas_ctx* ctx, int32_t rows);
+int elas_hip_desc_rows(elas_ctx* ctx, int32_t W, int32_t H, int32_t disp_max, int32_t* rows, float* tuned_us);
 
 /* ---- Caller-side image I/O (host only; no device needed) ------------------
  * elas_io_read_pgm replaces loadPGM (src/Image.h:129-160): binary "P5" with

@@ -85,3 +85,28 @@ def test_batch_mixing_tilings_equals_oracle(pkg, oracle, synth):
         ref = oracle.process(l, r, oracle.params(disp_max=dm), dump=False)
         assert st == ref["status"]
         assert np.array_equal(b1, ref["D1"]) and np.array_equal(b2, ref["D2"])
+
+
+@pytest.mark.parametrize("name", ["c1", "c2"])
+def test_every_descriptor_strip_height_bit_exact(pkg, oracle, pairs, name):
+    """Strips of 6, 12 and 18 rows give the oracle's descriptors; a shape's
+    tuned height is recorded with its time."""
+    left, right, _, dmax = pairs(name)
+    H, W = left.shape
+    d0, d1 = oracle.descriptor(left), oracle.descriptor(right)
+    el = pkg.Elas(pkg.default_params(disp_max=dmax))
+    el.set_debug(True)
+    for rows in (6, 12, 18):
+        print("descriptor rows", name, rows, flush=True)
+        el.set_desc_rows(rows)
+        assert el.desc_rows(W, H, dmax)[0] == rows
+        el.process(left, right)
+        assert np.array_equal(el.fetch("desc0"), d0) and np.array_equal(el.fetch("desc1"), d1), rows
+    with pytest.raises(pkg.ElasError):
+        el.set_desc_rows(7)
+    el.set_desc_rows(0)
+    tuned = pkg.Elas(pkg.default_params(disp_max=dmax))
+    tuned.process(left, right)
+    rows, us = tuned.desc_rows(W, H, dmax)
+    print("tuned descriptor rows", name, rows, us, flush=True)
+    assert rows in (6, 12, 18) and us > 0
