@@ -108,18 +108,66 @@ __global__ __launch_bounds__(256) void k_support(ElasLaunch L, ElasKParams kp) {
             int cx = RIGHT ? u + d + 2 : u - d - 2;
             const int dstep = RIGHT ? 4 : -4;
             uint4 pt = top[cx - dstep], pb = bot[cx - dstep];
-            #pragma unroll 2
-            for (; d <= dmaxv; d += 4, cx += dstep) {
-                const uint4 xt = top[cx], xb = bot[cx];
-                uint32_t E = elas_sad16(rXt, xt, 0u);
-                E = elas_sad16(rXb, xb, E);
-                E = elas_sad16(rPt, pt, E);
-                E = elas_sad16(rPb, pb, E);
-                const uint32_t key = (E << 15) | ((uint32_t)((d - dminv) & 31) << 10) | (uint32_t)d;
-                k2 = min(k2, max(k1, key));
+            // (k1, k2) <- the two smallest of (k1, k2, key), k1 <= k2: the new
+            // k2 is the median of the three
+            auto take = [&](uint32_t key) {
+                k2 = max(min(k1, k2), min(max(k1, k2), key));
                 k1 = min(k1, key);
-                pt = xt;
-                pb = xb;
+            };
+            // key = E << 15 | r << 10 | d with r = (d - dmin) mod 32 (r < 32 and
+            // d < 1024 fill disjoint bits, so the fields add)
+            const int n = d <= dmaxv ? (dmaxv - d) / 4 + 1 : 0;
+            const int n0 = __builtin_amdgcn_readfirstlane(n);
+            if (__ballot(n != n0) == 0) {
+                // the usual case: every computing lane of the wave walks the same
+                // count, so the loop runs on the scalar unit; blocks of four
+                // start at i = 0 mod 4, where r = j + 16 ((i / 4) mod 2) and
+                // r + 12 < 32, so within a block key(k) = E << 15 | (kd + 4100 k)
+                const uint32_t kdj = ((uint32_t)j << 10) | (uint32_t)d;
+                int i = 0;
+                for (; i + 4 <= n0; i += 4) {
+                    uint4 xt[4], xb[4];
+                    #pragma unroll
+                    for (int k = 0; k < 4; k++) {
+                        xt[k] = top[cx + k * dstep];
+                        xb[k] = bot[cx + k * dstep];
+                    }
+                    const uint32_t kd = kdj + (uint32_t)((((i & 4) << 2) << 10) + 4 * i);
+                    #pragma unroll
+                    for (int k = 0; k < 4; k++) {
+                        uint32_t E = elas_sad16(rXt, xt[k], 0u);
+                        E = elas_sad16(rXb, xb[k], E);
+                        E = elas_sad16(rPt, k ? xt[k - 1] : pt, E);
+                        E = elas_sad16(rPb, k ? xb[k - 1] : pb, E);
+                        take((E << 15) + kd + 4100u * (uint32_t)k);
+                    }
+                    pt = xt[3];
+                    pb = xb[3];
+                    cx += 4 * dstep;
+                }
+                for (; i < n0; i++, cx += dstep) {
+                    const uint4 xt = top[cx], xb = bot[cx];
+                    uint32_t E = elas_sad16(rXt, xt, 0u);
+                    E = elas_sad16(rXb, xb, E);
+                    E = elas_sad16(rPt, pt, E);
+                    E = elas_sad16(rPb, pb, E);
+                    const int dd = d + 4 * i;
+                    take((E << 15) | ((uint32_t)((dd - dminv) & 31) << 10) | (uint32_t)dd);
+                    pt = xt;
+                    pb = xb;
+                }
+            } else {
+                #pragma unroll 2
+                for (; d <= dmaxv; d += 4, cx += dstep) {
+                    const uint4 xt = top[cx], xb = bot[cx];
+                    uint32_t E = elas_sad16(rXt, xt, 0u);
+                    E = elas_sad16(rXb, xb, E);
+                    E = elas_sad16(rPt, pt, E);
+                    E = elas_sad16(rPb, pb, E);
+                    take((E << 15) | ((uint32_t)((d - dminv) & 31) << 10) | (uint32_t)d);
+                    pt = xt;
+                    pb = xb;
+                }
             }
         }
         #pragma unroll
