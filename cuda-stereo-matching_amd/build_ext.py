@@ -24,6 +24,7 @@ LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libelas_hip.so")
 REF_SRC = "/root/reference/src"
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
+HOST_CLANG = os.path.join(ROCM, "llvm", "bin", "clang++")  # host x86 code (Triangle)
 TRI_INSTANCES = 16
 
 HIP_SOURCES = ["k_desc.hip", "k_support.hip", "k_dense.hip", "k_post.hip"]
@@ -52,9 +53,10 @@ def _cmds():
     for k in range(TRI_INSTANCES):
         o = os.path.join(BUILD, "triangle_%d.o" % k)
         # Triangle's exact predicates need strict IEEE float: no contraction, no fast-math.
-        # x86-64-v3 (AVX2) only changes instruction selection; meshes are checked
-        # against the oracle's generic -O2 build of the same file.
-        cmds.append(["g++", "-O3", "-march=x86-64-v3", "-ffp-contract=off", "-fPIC", "-w",
+        # ROCm's clang at -O3 for x86-64-v3 (AVX2) runs these meshes about 7% faster than
+        # g++; both only change instruction selection, and the meshes are checked
+        # against the oracle's generic g++ -O2 build of the same file.
+        cmds.append([HOST_CLANG, "-O3", "-march=x86-64-v3", "-ffp-contract=off", "-fPIC", "-w",
                      "-DTRILIB=elas_tri%d" % k, "-c",
                      os.path.join(REF_SRC, "triangle.cpp"), "-o", o])
         objs.append(o)

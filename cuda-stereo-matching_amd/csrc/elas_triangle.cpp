@@ -195,10 +195,14 @@ public:
         static TriPool* pool = new TriPool();  // lives until process exit
         return *pool;
     }
+    // remaining: jobs not finished, under m; left: the same count, lowered by
+    // a worker as its last access of the batch, so a waiter that polls it and
+    // then takes m once may free the batch
     struct Batch {
         std::mutex m;
         std::condition_variable cv;
         int remaining = 0;
+        std::atomic<int> left{0};
     };
     void run(ElasTriJob* jobs, int n, int max_workers) {
         if (n <= 0) return;
@@ -214,6 +218,7 @@ public:
     }
     void enqueue(ElasTriJob* jobs, int n, Batch* b) {
         b->remaining = n;
+        b->left.store(n, std::memory_order_relaxed);
         {
             std::lock_guard<std::mutex> g(m_);
             ensure_workers();
@@ -249,6 +254,7 @@ private:
                 std::lock_guard<std::mutex> g(it.batch->m);
                 if (--it.batch->remaining == 0) it.batch->cv.notify_all();
             }
+            it.batch->left.fetch_sub(1, std::memory_order_release);
             last = now_s();
         }
     }
@@ -348,7 +354,12 @@ bool elas_tri_done(ElasTriTicket* t) {
     return t->batch.remaining == 0;
 }
 
+// Polls for up to ELAS_TRI_SPIN_S before sleeping: a synchronous call waits
+// here for meshes that take a fraction of a millisecond, and a sleeping
+// thread's wake-up would add tens of microseconds to it.
 void elas_tri_wait(ElasTriTicket* t) {
+    const double t0 = now_s();
+    while (t->batch.left.load(std::memory_order_acquire) > 0 && now_s() - t0 < ELAS_TRI_SPIN_S) _mm_pause();
     std::unique_lock<std::mutex> g(t->batch.m);
     t->batch.cv.wait(g, [&] { return t->batch.remaining == 0; });
 }
