@@ -23,6 +23,8 @@
 #include <cstring>
 #include <thread>
 
+#include <immintrin.h>
+
 #include "elas_launch.h"
 #include "elas_triangle.h"
 
@@ -35,6 +37,19 @@
 namespace {
 
 using clk = std::chrono::steady_clock;
+
+// Waits for an event by polling for up to 2 ms, then blocking: a synchronous
+// call waits here for GPU work of well under a millisecond, and a blocking
+// wait would add the wake-up of a sleeping thread to it.
+hipError_t poll_event(hipEvent_t e) {
+    const auto t0 = clk::now();
+    for (;;) {
+        const hipError_t r = hipEventQuery(e);
+        if (r != hipErrorNotReady) return r;
+        if (clk::now() - t0 > std::chrono::milliseconds(2)) return hipEventSynchronize(e);
+        _mm_pause();
+    }
+}
 
 size_t al(size_t x) { return (x + 255) & ~size_t(255); }
 
@@ -457,8 +472,9 @@ int elas_ctx::enqueue_a(ElasSlot& S, bool timed) {
     const int cap256 = std::max(1, num_cu) * 8;
     auto g = [&](const ElasLaunch& L, int cap) { return std::min(L.total, cap); };
     auto ev = [&](int i) { return timed ? hipEventRecord(S.ev[i], s1) : hipSuccess; };
-    HIPCHECK(hipMemcpyAsync(S.dpairs[0], S.hpairs[0], sizeof(ElasPair) * n, hipMemcpyHostToDevice, s1));
-    HIPCHECK(hipMemcpyAsync(S.dprefix[0], S.hprefix[0], sizeof(int32_t) * S.prefix.size(), hipMemcpyHostToDevice, s1));
+    // pair records and stage prefixes lie back to back in both tables: one copy
+    HIPCHECK(hipMemcpyAsync(S.dpairs[0], S.hpairs[0], S.tab_pairs + sizeof(int32_t) * S.prefix.size(),
+                            hipMemcpyHostToDevice, s1));
     HIPCHECK(ev(1));
     for (int sh : {18, 12, 6}) {
         const ElasLaunch L = launch_of(S, 0, sh == 18 ? S_DESC : sh == 12 ? S_DESC12 : S_DESC6);
@@ -539,7 +555,7 @@ bool elas_ctx::slot_a_ready(ElasSlot& S) { return hipEventQuery(S.ev[5]) == hipS
 int elas_ctx::slot_start_tri(ElasSlot& S) {
     const int n = S.n;
     HIPCHECK(hipSetDevice(device));
-    HIPCHECK(hipEventSynchronize(S.ev[5]));
+    HIPCHECK(poll_event(S.ev[5]));
     S.t_tri0 = clk::now();
     for (int i = 0; i < n; i++) {
         S.nsp[i] = S.dd[i].nblk > 0 ? S.hnsp[i] : 0;
@@ -550,7 +566,8 @@ int elas_ctx::slot_start_tri(ElasSlot& S) {
                                     hipMemcpyDeviceToHost, S.stream));
         }
     }
-    HIPCHECK(hipStreamSynchronize(S.stream));
+    HIPCHECK(hipEventRecord(S.ev[14], S.stream));
+    HIPCHECK(poll_event(S.ev[14]));
     S.t_sp = clk::now();
     S.tri_out.assign(2 * (size_t)n, {});
     S.toff.assign(2 * (size_t)n, {});
@@ -618,13 +635,17 @@ int elas_ctx::slot_phase_b(ElasSlot& S) {
         S.ntri1[i] = nt[1];
         S.pairs[i].ntri[0] = nt[0];
         S.pairs[i].ntri[1] = nt[1];
-        for (int k = 0; k < 2; k++)
-            if (nt[k] > 0) {
-                int32_t* dst = S.htri + S.tri_off[i] + (size_t)k * 3 * S.dd[i].tricap;
-                memcpy(dst, S.tri_out[2 * i + k].data(), 12 * (size_t)nt[k]);
-                HIPCHECK(hipMemcpyAsync((void*)S.pairs[i].tri[k], dst, 12 * (size_t)nt[k], hipMemcpyHostToDevice,
-                                        S.stream));
-            }
+        // both meshes back to back, one upload into the pair's two triangle
+        // regions (contiguous in the arena, each tricap triangles); the right
+        // mesh starts after the left one
+        if (nt[0] + nt[1] > 0) {
+            int32_t* dst = S.htri + S.tri_off[i];
+            for (int k = 0; k < 2; k++)
+                if (nt[k] > 0) memcpy(dst + (k ? 3 * (size_t)nt[0] : 0), S.tri_out[2 * i + k].data(), 12 * (size_t)nt[k]);
+            int32_t* base = const_cast<int32_t*>(S.pairs[i].tri[0]);
+            S.pairs[i].tri[1] = base + 3 * (size_t)nt[0];
+            HIPCHECK(hipMemcpyAsync(base, dst, 12 * (size_t)(nt[0] + nt[1]), hipMemcpyHostToDevice, S.stream));
+        }
     }
     build_prefix(prm, S);
     memcpy(S.hpairs[1], S.pairs.data(), sizeof(ElasPair) * n);
@@ -661,8 +682,8 @@ int elas_ctx::enqueue_b(ElasSlot& S, bool timed) {
     const int n = S.n;
     hipStream_t s1 = S.stream;
     auto ev = [&](int i) { return timed ? hipEventRecord(S.ev[i], s1) : hipSuccess; };
-    HIPCHECK(hipMemcpyAsync(S.dpairs[1], S.hpairs[1], sizeof(ElasPair) * n, hipMemcpyHostToDevice, s1));
-    HIPCHECK(hipMemcpyAsync(S.dprefix[1], S.hprefix[1], sizeof(int32_t) * S.prefix.size(), hipMemcpyHostToDevice, s1));
+    HIPCHECK(hipMemcpyAsync(S.dpairs[1], S.hpairs[1], S.tab_pairs + sizeof(int32_t) * S.prefix.size(),
+                            hipMemcpyHostToDevice, s1));
     const int cap256 = std::max(1, num_cu) * 8;
     auto g = [&](const ElasLaunch& L, int cap) { return std::min(L.total, cap); };
     HIPCHECK(ev(15));
@@ -863,7 +884,7 @@ bool elas_ctx::slot_b_ready(ElasSlot& S) { return hipEventQuery(S.ev[13]) == hip
 
 int elas_ctx::slot_finish(ElasSlot& S, int32_t* statuses) {
     HIPCHECK(hipSetDevice(device));
-    HIPCHECK(hipEventSynchronize(S.ev[13]));
+    HIPCHECK(poll_event(S.ev[13]));
     bool ovf = false;
     for (int i = 0; i < S.n; i++) ovf = ovf || (S.pairs[i].gpu_bins && S.hovf[i] != 0);
     if (ovf) {
