@@ -27,7 +27,7 @@ STAGES = ["desc0", "desc1", "can_left", "can_lr", "can_incon", "can_final", "sp0
           "grid1", "tri0", "tri1", "mean0", "mean1", "d1_raw", "d2_raw", "D1", "D2"]
 KERNELS = ["descriptor", "support_left", "support_right", "dense", "lr"]  # elas_kernel order
 TIMERS = ["descriptor", "support", "filter", "compact", "grid", "triangle", "plane", "dense", "lr",
-          "post", "total", "phase_a", "phase_b", "sp_download", "tri_wall", "tri_job", "bin_job"]
+          "post", "total", "phase_a", "phase_b", "sp_download", "tri_wall", "tri_job"]
 
 
 class ElasParams(ctypes.Structure):
@@ -446,10 +446,9 @@ class Elas:
         self._check(st)
         return [(d1[i], d2[i], int(stat[i])) for i in range(n)]
 
-    def set_debug(self, on=True, bin_fallback=False):
-        """Keep every intermediate for fetch(); bin_fallback forces the host
-        raster-binning path (a test of the GPU bins' overflow fallback)."""
-        self._check(self._lib.elas_hip_set_debug(self._ctx, (1 if on else 0) | (2 if bin_fallback else 0)))
+    def set_debug(self, on=True):
+        """Keep every intermediate of the next calls for fetch()."""
+        self._check(self._lib.elas_hip_set_debug(self._ctx, 1 if on else 0))
 
     def shape_info(self):
         info = ShapeInfo()

@@ -230,7 +230,6 @@ int elas_hip_set_debug(elas_ctx* c, int keep) {
     if (!c) return ELAS_ERR_INVALID_ARGUMENT;
     ELAS_NOT_SERVING(c);
     c->debug = (keep & 1) != 0;
-    c->bin_test = (keep & 2) != 0;
     c->param_gen++;
     return ELAS_OK;
 }

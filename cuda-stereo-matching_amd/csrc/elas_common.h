@@ -13,13 +13,16 @@
 #define ELAS_DENSE_ROWS 3    // image rows per dense tile (a wave = 20 px x 3 rows)
 #define ELAS_DENSE_WAVES 8   // waves per dense workgroup; a wave covers dcpw cells per step
 #define ELAS_DENSE_THREADS (64 * ELAS_DENSE_WAVES)
-#define ELAS_RASTER_T 32     // raster tile edge in pixels
 #define ELAS_MAX_DISP 511    // disp_max envelope (dense key packs d in 10 bits)
 
 // Per-triangle raster record (16 floats): plane and span edges, computed once.
 //   [0..2] plane a, b, c    [3..8] A_u, A_v, B_u, B_v, C_u, C_v (sorted by u)
 //   [9..14] AB_a, AB_b, AC_a, AC_b, BC_a, BC_b         [15] unused
 #define ELAS_TRIREC 16
+
+#define ELAS_RT_W 64         // raster tile: 64 x 32 pixels per workgroup
+#define ELAS_RT_H 32
+#define ELAS_ST_MAX 1024     // culling super-tiles per image, at most (LDS masks of k_plane)
 
 #define ELAS_DESC_SW 58      // descriptor strip: output columns of a 64-lane wave
 #define ELAS_DESC_SH 18      // descriptor strip: output rows by default (18 + 6 rows walked);
@@ -49,6 +52,7 @@ struct ElasPair {
     int32_t dsh;             // descriptor strip rows (6, 12 or 18)
     int32_t ntri[2];         // triangles of the left / right mesh (host-known)
     int32_t nblk;            // 256-cell blocks of the support lattice
+    int32_t sshift, nsx, nst;  // culling super-tiles: (1 << sshift)-pixel squares, nsx per row, nst in all
     const uint8_t* img[2];   // [H][W]
     uint8_t* desc[2];        // [H][W][16]
     int32_t* can[4];         // [ch][cw]: left pass, LR pass, inconsistency, final
@@ -60,16 +64,9 @@ struct ElasPair {
     int32_t* glist[2];       // [gh*gw][dmax+2] reference list layout (debug only)
     const int32_t* tri[2];   // [ntri][3]
     float* trirec[2];        // [ntri][ELAS_TRIREC]
-    int32_t ntx, nty;        // raster tiles (ELAS_RASTER_TILE px) across / down
-    const int32_t* toff[2];  // [ntx*nty+1] tile -> range of tlist
-    const int32_t* tlist[2]; // triangles meeting each tile (any order)
-    // raster bins built on the GPU (toff/tlist point here when gpu_bins)
-    int32_t* bcnt[2];        // [ntx*nty+1] triangles per tile, then list offsets
-    int32_t* bcur[2];        // [ntx*nty] fill cursors
-    int32_t* blist[2];       // [bcap] tile lists
-    int32_t bcap;            // list capacity per image
-    int32_t gpu_bins;        // 1: bins built on the GPU; 0: uploaded from the host
-    int32_t* bovf;           // set when the lists outgrow bcap (the host then bins)
+    int32_t* tbox[2];        // [ntri][4] pixel box of the triangle's spans: u0, u1, v0, v1
+                             //           (half-open, all 0 when empty)
+    unsigned long long* smask[2];  // [ceil(ntri/64)][nst] bit k of [c][s]: triangle 64c+k meets super-tile s
     int32_t* prior[2];       // [H][W] (int)plane(u,v) of the covering triangle
                              //        with the highest index, -1 if uncovered
     float* mean[2];          // [H][W] (float)prior: the reference's map (debug only)
@@ -114,9 +111,10 @@ struct ElasLaunch {
 // Stage ids (one prefix table per stage).  S_DENSE holds the dense units of
 // the pairs tiled with one cell per wave, S_DENSE2 those with two; S_DESC,
 // S_DESC12 and S_DESC6 the descriptor strips of the pairs with 18-, 12- and
-// 6-row strips.
+// 6-row strips.  S_PLANE counts 64-triangle chunks of both meshes, S_RTILES
+// the raster tiles of both images.
 enum ElasStageId {
     S_DESC = 0, S_SUPPORT, S_INCON, S_REDUN_H, S_REDUN_V, S_BLOCKS, S_GRID_CELLS,
-    S_PLANE, S_RASTER, S_DENSE, S_PIXELS, S_GRID_FILL, S_POST_TILES, S_POST_ROWS, S_POST_COLS,
-    S_BIN_CLEAR, S_BIN_SCAN, S_DENSE2, S_DESC12, S_DESC6, S_NUM_STAGES
+    S_PLANE, S_RTILES, S_DENSE, S_PIXELS, S_GRID_FILL, S_POST_TILES, S_POST_ROWS, S_POST_COLS,
+    S_DENSE2, S_DESC12, S_DESC6, S_NUM_STAGES
 };

@@ -91,19 +91,8 @@ size_t support_lds(int step, int dmax, bool right) {
 
 bool post_scratch(const elas_params& p) { return p.speckle_size > 0 || p.filter_adaptive_mean || p.filter_median; }
 
-size_t bin_tiles(int W, int H) {
-    return (size_t)((W + ELAS_RASTER_T - 1) / ELAS_RASTER_T) * ((H + ELAS_RASTER_T - 1) / ELAS_RASTER_T);
-}
-
-// capacity of one image's GPU raster bins: room for every triangle in 4 tiles
-// plus 4 per tile; a mesh that needs more is binned on the host instead.
-// `test` shrinks it to 1 so that tests exercise that fallback.
-int32_t bin_cap(const ElasDims& d, size_t ntiles, bool test) {
-    return test ? 1 : (int32_t)(4 * (size_t)d.tricap + 4 * ntiles);
-}
-
 // device bytes of one pair in a slot arena
-size_t pair_arena(const ElasPairSpec& s, const ElasDims& d, bool debug, bool post, bool bin_test) {
+size_t pair_arena(const ElasPairSpec& s, const ElasDims& d, bool debug, bool post) {
     size_t b = 0;
     if (!s.dev_in) b += 2 * al(d.npx);
     b += 2 * al(16 * d.npx);
@@ -111,11 +100,10 @@ size_t pair_arena(const ElasPairSpec& s, const ElasDims& d, bool debug, bool pos
     b += 4 * al(4 * d.ncell * d.words);
     if (debug) b += 2 * al(4 * d.ncell * (size_t)(s.dmax + 2)) + 2 * al(4 * d.npx);
     b += 2 * al(12 * (size_t)d.tricap) + 2 * al(4 * ELAS_TRIREC * (size_t)d.tricap);
+    b += 2 * al(16 * (size_t)d.tricap) + 2 * al(8 * (size_t)d.nst * cdiv(d.tricap, 64));  // boxes, culling masks
     b += 2 * al(4 * d.npx) + 2 * al(4 * d.npx);  // prior, draw
     if (!s.dev_out) b += 2 * al(4 * d.npx);
     if (post) b += 2 * al(8 * d.npx);  // post-filter scratch
-    const size_t nt = bin_tiles(s.W, s.H);
-    b += 2 * al(4 * (nt + 1)) + 2 * al(4 * nt) + 2 * al(4 * (size_t)bin_cap(d, nt, bin_test));
     return b;
 }
 
@@ -138,11 +126,17 @@ ElasDims elas_dims(const elas_params& p, int W, int H, int dmax) {
     d.ncell = (size_t)d.gw * d.gh;
     d.nblk = (int32_t)((d.ncan + 255) / 256);
     d.tricap = (int32_t)(2 * d.ncan + 8);
+    // raster culling super-tiles: 128-pixel squares, larger where an image
+    // would need more than ELAS_ST_MAX of them
+    d.sshift = 7;
+    while ((long)cdiv(W, 1 << d.sshift) * cdiv(H, 1 << d.sshift) > ELAS_ST_MAX) d.sshift++;
+    d.nsx = cdiv(W, 1 << d.sshift);
+    d.nst = d.nsx * cdiv(H, 1 << d.sshift);
     return d;
 }
 
 size_t elas_pair_device_bytes(const elas_params& p, const ElasPairSpec& s, bool debug) {
-    return pair_arena(s, elas_dims(p, s.W, s.H, s.dmax), debug, post_scratch(p), false);
+    return pair_arena(s, elas_dims(p, s.W, s.H, s.dmax), debug, post_scratch(p));
 }
 
 int elas_make_kparams(const elas_params* p, ElasKParams* kp, std::string* err) {
@@ -217,9 +211,7 @@ void elas_ctx::slot_free(ElasSlot& S) {
         if (e) (void)hipEventDestroy(e);
     if (S.arena) (void)hipFree(S.arena);
     if (S.table) (void)hipFree(S.table);
-    if (S.blist) (void)hipFree(S.blist);
     if (S.pin) (void)hipHostFree(S.pin);
-    if (S.blist_pin) (void)hipHostFree(S.blist_pin);
     if (S.tune) (void)hipFree(S.tune);
     if (S.stream) (void)hipStreamDestroy(S.stream);
     if (S.stream2) (void)hipStreamDestroy(S.stream2);
@@ -244,7 +236,7 @@ int elas_ctx::slot_need(const ElasPairSpec* sp, int n, size_t* arena_need, size_
         if (prm.ipol_gap_width > 0 && std::max(s.W, s.H) > ELAS_GAP_MAX_LINE)
             return fail(ELAS_ERR_UNSUPPORTED, "gap interpolation supports lines up to 15872 pixels");
         const ElasDims d = elas_dims(prm, s.W, s.H, s.dmax);
-        a += pair_arena(s, d, debug, post_scratch(prm), bin_test);
+        a += pair_arena(s, d, debug, post_scratch(prm));
         psp += 2 * 8 * d.ncan;
         ptri += 2 * 12 * (size_t)d.tricap;
     }
@@ -317,7 +309,7 @@ int elas_ctx::slot_prepare(ElasSlot& S, const ElasPairSpec* sp, int n) {
     for (int i = 0; i < n; i++) {
         S.dd[i] = elas_dims(prm, sp[i].W, sp[i].H, sp[i].dmax);
         S.base[i] = a;
-        a += pair_arena(sp[i], S.dd[i], debug, post_scratch(prm), bin_test);
+        a += pair_arena(sp[i], S.dd[i], debug, post_scratch(prm));
         S.sp_off[i] = so;
         so += 2 * 2 * S.dd[i].ncan;
         S.tri_off[i] = to;
@@ -341,13 +333,11 @@ int elas_ctx::slot_prepare(ElasSlot& S, const ElasPairSpec* sp, int n) {
     S.dprefix[0] = (int32_t*)(T + S.tab_pairs);
     S.dprefix[1] = (int32_t*)(T + S.tab_one + S.tab_pairs);
     S.dnsp = (int32_t*)(T + S.tab_pairs + S.tab_prefix);
-    S.dovf = (int32_t*)(T + S.tab_one + S.tab_pairs + S.tab_prefix);
     S.hpairs[0] = (ElasPair*)PIN;
     S.hprefix[0] = (int32_t*)(PIN + S.tab_pairs);
     S.hnsp = (int32_t*)(PIN + S.tab_pairs + S.tab_prefix);
     S.hpairs[1] = (ElasPair*)(PIN + S.tab_one);
     S.hprefix[1] = (int32_t*)(PIN + S.tab_one + S.tab_pairs);
-    S.hovf = (int32_t*)(PIN + S.tab_one + S.tab_pairs + S.tab_prefix);
     S.hsp = (float*)(PIN + 2 * S.tab_one);
     S.htri = (int32_t*)(PIN + 2 * S.tab_one + al(psp));
 
@@ -388,25 +378,15 @@ int elas_ctx::slot_prepare(ElasSlot& S, const ElasPairSpec* sp, int n) {
         }
         for (int k = 0; k < 2; k++) P.tri[k] = (int32_t*)take(12 * (size_t)d.tricap);
         for (int k = 0; k < 2; k++) P.trirec[k] = (float*)take(4 * ELAS_TRIREC * (size_t)d.tricap);
+        for (int k = 0; k < 2; k++) P.tbox[k] = (int32_t*)take(16 * (size_t)d.tricap);
+        for (int k = 0; k < 2; k++) P.smask[k] = (unsigned long long*)take(8 * (size_t)d.nst * cdiv(d.tricap, 64));
+        P.sshift = d.sshift; P.nsx = d.nsx; P.nst = d.nst;
         for (int k = 0; k < 2; k++) P.prior[k] = (int32_t*)take(4 * d.npx);
-        P.ntx = (s.W + ELAS_RASTER_T - 1) / ELAS_RASTER_T;
-        P.nty = (s.H + ELAS_RASTER_T - 1) / ELAS_RASTER_T;
         for (int k = 0; k < 2; k++) P.draw[k] = (float*)take(4 * d.npx);
         if (!s.dev_out) { P.dout[0] = (float*)take(4 * d.npx); P.dout[1] = (float*)take(4 * d.npx); }
         else { P.dout[0] = s.D1; P.dout[1] = s.D2; }
         if (post_scratch(prm))
             for (int k = 0; k < 2; k++) P.pbuf[k] = (int32_t*)take(8 * d.npx);
-        const size_t nt = bin_tiles(s.W, s.H);
-        P.bcap = bin_cap(d, nt, bin_test);
-        for (int k = 0; k < 2; k++) P.bcnt[k] = (int32_t*)take(4 * (nt + 1));
-        for (int k = 0; k < 2; k++) P.bcur[k] = (int32_t*)take(4 * nt);
-        for (int k = 0; k < 2; k++) P.blist[k] = (int32_t*)take(4 * (size_t)P.bcap);
-        for (int k = 0; k < 2; k++) {
-            P.toff[k] = P.bcnt[k];
-            P.tlist[k] = P.blist[k];
-        }
-        P.gpu_bins = 1;
-        P.bovf = S.dovf + i;
     }
     return ELAS_OK;
 }
@@ -427,16 +407,14 @@ int stage_tiles(const elas_params& prm, const ElasPairSpec& s, const ElasDims& d
         case S_BLOCKS: return d.nblk;
         case S_GRID_CELLS: return cdiv(2 * (long)d.ncell, 256);
         case S_GRID_FILL: return cdiv(2 * (long)d.ncell * d.words, 256);
-        case S_PLANE: return cdiv((long)P.ntri[0] + P.ntri[1], 64);
-        case S_RASTER: return 2 * P.ntx * P.nty;
+        case S_PLANE: return cdiv(P.ntri[0], 64) + cdiv(P.ntri[1], 64);
+        case S_RTILES: return 2 * cdiv(s.W, ELAS_RT_W) * cdiv(s.H, ELAS_RT_H);
         case S_DENSE: return P.dcpw == 1 ? dense_units(d, gs, 1, P.dspc) : 0;
         case S_DENSE2: return P.dcpw == 2 ? dense_units(d, gs, 2, P.dspc) : 0;
         case S_PIXELS: return cdiv((long)d.npx, 256);
         case S_POST_TILES: return 2 * cdiv(s.W, ELAS_POST_TW) * cdiv(s.H, ELAS_POST_TH);
         case S_POST_ROWS: return 2 * s.H;
         case S_POST_COLS: return 2 * s.W;
-        case S_BIN_CLEAR: return cdiv(2 * ((long)bin_tiles(s.W, s.H) + 1) + 1, 256);
-        case S_BIN_SCAN: return 2;
     }
     return 0;
 }
@@ -534,7 +512,8 @@ int elas_ctx::slot_phase_a(ElasSlot& S) {
         if (r != ELAS_OK) return r;
     }
     HIPCHECK(hipEventRecord(S.ev[5], s1));
-    // disparity grid on the second stream, overlapping the host Triangle step
+    // disparity grid (and the prior fill) on the second stream, overlapping
+    // the host Triangle step
     HIPCHECK(hipStreamWaitEvent(s2, S.ev[5], 0));
     HIPCHECK(hipEventRecord(S.ev[11], s2));
     { ElasLaunch L = launch_of(S, 0, S_GRID_FILL); if (L.total) elas_launch_grid_fill(L, g(L, cap256), s2); }
@@ -570,8 +549,6 @@ int elas_ctx::slot_start_tri(ElasSlot& S) {
     HIPCHECK(poll_event(S.ev[14]));
     S.t_sp = clk::now();
     S.tri_out.assign(2 * (size_t)n, {});
-    S.toff.assign(2 * (size_t)n, {});
-    S.tlist.assign(2 * (size_t)n, {});
     S.jobs.clear();
     S.job_of.assign(2 * (size_t)n, -1);
     for (int i = 0; i < n; i++) {
@@ -583,10 +560,6 @@ int elas_ctx::slot_start_tri(ElasSlot& S) {
             j.n = S.nsp[i];
             j.out = &S.tri_out[2 * i + k];
             j.result = 0;
-            j.W = S.specs[i].W;
-            j.H = S.specs[i].H;
-            j.tile_off = nullptr;  // raster bins are built on the GPU
-            j.tile_list = nullptr;
             S.jobs.push_back(j);
         }
     }
@@ -622,15 +595,7 @@ int elas_ctx::slot_phase_b(ElasSlot& S) {
             if (nt[0] > S.dd[i].tricap || nt[1] > S.dd[i].tricap)
                 return fail(ELAS_ERR_HIP, "triangle count exceeds capacity");
         }
-        if (S.status[i] != ELAS_OK) {
-            nt[0] = nt[1] = 0;
-            // a pair without a mesh rasterises nothing: every tile's list is empty
-            for (int k = 0; k < 2; k++)
-                if (S.job_of[2 * i + k] >= 0) {
-                    std::fill(S.toff[2 * i + k].begin(), S.toff[2 * i + k].end(), 0);
-                    S.tlist[2 * i + k].clear();
-                }
-        }
+        if (S.status[i] != ELAS_OK) nt[0] = nt[1] = 0;  // a pair without a mesh rasterises nothing
         S.ntri0[i] = nt[0];
         S.ntri1[i] = nt[1];
         S.pairs[i].ntri[0] = nt[0];
@@ -688,19 +653,16 @@ int elas_ctx::enqueue_b(ElasSlot& S, bool timed) {
     auto g = [&](const ElasLaunch& L, int cap) { return std::min(L.total, cap); };
     HIPCHECK(ev(15));
     {
-        // plane records + bin counts, bin offsets, bin lists; the triangle
-        // lanes' grid comes from the largest possible triangle count (a shape
-        // property; the kernels read the actual count from the table)
-        long maxtri = 0;
-        for (int i = 0; i < n; i++) maxtri += 2L * S.dd[i].tricap;
-        const int tgrid = std::max(1, std::min(4 * cap256, cdiv(maxtri, 64)));
-        ElasLaunch Lc = launch_of(S, 1, S_BIN_CLEAR), Lt = launch_of(S, 1, S_PLANE), Ls = launch_of(S, 1, S_BIN_SCAN);
-        if (Lc.total) elas_launch_bin_clear(Lc, g(Lc, cap256), s1);
-        elas_launch_plane(Lt, tgrid, s1);
-        if (Ls.total) elas_launch_bin_scan(Ls, g(Ls, std::max(1, num_cu) * 2), s1);
-        elas_launch_bin_fill(Lt, tgrid, s1);
+        // plane records, boxes and culling masks, then the tile raster that
+        // writes the priors.  Triangle counts are data: the plane grid comes
+        // from the largest possible count (a shape property) and the kernel
+        // reads the actual count from the table
+        long maxchunks = 0;
+        for (int i = 0; i < n; i++) maxchunks += 2L * cdiv(S.dd[i].tricap, 64);
+        elas_launch_plane(launch_of(S, 1, S_PLANE), (int)std::max(1L, std::min<long>(2 * cap256, maxchunks)), s1);
+        ElasLaunch L = launch_of(S, 1, S_RTILES);
+        if (L.total) elas_launch_rtile(L, g(L, cap256), s1);
     }
-    { ElasLaunch L = launch_of(S, 1, S_RASTER); if (L.total) elas_launch_raster(L, g(L, cap256), s1); }
     HIPCHECK(ev(7));
     for (int c = 1; c <= 2; c++) {
         ElasLaunch L = launch_of(S, 1, c == 1 ? S_DENSE : S_DENSE2);
@@ -726,91 +688,6 @@ int elas_ctx::enqueue_b(ElasSlot& S, bool timed) {
         if (in_tmp && Lp.total) elas_launch_post_copy(Lp, kp, cap256, s1);
     }
     HIPCHECK(hipGetLastError());
-    HIPCHECK(hipMemcpyAsync(S.hovf, S.dovf, 4 * (size_t)n, hipMemcpyDeviceToHost, s1));
-    return ELAS_OK;
-}
-
-// Raster bins on the host, for a chunk whose GPU bins outgrew their capacity
-// (a mesh of unusually large triangles): bin every mesh, upload
-// [toff0 | toff1 | tlist0 | tlist1] per pair, and re-run phase B eagerly.
-// Synchronous; the results are those of the GPU-binned path.
-int elas_ctx::slot_host_bins(ElasSlot& S) {
-    const int n = S.n;
-    S.toff.assign(2 * (size_t)n, {});
-    S.tlist.assign(2 * (size_t)n, {});
-    for (int i = 0; i < n; i++)
-        for (int k = 0; k < 2; k++) {
-            const int nt = k ? S.ntri1[i] : S.ntri0[i];
-            const float* pts = S.hsp + S.sp_off[i] + (size_t)k * 2 * S.dd[i].ncan;
-            elas_bin_triangles(pts, nt > 0 ? S.tri_out[2 * i + k].data() : nullptr, nt, S.specs[i].W, S.specs[i].H,
-                               S.toff[2 * i + k], S.tlist[2 * i + k]);
-            S.job_of[2 * i + k] = 0;  // every mesh has bins now
-        }
-    size_t bl_ints = 0;
-    std::vector<size_t> bl_off(n);
-    for (int i = 0; i < n; i++) {
-        const size_t nt = (size_t)S.pairs[i].ntx * S.pairs[i].nty + 1;
-        bl_off[i] = bl_ints;
-        bl_ints += 2 * nt;
-        for (int k = 0; k < 2; k++) bl_ints += (S.job_of[2 * i + k] >= 0) ? S.tlist[2 * i + k].size() : 0;
-        bl_ints = (bl_ints + 63) & ~(size_t)63;
-    }
-    const size_t bl_want = 4 * bl_ints + 4 * bl_ints / 4 + 256;
-    if (4 * bl_ints > S.blist_bytes) {
-        if (S.blist) (void)hipFree(S.blist);
-        S.blist = nullptr;
-        S.blist_bytes = 0;
-        if (hipMalloc(&S.blist, bl_want) != hipSuccess) {
-            S.blist = nullptr;
-            (void)hipGetLastError();
-            return fail(ELAS_ERR_OUT_OF_MEMORY, "raster bin allocation failed");
-        }
-        S.blist_bytes = bl_want;
-    }
-    if (4 * bl_ints > S.blist_pin_bytes) {
-        if (S.blist_pin) (void)hipHostFree(S.blist_pin);
-        S.blist_pin = nullptr;
-        S.blist_pin_bytes = 0;
-        if (hipHostMalloc(&S.blist_pin, bl_want, hipHostMallocDefault) != hipSuccess) {
-            S.blist_pin = nullptr;
-            (void)hipGetLastError();
-            return fail(ELAS_ERR_OUT_OF_MEMORY, "raster bin staging allocation failed");
-        }
-        S.blist_pin_bytes = bl_want;
-    }
-    {
-        int32_t* hb = (int32_t*)S.blist_pin;
-        int32_t* db = (int32_t*)S.blist;
-        for (int i = 0; i < n; i++) {
-            const size_t nt = (size_t)S.pairs[i].ntx * S.pairs[i].nty + 1;
-            size_t o = bl_off[i];
-            for (int k = 0; k < 2; k++) {
-                int32_t* dst = hb + o;
-                if (S.job_of[2 * i + k] >= 0) memcpy(dst, S.toff[2 * i + k].data(), 4 * nt);
-                else memset(dst, 0, 4 * nt);
-                S.pairs[i].toff[k] = db + o;
-                o += nt;
-            }
-            for (int k = 0; k < 2; k++) {
-                const size_t nl = (S.job_of[2 * i + k] >= 0) ? S.tlist[2 * i + k].size() : 0;
-                if (nl) memcpy(hb + o, S.tlist[2 * i + k].data(), 4 * nl);
-                S.pairs[i].tlist[k] = db + o;
-                o += nl;
-            }
-        }
-        if (bl_ints) HIPCHECK(hipMemcpyAsync(S.blist, S.blist_pin, 4 * bl_ints, hipMemcpyHostToDevice, S.stream));
-    }
-    for (int i = 0; i < n; i++) S.pairs[i].gpu_bins = 0;
-    memcpy(S.hpairs[1], S.pairs.data(), sizeof(ElasPair) * n);
-    int r = enqueue_b(S, false);
-    if (r != ELAS_OK) return r;
-    for (int i = 0; i < n; i++)
-        if (!S.specs[i].dev_out) {
-            HIPCHECK(hipMemcpyAsync(S.specs[i].D1, S.pairs[i].dout[0], 4 * S.dd[i].npx, hipMemcpyDeviceToHost, S.stream));
-            HIPCHECK(hipMemcpyAsync(S.specs[i].D2, S.pairs[i].dout[1], 4 * S.dd[i].npx, hipMemcpyDeviceToHost, S.stream));
-        }
-    HIPCHECK(hipStreamSynchronize(S.stream));
-    S.bin_fallbacks++;
     return ELAS_OK;
 }
 
@@ -885,12 +762,6 @@ bool elas_ctx::slot_b_ready(ElasSlot& S) { return hipEventQuery(S.ev[13]) == hip
 int elas_ctx::slot_finish(ElasSlot& S, int32_t* statuses) {
     HIPCHECK(hipSetDevice(device));
     HIPCHECK(poll_event(S.ev[13]));
-    bool ovf = false;
-    for (int i = 0; i < S.n; i++) ovf = ovf || (S.pairs[i].gpu_bins && S.hovf[i] != 0);
-    if (ovf) {
-        const int r = slot_host_bins(S);
-        if (r != ELAS_OK) return r;
-    }
     S.t_end = clk::now();
     auto el = [&](int a, int b) {
         float ms = 0.f;
@@ -916,13 +787,9 @@ int elas_ctx::slot_finish(ElasSlot& S, int32_t* statuses) {
     auto msd = [](clk::time_point a, clk::time_point b) { return std::chrono::duration<float, std::milli>(b - a).count(); };
     S.times[ELAS_T_SP_DOWNLOAD] = msd(S.t_tri0, S.t_sp);
     S.times[ELAS_T_TRI_WALL] = msd(S.t_sub, S.t_done);
-    float tj = 0.f, bj = 0.f;
-    for (const ElasTriJob& j : S.jobs) {
-        tj = std::max(tj, (float)((j.t_tri - j.t_begin) * 1e3));
-        bj = std::max(bj, (float)((j.t_end - j.t_tri) * 1e3));
-    }
+    float tj = 0.f;
+    for (const ElasTriJob& j : S.jobs) tj = std::max(tj, (float)((j.t_end - j.t_begin) * 1e3));
     S.times[ELAS_T_TRI_JOB] = tj;
-    S.times[ELAS_T_BIN_JOB] = bj;
     if (statuses)
         for (int i = 0; i < S.n; i++) statuses[i] = S.status[i];
     if (autotune && force_cpw == 0 && force_dsh == 0)

@@ -37,6 +37,7 @@ struct ElasPairSpec {
 // Derived sizes of one pair.
 struct ElasDims {
     int32_t cw, ch, gw, gh, words, nblk, tricap;
+    int32_t sshift, nsx, nst;  // raster culling super-tiles
     size_t npx, ncan, ncell;
 };
 
@@ -60,13 +61,9 @@ struct ElasSlot {
     size_t arena_bytes = 0;
     void* table = nullptr;  // 2 x (ElasPair[n] + prefix[S_NUM_STAGES][n+1] + nsp[n])
     size_t table_bytes = 0;
-    void* blist = nullptr;  // raster tile offsets + lists (known after Triangle)
-    size_t blist_bytes = 0;
     // pinned host staging
     void* pin = nullptr;
     size_t pin_bytes = 0;
-    void* blist_pin = nullptr;
-    size_t blist_pin_bytes = 0;
     void* tune = nullptr;  // dense tiling candidates: pair records + prefixes
     size_t tune_bytes = 0;
 
@@ -86,14 +83,12 @@ struct ElasSlot {
     ElasPair* dpairs[2] = {nullptr, nullptr};
     int32_t* dprefix[2] = {nullptr, nullptr};
     int32_t* dnsp = nullptr;
-    int32_t* dovf = nullptr;  // per pair: GPU raster bins outgrew their capacity
     ElasPair* hpairs[2] = {nullptr, nullptr};
     int32_t* hprefix[2] = {nullptr, nullptr};
     int32_t* hnsp = nullptr;
-    int32_t* hovf = nullptr;
     float* hsp = nullptr;
     int32_t* htri = nullptr;
-    std::vector<std::vector<int32_t>> tri_out, toff, tlist;
+    std::vector<std::vector<int32_t>> tri_out;
     std::vector<ElasTriJob> jobs;
     std::vector<int> job_of;
     ElasTriTicket* ticket = nullptr;
@@ -101,7 +96,7 @@ struct ElasSlot {
     float times[ELAS_T_COUNT] = {};
     // graphs of this slot's shape buckets
     std::vector<ElasGraphEntry> graphs;
-    uint64_t graph_clock = 0, graph_captures = 0, bin_fallbacks = 0;
+    uint64_t graph_clock = 0, graph_captures = 0;
     bool graphed = false;  // the chunk in flight runs its phases as graphs
 };
 
@@ -121,7 +116,6 @@ struct elas_ctx {
     elas_params prm;
     ElasKParams kp;
     bool debug = false;
-    bool bin_test = false;  // raster bin capacity 1 (tests the host-binning fallback)
     int tri_workers = ELAS_TRI_INSTANCES;
     int chunk = 0;  // pairs per pipeline chunk of a batch (0: whole batch)
     bool use_graphs = true;   // phases A and B replayed as graphs per shape bucket
@@ -174,7 +168,6 @@ struct elas_ctx {
     int enqueue_b(ElasSlot& S, bool timed);
     int slot_graph(ElasSlot& S, int which, hipGraphExec_t* out);
     void slot_drop_graphs(ElasSlot& S);
-    int slot_host_bins(ElasSlot& S);
 };
 
 // Serving loop (elas_serve.cpp).

@@ -17,10 +17,7 @@ void elas_launch_grid_scatter(const ElasLaunch& L, int grid, hipStream_t s);
 void elas_launch_grid_dilate(const ElasLaunch& L, int grid, hipStream_t s);
 void elas_launch_grid_list(const ElasLaunch& L, int grid, hipStream_t s);
 void elas_launch_plane(const ElasLaunch& L, int grid, hipStream_t s);
-void elas_launch_bin_clear(const ElasLaunch& L, int grid, hipStream_t s);
-void elas_launch_bin_scan(const ElasLaunch& L, int grid, hipStream_t s);
-void elas_launch_bin_fill(const ElasLaunch& L, int grid, hipStream_t s);
-void elas_launch_raster(const ElasLaunch& L, int grid, hipStream_t s);
+void elas_launch_rtile(const ElasLaunch& L, int grid, hipStream_t s);
 void elas_launch_mean(const ElasLaunch& L, int grid, hipStream_t s);
 void elas_launch_dense(const ElasLaunch& L, const ElasKParams& kp, int cpw, int grid, size_t lds, hipStream_t s);
 // Lets the dense kernels allocate up to `bytes` of dynamic LDS; 0 on success.

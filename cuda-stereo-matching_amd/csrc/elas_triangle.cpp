@@ -171,14 +171,10 @@ int run_instance(int k, ElasTriJob& j) {
     j.t_begin = now_s();
     if (j.n < 3) {  // Triangle would exit(1) (src/triangle.cpp:7652-7654)
         j.out->clear();
-        j.t_tri = now_s();
-        if (j.tile_off) elas_bin_triangles(j.pts, nullptr, 0, j.W, j.H, *j.tile_off, *j.tile_list);
         j.t_end = now_s();
         return -1;
     }
     const int nt = tri_call(k, j.pts, j.n, *j.out);
-    j.t_tri = now_s();
-    if (j.tile_off) elas_bin_triangles(j.pts, j.out->data(), nt < 0 ? 0 : nt, j.W, j.H, *j.tile_off, *j.tile_list);
     j.t_end = now_s();
     return nt;
 }
@@ -266,70 +262,6 @@ private:
 };
 
 }  // namespace
-
-// Covered pixels of a triangle lie in columns [min u, max u): the column
-// spans of the reference rasteriser are half-open between corner columns.
-// Their rows are truncated float evaluations of the edge lines
-// (KerComputeDisparity.cu:119-150), which stay within the corner rows up to
-// rounding: with slopes s, |u| <= U and rows <= V the float error of
-// s*u + (v0 - s*u0) is below 8 * 2^-24 * (3 s U + 3 V) rows.  The bins use
-// the corner rows widened by 2 plus that bound, so every tile holding a
-// covered pixel lists the triangle.
-void elas_bin_triangles(const float* pts, const int32_t* tri, int ntri, int W, int H,
-                        std::vector<int32_t>& off, std::vector<int32_t>& list) {
-    const int T = ELAS_RASTER_TILE;
-    const int ntx = (W + T - 1) / T, nty = (H + T - 1) / T;
-    off.assign((size_t)ntx * nty + 1, 0);
-    thread_local std::vector<int32_t> box;  // per worker, reused across calls
-    box.resize(4 * (size_t)ntri);
-    for (int t = 0; t < ntri; t++) {
-        int cu[3], cv[3];
-        for (int k = 0; k < 3; k++) {
-            const int c = tri[3 * t + k];
-            cu[k] = (int)pts[2 * c];
-            cv[k] = (int)pts[2 * c + 1];
-        }
-        const int umin = std::min(cu[0], std::min(cu[1], cu[2])), umax = std::max(cu[0], std::max(cu[1], cu[2]));
-        const int vmin = std::min(cv[0], std::min(cv[1], cv[2])), vmax = std::max(cv[0], std::max(cv[1], cv[2]));
-        // margin >= 2 + ceil(x), x = 8 (3 smax U + 3 V) / 2^24, evaluated in
-        // float: x is at most a few units here, so 1e-3 covers the rounding of
-        // x and floor(x + 1e-3) + 3 >= ceil(x) + 2; large x takes doubles
-        float smax = 0.0f;
-        for (int a = 0; a < 3; a++)
-            for (int b = a + 1; b < 3; b++)
-                if (cu[a] != cu[b]) smax = std::max(smax, std::fabs((float)(cv[a] - cv[b]) / (float)(cu[a] - cu[b])));
-        const int Ui = std::max(std::abs(umin), std::abs(umax)) + 1, Vi = std::max(std::abs(vmin), std::abs(vmax)) + 1;
-        const float xf = 8.0f * (3.0f * smax * (float)Ui + 3.0f * (float)Vi) * (1.0f / 16777216.0f);
-        int margin;
-        if (xf < 100.0f) {
-            margin = 3 + (int)(xf + 1e-3f);
-        } else {
-            double sd = 0.0;
-            for (int a = 0; a < 3; a++)
-                for (int b = a + 1; b < 3; b++)
-                    if (cu[a] != cu[b]) sd = std::max(sd, std::fabs((double)(cv[a] - cv[b]) / (double)(cu[a] - cu[b])));
-            margin = 2 + (int)std::ceil(8.0 * (3.0 * sd * Ui + 3.0 * Vi) / 16777216.0);
-        }
-        const int u0 = std::max(umin, 0), u1 = std::min(umax, W);                      // [u0, u1)
-        const int v0 = std::max(vmin - margin, 0), v1 = std::min(vmax + margin + 1, H);  // [v0, v1)
-        int* b = &box[4 * (size_t)t];
-        if (u0 >= u1 || v0 >= v1) { b[0] = 1; b[1] = 0; b[2] = 1; b[3] = 0; continue; }
-        b[0] = u0 / T; b[1] = (u1 - 1) / T; b[2] = v0 / T; b[3] = (v1 - 1) / T;
-        for (int ty = b[2]; ty <= b[3]; ty++)
-            for (int tx = b[0]; tx <= b[1]; tx++) off[(size_t)ty * ntx + tx + 1]++;
-    }
-    for (size_t i = 1; i < off.size(); i++) off[i] += off[i - 1];
-    list.resize((size_t)off.back());
-    // fill with running cursors kept in off[1..] (restored afterwards)
-    for (int t = 0; t < ntri; t++) {
-        const int* b = &box[4 * (size_t)t];
-        for (int ty = b[2]; ty <= b[3]; ty++)
-            for (int tx = b[0]; tx <= b[1]; tx++) list[(size_t)off[(size_t)ty * ntx + tx]++] = t;
-    }
-    // off[i] now holds the end of tile i = the start of tile i + 1
-    for (size_t i = off.size() - 1; i > 0; i--) off[i] = off[i - 1];
-    off[0] = 0;
-}
 
 void elas_triangulate_jobs(ElasTriJob* jobs, int n, int max_workers) {
     TriPool::get().run(jobs, n, max_workers);

@@ -5,26 +5,14 @@
 
 #define ELAS_TRI_INSTANCES 16
 
-#define ELAS_RASTER_TILE 32
-
 struct ElasTriJob {
     const float* pts;            // [n][2] support points
     int n;
     std::vector<int32_t>* out;   // [ntri][3] zero-based triangles
     int result;                  // ntri, or -1 for fewer than 3 points
-    // optional tile binning of the mesh (done by the same worker): for every
-    // ELAS_RASTER_TILE^2 pixel tile of a W x H image, the ascending list of
-    // triangles whose pixel bounding box meets it
-    int W, H;
-    std::vector<int32_t>* tile_off;   // [ntiles + 1]
-    std::vector<int32_t>* tile_list;  // [tile_off[ntiles]]
-    // filled by the worker: seconds (steady clock) around Triangle and binning
-    double t_begin = 0, t_tri = 0, t_end = 0;
+    // filled by the worker: seconds (steady clock) around the Triangle call
+    double t_begin = 0, t_end = 0;
 };
-
-// Tile binning of triangles with corners pts (pixel coordinates).
-void elas_bin_triangles(const float* pts, const int32_t* tri, int ntri, int W, int H,
-                        std::vector<int32_t>& off, std::vector<int32_t>& list);
 
 // The (x, y) order of a point set when it can be handed to Triangle in place
 // of the caller's order with the same mesh: integer x, no duplicates.  order

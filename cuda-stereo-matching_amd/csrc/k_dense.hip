@@ -41,162 +41,6 @@ __device__ bool elas_solve3(double A[3][3], double B[3]) {
     return true;
 }
 
-// ---- raster bins ------------------------------------------------------------
-// Tiles (ELAS_RASTER_T px) a triangle's covered pixels can fall in.  Covered
-// pixels lie in columns [min u, max u) (the reference's spans are half-open
-// between corner columns, KerComputeDisparity.cu:128-150) and in rows that
-// are truncated float evaluations of the edge lines, within the corner rows
-// up to rounding: with slopes s, |u| <= U and rows <= V the float error of
-// s*u + (v0 - s*u0) is below 8 * 2^-24 * (3 s U + 3 V) rows.  The rows are
-// widened by 3 + that bound (rounded up with room for the float evaluation
-// of the bound itself), so the tile range is a superset; rasterisation
-// tests every pixel exactly.
-__device__ __forceinline__ bool elas_bin_box(const float* rec, int W, int H, int& tx0, int& tx1, int& ty0,
-                                             int& ty1) {
-    const int cu0 = (int)rec[3], cv0 = (int)rec[4], cu1 = (int)rec[5], cv1 = (int)rec[6];
-    const int cu2 = (int)rec[7], cv2 = (int)rec[8];  // corners sorted by u
-    const int vmin = min(cv0, min(cv1, cv2)), vmax = max(cv0, max(cv1, cv2));
-    float smax = 0.0f;
-    if (cu0 != cu1) smax = fmaxf(smax, fabsf((float)(cv0 - cv1) / (float)(cu0 - cu1)));
-    if (cu0 != cu2) smax = fmaxf(smax, fabsf((float)(cv0 - cv2) / (float)(cu0 - cu2)));
-    if (cu1 != cu2) smax = fmaxf(smax, fabsf((float)(cv1 - cv2) / (float)(cu1 - cu2)));
-    const int Ui = max(abs(cu0), abs(cu2)) + 1, Vi = max(abs(vmin), abs(vmax)) + 1;
-    const float xf = 8.0f * (3.0f * smax * (float)Ui + 3.0f * (float)Vi) * (1.0f / 16777216.0f);
-    const int margin = xf < 100.0f ? 3 + (int)(xf + 1e-3f) : 4 + (int)(xf * 1.001f);
-    const int u0 = max(cu0, 0), u1 = min(cu2, W);
-    const int v0 = max(vmin - margin, 0), v1 = min(vmax + margin + 1, H);
-    if (u0 >= u1 || v0 >= v1) return false;
-    tx0 = u0 / ELAS_RASTER_T;
-    tx1 = (u1 - 1) / ELAS_RASTER_T;
-    ty0 = v0 / ELAS_RASTER_T;
-    ty1 = (v1 - 1) / ELAS_RASTER_T;
-    return true;
-}
-
-// The (triangle, tile) items of a 64-lane workgroup's triangles, spread evenly
-// over the lanes: a triangle's box is a few tiles on average but up to a few
-// hundred for slivers along the hull, and a lane walking its own box would
-// hold the whole wave.  Lane i's triangle owns items [incl[i-1], incl[i]); item
-// k is tile (ty0 + k / w, tx0 + k % w) of its box.  Calls f(tag, tile row,
-// tile column) per item.  s holds 5 * 64 ints of LDS.
-template <typename F>
-__device__ __forceinline__ void elas_bin_items(int* s, bool ok, int tx0, int tx1, int ty0, int ty1, int tag, F f) {
-    const int lane = threadIdx.x;
-    const int w = ok ? tx1 - tx0 + 1 : 1;
-    const int nt = ok ? w * (ty1 - ty0 + 1) : 0;
-    int incl = nt;
-    for (int o = 1; o < 64; o <<= 1) {
-        const int x = __shfl_up(incl, o);
-        if (lane >= o) incl += x;
-    }
-    const int total = __shfl(incl, 63);
-    int* s_incl = s;
-    int* s_tx = s + 64;
-    int* s_ty = s + 128;
-    int* s_w = s + 192;
-    int* s_tag = s + 256;
-    s_incl[lane] = incl; s_tx[lane] = tx0; s_ty[lane] = ty0; s_w[lane] = w; s_tag[lane] = tag;
-    elas_lds_barrier();
-    for (int item = lane; item < total; item += 64) {
-        // owner: the number of lanes whose inclusive count is <= item (lane 63's
-        // count, the total, is above every item)
-        int j = 0;
-        #pragma unroll
-        for (int step = 32; step > 0; step >>= 1)
-            if (s_incl[j + step - 1] <= item) j += step;
-        const int k = item - (j ? s_incl[j - 1] : 0);
-        const int ww = s_w[j];
-        const int q = k / ww;
-        f(s_tag[j], s_ty[j] + q, s_tx[j] + (k - q * ww));
-    }
-    elas_lds_barrier();  // the arrays are rewritten by the next grid-stride step
-}
-
-// Clears the per-tile counts of both images and the overflow flag.
-__global__ __launch_bounds__(256) void k_bin_clear(ElasLaunch L) {
-    for (int t = blockIdx.x; t < L.total; t += gridDim.x) {
-        const int p = elas_find_pair(L.prefix, L.npairs, t);
-        const auto& P = L.pairs[p];
-        if (!P.gpu_bins) continue;
-        const int nt1 = P.ntx * P.nty + 1;
-        const int i = (t - L.prefix[p]) * 256 + threadIdx.x;
-        if (i < nt1) elas_st(P.bcnt[0] + i, 0);
-        else if (i < 2 * nt1) elas_st(P.bcnt[1] + i - nt1, 0);
-        else if (i == 2 * nt1) elas_st(P.bovf, 0);
-    }
-}
-
-// Exclusive scan of one image's tile counts into list offsets (and fill
-// cursors); one workgroup per image.  Lists that would outgrow the capacity
-// set the overflow flag and are left empty.
-__global__ __launch_bounds__(1024) void k_bin_scan(ElasLaunch L) {
-    __shared__ int s_sum[1024];
-    const int tid = threadIdx.x;
-    for (int t = blockIdx.x; t < L.total; t += gridDim.x) {
-        const int p = elas_find_pair(L.prefix, L.npairs, t);
-        const auto& P = L.pairs[p];
-        if (!P.gpu_bins) continue;
-        const int img = t - L.prefix[p];
-        const int n = P.ntx * P.nty;
-        int32_t* cnt = P.bcnt[img];
-        int32_t* cur = P.bcur[img];
-        const int C = (n + 1023) / 1024;
-        const int a = min(n, tid * C), b = min(n, a + C);
-        int sum = 0;
-        for (int i = a; i < b; i++) sum += cnt[i];
-        s_sum[tid] = sum;
-        elas_lds_barrier();
-        for (int off = 1; off < 1024; off <<= 1) {
-            const int x = tid >= off ? s_sum[tid - off] : 0;
-            elas_lds_barrier();
-            s_sum[tid] += x;
-            elas_lds_barrier();
-        }
-        const int total = s_sum[1023];
-        const bool over = total > P.bcap;
-        int run = tid > 0 ? s_sum[tid - 1] : 0;
-        for (int i = a; i < b; i++) {
-            const int c = cnt[i];
-            cnt[i] = over ? 0 : run;
-            cur[i] = over ? 0 : run;
-            run += c;
-        }
-        if (tid == 0) {
-            cnt[n] = over ? 0 : total;
-            if (over) elas_st(P.bovf, 1);
-        }
-        elas_lds_barrier();
-    }
-}
-
-// Fills the tile lists from the plane records (which hold the sorted corners),
-// a workgroup's (triangle, tile) items spread over its lanes; the order within
-// a list does not matter to the raster.
-__global__ __launch_bounds__(64) void k_bin_fill(ElasLaunch L) {
-    __shared__ int s_items[5 * 64];
-    const int total = L.prefix[L.npairs];
-    for (int t = blockIdx.x; t < total; t += gridDim.x) {
-        const int p = elas_find_pair(L.prefix, L.npairs, t);
-        const auto& P = L.pairs[p];
-        if (!P.gpu_bins) continue;
-        const int idx = (t - L.prefix[p]) * 64 + threadIdx.x;
-        const int n0 = P.ntri[0], n1 = P.ntri[1];
-        // an image whose lists overflowed has all offsets 0 (k_bin_scan): its
-        // writes below stay within the capacity and are never read
-        const bool live = idx < n0 + n1;
-        const int img = live && idx >= n0;
-        const int ti = img ? idx - n0 : idx;
-        int tx0 = 0, tx1 = 0, ty0 = 0, ty1 = 0;
-        const bool ok = live && elas_bin_box(P.trirec[img] + (size_t)ti * ELAS_TRIREC, P.W, P.H, tx0, tx1, ty0, ty1);
-        const int cap = P.bcap, ntx = P.ntx;
-        elas_bin_items(s_items, ok, tx0, tx1, ty0, ty1, idx, [&](int tag, int ty, int tx) {
-            const int im = tag >= n0;
-            const int pos = atomicAdd(P.bcur[im] + ty * ntx + tx, 1);
-            if (pos < cap) P.blist[im][pos] = tag - (im ? n0 : 0);
-        });
-    }
-}
-
 // Plane coefficients and sorted-corner edge lines of triangle ti of image img.
 __device__ __forceinline__ void elas_plane_rec(const ElasPair& P, int img, int ti, float* rec) {
     const int32_t* tr = P.tri[img] + 3 * (size_t)ti;
@@ -239,37 +83,68 @@ __device__ __forceinline__ void elas_plane_rec(const ElasPair& P, int img, int t
     rec[15] = 0.f;
 }
 
-// ---- per-triangle record: KerComputeDisparity.cu:10-124 -------------------
-// One lane per triangle of either mesh, 64-lane workgroups so that a few
-// thousand triangles spread over many CUs.  Corners come from the left support
-// points sp0 for both meshes; the right mesh shifts them by -d (Elas.cu:105,
+// ---- per-triangle record and culling masks: KerComputeDisparity.cu:10-124 --
+// A 64-lane workgroup takes 64 consecutive triangles of one mesh (chunk c of
+// image img), one lane each.  Corners come from the left support points sp0
+// for both meshes; the right mesh shifts them by -d (Elas.cu:105,
 // KerComputeDisparity.cu:63-66,95-99).  A corner's d is sp0.u - sp1.u (exact
 // in float), which saves a dependent load of the lattice.
+//
+// Besides the record, a lane stores its triangle's pixel box: columns
+// [max(A_u, 0), min(C_u, W)) (the spans are half-open between corner
+// columns) and the corner rows widened by a margin that covers the float
+// error of the span ends.  A span end is the truncation of s*u + b, where
+// s = fl((v_a - v_b) / (u_a - u_b)) and b = fl(v_a - fl(s*u_a)) for integer
+// corners, evaluated at integer u between them as fl(fl(s*u) + b).  Against
+// the exact line through the corners this is off by at most
+// 2^-24 * (5*|s|*U + 2*V) (plus second-order terms), U and V bounding |u| and
+// |v|; the margin is 3 + floor of 8 * (3*smax*U + 3*V) * 2^-24 (a float
+// evaluation of a larger bound), so the span rows stay within
+// [min v - margin, max v + margin].  The box then marks, in LDS, every
+// super-tile ((1 << sshift)-pixel square) it meets; the chunk's masks go out
+// as smask[img][c][s], bit k for triangle 64*c + k.
 __global__ __launch_bounds__(64) void k_plane(ElasLaunch L) {
-    __shared__ int s_items[5 * 64];
-    // the triangle count is data: read the total from the table (a captured
+    __shared__ unsigned long long sm[ELAS_ST_MAX];
+    const int lane = threadIdx.x;
+    // the triangle counts are data: read the total from the table (a captured
     // launch keeps the grid of its shape bucket)
     const int total = L.prefix[L.npairs];
     for (int t = blockIdx.x; t < total; t += gridDim.x) {
         const int p = elas_find_pair(L.prefix, L.npairs, t);
         const auto& P = L.pairs[p];
-        const int idx = (t - L.prefix[p]) * 64 + threadIdx.x;
-        const int n0 = P.ntri[0], n1 = P.ntri[1];
-        const bool live = idx < n0 + n1;
-        const int img = live && idx >= n0;
-        const int ti = img ? idx - n0 : idx;
-        float* rec = P.trirec[img] + (size_t)ti * ELAS_TRIREC;
-        if (live) elas_plane_rec(P, img, ti, rec);
-        if (P.gpu_bins) {
-            // count the triangle in every tile of its box (k_bin_scan and
-            // k_bin_fill follow); the whole wave takes part
-            int tx0 = 0, tx1 = 0, ty0 = 0, ty1 = 0;
-            const bool ok = live && elas_bin_box(rec, P.W, P.H, tx0, tx1, ty0, ty1);
-            const int ntx = P.ntx;
-            elas_bin_items(s_items, ok, tx0, tx1, ty0, ty1, idx, [&](int tag, int ty, int tx) {
-                atomicAdd(P.bcnt[tag >= n0] + ty * ntx + tx, 1);
-            });
+        const int nc0 = (P.ntri[0] + 63) >> 6;
+        int c = t - L.prefix[p];
+        const int img = c >= nc0;
+        if (img) c -= nc0;
+        const int nst = P.nst, nsx = P.nsx, sh = P.sshift;
+        for (int s = lane; s < nst; s += 64) sm[s] = 0ull;
+        __syncthreads();
+        const int ti = c * 64 + lane;
+        if (ti < P.ntri[img]) {
+            float rec[ELAS_TRIREC];
+            elas_plane_rec(P, img, ti, rec);
+            float4* dst = (float4*)(P.trirec[img] + (size_t)ti * ELAS_TRIREC);
+            for (int k = 0; k < 4; k++) dst[k] = make_float4(rec[4 * k], rec[4 * k + 1], rec[4 * k + 2], rec[4 * k + 3]);
+            const float Au = rec[3], Av = rec[4], Bv = rec[6], Cu = rec[7], Cv = rec[8];
+            const float smax = fmaxf(fabsf(rec[9]), fmaxf(fabsf(rec[11]), fabsf(rec[13])));
+            const float U = fmaxf(fabsf(Au), fabsf(Cu)) + 1.0f;
+            const float V = fmaxf(fabsf(Av), fmaxf(fabsf(Bv), fabsf(Cv))) + 1.0f;
+            const float x = 8.0f * (3.0f * smax * U + 3.0f * V) * (1.0f / 16777216.0f);
+            const int margin = 3 + (int)fminf(x * 1.001f, 1.0e6f);
+            const int vmin = (int)fminf(Av, fminf(Bv, Cv)), vmax = (int)fmaxf(Av, fmaxf(Bv, Cv));
+            int4 bx = make_int4(max((int)Au, 0), min((int)Cu, P.W), max(vmin - margin, 0), min(vmax + margin + 1, P.H));
+            if (bx.x >= bx.y || bx.z >= bx.w) bx = make_int4(0, 0, 0, 0);
+            ((int4*)P.tbox[img])[ti] = bx;
+            if (bx.x < bx.y) {
+                const int sx0 = bx.x >> sh, sx1 = (bx.y - 1) >> sh, sy0 = bx.z >> sh, sy1 = (bx.w - 1) >> sh;
+                for (int sy = sy0; sy <= sy1; sy++)
+                    for (int sx = sx0; sx <= sx1; sx++) atomicOr(&sm[sy * nsx + sx], 1ull << lane);
+            }
         }
+        __syncthreads();
+        unsigned long long* out = P.smask[img] + (size_t)c * nst;
+        for (int s = lane; s < nst; s += 64) out[s] = sm[s];
+        __syncthreads();
     }
 }
 
@@ -278,79 +153,139 @@ __device__ __forceinline__ int elas_plane_d(const float* rec, int u, int v) {
 }
 
 // ---- rasterisation: KerComputeDisparity.cu:126-150 ------------------------
-// One workgroup per 32x32 pixel tile.  The tile's triangles (binned on the
-// GPU, or on the host as a fallback) are staged in LDS in chunks; every (triangle, column)
-// pair is one work item: it computes the column's half-open row span between
-// the long edge AC and AB (u in [A_u, B_u)) or BC (u in [B_u, C_u)) and
-// records the triangle id for the span's pixels in an LDS tile with
-// atomicMax.  Where spans overlap (the reference races there) the highest
-// triangle index wins -- the result of painting the triangles in index order.
-// Each pixel's owner then stores the plane prior (int)(a*u + b*v + c) of its
-// triangle, or -1.
-#define RS_CHUNK 128
-__global__ __launch_bounds__(256) void k_raster(ElasLaunch L) {
-    __shared__ float s_rec[RS_CHUNK][ELAS_TRIREC];
-    __shared__ int s_id[RS_CHUNK];
-    __shared__ int s_tile[ELAS_RASTER_T][ELAS_RASTER_T];
-    const int tid = threadIdx.x;
+// A workgroup owns an ELAS_RT_W x ELAS_RT_H pixel tile of one image.  It
+// lists in LDS the 64-triangle chunks with a triangle in the tile's
+// super-tile, tests those triangles' boxes against the tile (a lane per
+// triangle, 16 chunks per round) and gathers the hits, then each wave takes
+// one gathered triangle at a time, a lane per tile column u in the triangle's
+// half-open column range
+// [A_u, C_u): the column's row span between the long edge AC and AB
+// (u < B_u) or BC, as the reference computes it, clipped to the tile, and an
+// LDS atomic max of the triangle index into each pixel of it.  Where spans
+// overlap (the reference races there) the highest index wins: the result of
+// painting the triangles in index order.  Last, every pixel of the tile
+// becomes the plane prior (int)(a*u + b*v + c) of its triangle, or -1 when
+// no span covers it.  No global atomics: every prior pixel is written once,
+// by its tile.
+#define ELAS_RT_HITS 1024  // 16 chunks of 64 triangles per gather round
+__global__ __launch_bounds__(256) void k_rtile(ElasLaunch L) {
+    __shared__ int32_t ids[ELAS_RT_H * ELAS_RT_W];
+    __shared__ int32_t hits[ELAS_RT_HITS];
+    __shared__ unsigned long long nzm[256];
+    __shared__ int32_t nzc[256];
+    __shared__ int32_t nnz, nh;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     for (int t = blockIdx.x; t < L.total; t += gridDim.x) {
         const int p = elas_find_pair(L.prefix, L.npairs, t);
         const auto& P = L.pairs[p];
         const int W = P.W, H = P.H;
-        const int ntiles = P.ntx * P.nty;
-        int local = t - L.prefix[p];
-        const int img = local >= ntiles;
-        if (img) local -= ntiles;
-        const int tx0 = (local % P.ntx) * ELAS_RASTER_T, ty0 = (local / P.ntx) * ELAS_RASTER_T;
-        const float* recs = P.trirec[img];
-        const int* lst = P.tlist[img];
-        const int k0 = elas_ld(P.toff[img] + local), k1 = elas_ld(P.toff[img] + local + 1);
-        #pragma unroll
-        for (int r = 0; r < 4; r++) s_tile[(tid >> 5) + 8 * r][tid & 31] = -1;
-        elas_lds_barrier();
-        for (int kc = k0; kc < k1; kc += RS_CHUNK) {
-            const int n = min(RS_CHUNK, k1 - kc);
-            if (tid < n) {
-                const int ti = elas_ld(lst + kc + tid);
-                const uint4* src = reinterpret_cast<const uint4*>(recs + (size_t)ti * ELAS_TRIREC);
-                const uint4 r0 = elas_ld(src), r1 = elas_ld(src + 1), r2 = elas_ld(src + 2), r3 = elas_ld(src + 3);
-                uint4* dst = reinterpret_cast<uint4*>(s_rec[tid]);
-                dst[0] = r0; dst[1] = r1; dst[2] = r2; dst[3] = r3;
-                s_id[tid] = ti;
+        const int ntx = (W + ELAS_RT_W - 1) / ELAS_RT_W, nty = (H + ELAS_RT_H - 1) / ELAS_RT_H;
+        int k = t - L.prefix[p];
+        const int img = k >= ntx * nty;
+        if (img) k -= ntx * nty;
+        const int ty = k / ntx, tx = k - ty * ntx;
+        const int u0 = tx * ELAS_RT_W, v0 = ty * ELAS_RT_H;
+        const int nst = P.nst;
+        const int st = (u0 >> P.sshift) + (v0 >> P.sshift) * P.nsx;
+        for (int i = tid; i < ELAS_RT_H * ELAS_RT_W; i += 256) ids[i] = -1;
+        if (tid == 0) nnz = 0;
+        __syncthreads();
+        const int nch = (P.ntri[img] + 63) >> 6;
+        const unsigned long long* smask = P.smask[img] + st;
+        const int4* box = (const int4*)P.tbox[img];
+        const float* trirec = P.trirec[img];
+        for (int cb = 0; cb < nch; cb += 256) {
+            // the chunks of this pass that meet the super-tile
+            const int c = cb + tid;
+            const unsigned long long m = c < nch ? smask[(size_t)c * nst] : 0ull;
+            if (m) {
+                const int j = atomicAdd(&nnz, 1);
+                nzc[j] = c;
+                nzm[j] = m;
             }
-            elas_lds_barrier();
-            for (int item = tid; item < n * ELAS_RASTER_T; item += 256) {
-                const int j = item >> 5, c = item & 31;
-                const int u = tx0 + c;
-                const float* rec = s_rec[j];
-                const int Au = (int)rec[3], Bu = (int)rec[5], Cu = (int)rec[7];
-                if (u < Au || u >= Cu || u >= W) continue;
-                const bool first = u < Bu;
-                const float ea = first ? rec[9] : rec[13], eb = first ? rec[10] : rec[14];
-                const float fu = (float)u;
-                const int v1 = (int)elas_f2u_sat(__fadd_rn(__fmul_rn(rec[11], fu), rec[12]));
-                const int v2 = (int)elas_f2u_sat(__fadd_rn(__fmul_rn(ea, fu), eb));
-                const int vlo = max(max(min(v1, v2), 0), ty0);
-                const int vhi = min(min(max(v1, v2), H), ty0 + ELAS_RASTER_T);
-                const int ti = s_id[j];
-                for (int v = vlo; v < vhi; v++) atomicMax(&s_tile[v - ty0][c], ti);
-            }
-            elas_lds_barrier();
-        }
-        const int c = tid & 31;
-        const int u = tx0 + c;
-        if (u < W) {
-            #pragma unroll
-            for (int r = 0; r < 4; r++) {
-                const int rr = (tid >> 5) + 8 * r, v = ty0 + rr;
-                if (v < H) {
-                    const int id = s_tile[rr][c];
-                    elas_st(P.prior[img] + (size_t)v * W + u,
-                            id >= 0 ? elas_plane_d(recs + (size_t)id * ELAS_TRIREC, u, v) : -1);
+            __syncthreads();
+            const int nz = nnz;
+            for (int r = 0; r < nz; r += 16) {
+                if (tid == 0) nh = 0;
+                __syncthreads();
+                // box tests, a lane per triangle: wave w takes chunks r + w + 4q
+                int cq[4];
+                bool in[4];
+                int4 bq[4];
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const int j = r + wave + 4 * q;
+                    cq[q] = j < nz ? nzc[j] : 0;
+                    in[q] = j < nz && ((nzm[j] >> lane) & 1ull);
+                    bq[q] = in[q] ? box[cq[q] * 64 + lane] : make_int4(0, 0, 0, 0);
                 }
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const bool hit = in[q] && bq[q].x < u0 + ELAS_RT_W && bq[q].y > u0 &&
+                                     bq[q].z < v0 + ELAS_RT_H && bq[q].w > v0;
+                    const unsigned long long bal = __ballot(hit);
+                    if (bal) {
+                        int base = 0;
+                        if (lane == 0) base = atomicAdd(&nh, __popcll(bal));
+                        base = __builtin_amdgcn_readfirstlane(base);
+                        const int below = __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32),
+                                                                     __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0));
+                        if (hit) hits[base + below] = cq[q] * 64 + lane;
+                    }
+                }
+                __syncthreads();
+                // raster: wave w takes hits w + 4i; lane i loads the record
+                // of its hit, read back one hit at a time with readlane
+                const int n = nh;
+                const int cnt = n > wave ? (n - wave + 3) >> 2 : 0;
+                for (int g = 0; g < cnt; g += 64) {
+                    int tl = 0;
+                    float4 r0 = make_float4(0.f, 0.f, 0.f, 0.f), r1 = r0, r2 = r0, r3 = r0;
+                    if (g + lane < cnt) {
+                        tl = hits[wave + 4 * (g + lane)];
+                        const float4* rp = (const float4*)(trirec + (size_t)tl * ELAS_TRIREC);
+                        r0 = rp[0]; r1 = rp[1]; r2 = rp[2]; r3 = rp[3];
+                    }
+                    const int gn = min(64, cnt - g);
+                    for (int h = 0; h < gn; h++) {
+                        const int ti = __builtin_amdgcn_readlane(tl, h);
+                        const int Au = (int)__int_as_float(__builtin_amdgcn_readlane(__float_as_int(r0.w), h));
+                        const int Bu = (int)__int_as_float(__builtin_amdgcn_readlane(__float_as_int(r1.y), h));
+                        const int Cu = (int)__int_as_float(__builtin_amdgcn_readlane(__float_as_int(r1.w), h));
+                        const int u = u0 + lane;
+                        if (u >= max(Au, 0) && u < min(Cu, W)) {
+                            const float ABa = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(r2.y), h));
+                            const float ABb = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(r2.z), h));
+                            const float ACa = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(r2.w), h));
+                            const float ACb = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(r3.x), h));
+                            const float BCa = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(r3.y), h));
+                            const float BCb = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(r3.z), h));
+                            const bool first = u < Bu;
+                            const float ea = first ? ABa : BCa, eb = first ? ABb : BCb;
+                            const float fu = (float)u;
+                            const int v1 = (int)elas_f2u_sat(__fadd_rn(__fmul_rn(ACa, fu), ACb));
+                            const int v2 = (int)elas_f2u_sat(__fadd_rn(__fmul_rn(ea, fu), eb));
+                            const int vlo = max(max(min(v1, v2), 0), v0);
+                            const int vhi = min(min(max(v1, v2), H), v0 + ELAS_RT_H);
+                            for (int v = vlo; v < vhi; v++) atomicMax(&ids[(v - v0) * ELAS_RT_W + lane], ti);
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+            if (tid == 0) nnz = 0;
+            __syncthreads();
+        }
+        int32_t* prior = P.prior[img];
+        for (int i = tid; i < ELAS_RT_H * ELAS_RT_W; i += 256) {
+            const int u = u0 + (i % ELAS_RT_W), v = v0 + i / ELAS_RT_W;
+            if (u < W && v < H) {
+                const int id = ids[i];
+                prior[(size_t)v * W + u] = id >= 0 ? elas_plane_d(trirec + (size_t)id * ELAS_TRIREC, u, v) : -1;
             }
         }
-        elas_lds_barrier();
+        __syncthreads();
     }
 }
 
@@ -754,20 +689,11 @@ __global__ __launch_bounds__(256) void k_lr(ElasLaunch L, ElasKParams kp) {
     }
 }
 
-void elas_launch_bin_clear(const ElasLaunch& L, int grid, hipStream_t s) {
-    hipLaunchKernelGGL(k_bin_clear, dim3(grid), dim3(256), 0, s, L);
-}
-void elas_launch_bin_scan(const ElasLaunch& L, int grid, hipStream_t s) {
-    hipLaunchKernelGGL(k_bin_scan, dim3(grid), dim3(1024), 0, s, L);
-}
-void elas_launch_bin_fill(const ElasLaunch& L, int grid, hipStream_t s) {
-    hipLaunchKernelGGL(k_bin_fill, dim3(grid), dim3(64), 0, s, L);
-}
 void elas_launch_plane(const ElasLaunch& L, int grid, hipStream_t s) {
     hipLaunchKernelGGL(k_plane, dim3(grid), dim3(64), 0, s, L);
 }
-void elas_launch_raster(const ElasLaunch& L, int grid, hipStream_t s) {
-    hipLaunchKernelGGL(k_raster, dim3(grid), dim3(256), 0, s, L);
+void elas_launch_rtile(const ElasLaunch& L, int grid, hipStream_t s) {
+    hipLaunchKernelGGL(k_rtile, dim3(grid), dim3(256), 0, s, L);
 }
 void elas_launch_mean(const ElasLaunch& L, int grid, hipStream_t s) {
     hipLaunchKernelGGL(k_mean, dim3(grid), dim3(256), 0, s, L);

@@ -113,9 +113,9 @@ typedef enum elas_timer {
     ELAS_T_LR = 8, ELAS_T_POST = 9, ELAS_T_TOTAL = 10, ELAS_T_PHASE_A = 11,
     ELAS_T_PHASE_B = 12,
     /* host step (wall clock): support points down, Triangle jobs from submit
-     * to the last done, the longest job's Triangle and binning parts */
-    ELAS_T_SP_DOWNLOAD = 13, ELAS_T_TRI_WALL = 14, ELAS_T_TRI_JOB = 15, ELAS_T_BIN_JOB = 16,
-    ELAS_T_COUNT = 17
+     * to the last done, the longest job */
+    ELAS_T_SP_DOWNLOAD = 13, ELAS_T_TRI_WALL = 14, ELAS_T_TRI_JOB = 15,
+    ELAS_T_COUNT = 16
 } elas_timer;
 
 typedef struct elas_ctx elas_ctx;
@@ -176,9 +176,7 @@ int elas_hip_set_graphs(elas_ctx* ctx, int32_t on);
 int elas_hip_set_pipeline(elas_ctx* ctx, int32_t slots, int32_t chunk);
 
 /* Intermediates of the last process call on this context (kept only when
- * elas_hip_set_debug(ctx, 1)).  Bit 1 of the flags (value 2) shrinks the GPU
- * raster-bin capacity to 1 so that every mesh takes the host-binning
- * fallback (for tests; results are unchanged).  Copies min(bytes, stage size) bytes to dst
+ * elas_hip_set_debug(ctx, 1)).  Copies min(bytes, stage size) bytes to dst
  * and stores the stage size in *stage_bytes (either may be NULL/0 to query). */
 int elas_hip_set_debug(elas_ctx* ctx, int flags);
 int elas_hip_debug_fetch(elas_ctx* ctx, int stage, void* dst, size_t bytes, size_t* stage_bytes);

@@ -159,26 +159,23 @@ def test_graph_replays_equal_eager(pkg, synth):
         assert np.array_equal(e1, g1) and np.array_equal(e2, g2)
 
 
-def test_host_bin_fallback_equals_gpu_bins(pkg, oracle, pairs, synth):
-    """Raster bins that outgrow their GPU capacity are rebuilt on the host and
-    phase B re-runs: the results are those of the GPU bins (forced here with a
-    capacity of 1), at every stage, alone and in a batch."""
-    left, right, _, dmax = pairs("c0")
-    el = pkg.Elas(pkg.default_params(disp_max=dmax))
-    el.set_debug(True, bin_fallback=True)
+def test_sparse_texture_large_triangles(pkg, oracle, synth):
+    """Texture only in five patches: twenty triangles per mesh, most spanning
+    a large part of the image, so the raster's waves walk long column ranges
+    and many triangles cover each pixel's neighbourhood."""
+    l, r, _ = synth.make_pair(240, 180, 31, regions=3, seed=808)
+    mask = np.zeros(l.shape, bool)
+    for y, x in [(20, 30), (30, 180), (120, 40), (130, 170), (80, 110)]:
+        mask[y:y + 24, x:x + 24] = True
+    left = np.where(mask, l, 100).astype(np.uint8)
+    right = np.where(mask, r, 100).astype(np.uint8)
+    ref = oracle.process(left, right, oracle.params(disp_max=31))
+    assert ref["status"] == 0 and ref["ntri0"] >= 10
+    el = pkg.Elas(pkg.default_params(disp_max=31))
+    el.set_debug(True)
     D1, D2 = el.process(left, right)
-    ref = oracle.process(left, right, oracle.params(disp_max=dmax))
-    _compare_all(el, ref, "bin fallback")
+    _compare_all(el, ref, "sparse texture")
     assert np.array_equal(D1, ref["D1"]) and np.array_equal(D2, ref["D2"])
-    shapes = [(160, 120, 31), (97, 131, 15), (240, 180, 63)]
-    pairs_ = [synth.make_pair(W, H, dm, regions=4, seed=700 + i)[:2] + (dm,) for i, (W, H, dm) in enumerate(shapes)]
-    fb = pkg.Elas(pkg.default_params(disp_max=63))
-    fb.set_debug(False, bin_fallback=True)
-    fb.set_pipeline(2, 1)
-    for (l, r, dm), (b1, b2, st) in zip(pairs_, fb.process_batch(pairs_)):
-        one = pkg.Elas(pkg.default_params(disp_max=dm))
-        a1, a2 = one.process(l, r)
-        assert np.array_equal(b1, a1) and np.array_equal(b2, a2)
 
 
 def test_textureless_pair_reports_too_few_points(pkg, oracle):
