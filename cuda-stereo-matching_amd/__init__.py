@@ -446,9 +446,11 @@ class Elas:
         self._check(st)
         return [(d1[i], d2[i], int(stat[i])) for i in range(n)]
 
-    def set_debug(self, on=True):
-        """Keep every intermediate of the next calls for fetch()."""
-        self._check(self._lib.elas_hip_set_debug(self._ctx, 1 if on else 0))
+    def set_debug(self, on=True, tiny_lists=False):
+        """Keep every intermediate of the next calls for fetch(); tiny_lists
+        caps the raster's per-tile triangle lists at one entry (a test of the
+        overflow path, where a tile tests every triangle of the mesh)."""
+        self._check(self._lib.elas_hip_set_debug(self._ctx, (1 if on else 0) | (2 if tiny_lists else 0)))
 
     def shape_info(self):
         info = ShapeInfo()

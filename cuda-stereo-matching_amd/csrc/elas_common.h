@@ -20,9 +20,12 @@
 //   [9..14] AB_a, AB_b, AC_a, AC_b, BC_a, BC_b         [15] unused
 #define ELAS_TRIREC 16
 
-#define ELAS_RT_W 64         // raster tile: 64 x 32 pixels per workgroup
-#define ELAS_RT_H 32
+#define ELAS_RT_WS 6         // raster tile: 64 x 32 pixels per workgroup
+#define ELAS_RT_HS 5
+#define ELAS_RT_W (1 << ELAS_RT_WS)
+#define ELAS_RT_H (1 << ELAS_RT_HS)
 #define ELAS_ST_MAX 1024     // culling super-tiles per image, at most (LDS masks of k_plane)
+#define ELAS_SCAP_MAX 256    // list entries per super-tile, at most (more: the tile scans every chunk)
 
 #define ELAS_DESC_SW 58      // descriptor strip: output columns of a 64-lane wave
 #define ELAS_DESC_SH 18      // descriptor strip: output rows by default (18 + 6 rows walked);
@@ -52,7 +55,9 @@ struct ElasPair {
     int32_t dsh;             // descriptor strip rows (6, 12 or 18)
     int32_t ntri[2];         // triangles of the left / right mesh (host-known)
     int32_t nblk;            // 256-cell blocks of the support lattice
-    int32_t sshift, nsx, nst;  // culling super-tiles: (1 << sshift)-pixel squares, nsx per row, nst in all
+    int32_t sshift, nsx, nst;  // culling super-tiles: squares of 1 << sshift raster tiles a side,
+                               // nsx per row, nst in all
+    int32_t scap;            // list entries per super-tile: min(ceil(triangle capacity / 64), ELAS_SCAP_MAX)
     const uint8_t* img[2];   // [H][W]
     uint8_t* desc[2];        // [H][W][16]
     int32_t* can[4];         // [ch][cw]: left pass, LR pass, inconsistency, final
@@ -66,7 +71,10 @@ struct ElasPair {
     float* trirec[2];        // [ntri][ELAS_TRIREC]
     int32_t* tbox[2];        // [ntri][4] pixel box of the triangle's spans: u0, u1, v0, v1
                              //           (half-open, all 0 when empty)
-    unsigned long long* smask[2];  // [ceil(ntri/64)][nst] bit k of [c][s]: triangle 64c+k meets super-tile s
+    int32_t* scnt[2];        // [nst] entries appended to a super-tile's list (zeroed in phase A);
+                             //       above scap the list is incomplete
+    int32_t* slist[2];       // [nst][scap][4] per super-tile, in no particular order: chunk c and the
+                             //   64-bit mask (lo, hi) of its triangles 64c+k that meet the super-tile
     int32_t* prior[2];       // [H][W] (int)plane(u,v) of the covering triangle
                              //        with the highest index, -1 if uncovered
     float* mean[2];          // [H][W] (float)prior: the reference's map (debug only)

@@ -99,17 +99,19 @@ __device__ __forceinline__ uint32_t elas_sad16(const uint4 a, const uint4 b, uin
 }
 
 // float -> int32 / uint32 with saturation and NaN -> 0, written out so the
-// result does not depend on how the compiler treats out-of-range casts.
+// result does not depend on how the compiler treats out-of-range casts: the
+// cast only sees values clamped into range (2147483520 and 4294967040 are
+// the largest floats below 2^31 and 2^32), and selects, not branches, pick
+// the saturated ends.
 __device__ __forceinline__ int32_t elas_f2i_sat(float x) {
-    if (x != x) return 0;
-    if (x >= 2147483648.0f) return INT32_MAX;
-    if (x <= -2147483648.0f) return INT32_MIN;
-    return (int32_t)x;
+    int32_t r = (int32_t)fminf(fmaxf(x, -2147483648.0f), 2147483520.0f);
+    r = x >= 2147483648.0f ? INT32_MAX : r;
+    return x != x ? 0 : r;
 }
 __device__ __forceinline__ uint32_t elas_f2u_sat(float x) {
-    if (x != x || x <= 0.0f) return 0u;
-    if (x >= 4294967296.0f) return 0xffffffffu;
-    return (uint32_t)x;
+    uint32_t r = (uint32_t)fminf(fmaxf(x, 0.0f), 4294967040.0f);
+    r = x >= 4294967296.0f ? 0xffffffffu : r;
+    return x != x ? 0u : r;
 }
 
 // floor(x / g) for g > 0 (the reference evaluates floor((float)x / g), equal

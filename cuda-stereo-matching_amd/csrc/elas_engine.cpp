@@ -100,7 +100,8 @@ size_t pair_arena(const ElasPairSpec& s, const ElasDims& d, bool debug, bool pos
     b += 4 * al(4 * d.ncell * d.words);
     if (debug) b += 2 * al(4 * d.ncell * (size_t)(s.dmax + 2)) + 2 * al(4 * d.npx);
     b += 2 * al(12 * (size_t)d.tricap) + 2 * al(4 * ELAS_TRIREC * (size_t)d.tricap);
-    b += 2 * al(16 * (size_t)d.tricap) + 2 * al(8 * (size_t)d.nst * cdiv(d.tricap, 64));  // boxes, culling masks
+    b += 2 * al(16 * (size_t)d.tricap) + 2 * al(4 * (size_t)d.nst) +
+         2 * al(16 * (size_t)d.nst * d.scap);  // boxes, culling lists
     b += 2 * al(4 * d.npx) + 2 * al(4 * d.npx);  // prior, draw
     if (!s.dev_out) b += 2 * al(4 * d.npx);
     if (post) b += 2 * al(8 * d.npx);  // post-filter scratch
@@ -126,12 +127,14 @@ ElasDims elas_dims(const elas_params& p, int W, int H, int dmax) {
     d.ncell = (size_t)d.gw * d.gh;
     d.nblk = (int32_t)((d.ncan + 255) / 256);
     d.tricap = (int32_t)(2 * d.ncan + 8);
-    // raster culling super-tiles: 128-pixel squares, larger where an image
-    // would need more than ELAS_ST_MAX of them
-    d.sshift = 7;
-    while ((long)cdiv(W, 1 << d.sshift) * cdiv(H, 1 << d.sshift) > ELAS_ST_MAX) d.sshift++;
-    d.nsx = cdiv(W, 1 << d.sshift);
-    d.nst = d.nsx * cdiv(H, 1 << d.sshift);
+    // raster culling super-tiles: single raster tiles, or squares of 2, 4, ..
+    // tiles a side where an image has more than ELAS_ST_MAX tiles
+    const int ntx = cdiv(W, ELAS_RT_W), nty = cdiv(H, ELAS_RT_H);
+    d.sshift = 0;
+    while ((long)cdiv(ntx, 1 << d.sshift) * cdiv(nty, 1 << d.sshift) > ELAS_ST_MAX) d.sshift++;
+    d.nsx = cdiv(ntx, 1 << d.sshift);
+    d.nst = d.nsx * cdiv(nty, 1 << d.sshift);
+    d.scap = std::min(cdiv(d.tricap, 64), ELAS_SCAP_MAX);
     return d;
 }
 
@@ -343,6 +346,7 @@ int elas_ctx::slot_prepare(ElasSlot& S, const ElasPairSpec* sp, int n) {
 
     S.pairs.assign(n, ElasPair());
     S.dense_lds[0] = S.dense_lds[1] = 0;
+    S.plane_lds = 0;
     S.prefix.assign((size_t)S_NUM_STAGES * (n + 1), 0);
     S.nsp.assign(n, 0);
     S.ntri0.assign(n, 0);
@@ -379,8 +383,10 @@ int elas_ctx::slot_prepare(ElasSlot& S, const ElasPairSpec* sp, int n) {
         for (int k = 0; k < 2; k++) P.tri[k] = (int32_t*)take(12 * (size_t)d.tricap);
         for (int k = 0; k < 2; k++) P.trirec[k] = (float*)take(4 * ELAS_TRIREC * (size_t)d.tricap);
         for (int k = 0; k < 2; k++) P.tbox[k] = (int32_t*)take(16 * (size_t)d.tricap);
-        for (int k = 0; k < 2; k++) P.smask[k] = (unsigned long long*)take(8 * (size_t)d.nst * cdiv(d.tricap, 64));
-        P.sshift = d.sshift; P.nsx = d.nsx; P.nst = d.nst;
+        for (int k = 0; k < 2; k++) P.scnt[k] = (int32_t*)take(4 * (size_t)d.nst);
+        for (int k = 0; k < 2; k++) P.slist[k] = (int32_t*)take(16 * (size_t)d.nst * d.scap);
+        P.sshift = d.sshift; P.nsx = d.nsx; P.nst = d.nst; P.scap = tiny_lists ? 1 : d.scap;
+        S.plane_lds = std::max(S.plane_lds, 8 * (size_t)d.nst);
         for (int k = 0; k < 2; k++) P.prior[k] = (int32_t*)take(4 * d.npx);
         for (int k = 0; k < 2; k++) P.draw[k] = (float*)take(4 * d.npx);
         if (!s.dev_out) { P.dout[0] = (float*)take(4 * d.npx); P.dout[1] = (float*)take(4 * d.npx); }
@@ -659,7 +665,8 @@ int elas_ctx::enqueue_b(ElasSlot& S, bool timed) {
         // reads the actual count from the table
         long maxchunks = 0;
         for (int i = 0; i < n; i++) maxchunks += 2L * cdiv(S.dd[i].tricap, 64);
-        elas_launch_plane(launch_of(S, 1, S_PLANE), (int)std::max(1L, std::min<long>(2 * cap256, maxchunks)), s1);
+        elas_launch_plane(launch_of(S, 1, S_PLANE), (int)std::max(1L, std::min<long>(2 * cap256, maxchunks)),
+                          S.plane_lds, s1);
         ElasLaunch L = launch_of(S, 1, S_RTILES);
         if (L.total) elas_launch_rtile(L, g(L, cap256), s1);
     }

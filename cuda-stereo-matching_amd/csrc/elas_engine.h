@@ -37,7 +37,7 @@ struct ElasPairSpec {
 // Derived sizes of one pair.
 struct ElasDims {
     int32_t cw, ch, gw, gh, words, nblk, tricap;
-    int32_t sshift, nsx, nst;  // raster culling super-tiles
+    int32_t sshift, nsx, nst, scap;  // raster culling super-tiles and their list capacity
     size_t npx, ncan, ncell;
 };
 
@@ -78,6 +78,7 @@ struct ElasSlot {
     std::vector<int32_t> nsp, ntri0, ntri1, status;
     int maxd = 0;
     size_t dense_lds[2] = {0, 0};  // dynamic LDS of the dense launches (1, 2 cells per wave)
+    size_t plane_lds = 0;          // dynamic LDS of the plane launch (super-tile masks)
     ElasKParams kpl;
     size_t tab_pairs = 0, tab_prefix = 0, tab_one = 0, pin_sp = 0, pin_tri = 0;
     ElasPair* dpairs[2] = {nullptr, nullptr};
@@ -116,6 +117,7 @@ struct elas_ctx {
     elas_params prm;
     ElasKParams kp;
     bool debug = false;
+    bool tiny_lists = false;  // raster lists of one entry (tests the overflow path)
     int tri_workers = ELAS_TRI_INSTANCES;
     int chunk = 0;  // pairs per pipeline chunk of a batch (0: whole batch)
     bool use_graphs = true;   // phases A and B replayed as graphs per shape bucket

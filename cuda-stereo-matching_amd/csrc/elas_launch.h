@@ -16,7 +16,7 @@ void elas_launch_grid_fill(const ElasLaunch& L, int grid, hipStream_t s);
 void elas_launch_grid_scatter(const ElasLaunch& L, int grid, hipStream_t s);
 void elas_launch_grid_dilate(const ElasLaunch& L, int grid, hipStream_t s);
 void elas_launch_grid_list(const ElasLaunch& L, int grid, hipStream_t s);
-void elas_launch_plane(const ElasLaunch& L, int grid, hipStream_t s);
+void elas_launch_plane(const ElasLaunch& L, int grid, size_t lds, hipStream_t s);
 void elas_launch_rtile(const ElasLaunch& L, int grid, hipStream_t s);
 void elas_launch_mean(const ElasLaunch& L, int grid, hipStream_t s);
 void elas_launch_dense(const ElasLaunch& L, const ElasKParams& kp, int cpw, int grid, size_t lds, hipStream_t s);

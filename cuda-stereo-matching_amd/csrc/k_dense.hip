@@ -101,10 +101,15 @@ __device__ __forceinline__ void elas_plane_rec(const ElasPair& P, int img, int t
 // |v|; the margin is 3 + floor of 8 * (3*smax*U + 3*V) * 2^-24 (a float
 // evaluation of a larger bound), so the span rows stay within
 // [min v - margin, max v + margin].  The box then marks, in LDS, every
-// super-tile ((1 << sshift)-pixel square) it meets; the chunk's masks go out
-// as smask[img][c][s], bit k for triangle 64*c + k.
+// super-tile it meets (a square of 1 << sshift raster tiles a side: a single
+// tile unless the image has more than ELAS_ST_MAX tiles); every nonzero mask
+// of the chunk is appended to its super-tile's list as (c, mask), bit k for
+// triangle 64*c + k (the list order is immaterial: the raster takes a max).
+// A list holds scap entries; the count goes on, and a tile whose list
+// overflowed tests every triangle of the mesh instead.
 __global__ __launch_bounds__(64) void k_plane(ElasLaunch L) {
-    __shared__ unsigned long long sm[ELAS_ST_MAX];
+    extern __shared__ unsigned long long sm[];  // [nst] of the largest pair of the launch
+    __shared__ int32_t nzs[64];                 // a batch of super-tiles with nonzero masks
     const int lane = threadIdx.x;
     // the triangle counts are data: read the total from the table (a captured
     // launch keeps the grid of its shape bucket)
@@ -136,44 +141,67 @@ __global__ __launch_bounds__(64) void k_plane(ElasLaunch L) {
             if (bx.x >= bx.y || bx.z >= bx.w) bx = make_int4(0, 0, 0, 0);
             ((int4*)P.tbox[img])[ti] = bx;
             if (bx.x < bx.y) {
-                const int sx0 = bx.x >> sh, sx1 = (bx.y - 1) >> sh, sy0 = bx.z >> sh, sy1 = (bx.w - 1) >> sh;
+                const int sx0 = bx.x >> (ELAS_RT_WS + sh), sx1 = (bx.y - 1) >> (ELAS_RT_WS + sh);
+                const int sy0 = bx.z >> (ELAS_RT_HS + sh), sy1 = (bx.w - 1) >> (ELAS_RT_HS + sh);
                 for (int sy = sy0; sy <= sy1; sy++)
                     for (int sx = sx0; sx <= sx1; sx++) atomicOr(&sm[sy * nsx + sx], 1ull << lane);
             }
         }
         __syncthreads();
-        unsigned long long* out = P.smask[img] + (size_t)c * nst;
-        for (int s = lane; s < nst; s += 64) out[s] = sm[s];
+        // appends: the nonzero masks are compacted into nzs, 64 at a time, and
+        // each batch takes its list slots in one round trip (the workgroup is
+        // one wave: its barriers only order the LDS accesses)
+        auto flush = [&](int n) {
+            __syncthreads();
+            const int ms = lane < n ? nzs[lane] : 0;
+            const unsigned long long m = lane < n ? sm[ms] : 0ull;
+            const int slot = lane < n ? atomicAdd(P.scnt[img] + ms, 1) : 0;
+            if (lane < n && slot < P.scap)
+                ((int4*)P.slist[img])[(size_t)ms * P.scap + slot] =
+                    make_int4(c, (int)(uint32_t)m, (int)(uint32_t)(m >> 32), 0);
+            __syncthreads();
+        };
+        int filled = 0;
+        for (int s0 = 0; s0 < nst; s0 += 64) {
+            const int s = s0 + lane;
+            const bool nz = s < nst && sm[s] != 0ull;
+            const unsigned long long bal = __ballot(nz);
+            const int nb = __popcll(bal);
+            if (filled + nb > 64) {
+                flush(filled);
+                filled = 0;
+            }
+            if (nz) nzs[filled + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32),
+                                                   __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0))] = s;
+            filled += nb;
+        }
+        if (filled) flush(filled);
         __syncthreads();
     }
 }
 
-__device__ __forceinline__ int elas_plane_d(const float* rec, int u, int v) {
-    return elas_f2i_sat(__fadd_rn(__fadd_rn(__fmul_rn(rec[0], (float)u), __fmul_rn(rec[1], (float)v)), rec[2]));
-}
-
 // ---- rasterisation: KerComputeDisparity.cu:126-150 ------------------------
 // A workgroup owns an ELAS_RT_W x ELAS_RT_H pixel tile of one image.  It
-// lists in LDS the 64-triangle chunks with a triangle in the tile's
-// super-tile, tests those triangles' boxes against the tile (a lane per
-// triangle, 16 chunks per round) and gathers the hits, then each wave takes
-// one gathered triangle at a time, a lane per tile column u in the triangle's
-// half-open column range
-// [A_u, C_u): the column's row span between the long edge AC and AB
-// (u < B_u) or BC, as the reference computes it, clipped to the tile, and an
-// LDS atomic max of the triangle index into each pixel of it.  Where spans
-// overlap (the reference races there) the highest index wins: the result of
-// painting the triangles in index order.  Last, every pixel of the tile
-// becomes the plane prior (int)(a*u + b*v + c) of its triangle, or -1 when
-// no span covers it.  No global atomics: every prior pixel is written once,
-// by its tile.
+// reads its super-tile's list of 64-triangle chunks into LDS and gathers the
+// listed triangles, 16 chunks per round (their boxes are tested against the
+// tile, a lane per triangle, when the mask alone does not place a triangle
+// in the tile).  Wave w then takes the gathered triangles w, w + 4, ...: lane
+// i loads the record of the wave's i-th triangle, and the wave walks them one
+// at a time (readlane), a lane per tile column u in the triangle's half-open
+// column range [A_u, C_u): the column's row span between the long edge AC
+// and AB (u < B_u) or BC, as the reference computes it, clipped to the tile.
+// Each pixel of the span takes the LDS atomic max of the triangle index:
+// where spans overlap (the reference races there) the highest index wins,
+// the result of painting the triangles in index order.  Last, every pixel of
+// the tile is written once: the plane prior (int)(a*u + b*v + c) of its
+// triangle, or -1 where no span covers it.  No global atomics.
 #define ELAS_RT_HITS 1024  // 16 chunks of 64 triangles per gather round
 __global__ __launch_bounds__(256) void k_rtile(ElasLaunch L) {
     __shared__ int32_t ids[ELAS_RT_H * ELAS_RT_W];
     __shared__ int32_t hits[ELAS_RT_HITS];
     __shared__ unsigned long long nzm[256];
     __shared__ int32_t nzc[256];
-    __shared__ int32_t nnz, nh;
+    __shared__ int32_t nh;
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     for (int t = blockIdx.x; t < L.total; t += gridDim.x) {
@@ -186,39 +214,49 @@ __global__ __launch_bounds__(256) void k_rtile(ElasLaunch L) {
         if (img) k -= ntx * nty;
         const int ty = k / ntx, tx = k - ty * ntx;
         const int u0 = tx * ELAS_RT_W, v0 = ty * ELAS_RT_H;
-        const int nst = P.nst;
-        const int st = (u0 >> P.sshift) + (v0 >> P.sshift) * P.nsx;
+        const int sh = P.sshift;
+        const int st = (tx >> sh) + (ty >> sh) * P.nsx;
         for (int i = tid; i < ELAS_RT_H * ELAS_RT_W; i += 256) ids[i] = -1;
-        if (tid == 0) nnz = 0;
-        __syncthreads();
-        const int nch = (P.ntri[img] + 63) >> 6;
-        const unsigned long long* smask = P.smask[img] + st;
+        const int4* slist = (const int4*)P.slist[img] + (size_t)st * P.scap;
+        // the count and the list's first 64 entries in one round trip
+        const int nlist = P.scnt[img][st];
+        const int4 e0 = tid < 64 && tid < P.scap ? slist[tid] : make_int4(0, 0, 0, 0);
+        const int ntri = P.ntri[img], nch = (ntri + 63) >> 6;
+        const bool over = nlist > P.scap;        // incomplete list: every chunk, every triangle
+        const bool boxed = sh != 0 || over;      // a mask bit does not imply the tile
+        const int nent = over ? nch : nlist;
         const int4* box = (const int4*)P.tbox[img];
         const float* trirec = P.trirec[img];
-        for (int cb = 0; cb < nch; cb += 256) {
-            // the chunks of this pass that meet the super-tile
-            const int c = cb + tid;
-            const unsigned long long m = c < nch ? smask[(size_t)c * nst] : 0ull;
-            if (m) {
-                const int j = atomicAdd(&nnz, 1);
-                nzc[j] = c;
-                nzm[j] = m;
+        const int u = u0 + lane;
+        const float fu = (float)u;
+        for (int cb = 0; cb < nent; cb += 256) {
+            // up to 256 chunks per pass
+            const int j = cb + tid;
+            if (j < nent) {
+                if (over) {
+                    nzc[tid] = j;
+                    nzm[tid] = ntri - 64 * j >= 64 ? ~0ull : (1ull << (ntri - 64 * j)) - 1ull;
+                } else {
+                    const int4 e = j < 64 ? e0 : slist[j];
+                    nzc[tid] = e.x;
+                    nzm[tid] = (unsigned long long)(uint32_t)e.y | ((unsigned long long)(uint32_t)e.z << 32);
+                }
             }
             __syncthreads();
-            const int nz = nnz;
+            const int nz = min(256, nent - cb);
             for (int r = 0; r < nz; r += 16) {
                 if (tid == 0) nh = 0;
                 __syncthreads();
-                // box tests, a lane per triangle: wave w takes chunks r + w + 4q
+                // gather: wave w takes chunks r + w + 4q, a lane per triangle
                 int cq[4];
                 bool in[4];
                 int4 bq[4];
 #pragma unroll
                 for (int q = 0; q < 4; q++) {
-                    const int j = r + wave + 4 * q;
-                    cq[q] = j < nz ? nzc[j] : 0;
-                    in[q] = j < nz && ((nzm[j] >> lane) & 1ull);
-                    bq[q] = in[q] ? box[cq[q] * 64 + lane] : make_int4(0, 0, 0, 0);
+                    const int jq = r + wave + 4 * q;
+                    cq[q] = jq < nz ? nzc[jq] : 0;
+                    in[q] = jq < nz && ((nzm[jq] >> lane) & 1ull);
+                    bq[q] = in[q] && boxed ? box[cq[q] * 64 + lane] : make_int4(u0, u0 + 1, v0, v0 + 1);
                 }
 #pragma unroll
                 for (int q = 0; q < 4; q++) {
@@ -235,8 +273,6 @@ __global__ __launch_bounds__(256) void k_rtile(ElasLaunch L) {
                     }
                 }
                 __syncthreads();
-                // raster: wave w takes hits w + 4i; lane i loads the record
-                // of its hit, read back one hit at a time with readlane
                 const int n = nh;
                 const int cnt = n > wave ? (n - wave + 3) >> 2 : 0;
                 for (int g = 0; g < cnt; g += 64) {
@@ -249,12 +285,11 @@ __global__ __launch_bounds__(256) void k_rtile(ElasLaunch L) {
                     }
                     const int gn = min(64, cnt - g);
                     for (int h = 0; h < gn; h++) {
-                        const int ti = __builtin_amdgcn_readlane(tl, h);
                         const int Au = (int)__int_as_float(__builtin_amdgcn_readlane(__float_as_int(r0.w), h));
                         const int Bu = (int)__int_as_float(__builtin_amdgcn_readlane(__float_as_int(r1.y), h));
                         const int Cu = (int)__int_as_float(__builtin_amdgcn_readlane(__float_as_int(r1.w), h));
-                        const int u = u0 + lane;
                         if (u >= max(Au, 0) && u < min(Cu, W)) {
+                            const int ti = __builtin_amdgcn_readlane(tl, h);
                             const float ABa = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(r2.y), h));
                             const float ABb = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(r2.z), h));
                             const float ACa = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(r2.w), h));
@@ -263,7 +298,6 @@ __global__ __launch_bounds__(256) void k_rtile(ElasLaunch L) {
                             const float BCb = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(r3.z), h));
                             const bool first = u < Bu;
                             const float ea = first ? ABa : BCa, eb = first ? ABb : BCb;
-                            const float fu = (float)u;
                             const int v1 = (int)elas_f2u_sat(__fadd_rn(__fmul_rn(ACa, fu), ACb));
                             const int v2 = (int)elas_f2u_sat(__fadd_rn(__fmul_rn(ea, fu), eb));
                             const int vlo = max(max(min(v1, v2), 0), v0);
@@ -274,15 +308,16 @@ __global__ __launch_bounds__(256) void k_rtile(ElasLaunch L) {
                 }
                 __syncthreads();
             }
-            if (tid == 0) nnz = 0;
-            __syncthreads();
         }
+        __syncthreads();
         int32_t* prior = P.prior[img];
         for (int i = tid; i < ELAS_RT_H * ELAS_RT_W; i += 256) {
-            const int u = u0 + (i % ELAS_RT_W), v = v0 + i / ELAS_RT_W;
-            if (u < W && v < H) {
+            const int uu = u0 + (i % ELAS_RT_W), v = v0 + i / ELAS_RT_W;
+            if (uu < W && v < H) {
                 const int id = ids[i];
-                prior[(size_t)v * W + u] = id >= 0 ? elas_plane_d(trirec + (size_t)id * ELAS_TRIREC, u, v) : -1;
+                const float* rec = trirec + (size_t)(id >= 0 ? id : 0) * ELAS_TRIREC;
+                const int d = elas_f2i_sat(__fadd_rn(__fadd_rn(__fmul_rn(rec[0], (float)uu), __fmul_rn(rec[1], (float)v)), rec[2]));
+                prior[(size_t)v * W + uu] = id >= 0 ? d : -1;
             }
         }
         __syncthreads();
@@ -689,8 +724,8 @@ __global__ __launch_bounds__(256) void k_lr(ElasLaunch L, ElasKParams kp) {
     }
 }
 
-void elas_launch_plane(const ElasLaunch& L, int grid, hipStream_t s) {
-    hipLaunchKernelGGL(k_plane, dim3(grid), dim3(64), 0, s, L);
+void elas_launch_plane(const ElasLaunch& L, int grid, size_t lds, hipStream_t s) {
+    hipLaunchKernelGGL(k_plane, dim3(grid), dim3(64), lds, s, L);
 }
 void elas_launch_rtile(const ElasLaunch& L, int grid, hipStream_t s) {
     hipLaunchKernelGGL(k_rtile, dim3(grid), dim3(256), 0, s, L);

@@ -499,6 +499,9 @@ __global__ __launch_bounds__(256) void k_grid_fill(ElasLaunch L) {
         const int n = P.gw * P.gh * P.words;
         const int i = (t - L.prefix[p]) * 256 + threadIdx.x;
         if (i < 2 * n) P.gflag[i >= n][i >= n ? i - n : i] = 0u;
+        // the raster's culling lists of phase B start empty
+        if (t == L.prefix[p])
+            for (int s = threadIdx.x; s < 2 * P.nst; s += 256) P.scnt[s >= P.nst][s >= P.nst ? s - P.nst : s] = 0;
     }
 }
 

@@ -177,7 +177,10 @@ int elas_hip_set_pipeline(elas_ctx* ctx, int32_t slots, int32_t chunk);
 
 /* Intermediates of the last process call on this context (kept only when
  * elas_hip_set_debug(ctx, 1)).  Copies min(bytes, stage size) bytes to dst
- * and stores the stage size in *stage_bytes (either may be NULL/0 to query). */
+ * and stores the stage size in *stage_bytes (either may be NULL/0 to query).
+ * flags & 2 (a test of the raster's overflow path) caps the raster's
+ * per-tile triangle lists at one entry, so every tile with triangles from
+ * more than one 64-triangle chunk tests the whole mesh; results are the same. */
 int elas_hip_set_debug(elas_ctx* ctx, int flags);
 int elas_hip_debug_fetch(elas_ctx* ctx, int stage, void* dst, size_t bytes, size_t* stage_bytes);
 int elas_hip_shape_info(const elas_ctx* ctx, elas_shape_info* info);

@@ -159,6 +159,20 @@ def test_graph_replays_equal_eager(pkg, synth):
         assert np.array_equal(e1, g1) and np.array_equal(e2, g2)
 
 
+@pytest.mark.parametrize("name", ["c0", "c1"])
+def test_raster_list_overflow_path(pkg, oracle, pairs, name):
+    """Per-tile triangle lists capped at one entry: every tile with triangles
+    from two or more chunks takes the overflow path (box tests over the whole
+    mesh) and the priors, and everything after them, stay the oracle's."""
+    left, right, _, dmax = pairs(name)
+    ref = oracle.process(left, right, oracle.params(disp_max=dmax))
+    el = pkg.Elas(pkg.default_params(disp_max=dmax))
+    el.set_debug(True, tiny_lists=True)
+    D1, D2 = el.process(left, right)
+    _compare_all(el, ref, name + " tiny lists")
+    assert np.array_equal(D1, ref["D1"]) and np.array_equal(D2, ref["D2"])
+
+
 def test_sparse_texture_large_triangles(pkg, oracle, synth):
     """Texture only in five patches: twenty triangles per mesh, most spanning
     a large part of the image, so the raster's waves walk long column ranges
