@@ -64,6 +64,8 @@ struct ElasPair {
     int32_t* blkcnt;         // [nblk] valid support points per 256-cell block
     float* sp[2];            // [cw*ch][2] compacted support points (left, right)
     int32_t* nsp;            // [1]
+    float* hsp[2];           // sp[2] again, in mapped pinned host memory (for the host step)
+    int32_t* hnsp;           // nsp again, in mapped pinned host memory
     uint32_t* gflag[2];      // [gh*gw][words] scattered disparity flags
     uint32_t* gmask[2];      // [gh*gw][words] 3x3-dilated flags (interior cells)
     int32_t* glist[2];       // [gh*gw][dmax+2] reference list layout (debug only)

@@ -281,11 +281,21 @@ int elas_ctx::slot_reserve(ElasSlot& S, size_t arena_need, size_t pin_need, int 
         slot_drop_graphs(S);
         if (S.pin) (void)hipHostFree(S.pin);
         S.pin = nullptr;
+        S.pin_dev = nullptr;
         S.pin_bytes = 0;
-        if (hipHostMalloc(&S.pin, pin_need + pin_need / 4, hipHostMallocDefault) != hipSuccess) {
+        // mapped and coherent: the compaction kernel writes the support
+        // points and their count straight into it for the host step
+        if (hipHostMalloc(&S.pin, pin_need + pin_need / 4, hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess) {
             S.pin = nullptr;
             (void)hipGetLastError();
             return fail(ELAS_ERR_OUT_OF_MEMORY, "pinned staging allocation failed");
+        }
+        if (hipHostGetDevicePointer((void**)&S.pin_dev, S.pin, 0) != hipSuccess) {
+            (void)hipHostFree(S.pin);
+            S.pin = nullptr;
+            S.pin_dev = nullptr;
+            (void)hipGetLastError();
+            return fail(ELAS_ERR_OUT_OF_MEMORY, "pinned staging is not mapped for the device");
         }
         S.pin_bytes = pin_need + pin_need / 4;
     }
@@ -374,6 +384,14 @@ int elas_ctx::slot_prepare(ElasSlot& S, const ElasPairSpec* sp, int n) {
         P.blkcnt = (int32_t*)take(4 * (size_t)d.nblk);
         for (int k = 0; k < 2; k++) P.sp[k] = (float*)take(8 * d.ncan);
         P.nsp = S.dnsp + i;
+        {
+            // the same results in the pinned staging (device addresses), read
+            // by the host step without a copy
+            auto hdev = [&](const void* h) { return S.pin_dev + ((const char*)h - (const char*)S.pin); };
+            P.hsp[0] = (float*)hdev(S.hsp + S.sp_off[i]);
+            P.hsp[1] = (float*)hdev(S.hsp + S.sp_off[i] + 2 * d.ncan);
+            P.hnsp = (int32_t*)hdev(S.hnsp + i);
+        }
         for (int k = 0; k < 2; k++) P.gflag[k] = (uint32_t*)take(4 * d.ncell * d.words);
         for (int k = 0; k < 2; k++) P.gmask[k] = (uint32_t*)take(4 * d.ncell * d.words);
         if (debug) {
@@ -487,7 +505,6 @@ int elas_ctx::enqueue_a(ElasSlot& S, bool timed) {
     { ElasLaunch L = launch_of(S, 0, S_BLOCKS);
       if (L.total) { elas_launch_blkcount(L, g(L, cap256), s1); elas_launch_compact(L, g(L, cap256), s1); } }
     HIPCHECK(hipGetLastError());
-    HIPCHECK(hipMemcpyAsync(S.hnsp, S.dnsp, 4 * (size_t)n, hipMemcpyDeviceToHost, s1));
     return ELAS_OK;
 }
 
@@ -542,17 +559,9 @@ int elas_ctx::slot_start_tri(ElasSlot& S) {
     HIPCHECK(hipSetDevice(device));
     HIPCHECK(poll_event(S.ev[5]));
     S.t_tri0 = clk::now();
-    for (int i = 0; i < n; i++) {
-        S.nsp[i] = S.dd[i].nblk > 0 ? S.hnsp[i] : 0;
-        if (S.nsp[i] > 0) {
-            HIPCHECK(hipMemcpyAsync(S.hsp + S.sp_off[i], S.pairs[i].sp[0], 8 * (size_t)S.nsp[i],
-                                    hipMemcpyDeviceToHost, S.stream));
-            HIPCHECK(hipMemcpyAsync(S.hsp + S.sp_off[i] + 2 * S.dd[i].ncan, S.pairs[i].sp[1], 8 * (size_t)S.nsp[i],
-                                    hipMemcpyDeviceToHost, S.stream));
-        }
-    }
-    HIPCHECK(hipEventRecord(S.ev[14], S.stream));
-    HIPCHECK(poll_event(S.ev[14]));
+    // the compaction kernel wrote the counts and support points into the
+    // pinned staging; phase A is complete, so they are all there
+    for (int i = 0; i < n; i++) S.nsp[i] = S.dd[i].nblk > 0 ? S.hnsp[i] : 0;
     S.t_sp = clk::now();
     S.tri_out.assign(2 * (size_t)n, {});
     S.jobs.clear();

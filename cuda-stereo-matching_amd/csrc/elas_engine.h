@@ -63,6 +63,7 @@ struct ElasSlot {
     size_t table_bytes = 0;
     // pinned host staging
     void* pin = nullptr;
+    char* pin_dev = nullptr;  // device address of pin (mapped, coherent: kernels write results there)
     size_t pin_bytes = 0;
     void* tune = nullptr;  // dense tiling candidates: pair records + prefixes
     size_t tune_bytes = 0;

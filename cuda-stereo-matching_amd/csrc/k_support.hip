@@ -436,7 +436,9 @@ __global__ __launch_bounds__(256) void k_redun(ElasLaunch L) {
 // Compaction, KerSupportMatches.cu:361-423 + Elas.cu:338-394: row-major
 // order.  Pass 1 counts valid points per 256-cell block; pass 2 adds the
 // counts of earlier blocks and writes sp0 = (u*step, v*step) and
-// sp1 = (u*step - d, v*step); the last block stores the total.
+// sp1 = (u*step - d, v*step); the last block stores the total.  Points and
+// total go to the device arena and, for the host Delaunay step, to mapped
+// pinned host memory (so no copy sits between phase A and Triangle).
 __global__ __launch_bounds__(256) void k_blkcount(ElasLaunch L) {
     for (int t = blockIdx.x; t < L.total; t += gridDim.x) {
         const int p = elas_find_pair(L.prefix, L.npairs, t);
@@ -480,8 +482,14 @@ __global__ __launch_bounds__(256) void k_compact(ElasLaunch L) {
             const float pu = (float)(uc * P.step), pv = (float)(vc * P.step);
             reinterpret_cast<float2*>(P.sp[0])[off] = make_float2(pu, pv);
             reinterpret_cast<float2*>(P.sp[1])[off] = make_float2(pu - (float)d, pv);
+            reinterpret_cast<float2*>(P.hsp[0])[off] = make_float2(pu, pv);
+            reinterpret_cast<float2*>(P.hsp[1])[off] = make_float2(pu - (float)d, pv);
         }
-        if (b == P.nblk - 1 && tid == 0) P.nsp[0] = base + s_wave[0] + s_wave[1] + s_wave[2] + s_wave[3];
+        if (b == P.nblk - 1 && tid == 0) {
+            const int total = base + s_wave[0] + s_wave[1] + s_wave[2] + s_wave[3];
+            P.nsp[0] = total;
+            P.hnsp[0] = total;
+        }
         elas_lds_barrier();
     }
 }
