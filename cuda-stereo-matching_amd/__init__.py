@@ -27,7 +27,8 @@ STAGES = ["desc0", "desc1", "can_left", "can_lr", "can_incon", "can_final", "sp0
           "grid1", "tri0", "tri1", "mean0", "mean1", "d1_raw", "d2_raw", "D1", "D2"]
 KERNELS = ["descriptor", "support_left", "support_right", "dense", "lr"]  # elas_kernel order
 TIMERS = ["descriptor", "support", "filter", "compact", "grid", "triangle", "plane", "dense", "lr",
-          "post", "total", "phase_a", "phase_b", "sp_download", "tri_wall", "tri_job"]
+          "post", "total", "phase_a", "phase_b", "sp_download", "tri_wall", "tri_job",
+          "host_a", "host_b", "gpu_span", "ab_gap"]
 
 
 class ElasParams(ctypes.Structure):
@@ -446,11 +447,14 @@ class Elas:
         self._check(st)
         return [(d1[i], d2[i], int(stat[i])) for i in range(n)]
 
-    def set_debug(self, on=True, tiny_lists=False):
+    def set_debug(self, on=True, tiny_lists=False, gate=True):
         """Keep every intermediate of the next calls for fetch(); tiny_lists
         caps the raster's per-tile triangle lists at one entry (a test of the
-        overflow path, where a tile tests every triangle of the mesh)."""
-        self._check(self._lib.elas_hip_set_debug(self._ctx, (1 if on else 0) | (2 if tiny_lists else 0)))
+        overflow path, where a tile tests every triangle of the mesh); gate
+        False launches phase B after the host Delaunay step instead of
+        queueing it behind a gate during that step."""
+        self._check(self._lib.elas_hip_set_debug(
+            self._ctx, (1 if on else 0) | (2 if tiny_lists else 0) | (0 if gate else 4)))
 
     def shape_info(self):
         info = ShapeInfo()

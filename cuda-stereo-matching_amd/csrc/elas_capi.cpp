@@ -231,6 +231,7 @@ int elas_hip_set_debug(elas_ctx* c, int keep) {
     ELAS_NOT_SERVING(c);
     c->debug = (keep & 1) != 0;
     c->tiny_lists = (keep & 2) != 0;
+    c->gate_b = (keep & 4) == 0;
     c->param_gen++;
     return ELAS_OK;
 }

@@ -99,7 +99,7 @@ size_t pair_arena(const ElasPairSpec& s, const ElasDims& d, bool debug, bool pos
     b += 4 * al(4 * d.ncan) + al(4 * (size_t)d.nblk) + 2 * al(8 * d.ncan);
     b += 4 * al(4 * d.ncell * d.words);
     if (debug) b += 2 * al(4 * d.ncell * (size_t)(s.dmax + 2)) + 2 * al(4 * d.npx);
-    b += 2 * al(12 * (size_t)d.tricap) + 2 * al(4 * ELAS_TRIREC * (size_t)d.tricap);
+    b += 2 * al(4 * ELAS_TRIREC * (size_t)d.tricap);
     b += 2 * al(16 * (size_t)d.tricap) + 2 * al(4 * (size_t)d.nst) +
          2 * al(16 * (size_t)d.nst * d.scap);  // boxes, culling lists
     b += 2 * al(4 * d.npx) + 2 * al(4 * d.npx);  // prior, draw
@@ -200,11 +200,23 @@ int elas_ctx::slot_init(ElasSlot& S) {
     HIPCHECK(hipStreamCreateWithFlags(&S.stream, hipStreamNonBlocking));
     HIPCHECK(hipStreamCreateWithFlags(&S.stream2, hipStreamNonBlocking));
     for (auto& e : S.ev) HIPCHECK(hipEventCreate(&e));
+    HIPCHECK(hipHostMalloc((void**)&S.gate, 256, hipHostMallocMapped | hipHostMallocCoherent));
+    memset(S.gate, 0, 256);
+    HIPCHECK(hipHostGetDevicePointer((void**)&S.gate_dev, S.gate, 0));
     return ELAS_OK;
+}
+
+// Sets the phase B release flag if a gate is waiting on it (every path out
+// of a gated chunk goes through here or through slot_phase_b).
+void elas_ctx::gate_release(ElasSlot& S) {
+    if (!S.gate_armed) return;
+    __atomic_store_n(&S.gate[0], 1u, __ATOMIC_RELEASE);
+    S.gate_armed = false;
 }
 
 void elas_ctx::slot_free(ElasSlot& S) {
     (void)hipSetDevice(device);
+    gate_release(S);
     if (S.stream) (void)hipStreamSynchronize(S.stream);
     if (S.stream2) (void)hipStreamSynchronize(S.stream2);
     slot_drop_graphs(S);
@@ -218,6 +230,7 @@ void elas_ctx::slot_free(ElasSlot& S) {
     if (S.tune) (void)hipFree(S.tune);
     if (S.stream) (void)hipStreamDestroy(S.stream);
     if (S.stream2) (void)hipStreamDestroy(S.stream2);
+    if (S.gate) (void)hipHostFree(S.gate);
 }
 
 // Validates a chunk; returns the arena and pinned bytes it needs.
@@ -306,6 +319,7 @@ int elas_ctx::slot_reserve(ElasSlot& S, size_t arena_need, size_t pin_need, int 
 
 int elas_ctx::slot_prepare(ElasSlot& S, const ElasPairSpec* sp, int n) {
     S.t_start = clk::now();
+    S.gated = false;
     size_t arena_need = 0, pin_need = 0;
     int st = slot_need(sp, n, &arena_need, &pin_need);
     if (st != ELAS_OK) return st;
@@ -384,21 +398,21 @@ int elas_ctx::slot_prepare(ElasSlot& S, const ElasPairSpec* sp, int n) {
         P.blkcnt = (int32_t*)take(4 * (size_t)d.nblk);
         for (int k = 0; k < 2; k++) P.sp[k] = (float*)take(8 * d.ncan);
         P.nsp = S.dnsp + i;
-        {
-            // the same results in the pinned staging (device addresses), read
-            // by the host step without a copy
-            auto hdev = [&](const void* h) { return S.pin_dev + ((const char*)h - (const char*)S.pin); };
-            P.hsp[0] = (float*)hdev(S.hsp + S.sp_off[i]);
-            P.hsp[1] = (float*)hdev(S.hsp + S.sp_off[i] + 2 * d.ncan);
-            P.hnsp = (int32_t*)hdev(S.hnsp + i);
-        }
+        // device addresses in the mapped pinned staging: the support points
+        // and their count again (read by the host step without a copy), and
+        // the meshes (written by the host step, read by the plane kernel
+        // without a copy; the right mesh follows the left one)
+        auto hdev = [&](const void* h) { return S.pin_dev + ((const char*)h - (const char*)S.pin); };
+        P.hsp[0] = (float*)hdev(S.hsp + S.sp_off[i]);
+        P.hsp[1] = (float*)hdev(S.hsp + S.sp_off[i] + 2 * d.ncan);
+        P.hnsp = (int32_t*)hdev(S.hnsp + i);
+        P.tri[0] = P.tri[1] = (const int32_t*)hdev(S.htri + S.tri_off[i]);
         for (int k = 0; k < 2; k++) P.gflag[k] = (uint32_t*)take(4 * d.ncell * d.words);
         for (int k = 0; k < 2; k++) P.gmask[k] = (uint32_t*)take(4 * d.ncell * d.words);
         if (debug) {
             for (int k = 0; k < 2; k++) P.glist[k] = (int32_t*)take(4 * d.ncell * (size_t)(s.dmax + 2));
             for (int k = 0; k < 2; k++) P.mean[k] = (float*)take(4 * d.npx);
         }
-        for (int k = 0; k < 2; k++) P.tri[k] = (int32_t*)take(12 * (size_t)d.tricap);
         for (int k = 0; k < 2; k++) P.trirec[k] = (float*)take(4 * ELAS_TRIREC * (size_t)d.tricap);
         for (int k = 0; k < 2; k++) P.tbox[k] = (int32_t*)take(16 * (size_t)d.tricap);
         for (int k = 0; k < 2; k++) P.scnt[k] = (int32_t*)take(4 * (size_t)d.nst);
@@ -546,6 +560,10 @@ int elas_ctx::slot_phase_a(ElasSlot& S) {
                      if (debug) elas_launch_grid_list(L, g(L, cap256), s2); } }
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipEventRecord(S.ev[12], s2));
+    // phase B (on s1) needs the grid: the wait is queued now, while the GPU
+    // runs phase A, rather than after the host step
+    HIPCHECK(hipStreamWaitEvent(s1, S.ev[12], 0));
+    S.t_alaunch = clk::now();
     S.phase = SLOT_A;
     return ELAS_OK;
 }
@@ -586,6 +604,44 @@ int elas_ctx::slot_start_tri(ElasSlot& S) {
         S.ticket = elas_tri_submit(S.jobs.data(), (int)S.jobs.size());
     }
     S.phase = SLOT_TRI;
+    return S.gated ? slot_prelaunch_b(S) : ELAS_OK;
+}
+
+// Gated phase B, queued while the meshes are being computed: a placeholder
+// table (no triangles; every pointer valid) goes to pinned memory, then the
+// gate kernel and phase B are launched; slot_phase_b writes the real table
+// and sets the flag.
+int elas_ctx::slot_prelaunch_b(ElasSlot& S) {
+    const int n = S.n;
+    for (int i = 0; i < n; i++) {
+        S.pairs[i].ntri[0] = S.pairs[i].ntri[1] = 0;
+        S.pairs[i].tri[1] = S.pairs[i].tri[0];
+    }
+    build_prefix(prm, S);
+    // Nothing between the gate and its release may wait for the stream: no
+    // graph capture or instantiation (only a graph captured by an earlier
+    // call is launched here) and no copy to pageable memory (the result
+    // copies are queued after the release).
+    hipGraphExec_t ex = nullptr;
+    if (use_graphs) {
+        int r = slot_graph(S, 1, &ex, false);
+        if (r != ELAS_OK) return r;
+    }
+    if (!ex) return ELAS_OK;  // phase B is launched after the host step
+    memcpy(S.hpairs[1], S.pairs.data(), sizeof(ElasPair) * n);
+    memcpy(S.hprefix[1], S.prefix.data(), sizeof(int32_t) * S.prefix.size());
+    __atomic_store_n(&S.gate[0], 0u, __ATOMIC_RELEASE);
+    S.gate[1] = 0u;
+    hipStream_t s1 = S.stream;
+    auto hdev = [&](const void* h) { return S.pin_dev + ((const char*)h - (const char*)S.pin); };
+    const size_t bytes = S.tab_pairs + sizeof(int32_t) * S.prefix.size();
+    S.gate_armed = true;
+    elas_launch_gate(S.gate_dev, S.gate_dev + 1, (const uint32_t*)hdev(S.hpairs[1]), (uint32_t*)S.dpairs[1],
+                     (int)((bytes + 3) / 4), s1);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipEventRecord(S.ev[6], s1));
+    HIPCHECK(hipGraphLaunch(ex, s1));
+    HIPCHECK(hipEventRecord(S.ev[10], s1));
     return ELAS_OK;
 }
 
@@ -615,24 +671,34 @@ int elas_ctx::slot_phase_b(ElasSlot& S) {
         S.ntri1[i] = nt[1];
         S.pairs[i].ntri[0] = nt[0];
         S.pairs[i].ntri[1] = nt[1];
-        // both meshes back to back, one upload into the pair's two triangle
-        // regions (contiguous in the arena, each tricap triangles); the right
+        // both meshes back to back into the pair's region of the mapped
+        // pinned staging, which the plane kernel reads directly; the right
         // mesh starts after the left one
         if (nt[0] + nt[1] > 0) {
             int32_t* dst = S.htri + S.tri_off[i];
             for (int k = 0; k < 2; k++)
                 if (nt[k] > 0) memcpy(dst + (k ? 3 * (size_t)nt[0] : 0), S.tri_out[2 * i + k].data(), 12 * (size_t)nt[k]);
-            int32_t* base = const_cast<int32_t*>(S.pairs[i].tri[0]);
-            S.pairs[i].tri[1] = base + 3 * (size_t)nt[0];
-            HIPCHECK(hipMemcpyAsync(base, dst, 12 * (size_t)(nt[0] + nt[1]), hipMemcpyHostToDevice, S.stream));
         }
+        S.pairs[i].tri[1] = S.pairs[i].tri[0] + 3 * (size_t)nt[0];
     }
     build_prefix(prm, S);
     memcpy(S.hpairs[1], S.pairs.data(), sizeof(ElasPair) * n);
     memcpy(S.hprefix[1], S.prefix.data(), sizeof(int32_t) * S.prefix.size());
-    hipStream_t s1 = S.stream;
     S.t_tri1 = clk::now();
-    HIPCHECK(hipStreamWaitEvent(s1, S.ev[12], 0));
+    if (S.gate_armed) {
+        // phase B is queued: meshes and table are in place, open the gate
+        gate_release(S);
+        return slot_launch_copies(S);
+    }
+    if (S.gated)  // the gated form of phase B has no table copy of its own
+        HIPCHECK(hipMemcpyAsync(S.dpairs[1], S.hpairs[1], S.tab_pairs + sizeof(int32_t) * S.prefix.size(),
+                                hipMemcpyHostToDevice, S.stream));
+    return slot_launch_b(S);
+}
+
+// Launches phase B (graph or eager), then the result copies of the chunk.
+int elas_ctx::slot_launch_b(ElasSlot& S) {
+    hipStream_t s1 = S.stream;
     HIPCHECK(hipEventRecord(S.ev[6], s1));
     if (use_graphs) {
         hipGraphExec_t ex = nullptr;
@@ -644,12 +710,19 @@ int elas_ctx::slot_phase_b(ElasSlot& S) {
         if (r != ELAS_OK) return r;
     }
     HIPCHECK(hipEventRecord(S.ev[10], s1));
+    return slot_launch_copies(S);
+}
+
+int elas_ctx::slot_launch_copies(ElasSlot& S) {
+    const int n = S.n;
+    hipStream_t s1 = S.stream;
     for (int i = 0; i < n; i++)
         if (!S.specs[i].dev_out) {
             HIPCHECK(hipMemcpyAsync(S.specs[i].D1, S.pairs[i].dout[0], 4 * S.dd[i].npx, hipMemcpyDeviceToHost, s1));
             HIPCHECK(hipMemcpyAsync(S.specs[i].D2, S.pairs[i].dout[1], 4 * S.dd[i].npx, hipMemcpyDeviceToHost, s1));
         }
     HIPCHECK(hipEventRecord(S.ev[13], s1));
+    S.t_blaunch = clk::now();
     S.phase = SLOT_B;
     return ELAS_OK;
 }
@@ -662,8 +735,10 @@ int elas_ctx::enqueue_b(ElasSlot& S, bool timed) {
     const int n = S.n;
     hipStream_t s1 = S.stream;
     auto ev = [&](int i) { return timed ? hipEventRecord(S.ev[i], s1) : hipSuccess; };
-    HIPCHECK(hipMemcpyAsync(S.dpairs[1], S.hpairs[1], S.tab_pairs + sizeof(int32_t) * S.prefix.size(),
-                            hipMemcpyHostToDevice, s1));
+    // the table goes up here, or, behind a gate, with the gate kernel
+    if (!S.gated)
+        HIPCHECK(hipMemcpyAsync(S.dpairs[1], S.hpairs[1], S.tab_pairs + sizeof(int32_t) * S.prefix.size(),
+                                hipMemcpyHostToDevice, s1));
     const int cap256 = std::max(1, num_cu) * 8;
     auto g = [&](const ElasLaunch& L, int cap) { return std::min(L.total, cap); };
     HIPCHECK(ev(15));
@@ -712,12 +787,12 @@ int elas_ctx::enqueue_b(ElasSlot& S, bool timed) {
 // everything a launch depends on: pair shapes, where buffers live, the
 // parameter generation and the debug flag; the tables and pinned buffers
 // the graph reads are the slot's own, at fixed addresses.
-int elas_ctx::slot_graph(ElasSlot& S, int which, hipGraphExec_t* out) {
+int elas_ctx::slot_graph(ElasSlot& S, int which, hipGraphExec_t* out, bool capture) {
     std::vector<int64_t> key;
     key.reserve(6 + 5 * (size_t)S.n);
     key.push_back(S.n);
     key.push_back((int64_t)param_gen);
-    key.push_back(debug ? 1 : 0);
+    key.push_back((debug ? 1 : 0) | (S.gated ? 2 : 0));
     key.push_back((int64_t)(uintptr_t)S.arena);
     key.push_back((int64_t)(uintptr_t)S.table);
     for (int i = 0; i < S.n; i++) {
@@ -730,6 +805,10 @@ int elas_ctx::slot_graph(ElasSlot& S, int which, hipGraphExec_t* out) {
     ElasGraphEntry* e = nullptr;
     for (auto& x : S.graphs)
         if (x.key == key) e = &x;
+    if (!capture && (!e || !e->exec[which])) {
+        *out = nullptr;  // lookup only: no capture, no instantiation
+        return ELAS_OK;
+    }
     if (!e) {
         if (S.graphs.size() >= ELAS_GRAPHS_PER_SLOT) {
             auto it = std::min_element(S.graphs.begin(), S.graphs.end(),
@@ -779,6 +858,8 @@ int elas_ctx::slot_finish(ElasSlot& S, int32_t* statuses) {
     HIPCHECK(hipSetDevice(device));
     HIPCHECK(poll_event(S.ev[13]));
     S.t_end = clk::now();
+    if (S.gated && __atomic_load_n(&S.gate[1], __ATOMIC_ACQUIRE) != 0u)
+        return fail(ELAS_ERR_HIP, "phase B gate timed out waiting for the host step");
     auto el = [&](int a, int b) {
         float ms = 0.f;
         (void)hipEventElapsedTime(&ms, S.ev[a], S.ev[b]);
@@ -806,6 +887,10 @@ int elas_ctx::slot_finish(ElasSlot& S, int32_t* statuses) {
     float tj = 0.f;
     for (const ElasTriJob& j : S.jobs) tj = std::max(tj, (float)((j.t_end - j.t_begin) * 1e3));
     S.times[ELAS_T_TRI_JOB] = tj;
+    S.times[ELAS_T_HOST_A] = msd(S.t_start, S.t_alaunch);
+    S.times[ELAS_T_HOST_B] = msd(S.t_done, S.t_blaunch);
+    S.times[ELAS_T_GPU_SPAN] = el(0, 13);
+    S.times[ELAS_T_AB_GAP] = el(5, 6);
     if (statuses)
         for (int i = 0; i < S.n; i++) statuses[i] = S.status[i];
     if (autotune && force_cpw == 0 && force_dsh == 0)
@@ -973,6 +1058,7 @@ int elas_ctx::tune(ElasSlot& S, int i) {
 // its Triangle jobs (both write into the slot's buffers).
 void elas_ctx::slot_abort(ElasSlot& S) {
     (void)hipSetDevice(device);
+    gate_release(S);
     if (S.ticket) elas_tri_release(S.ticket);
     S.ticket = nullptr;
     if (S.stream) (void)hipStreamSynchronize(S.stream);
@@ -987,6 +1073,7 @@ int elas_ctx::run(const ElasPairSpec* sp, int n, int32_t* statuses) {
     ElasSlot& S = *slots[0];
     std::vector<int32_t> st(n > 0 ? n : 1, ELAS_OK);
     int r = slot_prepare(S, sp, n);
+    S.gated = gate_b;  // one slot: phase B may wait on the stream behind its gate
     if (r == ELAS_OK) r = slot_phase_a(S);
     if (r == ELAS_OK) r = slot_start_tri(S);
     if (r == ELAS_OK) r = slot_phase_b(S);
@@ -1111,7 +1198,7 @@ int elas_ctx::fetch(int stage, void* dst, size_t bytes, size_t* stage_bytes) {
     if (!dst || bytes == 0 || sz == 0) return ELAS_OK;
     if (!src) return fail(ELAS_ERR_INVALID_ARGUMENT, "stage not kept (enable debug before the call)");
     HIPCHECK(hipSetDevice(device));
-    HIPCHECK(hipMemcpy(dst, src, std::min(bytes, sz), hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(dst, src, std::min(bytes, sz), hipMemcpyDefault));  // the meshes live in pinned memory
     return ELAS_OK;
 }
 

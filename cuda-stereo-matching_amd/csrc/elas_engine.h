@@ -94,12 +94,20 @@ struct ElasSlot {
     std::vector<ElasTriJob> jobs;
     std::vector<int> job_of;
     ElasTriTicket* ticket = nullptr;
-    std::chrono::steady_clock::time_point t_start, t_tri0, t_sp, t_sub, t_done, t_tri1, t_end;
+    std::chrono::steady_clock::time_point t_start, t_alaunch, t_tri0, t_sp, t_sub, t_done, t_tri1, t_blaunch, t_end;
     float times[ELAS_T_COUNT] = {};
     // graphs of this slot's shape buckets
     std::vector<ElasGraphEntry> graphs;
     uint64_t graph_clock = 0, graph_captures = 0;
     bool graphed = false;  // the chunk in flight runs its phases as graphs
+    // Phase B behind a gate (single-pair path): it is queued while the host
+    // triangulates, behind a kernel that polls gate[0] and then copies the
+    // phase B table from pinned memory; the host sets gate[0] once the meshes
+    // and the table are written.  gate[1] reports a poll that timed out.
+    uint32_t* gate = nullptr;      // mapped coherent pinned words
+    uint32_t* gate_dev = nullptr;  // their device address
+    bool gated = false;            // this chunk's phase B goes through the gate
+    bool gate_armed = false;       // queued behind the gate, flag not yet set
 };
 
 // Tiling of one pair shape: dense grid cells per wave per step and steps per
@@ -122,6 +130,7 @@ struct elas_ctx {
     int tri_workers = ELAS_TRI_INSTANCES;
     int chunk = 0;  // pairs per pipeline chunk of a batch (0: whole batch)
     bool use_graphs = true;   // phases A and B replayed as graphs per shape bucket
+    bool gate_b = true;       // single-pair calls queue phase B behind a gate during Triangle
     uint64_t param_gen = 0;   // bumped when parameters or the debug flag change
     std::string err;
     std::vector<ElasSlot*> slots;
@@ -162,6 +171,10 @@ struct elas_ctx {
     int slot_phase_a(ElasSlot& S);
     bool slot_a_ready(ElasSlot& S);
     int slot_start_tri(ElasSlot& S);
+    int slot_prelaunch_b(ElasSlot& S);
+    int slot_launch_b(ElasSlot& S);
+    int slot_launch_copies(ElasSlot& S);
+    void gate_release(ElasSlot& S);
     bool slot_tri_ready(ElasSlot& S);
     int slot_phase_b(ElasSlot& S);
     bool slot_b_ready(ElasSlot& S);
@@ -169,7 +182,7 @@ struct elas_ctx {
     void slot_abort(ElasSlot& S);
     int enqueue_a(ElasSlot& S, bool timed);
     int enqueue_b(ElasSlot& S, bool timed);
-    int slot_graph(ElasSlot& S, int which, hipGraphExec_t* out);
+    int slot_graph(ElasSlot& S, int which, hipGraphExec_t* out, bool capture = true);
     void slot_drop_graphs(ElasSlot& S);
 };
 

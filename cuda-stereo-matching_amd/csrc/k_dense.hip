@@ -324,6 +324,36 @@ __global__ __launch_bounds__(256) void k_rtile(ElasLaunch L) {
     }
 }
 
+// ---- phase B gate ------------------------------------------------------------
+// Queued ahead of phase B while the host triangulates: lane 0 polls the
+// release flag in mapped pinned memory (system-scope acquire loads, about
+// 4 us apart) for at most 3e9 ticks of the 100 MHz wall clock
+// (30 s), then the workgroup copies the phase B table, which the host wrote
+// into pinned memory before setting the flag, into the device table.  After
+// a timeout the table is copied all the same (the host keeps it valid at
+// every moment) and status[0] tells the host.
+__global__ __launch_bounds__(256) void k_gate(const uint32_t* flag, uint32_t* status, const uint32_t* src,
+                                              uint32_t* dst, int words) {
+    if (threadIdx.x == 0) {
+        const uint64_t t0 = wall_clock64();
+        uint32_t ok = 0u;
+        for (;;) {
+            if (__hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM) != 0u) {
+                ok = 1u;
+                break;
+            }
+            if (wall_clock64() - t0 > 3000000000ull) break;
+            // about 4 us between polls: each poll is a coherent read of host
+            // memory, and polling much faster slows the host's Triangle
+            __builtin_amdgcn_s_sleep(127);
+        }
+        if (!ok) __hip_atomic_store(status, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < words; i += 256)
+        dst[i] = __hip_atomic_load(src + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 // Debug: the reference's mean map, (float)(int)plane(u,v) where covered, -1.
 __global__ __launch_bounds__(256) void k_mean(ElasLaunch L) {
     for (int t = blockIdx.x; t < L.total; t += gridDim.x) {
@@ -726,6 +756,10 @@ __global__ __launch_bounds__(256) void k_lr(ElasLaunch L, ElasKParams kp) {
 
 void elas_launch_plane(const ElasLaunch& L, int grid, size_t lds, hipStream_t s) {
     hipLaunchKernelGGL(k_plane, dim3(grid), dim3(64), lds, s, L);
+}
+void elas_launch_gate(const uint32_t* flag, uint32_t* status, const uint32_t* src, uint32_t* dst, int words,
+                      hipStream_t s) {
+    hipLaunchKernelGGL(k_gate, dim3(1), dim3(256), 0, s, flag, status, src, dst, words);
 }
 void elas_launch_rtile(const ElasLaunch& L, int grid, hipStream_t s) {
     hipLaunchKernelGGL(k_rtile, dim3(grid), dim3(256), 0, s, L);

@@ -115,7 +115,11 @@ typedef enum elas_timer {
     /* host step (wall clock): support points down, Triangle jobs from submit
      * to the last done, the longest job */
     ELAS_T_SP_DOWNLOAD = 13, ELAS_T_TRI_WALL = 14, ELAS_T_TRI_JOB = 15,
-    ELAS_T_COUNT = 16
+    /* host wall clock from the call's start to phase A launched, and from the
+     * meshes done to phase B launched; device time from phase A's start to the
+     * last copy of the call, and the device's idle gap between the phases */
+    ELAS_T_HOST_A = 16, ELAS_T_HOST_B = 17, ELAS_T_GPU_SPAN = 18, ELAS_T_AB_GAP = 19,
+    ELAS_T_COUNT = 20
 } elas_timer;
 
 typedef struct elas_ctx elas_ctx;
@@ -180,7 +184,10 @@ int elas_hip_set_pipeline(elas_ctx* ctx, int32_t slots, int32_t chunk);
  * and stores the stage size in *stage_bytes (either may be NULL/0 to query).
  * flags & 2 (a test of the raster's overflow path) caps the raster's
  * per-tile triangle lists at one entry, so every tile with triangles from
- * more than one 64-triangle chunk tests the whole mesh; results are the same. */
+ * more than one 64-triangle chunk tests the whole mesh; results are the same.
+ * flags & 4 turns off the phase B gate of single-pair calls (phase B is then
+ * launched after the host Delaunay step instead of being queued during it
+ * behind a kernel that waits for the host's release); results are the same. */
 int elas_hip_set_debug(elas_ctx* ctx, int flags);
 int elas_hip_debug_fetch(elas_ctx* ctx, int stage, void* dst, size_t bytes, size_t* stage_bytes);
 int elas_hip_shape_info(const elas_ctx* ctx, elas_shape_info* info);

@@ -173,6 +173,27 @@ def test_raster_list_overflow_path(pkg, oracle, pairs, name):
     assert np.array_equal(D1, ref["D1"]) and np.array_equal(D2, ref["D2"])
 
 
+def test_phase_b_gate_equals_ungated(pkg, pairs, synth):
+    """Single-pair calls queue phase B behind a gate while the host
+    triangulates; with the gate off phase B is launched afterwards.  Both give
+    the same maps, call after call (the gated graph is captured on the first
+    call and replayed behind the gate from the second), and across shapes."""
+    shapes = [pairs("c1")[:2] + (pairs("c1")[3],)]
+    shapes += [synth.make_pair(W, H, dm, regions=3, seed=900 + i)[:2] + (dm,)
+               for i, (W, H, dm) in enumerate([(320, 240, 63), (200, 150, 31)])]
+    on = pkg.Elas(pkg.default_params(disp_max=127))
+    off = pkg.Elas(pkg.default_params(disp_max=127))
+    off.set_debug(False, gate=False)
+    for rep in range(3):
+        for l, r, dm in shapes:
+            on.set_params(pkg.default_params(disp_max=dm))
+            off.set_params(pkg.default_params(disp_max=dm))
+            a1, a2 = on.process(l, r)
+            b1, b2 = off.process(l, r)
+            assert on.status == off.status == 0
+            assert np.array_equal(a1, b1) and np.array_equal(a2, b2), (rep, l.shape, dm)
+
+
 def test_sparse_texture_large_triangles(pkg, oracle, synth):
     """Texture only in five patches: twenty triangles per mesh, most spanning
     a large part of the image, so the raster's waves walk long column ranges
