@@ -283,6 +283,7 @@ int elas_ctx::slot_reserve(ElasSlot& S, size_t arena_need, size_t pin_need, int 
         if (S.table) (void)hipFree(S.table);
         S.table = nullptr;
         S.table_bytes = 0;
+        S.tab0_up.clear();
         if (hipMalloc(&S.table, tab + tab / 4) != hipSuccess) {
             S.table = nullptr;
             (void)hipGetLastError();
@@ -488,9 +489,6 @@ int elas_ctx::enqueue_a(ElasSlot& S, bool timed) {
     const int cap256 = std::max(1, num_cu) * 8;
     auto g = [&](const ElasLaunch& L, int cap) { return std::min(L.total, cap); };
     auto ev = [&](int i) { return timed ? hipEventRecord(S.ev[i], s1) : hipSuccess; };
-    // pair records and stage prefixes lie back to back in both tables: one copy
-    HIPCHECK(hipMemcpyAsync(S.dpairs[0], S.hpairs[0], S.tab_pairs + sizeof(int32_t) * S.prefix.size(),
-                            hipMemcpyHostToDevice, s1));
     HIPCHECK(ev(1));
     for (int sh : {18, 12, 6}) {
         const ElasLaunch L = launch_of(S, 0, sh == 18 ? S_DESC : sh == 12 ? S_DESC12 : S_DESC6);
@@ -529,6 +527,16 @@ int elas_ctx::slot_phase_a(ElasSlot& S) {
     memcpy(S.hpairs[0], S.pairs.data(), sizeof(ElasPair) * n);
     memcpy(S.hprefix[0], S.prefix.data(), sizeof(int32_t) * S.prefix.size());
     hipStream_t s1 = S.stream, s2 = S.stream2;
+    // pair records and stage prefixes lie back to back in both tables: one
+    // copy, skipped when the device table already holds these bytes (calls
+    // repeating a shape on the same buffers)
+    {
+        const size_t bytes = S.tab_pairs + sizeof(int32_t) * S.prefix.size();
+        if (S.tab0_up.size() != bytes || memcmp(S.tab0_up.data(), S.hpairs[0], bytes) != 0) {
+            HIPCHECK(hipMemcpyAsync(S.dpairs[0], S.hpairs[0], bytes, hipMemcpyHostToDevice, s1));
+            S.tab0_up.assign((const char*)S.hpairs[0], (const char*)S.hpairs[0] + bytes);
+        }
+    }
     for (int i = 0; i < n; i++)
         if (!S.specs[i].dev_in) {
             HIPCHECK(hipMemcpyAsync((void*)S.pairs[i].img[0], S.specs[i].left, S.dd[i].npx, hipMemcpyHostToDevice, s1));

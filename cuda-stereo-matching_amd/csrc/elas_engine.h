@@ -107,6 +107,7 @@ struct ElasSlot {
     uint32_t* gate = nullptr;      // mapped coherent pinned words
     uint32_t* gate_dev = nullptr;  // their device address
     bool gated = false;            // this chunk's phase B goes through the gate
+    std::vector<char> tab0_up;     // bytes last copied into the phase A device table
     bool gate_armed = false;       // queued behind the gate, flag not yet set
 };
 
