@@ -147,9 +147,12 @@ def load_library(path=LIB_PATH):
 
 def _torch_sync(t):
     """The library enqueues on its own streams: work torch queued on the
-    tensors' current stream has to finish first."""
+    tensors' current stream has to finish first (a query first: an idle
+    stream needs no synchronisation call)."""
     import torch
-    torch.cuda.current_stream(t.device).synchronize()
+    s = torch.cuda.current_stream(t.device)
+    if not s.query():
+        s.synchronize()
 
 
 def default_params(**overrides):
@@ -296,9 +299,11 @@ class Elas:
 
     def process_device(self, left, right, D1, D2):
         """Device-resident form: torch uint8 [H,W] cuda tensors in, float32 out."""
+        import torch
         H, W = int(left.shape[0]), int(left.shape[1])
-        for t, dt in ((left, "uint8"), (right, "uint8"), (D1, "float32"), (D2, "float32")):
-            if not t.is_cuda or not t.is_contiguous() or str(t.dtype) != "torch." + dt or tuple(t.shape) != (H, W):
+        for t, dt in ((left, torch.uint8), (right, torch.uint8), (D1, torch.float32), (D2, torch.float32)):
+            if t.dtype is not dt or not t.is_cuda or t.dim() != 2 or t.shape[0] != H or t.shape[1] != W \
+                    or not t.is_contiguous():
                 raise ValueError("process_device expects contiguous cuda tensors of shape [H, W]")
         _torch_sync(left)
         st = self._lib.elas_hip_process_device(self._ctx, left.data_ptr(), right.data_ptr(), D1.data_ptr(),
