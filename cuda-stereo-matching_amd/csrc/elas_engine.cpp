@@ -557,9 +557,11 @@ int elas_ctx::slot_phase_a(ElasSlot& S) {
         if (r != ELAS_OK) return r;
     }
     HIPCHECK(hipEventRecord(S.ev[5], s1));
-    // disparity grid (and the prior fill) on the second stream, overlapping
-    // the host Triangle step
-    HIPCHECK(hipStreamWaitEvent(s2, S.ev[5], 0));
+    // disparity grid on the second stream, overlapping the host Triangle
+    // step; a gated chunk (one slot, phase B queued behind the gate) keeps it
+    // on the first stream after phase A, where the device would only wait
+    if (S.gated) s2 = s1;
+    else HIPCHECK(hipStreamWaitEvent(s2, S.ev[5], 0));
     HIPCHECK(hipEventRecord(S.ev[11], s2));
     { ElasLaunch L = launch_of(S, 0, S_GRID_FILL); if (L.total) elas_launch_grid_fill(L, g(L, cap256), s2); }
     { ElasLaunch L = launch_of(S, 0, S_BLOCKS); if (L.total) elas_launch_grid_scatter(L, g(L, cap256), s2); }
@@ -570,7 +572,7 @@ int elas_ctx::slot_phase_a(ElasSlot& S) {
     HIPCHECK(hipEventRecord(S.ev[12], s2));
     // phase B (on s1) needs the grid: the wait is queued now, while the GPU
     // runs phase A, rather than after the host step
-    HIPCHECK(hipStreamWaitEvent(s1, S.ev[12], 0));
+    if (s2 != s1) HIPCHECK(hipStreamWaitEvent(s1, S.ev[12], 0));
     S.t_alaunch = clk::now();
     S.phase = SLOT_A;
     return ELAS_OK;
