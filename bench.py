@@ -225,9 +225,9 @@ KERNEL_SYMBOL = {"descriptor": "k_descriptor", "support_left": "k_support<false>
 
 def traffic_of(kernel, workload, W, H, dmax):
     """HBM bytes per launch of `kernel` from the committed PMC summary
-    profiles/hbm_traffic_<workload>_<W>x<H>_d<dmax+1>.txt (FETCH_SIZE x2 +
+    profiles/hbm_traffic_<W>x<H>_d<dmax+1>.txt (FETCH_SIZE x2 +
     WRITE_SIZE x1, scaled by a plain device copy), or None."""
-    path = os.path.join(ROOT, "profiles", "hbm_traffic_%s_%dx%d_d%d.txt" % (workload, W, H, dmax + 1))
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_%dx%d_d%d.txt" % (W, H, dmax + 1))
     if not os.path.isfile(path):
         return None
     for line in open(path):
@@ -264,8 +264,8 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the serving sweep")
     ap.add_argument("--serve-full", action="store_true",
                     help="serving sweep at 50..800 pairs/s with 2000 requests per rate")
-    ap.add_argument("--workload", default="c1", choices=["c0", "c1", "c2"],
-                    help="c1 (1242x375, d 0..127) is the headline; c0/c2 for profiling")
+    ap.add_argument("--workload", default="1242x375_d128", choices=["450x375_d64", "1242x375_d128", "2816x1880_d256"],
+                    help="1242x375_d128 is the headline; the others for profiling")
     ap.add_argument("--dense-tiles", default="",
                     help="force the dense tiling 'cells_per_wave,steps_per_chunk' (profiler runs, where the "
                          "autotuner's timings are distorted)")

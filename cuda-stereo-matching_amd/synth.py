@@ -92,12 +92,12 @@ def make_pair(W, H, dmax, regions=8, seed=0, slope=0.05, occlusion=0.0, noise=2.
 
 # Named workloads: (width, height, disp_max, regions, slope, occlusion, seed)
 CONFIGS = {
-    "c0": dict(W=450, H=375, dmax=63, regions=8, slope=0.05, occlusion=0.0, seed=0),
-    "c1": dict(W=1242, H=375, dmax=127, regions=16, slope=0.05, occlusion=0.05, seed=1),
-    "c2": dict(W=2816, H=1880, dmax=255, regions=32, slope=0.1, occlusion=0.0, seed=2),
+    "450x375_d64": dict(W=450, H=375, dmax=63, regions=8, slope=0.05, occlusion=0.0, seed=0),
+    "1242x375_d128": dict(W=1242, H=375, dmax=127, regions=16, slope=0.05, occlusion=0.05, seed=1),
+    "2816x1880_d256": dict(W=2816, H=1880, dmax=255, regions=32, slope=0.1, occlusion=0.0, seed=2),
 }
 
-# Batch mix: sizes drawn 3:2:1 from these shapes, config-0 generator.
+# Batch mix: sizes drawn 3:2:1 from these shapes, the same generator.
 BATCH_SHAPES = [(640, 480, 63), (1242, 375, 127), (1920, 1080, 255)]
 
 

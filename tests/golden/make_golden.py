@@ -27,7 +27,7 @@ def main():
     np.savez_compressed(os.path.join(HERE, "small_pair.npz"), left=l, right=r, disp_max=31,
                         D1=out["D1"], D2=out["D2"], sp0=out["sp0"], tri0=out["tri0"], tri1=out["tri1"],
                         can_final=out["can_final"])
-    l0, r0, _, _ = synth.config_pair("c0")
+    l0, r0, _, _ = synth.config_pair("450x375_d64")
     with open(os.path.join(HERE, "synth_sha256.txt"), "w") as f:
         f.write(hashlib.sha256(l0.tobytes()).hexdigest() + "\n" + hashlib.sha256(r0.tobytes()).hexdigest() + "\n")
 

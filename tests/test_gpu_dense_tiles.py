@@ -18,7 +18,7 @@ def _segs(W, cpw, gs=20):
     return -(-gw // (8 * cpw))
 
 
-@pytest.mark.parametrize("name", ["c0", "c1"])
+@pytest.mark.parametrize("name", ["450x375_d64", "1242x375_d128"])
 def test_every_tiling_bit_exact(pkg, oracle, pairs, name):
     left, right, _, dmax = pairs(name)
     H, W = left.shape
@@ -42,18 +42,18 @@ def test_every_tiling_bit_exact(pkg, oracle, pairs, name):
 
 def test_large_pair_tilings_bit_exact(pkg, oracle, pairs):
     """2816x1880, disparities 0..255: the rings wrap many times per chunk."""
-    left, right, _, dmax = pairs("c2")
+    left, right, _, dmax = pairs("2816x1880_d256")
     ref = oracle.process(left, right, oracle.params(disp_max=dmax), dump=False)
     el = pkg.Elas(pkg.default_params(disp_max=dmax))
     for cpw, spc in [(1, 1), (1, 5), (2, 3), (2, 9)]:
-        print("tiling c2", cpw, spc, flush=True)
+        print("tiling 2816x1880", cpw, spc, flush=True)
         el.set_dense_tiles(cpw, spc)
         D1, D2 = el.process(left, right)
         assert np.array_equal(D1, ref["D1"]) and np.array_equal(D2, ref["D2"]), (cpw, spc)
 
 
 def test_autotune_records_a_tiling_per_shape(pkg, pairs):
-    left, right, _, dmax = pairs("c1")
+    left, right, _, dmax = pairs("1242x375_d128")
     H, W = left.shape
     el = pkg.Elas(pkg.default_params(disp_max=dmax))
     assert el.dense_tiles(W, H, dmax)[2] == -1.0
@@ -87,7 +87,7 @@ def test_batch_mixing_tilings_equals_oracle(pkg, oracle, synth):
         assert np.array_equal(b1, ref["D1"]) and np.array_equal(b2, ref["D2"])
 
 
-@pytest.mark.parametrize("name", ["c1", "c2"])
+@pytest.mark.parametrize("name", ["1242x375_d128", "2816x1880_d256"])
 def test_every_descriptor_strip_height_bit_exact(pkg, oracle, pairs, name):
     """Strips of 6, 12 and 18 rows give the oracle's descriptors; a shape's
     tuned height is recorded with its time."""

@@ -24,7 +24,7 @@ def _compare_all(el, ref, label):
                 label, k, len(diff), diff[0].tolist(), got[tuple(diff[0])], exp[tuple(diff[0])]))
 
 
-@pytest.mark.parametrize("name", ["c0", "c1"])
+@pytest.mark.parametrize("name", ["450x375_d64", "1242x375_d128"])
 def test_every_stage_bit_exact(pkg, oracle, pairs, name):
     left, right, _, dmax = pairs(name)
     print("case", name, left.shape, "dmax", dmax, flush=True)
@@ -41,22 +41,22 @@ def test_every_stage_bit_exact(pkg, oracle, pairs, name):
     assert np.array_equal(el.fetch("D1"), D1)
 
 
-def test_large_c2_bit_exact(pkg, oracle, pairs):
+def test_large_2816x1880_bit_exact(pkg, oracle, pairs):
     """2816x1880, disparities 0..255: final maps and key intermediates."""
-    left, right, gt, dmax = pairs("c2")
-    print("case c2", left.shape, flush=True)
+    left, right, gt, dmax = pairs("2816x1880_d256")
+    print("case 2816x1880", left.shape, flush=True)
     el = pkg.Elas(pkg.default_params(disp_max=dmax))
     el.set_debug(True)
     D1, D2 = el.process(left, right)
     ref = oracle.process(left, right, oracle.params(disp_max=dmax))
-    _compare_all(el, ref, "c2")
+    _compare_all(el, ref, "2816x1880_d256")
     assert np.array_equal(D1, ref["D1"]) and np.array_equal(D2, ref["D2"])
     v = D1 >= 0
     assert v.mean() > 0.6 and (np.abs(D1[v] - gt[v]) <= 1).mean() > 0.9
 
 
 def test_repeat_runs_bitwise_equal(pkg, pairs):
-    left, right, _, dmax = pairs("c1")
+    left, right, _, dmax = pairs("1242x375_d128")
     el = pkg.Elas(pkg.default_params(disp_max=dmax))
     outs = [el.process(left, right) for _ in range(3)]
     for D1, D2 in outs[1:]:
@@ -65,7 +65,7 @@ def test_repeat_runs_bitwise_equal(pkg, pairs):
 
 def test_device_resident_equals_host(pkg, pairs):
     import torch
-    left, right, _, dmax = pairs("c0")
+    left, right, _, dmax = pairs("450x375_d64")
     el = pkg.Elas(pkg.default_params(disp_max=dmax))
     D1, D2 = el.process(left, right)
     l = torch.from_numpy(left).cuda()
@@ -159,7 +159,7 @@ def test_graph_replays_equal_eager(pkg, synth):
         assert np.array_equal(e1, g1) and np.array_equal(e2, g2)
 
 
-@pytest.mark.parametrize("name", ["c0", "c1"])
+@pytest.mark.parametrize("name", ["450x375_d64", "1242x375_d128"])
 def test_raster_list_overflow_path(pkg, oracle, pairs, name):
     """Per-tile triangle lists capped at one entry: every tile with triangles
     from two or more chunks takes the overflow path (box tests over the whole
@@ -178,7 +178,7 @@ def test_phase_b_gate_equals_ungated(pkg, pairs, synth):
     triangulates; with the gate off phase B is launched afterwards.  Both give
     the same maps, call after call (the gated graph is captured on the first
     call and replayed behind the gate from the second), and across shapes."""
-    shapes = [pairs("c1")[:2] + (pairs("c1")[3],)]
+    shapes = [pairs("1242x375_d128")[:2] + (pairs("1242x375_d128")[3],)]
     shapes += [synth.make_pair(W, H, dm, regions=3, seed=900 + i)[:2] + (dm,)
                for i, (W, H, dm) in enumerate([(320, 240, 63), (200, 150, 31)])]
     on = pkg.Elas(pkg.default_params(disp_max=127))

@@ -36,9 +36,9 @@ def test_post_filters_bit_exact(pkg, oracle, synth, case):
                 case, name, len(bad), bad[0].tolist(), got[tuple(bad[0])], exp[tuple(bad[0])]))
 
 
-def test_post_filters_c1_declared_defaults(pkg, oracle, pairs):
+def test_post_filters_1242x375_declared_defaults(pkg, oracle, pairs):
     """The Elas.h initialisers (post-processing on) on the 1242x375 workload."""
-    left, right, _, dmax = pairs("c1")
+    left, right, _, dmax = pairs("1242x375_d128")
     dp = pkg.declared_params(disp_max=dmax)
     el = pkg.Elas(dp)
     D1, D2 = el.process(left, right)

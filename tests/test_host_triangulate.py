@@ -62,7 +62,7 @@ def test_non_integer_points_keep_caller_order(pkg, oracle):
 
 
 def test_pipeline_meshes_of_c0(pkg, oracle, pairs):
-    left, right, _, dmax = pairs("c0")
+    left, right, _, dmax = pairs("450x375_d64")
     ref = oracle.process(left, right, oracle.params(disp_max=dmax))
     assert np.array_equal(pkg.triangulate(ref["sp0"]), ref["tri0"])
     assert np.array_equal(pkg.triangulate(ref["sp1"]), ref["tri1"])

@@ -11,14 +11,14 @@ GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 def test_generator_is_pinned(synth):
     """SHA-256 of the seeded 450x375 pair (numpy PCG64 + float64 arithmetic)."""
-    l, r, _, _ = synth.config_pair("c0")
+    l, r, _, _ = synth.config_pair("450x375_d64")
     pins = open(os.path.join(GOLDEN, "synth_sha256.txt")).read().split()
     assert hashlib.sha256(l.tobytes()).hexdigest() == pins[0]
     assert hashlib.sha256(r.tobytes()).hexdigest() == pins[1]
 
 
 def test_pipeline_recovers_ground_truth(oracle, pairs):
-    left, right, gt, dmax = pairs("c0")
+    left, right, gt, dmax = pairs("450x375_d64")
     out = oracle.process(left, right, oracle.params(disp_max=dmax))
     D1, D2 = out["D1"], out["D2"]
     v = D1 >= 0
@@ -30,7 +30,7 @@ def test_pipeline_recovers_ground_truth(oracle, pairs):
 
 
 def test_final_maps_are_lr_consistent(oracle, pairs):
-    left, right, _, dmax = pairs("c0")
+    left, right, _, dmax = pairs("450x375_d64")
     out = oracle.process(left, right, oracle.params(disp_max=dmax))
     D1, D2 = out["D1"], out["D2"]
     r1, r2 = out["d1_raw"], out["d2_raw"]
@@ -64,7 +64,7 @@ def _incircle_violations(pts, tri):
 
 
 def test_triangle_meshes_are_delaunay(oracle, pairs):
-    left, right, _, dmax = pairs("c0")
+    left, right, _, dmax = pairs("450x375_d64")
     out = oracle.process(left, right, oracle.params(disp_max=dmax))
     for sp, tri in ((out["sp0"], out["tri0"]), (out["sp1"], out["tri1"])):
         assert len(tri) > len(sp)
@@ -72,7 +72,7 @@ def test_triangle_meshes_are_delaunay(oracle, pairs):
 
 
 def test_oracle_is_deterministic(oracle, pairs):
-    left, right, _, dmax = pairs("c1")
+    left, right, _, dmax = pairs("1242x375_d128")
     a = oracle.process(left, right, oracle.params(disp_max=dmax), dump=False)
     b = oracle.process(left, right, oracle.params(disp_max=dmax), dump=False)
     assert np.array_equal(a["D1"], b["D1"]) and np.array_equal(a["D2"], b["D2"])

@@ -114,7 +114,7 @@ def test_plane_equals_exact_rational_plane(oracle, pairs):
     through three lattice corners (Cramer's rule over the rationals) to float
     precision; most coefficients are bit-identical (double rounding can leave
     e.g. 5.6e-17 where the exact value is 0)."""
-    left, right, _, dmax = pairs("c0")
+    left, right, _, dmax = pairs("450x375_d64")
     out = oracle.process(left, right, oracle.params(disp_max=dmax))
     can, sp0 = out["can_final"], out["sp0"]
     n_same = n_total = 0
