@@ -197,8 +197,9 @@ def kernel_rooflines(el, wc, W, H, dmax, reps=50):
     cw, ch = W // 5, H // 5
     work = {
         "descriptor": (0.0, 2 * 17.0 * W * H),
-        "support_left": (64.0 * wc["support_left"], 64.0 * W * ch * 2),
-        "support_right": (64.0 * wc["support_right"], 64.0 * W * ch * 2),
+        # both passes in one launch: the left pass reads the right image's
+        # descriptor rows v+-2 over the lattice rows, the right pass the left's
+        "support": (64.0 * (wc["support_left"] + wc["support_right"]), 2 * 64.0 * W * ch * 2),
         "dense": (16.0 * wc["dense"], 48.0 * W * H),
         "lr": (0.0, 16.0 * W * H),
     }
@@ -219,8 +220,7 @@ def kernel_rooflines(el, wc, W, H, dmax, reps=50):
     return out
 
 
-KERNEL_SYMBOL = {"descriptor": "k_descriptor", "support_left": "k_support<false>",
-                 "support_right": "k_support<true>", "dense": "k_dense<", "lr": "k_lr"}
+KERNEL_SYMBOL = {"descriptor": "k_descriptor", "support": "k_support_lr", "dense": "k_dense<", "lr": "k_lr"}
 
 
 def traffic_of(kernel, workload, W, H, dmax):

@@ -25,7 +25,7 @@ ELAS_OK, ELAS_ERR_TOO_FEW_POINTS, ELAS_ERR_BUSY, ELAS_ERR_IO = 0, 4, 7, 8
 
 STAGES = ["desc0", "desc1", "can_left", "can_lr", "can_incon", "can_final", "sp0", "sp1", "grid0",
           "grid1", "tri0", "tri1", "mean0", "mean1", "d1_raw", "d2_raw", "D1", "D2"]
-KERNELS = ["descriptor", "support_left", "support_right", "dense", "lr"]  # elas_kernel order
+KERNELS = ["descriptor", "support", "dense", "lr"]  # elas_kernel order
 TIMERS = ["descriptor", "support", "filter", "compact", "grid", "triangle", "plane", "dense", "lr",
           "post", "total", "phase_a", "phase_b", "sp_download", "tri_wall", "tri_job",
           "host_a", "host_b", "gpu_span", "ab_gap"]

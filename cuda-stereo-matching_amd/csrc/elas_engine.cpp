@@ -497,8 +497,7 @@ int elas_ctx::enqueue_a(ElasSlot& S, bool timed) {
     HIPCHECK(ev(2));
     { ElasLaunch L = launch_of(S, 0, S_SUPPORT);
       if (L.total) {
-          elas_launch_support(L, kp, false, g(L, cap256), support_lds(prm.candidate_stepsize, S.maxd, false), s1);
-          elas_launch_support(L, kp, true, g(L, cap256), support_lds(prm.candidate_stepsize, S.maxd, true), s1);
+          elas_launch_support(L, kp, g(L, cap256), support_lds(prm.candidate_stepsize, S.maxd, true), s1);
       } }
     HIPCHECK(ev(3));
     { ElasLaunch L = launch_of(S, 0, S_INCON); if (L.total) elas_launch_incon(L, kp, g(L, cap256), s1); }
@@ -1230,8 +1229,7 @@ int elas_ctx::time_kernel(int kernel, int reps, float* avg_ms) {
                 if (L.total) break;
             }
             break;
-        case ELAS_K_SUPPORT_LEFT:
-        case ELAS_K_SUPPORT_RIGHT: L = launch_of(S, 0, S_SUPPORT); break;
+        case ELAS_K_SUPPORT: L = launch_of(S, 0, S_SUPPORT); break;
         case ELAS_K_DENSE:
             // the launch of the pairs tiled with one cell per wave, else two
             L = launch_of(S, 1, S_DENSE);
@@ -1248,11 +1246,8 @@ int elas_ctx::time_kernel(int kernel, int reps, float* avg_ms) {
     auto launch = [&]() {
         switch (kernel) {
             case ELAS_K_DESCRIPTOR: elas_launch_descriptor(L, desc_sh, g(L, cap256), s1); break;
-            case ELAS_K_SUPPORT_LEFT:
-                elas_launch_support(L, kp, false, g(L, cap256), support_lds(prm.candidate_stepsize, S.maxd, false), s1);
-                break;
-            case ELAS_K_SUPPORT_RIGHT:
-                elas_launch_support(L, kp, true, g(L, cap256), support_lds(prm.candidate_stepsize, S.maxd, true), s1);
+            case ELAS_K_SUPPORT:
+                elas_launch_support(L, kp, g(L, cap256), support_lds(prm.candidate_stepsize, S.maxd, true), s1);
                 break;
             case ELAS_K_DENSE:
                 elas_launch_dense(L, S.kpl, dense_cpw, dense_grid(L.total, num_cu), S.dense_lds[dense_cpw - 1], s1);

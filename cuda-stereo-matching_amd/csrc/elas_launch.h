@@ -6,8 +6,9 @@
 #include "elas_common.h"
 
 void elas_launch_descriptor(const ElasLaunch& L, int sh, int grid, hipStream_t s);
-void elas_launch_support(const ElasLaunch& L, const ElasKParams& kp, bool right, int grid, size_t lds,
-                         hipStream_t s);
+// Support matching, left pass and right-to-left check in one launch (lds:
+// the right pass's staging, the larger of the two).
+void elas_launch_support(const ElasLaunch& L, const ElasKParams& kp, int grid, size_t lds, hipStream_t s);
 void elas_launch_incon(const ElasLaunch& L, const ElasKParams& kp, int grid, hipStream_t s);
 void elas_launch_redun(const ElasLaunch& L, bool vertical, int grid, size_t lds, hipStream_t s);
 void elas_launch_blkcount(const ElasLaunch& L, int grid, hipStream_t s);

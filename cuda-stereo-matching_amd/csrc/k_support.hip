@@ -33,19 +33,13 @@
 // The four lanes of a candidate merge their two smallest keys with two
 // shuffles.
 template <bool RIGHT>
-__global__ __launch_bounds__(256) void k_support(ElasLaunch L, ElasKParams kp) {
-    extern __shared__ uint4 s_rows[];  // [2][ncols]: row v-2, then row v+2
+__device__ __forceinline__ int support_pass(const ElasPair& P, const ElasKParams& kp, uint4* s_rows, int v_can,
+                                            int k0, int d_left_in) {
     const int tid = threadIdx.x;
     const int wave = tid >> 6, lane = tid & 63;
     const int j = lane >> 4, cl = lane & 15;
-    for (int t = blockIdx.x; t < L.total; t += gridDim.x) {
-        const int p = elas_find_pair(L.prefix, L.npairs, t);
-        const auto& P = L.pairs[p];
+    {
         const int W = P.W, H = P.H, cw = P.cw, step = P.step;
-        const int local = t - L.prefix[p];
-        const int per_row = (cw + 63) >> 6;
-        const int v_can = local / per_row;
-        const int k0 = (local - v_can * per_row) * 64;
         const int u_can = k0 + 16 * wave + cl;
         const int v = v_can * step;
         const int u_first = k0 * step, u_last = min(k0 + 63, cw - 1) * step;
@@ -75,9 +69,8 @@ __global__ __launch_bounds__(256) void k_support(ElasLaunch L, ElasKParams kp) {
         }
         int u = u_can * step;
         bool ok = row_ok && u_can < cw && u >= 5 && u <= W - 6;
-        int d_left = 0;
+        const int d_left = RIGHT ? d_left_in : 0;
         if (ok && RIGHT) {
-            d_left = elas_ld(P.can[0] + v_can * cw + u_can);
             if (d_left < 0) ok = false;
             else u -= d_left;
         }
@@ -177,20 +170,42 @@ __global__ __launch_bounds__(256) void k_support(ElasLaunch L, ElasKParams kp) {
             k2 = min(max(k1, o1), min(k2, o2));
             k1 = min(k1, o1);
         }
-        if (j == 0 && u_can < cw) {
-            int result = -1;
-            if (ok) {
-                const uint32_t m = k1 >> 15;
-                const uint32_t min2 = (((k1 >> 10) & 31u) == 0u) ? m : (k2 >> 15);
-                const int dstar = (int)(k1 & 1023u);
-                if ((float)m < kp.support_threshold * (float)min2) {
-                    if (!RIGHT) result = dstar;
-                    else result = (abs(d_left - dstar) > kp.lr_threshold) ? -1 : d_left;
-                }
+        int result = -1;
+        if (ok) {
+            const uint32_t m = k1 >> 15;
+            const uint32_t min2 = (((k1 >> 10) & 31u) == 0u) ? m : (k2 >> 15);
+            const int dstar = (int)(k1 & 1023u);
+            if ((float)m < kp.support_threshold * (float)min2) {
+                if (!RIGHT) result = dstar;
+                else result = (abs(d_left - dstar) > kp.lr_threshold) ? -1 : d_left;
             }
-            elas_st((RIGHT ? P.can[1] : P.can[0]) + v_can * cw + u_can, result);
         }
-        elas_lds_barrier();
+        elas_lds_barrier();  // s_rows is free for the next pass
+        return result;
+    }
+}
+
+// Both passes of a unit in one workgroup: the left pass's result for a
+// candidate stays in its four lanes and feeds the right-to-left check, which
+// stages the other image's rows in the same LDS.
+__global__ __launch_bounds__(256) void k_support_lr(ElasLaunch L, ElasKParams kp) {
+    extern __shared__ uint4 s_rows[];  // [2][ncols]: row v-2, then row v+2
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int t = blockIdx.x; t < L.total; t += gridDim.x) {
+        const int p = elas_find_pair(L.prefix, L.npairs, t);
+        const auto& P = L.pairs[p];
+        const int cw = P.cw;
+        const int local = t - L.prefix[p];
+        const int per_row = (cw + 63) >> 6;
+        const int v_can = local / per_row;
+        const int k0 = (local - v_can * per_row) * 64;
+        const int u_can = k0 + 16 * wave + (lane & 15);
+        const int dl = support_pass<false>(P, kp, s_rows, v_can, k0, 0);
+        const int dr = support_pass<true>(P, kp, s_rows, v_can, k0, dl);
+        if ((lane >> 4) == 0 && u_can < cw) {
+            elas_st(P.can[0] + v_can * cw + u_can, dl);
+            elas_st(P.can[1] + v_can * cw + u_can, dr);
+        }
     }
 }
 
@@ -577,10 +592,8 @@ __global__ __launch_bounds__(256) void k_grid_list(ElasLaunch L) {
     }
 }
 
-void elas_launch_support(const ElasLaunch& L, const ElasKParams& kp, bool right, int grid, size_t lds,
-                         hipStream_t s) {
-    if (right) hipLaunchKernelGGL(k_support<true>, dim3(grid), dim3(256), lds, s, L, kp);
-    else hipLaunchKernelGGL(k_support<false>, dim3(grid), dim3(256), lds, s, L, kp);
+void elas_launch_support(const ElasLaunch& L, const ElasKParams& kp, int grid, size_t lds, hipStream_t s) {
+    hipLaunchKernelGGL(k_support_lr, dim3(grid), dim3(256), lds, s, L, kp);
 }
 void elas_launch_incon(const ElasLaunch& L, const ElasKParams& kp, int grid, hipStream_t s) {
     hipLaunchKernelGGL(k_incon, dim3(grid), dim3(256), 0, s, L, kp);

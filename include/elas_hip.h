@@ -200,8 +200,8 @@ int elas_hip_stage_times(const elas_ctx* ctx, float* ms, int32_t n);
  * here recomputes the same outputs from the same inputs, so the results of
  * the last call are unchanged. */
 typedef enum elas_kernel {
-    ELAS_K_DESCRIPTOR = 0, ELAS_K_SUPPORT_LEFT = 1, ELAS_K_SUPPORT_RIGHT = 2,
-    ELAS_K_DENSE = 3, ELAS_K_LR = 4, ELAS_K_COUNT = 5
+    ELAS_K_DESCRIPTOR = 0, ELAS_K_SUPPORT = 1, /* both support passes, one launch */
+    ELAS_K_DENSE = 2, ELAS_K_LR = 3, ELAS_K_COUNT = 4
 } elas_kernel;
 int elas_hip_time_kernel(elas_ctx* ctx, int32_t kernel, int32_t reps, float* avg_ms);
 
