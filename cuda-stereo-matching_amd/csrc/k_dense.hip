@@ -7,37 +7,87 @@
 #include "elas_device.h"
 
 // ---- plane solve: kerSolve, KerComputeDisparity.cu:322-375 ---------------
-// Gauss-Jordan with full pivoting in double (eps 1e-20, ties on >= keep the
-// last pivot), the solution overwriting B.
+// Gauss-Jordan with full pivoting in double (eps 1e-20), the solution
+// overwriting B, operation for operation as the reference.  The reference
+// swaps the pivot row into position icol; here rows stay in their registers
+// and pos[r] is the position row r would hold, so every access is by a fixed
+// index and only the pivot's row and column are picked by selects.  Its
+// pivot search scans positions j then columns k and takes the last element
+// with |a| >= the running maximum: the largest (|a|, j, k), which is what the
+// search below takes over the live (row, column) pairs.
 __device__ bool elas_solve3(double A[3][3], double B[3]) {
     const double eps = 1e-20;
-    int ipiv[3] = {0, 0, 0};
-    int irow = 0, icol = 0;
+    int pos[3] = {0, 1, 2};
+    bool used[3] = {false, false, false}, cdone[3] = {false, false, false};
+    int pr = 0, pc = 0;  // pivot row (register index) and column
+#pragma unroll
     for (int i = 0; i < 3; i++) {
-        double big = 0.0;
-        for (int j = 0; j < 3; j++)
-            if (ipiv[j] != 1)
-                for (int k = 0; k < 3; k++)
-                    if (ipiv[k] == 0)
-                        if (fabs(A[j][k]) >= big) { big = fabs(A[j][k]); irow = j; icol = k; }
-        ++(ipiv[icol]);
-        if (irow != icol) {
-            for (int l = 0; l < 3; l++) { const double tmp = A[irow][l]; A[irow][l] = A[icol][l]; A[icol][l] = tmp; }
-            const double tb = B[irow]; B[irow] = B[icol]; B[icol] = tb;
-        }
-        if (fabs(A[icol][icol]) < eps) return false;
-        const double pivinv = 1.0 / A[icol][icol];
-        A[icol][icol] = 1.0;
-        for (int l = 0; l < 3; l++) A[icol][l] = __dmul_rn(A[icol][l], pivinv);
-        B[icol] = __dmul_rn(B[icol], pivinv);
-        for (int ll = 0; ll < 3; ll++)
-            if (ll != icol) {
-                const double dum = A[ll][icol];
-                A[ll][icol] = 0.0;
-                for (int l = 0; l < 3; l++) A[ll][l] = __dsub_rn(A[ll][l], __dmul_rn(A[icol][l], dum));
-                B[ll] = __dsub_rn(B[ll], __dmul_rn(B[icol], dum));
+        double big = -1.0;
+        int bj = -1, bk = -1;
+#pragma unroll
+        for (int r = 0; r < 3; r++)
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                const double a = fabs(A[r][k]);
+                const bool later = pos[r] > bj || (pos[r] == bj && k > bk);
+                if (!used[r] && !cdone[k] && (a > big || (a == big && later))) {
+                    big = a;
+                    bj = pos[r];
+                    bk = k;
+                    pr = r;
+                    pc = k;
+                }
             }
+        cdone[pc] = true;
+        // the row at position pc takes the pivot row's old position
+        const int irow = pos[pr];
+#pragma unroll
+        for (int r = 0; r < 3; r++)
+            if (pos[r] == pc) pos[r] = irow;
+        pos[pr] = pc;
+        double piv = 0.0, prow[3], pb = 0.0;
+#pragma unroll
+        for (int r = 0; r < 3; r++)
+            if (r == pr) {
+#pragma unroll
+                for (int l = 0; l < 3; l++) prow[l] = A[r][l];
+                pb = B[r];
+            }
+#pragma unroll
+        for (int l = 0; l < 3; l++)
+            if (l == pc) piv = prow[l];
+        if (fabs(piv) < eps) return false;
+        const double pivinv = 1.0 / piv;
+#pragma unroll
+        for (int l = 0; l < 3; l++) prow[l] = __dmul_rn(l == pc ? 1.0 : prow[l], pivinv);
+        pb = __dmul_rn(pb, pivinv);
+#pragma unroll
+        for (int r = 0; r < 3; r++) {
+            double dum = 0.0;
+#pragma unroll
+            for (int l = 0; l < 3; l++)
+                if (l == pc) dum = A[r][l];
+            const bool isp = r == pr;
+#pragma unroll
+            for (int l = 0; l < 3; l++) {
+                const double e = __dsub_rn(l == pc ? 0.0 : A[r][l], __dmul_rn(prow[l], dum));
+                A[r][l] = isp ? prow[l] : e;
+            }
+            const double eb = __dsub_rn(B[r], __dmul_rn(pb, dum));
+            B[r] = isp ? pb : eb;
+        }
+        used[pr] = true;
     }
+    // the reference's B holds the solution by position: variable k at
+    // position k
+    double x[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+#pragma unroll
+        for (int r = 0; r < 3; r++)
+            if (pos[r] == k) x[k] = B[r];
+#pragma unroll
+    for (int k = 0; k < 3; k++) B[k] = x[k];
     return true;
 }
 
