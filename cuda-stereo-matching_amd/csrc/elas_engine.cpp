@@ -8,10 +8,14 @@
 //   * device buffers live in per-slot arenas that grow on demand and are
 //     reused across calls (the reference allocates ~20 buffers per call and
 //     leaks its scan buffers, Elas.cu:349-371);
-//   * the disparity grid (Elas.cu:65) runs on a second stream while the host
-//     triangulates, the left and right meshes are triangulated in parallel by
-//     independent Triangle instances, and a batch keeps several chunks in
-//     flight so that the GPU phases of one overlap the host step of another;
+//   * the disparity grid (Elas.cu:65) runs while the host triangulates (on a
+//     second stream, or after phase A when phase B waits behind its gate),
+//     the left and right meshes are triangulated in parallel by independent
+//     Triangle instances, and a batch keeps several chunks in flight so that
+//     the GPU phases of one overlap the host step of another;
+//   * a single-pair call queues phase B while the host triangulates, behind a
+//     gate kernel that the host opens once the meshes are written, so no
+//     launch cost follows the host step;
 //   * buffers the reference initialises with thrust::fill (-1 for D1/D2 and
 //     the plane map, 0 for the grid, Elas.cu:94-95,415-420,488) are written in
 //     full by the kernels themselves, or cleared by fill kernels.
@@ -480,9 +484,10 @@ ElasLaunch launch_of(const ElasSlot& S, int table, int st) {
 
 // ---- phase A: descriptors, support points, filters, compaction, grid -------
 
-// Phase A on the pipeline stream: table upload, descriptors .. compaction,
-// support point count back to pinned memory.  With `timed`, stage events are
-// recorded between the kernels (not possible inside a captured graph).
+// Phase A on the pipeline stream: descriptors .. compaction, which writes the
+// support points and their count into the pinned staging as well (the table
+// goes up before, in slot_phase_a).  With `timed`, stage events are recorded
+// between the kernels (not possible inside a captured graph).
 int elas_ctx::enqueue_a(ElasSlot& S, bool timed) {
     const int n = S.n;
     hipStream_t s1 = S.stream;
@@ -579,7 +584,7 @@ int elas_ctx::slot_phase_a(ElasSlot& S) {
 
 bool elas_ctx::slot_a_ready(ElasSlot& S) { return hipEventQuery(S.ev[5]) == hipSuccess; }
 
-// ---- host step: support points down, Triangle jobs queued --------------------
+// ---- host step: support points read from pinned memory, Triangle jobs queued --
 
 int elas_ctx::slot_start_tri(ElasSlot& S) {
     const int n = S.n;
@@ -736,8 +741,8 @@ int elas_ctx::slot_launch_copies(ElasSlot& S) {
     return ELAS_OK;
 }
 
-// Phase B on the pipeline stream: table upload, plane records, raster, dense
-// matching, LR check, post filters.  Every launch depends only on the chunk's
+// Phase B on the pipeline stream: table upload (unless a gate kernel copies
+// the table), plane records, raster, dense matching, LR check, post filters.  Every launch depends only on the chunk's
 // shapes and parameters (the plane kernel reads its triangle count from the
 // table), so the sequence can be captured once per shape bucket.
 int elas_ctx::enqueue_b(ElasSlot& S, bool timed) {

@@ -112,8 +112,9 @@ typedef enum elas_timer {
     ELAS_T_GRID = 4, ELAS_T_TRIANGLE = 5, ELAS_T_PLANE = 6, ELAS_T_DENSE = 7,
     ELAS_T_LR = 8, ELAS_T_POST = 9, ELAS_T_TOTAL = 10, ELAS_T_PHASE_A = 11,
     ELAS_T_PHASE_B = 12,
-    /* host step (wall clock): support points down, Triangle jobs from submit
-     * to the last done, the longest job */
+    /* host step (wall clock): support points read (the compaction kernel
+     * writes them into pinned memory), Triangle jobs from submit to the last
+     * done, the longest job */
     ELAS_T_SP_DOWNLOAD = 13, ELAS_T_TRI_WALL = 14, ELAS_T_TRI_JOB = 15,
     /* host wall clock from the call's start to phase A launched, and from the
      * meshes done to phase B launched; device time from phase A's start to the
