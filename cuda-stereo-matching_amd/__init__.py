@@ -452,14 +452,15 @@ class Elas:
         self._check(st)
         return [(d1[i], d2[i], int(stat[i])) for i in range(n)]
 
-    def set_debug(self, on=True, tiny_lists=False, gate=True):
+    def set_debug(self, on=True, tiny_lists=False, gate=True, short_gate=False):
         """Keep every intermediate of the next calls for fetch(); tiny_lists
         caps the raster's per-tile triangle lists at one entry (a test of the
         overflow path, where a tile tests every triangle of the mesh); gate
         False launches phase B after the host Delaunay step instead of
-        queueing it behind a gate during that step."""
+        queueing it behind a gate during that step; short_gate bounds the
+        gate's wait at about 50 us (a test of its failure path)."""
         self._check(self._lib.elas_hip_set_debug(
-            self._ctx, (1 if on else 0) | (2 if tiny_lists else 0) | (0 if gate else 4)))
+            self._ctx, (1 if on else 0) | (2 if tiny_lists else 0) | (0 if gate else 4) | (8 if short_gate else 0)))
 
     def shape_info(self):
         info = ShapeInfo()

@@ -325,6 +325,7 @@ int elas_ctx::slot_reserve(ElasSlot& S, size_t arena_need, size_t pin_need, int 
 int elas_ctx::slot_prepare(ElasSlot& S, const ElasPairSpec* sp, int n) {
     S.t_start = clk::now();
     S.gated = false;
+    S.gate_used = false;
     size_t arena_need = 0, pin_need = 0;
     int st = slot_need(sp, n, &arena_need, &pin_need);
     if (st != ELAS_OK) return st;
@@ -650,8 +651,9 @@ int elas_ctx::slot_prelaunch_b(ElasSlot& S) {
     auto hdev = [&](const void* h) { return S.pin_dev + ((const char*)h - (const char*)S.pin); };
     const size_t bytes = S.tab_pairs + sizeof(int32_t) * S.prefix.size();
     S.gate_armed = true;
+    S.gate_used = true;
     elas_launch_gate(S.gate_dev, S.gate_dev + 1, (const uint32_t*)hdev(S.hpairs[1]), (uint32_t*)S.dpairs[1],
-                     (int)((bytes + 3) / 4), s1);
+                     (int)((bytes + 3) / 4), gate_ticks, s1);
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipEventRecord(S.ev[6], s1));
     HIPCHECK(hipGraphLaunch(ex, s1));
@@ -872,7 +874,7 @@ int elas_ctx::slot_finish(ElasSlot& S, int32_t* statuses) {
     HIPCHECK(hipSetDevice(device));
     HIPCHECK(poll_event(S.ev[13]));
     S.t_end = clk::now();
-    if (S.gated && __atomic_load_n(&S.gate[1], __ATOMIC_ACQUIRE) != 0u)
+    if (S.gate_used && __atomic_load_n(&S.gate[1], __ATOMIC_ACQUIRE) != 0u)
         return fail(ELAS_ERR_HIP, "phase B gate timed out waiting for the host step");
     auto el = [&](int a, int b) {
         float ms = 0.f;

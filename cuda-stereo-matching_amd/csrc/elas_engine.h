@@ -109,6 +109,7 @@ struct ElasSlot {
     bool gated = false;            // this chunk's phase B goes through the gate
     std::vector<char> tab0_up;     // bytes last copied into the phase A device table
     bool gate_armed = false;       // queued behind the gate, flag not yet set
+    bool gate_used = false;        // this call's phase B went through the gate (gate[1] is its status)
 };
 
 // Tiling of one pair shape: dense grid cells per wave per step and steps per
@@ -132,6 +133,7 @@ struct elas_ctx {
     int chunk = 0;  // pairs per pipeline chunk of a batch (0: whole batch)
     bool use_graphs = true;   // phases A and B replayed as graphs per shape bucket
     bool gate_b = true;       // single-pair calls queue phase B behind a gate during Triangle
+    uint64_t gate_ticks = 3000000000ull;  // the gate's poll bound (100 MHz ticks: 30 s)
     uint64_t param_gen = 0;   // bumped when parameters or the debug flag change
     std::string err;
     std::vector<ElasSlot*> slots;

@@ -377,13 +377,13 @@ __global__ __launch_bounds__(256) void k_rtile(ElasLaunch L) {
 // ---- phase B gate ------------------------------------------------------------
 // Queued ahead of phase B while the host triangulates: lane 0 polls the
 // release flag in mapped pinned memory (system-scope acquire loads, about
-// 4 us apart) for at most 3e9 ticks of the 100 MHz wall clock
-// (30 s), then the workgroup copies the phase B table, which the host wrote
+// 4 us apart) for at most `limit` ticks of the 100 MHz wall clock (3e9, 30 s,
+// unless a test shortens it), then the workgroup copies the phase B table, which the host wrote
 // into pinned memory before setting the flag, into the device table.  After
 // a timeout the table is copied all the same (the host keeps it valid at
 // every moment) and status[0] tells the host.
 __global__ __launch_bounds__(256) void k_gate(const uint32_t* flag, uint32_t* status, const uint32_t* src,
-                                              uint32_t* dst, int words) {
+                                              uint32_t* dst, int words, uint64_t limit) {
     if (threadIdx.x == 0) {
         const uint64_t t0 = wall_clock64();
         uint32_t ok = 0u;
@@ -392,7 +392,7 @@ __global__ __launch_bounds__(256) void k_gate(const uint32_t* flag, uint32_t* st
                 ok = 1u;
                 break;
             }
-            if (wall_clock64() - t0 > 3000000000ull) break;
+            if (wall_clock64() - t0 > limit) break;
             // about 4 us between polls: each poll is a coherent read of host
             // memory, and polling much faster slows the host's Triangle
             __builtin_amdgcn_s_sleep(127);
@@ -808,8 +808,8 @@ void elas_launch_plane(const ElasLaunch& L, int grid, size_t lds, hipStream_t s)
     hipLaunchKernelGGL(k_plane, dim3(grid), dim3(64), lds, s, L);
 }
 void elas_launch_gate(const uint32_t* flag, uint32_t* status, const uint32_t* src, uint32_t* dst, int words,
-                      hipStream_t s) {
-    hipLaunchKernelGGL(k_gate, dim3(1), dim3(256), 0, s, flag, status, src, dst, words);
+                      uint64_t limit, hipStream_t s) {
+    hipLaunchKernelGGL(k_gate, dim3(1), dim3(256), 0, s, flag, status, src, dst, words, limit);
 }
 void elas_launch_rtile(const ElasLaunch& L, int grid, hipStream_t s) {
     hipLaunchKernelGGL(k_rtile, dim3(grid), dim3(256), 0, s, L);

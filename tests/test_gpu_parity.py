@@ -194,6 +194,26 @@ def test_phase_b_gate_equals_ungated(pkg, pairs, synth):
             assert np.array_equal(a1, b1) and np.array_equal(a2, b2), (rep, l.shape, dm)
 
 
+def test_phase_b_gate_timeout_is_reported(pkg, oracle, pairs):
+    """The gate's bounded wait, forced short from the host: a call whose host
+    step outlasts it returns an error instead of hanging, and the next call
+    without the short bound is the oracle's again."""
+    left, right, _, dmax = pairs("450x375_d64")
+    ref = oracle.process(left, right, oracle.params(disp_max=dmax), dump=False)
+    el = pkg.Elas(pkg.default_params(disp_max=dmax))
+    el.set_autotune(False)  # the tiling, and so the shape's graph, stays put between calls
+    el.set_debug(False, short_gate=True)
+    D1, D2 = el.process(left, right)  # captures phase B's graph (not gated): the next call is
+    assert np.array_equal(D1, ref["D1"])
+    with pytest.raises(pkg.ElasError, match="gate timed out"):
+        el.process(left, right)
+    el.set_debug(False)
+    for _ in range(3):
+        D1, D2 = el.process(left, right)
+        assert el.status == 0
+        assert np.array_equal(D1, ref["D1"]) and np.array_equal(D2, ref["D2"])
+
+
 def test_sparse_texture_large_triangles(pkg, oracle, synth):
     """Texture only in five patches: twenty triangles per mesh, most spanning
     a large part of the image, so the raster's waves walk long column ranges
