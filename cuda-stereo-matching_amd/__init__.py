@@ -458,7 +458,7 @@ class Elas:
         overflow path, where a tile tests every triangle of the mesh); gate
         False launches phase B after the host Delaunay step instead of
         queueing it behind a gate during that step; short_gate bounds the
-        gate's wait at about 50 us (a test of its failure path)."""
+        gate's wait at one tick (a test of its failure path)."""
         self._check(self._lib.elas_hip_set_debug(
             self._ctx, (1 if on else 0) | (2 if tiny_lists else 0) | (0 if gate else 4) | (8 if short_gate else 0)))
 

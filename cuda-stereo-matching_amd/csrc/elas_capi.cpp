@@ -232,7 +232,7 @@ int elas_hip_set_debug(elas_ctx* c, int keep) {
     c->debug = (keep & 1) != 0;
     c->tiny_lists = (keep & 2) != 0;
     c->gate_b = (keep & 4) == 0;
-    c->gate_ticks = (keep & 8) ? 5000ull : 3000000000ull;
+    c->gate_ticks = (keep & 8) ? 1ull : 3000000000ull;
     c->param_gen++;
     return ELAS_OK;
 }

@@ -190,9 +190,9 @@ int elas_hip_set_pipeline(elas_ctx* ctx, int32_t slots, int32_t chunk);
  * launched after the host Delaunay step instead of being queued during it
  * behind a kernel that waits for the host's release); results are the same.
  * flags & 8 (a test of the gate's failure path) bounds the gate's wait at
- * about 50 us instead of 30 s, so a single-pair call whose host step takes
- * longer returns ELAS_ERR_HIP ("phase B gate timed out") with its maps
- * unspecified; the next call without the flag is unaffected. */
+ * one tick instead of 30 s, so a gated single-pair call returns ELAS_ERR_HIP
+ * ("phase B gate timed out") with its maps unspecified; the next call
+ * without the flag is unaffected. */
 int elas_hip_set_debug(elas_ctx* ctx, int flags);
 int elas_hip_debug_fetch(elas_ctx* ctx, int stage, void* dst, size_t bytes, size_t* stage_bytes);
 int elas_hip_shape_info(const elas_ctx* ctx, elas_shape_info* info);
