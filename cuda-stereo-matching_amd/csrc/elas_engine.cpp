@@ -551,15 +551,19 @@ int elas_ctx::slot_phase_a(ElasSlot& S) {
     auto g = [&](const ElasLaunch& L, int cap) { return std::min(L.total, cap); };
     HIPCHECK(hipEventRecord(S.ev[0], s1));
     S.graphed = false;
-    if (use_graphs) {
+    if (use_graphs && !S.gated) {
         hipGraphExec_t ex = nullptr;
         int r = slot_graph(S, 0, &ex);
         if (r != ELAS_OK) return r;
         HIPCHECK(hipGraphLaunch(ex, s1));
         S.graphed = true;
     } else {
-        int r = enqueue_a(S, true);
+        // a single-pair call launches phase A directly: a graph launch costs
+        // the host about 20 us before the first kernel can start, a launch
+        // about 5 us, and phase A's kernels mostly outlast a launch
+        int r = enqueue_a(S, !use_graphs);
         if (r != ELAS_OK) return r;
+        S.graphed = use_graphs;
     }
     HIPCHECK(hipEventRecord(S.ev[5], s1));
     // disparity grid on the second stream, overlapping the host Triangle
