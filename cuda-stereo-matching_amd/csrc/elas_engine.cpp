@@ -638,15 +638,11 @@ int elas_ctx::slot_prelaunch_b(ElasSlot& S) {
     }
     build_prefix(prm, S);
     // Nothing between the gate and its release may wait for the stream: no
-    // graph capture or instantiation (only a graph captured by an earlier
-    // call is launched here) and no copy to pageable memory (the result
-    // copies are queued after the release).
-    hipGraphExec_t ex = nullptr;
-    if (use_graphs) {
-        int r = slot_graph(S, 1, &ex, false);
-        if (r != ELAS_OK) return r;
-    }
-    if (!ex) return ELAS_OK;  // phase B is launched after the host step
+    // graph capture or instantiation and no copy to pageable memory (the
+    // result copies are queued after the release).  Phase B's kernels are
+    // launched directly behind the gate (their launch cost is hidden by the
+    // host step, and the device starts the first one sooner after the gate
+    // than a graph's).
     memcpy(S.hpairs[1], S.pairs.data(), sizeof(ElasPair) * n);
     memcpy(S.hprefix[1], S.prefix.data(), sizeof(int32_t) * S.prefix.size());
     __atomic_store_n(&S.gate[0], 0u, __ATOMIC_RELEASE);
@@ -660,7 +656,8 @@ int elas_ctx::slot_prelaunch_b(ElasSlot& S) {
                      (int)((bytes + 3) / 4), gate_ticks, s1);
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipEventRecord(S.ev[6], s1));
-    HIPCHECK(hipGraphLaunch(ex, s1));
+    int r = enqueue_b(S, !use_graphs);
+    if (r != ELAS_OK) return r;
     HIPCHECK(hipEventRecord(S.ev[10], s1));
     return ELAS_OK;
 }
@@ -807,7 +804,7 @@ int elas_ctx::enqueue_b(ElasSlot& S, bool timed) {
 // everything a launch depends on: pair shapes, where buffers live, the
 // parameter generation and the debug flag; the tables and pinned buffers
 // the graph reads are the slot's own, at fixed addresses.
-int elas_ctx::slot_graph(ElasSlot& S, int which, hipGraphExec_t* out, bool capture) {
+int elas_ctx::slot_graph(ElasSlot& S, int which, hipGraphExec_t* out) {
     std::vector<int64_t> key;
     key.reserve(6 + 5 * (size_t)S.n);
     key.push_back(S.n);
@@ -825,10 +822,6 @@ int elas_ctx::slot_graph(ElasSlot& S, int which, hipGraphExec_t* out, bool captu
     ElasGraphEntry* e = nullptr;
     for (auto& x : S.graphs)
         if (x.key == key) e = &x;
-    if (!capture && (!e || !e->exec[which])) {
-        *out = nullptr;  // lookup only: no capture, no instantiation
-        return ELAS_OK;
-    }
     if (!e) {
         if (S.graphs.size() >= ELAS_GRAPHS_PER_SLOT) {
             auto it = std::min_element(S.graphs.begin(), S.graphs.end(),

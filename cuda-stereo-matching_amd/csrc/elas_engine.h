@@ -185,7 +185,7 @@ struct elas_ctx {
     void slot_abort(ElasSlot& S);
     int enqueue_a(ElasSlot& S, bool timed);
     int enqueue_b(ElasSlot& S, bool timed);
-    int slot_graph(ElasSlot& S, int which, hipGraphExec_t* out, bool capture = true);
+    int slot_graph(ElasSlot& S, int which, hipGraphExec_t* out);
     void slot_drop_graphs(ElasSlot& S);
 };
 

@@ -176,8 +176,7 @@ def test_raster_list_overflow_path(pkg, oracle, pairs, name):
 def test_phase_b_gate_equals_ungated(pkg, pairs, synth):
     """Single-pair calls queue phase B behind a gate while the host
     triangulates; with the gate off phase B is launched afterwards.  Both give
-    the same maps, call after call (the gated graph is captured on the first
-    call and replayed behind the gate from the second), and across shapes."""
+    the same maps, call after call and across shapes."""
     shapes = [pairs("1242x375_d128")[:2] + (pairs("1242x375_d128")[3],)]
     shapes += [synth.make_pair(W, H, dm, regions=3, seed=900 + i)[:2] + (dm,)
                for i, (W, H, dm) in enumerate([(320, 240, 63), (200, 150, 31)])]
@@ -201,10 +200,9 @@ def test_phase_b_gate_timeout_is_reported(pkg, oracle, pairs):
     left, right, _, dmax = pairs("450x375_d64")
     ref = oracle.process(left, right, oracle.params(disp_max=dmax), dump=False)
     el = pkg.Elas(pkg.default_params(disp_max=dmax))
-    el.set_autotune(False)  # the tiling, and so the shape's graph, stays put between calls
-    el.set_debug(False, short_gate=True)
-    D1, D2 = el.process(left, right)  # captures phase B's graph (not gated): the next call is
+    D1, D2 = el.process(left, right)
     assert np.array_equal(D1, ref["D1"])
+    el.set_debug(False, short_gate=True)
     with pytest.raises(pkg.ElasError, match="gate timed out"):
         el.process(left, right)
     el.set_debug(False)
