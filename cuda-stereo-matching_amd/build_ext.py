@@ -7,6 +7,8 @@
                     (never copied), ELAS_TRI_INSTANCES times, each under its
                     own namespace elas_triK (-DTRILIB=elas_triK) so that each
                     host worker owns an independent, non-shared Triangle state;
+                    csrc/elas_tri_unit.cpp includes it (-I) and adds a driver
+                    that takes the divide-and-conquer vertex order as input;
 linked by hipcc into lib/libelas_hip.so (in-tree, loaded by ctypes), and
   * csrc/elas_cli.cpp, the command-line tool, linked against it as lib/elas_cli.
 Without the reference checkout an existing library is kept as it is.
@@ -57,8 +59,8 @@ def _cmds():
         # g++; both only change instruction selection, and the meshes are checked
         # against the oracle's generic g++ -O2 build of the same file.
         cmds.append([HOST_CLANG, "-O3", "-march=x86-64-v3", "-ffp-contract=off", "-fPIC", "-w",
-                     "-DTRILIB=elas_tri%d" % k, "-c",
-                     os.path.join(REF_SRC, "triangle.cpp"), "-o", o])
+                     "-DTRILIB=elas_tri%d" % k, "-I", REF_SRC, "-c",
+                     os.path.join(CSRC, "elas_tri_unit.cpp"), "-o", o])
         objs.append(o)
     return cmds, objs
 

@@ -4,8 +4,9 @@
 // (src/Elas.cu:282-328).  Triangle keeps its state in namespace-scope
 // globals (src/triangle.cpp:545-554, the RNG reset at :4034), so one instance
 // is not reentrant.  The build compiles the reference's triangle.cpp
-// unmodified ELAS_TRI_INSTANCES times with -DTRILIB=elas_triK: each copy gets
-// its own namespace and globals, and worker thread K of the pool below uses
+// unmodified ELAS_TRI_INSTANCES times with -DTRILIB=elas_triK, each inside
+// elas_tri_unit.cpp, which adds a driver for a precomputed vertex order: each
+// copy gets its own namespace and globals, and worker thread K of the pool below uses
 // only copy K.  Meshes are identical to a single serial Triangle because every
 // call starts from the same reset state.
 #include <algorithm>
@@ -72,6 +73,13 @@
 #include "triangle.h"
 #undef TRILIB
 
+#define ELAS_TRI_DECL(NS) \
+    namespace NS { int elas_triangulate_arranged(const float* pts, int n, const int* arr, int** trilist); }
+ELAS_TRI_DECL(elas_tri0) ELAS_TRI_DECL(elas_tri1) ELAS_TRI_DECL(elas_tri2) ELAS_TRI_DECL(elas_tri3)
+ELAS_TRI_DECL(elas_tri4) ELAS_TRI_DECL(elas_tri5) ELAS_TRI_DECL(elas_tri6) ELAS_TRI_DECL(elas_tri7)
+ELAS_TRI_DECL(elas_tri8) ELAS_TRI_DECL(elas_tri9) ELAS_TRI_DECL(elas_tri10) ELAS_TRI_DECL(elas_tri11)
+ELAS_TRI_DECL(elas_tri12) ELAS_TRI_DECL(elas_tri13) ELAS_TRI_DECL(elas_tri14) ELAS_TRI_DECL(elas_tri15)
+
 namespace {
 
 // "zQB": zero-based indices, quiet, no boundary markers (Elas.cu:315-317).
@@ -106,64 +114,123 @@ const TriFn kTri[ELAS_TRI_INSTANCES] = {
     run_elas_tri6, run_elas_tri7, run_elas_tri8, run_elas_tri9, run_elas_tri10, run_elas_tri11,
     run_elas_tri12, run_elas_tri13, run_elas_tri14, run_elas_tri15};
 
+typedef int (*ArrFn)(const float*, int, const int*, int**);
+const ArrFn kArr[ELAS_TRI_INSTANCES] = {
+    elas_tri0::elas_triangulate_arranged, elas_tri1::elas_triangulate_arranged,
+    elas_tri2::elas_triangulate_arranged, elas_tri3::elas_triangulate_arranged,
+    elas_tri4::elas_triangulate_arranged, elas_tri5::elas_triangulate_arranged,
+    elas_tri6::elas_triangulate_arranged, elas_tri7::elas_triangulate_arranged,
+    elas_tri8::elas_triangulate_arranged, elas_tri9::elas_triangulate_arranged,
+    elas_tri10::elas_triangulate_arranged, elas_tri11::elas_triangulate_arranged,
+    elas_tri12::elas_triangulate_arranged, elas_tri13::elas_triangulate_arranged,
+    elas_tri14::elas_triangulate_arranged, elas_tri15::elas_triangulate_arranged};
+
 double now_s() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
 }  // namespace
 
-// Triangle's mesh of a point set without duplicate points does not depend on
-// the order of the points: divconqdelaunay() first sorts its vertex pointers
-// by (x, y) (src/triangle.cpp:6176-6198) and from there on works on that
-// array and the mesh, and the output numbers each corner by its input
-// position (writenodes, src/triangle.cpp:7708-7768).  Its quicksort (src/triangle.cpp:
-// 5450-5505) runs about a quarter faster on sorted input, so a job hands
-// Triangle its points in (x, y) order and maps the corners back.  Support
-// points have integer coordinates, and those of one image arrive in row
-// order, so a counting sort on x that keeps the input order among equal x
-// yields (x, y) order.  The order is used only when the result is strictly
-// increasing in (x, y): a set with duplicates keeps the caller's order, as
-// which duplicate Triangle keeps depends on it.
-bool elas_tri_sorted_order(const float* p, int n, std::vector<int32_t>& order, std::vector<float>& sorted) {
-    if (n < 2) return false;
-    float xmin = p[0], xmax = p[0];
-    for (int i = 0; i < n; i++) {
-        const float x = p[2 * i];
-        if (!(std::fabs(x) < 16777216.0f) || x != std::floor(x)) return false;
-        xmin = std::min(xmin, x);
-        xmax = std::max(xmax, x);
-    }
-    const long range = (long)xmax - (long)xmin + 1;
-    if (range > 4L * n + 65536) return false;
-    thread_local std::vector<int32_t> cnt;
+namespace {
+
+// Stable counting sort of the indices in `in` by the integer coordinate c
+// (0 = x, 1 = y) of the points; lo and range bound that coordinate.
+void counting_sort(const float* p, int n, int c, int lo, long range, const int32_t* in, int32_t* out,
+                   std::vector<int32_t>& cnt) {
     cnt.assign((size_t)range + 1, 0);
-    const int x0 = (int)xmin;
-    for (int i = 0; i < n; i++) cnt[(int)p[2 * i] - x0 + 1]++;
+    for (int i = 0; i < n; i++) cnt[(int)p[2 * in[i] + c] - lo + 1]++;
     for (long b = 0; b < range; b++) cnt[b + 1] += cnt[b];
-    order.resize(n);
-    for (int i = 0; i < n; i++) order[cnt[(int)p[2 * i] - x0]++] = i;
-    sorted.resize(2 * (size_t)n);
-    for (int k = 0; k < n; k++) {
-        sorted[2 * k] = p[2 * order[k]];
-        sorted[2 * k + 1] = p[2 * order[k] + 1];
+    for (int i = 0; i < n; i++) out[cnt[(int)p[2 * in[i] + c] - lo]++] = in[i];
+}
+
+// Alternating cuts over a node [lo, lo + size): lx and ly hold the node's
+// points in (x, y) and (y, x) order.  A cut along `axis` keeps the size / 2
+// smallest points of that order on its left (as vertexmedian() does,
+// src/triangle.cpp:5517-5574) and splits the other list stably by side, so
+// both children keep both orders.  Nodes of 2 or 3 points are leaves sorted
+// by (x, y) (alternateaxes(), src/triangle.cpp:5586-5605).
+struct Cuts {
+    int32_t *lx, *ly, *tmp, *out;
+    uint8_t* side;
+    void node(int lo, int size, int axis) {
+        if (size <= 3) {
+            for (int k = 0; k < size; k++) out[lo + k] = lx[lo + k];
+            return;
+        }
+        const int divider = size >> 1;
+        int32_t* cut = axis ? ly : lx;
+        int32_t* other = axis ? lx : ly;
+        for (int k = 0; k < size; k++) side[cut[lo + k]] = k >= divider;
+        int a = 0, b = divider;
+        for (int k = 0; k < size; k++) {
+            const int32_t i = other[lo + k];
+            tmp[side[i] ? b++ : a++] = i;
+        }
+        memcpy(other + lo, tmp, sizeof(int32_t) * (size_t)size);
+        node(lo, divider, 1 - axis);
+        node(lo + divider, size - divider, 1 - axis);
     }
+};
+
+}  // namespace
+
+// Triangle's divide-and-conquer order (divconqdelaunay(), src/triangle.cpp:
+// 6164-6230): the points sorted by (x, y), halved, and each half cut
+// alternately along y and x down to 2- and 3-point leaves.  For distinct
+// points that order is unique, so it can be built here in O(n log n) from two
+// presorted lists instead of by Triangle's randomised quicksort and
+// quickselect, which take about half of a support-point mesh's time.  Support
+// points have integer coordinates, so both lists come from counting sorts.
+// False when a coordinate is not a bounded integer or two points coincide
+// (which duplicate Triangle keeps depends on its own sort); the caller then
+// runs Triangle unchanged.
+bool elas_tri_arrangement(const float* p, int n, std::vector<int32_t>& arr) {
+    if (n < 3) return false;
+    float lo[2] = {p[0], p[1]}, hi[2] = {p[0], p[1]};
+    for (int i = 0; i < n; i++) {
+        for (int c = 0; c < 2; c++) {
+            const float v = p[2 * i + c];
+            if (!(std::fabs(v) < 16777216.0f) || v != std::floor(v)) return false;
+            lo[c] = std::min(lo[c], v);
+            hi[c] = std::max(hi[c], v);
+        }
+    }
+    const long rx = (long)hi[0] - (long)lo[0] + 1, ry = (long)hi[1] - (long)lo[1] + 1;
+    if (rx > 4L * n + 65536 || ry > 4L * n + 65536) return false;
+    thread_local std::vector<int32_t> cnt, a, b, lx, ly, tmp;
+    thread_local std::vector<uint8_t> side;
+    a.resize(n);
+    b.resize(n);
+    lx.resize(n);
+    ly.resize(n);
+    for (int i = 0; i < n; i++) a[i] = i;
+    counting_sort(p, n, 1, (int)lo[1], ry, a.data(), b.data(), cnt);   // by y
+    counting_sort(p, n, 0, (int)lo[0], rx, b.data(), lx.data(), cnt);  // then x: (x, y)
+    counting_sort(p, n, 0, (int)lo[0], rx, a.data(), b.data(), cnt);   // by x
+    counting_sort(p, n, 1, (int)lo[1], ry, b.data(), ly.data(), cnt);  // then y: (y, x)
     for (int k = 1; k < n; k++) {
-        const float xa = sorted[2 * k - 2], ya = sorted[2 * k - 1], xb = sorted[2 * k], yb = sorted[2 * k + 1];
-        if (!(xa < xb || (xa == xb && ya < yb))) return false;
+        const int32_t i = lx[k - 1], j = lx[k];
+        if (p[2 * i] == p[2 * j] && p[2 * i + 1] == p[2 * j + 1]) return false;
     }
+    tmp.resize(n);
+    side.resize(n);
+    arr.resize(n);
+    Cuts cuts{lx.data(), ly.data(), tmp.data(), arr.data(), side.data()};
+    cuts.node(0, n, 0);
     return true;
 }
 
 namespace {
 
-// One "zQB" Triangle call on copy k: points handed over in (x, y) order where
-// that is safe, corners numbered by the caller's order.
+// One "zQB" Triangle call on copy k.  Corners are numbered by the caller's
+// order either way.
 int tri_call(int k, const float* pts, int n, std::vector<int32_t>& out) {
-    thread_local std::vector<int32_t> order;
-    thread_local std::vector<float> sorted;
-    if (!elas_tri_sorted_order(pts, n, order, sorted)) return kTri[k](pts, n, out);
-    const int nt = kTri[k](sorted.data(), n, out);
-    for (auto& c : out) c = order[c];
+    thread_local std::vector<int32_t> arr;
+    if (!elas_tri_arrangement(pts, n, arr)) return kTri[k](pts, n, out);
+    int* tl = nullptr;
+    const int nt = kArr[k](pts, n, arr.data(), &tl);
+    out.assign(tl, tl + 3 * (size_t)nt);
+    free(tl);
     return nt;
 }
 

@@ -14,10 +14,10 @@ struct ElasTriJob {
     double t_begin = 0, t_end = 0;
 };
 
-// The (x, y) order of a point set when it can be handed to Triangle in place
-// of the caller's order with the same mesh: integer x, no duplicates.  order
-// maps sorted position to input index.  False (order unused) otherwise.
-bool elas_tri_sorted_order(const float* p, int n, std::vector<int32_t>& order, std::vector<float>& sorted);
+// The vertex order Triangle's divide-and-conquer builds for a point set
+// (input indices), when the coordinates are bounded integers and no two points
+// coincide; false (arr unused) otherwise.
+bool elas_tri_arrangement(const float* p, int n, std::vector<int32_t>& arr);
 
 // Runs every job (each a "zQB" Triangle call) on up to max_workers of the
 // process-wide Triangle workers; returns when all are done.

@@ -1,5 +1,8 @@
-"""Host Delaunay step: the library hands Triangle the support points in
-(x, y) order when that leaves the mesh unchanged, and maps the corners back.
+"""Host Delaunay step: for integer points without duplicates the library
+builds Triangle's divide-and-conquer vertex order itself (two counting sorts
+and alternating median cuts) and runs Triangle's own merge from there; other
+sets go through Triangle unchanged.  The meshes, triangle numbering included,
+must be those of Triangle on the caller's order.
 
 Checked against the oracle's own build of the reference's triangle.cpp, which
 always triangulates the points in the caller's (row-major) order, as the
@@ -63,6 +66,41 @@ def test_non_integer_points_keep_caller_order(pkg, oracle):
 
 def test_pipeline_meshes_of_c0(pkg, oracle, pairs):
     left, right, _, dmax = pairs("450x375_d64")
+    ref = oracle.process(left, right, oracle.params(disp_max=dmax))
+    assert np.array_equal(pkg.triangulate(ref["sp0"]), ref["tri0"])
+    assert np.array_equal(pkg.triangulate(ref["sp1"]), ref["tri1"])
+
+
+@pytest.mark.parametrize("n", [3, 4, 5, 6, 7, 8, 9, 12, 13, 31, 64, 65, 257, 1000, 4001])
+def test_random_integer_sets_every_size(pkg, oracle, n):
+    """Sizes that exercise every leaf shape (2 and 3 points) and odd halves of
+    the alternating cuts; coordinates from a small range, so many points share
+    an x or a y and the cuts' tie order (other axis) decides."""
+    rng = np.random.default_rng(n)
+    side = max(4, int(np.sqrt(n) * 2))
+    flat = rng.choice(side * side, n, replace=False)
+    pts = np.stack([flat % side, flat // side], 1).astype(np.float32) * 3 - 7
+    assert np.array_equal(pkg.triangulate(pts), oracle.triangulate(pts))
+
+
+def test_full_lattice_cocircular(pkg, oracle):
+    """Every 4 neighbours of a full lattice are cocircular: the mesh depends
+    on the exact merge order, which the vertex order fixes."""
+    cu, cv = np.meshgrid(np.arange(37), np.arange(23))
+    pts = np.stack([cu.ravel(), cv.ravel()], 1).astype(np.float32) * 5
+    assert np.array_equal(pkg.triangulate(pts), oracle.triangulate(pts))
+
+
+def test_collinear_points(pkg, oracle):
+    for pts in (np.stack([np.arange(9) * 5, np.zeros(9)], 1), np.stack([np.zeros(7), np.arange(7) * 2], 1),
+                np.stack([np.arange(6), np.arange(6)], 1)):
+        pts = pts.astype(np.float32)
+        assert np.array_equal(pkg.triangulate(pts), oracle.triangulate(pts))
+
+
+def test_pipeline_meshes_of_c1(pkg, oracle, pairs):
+    """The 1242x375 pair's support points (about 1,400 per image)."""
+    left, right, _, dmax = pairs("1242x375_d128")
     ref = oracle.process(left, right, oracle.params(disp_max=dmax))
     assert np.array_equal(pkg.triangulate(ref["sp0"]), ref["tri0"])
     assert np.array_equal(pkg.triangulate(ref["sp1"]), ref["tri1"])
