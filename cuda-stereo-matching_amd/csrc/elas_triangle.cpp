@@ -133,14 +133,14 @@ double now_s() {
 
 namespace {
 
-// Stable counting sort of the indices in `in` by the integer coordinate c
-// (0 = x, 1 = y) of the points; lo and range bound that coordinate.
-void counting_sort(const float* p, int n, int c, int lo, long range, const int32_t* in, int32_t* out,
+// Stable counting sort of the indices in `in` by the integer keys key[]
+// (already offset to 0..range-1).
+void counting_sort(const int32_t* key, int n, long range, const int32_t* in, int32_t* out,
                    std::vector<int32_t>& cnt) {
     cnt.assign((size_t)range + 1, 0);
-    for (int i = 0; i < n; i++) cnt[(int)p[2 * in[i] + c] - lo + 1]++;
+    for (int i = 0; i < n; i++) cnt[key[in[i]] + 1]++;
     for (long b = 0; b < range; b++) cnt[b + 1] += cnt[b];
-    for (int i = 0; i < n; i++) out[cnt[(int)p[2 * in[i] + c] - lo]++] = in[i];
+    for (int i = 0; i < n; i++) out[cnt[key[in[i]]]++] = in[i];
 }
 
 // Alternating cuts over a node [lo, lo + size): lx and ly hold the node's
@@ -160,11 +160,16 @@ struct Cuts {
         const int divider = size >> 1;
         int32_t* cut = axis ? ly : lx;
         int32_t* other = axis ? lx : ly;
-        for (int k = 0; k < size; k++) side[cut[lo + k]] = k >= divider;
+        for (int k = 0; k < divider; k++) side[cut[lo + k]] = 0;
+        for (int k = divider; k < size; k++) side[cut[lo + k]] = 1;
+        // branch-free stable split: the side of a point is not predictable
         int a = 0, b = divider;
         for (int k = 0; k < size; k++) {
             const int32_t i = other[lo + k];
-            tmp[side[i] ? b++ : a++] = i;
+            const int r = side[i];
+            tmp[r ? b : a] = i;
+            a += 1 - r;
+            b += r;
         }
         memcpy(other + lo, tmp, sizeof(int32_t) * (size_t)size);
         node(lo, divider, 1 - axis);
@@ -186,31 +191,39 @@ struct Cuts {
 // runs Triangle unchanged.
 bool elas_tri_arrangement(const float* p, int n, std::vector<int32_t>& arr) {
     if (n < 3) return false;
-    float lo[2] = {p[0], p[1]}, hi[2] = {p[0], p[1]};
-    for (int i = 0; i < n; i++) {
-        for (int c = 0; c < 2; c++) {
+    thread_local std::vector<int32_t> cnt, a, b, lx, ly, tmp, key[2];
+    thread_local std::vector<uint8_t> side;
+    int lo[2] = {0, 0}, hi[2] = {0, 0};
+    for (int c = 0; c < 2; c++) {
+        key[c].resize(n);
+        int32_t* kc = key[c].data();
+        lo[c] = hi[c] = 0;
+        for (int i = 0; i < n; i++) {
             const float v = p[2 * i + c];
-            if (!(std::fabs(v) < 16777216.0f) || v != std::floor(v)) return false;
-            lo[c] = std::min(lo[c], v);
-            hi[c] = std::max(hi[c], v);
+            if (!(std::fabs(v) < 16777216.0f)) return false;
+            const int iv = (int)v;
+            if ((float)iv != v) return false;
+            kc[i] = iv;
+            if (i == 0 || iv < lo[c]) lo[c] = iv;
+            if (i == 0 || iv > hi[c]) hi[c] = iv;
         }
     }
-    const long rx = (long)hi[0] - (long)lo[0] + 1, ry = (long)hi[1] - (long)lo[1] + 1;
+    const long rx = (long)hi[0] - lo[0] + 1, ry = (long)hi[1] - lo[1] + 1;
     if (rx > 4L * n + 65536 || ry > 4L * n + 65536) return false;
-    thread_local std::vector<int32_t> cnt, a, b, lx, ly, tmp;
-    thread_local std::vector<uint8_t> side;
+    for (int c = 0; c < 2; c++)
+        for (int i = 0; i < n; i++) key[c][i] -= lo[c];
     a.resize(n);
     b.resize(n);
     lx.resize(n);
     ly.resize(n);
     for (int i = 0; i < n; i++) a[i] = i;
-    counting_sort(p, n, 1, (int)lo[1], ry, a.data(), b.data(), cnt);   // by y
-    counting_sort(p, n, 0, (int)lo[0], rx, b.data(), lx.data(), cnt);  // then x: (x, y)
-    counting_sort(p, n, 0, (int)lo[0], rx, a.data(), b.data(), cnt);   // by x
-    counting_sort(p, n, 1, (int)lo[1], ry, b.data(), ly.data(), cnt);  // then y: (y, x)
+    counting_sort(key[1].data(), n, ry, a.data(), b.data(), cnt);   // by y
+    counting_sort(key[0].data(), n, rx, b.data(), lx.data(), cnt);  // then x: (x, y)
+    counting_sort(key[0].data(), n, rx, a.data(), b.data(), cnt);   // by x
+    counting_sort(key[1].data(), n, ry, b.data(), ly.data(), cnt);  // then y: (y, x)
     for (int k = 1; k < n; k++) {
         const int32_t i = lx[k - 1], j = lx[k];
-        if (p[2 * i] == p[2 * j] && p[2 * i + 1] == p[2 * j + 1]) return false;
+        if (key[0][i] == key[0][j] && key[1][i] == key[1][j]) return false;
     }
     tmp.resize(n);
     side.resize(n);
