@@ -8,7 +8,9 @@
 // (divconqrecurse, removeghosts).  The one step it does not take from
 // Triangle is the vertex order that divconqdelaunay() builds with
 // vertexsort() and alternateaxes() (src/triangle.cpp:6164-6230); the caller
-// passes that order in.  For distinct points the order is unique: both
+// passes that order in, or (elas_triangulate_dedup) Triangle's own vertexsort()
+// and duplicate removal run first and only the alternating cuts are built
+// outside.  For distinct points the order is unique: both
 // routines compare by (axis, other axis), a strict total order, each cut
 // keeps the `divider` smallest points on its left, and every 2- or 3-point
 // leaf ends sorted by (x, y).  Their random pivots only change how fast the
@@ -18,12 +20,11 @@
 
 namespace TRILIB {
 
-// pts: [n][2], n >= 3, distinct; arr: the divide-and-conquer vertex order as
-// input indices.  *trilist receives a malloc'ed [ntri][3] zero-based list
-// (free() it); returns ntri.
-int elas_triangulate_arranged(const float* pts, int n, const int* arr, int** trilist) {
-    struct mesh m;
-    struct behavior b;
+namespace {
+
+// triangulate()'s steps up to the divide-and-conquer (src/triangle.cpp:
+// 8503-8530 and delaunay(), :6237-6250); byinput[i] is input point i's vertex.
+vertex* elas_begin(struct mesh& m, struct behavior& b, const float* pts, int n) {
     char sw[] = "zQB";
     char* swp = sw;
     triangleinit(&m);
@@ -33,18 +34,21 @@ int elas_triangulate_arranged(const float* pts, int n, const int* arr, int** tri
     m.eextras = 0;
     initializetrisubpools(&m, &b);
     vertex* byinput = (vertex*)trimalloc(n * (int)sizeof(vertex));
-    vertex* sortarray = (vertex*)trimalloc(n * (int)sizeof(vertex));
     traversalinit(&m.vertices);
     for (int i = 0; i < n; i++) byinput[i] = vertextraverse(&m);
-    for (int k = 0; k < n; k++) sortarray[k] = byinput[arr[k]];
-    trifree((int*)byinput);
+    return byinput;
+}
+
+// The merge on the arranged vertices (divconqdelaunay(), :6222-6230), then
+// the element list.  triangulate() numbers the vertices in writenodes();
+// without the point list numbernodes() gives the same numbers (no -j: every
+// vertex counts).
+int elas_finish(struct mesh& m, struct behavior& b, vertex* sortarray, int count, int** trilist) {
     struct otri hullleft, hullright;
-    divconqrecurse(&m, &b, sortarray, n, 0, &hullleft, &hullright);
+    divconqrecurse(&m, &b, sortarray, count, 0, &hullleft, &hullright);
     trifree((int*)sortarray);
     const long hull = removeghosts(&m, &b, &hullleft);
     m.hullsize = m.triangles.items == 0 ? 0l : hull;
-    // triangulate() numbers the vertices in writenodes(); without the point
-    // list numbernodes() gives the same numbers (no -j: every vertex counts).
     numbernodes(&m, &b);
     float* attr = (float*)NULL;
     *trilist = (int*)NULL;
@@ -52,6 +56,59 @@ int elas_triangulate_arranged(const float* pts, int n, const int* arr, int** tri
     const int nt = (int)m.triangles.items;
     triangledeinit(&m, &b);
     return nt;
+}
+
+}  // namespace
+
+// pts: [n][2], n >= 3, distinct; arr: the divide-and-conquer vertex order as
+// input indices.  *trilist receives a malloc'ed [ntri][3] zero-based list
+// (free() it); returns ntri.
+int elas_triangulate_arranged(const float* pts, int n, const int* arr, int** trilist) {
+    struct mesh m;
+    struct behavior b;
+    vertex* byinput = elas_begin(m, b, pts, n);
+    vertex* sortarray = (vertex*)trimalloc(n * (int)sizeof(vertex));
+    for (int k = 0; k < n; k++) sortarray[k] = byinput[arr[k]];
+    trifree((int*)byinput);
+    return elas_finish(m, b, sortarray, n, trilist);
+}
+
+// Any point set (n >= 3): Triangle's own vertexsort() on the caller's order
+// and its duplicate removal (:6176-6198), which decide which copy of a
+// repeated point survives; then the alternating-cut order of the survivors
+// from `arrange`, which gets their coordinates in (x, y) order and returns
+// positions into that list.
+int elas_triangulate_dedup(const float* pts, int n, bool (*arrange)(const float* xy, int count, int* arr),
+                           int** trilist) {
+    struct mesh mesh;
+    struct mesh* m = &mesh;  // Triangle's vertex macros name the mesh `m`
+    struct behavior b;
+    vertex* sorted = elas_begin(mesh, b, pts, n);
+    vertexsort(sorted, n);
+    int i = 0;
+    for (int j = 1; j < n; j++) {
+        if (sorted[i][0] == sorted[j][0] && sorted[i][1] == sorted[j][1]) {
+            setvertextype(sorted[j], UNDEADVERTEX);
+            m->undeads++;
+        } else {
+            i++;
+            sorted[i] = sorted[j];
+        }
+    }
+    const int count = i + 1;
+    float* xy = (float*)trimalloc(2 * count * (int)sizeof(float));
+    int* arr = (int*)trimalloc(count * (int)sizeof(int));
+    for (int k = 0; k < count; k++) {
+        xy[2 * k] = sorted[k][0];
+        xy[2 * k + 1] = sorted[k][1];
+    }
+    arrange(xy, count, arr);
+    vertex* sortarray = (vertex*)trimalloc(count * (int)sizeof(vertex));
+    for (int k = 0; k < count; k++) sortarray[k] = sorted[arr[k]];
+    trifree((int*)xy);
+    trifree(arr);
+    trifree((int*)sorted);
+    return elas_finish(mesh, b, sortarray, count, trilist);
 }
 
 }  // namespace TRILIB

@@ -73,8 +73,12 @@
 #include "triangle.h"
 #undef TRILIB
 
-#define ELAS_TRI_DECL(NS) \
-    namespace NS { int elas_triangulate_arranged(const float* pts, int n, const int* arr, int** trilist); }
+#define ELAS_TRI_DECL(NS)                                                                          \
+    namespace NS {                                                                                 \
+    int elas_triangulate_arranged(const float* pts, int n, const int* arr, int** trilist);         \
+    int elas_triangulate_dedup(const float* pts, int n, bool (*arrange)(const float*, int, int*), \
+                               int** trilist);                                                     \
+    }
 ELAS_TRI_DECL(elas_tri0) ELAS_TRI_DECL(elas_tri1) ELAS_TRI_DECL(elas_tri2) ELAS_TRI_DECL(elas_tri3)
 ELAS_TRI_DECL(elas_tri4) ELAS_TRI_DECL(elas_tri5) ELAS_TRI_DECL(elas_tri6) ELAS_TRI_DECL(elas_tri7)
 ELAS_TRI_DECL(elas_tri8) ELAS_TRI_DECL(elas_tri9) ELAS_TRI_DECL(elas_tri10) ELAS_TRI_DECL(elas_tri11)
@@ -82,37 +86,16 @@ ELAS_TRI_DECL(elas_tri12) ELAS_TRI_DECL(elas_tri13) ELAS_TRI_DECL(elas_tri14) EL
 
 namespace {
 
-// "zQB": zero-based indices, quiet, no boundary markers (Elas.cu:315-317).
-#define ELAS_TRI_RUN(NS)                                                        \
-    int run_##NS(const float* pts, int n, std::vector<int32_t>& out) {          \
-        NS::triangulateio in, o;                                                \
-        memset(&in, 0, sizeof(in));                                             \
-        memset(&o, 0, sizeof(o));                                               \
-        float* copy = (float*)malloc(sizeof(float) * 2 * (size_t)n);            \
-        if (!copy) return -1;                                                   \
-        memcpy(copy, pts, sizeof(float) * 2 * (size_t)n);                       \
-        in.numberofpoints = n;                                                  \
-        in.pointlist = copy;                                                    \
-        char sw[] = "zQB";                                                      \
-        NS::triangulate(sw, &in, &o, nullptr);                                  \
-        const int nt = o.numberoftriangles;                                     \
-        out.assign(o.trianglelist, o.trianglelist + 3 * (size_t)nt);            \
-        free(copy);                                                             \
-        free(o.pointlist);                                                      \
-        free(o.trianglelist);                                                   \
-        return nt;                                                              \
-    }
-
-ELAS_TRI_RUN(elas_tri0) ELAS_TRI_RUN(elas_tri1) ELAS_TRI_RUN(elas_tri2) ELAS_TRI_RUN(elas_tri3)
-ELAS_TRI_RUN(elas_tri4) ELAS_TRI_RUN(elas_tri5) ELAS_TRI_RUN(elas_tri6) ELAS_TRI_RUN(elas_tri7)
-ELAS_TRI_RUN(elas_tri8) ELAS_TRI_RUN(elas_tri9) ELAS_TRI_RUN(elas_tri10) ELAS_TRI_RUN(elas_tri11)
-ELAS_TRI_RUN(elas_tri12) ELAS_TRI_RUN(elas_tri13) ELAS_TRI_RUN(elas_tri14) ELAS_TRI_RUN(elas_tri15)
-
-typedef int (*TriFn)(const float*, int, std::vector<int32_t>&);
-const TriFn kTri[ELAS_TRI_INSTANCES] = {
-    run_elas_tri0, run_elas_tri1, run_elas_tri2, run_elas_tri3, run_elas_tri4, run_elas_tri5,
-    run_elas_tri6, run_elas_tri7, run_elas_tri8, run_elas_tri9, run_elas_tri10, run_elas_tri11,
-    run_elas_tri12, run_elas_tri13, run_elas_tri14, run_elas_tri15};
+typedef int (*DedupFn)(const float*, int, bool (*)(const float*, int, int*), int**);
+const DedupFn kDedup[ELAS_TRI_INSTANCES] = {
+    elas_tri0::elas_triangulate_dedup, elas_tri1::elas_triangulate_dedup,
+    elas_tri2::elas_triangulate_dedup, elas_tri3::elas_triangulate_dedup,
+    elas_tri4::elas_triangulate_dedup, elas_tri5::elas_triangulate_dedup,
+    elas_tri6::elas_triangulate_dedup, elas_tri7::elas_triangulate_dedup,
+    elas_tri8::elas_triangulate_dedup, elas_tri9::elas_triangulate_dedup,
+    elas_tri10::elas_triangulate_dedup, elas_tri11::elas_triangulate_dedup,
+    elas_tri12::elas_triangulate_dedup, elas_tri13::elas_triangulate_dedup,
+    elas_tri14::elas_triangulate_dedup, elas_tri15::elas_triangulate_dedup};
 
 typedef int (*ArrFn)(const float*, int, const int*, int**);
 const ArrFn kArr[ELAS_TRI_INSTANCES] = {
@@ -188,7 +171,7 @@ struct Cuts {
 // points have integer coordinates, so both lists come from counting sorts.
 // False when a coordinate is not a bounded integer or two points coincide
 // (which duplicate Triangle keeps depends on its own sort); the caller then
-// runs Triangle unchanged.
+// lets Triangle sort and drop duplicates itself and builds only the cuts.
 bool elas_tri_arrangement(const float* p, int n, std::vector<int32_t>& arr) {
     if (n < 3) return false;
     thread_local std::vector<int32_t> cnt, a, b, lx, ly, tmp, key[2];
@@ -237,11 +220,29 @@ namespace {
 
 // One "zQB" Triangle call on copy k.  Corners are numbered by the caller's
 // order either way.
+// The alternating cuts of distinct points given in (x, y) order: positions
+// into xy.  Used after Triangle's own sort and duplicate removal, for sets the
+// counting sorts do not take (duplicates, non-integer coordinates).
+bool arrange_sorted(const float* xy, int count, int* arr) {
+    thread_local std::vector<int32_t> lx, ly, tmp;
+    thread_local std::vector<uint8_t> side;
+    lx.resize(count);
+    ly.resize(count);
+    tmp.resize(count);
+    side.resize(count);
+    for (int i = 0; i < count; i++) lx[i] = ly[i] = i;
+    // (y, x) order: by y, the (x, y) order kept among equal y
+    std::stable_sort(ly.begin(), ly.end(), [xy](int32_t a, int32_t b) { return xy[2 * a + 1] < xy[2 * b + 1]; });
+    Cuts cuts{lx.data(), ly.data(), tmp.data(), arr, side.data()};
+    cuts.node(0, count, 0);
+    return true;
+}
+
 int tri_call(int k, const float* pts, int n, std::vector<int32_t>& out) {
     thread_local std::vector<int32_t> arr;
-    if (!elas_tri_arrangement(pts, n, arr)) return kTri[k](pts, n, out);
     int* tl = nullptr;
-    const int nt = kArr[k](pts, n, arr.data(), &tl);
+    const int nt = elas_tri_arrangement(pts, n, arr) ? kArr[k](pts, n, arr.data(), &tl)
+                                                     : kDedup[k](pts, n, arrange_sorted, &tl);
     out.assign(tl, tl + 3 * (size_t)nt);
     free(tl);
     return nt;

@@ -106,6 +106,20 @@ def test_pipeline_meshes_of_c1(pkg, oracle, pairs):
     assert np.array_equal(pkg.triangulate(ref["sp1"]), ref["tri1"])
 
 
+@pytest.mark.parametrize("seed", range(4))
+def test_heavy_duplicates(pkg, oracle, seed):
+    """Every point repeated up to 4 times in shuffled order, integer and
+    non-integer: Triangle's own sort decides which copy survives (the others
+    are dropped), and only the cuts of the survivors are built outside."""
+    rng = np.random.default_rng(500 + seed)
+    base = rng.integers(0, 60, (300, 2)).astype(np.float32)
+    if seed % 2:
+        base = base * 0.37 + 0.11
+    pts = np.repeat(base, rng.integers(1, 5, len(base)), axis=0)
+    pts = pts[rng.permutation(len(pts))]
+    assert np.array_equal(pkg.triangulate(pts), oracle.triangulate(pts))
+
+
 def test_too_few_points(pkg):
     with pytest.raises(pkg.ElasError) as e:
         pkg.triangulate(np.zeros((2, 2), np.float32))
