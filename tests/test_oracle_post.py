@@ -109,6 +109,33 @@ def test_adaptive_mean_invariants(oracle):
     assert got[R >= 0].min() >= R[R >= 0].min() and got.max() <= R.max()
 
 
+def test_adaptive_mean_hand_cases(oracle):
+    """Hand-computed values that fix the window radius (3), the weight
+    max(0, 4 - |d_k - d_c|) and the order (rows, then columns)."""
+    # one row: the centre (index 4, d = 0) sees index 1 (d = 2, w = 2), itself
+    # (w = 4) and index 7 (d = 3, w = 1); the 60s weigh nothing and indices 0
+    # and 8 (d = 3) lie outside the radius: (2*2 + 0*4 + 3*1) / (2 + 4 + 1) = 1
+    row = np.array([[3, 2, 60, 60, 0, 60, 60, 3, 3]], np.float32)
+    got = oracle.adaptive_mean(row)
+    assert got[0, 4] == 1.0
+    # index 7 (d = 3): index 4 (d = 0, w = 1), itself (4), index 8 (d = 3, w = 4);
+    # index 10 does not exist: (0*1 + 3*4 + 3*4) / 9
+    assert abs(got[0, 7] - 24.0 / 9.0) < 1e-6
+    # the same as a column
+    col = oracle.adaptive_mean(row.T.copy())
+    assert np.array_equal(col[:, 0], got[0])
+    # 2 x 2: rows first -> [[3/7, 4/7], [2, 6]]; then column 0 (3/7 and 2,
+    # |delta| = 11/7, w = 17/7): top (4*3/7 + 17/7*2) / (4 + 17/7) = 46/45,
+    # bottom (4*2 + 17/7*3/7) / (45/7) = 443/315; column 1 differs by 38/7 > 4
+    D = oracle.adaptive_mean(np.array([[0, 1], [2, 6]], np.float32))
+    exp = np.array([[46 / 45, 4 / 7], [443 / 315, 6]])
+    assert np.allclose(D, exp, rtol=0, atol=2e-6), D
+    # invalid neighbours are skipped, not counted as d = -1
+    D = oracle.adaptive_mean(np.array([[1, -1, 2]], np.float32))
+    # index 0: itself (4) and index 2 (|1| -> w = 3): (4 + 6) / 7
+    assert abs(D[0, 0] - 10 / 7) < 1e-6 and D[0, 1] == -1
+
+
 def test_median_equals_scipy_on_valid_interior(oracle):
     rng = np.random.default_rng(5)
     D = rng.integers(0, 100, (40, 50)).astype(np.float32)
