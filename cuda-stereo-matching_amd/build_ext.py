@@ -11,7 +11,9 @@
                     that takes the divide-and-conquer vertex order as input;
 linked by hipcc into lib/libelas_hip.so (in-tree, loaded by ctypes), and
   * csrc/elas_cli.cpp, the command-line tool, linked against it as lib/elas_cli.
-Without the reference checkout an existing library is kept as it is.
+Without the reference checkout (which holds Triangle) an existing library is
+kept only while it is newer than every source it is built from; otherwise the
+build fails rather than hand back a stale binary.
 """
 import concurrent.futures as cf
 import os
@@ -70,12 +72,24 @@ def _run(cmd):
     return cmd, r.returncode, r.stdout + r.stderr
 
 
+def _newest_source():
+    srcs = [os.path.join(CSRC, f) for f in os.listdir(CSRC)]
+    inc = os.path.join(ROOT, "include")
+    for d, _, fs in os.walk(inc):
+        srcs += [os.path.join(d, f) for f in fs]
+    srcs.append(os.path.abspath(__file__))
+    return max(os.path.getmtime(s) for s in srcs if os.path.isfile(s))
+
+
 def build(verbose=False, jobs=None):
     """Compile and link; returns the library path."""
     if not os.path.isfile(os.path.join(REF_SRC, "triangle.cpp")):
-        if os.path.isfile(LIB):
-            return LIB
-        raise RuntimeError("reference checkout missing and no prebuilt library at " + LIB)
+        if not os.path.isfile(LIB):
+            raise RuntimeError("reference checkout missing and no prebuilt library at " + LIB)
+        if os.path.getmtime(LIB) < _newest_source():
+            raise RuntimeError("sources changed after %s was built, and it cannot be rebuilt without the "
+                               "reference checkout (%s holds Triangle)" % (LIB, REF_SRC))
+        return LIB
     os.makedirs(BUILD, exist_ok=True)
     os.makedirs(LIBDIR, exist_ok=True)
     cmds, objs = _cmds()

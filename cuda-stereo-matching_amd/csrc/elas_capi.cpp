@@ -360,7 +360,7 @@ int elas_hip_triangulate(const float* pts, int32_t n, int32_t* tri, int64_t cap,
     elas_triangulate_jobs(&job, 1, 1);
     if (job.result < 0) {
         *ntri = 0;
-        return n < 3 ? ELAS_ERR_TOO_FEW_POINTS : ELAS_ERR_OUT_OF_MEMORY;
+        return ELAS_ERR_TOO_FEW_POINTS;  // n < 3, or every point the same point
     }
     *ntri = job.result;
     memcpy(tri, out.data(), sizeof(int32_t) * 3 * (size_t)std::min<int64_t>(cap, job.result));

@@ -261,7 +261,7 @@ int elas_ctx::slot_need(const ElasPairSpec* sp, int n, size_t* arena_need, size_
         ptri += 2 * 12 * (size_t)d.tricap;
     }
     *arena_need = a;
-    *pin_need = 2 * table_one(n) + al(psp) + al(ptri);
+    *pin_need = 3 * table_one(n) + al(psp) + al(ptri);
     return ELAS_OK;
 }
 
@@ -371,8 +371,10 @@ int elas_ctx::slot_prepare(ElasSlot& S, const ElasPairSpec* sp, int n) {
     S.hnsp = (int32_t*)(PIN + S.tab_pairs + S.tab_prefix);
     S.hpairs[1] = (ElasPair*)(PIN + S.tab_one);
     S.hprefix[1] = (int32_t*)(PIN + S.tab_one + S.tab_pairs);
-    S.hsp = (float*)(PIN + 2 * S.tab_one);
-    S.htri = (int32_t*)(PIN + 2 * S.tab_one + al(psp));
+    S.hpairs[2] = (ElasPair*)(PIN + 2 * S.tab_one);
+    S.hprefix[2] = (int32_t*)(PIN + 2 * S.tab_one + S.tab_pairs);
+    S.hsp = (float*)(PIN + 3 * S.tab_one);
+    S.htri = (int32_t*)(PIN + 3 * S.tab_one + al(psp));
 
     S.pairs.assign(n, ElasPair());
     S.dense_lds[0] = S.dense_lds[1] = 0;
@@ -477,7 +479,7 @@ ElasLaunch launch_of(const ElasSlot& S, int table, int st) {
     L.pairs = S.dpairs[table];
     L.prefix = S.dprefix[table] + (size_t)st * (S.n + 1);
     L.npairs = S.n;
-    L.total = S.hprefix[table][(size_t)st * (S.n + 1) + S.n];
+    L.total = S.hostpre[table][(size_t)st * (S.n + 1) + S.n];
     return L;
 }
 
@@ -531,6 +533,7 @@ int elas_ctx::slot_phase_a(ElasSlot& S) {
     build_prefix(prm, S);
     memcpy(S.hpairs[0], S.pairs.data(), sizeof(ElasPair) * n);
     memcpy(S.hprefix[0], S.prefix.data(), sizeof(int32_t) * S.prefix.size());
+    S.hostpre[0] = S.prefix;
     hipStream_t s1 = S.stream, s2 = S.stream2;
     // pair records and stage prefixes lie back to back in both tables: one
     // copy, skipped when the device table already holds these bytes (calls
@@ -645,6 +648,7 @@ int elas_ctx::slot_prelaunch_b(ElasSlot& S) {
     // than a graph's).
     memcpy(S.hpairs[1], S.pairs.data(), sizeof(ElasPair) * n);
     memcpy(S.hprefix[1], S.prefix.data(), sizeof(int32_t) * S.prefix.size());
+    S.hostpre[1] = S.prefix;
     __atomic_store_n(&S.gate[0], 0u, __ATOMIC_RELEASE);
     S.gate[1] = 0u;
     hipStream_t s1 = S.stream;
@@ -652,8 +656,12 @@ int elas_ctx::slot_prelaunch_b(ElasSlot& S) {
     const size_t bytes = S.tab_pairs + sizeof(int32_t) * S.prefix.size();
     S.gate_armed = true;
     S.gate_used = true;
-    elas_launch_gate(S.gate_dev, S.gate_dev + 1, (const uint32_t*)hdev(S.hpairs[1]), (uint32_t*)S.dpairs[1],
-                     (int)((bytes + 3) / 4), gate_ticks, s1);
+    // released: the real table (hpairs[2], written by slot_phase_b before the
+    // flag); timed out: this placeholder (hpairs[1], not written again while
+    // the gate is armed, and with no triangles, so phase B reads no mesh)
+    elas_launch_gate(S.gate_dev, S.gate_dev + 1, (const uint32_t*)hdev(S.hpairs[2]),
+                     (const uint32_t*)hdev(S.hpairs[1]), (uint32_t*)S.dpairs[1], (int)((bytes + 3) / 4), gate_ticks,
+                     s1);
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipEventRecord(S.ev[6], s1));
     int r = enqueue_b(S, !use_graphs);
@@ -699,8 +707,12 @@ int elas_ctx::slot_phase_b(ElasSlot& S) {
         S.pairs[i].tri[1] = S.pairs[i].tri[0] + 3 * (size_t)nt[0];
     }
     build_prefix(prm, S);
-    memcpy(S.hpairs[1], S.pairs.data(), sizeof(ElasPair) * n);
-    memcpy(S.hprefix[1], S.prefix.data(), sizeof(int32_t) * S.prefix.size());
+    // behind an armed gate the real table goes to its own pinned copy: a gate
+    // that timed out may be copying the placeholder in hpairs[1] right now
+    const int tab = S.gate_armed ? 2 : 1;
+    memcpy(S.hpairs[tab], S.pairs.data(), sizeof(ElasPair) * n);
+    memcpy(S.hprefix[tab], S.prefix.data(), sizeof(int32_t) * S.prefix.size());
+    S.hostpre[1] = S.prefix;
     S.t_tri1 = clk::now();
     if (S.gate_armed) {
         // phase B is queued: meshes and table are in place, open the gate

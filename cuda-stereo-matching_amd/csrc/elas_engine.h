@@ -85,8 +85,12 @@ struct ElasSlot {
     ElasPair* dpairs[2] = {nullptr, nullptr};
     int32_t* dprefix[2] = {nullptr, nullptr};
     int32_t* dnsp = nullptr;
-    ElasPair* hpairs[2] = {nullptr, nullptr};
-    int32_t* hprefix[2] = {nullptr, nullptr};
+    // pinned mirrors of the tables: [0] phase A, [1] phase B, [2] phase B
+    // behind a gate (the real table; [1] then holds the placeholder that a
+    // timed-out gate copies instead, never written while the gate waits)
+    ElasPair* hpairs[3] = {nullptr, nullptr, nullptr};
+    int32_t* hprefix[3] = {nullptr, nullptr, nullptr};
+    std::vector<int32_t> hostpre[2];  // host copies of the stage prefixes of tables 0 and 1
     int32_t* hnsp = nullptr;
     float* hsp = nullptr;
     int32_t* htri = nullptr;

@@ -20,10 +20,10 @@ void elas_launch_grid_list(const ElasLaunch& L, int grid, hipStream_t s);
 void elas_launch_plane(const ElasLaunch& L, int grid, size_t lds, hipStream_t s);
 void elas_launch_rtile(const ElasLaunch& L, int grid, hipStream_t s);
 // Polls flag (mapped pinned memory) until it is non-zero, for at most `limit`
-// ticks of the 100 MHz device clock (then status[0] = 1), and copies `words`
-// 32-bit words from src to dst.
-void elas_launch_gate(const uint32_t* flag, uint32_t* status, const uint32_t* src, uint32_t* dst, int words,
-                      uint64_t limit, hipStream_t s);
+// ticks of the 100 MHz device clock, and copies `words` 32-bit words to dst:
+// from `ready` when the flag was set, else from `fallback` (then status[0] = 1).
+void elas_launch_gate(const uint32_t* flag, uint32_t* status, const uint32_t* ready, const uint32_t* fallback,
+                      uint32_t* dst, int words, uint64_t limit, hipStream_t s);
 void elas_launch_mean(const ElasLaunch& L, int grid, hipStream_t s);
 void elas_launch_dense(const ElasLaunch& L, const ElasKParams& kp, int cpw, int grid, size_t lds, hipStream_t s);
 // Lets the dense kernels allocate up to `bytes` of dynamic LDS; 0 on success.

@@ -96,6 +96,16 @@ int elas_triangulate_dedup(const float* pts, int n, bool (*arrange)(const float*
         }
     }
     const int count = i + 1;
+    if (count < 2) {
+        // every point is the same point: Triangle's divconqrecurse() has no
+        // case for a single vertex and recurses until the stack overflows;
+        // report it as too few points instead (two distinct points give an
+        // empty mesh, as in Triangle)
+        trifree((int*)sorted);
+        triangledeinit(&mesh, &b);
+        *trilist = (int*)NULL;
+        return -1;
+    }
     float* xy = (float*)trimalloc(2 * count * (int)sizeof(float));
     int* arr = (int*)trimalloc(count * (int)sizeof(int));
     for (int k = 0; k < count; k++) {

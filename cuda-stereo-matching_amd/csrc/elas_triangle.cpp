@@ -243,6 +243,10 @@ int tri_call(int k, const float* pts, int n, std::vector<int32_t>& out) {
     int* tl = nullptr;
     const int nt = elas_tri_arrangement(pts, n, arr) ? kArr[k](pts, n, arr.data(), &tl)
                                                      : kDedup[k](pts, n, arrange_sorted, &tl);
+    if (nt < 0) {  // a single distinct point
+        out.clear();
+        return -1;
+    }
     out.assign(tl, tl + 3 * (size_t)nt);
     free(tl);
     return nt;

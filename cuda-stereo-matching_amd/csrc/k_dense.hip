@@ -378,12 +378,14 @@ __global__ __launch_bounds__(256) void k_rtile(ElasLaunch L) {
 // Queued ahead of phase B while the host triangulates: lane 0 polls the
 // release flag in mapped pinned memory (system-scope acquire loads, about
 // 4 us apart) for at most `limit` ticks of the 100 MHz wall clock (3e9, 30 s,
-// unless a test shortens it), then the workgroup copies the phase B table, which the host wrote
-// into pinned memory before setting the flag, into the device table.  After
-// a timeout the table is copied all the same (the host keeps it valid at
-// every moment) and status[0] tells the host.
-__global__ __launch_bounds__(256) void k_gate(const uint32_t* flag, uint32_t* status, const uint32_t* src,
-                                              uint32_t* dst, int words, uint64_t limit) {
+// unless a test shortens it), then the workgroup copies the phase B table
+// into the device table: once released, `ready`, which the host wrote into
+// pinned memory before setting the flag; after a timeout, `fallback`, a table
+// written before the launch and left alone while the gate waits (the host may
+// be writing `ready` at that moment), and status[0] tells the host.
+__global__ __launch_bounds__(256) void k_gate(const uint32_t* flag, uint32_t* status, const uint32_t* ready,
+                                              const uint32_t* fallback, uint32_t* dst, int words, uint64_t limit) {
+    __shared__ uint32_t released;
     if (threadIdx.x == 0) {
         const uint64_t t0 = wall_clock64();
         uint32_t ok = 0u;
@@ -398,8 +400,10 @@ __global__ __launch_bounds__(256) void k_gate(const uint32_t* flag, uint32_t* st
             __builtin_amdgcn_s_sleep(127);
         }
         if (!ok) __hip_atomic_store(status, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        released = ok;
     }
     __syncthreads();
+    const uint32_t* src = released ? ready : fallback;
     for (int i = threadIdx.x; i < words; i += 256)
         dst[i] = __hip_atomic_load(src + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
@@ -807,9 +811,9 @@ __global__ __launch_bounds__(256) void k_lr(ElasLaunch L, ElasKParams kp) {
 void elas_launch_plane(const ElasLaunch& L, int grid, size_t lds, hipStream_t s) {
     hipLaunchKernelGGL(k_plane, dim3(grid), dim3(64), lds, s, L);
 }
-void elas_launch_gate(const uint32_t* flag, uint32_t* status, const uint32_t* src, uint32_t* dst, int words,
-                      uint64_t limit, hipStream_t s) {
-    hipLaunchKernelGGL(k_gate, dim3(1), dim3(256), 0, s, flag, status, src, dst, words, limit);
+void elas_launch_gate(const uint32_t* flag, uint32_t* status, const uint32_t* ready, const uint32_t* fallback,
+                      uint32_t* dst, int words, uint64_t limit, hipStream_t s) {
+    hipLaunchKernelGGL(k_gate, dim3(1), dim3(256), 0, s, flag, status, ready, fallback, dst, words, limit);
 }
 void elas_launch_rtile(const ElasLaunch& L, int grid, hipStream_t s) {
     hipLaunchKernelGGL(k_rtile, dim3(grid), dim3(256), 0, s, L);

@@ -6,7 +6,8 @@ Compiles
     namespace ``elas_oracle_tri``;
   * this directory's C++ restatement of the reference kernels;
 into ``oracle/_build/libelas_oracle.so``.  Without the reference checkout the
-already-built library is left as it is (the GPU box only loads it).
+already-built library is kept only while it is newer than this directory's
+sources (the GPU box only loads it); otherwise the build fails.
 FP contraction is off everywhere: Triangle's exact predicates and the float
 plane evaluation both depend on it.
 """
@@ -33,9 +34,14 @@ def _run(cmd):
 def build(verbose=False):
     """Compile the oracle; returns the library path."""
     if not os.path.isfile(os.path.join(REF_SRC, "triangle.cpp")):
-        if os.path.isfile(LIB):
-            return LIB
-        raise RuntimeError("reference checkout missing and no prebuilt oracle at " + LIB)
+        if not os.path.isfile(LIB):
+            raise RuntimeError("reference checkout missing and no prebuilt oracle at " + LIB)
+        srcs = [os.path.join(HERE, f) for f in os.listdir(HERE) if f.endswith((".cpp", ".h", ".py"))]
+        srcs.append(os.path.join(ROOT, "include", "elas_hip.h"))
+        if os.path.getmtime(LIB) < max(os.path.getmtime(s) for s in srcs if os.path.isfile(s)):
+            raise RuntimeError("oracle sources changed after %s was built, and it cannot be rebuilt "
+                               "without the reference checkout (%s holds Triangle)" % (LIB, REF_SRC))
+        return LIB
     os.makedirs(REF_DIR, exist_ok=True)
     os.makedirs(OUT_DIR, exist_ok=True)
     tri_o = os.path.join(REF_DIR, "triangle_oracle.o")

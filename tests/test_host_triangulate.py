@@ -34,8 +34,9 @@ def test_lattice_sets_equal_reference_order(pkg, oracle, seed):
 @pytest.mark.parametrize("seed", range(6))
 def test_shifted_sets_with_duplicates_equal_reference_order(pkg, oracle, seed):
     """Right-image support points (u - d, v): duplicates occur where two cells of
-    a row differ in d by their u distance; the library then keeps the caller's
-    order, since which duplicate Triangle drops depends on it."""
+    a row differ in d by their u distance; the library then runs Triangle's own
+    sort and duplicate removal on the caller's order, since which duplicate
+    Triangle drops depends on that order."""
     rng = np.random.default_rng(100 + seed)
     pts = _lattice(rng, 60, 30, 5, 0.6)
     d = rng.integers(0, 12, len(pts)).astype(np.float32)
@@ -43,6 +44,23 @@ def test_shifted_sets_with_duplicates_equal_reference_order(pkg, oracle, seed):
     if seed % 2 == 0:  # force duplicates
         pts[1::7] = pts[0:-1:7][: len(pts[1::7])]
     assert np.array_equal(pkg.triangulate(pts), oracle.triangulate(pts))
+
+
+def test_all_points_equal_is_too_few_points(pkg):
+    """Three or more copies of one point leave a single vertex after duplicate
+    removal, where Triangle's divconqrecurse() recurses until the stack
+    overflows (the oracle's Triangle is not called here for that reason): the
+    library reports too few points instead."""
+    for n in (3, 5):
+        with pytest.raises(pkg.ElasError) as e:
+            pkg.triangulate(np.full((n, 2), 7.0, np.float32))
+        assert e.value.code == pkg.ELAS_ERR_TOO_FEW_POINTS
+
+
+def test_two_distinct_points_after_duplicates_give_an_empty_mesh(pkg, oracle):
+    pts = np.array([[0, 0], [0, 0], [5, 5], [5, 5], [0, 0]], np.float32)
+    assert len(oracle.triangulate(pts)) == 0
+    assert len(pkg.triangulate(pts)) == 0
 
 
 def test_order_independence_without_duplicates(pkg, oracle):
