@@ -268,3 +268,24 @@ def process(left, right, prm=None, dump=True):
         out["tri1"] = bufs["tri1"][:dp.ntri1]
         out["work_support"], out["work_dense"] = dp.work_support, dp.work_dense
     return out
+
+
+def _maps_job(args):
+    left, right, dmax = args
+    out = process(left, right, params(disp_max=dmax), dump=False)
+    return out["D1"], out["D2"], out["status"]
+
+
+def process_many(pairs, workers=16):
+    """[(D1, D2, status)] of the whole pipeline for each (left, right,
+    disp_max), computed in up to `workers` fresh processes (Triangle keeps its
+    state in globals, so one process runs one pipeline at a time).  The
+    processes are spawned, not forked, so a caller that has initialised a GPU
+    hands none of its state to them."""
+    import concurrent.futures as cf
+    import multiprocessing as mp
+    workers = max(1, min(workers, len(pairs), os.cpu_count() or 1))
+    if workers == 1:
+        return [_maps_job(p) for p in pairs]
+    with cf.ProcessPoolExecutor(workers, mp_context=mp.get_context("spawn")) as ex:
+        return list(ex.map(_maps_job, pairs))
