@@ -102,6 +102,8 @@ SIGNATURES = {
     "elas_hip_triangulate": (ctypes.c_int, [_P, ctypes.c_int32, _P, ctypes.c_int64,
                                             ctypes.POINTER(ctypes.c_int64)]),
     "elas_hip_set_autotune": (ctypes.c_int, [_P, ctypes.c_int32]),
+    "elas_hip_set_tri_split": (ctypes.c_int, [ctypes.c_int32]),
+    "elas_hip_tri_split_depth": (ctypes.c_int, [ctypes.c_int32]),
     "elas_io_read_pgm": (ctypes.c_int, [ctypes.c_char_p, _P, ctypes.c_int64, ctypes.POINTER(ctypes.c_int32),
                                         ctypes.POINTER(ctypes.c_int32)]),
     "elas_io_write_pgm": (ctypes.c_int, [ctypes.c_char_p, _P, ctypes.c_int32, ctypes.c_int32]),
@@ -193,6 +195,20 @@ def triangulate(points):
     if st != ELAS_OK:
         raise ElasError(st, "triangulate: status %d" % st)
     return tri[:min(n.value, cap)].copy()
+
+
+def set_tri_split(max_depth=-1):
+    """Process-wide split of each mesh's divide-and-conquer over the host
+    Triangle threads: up to 2**max_depth concurrent subtrees (0: serial, -1:
+    automatic).  Meshes are the same under every setting."""
+    st = load_library().elas_hip_set_tri_split(int(max_depth))
+    if st != ELAS_OK:
+        raise ElasError(st, "set_tri_split(%r)" % (max_depth,))
+
+
+def tri_split_depth(n):
+    """Split depth a mesh of n points gets under the current setting."""
+    return int(load_library().elas_hip_tri_split_depth(int(n)))
 
 
 def read_pgm(path):

@@ -357,7 +357,7 @@ int elas_hip_triangulate(const float* pts, int32_t n, int32_t* tri, int64_t cap,
     job.pts = pts;
     job.n = n;
     job.out = &out;
-    elas_triangulate_jobs(&job, 1, 1);
+    elas_triangulate_jobs(&job, 1, ELAS_TRI_INSTANCES);  // on this thread, the merge split over the pool
     if (job.result < 0) {
         *ntri = 0;
         return ELAS_ERR_TOO_FEW_POINTS;  // n < 3, or every point the same point
@@ -366,6 +366,14 @@ int elas_hip_triangulate(const float* pts, int32_t n, int32_t* tri, int64_t cap,
     memcpy(tri, out.data(), sizeof(int32_t) * 3 * (size_t)std::min<int64_t>(cap, job.result));
     return ELAS_OK;
 }
+
+int elas_hip_set_tri_split(int32_t max_depth) {
+    if (max_depth < -1 || max_depth > 12) return ELAS_ERR_INVALID_ARGUMENT;
+    elas_tri_set_split(max_depth);
+    return ELAS_OK;
+}
+
+int elas_hip_tri_split_depth(int32_t n) { return elas_tri_split_depth(n); }
 
 int elas_hip_serve_stats(elas_ctx* c, elas_serve_stats* st) {
     if (!c) return ELAS_ERR_INVALID_ARGUMENT;

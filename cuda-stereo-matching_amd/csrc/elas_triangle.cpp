@@ -78,6 +78,11 @@
     int elas_triangulate_arranged(const float* pts, int n, const int* arr, int** trilist);         \
     int elas_triangulate_dedup(const float* pts, int n, bool (*arrange)(const float*, int, int*), \
                                int** trilist);                                                     \
+    int elas_triangulate_split(const float* pts, int n, const int* arr, int depth,                 \
+                               ElasParallelFor par, void* parctx, std::vector<int32_t>* out);      \
+    int elas_triangulate_dedup_split(const float* pts, int n,                                      \
+                                     bool (*arrange)(const float*, int, int*), int depth,          \
+                                     ElasParallelFor par, void* parctx, std::vector<int32_t>* out); \
     }
 ELAS_TRI_DECL(elas_tri0) ELAS_TRI_DECL(elas_tri1) ELAS_TRI_DECL(elas_tri2) ELAS_TRI_DECL(elas_tri3)
 ELAS_TRI_DECL(elas_tri4) ELAS_TRI_DECL(elas_tri5) ELAS_TRI_DECL(elas_tri6) ELAS_TRI_DECL(elas_tri7)
@@ -107,6 +112,29 @@ const ArrFn kArr[ELAS_TRI_INSTANCES] = {
     elas_tri10::elas_triangulate_arranged, elas_tri11::elas_triangulate_arranged,
     elas_tri12::elas_triangulate_arranged, elas_tri13::elas_triangulate_arranged,
     elas_tri14::elas_triangulate_arranged, elas_tri15::elas_triangulate_arranged};
+
+typedef int (*SplitFn)(const float*, int, const int*, int, ElasParallelFor, void*, std::vector<int32_t>*);
+const SplitFn kSplit[ELAS_TRI_INSTANCES] = {
+    elas_tri0::elas_triangulate_split, elas_tri1::elas_triangulate_split,
+    elas_tri2::elas_triangulate_split, elas_tri3::elas_triangulate_split,
+    elas_tri4::elas_triangulate_split, elas_tri5::elas_triangulate_split,
+    elas_tri6::elas_triangulate_split, elas_tri7::elas_triangulate_split,
+    elas_tri8::elas_triangulate_split, elas_tri9::elas_triangulate_split,
+    elas_tri10::elas_triangulate_split, elas_tri11::elas_triangulate_split,
+    elas_tri12::elas_triangulate_split, elas_tri13::elas_triangulate_split,
+    elas_tri14::elas_triangulate_split, elas_tri15::elas_triangulate_split};
+
+typedef int (*DedupSplitFn)(const float*, int, bool (*)(const float*, int, int*), int, ElasParallelFor, void*,
+                            std::vector<int32_t>*);
+const DedupSplitFn kDedupSplit[ELAS_TRI_INSTANCES] = {
+    elas_tri0::elas_triangulate_dedup_split, elas_tri1::elas_triangulate_dedup_split,
+    elas_tri2::elas_triangulate_dedup_split, elas_tri3::elas_triangulate_dedup_split,
+    elas_tri4::elas_triangulate_dedup_split, elas_tri5::elas_triangulate_dedup_split,
+    elas_tri6::elas_triangulate_dedup_split, elas_tri7::elas_triangulate_dedup_split,
+    elas_tri8::elas_triangulate_dedup_split, elas_tri9::elas_triangulate_dedup_split,
+    elas_tri10::elas_triangulate_dedup_split, elas_tri11::elas_triangulate_dedup_split,
+    elas_tri12::elas_triangulate_dedup_split, elas_tri13::elas_triangulate_dedup_split,
+    elas_tri14::elas_triangulate_dedup_split, elas_tri15::elas_triangulate_dedup_split};
 
 double now_s() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
@@ -238,8 +266,36 @@ bool arrange_sorted(const float* xy, int count, int* arr) {
     return true;
 }
 
-int tri_call(int k, const float* pts, int n, std::vector<int32_t>& out) {
+// Split depth of a mesh of n points: the divide-and-conquer is cut into up to
+// 2^depth concurrent subtrees of at least ELAS_TRI_LEAF_MIN points, at most
+// g_split_max levels (elas_tri_set_split; -1 = ELAS_TRI_SPLIT_AUTO).
+#define ELAS_TRI_LEAF_MIN 384
+#define ELAS_TRI_SPLIT_AUTO 3
+std::atomic<int> g_split_max{-1};
+
+int split_depth(int n) {
+    int mx = g_split_max.load(std::memory_order_relaxed);
+    if (mx < 0) mx = ELAS_TRI_SPLIT_AUTO;
+    const int leaf_min = g_split_max.load(std::memory_order_relaxed) < 0 ? ELAS_TRI_LEAF_MIN : 4;
+    int d = 0;
+    while (d < mx && (n >> (d + 1)) >= leaf_min) d++;
+    return d;
+}
+
+void pool_parallel_for(void* ctx, int ntasks, void (*task)(void*, int), void* arg);
+
+// One "zQB" Triangle call on copy k; with split > 0 the merge's subtrees run
+// through the pool's helpers.  Corners are numbered by the caller's order.
+int tri_call(int k, const float* pts, int n, std::vector<int32_t>& out, bool split, void* pool) {
     thread_local std::vector<int32_t> arr;
+    const int depth = split ? split_depth(n) : 0;
+    if (depth > 0) {
+        const int nt = elas_tri_arrangement(pts, n, arr)
+                           ? kSplit[k](pts, n, arr.data(), depth, pool_parallel_for, pool, &out)
+                           : kDedupSplit[k](pts, n, arrange_sorted, depth, pool_parallel_for, pool, &out);
+        if (nt < 0) out.clear();
+        return nt;
+    }
     int* tl = nullptr;
     const int nt = elas_tri_arrangement(pts, n, arr) ? kArr[k](pts, n, arr.data(), &tl)
                                                      : kDedup[k](pts, n, arrange_sorted, &tl);
@@ -252,23 +308,27 @@ int tri_call(int k, const float* pts, int n, std::vector<int32_t>& out) {
     return nt;
 }
 
-int run_instance(int k, ElasTriJob& j) {
+int run_instance(int k, ElasTriJob& j, bool split, void* pool) {
     j.t_begin = now_s();
     if (j.n < 3) {  // Triangle would exit(1) (src/triangle.cpp:7652-7654)
         j.out->clear();
         j.t_end = now_s();
         return -1;
     }
-    const int nt = tri_call(k, j.pts, j.n, *j.out);
+    const int nt = tri_call(k, j.pts, j.n, *j.out, split, pool);
     j.t_end = now_s();
     return nt;
 }
 
-// Worker K owns Triangle copy K.  Jobs of one submit() are spread over the
-// workers; submit() returns when all of them are done.  A worker that just
-// finished a job polls the queue for up to ELAS_TRI_SPIN_S before sleeping: a
-// core that went idle between calls runs Triangle up to ~20% slower for its
-// first few hundred microseconds, which is the whole job at these sizes.
+// Worker K owns Triangle copy K and runs whole jobs on it.  A job that splits
+// its merge publishes the subtrees as a task set; idle workers, and a caller
+// waiting for its meshes, take tasks from the sets before taking new jobs.
+// Any thread may run a subtree of copy K: the merge only reads copy K's
+// namespace-scope state, which copy K's owner wrote in triangleinit() before
+// publishing the set.  A worker that just finished polls for work for up to
+// ELAS_TRI_SPIN_S before sleeping: a core that went idle between calls runs
+// Triangle up to ~20% slower for its first few hundred microseconds, which is
+// the whole job at these sizes.
 #define ELAS_TRI_SPIN_S 0.003
 class TriPool {
 public:
@@ -285,11 +345,24 @@ public:
         int remaining = 0;
         std::atomic<int> left{0};
     };
+    // tasks of one split merge; claimed through `next` (by helpers under m_),
+    // `done` counts finished tasks and is each helper's last access
+    struct TaskSet {
+        void (*fn)(void*, int);
+        void* arg;
+        int n;
+        std::atomic<int> next{0};
+        std::atomic<int> done{0};
+        bool listed = false;  // in help_, under m_
+    };
     void run(ElasTriJob* jobs, int n, int max_workers) {
         if (n <= 0) return;
         if (n == 1 || max_workers <= 1) {
+            // on the calling thread with copy 0; a single job still splits
+            // its merge over the workers unless one worker is asked for
             std::lock_guard<std::mutex> g(serial_);
-            for (int i = 0; i < n; i++) jobs[i].result = run_instance(0, jobs[i]);
+            if (max_workers > 1) ensure_workers();
+            for (int i = 0; i < n; i++) jobs[i].result = run_instance(0, jobs[i], max_workers > 1, this);
             return;
         }
         Batch b;
@@ -308,29 +381,103 @@ public:
         }
         for (int i = 0; i < n; i++) cv_.notify_one();
     }
+    // Runs the set's tasks here and on idle helpers; returns when all are done.
+    void parallel_for(int ntasks, void (*fn)(void*, int), void* arg) {
+        if (ntasks <= 0) return;
+        TaskSet ts;
+        ts.fn = fn;
+        ts.arg = arg;
+        ts.n = ntasks;
+        if (ntasks > 1) {
+            {
+                std::lock_guard<std::mutex> g(m_);
+                ts.listed = true;
+                help_.push_back(&ts);
+                helping_.fetch_add(1, std::memory_order_release);
+            }
+            for (int i = 1; i < ntasks; i++) cv_.notify_one();
+        }
+        for (;;) {
+            const int i = ts.next.fetch_add(1, std::memory_order_relaxed);
+            if (i >= ntasks) break;
+            fn(arg, i);
+            ts.done.fetch_add(1, std::memory_order_release);
+        }
+        if (ntasks > 1) {
+            std::lock_guard<std::mutex> g(m_);
+            unlist(&ts);
+        }
+        while (ts.done.load(std::memory_order_acquire) < ntasks) _mm_pause();
+    }
+    // Runs one task of a listed set on this thread; false if there is none.
+    bool help_one() {
+        if (helping_.load(std::memory_order_acquire) == 0) return false;
+        TaskSet* ts = nullptr;
+        int i = 0;
+        {
+            std::lock_guard<std::mutex> g(m_);
+            if (!claim(&ts, &i)) return false;
+        }
+        ts->fn(ts->arg, i);
+        ts->done.fetch_add(1, std::memory_order_release);
+        return true;
+    }
 
 private:
     struct Item {
         ElasTriJob* job;
         Batch* batch;
     };
+    void unlist(TaskSet* ts) {
+        if (!ts->listed) return;
+        ts->listed = false;
+        help_.erase(std::find(help_.begin(), help_.end(), ts));
+        helping_.fetch_sub(1, std::memory_order_release);
+    }
+    // Claims a task of the first listed set that has one (m_ held); exhausted
+    // sets are unlisted on the way.
+    bool claim(TaskSet** out, int* idx) {
+        while (!help_.empty()) {
+            TaskSet* ts = help_.front();
+            const int i = ts->next.fetch_add(1, std::memory_order_relaxed);
+            if (i < ts->n) {
+                *out = ts;
+                *idx = i;
+                return true;
+            }
+            unlist(ts);
+        }
+        return false;
+    }
     void ensure_workers() {
         if (!threads_.empty()) return;
         for (int k = 1; k < ELAS_TRI_INSTANCES; k++) threads_.emplace_back([this, k] { loop(k); });
     }
     void loop(int k) {
-        double last = 0.0;  // end of this worker's last job
+        double last = 0.0;  // end of this worker's last job or task
         for (;;) {
             Item it;
             {
-                while (pending_.load(std::memory_order_acquire) == 0 && now_s() - last < ELAS_TRI_SPIN_S) _mm_pause();
+                while (pending_.load(std::memory_order_acquire) == 0 && helping_.load(std::memory_order_acquire) == 0 &&
+                       now_s() - last < ELAS_TRI_SPIN_S)
+                    _mm_pause();
                 std::unique_lock<std::mutex> g(m_);
-                cv_.wait(g, [&] { return !q_.empty(); });
+                cv_.wait(g, [&] { return !q_.empty() || !help_.empty(); });
+                TaskSet* ts = nullptr;
+                int i = 0;
+                if (claim(&ts, &i)) {  // subtrees of a running job first
+                    g.unlock();
+                    ts->fn(ts->arg, i);
+                    ts->done.fetch_add(1, std::memory_order_release);
+                    last = now_s();
+                    continue;
+                }
+                if (q_.empty()) continue;
                 it = q_.front();
                 q_.pop_front();
                 pending_.fetch_sub(1, std::memory_order_relaxed);
             }
-            it.job->result = run_instance(k, *it.job);
+            it.job->result = run_instance(k, *it.job, true, this);
             {
                 std::lock_guard<std::mutex> g(it.batch->m);
                 if (--it.batch->remaining == 0) it.batch->cv.notify_all();
@@ -340,17 +487,27 @@ private:
         }
     }
     std::atomic<int> pending_{0};  // queued jobs, read without the lock while polling
+    std::atomic<int> helping_{0};  // listed task sets, read without the lock while polling
     std::mutex m_, serial_;
     std::condition_variable cv_;
     std::deque<Item> q_;
+    std::vector<TaskSet*> help_;
     std::vector<std::thread> threads_;
 };
+
+void pool_parallel_for(void* ctx, int ntasks, void (*task)(void*, int), void* arg) {
+    static_cast<TriPool*>(ctx)->parallel_for(ntasks, task, arg);
+}
 
 }  // namespace
 
 void elas_triangulate_jobs(ElasTriJob* jobs, int n, int max_workers) {
     TriPool::get().run(jobs, n, max_workers);
 }
+
+void elas_tri_set_split(int max_depth) { g_split_max.store(max_depth < 0 ? -1 : std::min(max_depth, 12)); }
+
+int elas_tri_split_depth(int n) { return split_depth(n); }
 
 struct ElasTriTicket {
     TriPool::Batch batch;
@@ -371,12 +528,15 @@ bool elas_tri_done(ElasTriTicket* t) {
     return t->batch.remaining == 0;
 }
 
-// Polls for up to ELAS_TRI_SPIN_S before sleeping: a synchronous call waits
-// here for meshes that take a fraction of a millisecond, and a sleeping
-// thread's wake-up would add tens of microseconds to it.
+// Polls for up to ELAS_TRI_SPIN_S before sleeping, running subtrees of the
+// jobs' split merges meanwhile: a synchronous call waits here for meshes that
+// take a fraction of a millisecond, and a sleeping thread's wake-up would add
+// tens of microseconds to it.
 void elas_tri_wait(ElasTriTicket* t) {
     const double t0 = now_s();
-    while (t->batch.left.load(std::memory_order_acquire) > 0 && now_s() - t0 < ELAS_TRI_SPIN_S) _mm_pause();
+    TriPool& pool = TriPool::get();
+    while (t->batch.left.load(std::memory_order_acquire) > 0 && now_s() - t0 < ELAS_TRI_SPIN_S)
+        if (!pool.help_one()) _mm_pause();
     std::unique_lock<std::mutex> g(t->batch.m);
     t->batch.cv.wait(g, [&] { return t->batch.remaining == 0; });
 }

@@ -257,6 +257,19 @@ int elas_io_disparity_to_u8(const float* d, uint8_t* out, int64_t n);
  * returns ELAS_ERR_TOO_FEW_POINTS (the reference's Triangle exits there). */
 int elas_hip_triangulate(const float* pts, int32_t n, int32_t* tri, int64_t cap, int64_t* ntri);
 
+/* Threads of the host Delaunay step (no reference counterpart: the reference
+ * runs Triangle serially).  Triangle's divide-and-conquer allocates its
+ * triangles in post-order (left half, right half, the merge of the two), so
+ * its top max_depth levels can run as up to 2^max_depth concurrent subtrees,
+ * each with its own triangle pool, merged by Triangle's own mergehulls() and
+ * read out in that order: the triangle list, numbering included, is the
+ * serial one under every setting.  Process-wide; 0 = serial Triangle, -1 =
+ * automatic (the default: subtrees of at least 384 points, at most 8 per
+ * mesh).  elas_hip_tri_split_depth reports the depth a mesh of n points gets
+ * under the current setting. */
+int elas_hip_set_tri_split(int32_t max_depth);
+int elas_hip_tri_split_depth(int32_t n);
+
 /* ---- Serving (no reference counterpart: the reference has one blocking
  * process call).  A scheduler thread owned by the context admits queued
  * requests into idle pipeline slots as chunks, FIFO, as long as the chunk's

@@ -142,3 +142,54 @@ def test_too_few_points(pkg):
     with pytest.raises(pkg.ElasError) as e:
         pkg.triangulate(np.zeros((2, 2), np.float32))
     assert e.value.code == pkg.ELAS_ERR_TOO_FEW_POINTS
+
+
+@pytest.fixture
+def split(pkg):
+    """Forces the split depth of the merge over the host threads; restores
+    the automatic setting afterwards."""
+    yield pkg.set_tri_split
+    pkg.set_tri_split(-1)
+
+
+def _split_sets(oracle_rng_seed=0):
+    rng = np.random.default_rng(oracle_rng_seed)
+    sets = [("lattice", _lattice(rng, 61, 37, 5, 0.55))]
+    for n in (4, 7, 13, 64, 65, 257, 1000, 3001):
+        side = max(4, int(np.sqrt(n) * 2))
+        flat = rng.choice(side * side, n, replace=False)
+        sets.append(("random%d" % n, np.stack([flat % side, flat // side], 1).astype(np.float32) * 3 - 7))
+    cu, cv = np.meshgrid(np.arange(41), np.arange(29))
+    sets.append(("full", np.stack([cu.ravel(), cv.ravel()], 1).astype(np.float32) * 5))
+    sets.append(("collinear", np.stack([np.arange(40) * 5, np.zeros(40)], 1).astype(np.float32)))
+    base = rng.integers(0, 60, (400, 2)).astype(np.float32)
+    dup = np.repeat(base, rng.integers(1, 4, len(base)), axis=0)
+    sets.append(("duplicates", dup[rng.permutation(len(dup))]))
+    sets.append(("nonint", (rng.random((900, 2)) * [300, 200]).astype(np.float32)))
+    return sets
+
+
+@pytest.mark.parametrize("depth", [0, 1, 2, 3, 4, 6])
+def test_split_merge_equals_serial_triangle(pkg, oracle, split, depth):
+    """The merge cut into up to 2^depth concurrent subtrees (each with its own
+    triangle pool, read out in post-order) gives Triangle's serial triangle
+    list on every kind of set: cocircular lattices, collinear points, sets
+    with duplicates and non-integer coordinates.  Small subtrees are forced
+    (the setting's lower bound on subtree size applies only to the automatic
+    depth), so leaves of 2 and 3 points meet the merge too."""
+    split(depth)
+    for name, pts in _split_sets(depth):
+        assert pkg.tri_split_depth(len(pts)) == (min(depth, int(np.log2(len(pts) / 4))) if len(pts) >= 8 else 0)
+        assert np.array_equal(pkg.triangulate(pts), oracle.triangulate(pts)), (name, depth)
+
+
+def test_split_on_pipeline_points(pkg, oracle, pairs, split):
+    left, right, _, dmax = pairs("1242x375_d128")
+    ref = oracle.process(left, right, oracle.params(disp_max=dmax))
+    for depth in (1, 2, 3, 5):
+        split(depth)
+        assert np.array_equal(pkg.triangulate(ref["sp0"]), ref["tri0"]), depth
+        assert np.array_equal(pkg.triangulate(ref["sp1"]), ref["tri1"]), depth
+    split(-1)
+    assert pkg.tri_split_depth(len(ref["sp0"])) == 1        # 1,427 points: two subtrees of >= 384
+    assert pkg.tri_split_depth(22418) == 3                   # capped at 8 subtrees
