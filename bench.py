@@ -1,16 +1,19 @@
 """Benchmark of the ELAS path on one MI355X.
 
-Workload (the headline metric's single-GPU configuration): one synthetic
-1242x375 uint8 stereo pair, disparities 0..127, seed 1 (16 slanted-plane
-regions, 5% occluded strips), reference-effective parameters.  A step is one
-Elas::process of that pair with images and disparity maps resident in HBM
-(elas_hip_process_device), host Delaunay included.
+Workload (the largest single-GPU configuration of the headline metric): one
+synthetic 2816x1880 uint8 stereo pair, disparities 0..255, seed 2 (32
+slanted-plane regions, slopes up to 0.1), reference-effective parameters.  A
+step is one Elas::process of that pair with images and disparity maps
+resident in HBM (elas_hip_process_device), host Delaunay included.
 
-value = Mpx*disp/s = W*H*(disp_max+1)/1e6 per pair / wall seconds.
-Also reported: pairs/s, per-stage device times (HIP events on the pipeline
-stream), the roofline of the dominant kernel, the batched-vs-plain pairs/s
-on the mixed 64-pair batch, and the oracle (the reference algorithm restated
-on the CPU) timed on this host on a bounded sample.
+value = Mpx*disp/s = W*H*(disp_max+1)/1e6 per pair / wall seconds over the
+timed steps.  Also reported: pairs/s, per-call median and p90 latency, the
+same pair through the host-buffer call (elas_hip_process, PCIe included),
+per-stage device times (HIP events on the pipeline stream), per-kernel HBM
+and SAD fractions with the roofline of the dominant kernel, the
+batched-vs-plain pairs/s on the mixed 64-pair batch, the serving sweep, and
+the oracle (the reference algorithm restated on the CPU) timed on this host
+on one core and on every core.
 """
 import argparse
 import json
@@ -119,13 +122,16 @@ def batch_vs_plain(pkg, synth, n=64, reps=3, pool=None):
             "host_buffers_batched_pairs_per_s": round(n / th, 2), "buffers": "device-resident"}
 
 
-def serving(pkg, synth, rates=(100, 400, 800), n_req=400, pool=None, seed=4000):
+def serving(pkg, synth, rates=(50, 100, 200, 400, 800), n_req=2000, pool=None, seed=4000,
+            knee=(1200, 1600, 2400, 3200, 4800, 6400)):
     """Serving sweep: Poisson arrivals at each rate of pairs drawn from the
     mixed-batch pool (request i uses pool pair i mod len(pool)), host buffers
     in pinned memory, submitted to the continuous-batching loop
     (elas_hip_submit / elas_hip_poll).  Latency = arrival -> D1/D2 in host
     memory (the feeder's own lateness included); sustained pairs/s = requests
-    / (first arrival -> last completion)."""
+    / (first arrival -> last completion).  After `rates`, the `knee` rates run
+    in turn until one is past the knee: p99 above 4x p50, or less than 90% of
+    the offered rate sustained."""
     import torch
     pool = pool or synth.batch_pairs(64)
     pin = lambda a: torch.from_numpy(a).pin_memory().numpy()
@@ -140,7 +146,9 @@ def serving(pkg, synth, rates=(100, 400, 800), n_req=400, pool=None, seed=4000):
     el.process_batch(list(seen.values()))
     out = {}
     rng = np.random.default_rng(seed)
-    for lam in rates:
+    for lam in list(rates) + list(knee):
+        if lam in knee and out and out[max(out, key=float)].get("past_knee"):
+            break
         el.serve_start()
         gaps = rng.exponential(1.0 / lam, n_req)
         lat, late, pending = [], [], {}
@@ -156,7 +164,7 @@ def serving(pkg, synth, rates=(100, 400, 800), n_req=400, pool=None, seed=4000):
                 late.append(time.perf_counter() - arr[i])
                 i += 1
                 continue
-            for t in list(pending):
+            for t in list(pending)[:64]:   # the oldest requests (completions are nearly in order)
                 res = el.poll(t)
                 if res is not None:
                     a = pending.pop(t)
@@ -176,6 +184,8 @@ def serving(pkg, synth, rates=(100, 400, 800), n_req=400, pool=None, seed=4000):
                          "chunks": int(st["chunks"]), "max_chunk_pairs": int(st["max_chunk_pairs"]),
                          "max_in_flight_pairs": int(st["max_in_flight_pairs"]),
                          "peak_in_flight_MB": round(st["peak_in_flight_bytes"] / 1e6, 1)}
+        r = out[str(lam)]
+        r["past_knee"] = bool(r["p99_ms"] > 4 * r["p50_ms"] or r["sustained_pairs_per_s"] < 0.9 * lam)
     el.close()
     return {"rates": out, "pool_pairs": len(P), "hbm_budget_GiB": 8, "buffers": "pinned host"}
 
@@ -186,8 +196,9 @@ def kernel_rooflines(el, wc, W, H, dmax, reps=50):
     pipeline stream (elas_hip_time_kernel).  Algorithmic work per launch:
       descriptor  17 B per pixel and image (1 B image read, 16 B written)
       support     64 byte-|diffs| per evaluated (candidate, d) (oracle count);
-                  HBM: descriptor rows v-2/v+2 of both images, 64 B per
-                  lattice-row pixel
+                  HBM: descriptor rows v-2 and v+2 of both images once per
+                  lattice row (64 B per lattice-row pixel, both passes), and
+                  the lattice written by both passes (8 B per candidate)
       dense       16 byte-|diffs| per evaluated (pixel, d) (oracle count);
                   HBM: both descriptor images read once, both priors read,
                   both dense maps written: 48 B per pixel of the pair
@@ -197,9 +208,9 @@ def kernel_rooflines(el, wc, W, H, dmax, reps=50):
     cw, ch = W // 5, H // 5
     work = {
         "descriptor": (0.0, 2 * 17.0 * W * H),
-        # both passes in one launch: the left pass reads the right image's
-        # descriptor rows v+-2 over the lattice rows, the right pass the left's
-        "support": (64.0 * (wc["support_left"] + wc["support_right"]), 2 * 64.0 * W * ch * 2),
+        # both passes in one launch need rows v-2 and v+2 of both descriptor
+        # images at each lattice row v (2 rows x 2 images x 16 B per pixel)
+        "support": (64.0 * (wc["support_left"] + wc["support_right"]), 64.0 * W * ch + 8.0 * cw * ch),
         "dense": (16.0 * wc["dense"], 48.0 * W * H),
         "lr": (0.0, 16.0 * W * H),
     }
@@ -242,6 +253,13 @@ def traffic_of(kernel, workload, W, H, dmax):
 
 
 def cpu_baseline(oracle, left, right, dmax, budget_s=12.0):
+    """The oracle (the reference algorithm restated in C++, one thread per
+    pipeline) on this host: whole pipelines of the same pair back to back on
+    one core for about budget_s, then one pipeline per logical CPU at once
+    (processes, since Triangle keeps its state in globals), timed after a
+    warm-up round of the same pool."""
+    import concurrent.futures as cf
+    import multiprocessing as mp
     prm = oracle.params(disp_max=dmax)
     n, t0 = 0, time.perf_counter()
     while True:
@@ -250,22 +268,31 @@ def cpu_baseline(oracle, left, right, dmax, budget_s=12.0):
         el = time.perf_counter() - t0
         if (el > budget_s and n >= 2) or n >= 1000:
             break
-    return n, el
+    cores = os.cpu_count() or 1
+    with cf.ProcessPoolExecutor(cores, mp_context=mp.get_context("spawn")) as ex:
+        list(ex.map(oracle._maps_job, [(left[:64, :64].copy(), right[:64, :64].copy(), 15)] * cores))
+        t1 = time.perf_counter()
+        list(ex.map(oracle._maps_job, [(left, right, dmax)] * cores))
+        el_all = time.perf_counter() - t1
+    return n, el, cores, el_all
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched comparison")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-serve", action="store_true", help="skip the serving sweep")
-    ap.add_argument("--serve-full", action="store_true",
-                    help="serving sweep at 50..800 pairs/s with 2000 requests per rate")
-    ap.add_argument("--workload", default="1242x375_d128", choices=["450x375_d64", "1242x375_d128", "2816x1880_d256"],
-                    help="1242x375_d128 is the headline; the others for profiling")
+    ap.add_argument("--serve-quick", action="store_true",
+                    help="reduced serving sweep (100, 400, 800 pairs/s, 400 requests each, no knee search)")
+    ap.add_argument("--workload", default="2816x1880_d256", choices=["450x375_d64", "1242x375_d128", "2816x1880_d256"],
+                    help="2816x1880_d256 is the headline; the others for profiling")
+    ap.add_argument("--tri-split", type=int, default=-1,
+                    help="host Delaunay split depth (0 serial Triangle, -1 automatic)")
+    ap.add_argument("--no-host", action="store_true", help="skip the host-buffer single-pair rate")
     ap.add_argument("--dense-tiles", default="",
                     help="force the dense tiling 'cells_per_wave,steps_per_chunk' (profiler runs, where the "
                          "autotuner's timings are distorted)")
@@ -295,6 +322,7 @@ def main():
     left, right, _, dmax = synth.config_pair(args.workload)
     H, W = left.shape
     mpxd = W * H * (dmax + 1) / 1e6
+    pkg.set_tri_split(args.tri_split)
     el = pkg.Elas(pkg.default_params(disp_max=dmax), device=0)
     if args.dense_tiles:
         el.set_dense_tiles(*(int(x) for x in args.dense_tiles.split(",")))
@@ -307,11 +335,17 @@ def main():
     for _ in range(args.warmup):
         el.process_device(L, R, D1, D2)
     torch.cuda.synchronize()
+    # each call is synchronous (it returns with the maps in HBM), so the
+    # per-call clock brackets one whole step
+    per_call = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
+        tc = time.perf_counter()
         el.process_device(L, R, D1, D2)
+        per_call.append(time.perf_counter() - tc)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    per_call = np.array(per_call) * 1e3
     # per-stage timers of the same calls, read in separate untimed steps so
     # that reading them stays out of the timed region
     stage_sum = {}
@@ -323,6 +357,24 @@ def main():
     torch.cuda.synchronize()
     stages = {k: v / nst for k, v in stage_sum.items()}
     value = args.steps * mpxd / wall
+    host = None
+    if not args.no_host:
+        # the reference's own call: pageable host buffers in and out
+        h1 = np.empty((H, W), np.float32)
+        h2 = np.empty((H, W), np.float32)
+        for _ in range(3):
+            el.process(left, right, h1, h2)
+        nh = max(10, args.steps // 2)
+        th = []
+        for _ in range(nh):
+            tc = time.perf_counter()
+            el.process(left, right, h1, h2)
+            th.append(time.perf_counter() - tc)
+        th = np.array(th) * 1e3
+        host = {"pairs_per_s": round(1e3 / th.mean(), 2), "mpx_disp_per_s": round(mpxd * 1e3 / th.mean(), 1),
+                "p50_ms": round(float(np.median(th)), 4), "p90_ms": round(float(np.percentile(th, 90)), 4),
+                "calls": nh, "buffers": "pageable host (elas_hip_process, PCIe copies included)",
+                "same_maps": bool(np.array_equal(h1, D1.cpu().numpy()) and np.array_equal(h2, D2.cpu().numpy()))}
 
     wc = work_counts(oracle, left, right, dmax)
     kernels = kernel_rooflines(el, wc, W, H, dmax)
@@ -340,6 +392,11 @@ def main():
         "config": {"workload": "elas_plain_%dx%d_d%d_device_resident" % (W, H, dmax + 1), "width": W, "height": H,
                    "disp_max": dmax, "pairs_per_step": 1, "host_delaunay": "included"},
         "pairs_per_s": round(args.steps / wall, 2),
+        "per_call_ms": {"p50": round(float(np.median(per_call)), 4), "p90": round(float(np.percentile(per_call, 90)), 4),
+                        "min": round(float(per_call.min()), 4), "max": round(float(per_call.max()), 4)},
+        "host_delaunay": {"split_depth": pkg.tri_split_depth(int(el.shape_info()["num_support"])),
+                          "support_points": int(el.shape_info()["num_support"])},
+        "host_buffers": host,
         "stages_ms": {k: round(v, 5) for k, v in stages.items()},
         "gpu_phase_ms_per_pair": round(stages["phase_a"] + stages["phase_b"], 5),
         "launch_mode": "hip graphs per shape bucket (stages inside a graph read -1)",
@@ -356,21 +413,23 @@ def main():
     if not args.no_batch:
         res["batch_vs_plain"] = batch_vs_plain(pkg, synth, pool=pool)
     if not args.no_serve:
-        if args.serve_full:
-            res["serving"] = serving(pkg, synth, rates=(50, 100, 200, 400, 800), n_req=2000, pool=pool)
+        if args.serve_quick:
+            res["serving"] = serving(pkg, synth, rates=(100, 400, 800), n_req=400, pool=pool, knee=())
         else:
             res["serving"] = serving(pkg, synth, pool=pool)
     if not args.no_cpu:
-        n, el_s = cpu_baseline(oracle, left, right, dmax, args.cpu_seconds)
+        n, el_s, cores, el_all = cpu_baseline(oracle, left, right, dmax, args.cpu_seconds)
         model = ""
         try:
             model = [l.split(":", 1)[1].strip() for l in open("/proc/cpuinfo") if l.startswith("model name")][0]
         except Exception:
             pass
         res["host_cpu"] = {"model": model, "logical_cpus": os.cpu_count()}
-        res["cpu_baseline"] = {"value": round(n * mpxd / el_s, 3), "unit": "Mpx*disp/s", "cores": 1,
-                               "kind": "port", "sample": "%d full pipelines of the same %dx%d d%d pair "
-                               "(oracle, 1 thread, %.1f s)" % (n, W, H, dmax + 1, el_s)}
+        res["cpu_baseline"] = {"value": round(cores * mpxd / el_all, 3), "unit": "Mpx*disp/s", "cores": cores,
+                               "kind": "port", "sample": "%d full pipelines of the same %dx%d d%d pair at once, one "
+                               "per logical CPU (oracle, %d processes, %.1f s)" % (cores, W, H, dmax + 1, cores, el_all),
+                               "single_core": {"value": round(n * mpxd / el_s, 3), "cores": 1,
+                                               "sample": "%d full pipelines back to back (%.1f s)" % (n, el_s)}}
     print(json.dumps(res), flush=True)
 
 

@@ -408,13 +408,15 @@ int elas_ctx::slot_prepare(ElasSlot& S, const ElasPairSpec* sp, int n) {
         P.nsp = S.dnsp + i;
         // device addresses in the mapped pinned staging: the support points
         // and their count again (read by the host step without a copy), and
-        // the meshes (written by the host step, read by the plane kernel
-        // without a copy; the right mesh follows the left one)
+        // the meshes (written there by the host step's Triangle workers, read
+        // by the plane kernel without a copy; the right mesh tricap
+        // triangles after the left one)
         auto hdev = [&](const void* h) { return S.pin_dev + ((const char*)h - (const char*)S.pin); };
         P.hsp[0] = (float*)hdev(S.hsp + S.sp_off[i]);
         P.hsp[1] = (float*)hdev(S.hsp + S.sp_off[i] + 2 * d.ncan);
         P.hnsp = (int32_t*)hdev(S.hnsp + i);
-        P.tri[0] = P.tri[1] = (const int32_t*)hdev(S.htri + S.tri_off[i]);
+        P.tri[0] = (const int32_t*)hdev(S.htri + S.tri_off[i]);
+        P.tri[1] = P.tri[0] + 3 * (size_t)d.tricap;
         for (int k = 0; k < 2; k++) P.gflag[k] = (uint32_t*)take(4 * d.ncell * d.words);
         for (int k = 0; k < 2; k++) P.gmask[k] = (uint32_t*)take(4 * d.ncell * d.words);
         if (debug) {
@@ -603,7 +605,6 @@ int elas_ctx::slot_start_tri(ElasSlot& S) {
     // pinned staging; phase A is complete, so they are all there
     for (int i = 0; i < n; i++) S.nsp[i] = S.dd[i].nblk > 0 ? S.hnsp[i] : 0;
     S.t_sp = clk::now();
-    S.tri_out.assign(2 * (size_t)n, {});
     S.jobs.clear();
     S.job_of.assign(2 * (size_t)n, -1);
     for (int i = 0; i < n; i++) {
@@ -613,7 +614,9 @@ int elas_ctx::slot_start_tri(ElasSlot& S) {
             ElasTriJob j;
             j.pts = S.hsp + S.sp_off[i] + (size_t)k * 2 * S.dd[i].ncan;
             j.n = S.nsp[i];
-            j.out = &S.tri_out[2 * i + k];
+            j.out = nullptr;
+            j.dst = S.htri + S.tri_off[i] + (size_t)k * 3 * S.dd[i].tricap;
+            j.cap = S.dd[i].tricap;
             j.result = 0;
             S.jobs.push_back(j);
         }
@@ -637,7 +640,6 @@ int elas_ctx::slot_prelaunch_b(ElasSlot& S) {
     const int n = S.n;
     for (int i = 0; i < n; i++) {
         S.pairs[i].ntri[0] = S.pairs[i].ntri[1] = 0;
-        S.pairs[i].tri[1] = S.pairs[i].tri[0];
     }
     build_prefix(prm, S);
     // Nothing between the gate and its release may wait for the stream: no
@@ -696,15 +698,6 @@ int elas_ctx::slot_phase_b(ElasSlot& S) {
         S.ntri1[i] = nt[1];
         S.pairs[i].ntri[0] = nt[0];
         S.pairs[i].ntri[1] = nt[1];
-        // both meshes back to back into the pair's region of the mapped
-        // pinned staging, which the plane kernel reads directly; the right
-        // mesh starts after the left one
-        if (nt[0] + nt[1] > 0) {
-            int32_t* dst = S.htri + S.tri_off[i];
-            for (int k = 0; k < 2; k++)
-                if (nt[k] > 0) memcpy(dst + (k ? 3 * (size_t)nt[0] : 0), S.tri_out[2 * i + k].data(), 12 * (size_t)nt[k]);
-        }
-        S.pairs[i].tri[1] = S.pairs[i].tri[0] + 3 * (size_t)nt[0];
     }
     build_prefix(prm, S);
     // behind an armed gate the real table goes to its own pinned copy: a gate

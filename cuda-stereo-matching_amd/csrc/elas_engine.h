@@ -94,7 +94,6 @@ struct ElasSlot {
     int32_t* hnsp = nullptr;
     float* hsp = nullptr;
     int32_t* htri = nullptr;
-    std::vector<std::vector<int32_t>> tri_out;
     std::vector<ElasTriJob> jobs;
     std::vector<int> job_of;
     ElasTriTicket* ticket = nullptr;

@@ -1,28 +1,57 @@
 // elas_tri_unit.cpp -- one Triangle instance, compiled with -DTRILIB=elas_triK.
 //
 // Includes the reference's src/triangle.cpp unmodified (found through -I) and
-// adds one driver to its namespace.  elas_triangulate_arranged() takes the
+// adds one driver to its namespace (elas_tri_setup, elas_tri_finish).  It takes the
 // steps of triangulate() for the switches "zQB" (src/triangle.cpp:8503-8640:
 // triangleinit, parsecommandline, transfernodes, delaunay, then the element
-// list and triangledeinit) and calls Triangle's own divide-and-conquer
-// (divconqrecurse, removeghosts).  The one step it does not take from
-// Triangle is the vertex order that divconqdelaunay() builds with
-// vertexsort() and alternateaxes() (src/triangle.cpp:6164-6230); the caller
-// passes that order in, or (elas_triangulate_dedup) Triangle's own vertexsort()
-// and duplicate removal run first and only the alternating cuts are built
-// outside.  For distinct points the order is unique: both
-// routines compare by (axis, other axis), a strict total order, each cut
-// keeps the `divider` smallest points on its left, and every 2- or 3-point
-// leaf ends sorted by (x, y).  Their random pivots only change how fast the
-// order is reached, and the divide-and-conquer path draws no other random
-// numbers, so the mesh and its triangle numbering are those of triangulate().
-#include "triangle.cpp"
+// list and triangledeinit) and runs Triangle's own divide-and-conquer
+// (divconqrecurse, mergehulls, removeghosts).  Two steps are its own:
+//
+//  * The vertex order that divconqdelaunay() builds with vertexsort() and
+//    alternateaxes() (src/triangle.cpp:6164-6230) comes from ElasTriOrder:
+//    from counting sorts for distinct integer points, or, for other sets,
+//    after Triangle's own vertexsort() and duplicate removal, which decide
+//    which copy of a repeated point survives.  For distinct points the order
+//    is unique: both routines compare by (axis, other axis), a strict total
+//    order, each cut keeps the `divider` smallest points on its left, and
+//    every 2- or 3-point leaf ends sorted by (x, y).  Their random pivots only
+//    change how fast the order is reached, and the divide-and-conquer path
+//    draws no other random numbers.
+//
+//  * The merge runs split over threads.  divconqrecurse() on n vertices
+//    triangulates the first n/2 of the order, then the rest, then merges the
+//    two (src/triangle.cpp:6092-6104), so the triangles are allocated in
+//    post-order: left subtree, right subtree, the merge's own; only
+//    removeghosts() frees triangles, after the whole merge.  The top `depth`
+//    levels of that recursion are cut out: every node of the cut gets its
+//    own triangle pool (sharing the mesh's "outer space" triangle, which the
+//    merge never writes), the leaves run divconqrecurse() concurrently, the
+//    internal nodes of a level run mergehulls() concurrently once the level
+//    below is done, and the triangles are read out pool by pool in
+//    post-order, each pool in allocation order, skipping the dead ghosts --
+//    the order in which writeelements() meets them in the single pool of a
+//    serial run.  Vertex handles, orientations and the merge arithmetic are
+//    Triangle's own, so the triangle list, numbering included, is that of
+//    triangulate().  Triangle's namespace-scope state (the exact-arithmetic
+//    bounds, the RNG) is written only by triangleinit() on the calling
+//    thread, before any task starts, and only read during the merge.
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
 
-#include <stdint.h>
-
+#include <algorithm>
 #include <vector>
 
 #include "elas_triangle.h"
+
+// Triangle allocates only in trimalloc() and frees only in trifree()
+// (src/triangle.cpp:1225-1240); both go to the block cache.
+#define malloc(bytes) elas_tri_block_alloc(bytes)
+#define free(ptr) elas_tri_block_free(ptr)
+#include "triangle.cpp"
+#undef malloc
+#undef free
 
 namespace TRILIB {
 
@@ -45,250 +74,237 @@ vertex* elas_begin(struct mesh& m, struct behavior& b, const float* pts, int n) 
     return byinput;
 }
 
-// The merge on the arranged vertices (divconqdelaunay(), :6222-6230), then
-// the element list.  triangulate() numbers the vertices in writenodes();
-// without the point list numbernodes() gives the same numbers (no -j: every
-// vertex counts).
-int elas_finish(struct mesh& m, struct behavior& b, vertex* sortarray, int count, int** trilist) {
-    struct otri hullleft, hullright;
-    divconqrecurse(&m, &b, sortarray, count, 0, &hullleft, &hullright);
-    trifree((int*)sortarray);
-    const long hull = removeghosts(&m, &b, &hullleft);
-    m.hullsize = m.triangles.items == 0 ? 0l : hull;
-    numbernodes(&m, &b);
-    float* attr = (float*)NULL;
-    *trilist = (int*)NULL;
-    writeelements(&m, &b, trilist, &attr);
-    const int nt = (int)m.triangles.items;
-    triangledeinit(&m, &b);
+struct SplitNode {
+    int lo, size, axis, level;
+    int left, right;  // child nodes, -1 for a leaf
+    struct mesh sub;  // the node's own triangle pool (the root uses the caller's mesh)
+    struct otri farleft, farright;
+    std::vector<int32_t> tris;  // its live triangles, read out of its pool
+    size_t off;                 // their place in the element list (ints)
+};
+
+struct SplitRun {
+    struct mesh* m;
+    struct behavior* b;
+    vertex* vert;       // vertices by the order's point index
+    vertex* sortarray;  // the divide-and-conquer order (filled by the leaves)
+    ElasTriOrder* order;
+    std::vector<SplitNode> nodes;
+    int root;
+    std::vector<int> todo;  // nodes of the current task set
+    int32_t* dst;
+};
+
+// post-order build of the cut: nodes of more than 3 vertices above `depth`
+// split as divconqrecurse() (and ElasTriOrder::cut) split them
+int split_build(std::vector<SplitNode>& nodes, int lo, int size, int axis, int level, int depth) {
+    SplitNode nd;
+    nd.lo = lo;
+    nd.size = size;
+    nd.axis = axis;
+    nd.level = level;
+    nd.left = nd.right = -1;
+    nd.off = 0;
+    if (depth > 0 && size > 3) {
+        const int divider = size >> 1;
+        nd.left = split_build(nodes, lo, divider, 1 - axis, level + 1, depth - 1);
+        nd.right = split_build(nodes, lo + divider, size - divider, 1 - axis, level + 1, depth - 1);
+    }
+    nodes.push_back(std::move(nd));
+    return (int)nodes.size() - 1;
+}
+
+struct mesh* node_mesh(SplitRun* r, int i) { return i == r->root ? r->m : &r->nodes[i].sub; }
+
+// a node's own pool: the mesh's sizes and flags, its dummytri, room for about
+// two triangles per vertex in the first block
+void node_pool(SplitRun* r, int i) {
+    if (i == r->root) return;
+    SplitNode& nd = r->nodes[i];
+    nd.sub = *r->m;
+    poolinit(&nd.sub.triangles, r->m->triangles.itembytes, TRIPERBLOCK, 2 * nd.size + 8,
+             r->m->triangles.alignbytes);
+}
+
+// a leaf: the cuts below the split, its part of the order, its triangulation
+void split_leaf(void* arg, int t) {
+    SplitRun* r = (SplitRun*)arg;
+    const int i = r->todo[t];
+    SplitNode& nd = r->nodes[i];
+    r->order->cut(nd.lo, nd.size, nd.axis, -1);
+    const int32_t* arr = r->order->arr.data();
+    for (int k = nd.lo; k < nd.lo + nd.size; k++) r->sortarray[k] = r->vert[arr[k]];
+    node_pool(r, i);
+    divconqrecurse(node_mesh(r, i), r->b, r->sortarray + nd.lo, nd.size, nd.axis, &nd.farleft, &nd.farright);
+}
+
+// an internal node: mergehulls() of its children's triangulations, as the
+// tail of divconqrecurse() does it
+void split_merge(void* arg, int t) {
+    SplitRun* r = (SplitRun*)arg;
+    const int i = r->todo[t];
+    SplitNode& nd = r->nodes[i];
+    node_pool(r, i);
+    struct otri farleft = r->nodes[nd.left].farleft, innerleft = r->nodes[nd.left].farright;
+    struct otri innerright = r->nodes[nd.right].farleft, farright = r->nodes[nd.right].farright;
+    mergehulls(node_mesh(r, i), r->b, &farleft, &innerleft, &innerright, &farright, nd.axis);
+    nd.farleft = farleft;
+    nd.farright = farright;
+}
+
+// a node's live triangles in allocation order, corners as writeelements()
+// numbers them (src/triangle.cpp:7804-7860, order 1)
+void split_read(void* arg, int t) {
+    SplitRun* r = (SplitRun*)arg;
+    struct mesh* m = r->m;  // Triangle's vertex macros name the mesh `m`
+    SplitNode& nd = r->nodes[t];
+    struct memorypool* pool = &node_mesh(r, t)->triangles;
+    nd.tris.clear();
+    nd.tris.reserve(6 * (size_t)nd.size + 6);
+    traversalinit(pool);
+    struct otri tri;
+    tri.orient = 0;
+    vertex p1, p2, p3;
+    while ((tri.tri = (triangle*)traverse(pool)) != (triangle*)NULL) {
+        if (deadtri(tri.tri)) continue;
+        org(tri, p1);
+        dest(tri, p2);
+        apex(tri, p3);
+        nd.tris.push_back(vertexmark(p1));
+        nd.tris.push_back(vertexmark(p2));
+        nd.tris.push_back(vertexmark(p3));
+    }
+    if (t != r->root) pooldeinit(pool);
+}
+
+void split_copy(void* arg, int t) {
+    SplitRun* r = (SplitRun*)arg;
+    const SplitNode& nd = r->nodes[t];
+    if (!nd.tris.empty()) memcpy(r->dst + nd.off, nd.tris.data(), 4 * nd.tris.size());
+}
+
+// The merge over `vert` (count vertices; order->lx / ly hold their sort
+// orders) cut into at most 2^depth subtrees, the element list into *out, or
+// into dst when it holds cap triangles; returns ntri.
+int elas_finish_split(struct mesh& mesh, struct behavior& b, vertex* vert, int count, ElasTriOrder* order,
+                      int depth, ElasParallelFor par, void* parctx, std::vector<int32_t>* out, int32_t* dst,
+                      int64_t cap) {
+    SplitRun r;
+    r.m = &mesh;
+    r.b = &b;
+    r.vert = vert;
+    r.order = order;
+    r.sortarray = (vertex*)trimalloc(count * (int)sizeof(vertex));
+    r.nodes.reserve(2u << std::min(depth, 12));
+    r.root = split_build(r.nodes, 0, count, 0, 0, depth);
+    r.dst = nullptr;
+    // the top of the order on this thread (the cuts the split nodes make),
+    // then the leaves, then the merges level by level up to the root
+    order->cut(0, count, 0, depth);
+    int deepest = 0;
+    for (const SplitNode& nd : r.nodes) deepest = std::max(deepest, nd.level);
+    r.todo.clear();
+    for (int i = 0; i < (int)r.nodes.size(); i++)
+        if (r.nodes[i].left < 0) r.todo.push_back(i);
+    par(parctx, (int)r.todo.size(), split_leaf, &r);
+    for (int lv = deepest - 1; lv >= 0; lv--) {
+        r.todo.clear();
+        for (int i = 0; i < (int)r.nodes.size(); i++)
+            if (r.nodes[i].left >= 0 && r.nodes[i].level == lv) r.todo.push_back(i);
+        if (!r.todo.empty()) par(parctx, (int)r.todo.size(), split_merge, &r);
+    }
+    trifree((int*)r.sortarray);
+    removeghosts(&mesh, &b, &r.nodes[r.root].farleft);
+    numbernodes(&mesh, &b);
+    par(parctx, (int)r.nodes.size(), split_read, &r);
+    size_t total = 0;
+    for (SplitNode& nd : r.nodes) {
+        nd.off = total;
+        total += nd.tris.size();
+    }
+    const int nt = (int)(total / 3);
+    if (dst) {
+        if (nt <= cap) {
+            r.dst = dst;
+            par(parctx, (int)r.nodes.size(), split_copy, &r);
+        }
+    } else {
+        out->resize(total);
+        r.dst = out->data();
+        par(parctx, (int)r.nodes.size(), split_copy, &r);
+    }
+    triangledeinit(&mesh, &b);
     return nt;
 }
 
 }  // namespace
 
-// pts: [n][2], n >= 3, distinct; arr: the divide-and-conquer vertex order as
-// input indices.  *trilist receives a malloc'ed [ntri][3] zero-based list
-// (free() it); returns ntri.
-int elas_triangulate_arranged(const float* pts, int n, const int* arr, int** trilist) {
-    struct mesh m;
-    struct behavior b;
-    vertex* byinput = elas_begin(m, b, pts, n);
-    vertex* sortarray = (vertex*)trimalloc(n * (int)sizeof(vertex));
-    for (int k = 0; k < n; k++) sortarray[k] = byinput[arr[k]];
-    trifree((int*)byinput);
-    return elas_finish(m, b, sortarray, n, trilist);
-}
-
-// Any point set (n >= 3): Triangle's own vertexsort() on the caller's order
-// and its duplicate removal (:6176-6198), which decide which copy of a
-// repeated point survives; then the alternating-cut order of the survivors
-// from `arrange`, which gets their coordinates in (x, y) order and returns
-// positions into that list.
 namespace {
 
-// elas_begin(), Triangle's sort and duplicate removal, then the cuts of the
-// survivors; returns the arranged vertices and their count, or NULL (mesh
-// released) when a single distinct point is left.
-vertex* elas_dedup_arrange(struct mesh& mesh, struct behavior& b, const float* pts, int n,
-                           bool (*arrange)(const float* xy, int count, int* arr), int* count_out) {
-    struct mesh* m = &mesh;  // Triangle's vertex macros name the mesh `m`
-    vertex* sorted = elas_begin(mesh, b, pts, n);
-    vertexsort(sorted, n);
-    int i = 0;
-    for (int j = 1; j < n; j++) {
-        if (sorted[i][0] == sorted[j][0] && sorted[i][1] == sorted[j][1]) {
-            setvertextype(sorted[j], UNDEADVERTEX);
-            m->undeads++;
-        } else {
-            i++;
-            sorted[i] = sorted[j];
-        }
-    }
-    const int count = i + 1;
-    if (count < 2) {
-        // every point is the same point: Triangle's divconqrecurse() has no
-        // case for a single vertex and recurses until the stack overflows;
-        // report it as too few points instead (two distinct points give an
-        // empty mesh, as in Triangle)
-        trifree((int*)sorted);
-        triangledeinit(&mesh, &b);
-        return (vertex*)NULL;
-    }
-    float* xy = (float*)trimalloc(2 * count * (int)sizeof(float));
-    int* arr = (int*)trimalloc(count * (int)sizeof(int));
-    for (int k = 0; k < count; k++) {
-        xy[2 * k] = sorted[k][0];
-        xy[2 * k + 1] = sorted[k][1];
-    }
-    arrange(xy, count, arr);
-    vertex* sortarray = (vertex*)trimalloc(count * (int)sizeof(vertex));
-    for (int k = 0; k < count; k++) sortarray[k] = sorted[arr[k]];
-    trifree((int*)xy);
-    trifree(arr);
-    trifree((int*)sorted);
-    *count_out = count;
-    return sortarray;
-}
+struct Setup {
+    struct mesh mesh;
+    struct behavior b;
+    vertex* byinput;
+    int n;
+};
 
 }  // namespace
 
-int elas_triangulate_dedup(const float* pts, int n, bool (*arrange)(const float* xy, int count, int* arr),
-                           int** trilist) {
-    struct mesh mesh;
-    struct behavior b;
-    int count = 0;
-    vertex* sortarray = elas_dedup_arrange(mesh, b, pts, n, arrange, &count);
-    *trilist = (int*)NULL;
-    if (!sortarray) return -1;
-    return elas_finish(mesh, b, sortarray, count, trilist);
+// Triangle's setup of a "zQB" triangulation of n >= 3 points [x0, y0, ...]
+// (any thread); elas_tri_finish() takes it over.
+void* elas_tri_setup(const float* pts, int n) {
+    Setup* st = new Setup();
+    st->byinput = elas_begin(st->mesh, st->b, pts, n);
+    st->n = n;
+    return st;
 }
 
-
-// ---- the divide-and-conquer split over threads ------------------------------
-//
-// divconqrecurse() on n vertices triangulates the first n/2 of the order,
-// then the rest, then merges the two (src/triangle.cpp:6092-6104), so the
-// triangles are allocated in post-order: left subtree, right subtree, the
-// merge's own.  Only removeghosts() frees triangles, after the whole merge.
-// The top `depth` levels of that recursion are cut out here: every node of
-// the cut gets its own triangle pool (sharing the mesh's "outer space"
-// triangle), the leaves run divconqrecurse() concurrently, the internal nodes
-// run mergehulls() once both children are done, and the triangles are read
-// out pool by pool in post-order, each pool in allocation order, skipping
-// the dead ghosts -- the order in which writeelements() meets them in the
-// single pool of a serial run.  Vertex handles, orientations and the merge
-// arithmetic are Triangle's own, so the triangle list is that of
-// triangulate().  Triangle's namespace-scope state (the exact-arithmetic
-// bounds, the RNG) is written only by triangleinit() on the calling thread,
-// before any task starts, and read-only during the merge.
-
-namespace {
-
-struct SplitNode {
-    int lo, size, axis;
-    int left, right;  // child nodes, -1 for a leaf
-    struct mesh sub;  // the node's own triangle pool (the root uses the caller's mesh)
-    struct otri farleft, farright;
-};
-
-struct SplitRun {
-    struct behavior* b;
-    vertex* sortarray;
-    std::vector<SplitNode>* nodes;
-    std::vector<int> leaves;
-};
-
-// post-order build of the cut: nodes of more than 3 vertices above `depth`
-// split as divconqrecurse() splits them
-int split_build(std::vector<SplitNode>& nodes, int lo, int size, int axis, int depth) {
-    SplitNode nd;
-    nd.lo = lo;
-    nd.size = size;
-    nd.axis = axis;
-    nd.left = nd.right = -1;
-    if (depth > 0 && size > 3) {
-        const int divider = size >> 1;
-        nd.left = split_build(nodes, lo, divider, 1 - axis, depth - 1);
-        nd.right = split_build(nodes, lo + divider, size - divider, 1 - axis, depth - 1);
+// The triangulation, corners numbered by input position, into *out (or dst,
+// see elas_finish_split).  `order` is scratch; with `integer` set its lx / ly
+// already hold the input points' sort orders (elas_tri_order_integer), else
+// Triangle's vertexsort() and duplicate removal (:6176-6198) run first and
+// the order is built over the survivors.  Returns ntri, or -1 when a single
+// distinct point is left: Triangle's divconqrecurse() has no case for one
+// vertex and recurses until the stack overflows (two distinct points give an
+// empty mesh, as in Triangle).
+int elas_tri_finish(void* setup, ElasTriOrder* order, bool integer, int depth, ElasParallelFor par, void* parctx,
+                    std::vector<int32_t>* out, int32_t* dst, int64_t cap) {
+    Setup* st = (Setup*)setup;
+    struct mesh* m = &st->mesh;  // Triangle's vertex macros name the mesh `m`
+    vertex* byinput = st->byinput;
+    const int n = st->n;
+    int count = n, nt = -1;
+    if (!integer) {
+        vertexsort(byinput, n);
+        int i = 0;
+        for (int j = 1; j < n; j++) {
+            if (byinput[i][0] == byinput[j][0] && byinput[i][1] == byinput[j][1]) {
+                setvertextype(byinput[j], UNDEADVERTEX);
+                m->undeads++;
+            } else {
+                i++;
+                byinput[i] = byinput[j];
+            }
+        }
+        count = i + 1;
+        if (count >= 2) {
+            std::vector<float> xy(2 * (size_t)count);
+            for (int k = 0; k < count; k++) {
+                xy[2 * k] = byinput[k][0];
+                xy[2 * k + 1] = byinput[k][1];
+            }
+            elas_tri_order_sorted(xy.data(), count, *order);
+        }
     }
-    nodes.push_back(nd);
-    return (int)nodes.size() - 1;
-}
-
-void split_leaf(void* arg, int i) {
-    SplitRun* r = (SplitRun*)arg;
-    SplitNode& nd = (*r->nodes)[r->leaves[i]];
-    divconqrecurse(&nd.sub, r->b, r->sortarray + nd.lo, nd.size, nd.axis, &nd.farleft, &nd.farright);
-}
-
-// The merge on the arranged vertices cut into at most 2^depth subtrees, then
-// the element list into *out ([ntri][3]); returns ntri.  depth 0 is the serial
-// divconqrecurse().
-int elas_finish_split(struct mesh& mesh, struct behavior& b, vertex* sortarray, int count, int depth,
-                      ElasParallelFor par, void* parctx, std::vector<int32_t>* out) {
-    struct mesh* m = &mesh;  // Triangle's vertex macros name the mesh `m`
-    std::vector<SplitNode> nodes;
-    nodes.reserve(2u << std::min(depth, 12));
-    const int root = split_build(nodes, 0, count, 0, depth);
-    SplitRun run;
-    run.b = &b;
-    run.sortarray = sortarray;
-    run.nodes = &nodes;
-    for (int i = 0; i < (int)nodes.size(); i++) {
-        SplitNode& nd = nodes[i];
-        if (i == root) continue;
-        nd.sub = mesh;  // dummytri, sizes and flags of the mesh; a pool of its own:
-        // room for the node's triangles (about 2 per vertex) in the first block
-        poolinit(&nd.sub.triangles, mesh.triangles.itembytes, TRIPERBLOCK, 2 * nd.size + 8,
-                 mesh.triangles.alignbytes);
-        if (nd.left < 0) run.leaves.push_back(i);
-    }
-    if (nodes[root].left < 0) {
-        nodes[root].sub = mesh;
-        divconqrecurse(m, &b, sortarray, count, 0, &nodes[root].farleft, &nodes[root].farright);
+    if (count >= 2) {
+        nt = elas_finish_split(st->mesh, st->b, byinput, count, order, depth, par, parctx, out, dst, cap);
     } else {
-        par(parctx, (int)run.leaves.size(), split_leaf, &run);
-        for (int i = 0; i < (int)nodes.size(); i++) {  // post-order: children before parents
-            SplitNode& nd = nodes[i];
-            if (nd.left < 0) continue;
-            struct otri farleft = nodes[nd.left].farleft, innerleft = nodes[nd.left].farright;
-            struct otri innerright = nodes[nd.right].farleft, farright = nodes[nd.right].farright;
-            mergehulls(i == root ? m : &nd.sub, &b, &farleft, &innerleft, &innerright, &farright, nd.axis);
-            nd.farleft = farleft;
-            nd.farright = farright;
-        }
+        triangledeinit(&st->mesh, &st->b);
+        if (out) out->clear();
     }
-    trifree((int*)sortarray);
-    removeghosts(m, &b, &nodes[root].farleft);
-    numbernodes(m, &b);
-    out->clear();
-    out->reserve(6 * (size_t)count + 6);
-    for (int i = 0; i < (int)nodes.size(); i++) {
-        struct memorypool* pool = i == root ? &mesh.triangles : &nodes[i].sub.triangles;
-        traversalinit(pool);
-        struct otri t;
-        t.orient = 0;
-        vertex p1, p2, p3;
-        while ((t.tri = (triangle*)traverse(pool)) != (triangle*)NULL) {
-            if (deadtri(t.tri)) continue;
-            org(t, p1);
-            dest(t, p2);
-            apex(t, p3);
-            out->push_back(vertexmark(p1));
-            out->push_back(vertexmark(p2));
-            out->push_back(vertexmark(p3));
-        }
-        if (i != root) pooldeinit(pool);
-    }
-    triangledeinit(&mesh, &b);
-    return (int)(out->size() / 3);
-}
-
-}  // namespace
-
-// elas_triangulate_arranged() with the merge cut into at most 2^depth
-// concurrent subtrees run through `par`; the triangles go to *out.
-int elas_triangulate_split(const float* pts, int n, const int* arr, int depth, ElasParallelFor par, void* parctx,
-                           std::vector<int32_t>* out) {
-    struct mesh m;
-    struct behavior b;
-    vertex* byinput = elas_begin(m, b, pts, n);
-    vertex* sortarray = (vertex*)trimalloc(n * (int)sizeof(vertex));
-    for (int k = 0; k < n; k++) sortarray[k] = byinput[arr[k]];
     trifree((int*)byinput);
-    return elas_finish_split(m, b, sortarray, n, depth, par, parctx, out);
-}
-
-// elas_triangulate_dedup() with the split merge; -1 for a single distinct point.
-int elas_triangulate_dedup_split(const float* pts, int n, bool (*arrange)(const float* xy, int count, int* arr),
-                                 int depth, ElasParallelFor par, void* parctx, std::vector<int32_t>* out) {
-    struct mesh mesh;
-    struct behavior b;
-    int count = 0;
-    vertex* sortarray = elas_dedup_arrange(mesh, b, pts, n, arrange, &count);
-    out->clear();
-    if (!sortarray) return -1;
-    return elas_finish_split(mesh, b, sortarray, count, depth, par, parctx, out);
+    delete st;
+    return nt;
 }
 
 }  // namespace TRILIB

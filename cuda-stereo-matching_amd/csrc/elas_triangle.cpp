@@ -18,6 +18,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <deque>
+#include <sched.h>
 #include <mutex>
 #include <thread>
 #include <vector>
@@ -73,16 +74,11 @@
 #include "triangle.h"
 #undef TRILIB
 
-#define ELAS_TRI_DECL(NS)                                                                          \
-    namespace NS {                                                                                 \
-    int elas_triangulate_arranged(const float* pts, int n, const int* arr, int** trilist);         \
-    int elas_triangulate_dedup(const float* pts, int n, bool (*arrange)(const float*, int, int*), \
-                               int** trilist);                                                     \
-    int elas_triangulate_split(const float* pts, int n, const int* arr, int depth,                 \
-                               ElasParallelFor par, void* parctx, std::vector<int32_t>* out);      \
-    int elas_triangulate_dedup_split(const float* pts, int n,                                      \
-                                     bool (*arrange)(const float*, int, int*), int depth,          \
-                                     ElasParallelFor par, void* parctx, std::vector<int32_t>* out); \
+#define ELAS_TRI_DECL(NS)                                                                             \
+    namespace NS {                                                                                    \
+    void* elas_tri_setup(const float* pts, int n);                                                    \
+    int elas_tri_finish(void* setup, ElasTriOrder* order, bool integer, int depth, ElasParallelFor par, \
+                        void* parctx, std::vector<int32_t>* out, int32_t* dst, int64_t cap);          \
     }
 ELAS_TRI_DECL(elas_tri0) ELAS_TRI_DECL(elas_tri1) ELAS_TRI_DECL(elas_tri2) ELAS_TRI_DECL(elas_tri3)
 ELAS_TRI_DECL(elas_tri4) ELAS_TRI_DECL(elas_tri5) ELAS_TRI_DECL(elas_tri6) ELAS_TRI_DECL(elas_tri7)
@@ -91,58 +87,26 @@ ELAS_TRI_DECL(elas_tri12) ELAS_TRI_DECL(elas_tri13) ELAS_TRI_DECL(elas_tri14) EL
 
 namespace {
 
-typedef int (*DedupFn)(const float*, int, bool (*)(const float*, int, int*), int**);
-const DedupFn kDedup[ELAS_TRI_INSTANCES] = {
-    elas_tri0::elas_triangulate_dedup, elas_tri1::elas_triangulate_dedup,
-    elas_tri2::elas_triangulate_dedup, elas_tri3::elas_triangulate_dedup,
-    elas_tri4::elas_triangulate_dedup, elas_tri5::elas_triangulate_dedup,
-    elas_tri6::elas_triangulate_dedup, elas_tri7::elas_triangulate_dedup,
-    elas_tri8::elas_triangulate_dedup, elas_tri9::elas_triangulate_dedup,
-    elas_tri10::elas_triangulate_dedup, elas_tri11::elas_triangulate_dedup,
-    elas_tri12::elas_triangulate_dedup, elas_tri13::elas_triangulate_dedup,
-    elas_tri14::elas_triangulate_dedup, elas_tri15::elas_triangulate_dedup};
+typedef void* (*SetupFn)(const float*, int);
+const SetupFn kSetup[ELAS_TRI_INSTANCES] = {
+    elas_tri0::elas_tri_setup,  elas_tri1::elas_tri_setup,  elas_tri2::elas_tri_setup,  elas_tri3::elas_tri_setup,
+    elas_tri4::elas_tri_setup,  elas_tri5::elas_tri_setup,  elas_tri6::elas_tri_setup,  elas_tri7::elas_tri_setup,
+    elas_tri8::elas_tri_setup,  elas_tri9::elas_tri_setup,  elas_tri10::elas_tri_setup, elas_tri11::elas_tri_setup,
+    elas_tri12::elas_tri_setup, elas_tri13::elas_tri_setup, elas_tri14::elas_tri_setup, elas_tri15::elas_tri_setup};
 
-typedef int (*ArrFn)(const float*, int, const int*, int**);
-const ArrFn kArr[ELAS_TRI_INSTANCES] = {
-    elas_tri0::elas_triangulate_arranged, elas_tri1::elas_triangulate_arranged,
-    elas_tri2::elas_triangulate_arranged, elas_tri3::elas_triangulate_arranged,
-    elas_tri4::elas_triangulate_arranged, elas_tri5::elas_triangulate_arranged,
-    elas_tri6::elas_triangulate_arranged, elas_tri7::elas_triangulate_arranged,
-    elas_tri8::elas_triangulate_arranged, elas_tri9::elas_triangulate_arranged,
-    elas_tri10::elas_triangulate_arranged, elas_tri11::elas_triangulate_arranged,
-    elas_tri12::elas_triangulate_arranged, elas_tri13::elas_triangulate_arranged,
-    elas_tri14::elas_triangulate_arranged, elas_tri15::elas_triangulate_arranged};
-
-typedef int (*SplitFn)(const float*, int, const int*, int, ElasParallelFor, void*, std::vector<int32_t>*);
-const SplitFn kSplit[ELAS_TRI_INSTANCES] = {
-    elas_tri0::elas_triangulate_split, elas_tri1::elas_triangulate_split,
-    elas_tri2::elas_triangulate_split, elas_tri3::elas_triangulate_split,
-    elas_tri4::elas_triangulate_split, elas_tri5::elas_triangulate_split,
-    elas_tri6::elas_triangulate_split, elas_tri7::elas_triangulate_split,
-    elas_tri8::elas_triangulate_split, elas_tri9::elas_triangulate_split,
-    elas_tri10::elas_triangulate_split, elas_tri11::elas_triangulate_split,
-    elas_tri12::elas_triangulate_split, elas_tri13::elas_triangulate_split,
-    elas_tri14::elas_triangulate_split, elas_tri15::elas_triangulate_split};
-
-typedef int (*DedupSplitFn)(const float*, int, bool (*)(const float*, int, int*), int, ElasParallelFor, void*,
-                            std::vector<int32_t>*);
-const DedupSplitFn kDedupSplit[ELAS_TRI_INSTANCES] = {
-    elas_tri0::elas_triangulate_dedup_split, elas_tri1::elas_triangulate_dedup_split,
-    elas_tri2::elas_triangulate_dedup_split, elas_tri3::elas_triangulate_dedup_split,
-    elas_tri4::elas_triangulate_dedup_split, elas_tri5::elas_triangulate_dedup_split,
-    elas_tri6::elas_triangulate_dedup_split, elas_tri7::elas_triangulate_dedup_split,
-    elas_tri8::elas_triangulate_dedup_split, elas_tri9::elas_triangulate_dedup_split,
-    elas_tri10::elas_triangulate_dedup_split, elas_tri11::elas_triangulate_dedup_split,
-    elas_tri12::elas_triangulate_dedup_split, elas_tri13::elas_triangulate_dedup_split,
-    elas_tri14::elas_triangulate_dedup_split, elas_tri15::elas_triangulate_dedup_split};
+typedef int (*FinishFn)(void*, ElasTriOrder*, bool, int, ElasParallelFor, void*, std::vector<int32_t>*, int32_t*,
+                        int64_t);
+const FinishFn kFinish[ELAS_TRI_INSTANCES] = {
+    elas_tri0::elas_tri_finish,  elas_tri1::elas_tri_finish,  elas_tri2::elas_tri_finish,
+    elas_tri3::elas_tri_finish,  elas_tri4::elas_tri_finish,  elas_tri5::elas_tri_finish,
+    elas_tri6::elas_tri_finish,  elas_tri7::elas_tri_finish,  elas_tri8::elas_tri_finish,
+    elas_tri9::elas_tri_finish,  elas_tri10::elas_tri_finish, elas_tri11::elas_tri_finish,
+    elas_tri12::elas_tri_finish, elas_tri13::elas_tri_finish, elas_tri14::elas_tri_finish,
+    elas_tri15::elas_tri_finish};
 
 double now_s() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
-
-}  // namespace
-
-namespace {
 
 // Stable counting sort of the indices in `in` by the integer keys key[]
 // (already offset to 0..range-1).
@@ -154,56 +118,118 @@ void counting_sort(const int32_t* key, int n, long range, const int32_t* in, int
     for (int i = 0; i < n; i++) out[cnt[key[in[i]]]++] = in[i];
 }
 
-// Alternating cuts over a node [lo, lo + size): lx and ly hold the node's
-// points in (x, y) and (y, x) order.  A cut along `axis` keeps the size / 2
-// smallest points of that order on its left (as vertexmedian() does,
-// src/triangle.cpp:5517-5574) and splits the other list stably by side, so
-// both children keep both orders.  Nodes of 2 or 3 points are leaves sorted
-// by (x, y) (alternateaxes(), src/triangle.cpp:5586-5605).
-struct Cuts {
-    int32_t *lx, *ly, *tmp, *out;
-    uint8_t* side;
-    void node(int lo, int size, int axis) {
-        if (size <= 3) {
-            for (int k = 0; k < size; k++) out[lo + k] = lx[lo + k];
-            return;
-        }
-        const int divider = size >> 1;
-        int32_t* cut = axis ? ly : lx;
-        int32_t* other = axis ? lx : ly;
-        for (int k = 0; k < divider; k++) side[cut[lo + k]] = 0;
-        for (int k = divider; k < size; k++) side[cut[lo + k]] = 1;
-        // branch-free stable split: the side of a point is not predictable
-        int a = 0, b = divider;
-        for (int k = 0; k < size; k++) {
-            const int32_t i = other[lo + k];
-            const int r = side[i];
-            tmp[r ? b : a] = i;
-            a += 1 - r;
-            b += r;
-        }
-        memcpy(other + lo, tmp, sizeof(int32_t) * (size_t)size);
-        node(lo, divider, 1 - axis);
-        node(lo + divider, size - divider, 1 - axis);
-    }
+}  // namespace
+
+// ---- block cache ---------------------------------------------------------------
+
+namespace {
+
+#define ELAS_BLOCK_MIN_SHIFT 16                 // 64 KiB
+#define ELAS_BLOCK_CLASSES 16                   // up to 2 GiB
+#define ELAS_BLOCK_CACHE_MAX ((size_t)512 << 20)  // bytes kept for reuse
+#define ELAS_BLOCK_HEADER 64                    // size class in front of a block (keeps 16-B alignment)
+
+struct BlockCache {
+    std::mutex m;
+    std::vector<void*> free_[ELAS_BLOCK_CLASSES];
+    size_t cached = 0;
 };
+
+BlockCache& block_cache() {
+    static BlockCache* c = new BlockCache();  // lives until process exit
+    return *c;
+}
 
 }  // namespace
 
-// Triangle's divide-and-conquer order (divconqdelaunay(), src/triangle.cpp:
-// 6164-6230): the points sorted by (x, y), halved, and each half cut
-// alternately along y and x down to 2- and 3-point leaves.  For distinct
-// points that order is unique, so it can be built here in O(n log n) from two
-// presorted lists instead of by Triangle's randomised quicksort and
-// quickselect, which take about half of a support-point mesh's time.  Support
-// points have integer coordinates, so both lists come from counting sorts.
-// False when a coordinate is not a bounded integer or two points coincide
-// (which duplicate Triangle keeps depends on its own sort); the caller then
-// lets Triangle sort and drop duplicates itself and builds only the cuts.
-bool elas_tri_arrangement(const float* p, int n, std::vector<int32_t>& arr) {
-    if (n < 3) return false;
-    thread_local std::vector<int32_t> cnt, a, b, lx, ly, tmp, key[2];
+void* elas_tri_block_alloc(size_t bytes) {
+    if (bytes + ELAS_BLOCK_HEADER < ((size_t)1 << ELAS_BLOCK_MIN_SHIFT)) {
+        char* p = (char*)malloc(bytes + ELAS_BLOCK_HEADER);
+        if (!p) return nullptr;
+        *(int*)p = -1;
+        return p + ELAS_BLOCK_HEADER;
+    }
+    int cls = 0;
+    while (((size_t)1 << (ELAS_BLOCK_MIN_SHIFT + cls)) < bytes + ELAS_BLOCK_HEADER) cls++;
+    if (cls >= ELAS_BLOCK_CLASSES) return nullptr;  // Triangle reports out of memory
+    char* p = nullptr;
+    {
+        BlockCache& c = block_cache();
+        std::lock_guard<std::mutex> g(c.m);
+        if (!c.free_[cls].empty()) {
+            p = (char*)c.free_[cls].back();
+            c.free_[cls].pop_back();
+            c.cached -= (size_t)1 << (ELAS_BLOCK_MIN_SHIFT + cls);
+        }
+    }
+    if (!p) p = (char*)malloc((size_t)1 << (ELAS_BLOCK_MIN_SHIFT + cls));
+    if (!p) return nullptr;
+    *(int*)p = cls;
+    return p + ELAS_BLOCK_HEADER;
+}
+
+void elas_tri_block_free(void* ptr) {
+    if (!ptr) return;
+    char* p = (char*)ptr - ELAS_BLOCK_HEADER;
+    const int cls = *(int*)p;
+    if (cls >= 0) {
+        const size_t sz = (size_t)1 << (ELAS_BLOCK_MIN_SHIFT + cls);
+        BlockCache& c = block_cache();
+        std::lock_guard<std::mutex> g(c.m);
+        if (c.cached + sz <= ELAS_BLOCK_CACHE_MAX) {
+            c.free_[cls].push_back(p);
+            c.cached += sz;
+            return;
+        }
+    }
+    free(p);
+}
+
+void ElasTriOrder::resize(int n) {
+    lx.resize(n);
+    ly.resize(n);
+    tmp.resize(n);
+    arr.resize(n);
+}
+
+void ElasTriOrder::cut(int lo, int size, int axis, int levels) {
+    if (size <= 3) {
+        for (int k = 0; k < size; k++) arr[lo + k] = lx[lo + k];
+        return;
+    }
+    if (levels == 0) return;
+    const int divider = size >> 1;
+    int32_t* cutl = axis ? ly.data() : lx.data();
+    int32_t* other = axis ? lx.data() : ly.data();
+    // the side marks are per thread: subtrees cut at once mark points whose
+    // indices share cache lines
     thread_local std::vector<uint8_t> side;
+    if (side.size() < lx.size()) side.resize(lx.size());
+    uint8_t* sd = side.data();
+    for (int k = 0; k < divider; k++) sd[cutl[lo + k]] = 0;
+    for (int k = divider; k < size; k++) sd[cutl[lo + k]] = 1;
+    // branch-free stable split of the other list (the side of a point is not
+    // predictable), through the node's own range of tmp
+    int32_t* t = tmp.data() + lo;
+    int a = 0, b = divider;
+    for (int k = 0; k < size; k++) {
+        const int32_t i = other[lo + k];
+        const int r = sd[i];
+        t[r ? b : a] = i;
+        a += 1 - r;
+        b += r;
+    }
+    memcpy(other + lo, t, sizeof(int32_t) * (size_t)size);
+    cut(lo, divider, 1 - axis, levels - 1);
+    cut(lo + divider, size - divider, 1 - axis, levels - 1);
+}
+
+// Support points have integer coordinates, so both sort orders come from
+// counting sorts, in O(n) instead of Triangle's randomised quicksort and
+// quickselect, which take about half of a support-point mesh's serial time.
+bool elas_tri_order_integer(const float* p, int n, ElasTriOrder& o) {
+    if (n < 3) return false;
+    thread_local std::vector<int32_t> cnt, a, b, key[2];
     int lo[2] = {0, 0}, hi[2] = {0, 0};
     for (int c = 0; c < 2; c++) {
         key[c].resize(n);
@@ -223,53 +249,50 @@ bool elas_tri_arrangement(const float* p, int n, std::vector<int32_t>& arr) {
     if (rx > 4L * n + 65536 || ry > 4L * n + 65536) return false;
     for (int c = 0; c < 2; c++)
         for (int i = 0; i < n; i++) key[c][i] -= lo[c];
+    o.resize(n);
     a.resize(n);
     b.resize(n);
-    lx.resize(n);
-    ly.resize(n);
     for (int i = 0; i < n; i++) a[i] = i;
-    counting_sort(key[1].data(), n, ry, a.data(), b.data(), cnt);   // by y
-    counting_sort(key[0].data(), n, rx, b.data(), lx.data(), cnt);  // then x: (x, y)
-    counting_sort(key[0].data(), n, rx, a.data(), b.data(), cnt);   // by x
-    counting_sort(key[1].data(), n, ry, b.data(), ly.data(), cnt);  // then y: (y, x)
+    // support points come row by row (y non-decreasing), the left image's
+    // also in (y, x) order: those passes are the identity
+    bool ysorted = true, yxsorted = true;
+    for (int i = 1; i < n; i++) {
+        const int32_t dy = key[1][i] - key[1][i - 1];
+        ysorted = ysorted && dy >= 0;
+        yxsorted = yxsorted && (dy > 0 || (dy == 0 && key[0][i] > key[0][i - 1]));
+    }
+    if (ysorted) {
+        counting_sort(key[0].data(), n, rx, a.data(), o.lx.data(), cnt);  // by x, stable: (x, y)
+    } else {
+        counting_sort(key[1].data(), n, ry, a.data(), b.data(), cnt);     // by y
+        counting_sort(key[0].data(), n, rx, b.data(), o.lx.data(), cnt);  // then x: (x, y)
+    }
+    if (yxsorted) {
+        o.ly = a;
+    } else {
+        counting_sort(key[0].data(), n, rx, a.data(), b.data(), cnt);     // by x
+        counting_sort(key[1].data(), n, ry, b.data(), o.ly.data(), cnt);  // then y: (y, x)
+    }
     for (int k = 1; k < n; k++) {
-        const int32_t i = lx[k - 1], j = lx[k];
+        const int32_t i = o.lx[k - 1], j = o.lx[k];
         if (key[0][i] == key[0][j] && key[1][i] == key[1][j]) return false;
     }
-    tmp.resize(n);
-    side.resize(n);
-    arr.resize(n);
-    Cuts cuts{lx.data(), ly.data(), tmp.data(), arr.data(), side.data()};
-    cuts.node(0, n, 0);
     return true;
+}
+
+void elas_tri_order_sorted(const float* xy, int count, ElasTriOrder& o) {
+    o.resize(count);
+    for (int i = 0; i < count; i++) o.lx[i] = o.ly[i] = i;
+    // (y, x) order: by y, the (x, y) order kept among equal y
+    std::stable_sort(o.ly.begin(), o.ly.end(), [xy](int32_t a, int32_t b) { return xy[2 * a + 1] < xy[2 * b + 1]; });
 }
 
 namespace {
 
-// One "zQB" Triangle call on copy k.  Corners are numbered by the caller's
-// order either way.
-// The alternating cuts of distinct points given in (x, y) order: positions
-// into xy.  Used after Triangle's own sort and duplicate removal, for sets the
-// counting sorts do not take (duplicates, non-integer coordinates).
-bool arrange_sorted(const float* xy, int count, int* arr) {
-    thread_local std::vector<int32_t> lx, ly, tmp;
-    thread_local std::vector<uint8_t> side;
-    lx.resize(count);
-    ly.resize(count);
-    tmp.resize(count);
-    side.resize(count);
-    for (int i = 0; i < count; i++) lx[i] = ly[i] = i;
-    // (y, x) order: by y, the (x, y) order kept among equal y
-    std::stable_sort(ly.begin(), ly.end(), [xy](int32_t a, int32_t b) { return xy[2 * a + 1] < xy[2 * b + 1]; });
-    Cuts cuts{lx.data(), ly.data(), tmp.data(), arr, side.data()};
-    cuts.node(0, count, 0);
-    return true;
-}
-
 // Split depth of a mesh of n points: the divide-and-conquer is cut into up to
 // 2^depth concurrent subtrees of at least ELAS_TRI_LEAF_MIN points, at most
 // g_split_max levels (elas_tri_set_split; -1 = ELAS_TRI_SPLIT_AUTO).
-#define ELAS_TRI_LEAF_MIN 384
+#define ELAS_TRI_LEAF_MIN 128
 #define ELAS_TRI_SPLIT_AUTO 3
 std::atomic<int> g_split_max{-1};
 
@@ -284,52 +307,61 @@ int split_depth(int n) {
 
 void pool_parallel_for(void* ctx, int ntasks, void (*task)(void*, int), void* arg);
 
-// One "zQB" Triangle call on copy k; with split > 0 the merge's subtrees run
-// through the pool's helpers.  Corners are numbered by the caller's order.
-int tri_call(int k, const float* pts, int n, std::vector<int32_t>& out, bool split, void* pool) {
-    thread_local std::vector<int32_t> arr;
-    const int depth = split ? split_depth(n) : 0;
+// One "zQB" Triangle call on copy k; with split set Triangle's setup and the
+// vertex order are built at once, and the merge's subtrees run through the
+// pool's helpers.  Corners are numbered by the caller's order.
+struct TriCall {
+    int k;
+    const ElasTriJob* job;
+    ElasTriOrder* order;
+    void* setup;
+    bool integer;
+};
+
+void tri_prepare(void* arg, int t) {
+    TriCall* c = (TriCall*)arg;
+    if (t == 0) c->setup = kSetup[c->k](c->job->pts, c->job->n);
+    else c->integer = elas_tri_order_integer(c->job->pts, c->job->n, *c->order);
+}
+
+int tri_call(int k, ElasTriJob& j, bool split, void* pool) {
+    thread_local ElasTriOrder order;
+    const int depth = split ? split_depth(j.n) : 0;
+    TriCall c{k, &j, &order, nullptr, false};
     if (depth > 0) {
-        const int nt = elas_tri_arrangement(pts, n, arr)
-                           ? kSplit[k](pts, n, arr.data(), depth, pool_parallel_for, pool, &out)
-                           : kDedupSplit[k](pts, n, arrange_sorted, depth, pool_parallel_for, pool, &out);
-        if (nt < 0) out.clear();
-        return nt;
+        pool_parallel_for(pool, 2, tri_prepare, &c);
+    } else {
+        tri_prepare(&c, 0);
+        tri_prepare(&c, 1);
     }
-    int* tl = nullptr;
-    const int nt = elas_tri_arrangement(pts, n, arr) ? kArr[k](pts, n, arr.data(), &tl)
-                                                     : kDedup[k](pts, n, arrange_sorted, &tl);
-    if (nt < 0) {  // a single distinct point
-        out.clear();
-        return -1;
-    }
-    out.assign(tl, tl + 3 * (size_t)nt);
-    free(tl);
-    return nt;
+    return kFinish[k](c.setup, &order, c.integer, depth, pool_parallel_for, pool, j.out, j.dst, j.cap);
 }
 
 int run_instance(int k, ElasTriJob& j, bool split, void* pool) {
     j.t_begin = now_s();
     if (j.n < 3) {  // Triangle would exit(1) (src/triangle.cpp:7652-7654)
-        j.out->clear();
+        if (j.out) j.out->clear();
         j.t_end = now_s();
         return -1;
     }
-    const int nt = tri_call(k, j.pts, j.n, *j.out, split, pool);
+    const int nt = tri_call(k, j, split, pool);
     j.t_end = now_s();
     return nt;
 }
 
 // Worker K owns Triangle copy K and runs whole jobs on it.  A job that splits
-// its merge publishes the subtrees as a task set; idle workers, and a caller
-// waiting for its meshes, take tasks from the sets before taking new jobs.
-// Any thread may run a subtree of copy K: the merge only reads copy K's
-// namespace-scope state, which copy K's owner wrote in triangleinit() before
-// publishing the set.  A worker that just finished polls for work for up to
-// ELAS_TRI_SPIN_S before sleeping: a core that went idle between calls runs
-// Triangle up to ~20% slower for its first few hundred microseconds, which is
-// the whole job at these sizes.
+// its merge publishes each step's subtrees as a task set in one of a fixed
+// number of slots; idle workers, and a caller waiting for its meshes, take
+// tasks from the published sets before taking new jobs.  Tasks are claimed
+// without a lock (an atomic counter per set); a slot's reader count keeps a
+// set alive while a helper may still touch it.  Any thread may run a subtree
+// of copy K: the merge only reads copy K's namespace-scope state, which copy
+// K's owner wrote in triangleinit() before publishing.  A worker that just
+// finished polls for work for up to ELAS_TRI_SPIN_S before sleeping: a core
+// that went idle between calls runs Triangle up to ~20% slower for its first
+// few hundred microseconds, which is the whole job at these sizes.
 #define ELAS_TRI_SPIN_S 0.003
+#define ELAS_TRI_SLOTS 32
 class TriPool {
 public:
     static TriPool& get() {
@@ -345,15 +377,14 @@ public:
         int remaining = 0;
         std::atomic<int> left{0};
     };
-    // tasks of one split merge; claimed through `next` (by helpers under m_),
-    // `done` counts finished tasks and is each helper's last access
+    // tasks of one step of a split merge, claimed through `next`; `done`
+    // counts finished tasks
     struct TaskSet {
         void (*fn)(void*, int);
         void* arg;
         int n;
         std::atomic<int> next{0};
         std::atomic<int> done{0};
-        bool listed = false;  // in help_, under m_
     };
     void run(ElasTriJob* jobs, int n, int max_workers) {
         if (n <= 0) return;
@@ -388,39 +419,53 @@ public:
         ts.fn = fn;
         ts.arg = arg;
         ts.n = ntasks;
+        int slot = -1;
         if (ntasks > 1) {
-            {
-                std::lock_guard<std::mutex> g(m_);
-                ts.listed = true;
-                help_.push_back(&ts);
-                helping_.fetch_add(1, std::memory_order_release);
+            for (int k = 0; k < ELAS_TRI_SLOTS && slot < 0; k++) {
+                TaskSet* expect = nullptr;
+                if (slots_[k].compare_exchange_strong(expect, &ts)) slot = k;
             }
-            for (int i = 1; i < ntasks; i++) cv_.notify_one();
+            if (slot >= 0) {  // (no free slot: the caller runs every task)
+                helping_.fetch_add(1);
+                { std::lock_guard<std::mutex> g(m_); }  // a worker between its check and its sleep sees the set
+                if (ntasks > 2) cv_.notify_all();
+                else cv_.notify_one();
+            }
         }
         for (;;) {
-            const int i = ts.next.fetch_add(1, std::memory_order_relaxed);
+            const int i = ts.next.fetch_add(1);
             if (i >= ntasks) break;
             fn(arg, i);
             ts.done.fetch_add(1, std::memory_order_release);
         }
-        if (ntasks > 1) {
-            std::lock_guard<std::mutex> g(m_);
-            unlist(&ts);
+        if (slot >= 0) {
+            // unpublish, then wait until no helper can still hold the set
+            slots_[slot].store(nullptr);
+            helping_.fetch_sub(1);
+            while (readers_[slot].load() != 0) _mm_pause();
         }
         while (ts.done.load(std::memory_order_acquire) < ntasks) _mm_pause();
     }
-    // Runs one task of a listed set on this thread; false if there is none.
+    // Runs one task of a published set on this thread; false if there is none.
     bool help_one() {
-        if (helping_.load(std::memory_order_acquire) == 0) return false;
-        TaskSet* ts = nullptr;
-        int i = 0;
-        {
-            std::lock_guard<std::mutex> g(m_);
-            if (!claim(&ts, &i)) return false;
+        if (helping_.load() == 0) return false;
+        for (int k = 0; k < ELAS_TRI_SLOTS; k++) {
+            if (!slots_[k].load(std::memory_order_relaxed)) continue;
+            readers_[k].fetch_add(1);
+            TaskSet* ts = slots_[k].load();
+            bool ran = false;
+            if (ts) {
+                const int i = ts->next.fetch_add(1);
+                if (i < ts->n) {
+                    ts->fn(ts->arg, i);
+                    ts->done.fetch_add(1, std::memory_order_release);
+                    ran = true;
+                }
+            }
+            readers_[k].fetch_sub(1);
+            if (ran) return true;
         }
-        ts->fn(ts->arg, i);
-        ts->done.fetch_add(1, std::memory_order_release);
-        return true;
+        return false;
     }
 
 private:
@@ -428,50 +473,38 @@ private:
         ElasTriJob* job;
         Batch* batch;
     };
-    void unlist(TaskSet* ts) {
-        if (!ts->listed) return;
-        ts->listed = false;
-        help_.erase(std::find(help_.begin(), help_.end(), ts));
-        helping_.fetch_sub(1, std::memory_order_release);
-    }
-    // Claims a task of the first listed set that has one (m_ held); exhausted
-    // sets are unlisted on the way.
-    bool claim(TaskSet** out, int* idx) {
-        while (!help_.empty()) {
-            TaskSet* ts = help_.front();
-            const int i = ts->next.fetch_add(1, std::memory_order_relaxed);
-            if (i < ts->n) {
-                *out = ts;
-                *idx = i;
-                return true;
-            }
-            unlist(ts);
-        }
-        return false;
-    }
+    // One worker per CPU this process may run on, less one for the caller
+    // (at most ELAS_TRI_INSTANCES - 1: copy 0 is the caller's): a worker
+    // that waits for a CPU while holding a claimed subtree stalls the whole
+    // merge, and idle workers poll before they sleep.
     void ensure_workers() {
         if (!threads_.empty()) return;
-        for (int k = 1; k < ELAS_TRI_INSTANCES; k++) threads_.emplace_back([this, k] { loop(k); });
+        int cpus = (int)std::thread::hardware_concurrency();
+        cpu_set_t set;
+        if (sched_getaffinity(0, sizeof(set), &set) == 0) cpus = CPU_COUNT(&set);
+        const int nw = std::max(1, std::min(ELAS_TRI_INSTANCES - 1, cpus - 1));
+        for (int k = 1; k <= nw; k++) threads_.emplace_back([this, k] { loop(k); });
     }
     void loop(int k) {
         double last = 0.0;  // end of this worker's last job or task
         for (;;) {
+            if (help_one()) {  // subtrees of a running job first
+                last = now_s();
+                continue;
+            }
             Item it;
             {
-                while (pending_.load(std::memory_order_acquire) == 0 && helping_.load(std::memory_order_acquire) == 0 &&
-                       now_s() - last < ELAS_TRI_SPIN_S)
-                    _mm_pause();
-                std::unique_lock<std::mutex> g(m_);
-                cv_.wait(g, [&] { return !q_.empty() || !help_.empty(); });
-                TaskSet* ts = nullptr;
-                int i = 0;
-                if (claim(&ts, &i)) {  // subtrees of a running job first
-                    g.unlock();
-                    ts->fn(ts->arg, i);
-                    ts->done.fetch_add(1, std::memory_order_release);
-                    last = now_s();
+                if (pending_.load(std::memory_order_acquire) == 0) {
+                    if (now_s() - last < ELAS_TRI_SPIN_S) {
+                        _mm_pause();
+                        continue;
+                    }
+                    std::unique_lock<std::mutex> g(m_);
+                    cv_.wait(g, [&] { return !q_.empty() || helping_.load() != 0; });
+                    last = now_s();  // woken: poll for a while before sleeping again
                     continue;
                 }
+                std::lock_guard<std::mutex> g(m_);
                 if (q_.empty()) continue;
                 it = q_.front();
                 q_.pop_front();
@@ -487,11 +520,12 @@ private:
         }
     }
     std::atomic<int> pending_{0};  // queued jobs, read without the lock while polling
-    std::atomic<int> helping_{0};  // listed task sets, read without the lock while polling
+    std::atomic<int> helping_{0};  // published task sets
+    std::atomic<TaskSet*> slots_[ELAS_TRI_SLOTS] = {};
+    std::atomic<int> readers_[ELAS_TRI_SLOTS] = {};
     std::mutex m_, serial_;
     std::condition_variable cv_;
     std::deque<Item> q_;
-    std::vector<TaskSet*> help_;
     std::vector<std::thread> threads_;
 };
 

@@ -1,5 +1,6 @@
 // elas_triangle.h -- host Delaunay jobs over independent Triangle instances.
 #pragma once
+#include <stddef.h>
 #include <stdint.h>
 #include <vector>
 
@@ -12,16 +13,45 @@ typedef void (*ElasParallelFor)(void* ctx, int ntasks, void (*task)(void* arg, i
 struct ElasTriJob {
     const float* pts;            // [n][2] support points
     int n;
-    std::vector<int32_t>* out;   // [ntri][3] zero-based triangles
-    int result;                  // ntri, or -1 for fewer than 3 points
+    std::vector<int32_t>* out;   // [ntri][3] zero-based triangles, unless dst is set
+    int32_t* dst = nullptr;      // else: written here directly, if ntri <= cap
+    int64_t cap = 0;             // capacity of dst in triangles
+    int result;                  // ntri, or -1 for fewer than 3 distinct points
     // filled by the worker: seconds (steady clock) around the Triangle call
     double t_begin = 0, t_end = 0;
 };
 
-// The vertex order Triangle's divide-and-conquer builds for a point set
-// (input indices), when the coordinates are bounded integers and no two points
-// coincide; false (arr unused) otherwise.
-bool elas_tri_arrangement(const float* p, int n, std::vector<int32_t>& arr);
+// Triangle's divide-and-conquer vertex order (divconqdelaunay(),
+// src/triangle.cpp:6164-6230): the points sorted by (x, y), halved, and each
+// half cut alternately along y and x down to 2- and 3-point leaves.  lx and ly
+// hold the points in (x, y) and (y, x) order; cut() works on a node
+// [lo, lo + size): a cut along `axis` keeps the size / 2 smallest points of
+// that order on the left (as vertexmedian() does, src/triangle.cpp:
+// 5517-5574) and splits the other list stably, so both children keep both
+// orders; 2- and 3-point nodes are leaves in (x, y) order (alternateaxes(),
+// :5586-5605) and go to arr.  `levels` bounds the depth of the cuts made now
+// (-1: down to the leaves): the top of the tree can be cut on one thread and
+// each subtree below it on another, since subtrees touch disjoint ranges.
+struct ElasTriOrder {
+    std::vector<int32_t> lx, ly, tmp, arr;
+    void resize(int n);
+    void cut(int lo, int size, int axis, int levels);
+};
+
+// lx and ly of a point set from two counting sorts, when the coordinates are
+// bounded integers and no two points coincide (support points); false
+// otherwise (which duplicate Triangle keeps then depends on its own sort).
+bool elas_tri_order_integer(const float* p, int n, ElasTriOrder& o);
+// lx and ly of `count` distinct points given in (x, y) order (positions).
+void elas_tri_order_sorted(const float* xy, int count, ElasTriOrder& o);
+
+// Triangle's allocator in this library (its trimalloc() and trifree() call
+// these): blocks of 64 KiB and more are kept in a process-wide cache by
+// power-of-two size class and handed out again, so the pools of one call
+// reuse memory that earlier calls already touched instead of faulting in
+// fresh pages from several threads at once; smaller blocks go to malloc.
+void* elas_tri_block_alloc(size_t bytes);
+void elas_tri_block_free(void* p);
 
 // Runs every job (each a "zQB" Triangle call) on up to max_workers of the
 // process-wide Triangle workers; returns when all are done.
@@ -29,7 +59,7 @@ void elas_triangulate_jobs(ElasTriJob* jobs, int n, int max_workers);
 
 // Largest split of one mesh's divide-and-conquer merge over the pool's
 // threads: up to 2^max_depth concurrent subtrees (0: serial Triangle; -1:
-// automatic, subtrees of at least a few hundred points, at most 8).  The
+// automatic, subtrees of at least 128 points, at most 8).  The
 // meshes are the same under every setting.  elas_tri_split_depth() is the
 // depth a mesh of n points gets now.
 void elas_tri_set_split(int max_depth);

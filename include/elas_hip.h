@@ -264,7 +264,7 @@ int elas_hip_triangulate(const float* pts, int32_t n, int32_t* tri, int64_t cap,
  * each with its own triangle pool, merged by Triangle's own mergehulls() and
  * read out in that order: the triangle list, numbering included, is the
  * serial one under every setting.  Process-wide; 0 = serial Triangle, -1 =
- * automatic (the default: subtrees of at least 384 points, at most 8 per
+ * automatic (the default: subtrees of at least 128 points, at most 8 per
  * mesh).  elas_hip_tri_split_depth reports the depth a mesh of n points gets
  * under the current setting. */
 int elas_hip_set_tri_split(int32_t max_depth);

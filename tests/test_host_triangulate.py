@@ -191,5 +191,5 @@ def test_split_on_pipeline_points(pkg, oracle, pairs, split):
         assert np.array_equal(pkg.triangulate(ref["sp0"]), ref["tri0"]), depth
         assert np.array_equal(pkg.triangulate(ref["sp1"]), ref["tri1"]), depth
     split(-1)
-    assert pkg.tri_split_depth(len(ref["sp0"])) == 1        # 1,427 points: two subtrees of >= 384
+    assert pkg.tri_split_depth(len(ref["sp0"])) == 3        # 1,427 points: 8 subtrees of >= 128
     assert pkg.tri_split_depth(22418) == 3                   # capped at 8 subtrees
