@@ -130,8 +130,9 @@ def serving(pkg, synth, rates=(50, 100, 200, 400, 800), n_req=2000, pool=None, s
     (elas_hip_submit / elas_hip_poll).  Latency = arrival -> D1/D2 in host
     memory (the feeder's own lateness included); sustained pairs/s = requests
     / (first arrival -> last completion).  After `rates`, the `knee` rates run
-    in turn until one is past the knee: p99 above 4x p50, or less than 90% of
-    the offered rate sustained."""
+    in turn until one is past the knee: less than 90% of the offered rate
+    sustained, or the median latency above twice the lowest rate's (queueing
+    builds up; the p99 of the mix is set by its largest pairs at any rate)."""
     import torch
     pool = pool or synth.batch_pairs(64)
     pin = lambda a: torch.from_numpy(a).pin_memory().numpy()
@@ -185,7 +186,8 @@ def serving(pkg, synth, rates=(50, 100, 200, 400, 800), n_req=2000, pool=None, s
                          "max_in_flight_pairs": int(st["max_in_flight_pairs"]),
                          "peak_in_flight_MB": round(st["peak_in_flight_bytes"] / 1e6, 1)}
         r = out[str(lam)]
-        r["past_knee"] = bool(r["p99_ms"] > 4 * r["p50_ms"] or r["sustained_pairs_per_s"] < 0.9 * lam)
+        p50_low = min(v["p50_ms"] for v in out.values())
+        r["past_knee"] = bool(r["sustained_pairs_per_s"] < 0.9 * lam or r["p50_ms"] > 2 * p50_low)
     el.close()
     return {"rates": out, "pool_pairs": len(P), "hbm_budget_GiB": 8, "buffers": "pinned host"}
 

@@ -306,6 +306,21 @@ int split_depth(int n) {
 }
 
 void pool_parallel_for(void* ctx, int ntasks, void (*task)(void*, int), void* arg);
+int pool_share(void* ctx);
+
+// The depth a job gets when it starts: split_depth(), and under the automatic
+// setting no more subtrees than the job's share of the threads (the jobs in
+// flight split them; a batch of many meshes runs each one serially).
+int job_depth(int n, void* pool) {
+    int d = split_depth(n);
+    if (g_split_max.load(std::memory_order_relaxed) < 0) {
+        const int share = pool_share(pool);
+        int cap = 0;
+        while ((2 << cap) <= share) cap++;
+        d = std::min(d, cap);
+    }
+    return d;
+}
 
 // One "zQB" Triangle call on copy k; with split set Triangle's setup and the
 // vertex order are built at once, and the merge's subtrees run through the
@@ -326,7 +341,7 @@ void tri_prepare(void* arg, int t) {
 
 int tri_call(int k, ElasTriJob& j, bool split, void* pool) {
     thread_local ElasTriOrder order;
-    const int depth = split ? split_depth(j.n) : 0;
+    const int depth = split ? job_depth(j.n, pool) : 0;
     TriCall c{k, &j, &order, nullptr, false};
     if (depth > 0) {
         pool_parallel_for(pool, 2, tri_prepare, &c);
@@ -393,7 +408,11 @@ public:
             // its merge over the workers unless one worker is asked for
             std::lock_guard<std::mutex> g(serial_);
             if (max_workers > 1) ensure_workers();
-            for (int i = 0; i < n; i++) jobs[i].result = run_instance(0, jobs[i], max_workers > 1, this);
+            active_.fetch_add(n);
+            for (int i = 0; i < n; i++) {
+                jobs[i].result = run_instance(0, jobs[i], max_workers > 1, this);
+                active_.fetch_sub(1);
+            }
             return;
         }
         Batch b;
@@ -408,6 +427,7 @@ public:
             std::lock_guard<std::mutex> g(m_);
             ensure_workers();
             for (int i = 0; i < n; i++) q_.push_back({&jobs[i], b});
+            active_.fetch_add(n);
             pending_.fetch_add(n, std::memory_order_release);
         }
         for (int i = 0; i < n; i++) cv_.notify_one();
@@ -428,8 +448,7 @@ public:
             if (slot >= 0) {  // (no free slot: the caller runs every task)
                 helping_.fetch_add(1);
                 { std::lock_guard<std::mutex> g(m_); }  // a worker between its check and its sleep sees the set
-                if (ntasks > 2) cv_.notify_all();
-                else cv_.notify_one();
+                for (int i = 1; i < ntasks; i++) cv_.notify_one();
             }
         }
         for (;;) {
@@ -495,8 +514,12 @@ private:
             Item it;
             {
                 if (pending_.load(std::memory_order_acquire) == 0) {
-                    if (now_s() - last < ELAS_TRI_SPIN_S) {
-                        _mm_pause();
+                    const double idle = now_s() - last;
+                    if (idle < ELAS_TRI_SPIN_S) {
+                        // poll; after the first microseconds, give the CPU
+                        // to any other thread that wants it meanwhile
+                        if (idle < 20e-6) _mm_pause();
+                        else std::this_thread::yield();
                         continue;
                     }
                     std::unique_lock<std::mutex> g(m_);
@@ -511,6 +534,7 @@ private:
                 pending_.fetch_sub(1, std::memory_order_relaxed);
             }
             it.job->result = run_instance(k, *it.job, true, this);
+            active_.fetch_sub(1);
             {
                 std::lock_guard<std::mutex> g(it.batch->m);
                 if (--it.batch->remaining == 0) it.batch->cv.notify_all();
@@ -519,7 +543,16 @@ private:
             last = now_s();
         }
     }
+public:
+    // threads that may run a job's subtrees: the workers and the caller
+    int share() const {
+        const int act = std::max(1, active_.load(std::memory_order_relaxed));
+        return std::max(1, ((int)threads_.size() + 1) / act);
+    }
+
+private:
     std::atomic<int> pending_{0};  // queued jobs, read without the lock while polling
+    std::atomic<int> active_{0};   // jobs queued or running
     std::atomic<int> helping_{0};  // published task sets
     std::atomic<TaskSet*> slots_[ELAS_TRI_SLOTS] = {};
     std::atomic<int> readers_[ELAS_TRI_SLOTS] = {};
@@ -528,6 +561,8 @@ private:
     std::deque<Item> q_;
     std::vector<std::thread> threads_;
 };
+
+int pool_share(void* ctx) { return static_cast<TriPool*>(ctx)->share(); }
 
 void pool_parallel_for(void* ctx, int ntasks, void (*task)(void*, int), void* arg) {
     static_cast<TriPool*>(ctx)->parallel_for(ntasks, task, arg);
