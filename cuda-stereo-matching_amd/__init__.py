@@ -102,6 +102,7 @@ SIGNATURES = {
     "elas_hip_triangulate": (ctypes.c_int, [_P, ctypes.c_int32, _P, ctypes.c_int64,
                                             ctypes.POINTER(ctypes.c_int64)]),
     "elas_hip_set_autotune": (ctypes.c_int, [_P, ctypes.c_int32]),
+    "elas_hip_process_device_noalloc": (ctypes.c_int, [_P, _P, _P, _P, _P, ctypes.c_int32, ctypes.c_int32]),
     "elas_hip_set_tri_split": (ctypes.c_int, [ctypes.c_int32]),
     "elas_hip_tri_split_depth": (ctypes.c_int, [ctypes.c_int32]),
     "elas_io_read_pgm": (ctypes.c_int, [ctypes.c_char_p, _P, ctypes.c_int64, ctypes.POINTER(ctypes.c_int32),
@@ -324,6 +325,22 @@ class Elas:
         _torch_sync(left)
         st = self._lib.elas_hip_process_device(self._ctx, left.data_ptr(), right.data_ptr(), D1.data_ptr(),
                                                D2.data_ptr(), W, H)
+        return self._check(st, allow_few=True)
+
+    def process_device_noalloc(self, left, right, D1, D2):
+        """process_device without any device or pinned allocation: the
+        intermediates live in the library's per-GPU scratch (128 MiB), so the
+        pair must fit it (ELAS_ERR_UNSUPPORTED otherwise); calls that overlap
+        on one GPU race."""
+        import torch
+        H, W = int(left.shape[0]), int(left.shape[1])
+        for t, dt in ((left, torch.uint8), (right, torch.uint8), (D1, torch.float32), (D2, torch.float32)):
+            if t.dtype is not dt or not t.is_cuda or t.dim() != 2 or t.shape[0] != H or t.shape[1] != W \
+                    or not t.is_contiguous():
+                raise ValueError("process_device_noalloc expects contiguous cuda tensors of shape [H, W]")
+        _torch_sync(left)
+        st = self._lib.elas_hip_process_device_noalloc(self._ctx, left.data_ptr(), right.data_ptr(), D1.data_ptr(),
+                                                       D2.data_ptr(), W, H)
         return self._check(st, allow_few=True)
 
     def time_kernel(self, kernel, reps=20):

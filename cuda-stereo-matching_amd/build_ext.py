@@ -31,7 +31,7 @@ ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HOST_CLANG = os.path.join(ROCM, "llvm", "bin", "clang++")  # host x86 code (Triangle)
 TRI_INSTANCES = 16
 
-HIP_SOURCES = ["k_desc.hip", "k_support.hip", "k_dense.hip", "k_post.hip"]
+HIP_SOURCES = ["k_desc.hip", "k_support.hip", "k_dense.hip", "k_post.hip", "k_scratch.hip"]
 HOST_SOURCES = ["elas_engine.cpp", "elas_capi.cpp", "elas_serve.cpp", "Elas.cpp", "elas_io.cpp"]
 CLI = os.path.join(LIBDIR, "elas_cli")
 HIPFLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",

@@ -107,6 +107,18 @@ int elas_hip_create(const elas_params* params, int device, elas_ctx** out) {
     }
     if (elas_dense_allow_lds(c->max_lds) != 0) c->max_lds = 65536;
     st = elas_hip_set_pipeline(c, 3, 0);
+    if (st == ELAS_OK && c->fixed_init() != ELAS_OK) {
+        // every other entry point works without it; the no-allocation one
+        // reports the reason (kept in fixed_err)
+        c->fixed_err = c->err;
+        c->err.clear();
+        if (c->fixed) {
+            c->slot_free(*c->fixed);
+            delete c->fixed;
+            c->fixed = nullptr;
+        }
+        (void)hipGetLastError();
+    }
     if (st != ELAS_OK) {
         elas_hip_destroy(c);
         return st;
@@ -128,6 +140,10 @@ int elas_hip_destroy(elas_ctx* c) {
         delete S;
     }
     c->slots.clear();
+    if (c->fixed) {
+        c->slot_free(*c->fixed);
+        delete c->fixed;
+    }
     delete c;
     return ELAS_OK;
 }
@@ -201,6 +217,14 @@ int elas_hip_process_device(elas_ctx* c, const uint8_t* d_left, const uint8_t* d
     ELAS_NOT_SERVING(c);
     ElasPairSpec s{width, height, c->prm.disp_max, d_left, d_right, d_D1, d_D2, true, true, -1};
     return c->run(&s, 1, nullptr);
+}
+
+int elas_hip_process_device_noalloc(elas_ctx* c, const uint8_t* d_left, const uint8_t* d_right, float* d_D1,
+                                    float* d_D2, int32_t width, int32_t height) {
+    if (!c) return ELAS_ERR_INVALID_ARGUMENT;
+    ELAS_NOT_SERVING(c);
+    ElasPairSpec s{width, height, c->prm.disp_max, d_left, d_right, d_D1, d_D2, true, true, -1};
+    return c->run_fixed(s);
 }
 
 static int batch(elas_ctx* c, int32_t n, const uint8_t* const* left, const uint8_t* const* right,

@@ -228,8 +228,8 @@ void elas_ctx::slot_free(ElasSlot& S) {
     S.ticket = nullptr;
     for (auto& e : S.ev)
         if (e) (void)hipEventDestroy(e);
-    if (S.arena) (void)hipFree(S.arena);
-    if (S.table) (void)hipFree(S.table);
+    if (S.arena && !S.external) (void)hipFree(S.arena);
+    if (S.table && !S.external) (void)hipFree(S.table);
     if (S.pin) (void)hipHostFree(S.pin);
     if (S.tune) (void)hipFree(S.tune);
     if (S.stream) (void)hipStreamDestroy(S.stream);
@@ -329,8 +329,14 @@ int elas_ctx::slot_prepare(ElasSlot& S, const ElasPairSpec* sp, int n) {
     size_t arena_need = 0, pin_need = 0;
     int st = slot_need(sp, n, &arena_need, &pin_need);
     if (st != ELAS_OK) return st;
-    st = slot_reserve(S, arena_need, pin_need, n);
-    if (st != ELAS_OK) return st;
+    if (S.external) {
+        // buffers are in place (run_fixed checked that the pair fits)
+        if (arena_need > S.arena_bytes || pin_need > S.pin_bytes || 2 * table_one(n) > S.table_bytes)
+            return fail(ELAS_ERR_UNSUPPORTED, "pair does not fit the scratch of the no-allocation entry point");
+    } else {
+        st = slot_reserve(S, arena_need, pin_need, n);
+        if (st != ELAS_OK) return st;
+    }
     S.n = n;
     S.specs.assign(sp, sp + n);
     S.dd.resize(n);
@@ -911,7 +917,7 @@ int elas_ctx::slot_finish(ElasSlot& S, int32_t* statuses) {
     S.times[ELAS_T_AB_GAP] = el(5, 6);
     if (statuses)
         for (int i = 0; i < S.n; i++) statuses[i] = S.status[i];
-    if (autotune && force_cpw == 0 && force_dsh == 0)
+    if (autotune && force_cpw == 0 && force_dsh == 0 && !S.external)
         for (int i = 0; i < S.n; i++)
             if (S.status[i] == ELAS_OK && !tuned.count({S.specs[i].W, S.specs[i].H, S.specs[i].dmax})) {
                 const int r = tune(S, i);
@@ -1110,6 +1116,72 @@ int elas_ctx::run(const ElasPairSpec* sp, int n, int32_t* statuses) {
         err = "fewer than 3 support points (or a degenerate mesh): D1/D2 are all -1";
     }
     return ret;
+}
+
+// The fixed slot of the no-allocation entry point: streams, events, gate and
+// pinned staging now (at context creation), the GPU's scratch address from
+// k_scratch.hip.  The device table takes the scratch's first 64 KiB, the
+// arena the rest.
+#define ELAS_FIXED_TABLE ((size_t)64 << 10)
+#define ELAS_FIXED_PIN ((size_t)8 << 20)
+int elas_ctx::fixed_init() {
+    HIPCHECK(hipSetDevice(device));
+    void* p = nullptr;
+    if (elas_scratch_setup(device, &p) != 0 || !p)
+        return fail(ELAS_ERR_HIP, "scratch of the no-allocation entry point: address lookup failed");
+    scratch = (char*)p;
+    ElasSlot* S = new (std::nothrow) ElasSlot();
+    if (!S) return fail(ELAS_ERR_OUT_OF_MEMORY, "fixed slot");
+    fixed = S;
+    int r = slot_init(*S);
+    if (r != ELAS_OK) return r;
+    S->external = true;
+    if (hipHostMalloc(&S->pin, ELAS_FIXED_PIN, hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess) {
+        S->pin = nullptr;
+        (void)hipGetLastError();
+        return fail(ELAS_ERR_OUT_OF_MEMORY, "pinned staging of the no-allocation entry point");
+    }
+    if (hipHostGetDevicePointer((void**)&S->pin_dev, S->pin, 0) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(ELAS_ERR_OUT_OF_MEMORY, "pinned staging is not mapped for the device");
+    }
+    S->pin_bytes = ELAS_FIXED_PIN;
+    S->table = scratch;
+    S->table_bytes = ELAS_FIXED_TABLE;
+    S->arena = scratch + ELAS_FIXED_TABLE;
+    S->arena_bytes = ELAS_SCRATCH_BYTES - ELAS_FIXED_TABLE;
+    return ELAS_OK;
+}
+
+// One device-resident pair through the fixed slot.  No hipMalloc, hipFree,
+// pinned allocation, graph instantiation or autotuning happens here: the
+// slot's buffers were made by fixed_init, phases A and B are launched
+// directly (phase B behind the gate), and a shape's tiling is the tuned one
+// if an earlier call on this context tuned it, else the heuristic.
+int elas_ctx::run_fixed(const ElasPairSpec& sp) {
+    if (!fixed || !scratch) return fail(ELAS_ERR_HIP, "no-allocation entry point not set up: " + fixed_err);
+    ElasSlot& S = *fixed;
+    size_t arena_need = 0, pin_need = 0;
+    int r = slot_need(&sp, 1, &arena_need, &pin_need);
+    if (r != ELAS_OK) return r;
+    if (arena_need > S.arena_bytes || pin_need > S.pin_bytes)
+        return fail(ELAS_ERR_UNSUPPORTED, "pair needs " + std::to_string(arena_need) + " B of device scratch and " +
+                                              std::to_string(pin_need) + " B of pinned staging; the no-allocation "
+                                              "entry point holds " + std::to_string(S.arena_bytes) + " and " +
+                                              std::to_string(S.pin_bytes));
+    int32_t st = ELAS_OK;
+    r = slot_prepare(S, &sp, 1);
+    S.gated = true;  // direct launches only (a graph would be instantiated)
+    if (r == ELAS_OK) r = slot_phase_a(S);
+    if (r == ELAS_OK) r = slot_start_tri(S);
+    if (r == ELAS_OK) r = slot_phase_b(S);
+    if (r == ELAS_OK) r = slot_finish(S, &st);
+    if (r != ELAS_OK) {
+        slot_abort(S);
+        return r;
+    }
+    if (st != ELAS_OK) err = "fewer than 3 support points (or a degenerate mesh): D1/D2 are all -1";
+    return st;
 }
 
 // Chunks of the batch move through the slots: a slot whose phase A is done

@@ -113,6 +113,9 @@ struct ElasSlot {
     std::vector<char> tab0_up;     // bytes last copied into the phase A device table
     bool gate_armed = false;       // queued behind the gate, flag not yet set
     bool gate_used = false;        // this call's phase B went through the gate (gate[1] is its status)
+    // arena and device table lie in the per-GPU scratch of the no-allocation
+    // entry point: never grown, never freed, no autotuning (which allocates)
+    bool external = false;
 };
 
 // Tiling of one pair shape: dense grid cells per wave per step and steps per
@@ -142,6 +145,11 @@ struct elas_ctx {
     std::vector<ElasSlot*> slots;
     ElasSlot* last = nullptr;  // slot of the last finished call (fetch, shape info, times)
     ElasServer* server = nullptr;
+    // the no-allocation entry point: its own slot (streams, events, gate and
+    // pinned staging made at context creation) over the GPU's scratch
+    ElasSlot* fixed = nullptr;
+    char* scratch = nullptr;
+    std::string fixed_err;  // why fixed_init failed, if it did
     size_t max_lds = 65536;   // dynamic LDS a workgroup may allocate on this device
     // Tilings per (W, H, disp_max): with autotune on, the first call with a
     // new shape times every candidate dense tiling and descriptor strip
@@ -157,6 +165,10 @@ struct elas_ctx {
 
     // blocking single-chunk run on slot 0 (Elas::process and the device form)
     int run(const ElasPairSpec* specs, int n, int32_t* statuses);
+    // the same for one device-resident pair in the fixed slot over the
+    // scratch, with no device or pinned allocation
+    int run_fixed(const ElasPairSpec& spec);
+    int fixed_init();
     // pipelined run of n pairs in chunks of up to `chunk` pairs over the slots
     int run_batch(const ElasPairSpec* specs, int n, int32_t* statuses, int chunk);
     int fetch(int stage, void* dst, size_t bytes, size_t* stage_bytes);

@@ -5,6 +5,15 @@
 #include <stddef.h>
 #include "elas_common.h"
 
+// Scratch of the entry point that allocates nothing: ELAS_SCRATCH_BYTES of
+// device memory per GPU (a file-scope device array).  elas_scratch_setup()
+// returns its address on `device`, which must be the current device; the
+// first call per device looks the address up (no allocation), later calls
+// return the kept address.  0 on success.
+#define ELAS_SCRATCH_BYTES ((size_t)128 << 20)
+#define ELAS_SCRATCH_DEVICES 64
+int elas_scratch_setup(int device, void** ptr);
+
 void elas_launch_descriptor(const ElasLaunch& L, int sh, int grid, hipStream_t s);
 // Support matching, left pass and right-to-left check in one launch (lds:
 // the right pass's staging, the larger of the two).

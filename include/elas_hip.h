@@ -152,6 +152,24 @@ int elas_hip_process(elas_ctx* ctx, const uint8_t* left, const uint8_t* right,
 int elas_hip_process_device(elas_ctx* ctx, const uint8_t* d_left, const uint8_t* d_right,
                             float* d_D1, float* d_D2, int32_t width, int32_t height);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): elas_hip_process_device without any device or pinned
+ * allocation.  Its intermediates live in a per-GPU scratch array of 128 MiB
+ * (a file-scope device array of the library, held by every GPU the library
+ * runs on until the process ends) and in pinned staging and streams made when
+ * the context was created; phases run as direct launches, and no autotuning
+ * happens (a shape uses the tiling an earlier call on this context tuned, else
+ * the built-in heuristic).  The scratch bounds the pair: about 1.5 Mpx at
+ * disparities 0..255 with post-processing off (1242x375, 1280x960, 1920x720);
+ * a larger pair returns ELAS_ERR_UNSUPPORTED before any work.  All contexts
+ * on one GPU share its scratch, which keeps the last call's contents: calls
+ * that overlap on one GPU (two threads, two contexts) race and must be
+ * serialised by the caller.  Synchronous; same results as
+ * elas_hip_process_device.  The host Delaunay step still uses the host heap. */
+int elas_hip_process_device_noalloc(elas_ctx* ctx, const uint8_t* d_left, const uint8_t* d_right,
+                                    float* d_D1, float* d_D2, int32_t width, int32_t height);
+
 /* Batched form: n pairs of possibly different sizes and disparity ranges,
  * each stage one launch over all pairs.  Arrays of n host pointers; dims is
  * n*3 int32: (W, H, disp_max), disp_max < 0 meaning the context's.
