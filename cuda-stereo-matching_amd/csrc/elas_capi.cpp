@@ -77,6 +77,17 @@ void elas_params_declared(elas_params* p) {
 int elas_hip_create(const elas_params* params, int device, elas_ctx** out) {
     if (!out) return ELAS_ERR_INVALID_ARGUMENT;
     *out = nullptr;
+    {
+        // parameters first: a request the library cannot honour is refused
+        // the same way with or without a device
+        elas_params p;
+        if (params) p = *params;
+        else elas_params_default(&p);
+        ElasKParams kp;
+        std::string e;
+        const int st = elas_make_kparams(&p, &kp, &e);
+        if (st != ELAS_OK) return st;
+    }
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
         (void)hipGetLastError();

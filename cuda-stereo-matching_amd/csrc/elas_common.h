@@ -62,7 +62,8 @@ struct ElasPair {
     uint8_t* desc[2];        // [H][W][16]
     int32_t* can[4];         // [ch][cw]: left pass, LR pass, inconsistency, final
     int32_t* blkcnt;         // [nblk] valid support points per 256-cell block
-    float* sp[2];            // [cw*ch][2] compacted support points (left, right)
+    float* sp[2];            // [cw*ch+8][2] compacted support points (left, right), then the
+                             //   corner points of add_corners
     int32_t* nsp;            // [1]
     float* hsp[2];           // sp[2] again, in mapped pinned host memory (for the host step)
     int32_t* hnsp;           // nsp again, in mapped pinned host memory
@@ -99,6 +100,7 @@ struct ElasKParams {
     // post-processing (off unless enabled)
     float speckle_sim;
     int32_t speckle_size, ipol_gap, adaptive_mean, median, post_images;  // post_images: 1 (left) or 2
+    int32_t add_corners;     // corner support points after compaction
 };
 
 // Launch a stage over a batch: pairs/prefix live in device memory, prefix has
@@ -122,9 +124,9 @@ struct ElasLaunch {
 // the pairs tiled with one cell per wave, S_DENSE2 those with two; S_DESC,
 // S_DESC12 and S_DESC6 the descriptor strips of the pairs with 18-, 12- and
 // 6-row strips.  S_PLANE counts 64-triangle chunks of both meshes, S_RTILES
-// the raster tiles of both images.
+// the raster tiles of both images, S_PAIRS one tile per pair.
 enum ElasStageId {
     S_DESC = 0, S_SUPPORT, S_INCON, S_REDUN_H, S_REDUN_V, S_BLOCKS, S_GRID_CELLS,
     S_PLANE, S_RTILES, S_DENSE, S_PIXELS, S_GRID_FILL, S_POST_TILES, S_POST_ROWS, S_POST_COLS,
-    S_DENSE2, S_DESC12, S_DESC6, S_NUM_STAGES
+    S_DENSE2, S_DESC12, S_DESC6, S_PAIRS, S_NUM_STAGES
 };

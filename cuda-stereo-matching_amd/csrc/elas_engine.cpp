@@ -100,7 +100,7 @@ size_t pair_arena(const ElasPairSpec& s, const ElasDims& d, bool debug, bool pos
     size_t b = 0;
     if (!s.dev_in) b += 2 * al(d.npx);
     b += 2 * al(16 * d.npx);
-    b += 4 * al(4 * d.ncan) + al(4 * (size_t)d.nblk) + 2 * al(8 * d.ncan);
+    b += 4 * al(4 * d.ncan) + al(4 * (size_t)d.nblk) + 2 * al(8 * d.nspcap);
     b += 4 * al(4 * d.ncell * d.words);
     if (debug) b += 2 * al(4 * d.ncell * (size_t)(s.dmax + 2)) + 2 * al(4 * d.npx);
     b += 2 * al(4 * ELAS_TRIREC * (size_t)d.tricap);
@@ -130,7 +130,8 @@ ElasDims elas_dims(const elas_params& p, int W, int H, int dmax) {
     d.ncan = (size_t)d.cw * d.ch;
     d.ncell = (size_t)d.gw * d.gh;
     d.nblk = (int32_t)((d.ncan + 255) / 256);
-    d.tricap = (int32_t)(2 * d.ncan + 8);
+    d.nspcap = d.ncan + 8;  // the lattice's points and the corner points of add_corners
+    d.tricap = (int32_t)(2 * d.nspcap + 8);
     // raster culling super-tiles: single raster tiles, or squares of 2, 4, ..
     // tiles a side where an image has more than ELAS_ST_MAX tiles
     const int ntx = cdiv(W, ELAS_RT_W), nty = cdiv(H, ELAS_RT_H);
@@ -194,6 +195,14 @@ int elas_make_kparams(const elas_params* p, ElasKParams* kp, std::string* err) {
     kp->adaptive_mean = p->filter_adaptive_mean ? 1 : 0;
     kp->median = p->filter_median ? 1 : 0;
     kp->post_images = p->postprocess_only_left ? 1 : 2;
+    kp->add_corners = p->add_corners ? 1 : 0;
+    if (p->subsampling) {
+        // src/Elas.h:36-38: D1/D2 of W/2 x H/2; the reference gives no
+        // semantics beyond that comment (which pixels, and the left/right
+        // check at half resolution), so it is refused rather than guessed
+        *err = "subsampling is not supported (the reference declares it without semantics)";
+        return ELAS_ERR_UNSUPPORTED;
+    }
     return ELAS_OK;
 }
 
@@ -257,7 +266,7 @@ int elas_ctx::slot_need(const ElasPairSpec* sp, int n, size_t* arena_need, size_
             return fail(ELAS_ERR_UNSUPPORTED, "gap interpolation supports lines up to 15872 pixels");
         const ElasDims d = elas_dims(prm, s.W, s.H, s.dmax);
         a += pair_arena(s, d, debug, post_scratch(prm));
-        psp += 2 * 8 * d.ncan;
+        psp += 2 * 8 * d.nspcap;
         ptri += 2 * 12 * (size_t)d.tricap;
     }
     *arena_need = a;
@@ -350,10 +359,10 @@ int elas_ctx::slot_prepare(ElasSlot& S, const ElasPairSpec* sp, int n) {
         S.base[i] = a;
         a += pair_arena(sp[i], S.dd[i], debug, post_scratch(prm));
         S.sp_off[i] = so;
-        so += 2 * 2 * S.dd[i].ncan;
+        so += 2 * 2 * S.dd[i].nspcap;
         S.tri_off[i] = to;
         to += 2 * 3 * (size_t)S.dd[i].tricap;
-        psp += 2 * 8 * S.dd[i].ncan;
+        psp += 2 * 8 * S.dd[i].nspcap;
         ptri += 2 * 12 * (size_t)S.dd[i].tricap;
         S.maxd = std::max(S.maxd, sp[i].dmax);
     }
@@ -410,7 +419,7 @@ int elas_ctx::slot_prepare(ElasSlot& S, const ElasPairSpec* sp, int n) {
         for (int k = 0; k < 2; k++) P.desc[k] = (uint8_t*)take(16 * d.npx);
         for (int k = 0; k < 4; k++) P.can[k] = (int32_t*)take(4 * d.ncan);
         P.blkcnt = (int32_t*)take(4 * (size_t)d.nblk);
-        for (int k = 0; k < 2; k++) P.sp[k] = (float*)take(8 * d.ncan);
+        for (int k = 0; k < 2; k++) P.sp[k] = (float*)take(8 * d.nspcap);
         P.nsp = S.dnsp + i;
         // device addresses in the mapped pinned staging: the support points
         // and their count again (read by the host step without a copy), and
@@ -419,7 +428,7 @@ int elas_ctx::slot_prepare(ElasSlot& S, const ElasPairSpec* sp, int n) {
         // triangles after the left one)
         auto hdev = [&](const void* h) { return S.pin_dev + ((const char*)h - (const char*)S.pin); };
         P.hsp[0] = (float*)hdev(S.hsp + S.sp_off[i]);
-        P.hsp[1] = (float*)hdev(S.hsp + S.sp_off[i] + 2 * d.ncan);
+        P.hsp[1] = (float*)hdev(S.hsp + S.sp_off[i] + 2 * d.nspcap);
         P.hnsp = (int32_t*)hdev(S.hnsp + i);
         P.tri[0] = (const int32_t*)hdev(S.htri + S.tri_off[i]);
         P.tri[1] = P.tri[0] + 3 * (size_t)d.tricap;
@@ -469,6 +478,7 @@ int stage_tiles(const elas_params& prm, const ElasPairSpec& s, const ElasDims& d
         case S_POST_TILES: return 2 * cdiv(s.W, ELAS_POST_TW) * cdiv(s.H, ELAS_POST_TH);
         case S_POST_ROWS: return 2 * s.H;
         case S_POST_COLS: return 2 * s.W;
+        case S_PAIRS: return 1;
     }
     return 0;
 }
@@ -531,6 +541,10 @@ int elas_ctx::enqueue_a(ElasSlot& S, bool timed) {
     HIPCHECK(ev(4));
     { ElasLaunch L = launch_of(S, 0, S_BLOCKS);
       if (L.total) { elas_launch_blkcount(L, g(L, cap256), s1); elas_launch_compact(L, g(L, cap256), s1); } }
+    if (kp.add_corners) {
+        const ElasLaunch L = launch_of(S, 0, S_PAIRS);
+        if (L.total) elas_launch_corners(L, g(L, cap256), s1);
+    }
     HIPCHECK(hipGetLastError());
     return ELAS_OK;
 }
@@ -618,7 +632,7 @@ int elas_ctx::slot_start_tri(ElasSlot& S) {
         for (int k = 0; k < 2; k++) {
             S.job_of[2 * i + k] = (int)S.jobs.size();
             ElasTriJob j;
-            j.pts = S.hsp + S.sp_off[i] + (size_t)k * 2 * S.dd[i].ncan;
+            j.pts = S.hsp + S.sp_off[i] + (size_t)k * 2 * S.dd[i].nspcap;
             j.n = S.nsp[i];
             j.out = nullptr;
             j.dst = S.htri + S.tri_off[i] + (size_t)k * 3 * S.dd[i].tricap;

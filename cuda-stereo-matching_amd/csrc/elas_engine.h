@@ -38,7 +38,7 @@ struct ElasPairSpec {
 struct ElasDims {
     int32_t cw, ch, gw, gh, words, nblk, tricap;
     int32_t sshift, nsx, nst, scap;  // raster culling super-tiles and their list capacity
-    size_t npx, ncan, ncell;
+    size_t npx, ncan, ncell, nspcap;  // nspcap: support point capacity (lattice + corners)
 };
 
 enum ElasSlotPhase { SLOT_IDLE = 0, SLOT_A = 1, SLOT_TRI = 2, SLOT_B = 3 };

@@ -22,6 +22,9 @@ void elas_launch_incon(const ElasLaunch& L, const ElasKParams& kp, int grid, hip
 void elas_launch_redun(const ElasLaunch& L, bool vertical, int grid, size_t lds, hipStream_t s);
 void elas_launch_blkcount(const ElasLaunch& L, int grid, hipStream_t s);
 void elas_launch_compact(const ElasLaunch& L, int grid, hipStream_t s);
+// Corner support points of add_corners, appended after compaction (one
+// workgroup per pair; S_PAIRS tiles).
+void elas_launch_corners(const ElasLaunch& L, int grid, hipStream_t s);
 void elas_launch_grid_fill(const ElasLaunch& L, int grid, hipStream_t s);
 void elas_launch_grid_scatter(const ElasLaunch& L, int grid, hipStream_t s);
 void elas_launch_grid_dilate(const ElasLaunch& L, int grid, hipStream_t s);

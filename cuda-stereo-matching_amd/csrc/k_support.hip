@@ -605,6 +605,66 @@ void elas_launch_redun(const ElasLaunch& L, bool vertical, int grid, size_t lds,
 void elas_launch_blkcount(const ElasLaunch& L, int grid, hipStream_t s) {
     hipLaunchKernelGGL(k_blkcount, dim3(grid), dim3(256), 0, s, L);
 }
+// ---- corner support points (add_corners, src/Elas.h:23) ---------------------
+// The reference declares the switch without implementing it; its comment
+// ("add support points at image corners with nearest neighbor disparities")
+// is the specification.  One workgroup per pair, after compaction: each image
+// corner (0,0), (0,H-1), (W-1,0), (W-1,H-1) takes the disparity of the
+// support point nearest to it (smallest squared distance, the first in
+// compaction order on ties: a minimum of (distance, index) keys), and six
+// points are appended after the nsp support points: the four corners, then
+// the two right corners again at u = W-1+d (so that the right image's mesh,
+// over (u-d, v), reaches its right border).  The count becomes nsp + 6, in
+// device memory and in the pinned copy the host step reads.
+__global__ __launch_bounds__(256) void k_corners(ElasLaunch L) {
+    __shared__ unsigned long long red[4][256];
+    const int tid = threadIdx.x;
+    for (int t = blockIdx.x; t < L.total; t += gridDim.x) {
+        const int p = elas_find_pair(L.prefix, L.npairs, t);
+        const auto& P = L.pairs[p];
+        const int n = P.nsp[0];
+        if (n < 1) continue;  // uniform: no support point, no corners
+        const int cu[4] = {0, 0, P.W - 1, P.W - 1}, cv[4] = {0, P.H - 1, 0, P.H - 1};
+        unsigned long long key[4] = {~0ull, ~0ull, ~0ull, ~0ull};
+        const float2* s0 = reinterpret_cast<const float2*>(P.sp[0]);
+        for (int i = tid; i < n; i += 256) {
+            const float2 q = s0[i];
+            const int u = (int)q.x, v = (int)q.y;
+            for (int c = 0; c < 4; c++) {
+                const unsigned du = (unsigned)abs(u - cu[c]), dv = (unsigned)abs(v - cv[c]);
+                const unsigned long long k2 = ((unsigned long long)(du * du + dv * dv) << 32) | (unsigned)i;
+                key[c] = k2 < key[c] ? k2 : key[c];
+            }
+        }
+        for (int c = 0; c < 4; c++) red[c][tid] = key[c];
+        elas_lds_barrier();
+        for (int s = 128; s > 0; s >>= 1) {
+            if (tid < s)
+                for (int c = 0; c < 4; c++) red[c][tid] = red[c][tid + s] < red[c][tid] ? red[c][tid + s] : red[c][tid];
+            elas_lds_barrier();
+        }
+        if (tid < 6) {
+            const int c = tid < 4 ? tid : tid - 2;  // the last two repeat the right corners
+            const int i = (int)(unsigned)red[c][0];
+            const int d = (int)P.sp[0][2 * i] - (int)P.sp[1][2 * i];
+            const int u = tid < 4 ? cu[c] : P.W - 1 + d, v = cv[c];
+            const float2 a = make_float2((float)u, (float)v), b = make_float2((float)(u - d), (float)v);
+            reinterpret_cast<float2*>(P.sp[0])[n + tid] = a;
+            reinterpret_cast<float2*>(P.sp[1])[n + tid] = b;
+            reinterpret_cast<float2*>(P.hsp[0])[n + tid] = a;
+            reinterpret_cast<float2*>(P.hsp[1])[n + tid] = b;
+        }
+        elas_lds_barrier();
+        if (tid == 0) {
+            P.nsp[0] = n + 6;
+            P.hnsp[0] = n + 6;
+        }
+    }
+}
+
+void elas_launch_corners(const ElasLaunch& L, int grid, hipStream_t s) {
+    hipLaunchKernelGGL(k_corners, dim3(grid), dim3(256), 0, s, L);
+}
 void elas_launch_compact(const ElasLaunch& L, int grid, hipStream_t s) {
     hipLaunchKernelGGL(k_compact, dim3(grid), dim3(256), 0, s, L);
 }

@@ -41,7 +41,7 @@ typedef struct elas_params {
     int32_t incon_window_size;     /* half window of the inconsistency check     */
     int32_t incon_threshold;       /* disparity similarity for consistency       */
     int32_t incon_min_support;     /* min number of consistent support points    */
-    bool    add_corners;           /* not implemented by the reference (ignored) */
+    bool    add_corners;           /* corner support points (see below)          */
     int32_t grid_size;             /* cell size of the disparity grid            */
     float   beta;                  /* image likelihood parameter                 */
     float   gamma;                 /* prior constant                             */
@@ -55,9 +55,24 @@ typedef struct elas_params {
     bool    filter_median;         /* optional median filter                     */
     bool    filter_adaptive_mean;  /* optional adaptive mean filter              */
     bool    postprocess_only_left; /* post-process D1 only                       */
-    bool    subsampling;           /* not implemented by the reference (ignored) */
+    bool    subsampling;           /* refused: ELAS_ERR_UNSUPPORTED (see below)  */
 } elas_params;
 
+/* Two switches the reference declares but never implements (src/Elas.h:23,
+ * 36-38):
+ *   add_corners  after compaction, each image corner (0,0), (0,H-1), (W-1,0),
+ *                (W-1,H-1) becomes a support point with the disparity of the
+ *                nearest support point (smallest squared distance, the first in
+ *                compaction order on ties), and the two right corners are
+ *                added again at u = W-1+d so the right image's mesh reaches its
+ *                right border: 6 points after the nsp lattice points (only when
+ *                nsp >= 1).  The disparity grid stays the lattice's.  This is
+ *                this project's reading of the field comment; the oracle
+ *                defines it the same way.
+ *   subsampling  (D1/D2 of W/2 x H/2) is refused with ELAS_ERR_UNSUPPORTED by
+ *                create and set_params: the reference gives no semantics for
+ *                which pixels are matched or for the left/right check at half
+ *                resolution. */
 typedef enum elas_status {
     ELAS_OK = 0,
     ELAS_ERR_INVALID_ARGUMENT = 1, /* bad dims, null pointer, bad parameters     */

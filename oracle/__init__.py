@@ -163,6 +163,17 @@ def triangulate(sp):
     return tri[:n].copy()
 
 
+def add_corners(sp0, sp1, W, H):
+    """The support points with the corner points of add_corners appended
+    (eo_add_corners): (sp0, sp1) as float32 [n][2] arrays."""
+    n = len(sp0)
+    a = np.zeros((n + 6, 2), np.float32)
+    b = np.zeros((n + 6, 2), np.float32)
+    a[:n], b[:n] = sp0, sp1
+    m = lib().eo_add_corners(_p(a), _p(b), n, int(W), int(H))
+    return a[:m].copy(), b[:m].copy()
+
+
 def plane(can, step, sp, tri, right):
     can = np.ascontiguousarray(can, np.int32)
     abc = np.zeros(3, np.float32)
@@ -245,11 +256,11 @@ def process(left, right, prm=None, dump=True):
         step = prm.candidate_stepsize
         cw, ch = W // step, H // step
         gw, gh = grid_dims(W, H, prm)
-        cap = 2 * cw * ch + 8
+        cap = 2 * (cw * ch + 8) + 8
         bufs = dict(desc0=np.zeros((H, W, 16), np.uint8), desc1=np.zeros((H, W, 16), np.uint8),
                     can_left=np.zeros((ch, cw), np.int32), can_lr=np.zeros((ch, cw), np.int32),
                     can_incon=np.zeros((ch, cw), np.int32), can_final=np.zeros((ch, cw), np.int32),
-                    sp0=np.zeros((cw * ch + 1, 2), np.float32), sp1=np.zeros((cw * ch + 1, 2), np.float32),
+                    sp0=np.zeros((cw * ch + 8, 2), np.float32), sp1=np.zeros((cw * ch + 8, 2), np.float32),
                     grid0=np.zeros((gh, gw, prm.disp_max + 2), np.int32),
                     grid1=np.zeros((gh, gw, prm.disp_max + 2), np.int32),
                     tri0=np.zeros((cap, 3), np.int32), tri1=np.zeros((cap, 3), np.int32),

@@ -18,7 +18,10 @@
  *   - overlapping triangle spans: the highest triangle index wins
  *     (KerComputeDisparity.cu:128-150 races); float->int casts saturate with
  *     NaN -> 0 (the GPU conversion), FP contraction is off (build flag);
- *   - every compile-time macro of KernelsCommon.h:8-27 is read from elas_params.
+ *   - every compile-time macro of KernelsCommon.h:8-27 is read from elas_params;
+ *   - add_corners, declared but not implemented by the reference (Elas.h:23,
+ *     "add support points at image corners with nearest neighbor
+ *     disparities"), is this project's definition (eo_add_corners).
  *
  * Parity pins (tests/): closed forms for Sobel/descriptor/P/ratio test,
  * brute-force numpy restatements of the support rule (closed form of the
@@ -268,6 +271,44 @@ int eo_compact(const int32_t* can, int cw, int ch, int step, float* sp0, float* 
     return n;
 }
 
+/* ---- Corner support points (add_corners, Elas.h:23) ---------------------------
+ * The reference declares the switch and never implements it; its field
+ * comment is the specification.  Definition used here and on the GPU: each
+ * image corner (0, 0), (0, H-1), (W-1, 0), (W-1, H-1) becomes a support point
+ * with the disparity d of the nearest support point (smallest squared
+ * distance in pixels to its left-image position, the first in compaction
+ * order on ties), appended after the nsp support points in that order; then,
+ * so that the right image's mesh also reaches its right border, the two right
+ * corners again at u = W-1+d (their right-image position is u - d = W-1).
+ * Each point is (u, v) in sp0 and (u - d, v) in sp1.  Only when nsp >= 1;
+ * returns the new count. */
+int eo_add_corners(float* sp0, float* sp1, int nsp, int W, int H) {
+    if (nsp < 1) return nsp;
+    const int cu[4] = {0, 0, W - 1, W - 1}, cv[4] = {0, H - 1, 0, H - 1};
+    int cd[4];
+    for (int c = 0; c < 4; c++) {
+        long best = -1;
+        for (int i = 0; i < nsp; i++) {
+            const long du = (long)sp0[2 * i] - cu[c], dv = (long)sp0[2 * i + 1] - cv[c];
+            const long dist = du * du + dv * dv;
+            if (best < 0 || dist < best) {
+                best = dist;
+                cd[c] = (int)(sp0[2 * i] - sp1[2 * i]);
+            }
+        }
+    }
+    const int u[6] = {cu[0], cu[1], cu[2], cu[3], W - 1 + cd[2], W - 1 + cd[3]};
+    const int v[6] = {cv[0], cv[1], cv[2], cv[3], 0, H - 1};
+    const int d[6] = {cd[0], cd[1], cd[2], cd[3], cd[2], cd[3]};
+    for (int k = 0; k < 6; k++) {
+        sp0[2 * (nsp + k)] = (float)u[k];
+        sp0[2 * (nsp + k) + 1] = (float)v[k];
+        sp1[2 * (nsp + k)] = (float)(u[k] - d[k]);
+        sp1[2 * (nsp + k) + 1] = (float)v[k];
+    }
+    return nsp + 6;
+}
+
 /* ---- Disparity grid, KerGrid.cu:9-126 + Elas.cu:403-464 ------------------ */
 void eo_grid_dims(int W, int H, const elas_params* p, int* gw, int* gh) {
     *gw = (int32_t)ceil((float)W / (float)p->grid_size);
@@ -378,15 +419,23 @@ static bool solve3(double A[3][3], double B[3]) {
     return true;
 }
 
-/* Plane (a,b,c) of one triangle as floats; right=1 shifts corners by -d. */
-void eo_plane(const int32_t* can, int cw, int step, const float* sp, const int32_t* t, int right,
-              float* abc) {
+/* A support point's disparity: from the lattice at its position
+ * (KerComputeDisparity.cu:54-56), or, when sp1 is given, as sp0.u - sp1.u,
+ * which is the same number for lattice points and the only one for the
+ * corner points of eo_add_corners. */
+static inline int32_t point_d(const int32_t* can, int cw, int step, const float* sp, const float* sp1, int j) {
+    if (sp1) return (int32_t)(sp[2 * j] - sp1[2 * j]);
+    return can[((int32_t)sp[2 * j + 1] * cw + (int32_t)sp[2 * j]) / step];
+}
+
+static void plane_of(const int32_t* can, int cw, int step, const float* sp, const float* sp1, const int32_t* t,
+                     int right, float* abc) {
     int32_t tu[3], tv[3], td[3];
     double A[3][3], b[3];
     for (int i = 0; i < 3; i++) {
         tu[i] = (int32_t)sp[2 * t[i]];
         tv[i] = (int32_t)sp[2 * t[i] + 1];
-        td[i] = can[(tv[i] * cw + tu[i]) / step];
+        td[i] = point_d(can, cw, step, sp, sp1, t[i]);
         A[i][0] = right ? (double)(tu[i] - td[i]) : (double)tu[i];
         A[i][1] = tv[i];
         A[i][2] = 1;
@@ -396,21 +445,28 @@ void eo_plane(const int32_t* can, int cw, int step, const float* sp, const int32
     else { abc[0] = 0; abc[1] = 0; abc[2] = 0; }
 }
 
+/* Plane (a,b,c) of one triangle as floats; right=1 shifts corners by -d. */
+void eo_plane(const int32_t* can, int cw, int step, const float* sp, const int32_t* t, int right,
+              float* abc) {
+    plane_of(can, cw, step, sp, nullptr, t, right, abc);
+}
+
 /* Rasterise every triangle in index order into mean (W*H, caller-initialised
  * to -1): a later triangle overwrites an earlier one (highest index wins).
- * Span rows are clamped to [0, H) (the reference would write out of bounds). */
-void eo_raster(const int32_t* can, int cw, const float* sp0, const int32_t* tri, int ntri, int right,
-               int W, int H, const elas_params* p, float* mean) {
+ * Span rows are clamped to [0, H) (the reference would write out of bounds).
+ * With sp1, point disparities are sp0.u - sp1.u (see point_d). */
+void eo_raster_sp(const int32_t* can, int cw, const float* sp0, const float* sp1, const int32_t* tri, int ntri,
+                  int right, int W, int H, const elas_params* p, float* mean) {
     const int step = p->candidate_stepsize;
     for (int t = 0; t < ntri; t++) {
         float abc[3];
-        eo_plane(can, cw, step, sp0, tri + 3 * t, right, abc);
+        plane_of(can, cw, step, sp0, sp1, tri + 3 * t, right, abc);
         const float pa = abc[0], pb = abc[1], pc = abc[2];
         int32_t tu[3], tv[3], td[3];
         for (int i = 0; i < 3; i++) {
             tu[i] = (int32_t)sp0[2 * tri[3 * t + i]];
             tv[i] = (int32_t)sp0[2 * tri[3 * t + i] + 1];
-            td[i] = can[(tv[i] * cw + tu[i]) / step];
+            td[i] = point_d(can, cw, step, sp0, sp1, tri[3 * t + i]);
             if (right) tu[i] -= td[i];
         }
         /* insertion-style sort by u, :101-110 */
@@ -442,6 +498,11 @@ void eo_raster(const int32_t* can, int cw, const float* sp0, const int32_t* tri,
             }
         }
     }
+}
+
+void eo_raster(const int32_t* can, int cw, const float* sp0, const int32_t* tri, int ntri, int right,
+               int W, int H, const elas_params* p, float* mean) {
+    eo_raster_sp(can, cw, sp0, nullptr, tri, ntri, right, W, H, p, mean);
 }
 
 /* ---- Dense matching, KerComputeDisparity.cu:156-316,378-402 -------------- */
@@ -562,8 +623,9 @@ int eo_process(const uint8_t* L, const uint8_t* R, int W, int H, const elas_para
     if (dump && dump->can_incon) memcpy(dump->can_incon, can2.data(), 4 * can.size());
     eo_redun_filter(can2.data(), cw, ch);
     if (dump && dump->can_final) memcpy(dump->can_final, can2.data(), 4 * can.size());
-    std::vector<float> sp0(2 * (size_t)cw * ch + 2), sp1(2 * (size_t)cw * ch + 2);
+    std::vector<float> sp0(2 * (size_t)cw * ch + 16), sp1(2 * (size_t)cw * ch + 16);
     int nsp = eo_compact(can2.data(), cw, ch, step, sp0.data(), sp1.data());
+    if (p->add_corners) nsp = eo_add_corners(sp0.data(), sp1.data(), nsp, W, H);
     const size_t gcell = (size_t)p->disp_max + 2;
     std::vector<int32_t> grid0((size_t)gw * gh * gcell), grid1((size_t)gw * gh * gcell);
     eo_grid(can2.data(), cw, ch, W, H, p, grid0.data(), grid1.data());
@@ -587,8 +649,9 @@ int eo_process(const uint8_t* L, const uint8_t* R, int W, int H, const elas_para
     int nt1 = eo_triangulate(sp1.data(), nsp, tri1.data(), cap);
     if (nt0 < 0 || nt1 < 0) return ELAS_ERR_TOO_FEW_POINTS;
     std::vector<float> mean0(npx, -1.0f), mean1(npx, -1.0f);
-    eo_raster(can2.data(), cw, sp0.data(), tri0.data(), nt0, 0, W, H, p, mean0.data());
-    eo_raster(can2.data(), cw, sp0.data(), tri1.data(), nt1, 1, W, H, p, mean1.data());
+    const float* dsrc = p->add_corners ? sp1.data() : nullptr;  // corners carry their own d
+    eo_raster_sp(can2.data(), cw, sp0.data(), dsrc, tri0.data(), nt0, 0, W, H, p, mean0.data());
+    eo_raster_sp(can2.data(), cw, sp0.data(), dsrc, tri1.data(), nt1, 1, W, H, p, mean1.data());
     eo_dense(grid0.data(), desc0.data(), desc1.data(), mean0.data(), 0, W, H, p, D1, &wden);
     eo_dense(grid1.data(), desc0.data(), desc1.data(), mean1.data(), 1, W, H, p, D2, &wden);
     if (dump) {
