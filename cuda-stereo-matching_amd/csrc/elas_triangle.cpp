@@ -384,8 +384,8 @@ public:
         return *pool;
     }
     // remaining: jobs not finished, under m; left: the same count, lowered by
-    // a worker as its last access of the batch, so a waiter that polls it and
-    // then takes m once may free the batch
+    // a worker as its last access of the batch (after releasing m), so a
+    // waiter frees the batch only once left is 0
     struct Batch {
         std::mutex m;
         std::condition_variable cv;
@@ -417,8 +417,13 @@ public:
         }
         Batch b;
         enqueue(jobs, n, &b);
-        std::unique_lock<std::mutex> g(b.m);
-        b.cv.wait(g, [&] { return b.remaining == 0; });
+        {
+            std::unique_lock<std::mutex> g(b.m);
+            b.cv.wait(g, [&] { return b.remaining == 0; });
+        }
+        // a worker's last access of the batch is its decrement of `left`,
+        // after it released m: b (on this stack) outlives it
+        while (b.left.load(std::memory_order_acquire) > 0) _mm_pause();
     }
     void enqueue(ElasTriJob* jobs, int n, Batch* b) {
         b->remaining = n;
@@ -606,8 +611,13 @@ void elas_tri_wait(ElasTriTicket* t) {
     TriPool& pool = TriPool::get();
     while (t->batch.left.load(std::memory_order_acquire) > 0 && now_s() - t0 < ELAS_TRI_SPIN_S)
         if (!pool.help_one()) _mm_pause();
-    std::unique_lock<std::mutex> g(t->batch.m);
-    t->batch.cv.wait(g, [&] { return t->batch.remaining == 0; });
+    {
+        std::unique_lock<std::mutex> g(t->batch.m);
+        t->batch.cv.wait(g, [&] { return t->batch.remaining == 0; });
+    }
+    // the ticket may be freed once no worker can touch it: a worker's last
+    // access is its decrement of `left`, after it released m
+    while (t->batch.left.load(std::memory_order_acquire) > 0) _mm_pause();
 }
 
 void elas_tri_release(ElasTriTicket* t) {
