@@ -58,19 +58,46 @@ namespace TRILIB {
 namespace {
 
 // triangulate()'s steps up to the divide-and-conquer (src/triangle.cpp:
-// 8503-8530 and delaunay(), :6237-6250); byinput[i] is input point i's vertex.
-vertex* elas_begin(struct mesh& m, struct behavior& b, const float* pts, int n) {
+// 8503-8530 and delaunay(), :6237-6250) for n >= 3 points; byinput[i] is
+// input point i's vertex.  transfernodes() (:7638-7700) is taken apart for
+// points without attributes or markers: one tight pass takes the vertices
+// from the pool in input order and fills them with the same coordinates,
+// marks, types and bounding box (byinput needs no traversal).
+vertex* elas_begin(struct mesh& mesh, struct behavior& b, const float* pts, int n) {
+    struct mesh* m = &mesh;  // Triangle's vertex macros name the mesh `m`
     char sw[] = "zQB";
     char* swp = sw;
-    triangleinit(&m);
+    triangleinit(m);
     parsecommandline(1, &swp, &b);
-    m.steinerleft = b.steiner;
-    transfernodes(&m, &b, (float*)pts, (float*)NULL, (int*)NULL, n, 0);
-    m.eextras = 0;
-    initializetrisubpools(&m, &b);
+    m->steinerleft = b.steiner;
+    m->invertices = n;
+    m->mesh_dim = 2;
+    m->nextras = 0;
+    m->readnodefile = 0;
+    b.weighted = 0;
+    initializevertexpool(m, &b);
     vertex* byinput = (vertex*)trimalloc(n * (int)sizeof(vertex));
-    traversalinit(&m.vertices);
-    for (int i = 0; i < n; i++) byinput[i] = vertextraverse(&m);
+    float xmin = pts[0], xmax = pts[0], ymin = pts[1], ymax = pts[1];
+    for (int i = 0; i < n; i++) {
+        vertex v = (vertex)poolalloc(&m->vertices);
+        byinput[i] = v;
+        const float x = pts[2 * i], y = pts[2 * i + 1];
+        v[0] = x;
+        v[1] = y;
+        setvertexmark(v, 0);
+        setvertextype(v, INPUTVERTEX);
+        xmin = (x < xmin) ? x : xmin;
+        xmax = (x > xmax) ? x : xmax;
+        ymin = (y < ymin) ? y : ymin;
+        ymax = (y > ymax) ? y : ymax;
+    }
+    m->xmin = xmin;
+    m->xmax = xmax;
+    m->ymin = ymin;
+    m->ymax = ymax;
+    m->xminextreme = 10 * m->xmin - 9 * m->xmax;
+    m->eextras = 0;
+    initializetrisubpools(m, &b);
     return byinput;
 }
 
@@ -93,6 +120,16 @@ struct SplitRun {
     int root;
     std::vector<int> todo;  // nodes of the current task set
     int32_t* dst;
+    // Vertex numbers.  numbernodes() (src/triangle.cpp:7781-7796) numbers
+    // the vertices in pool order, and here that is their allocation order:
+    // no vertex is ever freed (vertexdealloc() has no caller) and "zQB" does
+    // not jettison duplicates.  initializevertexpool() places every input
+    // vertex in the pool's first block, so a vertex's number is its offset
+    // from the first one over the item size: with a power-of-two item size
+    // (16 bytes here) the read shifts by vshift, else (vshift -1) it reads
+    // the marks numbernodes() wrote.
+    const char* vbase;
+    int vshift;
 };
 
 // post-order build of the cut: nodes of more than 3 vertices above `depth`
@@ -170,9 +207,15 @@ void split_read(void* arg, int t) {
         org(tri, p1);
         dest(tri, p2);
         apex(tri, p3);
-        nd.tris.push_back(vertexmark(p1));
-        nd.tris.push_back(vertexmark(p2));
-        nd.tris.push_back(vertexmark(p3));
+        if (r->vshift >= 0) {
+            nd.tris.push_back((int32_t)(((const char*)p1 - r->vbase) >> r->vshift));
+            nd.tris.push_back((int32_t)(((const char*)p2 - r->vbase) >> r->vshift));
+            nd.tris.push_back((int32_t)(((const char*)p3 - r->vbase) >> r->vshift));
+        } else {
+            nd.tris.push_back(vertexmark(p1));
+            nd.tris.push_back(vertexmark(p2));
+            nd.tris.push_back(vertexmark(p3));
+        }
     }
     if (t != r->root) pooldeinit(pool);
 }
@@ -215,7 +258,14 @@ int elas_finish_split(struct mesh& mesh, struct behavior& b, vertex* vert, int c
     }
     trifree((int*)r.sortarray);
     removeghosts(&mesh, &b, &r.nodes[r.root].farleft);
-    numbernodes(&mesh, &b);
+    traversalinit(&mesh.vertices);
+    r.vbase = (const char*)mesh.vertices.pathitem;
+    r.vshift = -1;
+    const int ib = mesh.vertices.itembytes;
+    if (ib > 0 && (ib & (ib - 1)) == 0 && mesh.vertices.maxitems <= mesh.vertices.itemsfirstblock)
+        for (r.vshift = 0; (1 << r.vshift) < ib;) r.vshift++;
+    else
+        numbernodes(&mesh, &b);
     par(parctx, (int)r.nodes.size(), split_read, &r);
     size_t total = 0;
     for (SplitNode& nd : r.nodes) {

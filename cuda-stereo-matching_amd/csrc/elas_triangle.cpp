@@ -14,6 +14,7 @@
 #include <immintrin.h>
 #include <chrono>
 #include <cmath>
+#include <cstdint>
 #include <condition_variable>
 #include <cstdlib>
 #include <cstring>
@@ -108,16 +109,6 @@ double now_s() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
-// Stable counting sort of the indices in `in` by the integer keys key[]
-// (already offset to 0..range-1).
-void counting_sort(const int32_t* key, int n, long range, const int32_t* in, int32_t* out,
-                   std::vector<int32_t>& cnt) {
-    cnt.assign((size_t)range + 1, 0);
-    for (int i = 0; i < n; i++) cnt[key[in[i]] + 1]++;
-    for (long b = 0; b < range; b++) cnt[b + 1] += cnt[b];
-    for (int i = 0; i < n; i++) out[cnt[key[in[i]]]++] = in[i];
-}
-
 }  // namespace
 
 // ---- block cache ---------------------------------------------------------------
@@ -188,94 +179,210 @@ void elas_tri_block_free(void* ptr) {
 void ElasTriOrder::resize(int n) {
     lx.resize(n);
     ly.resize(n);
-    tmp.resize(n);
     arr.resize(n);
+    rank.resize(n);
+    px.resize(n);
+    py.resize(n);
+    tl.resize(n);
+    tr.resize(n);
+}
+
+void ElasTriOrder::pack() {
+    const int n = (int)lx.size();
+    for (int k = 0; k < n; k++) rank[ly[k]] = k;
+    for (int k = 0; k < n; k++) px[k] = (uint64_t)(uint32_t)k << 32 | (uint32_t)rank[lx[k]];
+    for (int k = 0; k < n; k++) rank[lx[k]] = k;
+    for (int k = 0; k < n; k++) py[k] = (uint64_t)(uint32_t)rank[ly[k]] << 32 | (uint32_t)k;
 }
 
 void ElasTriOrder::cut(int lo, int size, int axis, int levels) {
     if (size <= 3) {
-        for (int k = 0; k < size; k++) arr[lo + k] = lx[lo + k];
+        for (int k = 0; k < size; k++) arr[lo + k] = lx[px[lo + k] >> 32];
         return;
     }
     if (levels == 0) return;
     const int divider = size >> 1;
-    int32_t* cutl = axis ? ly.data() : lx.data();
-    int32_t* other = axis ? lx.data() : ly.data();
-    // the side marks are per thread: subtrees cut at once mark points whose
-    // indices share cache lines
-    thread_local std::vector<uint8_t> side;
-    if (side.size() < lx.size()) side.resize(lx.size());
-    uint8_t* sd = side.data();
-    for (int k = 0; k < divider; k++) sd[cutl[lo + k]] = 0;
-    for (int k = divider; k < size; k++) sd[cutl[lo + k]] = 1;
-    // branch-free stable split of the other list (the side of a point is not
-    // predictable), through the node's own range of tmp
-    int32_t* t = tmp.data() + lo;
-    int a = 0, b = divider;
+    const uint64_t* cutl = axis ? py.data() : px.data();
+    uint64_t* other = axis ? px.data() : py.data();
+    const int sh = axis ? 0 : 32;
+    const uint32_t last = (uint32_t)(cutl[lo + divider - 1] >> sh);  // rank of the left's last point
+    // stable split of the other list without branches (the side of a point
+    // is not predictable): every entry is stored to both sides and only its
+    // own side's cursor moves on.  The store past a side's end lands inside
+    // the node's range of tl / tr: a < divider and b < size - divider until
+    // the last entry.
+    uint64_t* l = tl.data() + lo;
+    uint64_t* r = tr.data() + lo;
+    const uint64_t* o = other + lo;
+    int a = 0, b = 0;
     for (int k = 0; k < size; k++) {
-        const int32_t i = other[lo + k];
-        const int r = sd[i];
-        t[r ? b : a] = i;
-        a += 1 - r;
-        b += r;
+        const uint64_t e = o[k];
+        const int right = (uint32_t)(e >> sh) > last;
+        l[a] = e;
+        r[b] = e;
+        a += 1 - right;
+        b += right;
     }
-    memcpy(other + lo, t, sizeof(int32_t) * (size_t)size);
+    memcpy(other + lo, l, sizeof(uint64_t) * (size_t)divider);
+    memcpy(other + lo + divider, r, sizeof(uint64_t) * (size_t)(size - divider));
     cut(lo, divider, 1 - axis, levels - 1);
     cut(lo + divider, size - divider, 1 - axis, levels - 1);
+}
+
+// The integer keys of n points [x0, y0, ...]: kx, ky, their bounds in
+// lohi (min x, max x, min y, max y); returns nonzero when a coordinate is
+// not an integer of magnitude below 2^24 (NaN included).  Eight points per
+// step with AVX2 (the library's Triangle copies are built for x86-64-v3), the
+// rest one by one; a coordinate out of range converts as 0 and fails the
+// exactness test.
+static __attribute__((target("avx2"))) int convert_keys(const float* p, int n, int32_t* kx, int32_t* ky, int* lohi) {
+    int i = 0, bad = 0;
+    int lox = INT32_MAX, hix = INT32_MIN, loy = INT32_MAX, hiy = INT32_MIN;
+    if (n >= 8) {
+        const __m256 lim = _mm256_set1_ps(16777216.0f);
+        const __m256 sign = _mm256_set1_ps(-0.0f);
+        __m256i vlox = _mm256_set1_epi32(INT32_MAX), vhix = _mm256_set1_epi32(INT32_MIN);
+        __m256i vloy = vlox, vhiy = vhix;
+        __m256 vbad = _mm256_setzero_ps();
+        for (; i + 8 <= n; i += 8) {
+            const __m256 a = _mm256_loadu_ps(p + 2 * i), b = _mm256_loadu_ps(p + 2 * i + 8);
+            // (x0 y0 x1 y1 | x2 y2 x3 y3), (x4 y4 x5 y5 | x6 y6 x7 y7) -> x0..x7, y0..y7
+            const __m256 xs = _mm256_castpd_ps(_mm256_permute4x64_pd(_mm256_castps_pd(_mm256_shuffle_ps(a, b, 0x88)), 0xD8));
+            const __m256 ys = _mm256_castpd_ps(_mm256_permute4x64_pd(_mm256_castps_pd(_mm256_shuffle_ps(a, b, 0xDD)), 0xD8));
+            const __m256 okx = _mm256_cmp_ps(_mm256_andnot_ps(sign, xs), lim, _CMP_LT_OQ);
+            const __m256 oky = _mm256_cmp_ps(_mm256_andnot_ps(sign, ys), lim, _CMP_LT_OQ);
+            const __m256i ix = _mm256_cvttps_epi32(_mm256_and_ps(xs, okx));
+            const __m256i iy = _mm256_cvttps_epi32(_mm256_and_ps(ys, oky));
+            vbad = _mm256_or_ps(vbad, _mm256_cmp_ps(_mm256_cvtepi32_ps(ix), xs, _CMP_NEQ_UQ));
+            vbad = _mm256_or_ps(vbad, _mm256_cmp_ps(_mm256_cvtepi32_ps(iy), ys, _CMP_NEQ_UQ));
+            _mm256_storeu_si256((__m256i*)(kx + i), ix);
+            _mm256_storeu_si256((__m256i*)(ky + i), iy);
+            vlox = _mm256_min_epi32(vlox, ix);
+            vhix = _mm256_max_epi32(vhix, ix);
+            vloy = _mm256_min_epi32(vloy, iy);
+            vhiy = _mm256_max_epi32(vhiy, iy);
+        }
+        bad = _mm256_movemask_ps(vbad);
+        alignas(32) int32_t t[4][8];
+        _mm256_store_si256((__m256i*)t[0], vlox);
+        _mm256_store_si256((__m256i*)t[1], vhix);
+        _mm256_store_si256((__m256i*)t[2], vloy);
+        _mm256_store_si256((__m256i*)t[3], vhiy);
+        for (int k = 0; k < 8; k++) {
+            lox = std::min(lox, t[0][k]);
+            hix = std::max(hix, t[1][k]);
+            loy = std::min(loy, t[2][k]);
+            hiy = std::max(hiy, t[3][k]);
+        }
+    }
+    for (; i < n; i++) {
+        const float x = p[2 * i], y = p[2 * i + 1];
+        const int ix = std::fabs(x) < 16777216.0f ? (int)x : 0, iy = std::fabs(y) < 16777216.0f ? (int)y : 0;
+        bad |= ((float)ix != x) | ((float)iy != y);
+        kx[i] = ix;
+        ky[i] = iy;
+        lox = std::min(lox, ix);
+        hix = std::max(hix, ix);
+        loy = std::min(loy, iy);
+        hiy = std::max(hiy, iy);
+    }
+    lohi[0] = lox;
+    lohi[1] = hix;
+    lohi[2] = loy;
+    lohi[3] = hiy;
+    return bad;
 }
 
 // Support points have integer coordinates, so both sort orders come from
 // counting sorts, in O(n) instead of Triangle's randomised quicksort and
 // quickselect, which take about half of a support-point mesh's serial time.
+// The (y, x) order comes first: support points come row by row (y
+// non-decreasing), the left image's also in (y, x) order, the right image's
+// nearly so (a row's points move by their disparities), so it is the input
+// order, or the input with each row sorted by insertion, or (any other
+// input, or rows far from sorted) two counting sorts.  One stable counting
+// sort of that order by x then gives the (x, y) order, each point's x rank
+// and, with its y rank k, both packed entries.  Equal points are neighbours
+// in (y, x) order.
 bool elas_tri_order_integer(const float* p, int n, ElasTriOrder& o) {
     if (n < 3) return false;
-    thread_local std::vector<int32_t> cnt, a, b, key[2];
-    int lo[2] = {0, 0}, hi[2] = {0, 0};
-    for (int c = 0; c < 2; c++) {
-        key[c].resize(n);
-        int32_t* kc = key[c].data();
-        lo[c] = hi[c] = 0;
-        for (int i = 0; i < n; i++) {
-            const float v = p[2 * i + c];
-            if (!(std::fabs(v) < 16777216.0f)) return false;
-            const int iv = (int)v;
-            if ((float)iv != v) return false;
-            kc[i] = iv;
-            if (i == 0 || iv < lo[c]) lo[c] = iv;
-            if (i == 0 || iv > hi[c]) hi[c] = iv;
+    thread_local std::vector<int32_t> cnt, kxv, kyv, tmp;
+    kxv.resize(n);
+    kyv.resize(n);
+    int32_t* kx = kxv.data();
+    int32_t* ky = kyv.data();
+    // integer coordinates below 2^24 in magnitude
+    int lohi[4];
+    int bad = convert_keys(p, n, kx, ky, lohi);
+    const int lox = lohi[0], hix = lohi[1], loy = lohi[2], hiy = lohi[3];
+    if (bad) return false;
+    int ys = 1, yxs = 1;
+    for (int i = 1; i < n; i++) {
+        const int dy = ky[i] - ky[i - 1];
+        ys &= dy >= 0;
+        yxs &= (dy > 0) | ((dy == 0) & (kx[i] > kx[i - 1]));
+    }
+    const bool ysorted = ys, yxsorted = yxs;
+    const long rx = (long)hix - lox + 1, ry = (long)hiy - loy + 1;
+    if (rx > 4L * n + 65536 || ry > 4L * n + 65536) return false;
+    o.resize(n);
+    int32_t* ly = o.ly.data();
+    for (int i = 0; i < n; i++) ly[i] = i;
+    bool done = yxsorted;  // strictly increasing (y, x): no equal points either
+    if (!done && ysorted) {
+        // insertion sort of each row by x, while the moves stay within 8n
+        long moves = 0;
+        done = true;
+        for (int s = 0; s < n && done;) {
+            int e = s + 1;
+            while (e < n && ky[e] == ky[s]) e++;
+            for (int j = s + 1; j < e && moves <= 8L * n; j++) {
+                const int32_t v = ly[j], kv = kx[v];
+                int m = j;
+                while (m > s && kx[ly[m - 1]] > kv) {
+                    ly[m] = ly[m - 1];
+                    m--;
+                }
+                moves += j - m;
+                ly[m] = v;
+            }
+            if (moves > 8L * n) {
+                done = false;
+                break;
+            }
+            for (int j = s + 1; j < e; j++)
+                if (kx[ly[j]] == kx[ly[j - 1]]) return false;
+            s = e;
         }
     }
-    const long rx = (long)hi[0] - lo[0] + 1, ry = (long)hi[1] - lo[1] + 1;
-    if (rx > 4L * n + 65536 || ry > 4L * n + 65536) return false;
-    for (int c = 0; c < 2; c++)
-        for (int i = 0; i < n; i++) key[c][i] -= lo[c];
-    o.resize(n);
-    a.resize(n);
-    b.resize(n);
-    for (int i = 0; i < n; i++) a[i] = i;
-    // support points come row by row (y non-decreasing), the left image's
-    // also in (y, x) order: those passes are the identity
-    bool ysorted = true, yxsorted = true;
-    for (int i = 1; i < n; i++) {
-        const int32_t dy = key[1][i] - key[1][i - 1];
-        ysorted = ysorted && dy >= 0;
-        yxsorted = yxsorted && (dy > 0 || (dy == 0 && key[0][i] > key[0][i - 1]));
+    if (!done) {
+        tmp.resize(n);
+        cnt.assign((size_t)rx + 1, 0);  // by x, then stably by y
+        for (int i = 0; i < n; i++) cnt[kx[i] - lox + 1]++;
+        for (long b = 0; b < rx; b++) cnt[b + 1] += cnt[b];
+        for (int i = 0; i < n; i++) tmp[cnt[kx[i] - lox]++] = i;
+        cnt.assign((size_t)ry + 1, 0);
+        for (int i = 0; i < n; i++) cnt[ky[i] - loy + 1]++;
+        for (long b = 0; b < ry; b++) cnt[b + 1] += cnt[b];
+        for (int k = 0; k < n; k++) ly[cnt[ky[tmp[k]] - loy]++] = tmp[k];
+        for (int k = 1; k < n; k++)
+            if (kx[ly[k]] == kx[ly[k - 1]] && ky[ly[k]] == ky[ly[k - 1]]) return false;
     }
-    if (ysorted) {
-        counting_sort(key[0].data(), n, rx, a.data(), o.lx.data(), cnt);  // by x, stable: (x, y)
-    } else {
-        counting_sort(key[1].data(), n, ry, a.data(), b.data(), cnt);     // by y
-        counting_sort(key[0].data(), n, rx, b.data(), o.lx.data(), cnt);  // then x: (x, y)
-    }
-    if (yxsorted) {
-        o.ly = a;
-    } else {
-        counting_sort(key[0].data(), n, rx, a.data(), b.data(), cnt);     // by x
-        counting_sort(key[1].data(), n, ry, b.data(), o.ly.data(), cnt);  // then y: (y, x)
-    }
-    for (int k = 1; k < n; k++) {
-        const int32_t i = o.lx[k - 1], j = o.lx[k];
-        if (key[0][i] == key[0][j] && key[1][i] == key[1][j]) return false;
+    // (x, y) order, ranks and entries: ly stably by x
+    int32_t* lx = o.lx.data();
+    uint64_t* px = o.px.data();
+    uint64_t* py = o.py.data();
+    cnt.assign((size_t)rx + 1, 0);
+    for (int i = 0; i < n; i++) cnt[kx[i] - lox + 1]++;
+    for (long b = 0; b < rx; b++) cnt[b + 1] += cnt[b];
+    int32_t* c = cnt.data();
+    for (int k = 0; k < n; k++) {
+        const int32_t i = ly[k];
+        const int32_t pos = c[kx[i] - lox]++;
+        const uint64_t e = (uint64_t)(uint32_t)pos << 32 | (uint32_t)k;
+        lx[pos] = i;
+        px[pos] = e;
+        py[k] = e;
     }
     return true;
 }
@@ -285,6 +392,7 @@ void elas_tri_order_sorted(const float* xy, int count, ElasTriOrder& o) {
     for (int i = 0; i < count; i++) o.lx[i] = o.ly[i] = i;
     // (y, x) order: by y, the (x, y) order kept among equal y
     std::stable_sort(o.ly.begin(), o.ly.end(), [xy](int32_t a, int32_t b) { return xy[2 * a + 1] < xy[2 * b + 1]; });
+    o.pack();
 }
 
 namespace {

@@ -32,9 +32,17 @@ struct ElasTriJob {
 // :5586-5605) and go to arr.  `levels` bounds the depth of the cuts made now
 // (-1: down to the leaves): the top of the tree can be cut on one thread and
 // each subtree below it on another, since subtrees touch disjoint ranges.
+// The cuts work on px and py, the two orders as entries (x rank << 32 |
+// y rank), where a point's x rank is its place in lx and its y rank its
+// place in ly: a point belongs to the left of a cut along x exactly when its
+// x rank is at most that of the node's `divider`-th point in x order, so a
+// cut reads its entries in sequence and looks nothing up.  pack() builds px
+// and py from lx and ly.
 struct ElasTriOrder {
-    std::vector<int32_t> lx, ly, tmp, arr;
+    std::vector<int32_t> lx, ly, arr, rank;
+    std::vector<uint64_t> px, py, tl, tr;
     void resize(int n);
+    void pack();
     void cut(int lo, int size, int axis, int levels);
 };
 

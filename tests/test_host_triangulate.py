@@ -82,6 +82,19 @@ def test_non_integer_points_keep_caller_order(pkg, oracle):
     assert np.array_equal(pkg.triangulate(pts), oracle.triangulate(pts))
 
 
+@pytest.mark.parametrize("k", [0, 3, 7, 8, 15, 100, 200, 202])
+def test_one_non_integer_coordinate_anywhere(pkg, oracle, k):
+    """The integer keys are converted eight points at a time, the last n % 8
+    one by one: a single half-integer coordinate in either part must send the
+    set to Triangle's own sort (taken as an integer it would tie with, and be
+    ordered among, the points of its truncated column)."""
+    rng = np.random.default_rng(40 + k)
+    flat = rng.choice(30 * 30, 203, replace=False)
+    pts = np.stack([flat % 30, flat // 30], 1).astype(np.float32)
+    pts[k, k % 2] += 0.5
+    assert np.array_equal(pkg.triangulate(pts), oracle.triangulate(pts))
+
+
 def test_pipeline_meshes_of_c0(pkg, oracle, pairs):
     left, right, _, dmax = pairs("450x375_d64")
     ref = oracle.process(left, right, oracle.params(disp_max=dmax))
