@@ -231,7 +231,7 @@ void split_copy(void* arg, int t) {
 // into dst when it holds cap triangles; returns ntri.
 int elas_finish_split(struct mesh& mesh, struct behavior& b, vertex* vert, int count, ElasTriOrder* order,
                       int depth, ElasParallelFor par, void* parctx, std::vector<int32_t>* out, int32_t* dst,
-                      int64_t cap) {
+                      int64_t cap, bool topcut_done) {
     SplitRun r;
     r.m = &mesh;
     r.b = &b;
@@ -241,9 +241,10 @@ int elas_finish_split(struct mesh& mesh, struct behavior& b, vertex* vert, int c
     r.nodes.reserve(2u << std::min(depth, 12));
     r.root = split_build(r.nodes, 0, count, 0, 0, depth);
     r.dst = nullptr;
-    // the top of the order on this thread (the cuts the split nodes make),
-    // then the leaves, then the merges level by level up to the root
-    order->cut(0, count, 0, depth);
+    // the top of the order (the cuts the split nodes make; already made when
+    // the caller built the order), then the leaves, then the merges level by
+    // level up to the root
+    if (!topcut_done) order->cut(0, count, 0, depth);
     int deepest = 0;
     for (const SplitNode& nd : r.nodes) deepest = std::max(deepest, nd.level);
     r.todo.clear();
@@ -310,8 +311,9 @@ void* elas_tri_setup(const float* pts, int n) {
 }
 
 // The triangulation, corners numbered by input position, into *out (or dst,
-// see elas_finish_split).  `order` is scratch; with `integer` set its lx / ly
-// already hold the input points' sort orders (elas_tri_order_integer), else
+// see elas_finish_split).  `order` is scratch; with `integer` set it already
+// holds the input points' sort orders (elas_tri_order_integer) and its top
+// `depth` levels of cuts, else
 // Triangle's vertexsort() and duplicate removal (:6176-6198) run first and
 // the order is built over the survivors.  Returns ntri, or -1 when a single
 // distinct point is left: Triangle's divconqrecurse() has no case for one
@@ -347,7 +349,7 @@ int elas_tri_finish(void* setup, ElasTriOrder* order, bool integer, int depth, E
         }
     }
     if (count >= 2) {
-        nt = elas_finish_split(st->mesh, st->b, byinput, count, order, depth, par, parctx, out, dst, cap);
+        nt = elas_finish_split(st->mesh, st->b, byinput, count, order, depth, par, parctx, out, dst, cap, integer);
     } else {
         triangledeinit(&st->mesh, &st->b);
         if (out) out->clear();

@@ -115,10 +115,11 @@ double now_s() {
 
 namespace {
 
-#define ELAS_BLOCK_MIN_SHIFT 16                 // 64 KiB
-#define ELAS_BLOCK_CLASSES 16                   // up to 2 GiB
-#define ELAS_BLOCK_CACHE_MAX ((size_t)512 << 20)  // bytes kept for reuse
-#define ELAS_BLOCK_HEADER 64                    // size class in front of a block (keeps 16-B alignment)
+#define ELAS_BLOCK_MIN_SHIFT 16                   // 64 KiB
+#define ELAS_BLOCK_CLASSES 16                     // up to 2 GiB
+#define ELAS_BLOCK_CACHE_MAX ((size_t)512 << 20)  // bytes kept for reuse, shared
+#define ELAS_BLOCK_LOCAL_MAX ((size_t)32 << 20)   // bytes kept for reuse, per thread
+#define ELAS_BLOCK_HEADER 64                      // size class in front of a block (keeps 16-B alignment)
 
 struct BlockCache {
     std::mutex m;
@@ -131,6 +132,34 @@ BlockCache& block_cache() {
     return *c;
 }
 
+size_t class_bytes(int cls) { return (size_t)1 << (ELAS_BLOCK_MIN_SHIFT + cls); }
+
+// Blocks a thread freed go to its own list first: the thread takes them back
+// without a lock, and mostly while their lines are still in its cache.  At
+// thread exit they move to the shared cache (or back to malloc).
+struct LocalCache {
+    std::vector<void*> free_[ELAS_BLOCK_CLASSES];
+    size_t cached = 0;
+    ~LocalCache() {
+        BlockCache& c = block_cache();
+        std::lock_guard<std::mutex> g(c.m);
+        for (int cls = 0; cls < ELAS_BLOCK_CLASSES; cls++)
+            for (void* p : free_[cls]) {
+                if (c.cached + class_bytes(cls) <= ELAS_BLOCK_CACHE_MAX) {
+                    c.free_[cls].push_back(p);
+                    c.cached += class_bytes(cls);
+                } else {
+                    free(p);
+                }
+            }
+    }
+};
+
+LocalCache& local_cache() {
+    thread_local LocalCache c;
+    return c;
+}
+
 }  // namespace
 
 void* elas_tri_block_alloc(size_t bytes) {
@@ -141,19 +170,24 @@ void* elas_tri_block_alloc(size_t bytes) {
         return p + ELAS_BLOCK_HEADER;
     }
     int cls = 0;
-    while (((size_t)1 << (ELAS_BLOCK_MIN_SHIFT + cls)) < bytes + ELAS_BLOCK_HEADER) cls++;
+    while (class_bytes(cls) < bytes + ELAS_BLOCK_HEADER) cls++;
     if (cls >= ELAS_BLOCK_CLASSES) return nullptr;  // Triangle reports out of memory
     char* p = nullptr;
-    {
+    LocalCache& lc = local_cache();
+    if (!lc.free_[cls].empty()) {
+        p = (char*)lc.free_[cls].back();
+        lc.free_[cls].pop_back();
+        lc.cached -= class_bytes(cls);
+    } else {
         BlockCache& c = block_cache();
         std::lock_guard<std::mutex> g(c.m);
         if (!c.free_[cls].empty()) {
             p = (char*)c.free_[cls].back();
             c.free_[cls].pop_back();
-            c.cached -= (size_t)1 << (ELAS_BLOCK_MIN_SHIFT + cls);
+            c.cached -= class_bytes(cls);
         }
     }
-    if (!p) p = (char*)malloc((size_t)1 << (ELAS_BLOCK_MIN_SHIFT + cls));
+    if (!p) p = (char*)malloc(class_bytes(cls));
     if (!p) return nullptr;
     *(int*)p = cls;
     return p + ELAS_BLOCK_HEADER;
@@ -164,7 +198,13 @@ void elas_tri_block_free(void* ptr) {
     char* p = (char*)ptr - ELAS_BLOCK_HEADER;
     const int cls = *(int*)p;
     if (cls >= 0) {
-        const size_t sz = (size_t)1 << (ELAS_BLOCK_MIN_SHIFT + cls);
+        const size_t sz = class_bytes(cls);
+        LocalCache& lc = local_cache();
+        if (lc.cached + sz <= ELAS_BLOCK_LOCAL_MAX) {
+            lc.free_[cls].push_back(p);
+            lc.cached += sz;
+            return;
+        }
         BlockCache& c = block_cache();
         std::lock_guard<std::mutex> g(c.m);
         if (c.cached + sz <= ELAS_BLOCK_CACHE_MAX) {
@@ -439,18 +479,25 @@ struct TriCall {
     ElasTriOrder* order;
     void* setup;
     bool integer;
+    int depth;
 };
 
+// Triangle's setup on one thread; on the other the vertex order and, when
+// it was built here, the top `depth` levels of its cuts
 void tri_prepare(void* arg, int t) {
     TriCall* c = (TriCall*)arg;
-    if (t == 0) c->setup = kSetup[c->k](c->job->pts, c->job->n);
-    else c->integer = elas_tri_order_integer(c->job->pts, c->job->n, *c->order);
+    if (t == 0) {
+        c->setup = kSetup[c->k](c->job->pts, c->job->n);
+    } else {
+        c->integer = elas_tri_order_integer(c->job->pts, c->job->n, *c->order);
+        if (c->integer) c->order->cut(0, c->job->n, 0, c->depth);
+    }
 }
 
 int tri_call(int k, ElasTriJob& j, bool split, void* pool) {
     thread_local ElasTriOrder order;
     const int depth = split ? job_depth(j.n, pool) : 0;
-    TriCall c{k, &j, &order, nullptr, false};
+    TriCall c{k, &j, &order, nullptr, false, depth};
     if (depth > 0) {
         pool_parallel_for(pool, 2, tri_prepare, &c);
     } else {

@@ -46,18 +46,21 @@ struct ElasTriOrder {
     void cut(int lo, int size, int axis, int levels);
 };
 
-// lx and ly of a point set from two counting sorts, when the coordinates are
-// bounded integers and no two points coincide (support points); false
-// otherwise (which duplicate Triangle keeps then depends on its own sort).
+// The order (lx, ly, px, py) of a point set by counting sorts, when the
+// coordinates are bounded integers and no two points coincide (support
+// points); false otherwise (which duplicate Triangle keeps then depends on
+// its own sort).
 bool elas_tri_order_integer(const float* p, int n, ElasTriOrder& o);
 // lx and ly of `count` distinct points given in (x, y) order (positions).
 void elas_tri_order_sorted(const float* xy, int count, ElasTriOrder& o);
 
 // Triangle's allocator in this library (its trimalloc() and trifree() call
-// these): blocks of 64 KiB and more are kept in a process-wide cache by
-// power-of-two size class and handed out again, so the pools of one call
-// reuse memory that earlier calls already touched instead of faulting in
-// fresh pages from several threads at once; smaller blocks go to malloc.
+// these): blocks of 64 KiB and more are kept by power-of-two size class and
+// handed out again, first from the freeing thread's own list (no lock, lines
+// likely still in its cache), then from a process-wide cache, so the pools
+// of one call reuse memory that earlier calls already touched instead of
+// faulting in fresh pages from several threads at once; smaller blocks go
+// to malloc.
 void* elas_tri_block_alloc(size_t bytes);
 void elas_tri_block_free(void* p);
 
