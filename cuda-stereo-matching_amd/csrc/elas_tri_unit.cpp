@@ -25,9 +25,10 @@
 //    removeghosts() frees triangles, after the whole merge.  The top `depth`
 //    levels of that recursion are cut out: every node of the cut gets its
 //    own triangle pool (sharing the mesh's "outer space" triangle, which the
-//    merge never writes), the leaves run divconqrecurse() concurrently, the
-//    internal nodes of a level run mergehulls() concurrently once the level
-//    below is done, and the triangles are read out pool by pool in
+//    merge never writes), the leaves run divconqrecurse() concurrently, an
+//    internal node runs mergehulls() as soon as both its children are done
+//    (on the thread that finished the second), and the triangles are read
+//    out pool by pool in
 //    post-order, each pool in allocation order, skipping the dead ghosts --
 //    the order in which writeelements() meets them in the single pool of a
 //    serial run.  Vertex handles, orientations and the merge arithmetic are
@@ -41,6 +42,8 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
+#include <memory>
 #include <vector>
 
 #include "elas_triangle.h"
@@ -104,6 +107,7 @@ vertex* elas_begin(struct mesh& mesh, struct behavior& b, const float* pts, int 
 struct SplitNode {
     int lo, size, axis, level;
     int left, right;  // child nodes, -1 for a leaf
+    int parent;       // -1 for the root
     struct mesh sub;  // the node's own triangle pool (the root uses the caller's mesh)
     struct otri farleft, farright;
     std::vector<int32_t> tris;  // its live triangles, read out of its pool
@@ -118,7 +122,8 @@ struct SplitRun {
     ElasTriOrder* order;
     std::vector<SplitNode> nodes;
     int root;
-    std::vector<int> todo;  // nodes of the current task set
+    std::vector<int> todo;  // the leaves
+    std::unique_ptr<std::atomic<int>[]> arrived;  // per node: children done
     int32_t* dst;
     // Vertex numbers.  numbernodes() (src/triangle.cpp:7781-7796) numbers
     // the vertices in pool order, and here that is their allocation order:
@@ -140,7 +145,7 @@ int split_build(std::vector<SplitNode>& nodes, int lo, int size, int axis, int l
     nd.size = size;
     nd.axis = axis;
     nd.level = level;
-    nd.left = nd.right = -1;
+    nd.left = nd.right = nd.parent = -1;
     nd.off = 0;
     if (depth > 0 && size > 3) {
         const int divider = size >> 1;
@@ -148,7 +153,9 @@ int split_build(std::vector<SplitNode>& nodes, int lo, int size, int axis, int l
         nd.right = split_build(nodes, lo + divider, size - divider, 1 - axis, level + 1, depth - 1);
     }
     nodes.push_back(std::move(nd));
-    return (int)nodes.size() - 1;
+    const int me = (int)nodes.size() - 1;
+    if (nodes[me].left >= 0) nodes[nodes[me].left].parent = nodes[nodes[me].right].parent = me;
+    return me;
 }
 
 struct mesh* node_mesh(SplitRun* r, int i) { return i == r->root ? r->m : &r->nodes[i].sub; }
@@ -163,7 +170,23 @@ void node_pool(SplitRun* r, int i) {
              r->m->triangles.alignbytes);
 }
 
-// a leaf: the cuts below the split, its part of the order, its triangulation
+// an internal node: mergehulls() of its children's triangulations, as the
+// tail of divconqrecurse() does it
+void split_merge(SplitRun* r, int i) {
+    SplitNode& nd = r->nodes[i];
+    node_pool(r, i);
+    struct otri farleft = r->nodes[nd.left].farleft, innerleft = r->nodes[nd.left].farright;
+    struct otri innerright = r->nodes[nd.right].farleft, farright = r->nodes[nd.right].farright;
+    mergehulls(node_mesh(r, i), r->b, &farleft, &innerleft, &innerright, &farright, nd.axis);
+    nd.farleft = farleft;
+    nd.farright = farright;
+}
+
+// a leaf: the cuts below the split, its part of the order, its
+// triangulation; then up the tree, each merge run by the thread that
+// finished the second of its two children (the first one stops there), so a
+// merge starts as soon as both halves exist.  The counter's acquire-release
+// increment hands the first child's hull and pool over to the merging thread.
 void split_leaf(void* arg, int t) {
     SplitRun* r = (SplitRun*)arg;
     const int i = r->todo[t];
@@ -173,20 +196,10 @@ void split_leaf(void* arg, int t) {
     for (int k = nd.lo; k < nd.lo + nd.size; k++) r->sortarray[k] = r->vert[arr[k]];
     node_pool(r, i);
     divconqrecurse(node_mesh(r, i), r->b, r->sortarray + nd.lo, nd.size, nd.axis, &nd.farleft, &nd.farright);
-}
-
-// an internal node: mergehulls() of its children's triangulations, as the
-// tail of divconqrecurse() does it
-void split_merge(void* arg, int t) {
-    SplitRun* r = (SplitRun*)arg;
-    const int i = r->todo[t];
-    SplitNode& nd = r->nodes[i];
-    node_pool(r, i);
-    struct otri farleft = r->nodes[nd.left].farleft, innerleft = r->nodes[nd.left].farright;
-    struct otri innerright = r->nodes[nd.right].farleft, farright = r->nodes[nd.right].farright;
-    mergehulls(node_mesh(r, i), r->b, &farleft, &innerleft, &innerright, &farright, nd.axis);
-    nd.farleft = farleft;
-    nd.farright = farright;
+    for (int j = nd.parent; j >= 0; j = r->nodes[j].parent) {
+        if (r->arrived[j].fetch_add(1, std::memory_order_acq_rel) == 0) break;
+        split_merge(r, j);
+    }
 }
 
 // a node's live triangles in allocation order, corners as writeelements()
@@ -242,21 +255,15 @@ int elas_finish_split(struct mesh& mesh, struct behavior& b, vertex* vert, int c
     r.root = split_build(r.nodes, 0, count, 0, 0, depth);
     r.dst = nullptr;
     // the top of the order (the cuts the split nodes make; already made when
-    // the caller built the order), then the leaves, then the merges level by
-    // level up to the root
+    // the caller built the order), then the leaves, each followed by the
+    // merges its completion unlocks, up to the root
     if (!topcut_done) order->cut(0, count, 0, depth);
-    int deepest = 0;
-    for (const SplitNode& nd : r.nodes) deepest = std::max(deepest, nd.level);
+    r.arrived.reset(new std::atomic<int>[r.nodes.size()]);
+    for (size_t i = 0; i < r.nodes.size(); i++) r.arrived[i].store(0, std::memory_order_relaxed);
     r.todo.clear();
     for (int i = 0; i < (int)r.nodes.size(); i++)
         if (r.nodes[i].left < 0) r.todo.push_back(i);
     par(parctx, (int)r.todo.size(), split_leaf, &r);
-    for (int lv = deepest - 1; lv >= 0; lv--) {
-        r.todo.clear();
-        for (int i = 0; i < (int)r.nodes.size(); i++)
-            if (r.nodes[i].left >= 0 && r.nodes[i].level == lv) r.todo.push_back(i);
-        if (!r.todo.empty()) par(parctx, (int)r.todo.size(), split_merge, &r);
-    }
     trifree((int*)r.sortarray);
     removeghosts(&mesh, &b, &r.nodes[r.root].farleft);
     traversalinit(&mesh.vertices);
