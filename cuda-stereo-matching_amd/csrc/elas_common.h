@@ -14,6 +14,11 @@
 #define ELAS_DENSE_WAVES 8   // waves per dense workgroup; a wave covers dcpw cells per step
 #define ELAS_DENSE_THREADS (64 * ELAS_DENSE_WAVES)
 #define ELAS_MAX_DISP 511    // disp_max envelope (dense key packs d in 10 bits)
+#define ELAS_MAX_KEY_BIAS 16384  // largest -P(k) accepted: cost + bias stays below 2^15 in the dense key
+
+// 16-bit entries of one cell's candidate list (count, largest d, up to
+// dmax + 1 disparities), rounded up to 32 bytes
+constexpr int elas_cand_stride(int dmax) { return (dmax + 3 + 15) & ~15; }
 
 // Per-triangle raster record (16 floats): plane and span edges, computed once.
 //   [0..2] plane a, b, c    [3..8] A_u, A_v, B_u, B_v, C_u, C_v (sorted by u)
@@ -50,6 +55,7 @@ struct ElasPair {
     int32_t W, H, dmin, dmax;
     int32_t cw, ch, gw, gh;
     int32_t words;           // 32-bit words of a cell's disparity bitmask: (dmax+32)/32
+    int32_t cstride;         // 16-bit entries of a cell's candidate list: elas_cand_stride(dmax)
     int32_t step, gsize;     // candidate_stepsize, grid_size
     int32_t dcpw, dspc;      // dense tiling: grid cells per wave per step (1, 2), steps per chunk
     int32_t dsh;             // descriptor strip rows (6, 12 or 18)
@@ -68,7 +74,10 @@ struct ElasPair {
     float* hsp[2];           // sp[2] again, in mapped pinned host memory (for the host step)
     int32_t* hnsp;           // nsp again, in mapped pinned host memory
     uint32_t* gflag[2];      // [gh*gw][words] scattered disparity flags
-    uint32_t* gmask[2];      // [gh*gw][words] 3x3-dilated flags (interior cells)
+    uint16_t* gcand[2];      // [gh*gw][cstride] candidate list of the 3x3-dilated flags: entry 0 the
+                             //   count n, entry 1 the largest d (0 when n = 0), entries 2.. the n
+                             //   disparities ascending; border cells n = 0.  gcand[1] follows
+                             //   gcand[0] in one block
     int32_t* glist[2];       // [gh*gw][dmax+2] reference list layout (debug only)
     const int32_t* tri[2];   // [ntri][3]
     float* trirec[2];        // [ntri][ELAS_TRIREC]

@@ -101,7 +101,7 @@ size_t pair_arena(const ElasPairSpec& s, const ElasDims& d, bool debug, bool pos
     if (!s.dev_in) b += 2 * al(d.npx);
     b += 2 * al(16 * d.npx);
     b += 4 * al(4 * d.ncan) + al(4 * (size_t)d.nblk) + 2 * al(8 * d.nspcap);
-    b += 4 * al(4 * d.ncell * d.words);
+    b += 2 * al(4 * d.ncell * d.words) + al(4 * d.ncell * (size_t)elas_cand_stride(s.dmax));
     if (debug) b += 2 * al(4 * d.ncell * (size_t)(s.dmax + 2)) + 2 * al(4 * d.npx);
     b += 2 * al(4 * ELAS_TRIREC * (size_t)d.tricap);
     b += 2 * al(16 * (size_t)d.tricap) + 2 * al(4 * (size_t)d.nst) +
@@ -186,7 +186,7 @@ int elas_make_kparams(const elas_params* p, ElasKParams* kp, std::string* err) {
         pmin = std::min(pmin, P);
         allneg = allneg && P < 0;
     }
-    if (pmin < -(1 << 18)) return bad("prior too strong (beta too small)");
+    if (pmin < -ELAS_MAX_KEY_BIAS) return bad("prior too strong (beta too small): -P(k) must be <= 16384");
     kp->key_bias = -pmin;
     kp->all_p_negative = allneg ? 1 : 0;
     kp->speckle_sim = p->speckle_sim_threshold;
@@ -406,6 +406,7 @@ int elas_ctx::slot_prepare(ElasSlot& S, const ElasPairSpec* sp, int n) {
         memset(&P, 0, sizeof(P));
         P.W = s.W; P.H = s.H; P.dmin = prm.disp_min; P.dmax = s.dmax;
         P.cw = d.cw; P.ch = d.ch; P.gw = d.gw; P.gh = d.gh; P.words = d.words;
+        P.cstride = elas_cand_stride(s.dmax);
         P.step = prm.candidate_stepsize; P.gsize = prm.grid_size; P.nblk = d.nblk;
         const ElasTiles dt = tiles(d, s.W, s.H, s.dmax);
         P.dcpw = dt.cpw;
@@ -433,7 +434,9 @@ int elas_ctx::slot_prepare(ElasSlot& S, const ElasPairSpec* sp, int n) {
         P.tri[0] = (const int32_t*)hdev(S.htri + S.tri_off[i]);
         P.tri[1] = P.tri[0] + 3 * (size_t)d.tricap;
         for (int k = 0; k < 2; k++) P.gflag[k] = (uint32_t*)take(4 * d.ncell * d.words);
-        for (int k = 0; k < 2; k++) P.gmask[k] = (uint32_t*)take(4 * d.ncell * d.words);
+        // both candidate lists in one block (the dense kernel reads them through one buffer resource)
+        P.gcand[0] = (uint16_t*)take(4 * d.ncell * (size_t)elas_cand_stride(s.dmax));
+        P.gcand[1] = P.gcand[0] + d.ncell * (size_t)elas_cand_stride(s.dmax);
         if (debug) {
             for (int k = 0; k < 2; k++) P.glist[k] = (int32_t*)take(4 * d.ncell * (size_t)(s.dmax + 2));
             for (int k = 0; k < 2; k++) P.mean[k] = (float*)take(4 * d.npx);

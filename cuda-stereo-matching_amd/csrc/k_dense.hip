@@ -425,31 +425,37 @@ __global__ __launch_bounds__(256) void k_mean(ElasLaunch L) {
 // inside one grid-cell row, matched for both images at once; a chunk is a run
 // of up to P.dspc steps along the band, a step covering tw = WAVES*P.dcpw*gs
 // columns.  In a step, wave w owns cells s*WAVES*dcpw + w + WAVES*j and a lane
-// is a (row, column) of a cell, so the cell's candidate set is wave-uniform
-// and is walked on the scalar unit as the set bits of its bitmask.
+// is a (row, column) of a cell, so the cell's candidate list is wave-uniform:
+// the grid kernel wrote it as 16-bit entries, a wave loads a step's lists
+// into lanes of one register (word k of a list in lane k of its share) and
+// reads two candidates at a time into a scalar register.
 //
 // LDS holds two rings of R descriptor rows:
 //   ring 0: desc0 over [u_lo, u_hi + dmax): the left pixels' own blocks and
 //           the right pixels' matches at u + d;
 //   ring 1: desc1 over [u_lo - dmax, u_hi): the right pixels' own blocks and
 //           the left pixels' matches at u - d.
-// A ring row has RW slots, the smallest power of two >= tw + dmax.  Column c
-// of step s sits at logical slot (ph + c - base) mod RW, with
-// ph = (s - s0)*tw mod RW and base = u_lo or u_lo - dmax, so the tw columns a
-// step brings in take the slots of columns that have left, and each
-// descriptor row of a chunk is read from HBM once.  With RW a power of two a
-// slot address is one subtraction and one and-or.  Row l of the rings is
-// rotated by 4 l slots so that a wave's rows start on different banks.  The
-// next step's columns, priors and candidate masks are loaded into registers
-// while the current step matches and are written after it.
+// A ring row has RW slots, the smallest power of two >= tw + dmax, and image
+// column c sits in slot c mod RW of its row, so the tw columns a step brings
+// in take the slots of columns that have left, and each descriptor row of a
+// chunk is read from HBM once.  Row l of the rings is rotated by 4 l slots so
+// that a wave's rows start on different banks: column c of line l is at byte
+// 16 RW l | ((16 c + 64 l) mod 16 RW), one add and one and-or per address.
+// The next step's columns, priors and candidate lists are loaded into
+// registers while the current step matches and are written after it.
 //
 // Reference order and tie rule: grid candidates outside the plane window
 // [dp-r, dp+r] ascending, then the window ascending with the prior P added,
 // strict '<'.  This equals the minimum of the key
-//   ((cost + bias) << 11) | (phase << 10) | d,   phase 0 = grid, 1 = window.
-// When every P(k) < 0, a grid candidate inside the window is dominated by its
-// own window entry (same d, lower cost), so phase 0 need not exclude the
-// window; otherwise it does.
+//   ((cost + bias) << 16) | (phase << 10) | x,   phase 0 = grid, x = d;
+//                                                phase 1 = window, x = d - dp + r,
+// built by v_sad_hi_u8, which adds each byte block's SAD << 16 onto the
+// candidate's constant ((bias [+ P]) << 16 | phase << 10 | x).  cost + bias
+// stays below 2^15 (bias <= ELAS_MAX_KEY_BIAS), and a candidate the reference
+// does not evaluate starts from KEY_BAD = 2^31, above every valid key, with
+// room for the SAD below 2^32.  When every P(k) < 0, a grid candidate inside
+// the window is dominated by its own window entry (same d, lower cost), so
+// phase 0 need not exclude the window; otherwise it does.
 
 // smallest power of two >= x (x >= 1)
 __device__ __forceinline__ uint32_t elas_pow2_at_least(uint32_t x) { return x <= 1u ? 1u : 1u << (32 - __clz(x - 1u)); }
@@ -465,27 +471,49 @@ __device__ __forceinline__ int elas_xcd_order(int t, int total, int grid) {
     return x * q + min(x, r) + i;
 }
 
+// key of one candidate: c + (SAD of the 16 bytes << 16)
+__device__ __forceinline__ uint32_t elas_sad16_hi(const uint4 a, const uint4 b, uint32_t c) {
+    c = __builtin_amdgcn_sad_hi_u8(a.x, b.x, c);
+    c = __builtin_amdgcn_sad_hi_u8(a.y, b.y, c);
+    c = __builtin_amdgcn_sad_hi_u8(a.z, b.z, c);
+    c = __builtin_amdgcn_sad_hi_u8(a.w, b.w, c);
+    return c;
+}
+
+template <int N>
+struct elas_ic {
+    static constexpr int value = N;
+};
+
 // CPW = P.dcpw: cells per wave per step (a launch holds the pairs of one CPW).
-// Every global load and store is a buffer access issued without a branch
-// (lanes with nothing to move take an out-of-range offset), so the waits
-// below are for exactly the loads they need.
-template <int CPW, bool PNEG>
+// Global loads and stores are buffer accesses issued without a branch (lanes
+// with nothing to move take an out-of-range offset), so the waits below are
+// for exactly the loads they need.  Ring slots of columns outside [3, W-4]
+// hold whatever the loads returned: no valid key reads them (a pixel is
+// matched only for u in [4, W-4] and u -/+ d in [3, W-4]).
+template <int CPW, bool PNEG, bool TEX>
 __global__ __launch_bounds__(ELAS_DENSE_THREADS, CPW == 1 ? 6 : 4) void k_dense(ElasLaunch L, ElasKParams kp) {
     extern __shared__ uint4 s_ring[];
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform: scalar registers
-    // a candidate outside the image (or of an excluded kind) starts its SAD
-    // from KEY_BAD: its key exceeds every valid key and stays below 2^32
-    constexpr uint32_t KEY_BAD = 0x1ff000u;
+    constexpr uint32_t KEY_BAD = 0x80000000u;
+    // lanes per candidate list in the list register: HC words of list h in
+    // lanes [HC h, HC h + HC), h = img * CPW + ci
+    constexpr int HC = 64 / (2 * CPW);
     // ring reads by LDS byte address: the dynamic LDS (the rings) starts
     // right after the kernel's static LDS, of which it has none, so at 0
     typedef const __attribute__((address_space(3))) elas_u32x4 lds_u4;
-    constexpr uint32_t lds0 = 0u;
+    auto ld = [&](uint32_t a) {
+        const elas_u32x4 x = *(lds_u4*)(uintptr_t)a;
+        return make_uint4(x.x, x.y, x.z, x.w);
+    };
+    const int pr = kp.plane_radius, nwin = 2 * pr + 1;
+    const uint32_t kgrid = (uint32_t)kp.key_bias << 16;
     for (int t0 = blockIdx.x; t0 < L.total; t0 += gridDim.x) {
         const int t = elas_xcd_order(t0, L.total, gridDim.x);
         const int p = elas_find_pair(L.prefix, L.npairs, t);
         const auto& P = L.pairs[p];
-        const int W = P.W, H = P.H, gs = P.gsize, gw = P.gw, dmax = P.dmax, words = P.words;
+        const int W = P.W, H = P.H, gs = P.gsize, gw = P.gw, dmax = P.dmax;
         const int R = max(1, min(64 / gs, gs));
         const int sub = (gs + R - 1) / R;
         const int tcells = ELAS_DENSE_WAVES * CPW, tw = tcells * gs;
@@ -498,58 +526,54 @@ __global__ __launch_bounds__(ELAS_DENSE_THREADS, CPW == 1 ? 6 : 4) void k_dense(
         const int rows = max(0, min(min(R, gs - kb * R), H - v0));
         if (rows == 0) continue;  // workgroup-uniform
         const int s0 = chunk * P.dspc, s1 = min(segs, s0 + P.dspc);
-        // ring rows of RW = 2^k >= tw + dmax slots; line l (ring 0 rows, then
-        // ring 1 rows) is rotated by 4 l slots so that a wave's rows start
-        // on different banks: logical slot q of line l sits at
-        // l*RW + ((q + 4 l) mod RW)
         const int RW = (int)elas_pow2_at_least((uint32_t)(tw + dmax));
         const uint32_t M16 = 16u * (uint32_t)RW - 1u;
-        auto phys = [&](int line, int q) { return line * RW + ((q + 4 * line) & (RW - 1)); };
+        // uint4 index of column c in line l
+        auto phys = [&](int line, int c) { return line * RW + ((c + 4 * line) & (RW - 1)); };
         const uint32_t npx = (uint32_t)W * (uint32_t)H;  // <= 2^27
         const __amdgpu_buffer_rsrc_t rd0 = elas_rsrc(P.desc[0], 16u * npx);
         const __amdgpu_buffer_rsrc_t rd1 = elas_rsrc(P.desc[1], 16u * npx);
-        // descriptor of image k, band row rr, column c (0 off the image); k is
-        // a constant at every call, so the buffer resource stays scalar
-        auto dload = [&](int k, int rr, int c, bool ok) {
-            const uint32_t off = ok && c >= 0 && c < W ? ((uint32_t)(v0 + rr) * (uint32_t)W + (uint32_t)c) * 16u : ELAS_OOR;
-            return elas_ld16_ranged(k ? rd1 : rd0, off);
-        };
         const int row = lane / gs, col = lane - row * gs;
         const int lr = row < rows ? row : 0;
         const int v = v0 + row;
         const bool lane_on = lane < R * gs && row < rows;
+        const bool row_ok = lane_on && v >= 4 && v <= H - 4;
         const uint32_t vrow4 = 4u * (uint32_t)v * (uint32_t)W;  // byte offset of this lane's row in a float/int map
         const __amdgpu_buffer_rsrc_t rp[2] = {elas_rsrc(P.prior[0], 4u * npx), elas_rsrc(P.prior[1], 4u * npx)};
-        const uint32_t mbytes = 4u * (uint32_t)(P.gh * gw * words);
-        const __amdgpu_buffer_rsrc_t rm[2] = {elas_rsrc(P.gmask[0], mbytes), elas_rsrc(P.gmask[1], mbytes)};
+        const uint32_t cwords = (uint32_t)P.cstride >> 1;  // 32-bit words of a cell's list
+        const uint32_t lbytes = 4u * (uint32_t)(P.gh * gw) * cwords;  // one image's lists
+        const __amdgpu_buffer_rsrc_t rc = elas_rsrc(P.gcand[0], 2u * lbytes);
         const __amdgpu_buffer_rsrc_t ro[2] = {elas_rsrc(P.draw[0], 4u * npx), elas_rsrc(P.draw[1], 4u * npx)};
-        // priors of this lane's pixel in each (image, cell) of step s (0 off
-        // the matching domain), and the cells' candidate masks: lane 32*ci + w
-        // holds word w of cell ci
-        auto load_px = [&](int s, int (&pr)[2][CPW], uint32_t (&mk)[2]) {
-            #pragma unroll
-            for (int img = 0; img < 2; img++) {
-                const int ci = lane >> 5, w = lane & 31;
+        // priors of this lane's pixel in each (image, cell) of step s, and the
+        // step's candidate lists (lane HC h + k: word k of list h)
+        auto load_px = [&](int s, int (&prv)[2][CPW], uint32_t& lst) {
+            {
+                const int h = lane / HC, w = lane - h * HC, img = h / CPW, ci = h - img * CPW;
                 const int gx = s * tcells + wave + ci * ELAS_DENSE_WAVES;
-                const bool okm = ci < CPW && w < words && gx < gw && s < s1;
-                mk[img] = elas_ld4_ranged(rm[img], okm ? ((uint32_t)(gy * gw + gx) * (uint32_t)words + (uint32_t)w) * 4u
-                                                       : ELAS_OOR);
+                const bool ok = gx < gw && s < s1 && w < (int)cwords;
+                lst = elas_ld4_ranged(rc, ok ? (uint32_t)img * lbytes + ((uint32_t)(gy * gw + gx) * cwords + (uint32_t)w) * 4u
+                                             : ELAS_OOR);
+            }
+            #pragma unroll
+            for (int c = 0; c < CPW; c++) {
+                const int gx = s * tcells + wave + c * ELAS_DENSE_WAVES;
+                const uint32_t off = gx < gw && s < s1 ? vrow4 + 4u * (uint32_t)(gx * gs + col) : ELAS_OOR;
                 #pragma unroll
-                for (int c = 0; c < CPW; c++) {
-                    const int gxc = s * tcells + wave + c * ELAS_DENSE_WAVES;
-                    const int u = gxc * gs + col;
-                    const bool ok = s < s1 && gxc < gw && lane_on && u >= 4 && u <= W - 4 && v >= 4 && v <= H - 4;
-                    pr[img][c] = (int)elas_ld4_ranged(rp[img], ok ? vrow4 + 4u * (uint32_t)u : ELAS_OOR);
-                }
+                for (int img = 0; img < 2; img++) prv[img][c] = (int)elas_ld4_ranged(rp[img], off);
             }
         };
-        // this step's priors and masks are in flight with the ring loads below
+        // this step's priors and lists are in flight with the ring loads below
         int pr_cur[2][CPW];
-        uint32_t mk_cur[2];
-        load_px(s0, pr_cur, mk_cur);
-        // prologue: step s0's windows (slot = column - base), both rings at
-        // the same (row, slot), every load of a batch issued before its stores
+        uint32_t ls_cur;
+        load_px(s0, pr_cur, ls_cur);
+        // prologue: step s0's windows, both rings at the same (row, j), every
+        // load of a batch issued before its stores
         {
+            auto dload = [&](int k, int rr, int c, bool ok) {
+                const uint32_t off =
+                    ok && c >= 0 && c < W ? ((uint32_t)(v0 + rr) * (uint32_t)W + (uint32_t)c) * 16u : ELAS_OOR;
+                return elas_ld16_ranged(k ? rd1 : rd0, off);
+            };
             const int span = tw + dmax, n = rows * span, u_lo = s0 * tw;
             int rr = tid / span, j = tid - (tid / span) * span;
             for (int base = 0; base < n; base += 2 * ELAS_DENSE_THREADS) {
@@ -570,206 +594,186 @@ __global__ __launch_bounds__(ELAS_DENSE_THREADS, CPW == 1 ? 6 : 4) void k_dense(
                 #pragma unroll
                 for (int e = 0; e < 2; e++)
                     if (li[e] >= 0) {
-                        s_ring[phys(li[e] >> 16, li[e] & 0xffff)] = q[0][e];
-                        s_ring[phys(rows + (li[e] >> 16), li[e] & 0xffff)] = q[1][e];
+                        const int lrr = li[e] >> 16, jj = li[e] & 0xffff;
+                        s_ring[phys(lrr, u_lo + jj)] = q[0][e];
+                        s_ring[phys(rows + lrr, u_lo - dmax + jj)] = q[1][e];
                     }
             }
         }
-        // this thread's share of a step's new columns, the same (row, column)
-        // in both rings (rows x tw <= 64 * 8 * CPW per ring):
-        // (row << 16) | column, or -1
-        int pf_at[CPW];
-        {
-            const int n = rows * tw;
-            #pragma unroll
-            for (int e = 0; e < CPW; e++) {
-                const int i = tid + e * ELAS_DENSE_THREADS;
-                pf_at[e] = i < n ? ((i / tw) << 16) | (i - (i / tw) * tw) : -1;
-            }
+        // this thread's share of a step's new columns, the same (row, j) in
+        // both rings (rows x tw <= 64 * 8 * CPW per ring): the ring's row and
+        // column offset of step s + 1's j-th new column, ring 0 at
+        // (s + 1) tw + dmax + j, ring 1 at (s + 1) tw + j; byte offsets of
+        // those columns in the descriptor images at s + 1 = 0
+        int pf_line[CPW], pf_j[CPW];
+        uint32_t pf_off[CPW];
+        #pragma unroll
+        for (int e = 0; e < CPW; e++) {
+            const int i = tid + e * ELAS_DENSE_THREADS;
+            const bool has = i < rows * tw;
+            pf_line[e] = has ? i / tw : -1;
+            pf_j[e] = has ? i - (i / tw) * tw : 0;
+            pf_off[e] = has ? ((uint32_t)(v0 + pf_line[e]) * (uint32_t)W + (uint32_t)pf_j[e]) * 16u : ELAS_OOR;
         }
         elas_lds_barrier();
-        int ph = 0;
         for (int s = s0; s < s1; s++) {
-            // next step's new columns (ring 0: [u + dmax, u + dmax + tw),
-            // ring 1: [u, u + tw) with u = (s + 1) tw), priors and masks; past
-            // the chunk's last step these loads read nothing
+            // next step's new columns, priors and lists; past the chunk's last
+            // step these loads read nothing
             const bool more = s + 1 < s1;
             const int u_nx = (s + 1) * tw;
             uint4 pf[2][CPW];
             #pragma unroll
             for (int e = 0; e < CPW; e++) {
-                const int rr = pf_at[e] >> 16, j = pf_at[e] & 0xffff;
-                pf[0][e] = dload(0, rr, u_nx + dmax + j, more && pf_at[e] >= 0);
-                pf[1][e] = dload(1, rr, u_nx + j, more && pf_at[e] >= 0);
+                const bool ok = more && pf_line[e] >= 0;
+                pf[0][e] = elas_ld16_ranged(rd0, ok ? pf_off[e] + 16u * (uint32_t)(u_nx + dmax) : ELAS_OOR);
+                pf[1][e] = elas_ld16_ranged(rd1, ok ? pf_off[e] + 16u * (uint32_t)u_nx : ELAS_OOR);
             }
             int pr_nx[2][CPW];
-            uint32_t mk_nx[2];
-            load_px(s + 1, pr_nx, mk_nx);
+            uint32_t ls_nx;
+            load_px(s + 1, pr_nx, ls_nx);
             #pragma unroll
-            for (int img = 0; img < 2; img++) {
-                const int sgn = img ? 1 : -1;
-                // own blocks in ring img, matches in ring 1 - img: byte address of
-                // logical slot q of line l = ((16 (q + 4 l)) & M16) | 16 RW l
-                const int own_line = img * rows + lr, oth_line = (1 - img) * rows + lr;
-                const uint32_t own_base = 16u * (uint32_t)(RW * own_line), oth_base = 16u * (uint32_t)(RW * oth_line);
-                auto ld = [&](uint32_t a) {
-                    const elas_u32x4 x = *(lds_u4*)(uintptr_t)(lds0 + a);
-                    return make_uint4(x.x, x.y, x.z, x.w);
-                };
+            for (int ci = 0; ci < CPW; ci++) {
+                const int gx = s * tcells + wave + ci * ELAS_DENSE_WAVES;
+                if (gx >= gw) continue;  // wave-uniform
+                const int u = gx * gs + col;
+                const bool active = lane_on && u < W;
+                const bool col_ok = row_ok && u >= 4 && u <= W - 4;
+                const uint32_t u16 = 16u * (uint32_t)u;
                 #pragma unroll
-                for (int ci = 0; ci < CPW; ci++) {
-                    const int gx = s * tcells + wave + ci * ELAS_DENSE_WAVES;
-                    const int u = gx * gs + col;
-                    const bool active = lane_on && gx < gw && u < W;
-                    float res = -1.0f;
-                    if (gx < gw) {  // wave-uniform
-                        const int xo = (gx - s * tcells) * gs + col;  // u - u_lo
-                        const int dp = pr_cur[img][ci];
-                        const uint4 b1 = ld((16u * (uint32_t)(ph + 4 * own_line + xo + (img ? dmax : 0)) & M16) | own_base);
-                        bool valid = active && u >= 4 && u <= W - 4 && v >= 4 && v <= H - 4 && dp > 0;
-                        if (kp.match_texture > 0) {
-                            const uint32_t tex =
-                                elas_sad16(b1, make_uint4(0x80808080u, 0x80808080u, 0x80808080u, 0x80808080u), 0u);
-                            if ((int)tex < kp.match_texture) valid = false;
+                for (int img = 0; img < 2; img++) {
+                    const int h = img * CPW + ci;
+                    const int dp = pr_cur[img][ci];
+                    const int own_line = img * rows + lr, oth_line = (1 - img) * rows + lr;
+                    const uint32_t ob = 16u * (uint32_t)(RW * oth_line);
+                    // own block; the match at disparity d is column u -/+ d of the other ring
+                    const uint4 b1 = ld((16u * (uint32_t)(RW * own_line)) | ((u16 + 64u * (uint32_t)own_line) & M16));
+                    const uint32_t xo = u16 + 64u * (uint32_t)oth_line;
+                    bool valid = col_ok && dp > 0;
+                    if (TEX) {  // match_texture > 0
+                        const uint32_t tex =
+                            elas_sad16(b1, make_uint4(0x80808080u, 0x80808080u, 0x80808080u, 0x80808080u), 0u);
+                        valid = valid && (int)tex >= kp.match_texture;
+                    }
+                    // u_warp = u -/+ d must lie in [3, W-4] (KerComputeDisparity.cu:259,267): d <= dlim
+                    const int dlim = img ? W - 4 - u : u - 3;
+                    const int wlim = min(dmax, dlim);
+                    uint32_t kmin = 0xffffffffu;
+                    // plane window d = dp - r + q, q = 0..2r
+                    const uint32_t xw = img ? xo + 16u * (uint32_t)(dp - pr) : xo - 16u * (uint32_t)(dp - pr);
+                    auto waddr = [&](int q) {
+                        return ob | ((img ? xw + 16u * (uint32_t)q : xw - 16u * (uint32_t)q) & M16);
+                    };
+                    // every valid lane's window inside [0, wlim] (the usual case): no per-lane test
+                    const bool win_in = __ballot(valid && !(dp >= pr && dp + pr <= wlim)) == 0;
+                    // window keys: ((bias + P(|q - r|)) << 16) | 1024 | q, plus the SAD;
+                    // an entry off [0, wlim] starts from KEY_BAD (its wrapped,
+                    // in-bounds read need not be clamped)
+                    const uint32_t bias = (uint32_t)kp.key_bias;
+                    if (nwin == 5) {
+                        // the effective parameters: the priors P(2), P(1), P(0), P(1),
+                        // P(2) indexed by constants (a runtime index into the kernel
+                        // arguments is a scalar memory load per entry)
+                        const uint32_t kw[5] = {((bias + kp.P[2]) << 16) | 1024u, ((bias + kp.P[1]) << 16) | 1025u,
+                                                ((bias + kp.P[0]) << 16) | 1026u, ((bias + kp.P[1]) << 16) | 1027u,
+                                                ((bias + kp.P[2]) << 16) | 1028u};
+                        uint4 wb[5];
+                        #pragma unroll
+                        for (int q = 0; q < 5; q++) wb[q] = ld(waddr(q));
+                        if (win_in) {
+                            #pragma unroll
+                            for (int q = 0; q < 5; q++) kmin = min(kmin, elas_sad16_hi(b1, wb[q], kw[q]));
+                        } else {
+                            #pragma unroll
+                            for (int q = 0; q < 5; q++) {
+                                const bool ok = (uint32_t)(dp - 2 + q) <= (uint32_t)wlim;
+                                kmin = min(kmin, elas_sad16_hi(b1, wb[q], ok ? kw[q] : KEY_BAD));
+                            }
                         }
-                        // u_warp = u -/+ d must lie in [3, W-4] (KerComputeDisparity.cu:259,267): d <= dlim
-                        const int dlim = img ? W - 4 - u : u - 3;
-                        // the match at disparity d: logical slot bm + sgn*d
-                        const uint32_t bm = 16u * (uint32_t)(ph + 4 * oth_line + xo + (img ? 0 : dmax));
-                        auto maddr = [&](int d) { return ((bm + (uint32_t)(sgn * 16 * d)) & M16) | oth_base; };
-                        // every column of the cell takes d <= dlim_min
+                    } else {
+                        for (int q = 0; q < nwin; q++) {
+                            const bool ok = (uint32_t)(dp - pr + q) <= (uint32_t)wlim;
+                            const uint32_t c = ((bias + (uint32_t)kp.P[abs(q - pr)]) << 16) | 1024u | (uint32_t)q;
+                            kmin = min(kmin, elas_sad16_hi(b1, ld(waddr(q)), ok ? c : KEY_BAD));
+                        }
+                    }
+                    // grid candidates: entries k of list h, two per list word
+                    {
+                        const uint32_t w0 = __builtin_amdgcn_readlane(ls_cur, HC * h);
+                        const int n = (int)(w0 & 0xffffu), last = (int)(w0 >> 16);
                         const int dlim_min = img ? W - 4 - (gx * gs + gs - 1) : gx * gs - 3;
-                        const uint32_t bias = (uint32_t)kp.key_bias;
-                        const int dpmin = max(dp - kp.plane_radius, 0), dpmax = min(dp + kp.plane_radius, dmax);
-                        uint32_t kmin = 0xffffffffu;
-                        // grid candidates (wave-uniform d): key = (sad + bias) << 11 | d
-                        auto grid_acc = [&](int d) {
-                            bool ok = d <= dlim;
-                            if (!PNEG) ok = ok && (d < dpmin || d > dpmax);
-                            return ok ? bias : KEY_BAD;
-                        };
-                        const uint32_t mkv = mk_cur[img];
-                        // candidates in ascending d across the cell's non-empty mask
-                        // words (one ballot finds them), four at a time so that their
-                        // LDS reads overlap, then the last one to three singly
-                        uint32_t wleft = (uint32_t)(__ballot(mkv != 0u) >> (32 * ci));
-                        if (words < 32) wleft &= (1u << words) - 1u;
-                        int left = 0;
-                        for (uint32_t m = wleft; m; m &= m - 1u)
-                            left += __builtin_popcount(__builtin_amdgcn_readlane(mkv, 32 * ci + __builtin_ctz(m)));
-                        uint32_t bits = 0u;
-                        int wbase = 0;
-                        auto next = [&]() {
-                            while (!bits) {
-                                const int w = __builtin_ctz(wleft);
-                                wleft &= wleft - 1u;
-                                bits = __builtin_amdgcn_readlane(mkv, 32 * ci + w);
-                                wbase = 32 * w;
-                            }
-                            const int d = wbase + __builtin_ctz(bits);
-                            bits &= bits - 1u;
-                            return d;
-                        };
-                        for (; left >= 4; left -= 4) {
-                            int dq[4];
-                            uint4 b2[4];
+                        const bool fast = PNEG && last <= dlim_min;
+                        const int dpmin = max(dp - pr, 0), dpmax = min(dp + pr, dmax);
+                        const ELAS_CONST_AS uint32_t* gl =
+                            (const ELAS_CONST_AS uint32_t*)P.gcand[img] + (size_t)(gy * gw + gx) * cwords;
+                        auto word = [&](int kw) { return (uint32_t)__builtin_amdgcn_readlane(ls_cur, HC * h + kw); };
+                        auto batch = [&](auto nb, const int* dq) {
+                            constexpr int NB = decltype(nb)::value;
+                            uint4 b2[NB];
                             #pragma unroll
-                            for (int q = 0; q < 4; q++) {
-                                dq[q] = next();
-                                b2[q] = ld(maddr(dq[q]));
-                            }
-                            if (PNEG && dq[3] <= dlim_min) {  // wave-uniform: no per-lane test
+                            for (int q = 0; q < NB; q++)
+                                b2[q] = ld(ob | ((img ? xo + 16u * (uint32_t)dq[q] : xo - 16u * (uint32_t)dq[q]) & M16));
+                            if (fast) {
                                 #pragma unroll
-                                for (int q = 0; q < 4; q++)
-                                    kmin = min(kmin, (elas_sad16(b1, b2[q], bias) << 11) | (uint32_t)dq[q]);
+                                for (int q = 0; q < NB; q++)
+                                    kmin = min(kmin, elas_sad16_hi(b1, b2[q], kgrid | (uint32_t)dq[q]));
                             } else {
                                 #pragma unroll
-                                for (int q = 0; q < 4; q++)
-                                    kmin = min(kmin, (elas_sad16(b1, b2[q], grid_acc(dq[q])) << 11) | (uint32_t)dq[q]);
-                            }
-                        }
-                        for (; left > 0; left--) {
-                            const int d = next();
-                            kmin = min(kmin, (elas_sad16(b1, ld(maddr(d)), grid_acc(d)) << 11) | (uint32_t)d);
-                        }
-                        // plane window d = dp - r + k, k = 0..2r, with the prior P(|k - r|)
-                        // added: key = (sad + P + bias) << 11 | 1024 | d
-                        const int pr = kp.plane_radius, nwin = 2 * pr + 1;
-                        const int wlim = min(dmax, dlim);
-                        const int d0 = dp - pr;
-                        auto win_key = [&](int k, const uint4 b2) {
-                            const int d = d0 + k;
-                            const uint32_t acc =
-                                (uint32_t)d <= (uint32_t)wlim ? (uint32_t)((int)bias + kp.P[abs(k - pr)]) : KEY_BAD;
-                            return (elas_sad16(b1, b2, acc) << 11) | (1024u | ((uint32_t)d & 1023u));
-                        };
-                        // a window entry off [0, wlim] starts from KEY_BAD, so its
-                        // (wrapped, in-bounds) read need not be clamped
-                        const uint32_t aw = bm + (uint32_t)(sgn * 16 * d0);
-                        auto win_lds = [&](int k) { return ld(((aw + (uint32_t)(sgn * 16 * k)) & M16) | oth_base); };
-                        int k0 = 0;
-                        if (nwin == 5) {
-                            // the effective parameters: one batch of five, the priors
-                            // P(2), P(1), P(0), P(1), P(2) indexed by constants (a
-                            // runtime index into the kernel arguments is a scalar
-                            // memory load per entry)
-                            const int pw[5] = {kp.P[2], kp.P[1], kp.P[0], kp.P[1], kp.P[2]};
-                            uint4 b2[5];
-                            #pragma unroll
-                            for (int q = 0; q < 5; q++) b2[q] = win_lds(q);
-                            // when every valid lane's window lies in [0, wlim] (the usual
-                            // case) the prior is added without a per-lane test; other
-                            // lanes' results are discarded
-                            const bool inside = d0 >= 0 && d0 + 4 <= wlim;
-                            if (__ballot(valid && !inside) == 0) {
-                                const uint32_t kd = 1024u | (uint32_t)d0;
-                                #pragma unroll
-                                for (int q = 0; q < 5; q++)
-                                    kmin = min(kmin, (elas_sad16(b1, b2[q], (uint32_t)((int)bias + pw[q])) << 11) |
-                                                         ((kd + (uint32_t)q) & 2047u));
-                            } else {
-                                #pragma unroll
-                                for (int q = 0; q < 5; q++) {
-                                    const int d = d0 + q;
-                                    const uint32_t acc =
-                                        (uint32_t)d <= (uint32_t)wlim ? (uint32_t)((int)bias + pw[q]) : KEY_BAD;
-                                    kmin = min(kmin, (elas_sad16(b1, b2[q], acc) << 11) | (1024u | ((uint32_t)d & 1023u)));
+                                for (int q = 0; q < NB; q++) {
+                                    bool ok = dq[q] <= dlim;
+                                    if (!PNEG) ok = ok && (dq[q] < dpmin || dq[q] > dpmax);
+                                    kmin = min(kmin, elas_sad16_hi(b1, b2[q], ok ? kgrid | (uint32_t)dq[q] : KEY_BAD));
                                 }
                             }
-                            k0 = 5;
+                        };
+                        // entries in the list register, then (lists longer than 2 HC - 2
+                        // entries) the rest from memory
+                        const int nr = min(n, 2 * HC - 2);
+                        int k = 0;
+                        for (; k + 4 <= nr; k += 4) {
+                            const uint32_t wa = word(1 + (k >> 1)), wb2 = word(2 + (k >> 1));
+                            const int dq[4] = {(int)(wa & 0xffffu), (int)(wa >> 16), (int)(wb2 & 0xffffu), (int)(wb2 >> 16)};
+                            batch(elas_ic<4>(), dq);
                         }
-                        for (; k0 + 4 <= nwin; k0 += 4) {
-                            uint4 b2[4];
-                            #pragma unroll
-                            for (int q = 0; q < 4; q++) b2[q] = win_lds(k0 + q);
-                            #pragma unroll
-                            for (int q = 0; q < 4; q++) kmin = min(kmin, win_key(k0 + q, b2[q]));
+                        if (k + 2 <= nr) {
+                            const uint32_t wa = word(1 + (k >> 1));
+                            const int dq[2] = {(int)(wa & 0xffffu), (int)(wa >> 16)};
+                            batch(elas_ic<2>(), dq);
+                            k += 2;
                         }
-                        for (; k0 < nwin; k0++) kmin = min(kmin, win_key(k0, win_lds(k0)));
-                        if (valid && kmin < (KEY_BAD << 11)) res = (float)(kmin & 1023u);
+                        if (k < nr) {
+                            const int dq[1] = {(int)(word(1 + (k >> 1)) & 0xffffu)};
+                            batch(elas_ic<1>(), dq);
+                            k++;
+                        }
+                        for (; k < n; k++) {
+                            const uint32_t wk = gl[1 + (k >> 1)];
+                            const int dq[1] = {(int)((k & 1) ? wk >> 16 : wk & 0xffffu)};
+                            batch(elas_ic<1>(), dq);
+                        }
+                    }
+                    float res = -1.0f;
+                    if (valid && kmin < KEY_BAD) {
+                        const int x = (int)(kmin & 1023u);
+                        res = (float)((kmin & 1024u) ? dp - pr + x : x);
                     }
                     elas_st4_ranged(ro[img], active ? vrow4 + 4u * (uint32_t)u : ELAS_OOR, __float_as_uint(res));
                 }
             }
-            const int phn = (ph + tw) & (RW - 1);
             if (more) {
                 elas_lds_barrier();  // every wave is done with step s's slots
                 #pragma unroll
                 for (int e = 0; e < CPW; e++)
-                    if (pf_at[e] >= 0) {
-                        const int rr = pf_at[e] >> 16, q = phn + dmax + (pf_at[e] & 0xffff);
-                        s_ring[phys(rr, q)] = pf[0][e];
-                        s_ring[phys(rows + rr, q)] = pf[1][e];
+                    if (pf_line[e] >= 0) {
+                        s_ring[phys(pf_line[e], u_nx + dmax + pf_j[e])] = pf[0][e];
+                        s_ring[phys(rows + pf_line[e], u_nx + pf_j[e])] = pf[1][e];
                     }
                 elas_lds_barrier();
             }
             #pragma unroll
-            for (int img = 0; img < 2; img++) {
-                mk_cur[img] = mk_nx[img];
+            for (int img = 0; img < 2; img++)
                 #pragma unroll
                 for (int c = 0; c < CPW; c++) pr_cur[img][c] = pr_nx[img][c];
-            }
-            ph = phn;
+            ls_cur = ls_nx;
         }
         elas_lds_barrier();  // the next unit's prologue rewrites the rings
     }
@@ -823,17 +827,30 @@ void elas_launch_mean(const ElasLaunch& L, int grid, hipStream_t s) {
 }
 void elas_launch_dense(const ElasLaunch& L, const ElasKParams& kp, int cpw, int grid, size_t lds, hipStream_t s) {
     const dim3 b(ELAS_DENSE_THREADS);
-    if (cpw == 2) {
-        if (kp.all_p_negative) hipLaunchKernelGGL((k_dense<2, true>), dim3(grid), b, lds, s, L, kp);
-        else hipLaunchKernelGGL((k_dense<2, false>), dim3(grid), b, lds, s, L, kp);
-    } else {
-        if (kp.all_p_negative) hipLaunchKernelGGL((k_dense<1, true>), dim3(grid), b, lds, s, L, kp);
-        else hipLaunchKernelGGL((k_dense<1, false>), dim3(grid), b, lds, s, L, kp);
+    const bool pn = kp.all_p_negative != 0, tex = kp.match_texture > 0;
+#define ELAS_DENSE_GO(C, PN, T) hipLaunchKernelGGL((k_dense<C, PN, T>), dim3(grid), b, lds, s, L, kp)
+    const int which = (cpw == 2 ? 4 : 0) | (pn ? 2 : 0) | (tex ? 1 : 0);
+    switch (which) {
+        case 0: ELAS_DENSE_GO(1, false, false); break;
+        case 1: ELAS_DENSE_GO(1, false, true); break;
+        case 2: ELAS_DENSE_GO(1, true, false); break;
+        case 3: ELAS_DENSE_GO(1, true, true); break;
+        case 4: ELAS_DENSE_GO(2, false, false); break;
+        case 5: ELAS_DENSE_GO(2, false, true); break;
+        case 6: ELAS_DENSE_GO(2, true, false); break;
+        default: ELAS_DENSE_GO(2, true, true); break;
     }
+#undef ELAS_DENSE_GO
 }
 int elas_dense_allow_lds(size_t bytes) {
-    const void* fn[4] = {reinterpret_cast<const void*>(&k_dense<1, true>), reinterpret_cast<const void*>(&k_dense<1, false>),
-                         reinterpret_cast<const void*>(&k_dense<2, true>), reinterpret_cast<const void*>(&k_dense<2, false>)};
+    const void* fn[8] = {reinterpret_cast<const void*>(&k_dense<1, true, false>),
+                         reinterpret_cast<const void*>(&k_dense<1, false, false>),
+                         reinterpret_cast<const void*>(&k_dense<2, true, false>),
+                         reinterpret_cast<const void*>(&k_dense<2, false, false>),
+                         reinterpret_cast<const void*>(&k_dense<1, true, true>),
+                         reinterpret_cast<const void*>(&k_dense<1, false, true>),
+                         reinterpret_cast<const void*>(&k_dense<2, true, true>),
+                         reinterpret_cast<const void*>(&k_dense<2, false, true>)};
     for (const void* f : fn)
         if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) != hipSuccess) {
             (void)hipGetLastError();

@@ -511,10 +511,11 @@ __global__ __launch_bounds__(256) void k_compact(ElasLaunch L) {
 
 // Disparity grid, KerGrid.cu:9-126 + Elas.cu:403-464.  Flags are bitmasks
 // (words x 32 disparities per cell): fill, scatter d-1..d+1 of each support
-// point with atomicOr, then OR the 3x3 neighbourhood of interior cells.
-// The dense kernel walks the set bits in ascending order, which is the
-// reference's ascending candidate list; k_grid_list renders that list layout
-// ([0] = count, [1..] = d, zero tail) for parity dumps.
+// point with atomicOr, then OR the 3x3 neighbourhood of interior cells and
+// write its set bits as the cell's candidate list, ascending (the reference's
+// list, KerGrid.cu:112-124), in 16-bit entries behind the count and the
+// largest d.  k_grid_list renders the reference's int32 layout ([0] = count,
+// [1..] = d, zero tail) for parity dumps.
 __global__ __launch_bounds__(256) void k_grid_fill(ElasLaunch L) {
     for (int t = blockIdx.x; t < L.total; t += gridDim.x) {
         const int p = elas_find_pair(L.prefix, L.npairs, t);
@@ -562,15 +563,21 @@ __global__ __launch_bounds__(256) void k_grid_dilate(ElasLaunch L) {
         const int gy = c / P.gw, gx = c - gy * P.gw;
         const bool interior = gx > 0 && gx < P.gw - 1 && gy > 0 && gy < P.gh - 1;
         const uint32_t* f = P.gflag[img];
-        uint32_t* m = P.gmask[img] + (size_t)c * P.words;
-        for (int w = 0; w < P.words; w++) {
-            uint32_t acc = 0u;
-            if (interior)
+        uint16_t* g = P.gcand[img] + (size_t)c * P.cstride;
+        int n = 0, last = 0;
+        if (interior)
+            for (int w = 0; w < P.words; w++) {
+                uint32_t acc = 0u;
                 for (int dy = -1; dy <= 1; dy++)
                     for (int dx = -1; dx <= 1; dx++)
                         acc |= f[((gy + dy) * P.gw + (gx + dx)) * P.words + w];
-            m[w] = acc;
-        }
+                for (; acc; acc &= acc - 1u) {
+                    last = 32 * w + __builtin_ctz(acc);
+                    g[2 + n++] = (uint16_t)last;
+                }
+            }
+        g[0] = (uint16_t)n;
+        g[1] = (uint16_t)last;
     }
 }
 
@@ -582,11 +589,10 @@ __global__ __launch_bounds__(256) void k_grid_list(ElasLaunch L) {
         const int i = (t - L.prefix[p]) * 256 + threadIdx.x;
         if (i >= 2 * ncell) continue;
         const int img = i >= ncell, c = img ? i - ncell : i;
-        const uint32_t* m = P.gmask[img] + (size_t)c * P.words;
+        const uint16_t* m = P.gcand[img] + (size_t)c * P.cstride;
         int32_t* g = P.glist[img] + (size_t)c * (P.dmax + 2);
-        int n = 0;
-        for (int d = 0; d <= P.dmax; d++)
-            if ((m[d >> 5] >> (d & 31)) & 1u) g[1 + n++] = d;
+        const int n = m[0];
+        for (int k = 0; k < n; k++) g[1 + k] = m[2 + k];
         for (int k = 1 + n; k < P.dmax + 2; k++) g[k] = 0;
         g[0] = n;
     }
