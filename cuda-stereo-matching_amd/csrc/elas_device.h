@@ -98,6 +98,16 @@ __device__ __forceinline__ uint32_t elas_sad16(const uint4 a, const uint4 b, uin
     return acc;
 }
 
+// c + (sum of |a_i - b_i| over 16 bytes) << 16 (4 x v_sad_hi_u8): a key whose
+// low 16 bits, taken from c, identify the candidate
+__device__ __forceinline__ uint32_t elas_sad16_hi(const uint4 a, const uint4 b, uint32_t c) {
+    c = __builtin_amdgcn_sad_hi_u8(a.x, b.x, c);
+    c = __builtin_amdgcn_sad_hi_u8(a.y, b.y, c);
+    c = __builtin_amdgcn_sad_hi_u8(a.z, b.z, c);
+    c = __builtin_amdgcn_sad_hi_u8(a.w, b.w, c);
+    return c;
+}
+
 // float -> int32 / uint32 with saturation and NaN -> 0, written out so the
 // result does not depend on how the compiler treats out-of-range casts: the
 // cast only sees values clamped into range (2147483520 and 4294967040 are

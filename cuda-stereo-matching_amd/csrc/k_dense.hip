@@ -471,15 +471,6 @@ __device__ __forceinline__ int elas_xcd_order(int t, int total, int grid) {
     return x * q + min(x, r) + i;
 }
 
-// key of one candidate: c + (SAD of the 16 bytes << 16)
-__device__ __forceinline__ uint32_t elas_sad16_hi(const uint4 a, const uint4 b, uint32_t c) {
-    c = __builtin_amdgcn_sad_hi_u8(a.x, b.x, c);
-    c = __builtin_amdgcn_sad_hi_u8(a.y, b.y, c);
-    c = __builtin_amdgcn_sad_hi_u8(a.z, b.z, c);
-    c = __builtin_amdgcn_sad_hi_u8(a.w, b.w, c);
-    return c;
-}
-
 template <int N>
 struct elas_ic {
     static constexpr int value = N;
@@ -669,82 +660,112 @@ __global__ __launch_bounds__(ELAS_DENSE_THREADS, CPW == 1 ? 6 : 4) void k_dense(
                     // an entry off [0, wlim] starts from KEY_BAD (its wrapped,
                     // in-bounds read need not be clamped)
                     const uint32_t bias = (uint32_t)kp.key_bias;
-                    if (nwin == 5) {
-                        // the effective parameters: the priors P(2), P(1), P(0), P(1),
-                        // P(2) indexed by constants (a runtime index into the kernel
-                        // arguments is a scalar memory load per entry)
-                        const uint32_t kw[5] = {((bias + kp.P[2]) << 16) | 1024u, ((bias + kp.P[1]) << 16) | 1025u,
-                                                ((bias + kp.P[0]) << 16) | 1026u, ((bias + kp.P[1]) << 16) | 1027u,
-                                                ((bias + kp.P[2]) << 16) | 1028u};
-                        uint4 wb[5];
-                        #pragma unroll
-                        for (int q = 0; q < 5; q++) wb[q] = ld(waddr(q));
-                        if (win_in) {
+                    auto window = [&]() {
+                        if (nwin == 5) {
+                            // the effective parameters: the priors P(2), P(1), P(0), P(1),
+                            // P(2) indexed by constants (a runtime index into the kernel
+                            // arguments is a scalar memory load per entry)
+                            const uint32_t kw[5] = {((bias + kp.P[2]) << 16) | 1024u, ((bias + kp.P[1]) << 16) | 1025u,
+                                                    ((bias + kp.P[0]) << 16) | 1026u, ((bias + kp.P[1]) << 16) | 1027u,
+                                                    ((bias + kp.P[2]) << 16) | 1028u};
+                            uint4 wb[5];
                             #pragma unroll
-                            for (int q = 0; q < 5; q++) kmin = min(kmin, elas_sad16_hi(b1, wb[q], kw[q]));
-                        } else {
-                            #pragma unroll
-                            for (int q = 0; q < 5; q++) {
-                                const bool ok = (uint32_t)(dp - 2 + q) <= (uint32_t)wlim;
-                                kmin = min(kmin, elas_sad16_hi(b1, wb[q], ok ? kw[q] : KEY_BAD));
-                            }
-                        }
-                    } else {
-                        for (int q = 0; q < nwin; q++) {
-                            const bool ok = (uint32_t)(dp - pr + q) <= (uint32_t)wlim;
-                            const uint32_t c = ((bias + (uint32_t)kp.P[abs(q - pr)]) << 16) | 1024u | (uint32_t)q;
-                            kmin = min(kmin, elas_sad16_hi(b1, ld(waddr(q)), ok ? c : KEY_BAD));
-                        }
-                    }
-                    // grid candidates: entries k of list h, two per list word
-                    {
-                        const uint32_t w0 = __builtin_amdgcn_readlane(ls_cur, HC * h);
-                        const int n = (int)(w0 & 0xffffu), last = (int)(w0 >> 16);
-                        const int dlim_min = img ? W - 4 - (gx * gs + gs - 1) : gx * gs - 3;
-                        const bool fast = PNEG && last <= dlim_min;
-                        const int dpmin = max(dp - pr, 0), dpmax = min(dp + pr, dmax);
-                        const ELAS_CONST_AS uint32_t* gl =
-                            (const ELAS_CONST_AS uint32_t*)P.gcand[img] + (size_t)(gy * gw + gx) * cwords;
-                        auto word = [&](int kw) { return (uint32_t)__builtin_amdgcn_readlane(ls_cur, HC * h + kw); };
-                        auto batch = [&](auto nb, const int* dq) {
-                            constexpr int NB = decltype(nb)::value;
-                            uint4 b2[NB];
-                            #pragma unroll
-                            for (int q = 0; q < NB; q++)
-                                b2[q] = ld(ob | ((img ? xo + 16u * (uint32_t)dq[q] : xo - 16u * (uint32_t)dq[q]) & M16));
-                            if (fast) {
+                            for (int q = 0; q < 5; q++) wb[q] = ld(waddr(q));
+                            if (win_in) {
                                 #pragma unroll
-                                for (int q = 0; q < NB; q++)
-                                    kmin = min(kmin, elas_sad16_hi(b1, b2[q], kgrid | (uint32_t)dq[q]));
+                                for (int q = 0; q < 5; q++) kmin = min(kmin, elas_sad16_hi(b1, wb[q], kw[q]));
                             } else {
                                 #pragma unroll
-                                for (int q = 0; q < NB; q++) {
-                                    bool ok = dq[q] <= dlim;
-                                    if (!PNEG) ok = ok && (dq[q] < dpmin || dq[q] > dpmax);
-                                    kmin = min(kmin, elas_sad16_hi(b1, b2[q], ok ? kgrid | (uint32_t)dq[q] : KEY_BAD));
+                                for (int q = 0; q < 5; q++) {
+                                    const bool ok = (uint32_t)(dp - 2 + q) <= (uint32_t)wlim;
+                                    kmin = min(kmin, elas_sad16_hi(b1, wb[q], ok ? kw[q] : KEY_BAD));
                                 }
                             }
-                        };
-                        // entries in the list register, then (lists longer than 2 HC - 2
-                        // entries) the rest from memory
-                        const int nr = min(n, 2 * HC - 2);
-                        int k = 0;
-                        for (; k + 4 <= nr; k += 4) {
-                            const uint32_t wa = word(1 + (k >> 1)), wb2 = word(2 + (k >> 1));
-                            const int dq[4] = {(int)(wa & 0xffffu), (int)(wa >> 16), (int)(wb2 & 0xffffu), (int)(wb2 >> 16)};
-                            batch(elas_ic<4>(), dq);
+                        } else {
+                            for (int q = 0; q < nwin; q++) {
+                                const bool ok = (uint32_t)(dp - pr + q) <= (uint32_t)wlim;
+                                const uint32_t c = ((bias + (uint32_t)kp.P[abs(q - pr)]) << 16) | 1024u | (uint32_t)q;
+                                kmin = min(kmin, elas_sad16_hi(b1, ld(waddr(q)), ok ? c : KEY_BAD));
+                            }
                         }
-                        if (k + 2 <= nr) {
-                            const uint32_t wa = word(1 + (k >> 1));
-                            const int dq[2] = {(int)(wa & 0xffffu), (int)(wa >> 16)};
-                            batch(elas_ic<2>(), dq);
-                            k += 2;
+                    };
+                    // grid candidates: entries k of list h, two per list word, in
+                    // batches whose reads are all issued before their SADs; the
+                    // first batch's reads go out with the window's
+                    const uint32_t w0 = __builtin_amdgcn_readlane(ls_cur, HC * h);
+                    const int n = (int)(w0 & 0xffffu), last = (int)(w0 >> 16);
+                    const int dlim_min = img ? W - 4 - (gx * gs + gs - 1) : gx * gs - 3;
+                    const bool fast = PNEG && last <= dlim_min;
+                    const int dpmin = max(dp - pr, 0), dpmax = min(dp + pr, dmax);
+                    auto word = [&](int kw) { return (uint32_t)__builtin_amdgcn_readlane(ls_cur, HC * h + kw); };
+                    auto issue = [&](auto nb, const int* dq, uint4* b2) {
+                        constexpr int NB = decltype(nb)::value;
+                        #pragma unroll
+                        for (int q = 0; q < NB; q++)
+                            b2[q] = ld(ob | ((img ? xo + 16u * (uint32_t)dq[q] : xo - 16u * (uint32_t)dq[q]) & M16));
+                    };
+                    auto consume = [&](auto nb, const int* dq, const uint4* b2) {
+                        constexpr int NB = decltype(nb)::value;
+                        if (fast) {
+                            #pragma unroll
+                            for (int q = 0; q < NB; q++) kmin = min(kmin, elas_sad16_hi(b1, b2[q], kgrid | (uint32_t)dq[q]));
+                        } else {
+                            #pragma unroll
+                            for (int q = 0; q < NB; q++) {
+                                bool ok = dq[q] <= dlim;
+                                if (!PNEG) ok = ok && (dq[q] < dpmin || dq[q] > dpmax);
+                                kmin = min(kmin, elas_sad16_hi(b1, b2[q], ok ? kgrid | (uint32_t)dq[q] : KEY_BAD));
+                            }
                         }
-                        if (k < nr) {
-                            const int dq[1] = {(int)(word(1 + (k >> 1)) & 0xffffu)};
-                            batch(elas_ic<1>(), dq);
-                            k++;
-                        }
+                    };
+                    auto batch = [&](auto nb, const int* dq) {
+                        uint4 b2[decltype(nb)::value];
+                        issue(nb, dq, b2);
+                        consume(nb, dq, b2);
+                    };
+                    int k;
+                    if (n >= 4) {
+                        const uint32_t wa = word(1), wb2 = word(2);
+                        const int dq[4] = {(int)(wa & 0xffffu), (int)(wa >> 16), (int)(wb2 & 0xffffu), (int)(wb2 >> 16)};
+                        uint4 b2[4];
+                        issue(elas_ic<4>(), dq, b2);
+                        window();
+                        consume(elas_ic<4>(), dq, b2);
+                        k = 4;
+                    } else if (n >= 2) {
+                        const uint32_t wa = word(1);
+                        const int dq[2] = {(int)(wa & 0xffffu), (int)(wa >> 16)};
+                        uint4 b2[2];
+                        issue(elas_ic<2>(), dq, b2);
+                        window();
+                        consume(elas_ic<2>(), dq, b2);
+                        k = 2;
+                    } else {
+                        window();
+                        k = 0;
+                    }
+                    // entries in the list register, then (lists longer than 2 HC - 2
+                    // entries) the rest from memory
+                    const int nr = min(n, 2 * HC - 2);
+                    for (; k + 4 <= nr; k += 4) {
+                        const uint32_t wa = word(1 + (k >> 1)), wb2 = word(2 + (k >> 1));
+                        const int dq[4] = {(int)(wa & 0xffffu), (int)(wa >> 16), (int)(wb2 & 0xffffu), (int)(wb2 >> 16)};
+                        batch(elas_ic<4>(), dq);
+                    }
+                    if (k + 2 <= nr) {
+                        const uint32_t wa = word(1 + (k >> 1));
+                        const int dq[2] = {(int)(wa & 0xffffu), (int)(wa >> 16)};
+                        batch(elas_ic<2>(), dq);
+                        k += 2;
+                    }
+                    if (k < nr) {
+                        const int dq[1] = {(int)(word(1 + (k >> 1)) & 0xffffu)};
+                        batch(elas_ic<1>(), dq);
+                        k++;
+                    }
+                    if (k < n) {
+                        const ELAS_CONST_AS uint32_t* gl =
+                            (const ELAS_CONST_AS uint32_t*)P.gcand[img] + (size_t)(gy * gw + gx) * cwords;
                         for (; k < n; k++) {
                             const uint32_t wk = gl[1 + (k >> 1)];
                             const int dq[1] = {(int)((k & 1) ? wk >> 16 : wk & 0xffffu)};

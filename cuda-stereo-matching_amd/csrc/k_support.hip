@@ -18,7 +18,7 @@
 //   min2   = m if r* == 0 (thread 0 holds a minimiser: duplicated), else the
 //            second smallest E counted with multiplicity;
 //   accept = (float)m < support_threshold * (float)min2,   winner d*.
-// With key(d) = E << 15 | r << 10 | d (distinct per d), m, r*, d* come from
+// With key(d) = E << 16 | r << 10 | d (distinct per d), m, r*, d* come from
 // the smallest key and the second smallest E from the second smallest key.
 //
 // Mapping: one workgroup (4 waves) = 64 consecutive candidates of a lattice
@@ -91,8 +91,8 @@ __device__ __forceinline__ int support_pass(const ElasPair& P, const ElasKParams
         elas_lds_barrier();
         uint32_t k1 = 0xffffffffu, k2 = 0xffffffffu;
         if (ok) {
-            // new pair at column cx(d) = u - d - 2 (left) / u + d + 2 (right),
-            // reused pair at cx(d) + 4 / cx(d) - 4
+            // block pair at column cx(d) = u - d - 2 (left) / u + d + 2 (right),
+            // and the pair at cx(d) + 4 / cx(d) - 4, which is cx(d - 4)
             const uint4 rXt = RIGHT ? rTR : rTL, rXb = RIGHT ? rBR : rBL;
             const uint4 rPt = RIGHT ? rTL : rTR, rPb = RIGHT ? rBL : rBR;
             const uint4* top = s_rows - c_lo;
@@ -100,23 +100,32 @@ __device__ __forceinline__ int support_pass(const ElasPair& P, const ElasKParams
             int d = dminv + j;
             int cx = RIGHT ? u + d + 2 : u - d - 2;
             const int dstep = RIGHT ? 4 : -4;
-            uint4 pt = top[cx - dstep], pb = bot[cx - dstep];
             // (k1, k2) <- the two smallest of (k1, k2, key), k1 <= k2: the new
             // k2 is the median of the three
             auto take = [&](uint32_t key) {
                 k2 = max(min(k1, k2), min(max(k1, k2), key));
                 k1 = min(k1, key);
             };
-            // key = E << 15 | r << 10 | d with r = (d - dmin) mod 32 (r < 32 and
-            // d < 1024 fill disjoint bits, so the fields add)
+            // A lane scans d = dmin + j + 4 m, m = 0, 1, ..  Within the scan its
+            // key is E << 16 | low(m) with low(m) = (m mod 8) << 7 | m / 8: the
+            // same order as the final key's (r, d) with r = (d - dmin) mod 32
+            // = j + 4 (m mod 8), and uniform across the lanes, so it enters
+            // the SAD chain (v_sad_hi_u8 adds each block's SAD << 16) as the
+            // scalar start value.
+            auto low = [](int m) { return (uint32_t)(((m & 7) << 7) | (m >> 3)); };
+            auto energy = [&](const uint4 xt, const uint4 xb, const uint4 pt, const uint4 pb, uint32_t c) {
+                c = elas_sad16_hi(rXt, xt, c);
+                c = elas_sad16_hi(rXb, xb, c);
+                c = elas_sad16_hi(rPt, pt, c);
+                return elas_sad16_hi(rPb, pb, c);
+            };
             const int n = d <= dmaxv ? (dmaxv - d) / 4 + 1 : 0;
             const int n0 = __builtin_amdgcn_readfirstlane(n);
+            // the pair of the previous step, carried
+            uint4 pt = top[cx - dstep], pb = bot[cx - dstep];
             if (__ballot(n != n0) == 0) {
                 // the usual case: every computing lane of the wave walks the same
-                // count, so the loop runs on the scalar unit; blocks of four
-                // start at i = 0 mod 4, where r = j + 16 ((i / 4) mod 2) and
-                // r + 12 < 32, so within a block key(k) = E << 15 | (kd + 4100 k)
-                const uint32_t kdj = ((uint32_t)j << 10) | (uint32_t)d;
+                // count, so the loop runs on the scalar unit
                 int i = 0;
                 for (; i + 4 <= n0; i += 4) {
                     uint4 xt[4], xb[4];
@@ -125,43 +134,37 @@ __device__ __forceinline__ int support_pass(const ElasPair& P, const ElasKParams
                         xt[k] = top[cx + k * dstep];
                         xb[k] = bot[cx + k * dstep];
                     }
-                    const uint32_t kd = kdj + (uint32_t)((((i & 4) << 2) << 10) + 4 * i);
                     #pragma unroll
-                    for (int k = 0; k < 4; k++) {
-                        uint32_t E = elas_sad16(rXt, xt[k], 0u);
-                        E = elas_sad16(rXb, xb[k], E);
-                        E = elas_sad16(rPt, k ? xt[k - 1] : pt, E);
-                        E = elas_sad16(rPb, k ? xb[k - 1] : pb, E);
-                        take((E << 15) + kd + 4100u * (uint32_t)k);
-                    }
+                    for (int k = 0; k < 4; k++)
+                        take(energy(xt[k], xb[k], k ? xt[k - 1] : pt, k ? xb[k - 1] : pb, low(i + k)));
                     pt = xt[3];
                     pb = xb[3];
                     cx += 4 * dstep;
                 }
                 for (; i < n0; i++, cx += dstep) {
                     const uint4 xt = top[cx], xb = bot[cx];
-                    uint32_t E = elas_sad16(rXt, xt, 0u);
-                    E = elas_sad16(rXb, xb, E);
-                    E = elas_sad16(rPt, pt, E);
-                    E = elas_sad16(rPb, pb, E);
-                    const int dd = d + 4 * i;
-                    take((E << 15) | ((uint32_t)((dd - dminv) & 31) << 10) | (uint32_t)dd);
+                    take(energy(xt, xb, pt, pb, low(i)));
                     pt = xt;
                     pb = xb;
                 }
             } else {
-                #pragma unroll 2
-                for (; d <= dmaxv; d += 4, cx += dstep) {
+                for (int m = 0; d <= dmaxv; m++, d += 4, cx += dstep) {
                     const uint4 xt = top[cx], xb = bot[cx];
-                    uint32_t E = elas_sad16(rXt, xt, 0u);
-                    E = elas_sad16(rXb, xb, E);
-                    E = elas_sad16(rPt, pt, E);
-                    E = elas_sad16(rPb, pb, E);
-                    take((E << 15) | ((uint32_t)((d - dminv) & 31) << 10) | (uint32_t)d);
+                    take(energy(xt, xb, pt, pb, low(m)));
                     pt = xt;
                     pb = xb;
                 }
             }
+            // final keys E << 16 | r << 10 | d, r = (d - dmin) mod 32 (r < 32 and
+            // d < 1024 fill disjoint bits)
+            auto final_key = [&](uint32_t k) {
+                const uint32_t lo = k & 0xffffu;
+                const int m = (int)(((lo & 127u) << 3) | (lo >> 7));
+                const uint32_t r = (uint32_t)(j + 4 * (m & 7)), dd = (uint32_t)(dminv + j + 4 * m);
+                return k == 0xffffffffu ? k : (k & 0xffff0000u) | (r << 10) | dd;
+            };
+            k1 = final_key(k1);
+            k2 = final_key(k2);
         }
         #pragma unroll
         for (int off = 16; off <= 32; off <<= 1) {
@@ -172,8 +175,8 @@ __device__ __forceinline__ int support_pass(const ElasPair& P, const ElasKParams
         }
         int result = -1;
         if (ok) {
-            const uint32_t m = k1 >> 15;
-            const uint32_t min2 = (((k1 >> 10) & 31u) == 0u) ? m : (k2 >> 15);
+            const uint32_t m = k1 >> 16;
+            const uint32_t min2 = (((k1 >> 10) & 31u) == 0u) ? m : (k2 >> 16);
             const int dstar = (int)(k1 & 1023u);
             if ((float)m < kp.support_threshold * (float)min2) {
                 if (!RIGHT) result = dstar;
