@@ -22,7 +22,8 @@
 // the smallest key and the second smallest E from the second smallest key.
 //
 // Mapping: one workgroup (4 waves) = 64 consecutive candidates of a lattice
-// row.  Lane (j, c) of wave w works on candidate 16w + c and the disparities
+// row.  Lane (j, c) of wave w works on candidate 16w + c (in the right pass
+// the candidate dealt to that place, see k_support_lr) and the disparities
 // d = dmin + j + 4m (j = lane / 16).  The other image's descriptor rows v-2
 // and v+2 are staged in LDS over the columns the 64 candidates can reach.
 // E(d) pairs the reference blocks with the block pair X(d) at column
@@ -34,13 +35,13 @@
 // shuffles.
 template <bool RIGHT>
 __device__ __forceinline__ int support_pass(const ElasPair& P, const ElasKParams& kp, uint4* s_rows, int v_can,
-                                            int k0, int d_left_in) {
+                                            int k0, int cidx, int d_left_in) {
     const int tid = threadIdx.x;
-    const int wave = tid >> 6, lane = tid & 63;
-    const int j = lane >> 4, cl = lane & 15;
+    const int lane = tid & 63;
+    const int j = lane >> 4;
     {
         const int W = P.W, H = P.H, cw = P.cw, step = P.step;
-        const int u_can = k0 + 16 * wave + cl;
+        const int u_can = k0 + cidx;  // cidx < 0: no candidate
         const int v = v_can * step;
         const int u_first = k0 * step, u_last = min(k0 + 63, cw - 1) * step;
         const int c_lo = max(u_first - P.dmax - 2, 0);
@@ -68,7 +69,7 @@ __device__ __forceinline__ int support_pass(const ElasPair& P, const ElasKParams
             }
         }
         int u = u_can * step;
-        bool ok = row_ok && u_can < cw && u >= 5 && u <= W - 6;
+        bool ok = row_ok && cidx >= 0 && u_can < cw && u >= 5 && u <= W - 6;
         const int d_left = RIGHT ? d_left_in : 0;
         if (ok && RIGHT) {
             if (d_left < 0) ok = false;
@@ -188,12 +189,24 @@ __device__ __forceinline__ int support_pass(const ElasPair& P, const ElasKParams
     }
 }
 
-// Both passes of a unit in one workgroup: the left pass's result for a
-// candidate stays in its four lanes and feeds the right-to-left check, which
-// stages the other image's rows in the same LDS.
+// Both passes of a unit in one workgroup.  The left pass's result of a
+// candidate stays in LDS and feeds the right-to-left check, which stages the
+// other image's rows in the same LDS.  The right pass walks the columns
+// u - d_left + d + 2, so neighbouring candidates whose d_left differ start
+// their walks in colliding LDS bank groups (a 16-byte column is bank group
+// column mod 16, and the walk moves every lane of a wave by the same amount).
+// Its candidates (those with a left match) are therefore dealt to the waves
+// in order of that residue: rank r goes to wave r mod nw, lane r / nw, with
+// nw = ceil(count / 16), so a wave's candidates have distinct residues unless
+// more than nw share one, and the waves beyond nw have nothing to do.  The
+// results do not depend on the placement.
 __global__ __launch_bounds__(256) void k_support_lr(ElasLaunch L, ElasKParams kp) {
     extern __shared__ uint4 s_rows[];  // [2][ncols]: row v-2, then row v+2
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    __shared__ int32_t s_dl[64];       // the unit's left results
+    __shared__ int32_t s_map[64];      // right pass: the candidate at 16 w + c, or -1
+    __shared__ int32_t s_bin[17];      // residue counts, then their starts; [16]: candidates without a match
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int c = 16 * wave + (lane & 15);  // this lane's candidate in the left pass
     for (int t = blockIdx.x; t < L.total; t += gridDim.x) {
         const int p = elas_find_pair(L.prefix, L.npairs, t);
         const auto& P = L.pairs[p];
@@ -202,12 +215,40 @@ __global__ __launch_bounds__(256) void k_support_lr(ElasLaunch L, ElasKParams kp
         const int per_row = (cw + 63) >> 6;
         const int v_can = local / per_row;
         const int k0 = (local - v_can * per_row) * 64;
-        const int u_can = k0 + 16 * wave + (lane & 15);
-        const int dl = support_pass<false>(P, kp, s_rows, v_can, k0, 0);
-        const int dr = support_pass<true>(P, kp, s_rows, v_can, k0, dl);
-        if ((lane >> 4) == 0 && u_can < cw) {
-            elas_st(P.can[0] + v_can * cw + u_can, dl);
-            elas_st(P.can[1] + v_can * cw + u_can, dr);
+        const int dl = support_pass<false>(P, kp, s_rows, v_can, k0, c, 0);
+        if (tid < 17) s_bin[tid] = 0;
+        if (tid < 64) s_map[tid] = -1;
+        elas_lds_barrier();
+        const bool lead = (lane >> 4) == 0;  // one lane per candidate
+        const int key = dl >= 0 ? ((P.step * (k0 + c) - dl) & 15) : 16;
+        int rank = 0;
+        if (lead) {
+            s_dl[c] = dl;
+            rank = atomicAdd(&s_bin[key], 1);
+        }
+        elas_lds_barrier();
+        if (tid == 0) {
+            int acc = 0;
+            for (int k = 0; k < 17; k++) {
+                const int n = s_bin[k];
+                s_bin[k] = acc;
+                acc += n;
+            }
+        }
+        elas_lds_barrier();
+        if (lead && key < 16) {
+            const int r = s_bin[key] + rank, nw = max(1, (s_bin[16] + 15) >> 4);
+            s_map[16 * (r % nw) + r / nw] = c;
+        }
+        elas_lds_barrier();
+        const int cr = s_map[c];
+        const int dr = support_pass<true>(P, kp, s_rows, v_can, k0, cr, cr >= 0 ? s_dl[cr] : -1);
+        if (lead) {
+            if (k0 + c < cw) {
+                elas_st(P.can[0] + v_can * cw + k0 + c, dl);
+                if (dl < 0) elas_st(P.can[1] + v_can * cw + k0 + c, -1);
+            }
+            if (cr >= 0) elas_st(P.can[1] + v_can * cw + k0 + cr, dr);
         }
     }
 }
