@@ -192,7 +192,7 @@ def serving(pkg, synth, rates=(50, 100, 200, 400, 800), n_req=2000, pool=None, s
     return {"rates": out, "pool_pairs": len(P), "hbm_budget_GiB": 8, "buffers": "pinned host"}
 
 
-def kernel_rooflines(el, wc, W, H, dmax, reps=50):
+def kernel_rooflines(el, wc, W, H, dmax, reps=50, warm_ms=100.0):
     """Per-kernel roofline of the last call.  Each kernel's average launch
     time comes from HIP events around `reps` back-to-back re-launches on the
     pipeline stream (elas_hip_time_kernel), after about 100 ms of untimed
@@ -223,7 +223,8 @@ def kernel_rooflines(el, wc, W, H, dmax, reps=50):
         # clock is still rising; the kernel is launched back to back for about
         # 100 ms first, so the figure is its steady-state time
         cold = el.time_kernel(k, 10)
-        el.time_kernel(k, max(10, min(5000, int(100.0 / max(cold, 1e-3)))))
+        if warm_ms > 0:
+            el.time_kernel(k, max(10, min(5000, int(warm_ms / max(cold, 1e-3)))))
         ms = el.time_kernel(k, reps)
         t_sad = ops / (SAD_PEAK_TBYTE * 1e12)
         t_hbm = byts / (HBM_PEAK_GBS * 1e9)
@@ -305,6 +306,8 @@ def main():
     ap.add_argument("--dense-tiles", default="",
                     help="force the dense tiling 'cells_per_wave,steps_per_chunk' (profiler runs, where the "
                          "autotuner's timings are distorted)")
+    ap.add_argument("--kernel-warm-ms", type=float, default=100.0,
+                    help="back-to-back launches of each kernel before its timed ones (0: none; for counter runs)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy a 256 MiB device buffer 5 times (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -386,7 +389,7 @@ def main():
                 "same_maps": bool(np.array_equal(h1, D1.cpu().numpy()) and np.array_equal(h2, D2.cpu().numpy()))}
 
     wc = work_counts(oracle, left, right, dmax)
-    kernels = kernel_rooflines(el, wc, W, H, dmax)
+    kernels = kernel_rooflines(el, wc, W, H, dmax, warm_ms=args.kernel_warm_ms)
     dom = max(kernels, key=lambda k: kernels[k]["ms"])
     d = kernels[dom]
     roofline = {"bound": d["bound"], "achieved": round(d["achieved"], 4), "peak": round(d["peak"], 3),
