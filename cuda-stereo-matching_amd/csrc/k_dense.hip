@@ -235,11 +235,11 @@ __global__ __launch_bounds__(64) void k_plane(ElasLaunch L) {
 // reads its super-tile's list of 64-triangle chunks into LDS and gathers the
 // listed triangles, 16 chunks per round (their boxes are tested against the
 // tile, a lane per triangle, when the mask alone does not place a triangle
-// in the tile).  Wave w then takes the gathered triangles w, w + 4, ...: lane
-// i loads the record of the wave's i-th triangle, and the wave walks them one
-// at a time (readlane), a lane per tile column u in the triangle's half-open
-// column range [A_u, C_u): the column's row span between the long edge AC
-// and AB (u < B_u) or BC, as the reference computes it, clipped to the tile.
+// in the tile).  Wave w then takes the gathered triangles w, w + 4, ..., four
+// at a time, one per 16-lane group; a group's lanes load its record and walk
+// the tile columns u of the triangle's half-open column range [A_u, C_u), 16
+// apart: the column's row span between the long edge AC and AB (u < B_u) or
+// BC, as the reference computes it, clipped to the tile.
 // Each pixel of the span takes the LDS atomic max of the triangle index:
 // where spans overlap (the reference races there) the highest index wins,
 // the result of painting the triangles in index order.  Last, every pixel of
@@ -277,8 +277,6 @@ __global__ __launch_bounds__(256) void k_rtile(ElasLaunch L) {
         const int nent = over ? nch : nlist;
         const int4* box = (const int4*)P.tbox[img];
         const float* trirec = P.trirec[img];
-        const int u = u0 + lane;
-        const float fu = (float)u;
         for (int cb = 0; cb < nent; cb += 256) {
             // up to 256 chunks per pass
             const int j = cb + tid;
@@ -325,34 +323,27 @@ __global__ __launch_bounds__(256) void k_rtile(ElasLaunch L) {
                 __syncthreads();
                 const int n = nh;
                 const int cnt = n > wave ? (n - wave + 3) >> 2 : 0;
-                for (int g = 0; g < cnt; g += 64) {
-                    int tl = 0;
-                    float4 r0 = make_float4(0.f, 0.f, 0.f, 0.f), r1 = r0, r2 = r0, r3 = r0;
-                    if (g + lane < cnt) {
-                        tl = hits[wave + 4 * (g + lane)];
-                        const float4* rp = (const float4*)(trirec + (size_t)tl * ELAS_TRIREC);
-                        r0 = rp[0]; r1 = rp[1]; r2 = rp[2]; r3 = rp[3];
-                    }
-                    const int gn = min(64, cnt - g);
-                    for (int h = 0; h < gn; h++) {
-                        const int Au = (int)__int_as_float(__builtin_amdgcn_readlane(__float_as_int(r0.w), h));
-                        const int Bu = (int)__int_as_float(__builtin_amdgcn_readlane(__float_as_int(r1.y), h));
-                        const int Cu = (int)__int_as_float(__builtin_amdgcn_readlane(__float_as_int(r1.w), h));
-                        if (u >= max(Au, 0) && u < min(Cu, W)) {
-                            const int ti = __builtin_amdgcn_readlane(tl, h);
-                            const float ABa = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(r2.y), h));
-                            const float ABb = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(r2.z), h));
-                            const float ACa = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(r2.w), h));
-                            const float ACb = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(r3.x), h));
-                            const float BCa = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(r3.y), h));
-                            const float BCb = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(r3.z), h));
-                            const bool first = u < Bu;
+                // the wave's triangles hits[wave + 4 i], four at a time: lane
+                // group q (16 lanes) takes triangle g + q and walks its columns in
+                // the tile, lane c the columns u = first + c + 16 k
+                const int q = lane >> 4, c = lane & 15;
+                for (int g = 0; g < cnt; g += 4) {
+                    if (g + q < cnt) {
+                        const int ti = hits[wave + 4 * (g + q)];
+                        const float4* rp = (const float4*)(trirec + (size_t)ti * ELAS_TRIREC);
+                        const float4 r0 = rp[0], r1 = rp[1], r2 = rp[2], r3 = rp[3];
+                        const int Au = (int)r0.w, Bu = (int)r1.y, Cu = (int)r1.w;
+                        const float ABa = r2.y, ABb = r2.z, ACa = r2.w, ACb = r3.x, BCa = r3.y, BCb = r3.z;
+                        const int ulo = max(max(Au, 0), u0), uhi = min(min(Cu, W), u0 + ELAS_RT_W);
+                        for (int uu = ulo + c; uu < uhi; uu += 16) {
+                            const float fu = (float)uu;
+                            const bool first = uu < Bu;
                             const float ea = first ? ABa : BCa, eb = first ? ABb : BCb;
                             const int v1 = (int)elas_f2u_sat(__fadd_rn(__fmul_rn(ACa, fu), ACb));
                             const int v2 = (int)elas_f2u_sat(__fadd_rn(__fmul_rn(ea, fu), eb));
                             const int vlo = max(max(min(v1, v2), 0), v0);
                             const int vhi = min(min(max(v1, v2), H), v0 + ELAS_RT_H);
-                            for (int v = vlo; v < vhi; v++) atomicMax(&ids[(v - v0) * ELAS_RT_W + lane], ti);
+                            for (int v = vlo; v < vhi; v++) atomicMax(&ids[(v - v0) * ELAS_RT_W + (uu - u0)], ti);
                         }
                     }
                 }
