@@ -351,14 +351,24 @@ __global__ __launch_bounds__(256) void k_rtile(ElasLaunch L) {
             }
         }
         __syncthreads();
+        // every pixel once: all of a thread's plane loads (one 16-byte load of
+        // the record's a, b, c) go out before its stores
         int32_t* prior = P.prior[img];
-        for (int i = tid; i < ELAS_RT_H * ELAS_RT_W; i += 256) {
+        constexpr int PER = ELAS_RT_H * ELAS_RT_W / 256;
+        int idv[PER];
+        float4 pl[PER];
+        #pragma unroll
+        for (int k = 0; k < PER; k++) {
+            idv[k] = ids[tid + 256 * k];
+            pl[k] = ((const float4*)trirec)[(size_t)(idv[k] >= 0 ? idv[k] : 0) * (ELAS_TRIREC / 4)];
+        }
+        #pragma unroll
+        for (int k = 0; k < PER; k++) {
+            const int i = tid + 256 * k;
             const int uu = u0 + (i % ELAS_RT_W), v = v0 + i / ELAS_RT_W;
             if (uu < W && v < H) {
-                const int id = ids[i];
-                const float* rec = trirec + (size_t)(id >= 0 ? id : 0) * ELAS_TRIREC;
-                const int d = elas_f2i_sat(__fadd_rn(__fadd_rn(__fmul_rn(rec[0], (float)uu), __fmul_rn(rec[1], (float)v)), rec[2]));
-                prior[(size_t)v * W + uu] = id >= 0 ? d : -1;
+                const int d = elas_f2i_sat(__fadd_rn(__fadd_rn(__fmul_rn(pl[k].x, (float)uu), __fmul_rn(pl[k].y, (float)v)), pl[k].z));
+                prior[(size_t)v * W + uu] = idv[k] >= 0 ? d : -1;
             }
         }
         __syncthreads();
