@@ -60,6 +60,7 @@ struct ElasPair {
     int32_t dcpw, dspc;      // dense tiling: grid cells per wave per step (1, 2), steps per chunk
     int32_t dsh;             // descriptor strip rows (6, 12 or 18)
     int32_t ntri[2];         // triangles of the left / right mesh (host-known)
+    int32_t tri16;           // 1: tri[] holds 16-bit corner indices (fewer than 65536 support points)
     int32_t nblk;            // 256-cell blocks of the support lattice
     int32_t sshift, nsx, nst;  // culling super-tiles: squares of 1 << sshift raster tiles a side,
                                // nsx per row, nst in all
@@ -79,7 +80,7 @@ struct ElasPair {
                              //   disparities ascending; border cells n = 0.  gcand[1] follows
                              //   gcand[0] in one block
     int32_t* glist[2];       // [gh*gw][dmax+2] reference list layout (debug only)
-    const int32_t* tri[2];   // [ntri][3]
+    const int32_t* tri[2];   // [ntri][3] int32, or uint16 when tri16 (mapped pinned memory)
     float* trirec[2];        // [ntri][ELAS_TRIREC]
     int32_t* tbox[2];        // [ntri][4] pixel box of the triangle's spans: u0, u1, v0, v1
                              //           (half-open, all 0 when empty)

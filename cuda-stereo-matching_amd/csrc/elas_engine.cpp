@@ -631,6 +631,8 @@ int elas_ctx::slot_start_tri(ElasSlot& S) {
     S.jobs.clear();
     S.job_of.assign(2 * (size_t)n, -1);
     for (int i = 0; i < n; i++) {
+        // meshes of fewer than 65536 points cross PCIe as 16-bit corner indices
+        S.pairs[i].tri16 = S.nsp[i] <= 65536 ? 1 : 0;
         if (S.nsp[i] < 3) { S.status[i] = ELAS_ERR_TOO_FEW_POINTS; continue; }
         for (int k = 0; k < 2; k++) {
             S.job_of[2 * i + k] = (int)S.jobs.size();
@@ -640,6 +642,7 @@ int elas_ctx::slot_start_tri(ElasSlot& S) {
             j.out = nullptr;
             j.dst = S.htri + S.tri_off[i] + (size_t)k * 3 * S.dd[i].tricap;
             j.cap = S.dd[i].tricap;
+            j.dst16 = S.pairs[i].tri16 != 0;
             j.result = 0;
             S.jobs.push_back(j);
         }
@@ -1291,8 +1294,19 @@ int elas_ctx::fetch(int stage, void* dst, size_t bytes, size_t* stage_bytes) {
         case ELAS_STAGE_SP1: src = P.sp[1]; sz = 8 * (size_t)S.nsp[0]; break;
         case ELAS_STAGE_GRID0: src = P.glist[0]; sz = 4 * (size_t)P.gw * P.gh * (P.dmax + 2); break;
         case ELAS_STAGE_GRID1: src = P.glist[1]; sz = 4 * (size_t)P.gw * P.gh * (P.dmax + 2); break;
-        case ELAS_STAGE_TRI0: src = P.tri[0]; sz = 12 * (size_t)S.ntri0[0]; break;
-        case ELAS_STAGE_TRI1: src = P.tri[1]; sz = 12 * (size_t)S.ntri1[0]; break;
+        case ELAS_STAGE_TRI0:
+        case ELAS_STAGE_TRI1: {
+            // the meshes live in pinned memory, as 16-bit indices when P.tri16
+            const int k = stage == ELAS_STAGE_TRI1;
+            const size_t nt = (size_t)(k ? S.ntri1[0] : S.ntri0[0]);
+            const int32_t* h = S.htri + S.tri_off[0] + (size_t)k * 3 * S.dd[0].tricap;
+            if (stage_bytes) *stage_bytes = 12 * nt;
+            if (!dst || bytes == 0 || nt == 0) return ELAS_OK;
+            const size_t ne = std::min(bytes / 4, 3 * nt);
+            int32_t* o = (int32_t*)dst;
+            for (size_t e = 0; e < ne; e++) o[e] = P.tri16 ? (int32_t)((const uint16_t*)h)[e] : h[e];
+            return ELAS_OK;
+        }
         case ELAS_STAGE_MEAN0: src = P.mean[0]; sz = 4 * npx; break;
         case ELAS_STAGE_MEAN1: src = P.mean[1]; sz = 4 * npx; break;
         case ELAS_STAGE_D1_RAW: src = P.draw[0]; sz = 4 * npx; break;

@@ -125,6 +125,7 @@ struct SplitRun {
     std::vector<int> todo;  // the leaves
     std::unique_ptr<std::atomic<int>[]> arrived;  // per node: children done
     int32_t* dst;
+    bool dst16;  // dst takes 16-bit corner indices
     // Vertex numbers.  numbernodes() (src/triangle.cpp:7781-7796) numbers
     // the vertices in pool order, and here that is their allocation order:
     // no vertex is ever freed (vertexdealloc() has no caller) and "zQB" does
@@ -236,15 +237,22 @@ void split_read(void* arg, int t) {
 void split_copy(void* arg, int t) {
     SplitRun* r = (SplitRun*)arg;
     const SplitNode& nd = r->nodes[t];
-    if (!nd.tris.empty()) memcpy(r->dst + nd.off, nd.tris.data(), 4 * nd.tris.size());
+    if (nd.tris.empty()) return;
+    if (r->dst16) {
+        uint16_t* d = (uint16_t*)r->dst + nd.off;
+        for (size_t k = 0; k < nd.tris.size(); k++) d[k] = (uint16_t)nd.tris[k];
+    } else {
+        memcpy(r->dst + nd.off, nd.tris.data(), 4 * nd.tris.size());
+    }
 }
 
 // The merge over `vert` (count vertices; order->lx / ly hold their sort
 // orders) cut into at most 2^depth subtrees, the element list into *out, or
-// into dst when it holds cap triangles; returns ntri.
+// into dst when it holds cap triangles (as 16-bit indices with dst16);
+// returns ntri.
 int elas_finish_split(struct mesh& mesh, struct behavior& b, vertex* vert, int count, ElasTriOrder* order,
                       int depth, ElasParallelFor par, void* parctx, std::vector<int32_t>* out, int32_t* dst,
-                      int64_t cap, bool topcut_done) {
+                      int64_t cap, bool dst16, bool topcut_done) {
     SplitRun r;
     r.m = &mesh;
     r.b = &b;
@@ -254,6 +262,7 @@ int elas_finish_split(struct mesh& mesh, struct behavior& b, vertex* vert, int c
     r.nodes.reserve(2u << std::min(depth, 12));
     r.root = split_build(r.nodes, 0, count, 0, 0, depth);
     r.dst = nullptr;
+    r.dst16 = false;
     // the top of the order (the cuts the split nodes make; already made when
     // the caller built the order), then the leaves, each followed by the
     // merges its completion unlocks, up to the root
@@ -284,6 +293,7 @@ int elas_finish_split(struct mesh& mesh, struct behavior& b, vertex* vert, int c
     if (dst) {
         if (nt <= cap) {
             r.dst = dst;
+            r.dst16 = dst16;
             par(parctx, (int)r.nodes.size(), split_copy, &r);
         }
     } else {
@@ -327,7 +337,7 @@ void* elas_tri_setup(const float* pts, int n) {
 // vertex and recurses until the stack overflows (two distinct points give an
 // empty mesh, as in Triangle).
 int elas_tri_finish(void* setup, ElasTriOrder* order, bool integer, int depth, ElasParallelFor par, void* parctx,
-                    std::vector<int32_t>* out, int32_t* dst, int64_t cap) {
+                    std::vector<int32_t>* out, int32_t* dst, int64_t cap, bool dst16) {
     Setup* st = (Setup*)setup;
     struct mesh* m = &st->mesh;  // Triangle's vertex macros name the mesh `m`
     vertex* byinput = st->byinput;
@@ -356,7 +366,8 @@ int elas_tri_finish(void* setup, ElasTriOrder* order, bool integer, int depth, E
         }
     }
     if (count >= 2) {
-        nt = elas_finish_split(st->mesh, st->b, byinput, count, order, depth, par, parctx, out, dst, cap, integer);
+        nt = elas_finish_split(st->mesh, st->b, byinput, count, order, depth, par, parctx, out, dst, cap, dst16,
+                               integer);
     } else {
         triangledeinit(&st->mesh, &st->b);
         if (out) out->clear();

@@ -79,7 +79,7 @@
     namespace NS {                                                                                    \
     void* elas_tri_setup(const float* pts, int n);                                                    \
     int elas_tri_finish(void* setup, ElasTriOrder* order, bool integer, int depth, ElasParallelFor par, \
-                        void* parctx, std::vector<int32_t>* out, int32_t* dst, int64_t cap);          \
+                        void* parctx, std::vector<int32_t>* out, int32_t* dst, int64_t cap, bool dst16); \
     }
 ELAS_TRI_DECL(elas_tri0) ELAS_TRI_DECL(elas_tri1) ELAS_TRI_DECL(elas_tri2) ELAS_TRI_DECL(elas_tri3)
 ELAS_TRI_DECL(elas_tri4) ELAS_TRI_DECL(elas_tri5) ELAS_TRI_DECL(elas_tri6) ELAS_TRI_DECL(elas_tri7)
@@ -96,7 +96,7 @@ const SetupFn kSetup[ELAS_TRI_INSTANCES] = {
     elas_tri12::elas_tri_setup, elas_tri13::elas_tri_setup, elas_tri14::elas_tri_setup, elas_tri15::elas_tri_setup};
 
 typedef int (*FinishFn)(void*, ElasTriOrder*, bool, int, ElasParallelFor, void*, std::vector<int32_t>*, int32_t*,
-                        int64_t);
+                        int64_t, bool);
 const FinishFn kFinish[ELAS_TRI_INSTANCES] = {
     elas_tri0::elas_tri_finish,  elas_tri1::elas_tri_finish,  elas_tri2::elas_tri_finish,
     elas_tri3::elas_tri_finish,  elas_tri4::elas_tri_finish,  elas_tri5::elas_tri_finish,
@@ -504,7 +504,7 @@ int tri_call(int k, ElasTriJob& j, bool split, void* pool) {
         tri_prepare(&c, 0);
         tri_prepare(&c, 1);
     }
-    return kFinish[k](c.setup, &order, c.integer, depth, pool_parallel_for, pool, j.out, j.dst, j.cap);
+    return kFinish[k](c.setup, &order, c.integer, depth, pool_parallel_for, pool, j.out, j.dst, j.cap, j.dst16);
 }
 
 int run_instance(int k, ElasTriJob& j, bool split, void* pool) {

@@ -16,6 +16,7 @@ struct ElasTriJob {
     std::vector<int32_t>* out;   // [ntri][3] zero-based triangles, unless dst is set
     int32_t* dst = nullptr;      // else: written here directly, if ntri <= cap
     int64_t cap = 0;             // capacity of dst in triangles
+    bool dst16 = false;          // dst takes 16-bit corner indices ([ntri][3] uint16; n <= 65536)
     int result;                  // ntri, or -1 for fewer than 3 distinct points
     // filled by the worker: seconds (steady clock) around the Triangle call
     double t_begin = 0, t_end = 0;

@@ -93,13 +93,18 @@ __device__ bool elas_solve3(double A[3][3], double B[3]) {
 
 // Plane coefficients and sorted-corner edge lines of triangle ti of image img.
 __device__ __forceinline__ void elas_plane_rec(const ElasPair& P, int img, int ti, float* rec) {
+    // corners: 16-bit indices (meshes of fewer than 65536 points) or 32-bit
     const int32_t* tr = P.tri[img] + 3 * (size_t)ti;
+    const uint16_t* tr16 = (const uint16_t*)P.tri[img] + 3 * (size_t)ti;
     const float* sp = P.sp[0];
     const float* sq = P.sp[1];
     int tu[3], tv[3], td[3];
     double A[3][3], b[3];
     int ci[3];
-    for (int i = 0; i < 3; i++) ci[i] = tr[i];
+    if (P.tri16)
+        for (int i = 0; i < 3; i++) ci[i] = tr16[i];
+    else
+        for (int i = 0; i < 3; i++) ci[i] = tr[i];
     for (int i = 0; i < 3; i++) {
         tu[i] = (int)sp[2 * ci[i]];
         tv[i] = (int)sp[2 * ci[i] + 1];
