@@ -351,8 +351,9 @@ __global__ __launch_bounds__(256) void k_incon(ElasLaunch L, ElasKParams kp) {
 //   removed(x)= valid(x) && ahead(x) && (M(x) & ~H(x)) != 0,
 // where H(x) holds the removed flags of x-1..x-5.  A workgroup takes lpw lines
 // (elas_redun_lines) into LDS with coalesced loads, computes ahead/M for every
-// cell in parallel, then one wave per line runs the 5-bit recurrence on the
-// scalar unit, and all threads write the lines back.
+// cell in parallel, then one wave per line runs the 5-bit recurrence (16-cell
+// segments for all 32 entry states in parallel, chained by a scalar walk), and
+// all threads write the lines back.
 #define RD_INV (-30000)  // invalid cell in the int16 strip: never within 1 of a valid d
 __device__ __forceinline__ bool rd_near(int a, int d) { return (unsigned)(a - d + 1) <= 2u; }
 
@@ -429,48 +430,54 @@ __global__ __launch_bounds__(256) void k_redun(ElasLaunch L) {
         }
         elas_lds_barrier();
         {
-            // The recurrence runs on the scalar unit, one line per wave: nh has
-            // bit j-1 set when x-j was kept, and x is removed when its flag
-            // meets a kept neighbour.  Each lane holds 4 flag bytes of a
-            // 256-cell chunk; only the non-zero words are visited (a ballot
-            // lists them): across a run of zero flags every cell is kept, so the
-            // state just shifts in ones.  Lane k keeps the removal nibble of
-            // its 4 cells for a parallel apply.
+            // The recurrence, one line per wave.  nh has bit j-1 set when x-j
+            // was kept; x is removed when its flag meets a kept neighbour, and
+            // nh becomes ((nh << 1) & 30) | !removed.  A lane takes a segment of
+            // 16 cells and runs them for all 32 entry states at once: plane b
+            // holds bit b of nh for entry state s in its bit s, and the
+            // removals in bit s of one word per cell.  A scalar walk over the
+            // segments (five readlanes each) then finds every segment's actual
+            // entry state, and each lane applies its removals.
             const int wv = tid >> 6, lane = tid & 63;
             if (wv < nl) {
-                const uint32_t* fl = reinterpret_cast<const uint32_t*>(s_flag + wv * lenp);
+                const uint8_t* fl = s_flag + wv * lenp;
                 int16_t* row = s_val + wv * len;
-                uint32_t nh = 0;
-                auto skip = [&](int cells) {
-                    const int g = min(cells, 5);
-                    nh = ((nh << g) | ((1u << g) - 1u)) & 31u;
-                };
-                for (int c0 = 0; c0 < lenp; c0 += 256) {
-                    const int nd = min(64, (lenp - c0) >> 2);
-                    const uint32_t fv = lane < nd ? fl[(c0 >> 2) + lane] : 0u;
-                    uint64_t nz = __ballot(fv != 0u);
-                    uint32_t rmv = 0;
-                    int kprev = -1;
-                    while (nz) {
-                        const int k = __builtin_ctzll(nz);
-                        nz &= nz - 1u;
-                        skip(4 * (k - kprev - 1));
-                        const uint32_t f4 = __builtin_amdgcn_readlane(fv, k);
-                        uint32_t r4 = 0;
-                        #pragma unroll
-                        for (int i = 0; i < 4; i++) {
-                            const uint32_t m = (f4 >> (8 * i)) & 31u;
-                            const uint32_t r = ((m & nh) + 31u) >> 5;  // (m & nh) != 0, without a branch
-                            nh = ((nh << 1) & 30u) | (r ^ 1u);
-                            r4 |= r << i;
-                        }
-                        rmv = lane == k ? r4 : rmv;
-                        kprev = k;
-                    }
-                    skip(4 * (nd - kprev - 1));
+                const int nseg = lenp >> 4;
+                uint32_t s_in = 0;  // the state entering the first segment of the chunk
+                for (int g0 = 0; g0 < nseg; g0 += 64) {
+                    const int seg = g0 + lane;
+                    const uint4 f = seg < nseg ? *reinterpret_cast<const uint4*>(fl + 16 * seg) : make_uint4(0, 0, 0, 0);
+                    const uint32_t fw[4] = {f.x, f.y, f.z, f.w};
+                    uint32_t p0 = 0xAAAAAAAAu, p1 = 0xCCCCCCCCu, p2 = 0xF0F0F0F0u, p3 = 0xFF00FF00u, p4 = 0xFFFF0000u;
+                    uint32_t rp[16];
                     #pragma unroll
-                    for (int i = 0; i < 4; i++)
-                        if ((rmv >> i) & 1u) row[c0 + 4 * lane + i] = (int16_t)RD_INV;
+                    for (int c = 0; c < 16; c++) {
+                        const uint32_t m = (fw[c >> 2] >> (8 * (c & 3))) & 31u;
+                        const uint32_t r = ((m & 1u) ? p0 : 0u) | ((m & 2u) ? p1 : 0u) | ((m & 4u) ? p2 : 0u) |
+                                           ((m & 8u) ? p3 : 0u) | ((m & 16u) ? p4 : 0u);
+                        rp[c] = r;
+                        p4 = p3;
+                        p3 = p2;
+                        p2 = p1;
+                        p1 = p0;
+                        p0 = ~r;
+                    }
+                    const int ns = min(64, nseg - g0);
+                    uint32_t st = s_in, my_in = 0;
+                    for (int k = 0; k < ns; k++) {
+                        my_in = lane == k ? st : my_in;
+                        const uint32_t q0 = __builtin_amdgcn_readlane(p0, k), q1 = __builtin_amdgcn_readlane(p1, k),
+                                       q2 = __builtin_amdgcn_readlane(p2, k), q3 = __builtin_amdgcn_readlane(p3, k),
+                                       q4 = __builtin_amdgcn_readlane(p4, k);
+                        st = ((q0 >> st) & 1u) | (((q1 >> st) & 1u) << 1) | (((q2 >> st) & 1u) << 2) |
+                             (((q3 >> st) & 1u) << 3) | (((q4 >> st) & 1u) << 4);
+                    }
+                    s_in = st;
+                    if (seg < nseg) {
+                        #pragma unroll
+                        for (int c = 0; c < 16; c++)
+                            if (((rp[c] >> my_in) & 1u) && 16 * seg + c < len) row[16 * seg + c] = (int16_t)RD_INV;
+                    }
                 }
             }
         }
