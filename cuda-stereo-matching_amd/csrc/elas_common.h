@@ -81,9 +81,9 @@ struct ElasPair {
                              //   gcand[0] in one block
     int32_t* glist[2];       // [gh*gw][dmax+2] reference list layout (debug only)
     const int32_t* tri[2];   // [ntri][3] int32, or uint16 when tri16 (mapped pinned memory)
-    float* trirec[2];        // [ntri][ELAS_TRIREC]
-    int32_t* tbox[2];        // [ntri][4] pixel box of the triangle's spans: u0, u1, v0, v1
-                             //           (half-open, all 0 when empty)
+    float* trirec[2];        // [ntri][ELAS_TRIREC]: (A_u, B_u, C_u, box rows v0 | v1 << 16), AB and AC
+                             //   edges, BC edge, plane (a, b, c); the box is half-open, rows 0
+                             //   when empty
     int32_t* scnt[2];        // [nst] entries appended to a super-tile's list (zeroed in phase A);
                              //       above scap the list is incomplete
     int32_t* slist[2];       // [nst][scap][4] per super-tile, in no particular order: chunk c and the

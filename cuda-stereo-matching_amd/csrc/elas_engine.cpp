@@ -104,8 +104,7 @@ size_t pair_arena(const ElasPairSpec& s, const ElasDims& d, bool debug, bool pos
     b += 2 * al(4 * d.ncell * d.words) + al(4 * d.ncell * (size_t)elas_cand_stride(s.dmax));
     if (debug) b += 2 * al(4 * d.ncell * (size_t)(s.dmax + 2)) + 2 * al(4 * d.npx);
     b += 2 * al(4 * ELAS_TRIREC * (size_t)d.tricap);
-    b += 2 * al(16 * (size_t)d.tricap) + 2 * al(4 * (size_t)d.nst) +
-         2 * al(16 * (size_t)d.nst * d.scap);  // boxes, culling lists
+    b += 2 * al(4 * (size_t)d.nst) + 2 * al(16 * (size_t)d.nst * d.scap);  // culling lists
     b += 2 * al(4 * d.npx) + 2 * al(4 * d.npx);  // prior, draw
     if (!s.dev_out) b += 2 * al(4 * d.npx);
     if (post) b += 2 * al(8 * d.npx);  // post-filter scratch
@@ -442,7 +441,6 @@ int elas_ctx::slot_prepare(ElasSlot& S, const ElasPairSpec* sp, int n) {
             for (int k = 0; k < 2; k++) P.mean[k] = (float*)take(4 * d.npx);
         }
         for (int k = 0; k < 2; k++) P.trirec[k] = (float*)take(4 * ELAS_TRIREC * (size_t)d.tricap);
-        for (int k = 0; k < 2; k++) P.tbox[k] = (int32_t*)take(16 * (size_t)d.tricap);
         for (int k = 0; k < 2; k++) P.scnt[k] = (int32_t*)take(4 * (size_t)d.nst);
         for (int k = 0; k < 2; k++) P.slist[k] = (int32_t*)take(16 * (size_t)d.nst * d.scap);
         P.sshift = d.sshift; P.nsx = d.nsx; P.nst = d.nst; P.scap = tiny_lists ? 1 : d.scap;

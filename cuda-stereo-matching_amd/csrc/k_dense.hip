@@ -145,10 +145,10 @@ __device__ __forceinline__ void elas_plane_rec(const ElasPair& P, int img, int t
 // KerComputeDisparity.cu:63-66,95-99).  A corner's d is sp0.u - sp1.u (exact
 // in float), which saves a dependent load of the lattice.
 //
-// Besides the record, a lane stores its triangle's pixel box: columns
-// [max(A_u, 0), min(C_u, W)) (the spans are half-open between corner
-// columns) and the corner rows widened by a margin that covers the float
-// error of the span ends.  A span end is the truncation of s*u + b, where
+// The record carries the triangle's pixel box: columns [max(A_u, 0),
+// min(C_u, W)) (the spans are half-open between corner columns) and the
+// corner rows widened by a margin that covers the float error of the span
+// ends (rows < 2^14, so both ends pack into one word).  A span end is the truncation of s*u + b, where
 // s = fl((v_a - v_b) / (u_a - u_b)) and b = fl(v_a - fl(s*u_a)) for integer
 // corners, evaluated at integer u between them as fl(fl(s*u) + b).  Against
 // the exact line through the corners this is off by at most
@@ -183,8 +183,6 @@ __global__ __launch_bounds__(64) void k_plane(ElasLaunch L) {
         if (ti < P.ntri[img]) {
             float rec[ELAS_TRIREC];
             elas_plane_rec(P, img, ti, rec);
-            float4* dst = (float4*)(P.trirec[img] + (size_t)ti * ELAS_TRIREC);
-            for (int k = 0; k < 4; k++) dst[k] = make_float4(rec[4 * k], rec[4 * k + 1], rec[4 * k + 2], rec[4 * k + 3]);
             const float Au = rec[3], Av = rec[4], Bv = rec[6], Cu = rec[7], Cv = rec[8];
             const float smax = fmaxf(fabsf(rec[9]), fmaxf(fabsf(rec[11]), fabsf(rec[13])));
             const float U = fmaxf(fabsf(Au), fabsf(Cu)) + 1.0f;
@@ -194,7 +192,13 @@ __global__ __launch_bounds__(64) void k_plane(ElasLaunch L) {
             const int vmin = (int)fminf(Av, fminf(Bv, Cv)), vmax = (int)fmaxf(Av, fmaxf(Bv, Cv));
             int4 bx = make_int4(max((int)Au, 0), min((int)Cu, P.W), max(vmin - margin, 0), min(vmax + margin + 1, P.H));
             if (bx.x >= bx.y || bx.z >= bx.w) bx = make_int4(0, 0, 0, 0);
-            ((int4*)P.tbox[img])[ti] = bx;
+            // the raster's record: (A_u, B_u, C_u, box rows v0 | v1 << 16),
+            // the AB and AC edges, the BC edge, the plane
+            float4* dst = (float4*)(P.trirec[img] + (size_t)ti * ELAS_TRIREC);
+            dst[0] = make_float4(rec[3], rec[5], rec[7], __int_as_float(bx.z | (bx.w << 16)));
+            dst[1] = make_float4(rec[9], rec[10], rec[11], rec[12]);
+            dst[2] = make_float4(rec[13], rec[14], 0.f, 0.f);
+            dst[3] = make_float4(rec[0], rec[1], rec[2], 0.f);
             if (bx.x < bx.y) {
                 const int sx0 = bx.x >> (ELAS_RT_WS + sh), sx1 = (bx.y - 1) >> (ELAS_RT_WS + sh);
                 const int sy0 = bx.z >> (ELAS_RT_HS + sh), sy1 = (bx.w - 1) >> (ELAS_RT_HS + sh);
@@ -238,24 +242,29 @@ __global__ __launch_bounds__(64) void k_plane(ElasLaunch L) {
 // ---- rasterisation: KerComputeDisparity.cu:126-150 ------------------------
 // A workgroup owns an ELAS_RT_W x ELAS_RT_H pixel tile of one image.  It
 // reads its super-tile's list of 64-triangle chunks into LDS and gathers the
-// listed triangles, 16 chunks per round (their boxes are tested against the
-// tile, a lane per triangle, when the mask alone does not place a triangle
-// in the tile).  Wave w then takes the gathered triangles w, w + 4, ..., four
-// at a time, one per 16-lane group; a group's lanes load its record and walk
-// the tile columns u of the triangle's half-open column range [A_u, C_u), 16
-// apart: the column's row span between the long edge AC and AB (u < B_u) or
-// BC, as the reference computes it, clipped to the tile.
+// listed triangles four chunks per round, a wave per chunk and a lane per
+// triangle: a lane whose mask bit is set loads the triangle's record, tests
+// its box against the tile and, on a hit, appends the walk's part of it to
+// LDS (the edges, the triangle index and its columns in the tile).  Wave w
+// then takes the gathered triangles w, w + 4, ..., four at a time, one per
+// 16-lane group; a group's lanes walk the tile columns u of the triangle's
+// half-open column range [A_u, C_u), 16 apart: the column's row span between
+// the long edge AC and AB (u < B_u) or BC, as the reference computes it,
+// clipped to the tile.
 // Each pixel of the span takes the LDS atomic max of the triangle index:
 // where spans overlap (the reference races there) the highest index wins,
 // the result of painting the triangles in index order.  Last, every pixel of
 // the tile is written once: the plane prior (int)(a*u + b*v + c) of its
 // triangle, or -1 where no span covers it.  No global atomics.
-#define ELAS_RT_HITS 1024  // 16 chunks of 64 triangles per gather round
+#define ELAS_RT_CH 64  // list entries (chunks) read into LDS per pass
 __global__ __launch_bounds__(256) void k_rtile(ElasLaunch L) {
     __shared__ int32_t ids[ELAS_RT_H * ELAS_RT_W];
-    __shared__ int32_t hits[ELAS_RT_HITS];
-    __shared__ unsigned long long nzm[256];
-    __shared__ int32_t nzc[256];
+    // gathered triangle s: srec[2s] = (ABa, ABb, ACa, ACb), srec[2s+1] =
+    // (BCa, BCb, columns, index); columns = ulo | uhi << 8 | bu << 16, the
+    // triangle's columns [ulo, uhi) and B_u clamped to [0, 64], tile-relative
+    __shared__ float4 srec[2 * 256];
+    __shared__ unsigned long long nzm[ELAS_RT_CH];
+    __shared__ int32_t nzc[ELAS_RT_CH];
     __shared__ int32_t nh;
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -273,82 +282,79 @@ __global__ __launch_bounds__(256) void k_rtile(ElasLaunch L) {
         const int st = (tx >> sh) + (ty >> sh) * P.nsx;
         for (int i = tid; i < ELAS_RT_H * ELAS_RT_W; i += 256) ids[i] = -1;
         const int4* slist = (const int4*)P.slist[img] + (size_t)st * P.scap;
-        // the count and the list's first 64 entries in one round trip
+        // the count and the list's first entries in one round trip
         const int nlist = P.scnt[img][st];
-        const int4 e0 = tid < 64 && tid < P.scap ? slist[tid] : make_int4(0, 0, 0, 0);
+        const int4 e0 = tid < ELAS_RT_CH && tid < P.scap ? slist[tid] : make_int4(0, 0, 0, 0);
         const int ntri = P.ntri[img], nch = (ntri + 63) >> 6;
-        const bool over = nlist > P.scap;        // incomplete list: every chunk, every triangle
-        const bool boxed = sh != 0 || over;      // a mask bit does not imply the tile
+        const bool over = nlist > P.scap;  // incomplete list: every chunk, every triangle
         const int nent = over ? nch : nlist;
-        const int4* box = (const int4*)P.tbox[img];
-        const float* trirec = P.trirec[img];
-        for (int cb = 0; cb < nent; cb += 256) {
-            // up to 256 chunks per pass
+        const float4* rec4 = (const float4*)P.trirec[img];
+        for (int cb = 0; cb < nent; cb += ELAS_RT_CH) {
             const int j = cb + tid;
-            if (j < nent) {
+            if (tid < ELAS_RT_CH && j < nent) {
                 if (over) {
                     nzc[tid] = j;
                     nzm[tid] = ntri - 64 * j >= 64 ? ~0ull : (1ull << (ntri - 64 * j)) - 1ull;
                 } else {
-                    const int4 e = j < 64 ? e0 : slist[j];
+                    const int4 e = j < ELAS_RT_CH ? e0 : slist[j];
                     nzc[tid] = e.x;
                     nzm[tid] = (unsigned long long)(uint32_t)e.y | ((unsigned long long)(uint32_t)e.z << 32);
                 }
             }
-            __syncthreads();
-            const int nz = min(256, nent - cb);
-            for (int r = 0; r < nz; r += 16) {
+            const int nz = min(ELAS_RT_CH, nent - cb);
+            for (int r = 0; r < nz; r += 4) {
                 if (tid == 0) nh = 0;
                 __syncthreads();
-                // gather: wave w takes chunks r + w + 4q, a lane per triangle
-                int cq[4];
-                bool in[4];
-                int4 bq[4];
-#pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    const int jq = r + wave + 4 * q;
-                    cq[q] = jq < nz ? nzc[jq] : 0;
-                    in[q] = jq < nz && ((nzm[jq] >> lane) & 1ull);
-                    bq[q] = in[q] && boxed ? box[cq[q] * 64 + lane] : make_int4(u0, u0 + 1, v0, v0 + 1);
+                // gather: wave w takes chunk r + w, a lane per triangle
+                const int jq = r + wave;
+                const int ti = (jq < nz ? nzc[jq] : 0) * 64 + lane;
+                const bool in = jq < nz && ((nzm[jq] >> lane) & 1ull);
+                float4 a0 = make_float4(0.f, 0.f, 0.f, 0.f), a1 = a0, a2 = a0;
+                if (in) {
+                    a0 = rec4[(size_t)ti * 4];
+                    a1 = rec4[(size_t)ti * 4 + 1];
+                    a2 = rec4[(size_t)ti * 4 + 2];
                 }
-#pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    const bool hit = in[q] && bq[q].x < u0 + ELAS_RT_W && bq[q].y > u0 &&
-                                     bq[q].z < v0 + ELAS_RT_H && bq[q].w > v0;
-                    const unsigned long long bal = __ballot(hit);
-                    if (bal) {
-                        int base = 0;
-                        if (lane == 0) base = atomicAdd(&nh, __popcll(bal));
-                        base = __builtin_amdgcn_readfirstlane(base);
-                        const int below = __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32),
-                                                                     __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0));
-                        if (hit) hits[base + below] = cq[q] * 64 + lane;
+                const int rows = __float_as_int(a0.w);
+                const int ulo = max(max((int)a0.x, 0), u0), uhi = min(min((int)a0.z, W), u0 + ELAS_RT_W);
+                const bool hit = in && ulo < uhi && (rows & 0xffff) < v0 + ELAS_RT_H && (rows >> 16) > v0;
+                const unsigned long long bal = __ballot(hit);
+                if (bal) {
+                    int base = 0;
+                    if (lane == 0) base = atomicAdd(&nh, __popcll(bal));
+                    base = __builtin_amdgcn_readfirstlane(base);
+                    const int below = __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32),
+                                                                 __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0));
+                    if (hit) {
+                        const int bu = min(max((int)a0.y - u0, 0), ELAS_RT_W);
+                        const int cols = (ulo - u0) | ((uhi - u0) << 8) | (bu << 16);
+                        srec[2 * (base + below)] = a1;
+                        srec[2 * (base + below) + 1] = make_float4(a2.x, a2.y, __int_as_float(cols), __int_as_float(ti));
                     }
                 }
                 __syncthreads();
                 const int n = nh;
                 const int cnt = n > wave ? (n - wave + 3) >> 2 : 0;
-                // the wave's triangles hits[wave + 4 i], four at a time: lane
-                // group q (16 lanes) takes triangle g + q and walks its columns in
-                // the tile, lane c the columns u = first + c + 16 k
+                // the wave's triangles w + 4 i, four at a time: lane group q
+                // (16 lanes) takes triangle g + q and walks its columns in the
+                // tile, lane c the columns first + c + 16 k
                 const int q = lane >> 4, c = lane & 15;
                 for (int g = 0; g < cnt; g += 4) {
                     if (g + q < cnt) {
-                        const int ti = hits[wave + 4 * (g + q)];
-                        const float4* rp = (const float4*)(trirec + (size_t)ti * ELAS_TRIREC);
-                        const float4 r0 = rp[0], r1 = rp[1], r2 = rp[2], r3 = rp[3];
-                        const int Au = (int)r0.w, Bu = (int)r1.y, Cu = (int)r1.w;
-                        const float ABa = r2.y, ABb = r2.z, ACa = r2.w, ACb = r3.x, BCa = r3.y, BCb = r3.z;
-                        const int ulo = max(max(Au, 0), u0), uhi = min(min(Cu, W), u0 + ELAS_RT_W);
-                        for (int uu = ulo + c; uu < uhi; uu += 16) {
-                            const float fu = (float)uu;
-                            const bool first = uu < Bu;
+                        const int sl = wave + 4 * (g + q);
+                        const float4 r0 = srec[2 * sl], r1 = srec[2 * sl + 1];
+                        const float ABa = r0.x, ABb = r0.y, ACa = r0.z, ACb = r0.w, BCa = r1.x, BCb = r1.y;
+                        const int cols = __float_as_int(r1.z), tr = __float_as_int(r1.w);
+                        const int uhi = (cols >> 8) & 255, bu = cols >> 16;
+                        for (int uo = (cols & 255) + c; uo < uhi; uo += 16) {
+                            const float fu = (float)(u0 + uo);
+                            const bool first = uo < bu;
                             const float ea = first ? ABa : BCa, eb = first ? ABb : BCb;
                             const int v1 = (int)elas_f2u_sat(__fadd_rn(__fmul_rn(ACa, fu), ACb));
                             const int v2 = (int)elas_f2u_sat(__fadd_rn(__fmul_rn(ea, fu), eb));
                             const int vlo = max(max(min(v1, v2), 0), v0);
                             const int vhi = min(min(max(v1, v2), H), v0 + ELAS_RT_H);
-                            for (int v = vlo; v < vhi; v++) atomicMax(&ids[(v - v0) * ELAS_RT_W + (uu - u0)], ti);
+                            for (int v = vlo; v < vhi; v++) atomicMax(&ids[(v - v0) * ELAS_RT_W + uo], tr);
                         }
                     }
                 }
@@ -365,7 +371,7 @@ __global__ __launch_bounds__(256) void k_rtile(ElasLaunch L) {
         #pragma unroll
         for (int k = 0; k < PER; k++) {
             idv[k] = ids[tid + 256 * k];
-            pl[k] = ((const float4*)trirec)[(size_t)(idv[k] >= 0 ? idv[k] : 0) * (ELAS_TRIREC / 4)];
+            pl[k] = rec4[(size_t)(idv[k] >= 0 ? idv[k] : 0) * 4 + 3];
         }
         #pragma unroll
         for (int k = 0; k < PER; k++) {
