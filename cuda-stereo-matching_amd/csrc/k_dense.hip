@@ -245,7 +245,8 @@ __global__ __launch_bounds__(64) void k_plane(ElasLaunch L) {
 // listed triangles four chunks per round, a wave per chunk and a lane per
 // triangle: a lane whose mask bit is set loads the triangle's record, tests
 // its box against the tile and, on a hit, appends the walk's part of it to
-// LDS (the edges, the triangle index and its columns in the tile).  Wave w
+// LDS (the edges, the triangle index and its columns in the tile).  The next
+// round's records are loaded before this round's walk.  Wave w
 // then takes the gathered triangles w, w + 4, ..., four at a time, one per
 // 16-lane group; a group's lanes walk the tile columns u of the triangle's
 // half-open column range [A_u, C_u), 16 apart: the column's row span between
@@ -257,7 +258,7 @@ __global__ __launch_bounds__(64) void k_plane(ElasLaunch L) {
 // the tile is written once: the plane prior (int)(a*u + b*v + c) of its
 // triangle, or -1 where no span covers it.  No global atomics.
 #define ELAS_RT_CH 64  // list entries (chunks) read into LDS per pass
-__global__ __launch_bounds__(256) void k_rtile(ElasLaunch L) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_rtile(ElasLaunch L) {
     __shared__ int32_t ids[ELAS_RT_H * ELAS_RT_W];
     // gathered triangle s: srec[2s] = (ABa, ABb, ACa, ACb), srec[2s+1] =
     // (BCa, BCb, columns, index); columns = ulo | uhi << 8 | bu << 16, the
@@ -265,7 +266,7 @@ __global__ __launch_bounds__(256) void k_rtile(ElasLaunch L) {
     __shared__ float4 srec[2 * 256];
     __shared__ unsigned long long nzm[ELAS_RT_CH];
     __shared__ int32_t nzc[ELAS_RT_CH];
-    __shared__ int32_t nh;
+    __shared__ int32_t nh[2];
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     for (int t = blockIdx.x; t < L.total; t += gridDim.x) {
@@ -301,27 +302,37 @@ __global__ __launch_bounds__(256) void k_rtile(ElasLaunch L) {
                     nzm[tid] = (unsigned long long)(uint32_t)e.y | ((unsigned long long)(uint32_t)e.z << 32);
                 }
             }
+            if (tid == 0) nh[0] = 0;
+            __syncthreads();
             const int nz = min(ELAS_RT_CH, nent - cb);
-            for (int r = 0; r < nz; r += 4) {
-                if (tid == 0) nh = 0;
-                __syncthreads();
-                // gather: wave w takes chunk r + w, a lane per triangle
-                const int jq = r + wave;
-                const int ti = (jq < nz ? nzc[jq] : 0) * 64 + lane;
-                const bool in = jq < nz && ((nzm[jq] >> lane) & 1ull);
-                float4 a0 = make_float4(0.f, 0.f, 0.f, 0.f), a1 = a0, a2 = a0;
+            // a round's records: wave w takes chunk jq = r + w, a lane per
+            // triangle; the next round's go out before this round's walk
+            int ti;
+            bool in;
+            float4 a0, a1, a2;
+            auto fetch = [&](int jq) {
+                ti = (jq < nz ? nzc[jq] : 0) * 64 + lane;
+                in = jq < nz && ((nzm[jq] >> lane) & 1ull);
+                a0 = make_float4(0.f, 0.f, 0.f, 0.f);
+                a1 = a2 = a0;
                 if (in) {
                     a0 = rec4[(size_t)ti * 4];
                     a1 = rec4[(size_t)ti * 4 + 1];
                     a2 = rec4[(size_t)ti * 4 + 2];
                 }
+            };
+            fetch(wave);
+            for (int r = 0; r < nz; r += 4) {
+                // round r appends to nh[h]; the other count is cleared for the
+                // next round once every wave has read it
+                const int h = (r >> 2) & 1;
                 const int rows = __float_as_int(a0.w);
                 const int ulo = max(max((int)a0.x, 0), u0), uhi = min(min((int)a0.z, W), u0 + ELAS_RT_W);
                 const bool hit = in && ulo < uhi && (rows & 0xffff) < v0 + ELAS_RT_H && (rows >> 16) > v0;
                 const unsigned long long bal = __ballot(hit);
                 if (bal) {
                     int base = 0;
-                    if (lane == 0) base = atomicAdd(&nh, __popcll(bal));
+                    if (lane == 0) base = atomicAdd(&nh[h], __popcll(bal));
                     base = __builtin_amdgcn_readfirstlane(base);
                     const int below = __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32),
                                                                  __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0));
@@ -333,7 +344,9 @@ __global__ __launch_bounds__(256) void k_rtile(ElasLaunch L) {
                     }
                 }
                 __syncthreads();
-                const int n = nh;
+                const int n = nh[h];
+                if (tid == 0) nh[h ^ 1] = 0;
+                fetch(r + 4 + wave);
                 const int cnt = n > wave ? (n - wave + 3) >> 2 : 0;
                 // the wave's triangles w + 4 i, four at a time: lane group q
                 // (16 lanes) takes triangle g + q and walks its columns in the
