@@ -128,6 +128,35 @@ __device__ __forceinline__ int support_pass(const ElasPair& P, const ElasKParams
                 // the usual case: every computing lane of the wave walks the same
                 // count, so the loop runs on the scalar unit
                 int i = 0;
+                // eight steps an iteration, as two groups of four whose loads
+                // are not hoisted above the other group's SADs: the second
+                // group's last pair can take the registers of the carried
+                // pair, so the carry costs no copies
+                for (; i + 8 <= n0; i += 8) {
+                    uint4 xt[4], xb[4];
+                    #pragma unroll
+                    for (int k = 0; k < 4; k++) {
+                        xt[k] = top[cx + k * dstep];
+                        xb[k] = bot[cx + k * dstep];
+                    }
+                    #pragma unroll
+                    for (int k = 0; k < 4; k++)
+                        take(energy(xt[k], xb[k], k ? xt[k - 1] : pt, k ? xb[k - 1] : pb, low(i + k)));
+                    __builtin_amdgcn_sched_barrier(0);
+                    uint4 yt[4], yb[4];
+                    #pragma unroll
+                    for (int k = 0; k < 4; k++) {
+                        yt[k] = top[cx + (4 + k) * dstep];
+                        yb[k] = bot[cx + (4 + k) * dstep];
+                    }
+                    #pragma unroll
+                    for (int k = 0; k < 4; k++)
+                        take(energy(yt[k], yb[k], k ? yt[k - 1] : xt[3], k ? yb[k - 1] : xb[3], low(i + 4 + k)));
+                    __builtin_amdgcn_sched_barrier(0);
+                    pt = yt[3];
+                    pb = yb[3];
+                    cx += 8 * dstep;
+                }
                 for (; i + 4 <= n0; i += 4) {
                     uint4 xt[4], xb[4];
                     #pragma unroll
@@ -149,11 +178,41 @@ __device__ __forceinline__ int support_pass(const ElasPair& P, const ElasKParams
                     pb = xb;
                 }
             } else {
-                for (int m = 0; d <= dmaxv; m++, d += 4, cx += dstep) {
-                    const uint4 xt = top[cx], xb = bot[cx];
-                    take(energy(xt, xb, pt, pb, low(m)));
-                    pt = xt;
-                    pb = xb;
+                // lanes walk different counts (near the image border): the
+                // largest count in groups of eight steps, a lane's steps past
+                // its own count starting from 0xC0000000, which stays above
+                // every real key (energies < 2^14) and below 0xffffffff, and
+                // never ranks among a candidate's two smallest (every
+                // computing candidate has at least 11 disparities)
+                int nmax = n0;
+                for (uint64_t more = __ballot(n > nmax); more; more = __ballot(n > nmax))
+                    nmax = __builtin_amdgcn_readlane(n, __builtin_ctzll(more));
+                for (int i = 0; i < nmax; i += 8) {
+                    uint4 xt[4], xb[4];
+                    #pragma unroll
+                    for (int k = 0; k < 4; k++) {
+                        xt[k] = top[cx + k * dstep];
+                        xb[k] = bot[cx + k * dstep];
+                    }
+                    #pragma unroll
+                    for (int k = 0; k < 4; k++)
+                        take(energy(xt[k], xb[k], k ? xt[k - 1] : pt, k ? xb[k - 1] : pb,
+                                    i + k < n ? low(i + k) : 0xC0000000u));
+                    __builtin_amdgcn_sched_barrier(0);
+                    uint4 yt[4], yb[4];
+                    #pragma unroll
+                    for (int k = 0; k < 4; k++) {
+                        yt[k] = top[cx + (4 + k) * dstep];
+                        yb[k] = bot[cx + (4 + k) * dstep];
+                    }
+                    #pragma unroll
+                    for (int k = 0; k < 4; k++)
+                        take(energy(yt[k], yb[k], k ? yt[k - 1] : xt[3], k ? yb[k - 1] : xb[3],
+                                    i + 4 + k < n ? low(i + 4 + k) : 0xC0000000u));
+                    __builtin_amdgcn_sched_barrier(0);
+                    pt = yt[3];
+                    pb = yb[3];
+                    cx += 8 * dstep;
                 }
             }
             // final keys E << 16 | r << 10 | d, r = (d - dmin) mod 32 (r < 32 and
