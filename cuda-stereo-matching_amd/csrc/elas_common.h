@@ -40,12 +40,13 @@ constexpr int elas_cand_stride(int dmax) { return (dmax + 3 + 15) & ~15; }
 #define ELAS_GAP_MAX_LINE 15872  // gap interpolation keeps a line in LDS (<= 62 KiB)
 
 // Lines per workgroup of the redundancy filters: a line of len cells takes
-// 3 bytes per cell of LDS (int16 value + flag byte); at most 4 lines (more
-// workgroups for the parallel phases), 48 KiB (plus alignment of the flags).
+// 3 bytes per cell of LDS (int16 value + flag byte).  One line: the
+// recurrence takes one wave per line whatever the count, and more
+// workgroups in flight beat sharing a workgroup's loads (measured faster
+// than two or four lines at 1242x375 and 2816x1880).
 constexpr int elas_redun_lines(int len) {
-    int l = 4;
-    while (l > 1 && l * len * 3 > 49152) l >>= 1;
-    return l;
+    (void)len;
+    return 1;
 }
 constexpr int elas_redun_lds(int len) {
     return ((2 * elas_redun_lines(len) * len + 15) & ~15) + elas_redun_lines(len) * ((len + 15) & ~15);
