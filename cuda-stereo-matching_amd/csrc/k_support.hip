@@ -49,22 +49,28 @@ __device__ __forceinline__ int support_pass(const ElasPair& P, const ElasKParams
         const int ncols = max(c_hi - c_lo, 0);
         const bool row_ok = v >= 5 && v <= H - 6;
         const uint8_t* I1 = P.desc[RIGHT ? 1 : 0] + (size_t)16 * W * v;
-        const uint8_t* I2 = P.desc[RIGHT ? 0 : 1] + (size_t)16 * W * v;
         if (row_ok) {
-            // every thread issues all of its loads before storing any
-            const uint4* top = reinterpret_cast<const uint4*>(I2 - 32 * (size_t)W) + c_lo;
-            const uint4* bot = reinterpret_cast<const uint4*>(I2 + 32 * (size_t)W) + c_lo;
-            for (int base = 0; base < 2 * ncols; base += 8 * 256) {
+            // every thread issues all of its loads (buffer loads: a lane past
+            // the row reads nothing, without a branch) before storing any;
+            // rows v-2 and v+2 side by side
+            const __amdgpu_buffer_rsrc_t rr = elas_rsrc((void*)P.desc[RIGHT ? 0 : 1], 16u * (uint32_t)W * (uint32_t)H);
+            const uint32_t top0 = 16u * ((uint32_t)(v - 2) * (uint32_t)W + (uint32_t)c_lo);
+            const uint32_t bot0 = top0 + 64u * (uint32_t)W;
+            for (int base = 0; base < ncols; base += 4 * 256) {
                 uint4 q[8];
                 #pragma unroll
-                for (int k = 0; k < 8; k++) {
+                for (int k = 0; k < 4; k++) {
                     const int i = base + tid + k * 256;
-                    if (i < 2 * ncols) q[k] = elas_ld(i < ncols ? top + i : bot + (i - ncols));
+                    q[k] = elas_ld16_ranged(rr, i < ncols ? top0 + 16u * (uint32_t)i : ELAS_OOR);
+                    q[4 + k] = elas_ld16_ranged(rr, i < ncols ? bot0 + 16u * (uint32_t)i : ELAS_OOR);
                 }
                 #pragma unroll
-                for (int k = 0; k < 8; k++) {
+                for (int k = 0; k < 4; k++) {
                     const int i = base + tid + k * 256;
-                    if (i < 2 * ncols) s_rows[i] = q[k];
+                    if (i < ncols) {
+                        s_rows[i] = q[k];
+                        s_rows[ncols + i] = q[4 + k];
+                    }
                 }
             }
         }
