@@ -61,12 +61,14 @@ namespace TRILIB {
 namespace {
 
 // triangulate()'s steps up to the divide-and-conquer (src/triangle.cpp:
-// 8503-8530 and delaunay(), :6237-6250) for n >= 3 points; byinput[i] is
-// input point i's vertex.  transfernodes() (:7638-7700) is taken apart for
-// points without attributes or markers: one tight pass takes the vertices
-// from the pool in input order and fills them with the same coordinates,
-// marks, types and bounding box (byinput needs no traversal).
-vertex* elas_begin(struct mesh& mesh, struct behavior& b, const float* pts, int n) {
+// 8503-8530 and delaunay(), :6237-6250) for n >= 3 points.  transfernodes()
+// (:7638-7700) is taken apart for points without attributes or markers: its
+// n poolalloc() calls take the first n items of the vertex pool's first
+// block, which holds them all (no item is dead yet), so they are taken here
+// at once; vertex i is the i-th item.  Filling them (coordinates, mark,
+// type) and the bounding box are left to elas_fill and the caller, so the
+// fill can run on the threads that use the vertices.
+vertex elas_begin(struct mesh& mesh, struct behavior& b, int n) {
     struct mesh* m = &mesh;  // Triangle's vertex macros name the mesh `m`
     char sw[] = "zQB";
     char* swp = sw;
@@ -79,29 +81,35 @@ vertex* elas_begin(struct mesh& mesh, struct behavior& b, const float* pts, int 
     m->readnodefile = 0;
     b.weighted = 0;
     initializevertexpool(m, &b);
-    vertex* byinput = (vertex*)trimalloc(n * (int)sizeof(vertex));
-    float xmin = pts[0], xmax = pts[0], ymin = pts[1], ymax = pts[1];
-    for (int i = 0; i < n; i++) {
-        vertex v = (vertex)poolalloc(&m->vertices);
-        byinput[i] = v;
-        const float x = pts[2 * i], y = pts[2 * i + 1];
-        v[0] = x;
-        v[1] = y;
-        setvertexmark(v, 0);
-        setvertextype(v, INPUTVERTEX);
-        xmin = (x < xmin) ? x : xmin;
-        xmax = (x > xmax) ? x : xmax;
-        ymin = (y < ymin) ? y : ymin;
-        ymax = (y > ymax) ? y : ymax;
-    }
+    struct memorypool* vp = &m->vertices;
+    const vertex base = (vertex)vp->nextitem;
+    vp->nextitem = (int*)((char*)vp->nextitem + (size_t)n * (size_t)vp->itembytes);
+    vp->unallocateditems -= n;
+    vp->maxitems += n;
+    vp->items += n;
+    m->eextras = 0;
+    initializetrisubpools(m, &b);
+    return base;
+}
+
+// input point i's vertex, as transfernodes() fills it
+inline vertex elas_vertex(struct mesh* m, vertex base, int i) {
+    return (vertex)((char*)base + (size_t)i * (size_t)m->vertices.itembytes);
+}
+inline void elas_fill(struct mesh* m, vertex v, const float* p) {
+    v[0] = p[0];
+    v[1] = p[1];
+    setvertexmark(v, 0);
+    setvertextype(v, INPUTVERTEX);
+}
+
+// transfernodes()'s bounding box and sweepline flag (:7684-7696)
+void elas_bounds(struct mesh* m, float xmin, float xmax, float ymin, float ymax) {
     m->xmin = xmin;
     m->xmax = xmax;
     m->ymin = ymin;
     m->ymax = ymax;
     m->xminextreme = 10 * m->xmin - 9 * m->xmax;
-    m->eextras = 0;
-    initializetrisubpools(m, &b);
-    return byinput;
 }
 
 struct SplitNode {
@@ -110,19 +118,30 @@ struct SplitNode {
     int parent;       // -1 for the root
     struct mesh sub;  // the node's own triangle pool (the root uses the caller's mesh)
     struct otri farleft, farright;
-    std::vector<int32_t> tris;  // its live triangles, read out of its pool
-    size_t off;                 // their place in the element list (ints)
 };
+
+// A run of a node's triangle pool for the readout: items [first, first +
+// count) of the pool's first block (when it holds them all), or the whole
+// pool (first -1).
+struct ReadChunk {
+    int node, first, count;
+    int live;    // its live triangles
+    size_t off;  // their place in the element list (ints)
+};
+#define ELAS_TRI_READ_CHUNK 2048
 
 struct SplitRun {
     struct mesh* m;
     struct behavior* b;
-    vertex* vert;       // vertices by the order's point index
+    vertex* vert;       // vertices by the order's point index, or (null) input point i's vertex
+    vertex base;        //   is item i of the pool, filled by the leaf that holds it from
+    const float* pts;   //   the input points
     vertex* sortarray;  // the divide-and-conquer order (filled by the leaves)
     ElasTriOrder* order;
     std::vector<SplitNode> nodes;
     int root;
     std::vector<int> todo;  // the leaves
+    std::vector<ReadChunk> chunks;  // the readout, in element-list order
     std::unique_ptr<std::atomic<int>[]> arrived;  // per node: children done
     int32_t* dst;
     bool dst16;  // dst takes 16-bit corner indices
@@ -147,7 +166,6 @@ int split_build(std::vector<SplitNode>& nodes, int lo, int size, int axis, int l
     nd.axis = axis;
     nd.level = level;
     nd.left = nd.right = nd.parent = -1;
-    nd.off = 0;
     if (depth > 0 && size > 3) {
         const int divider = size >> 1;
         nd.left = split_build(nodes, lo, divider, 1 - axis, level + 1, depth - 1);
@@ -194,7 +212,16 @@ void split_leaf(void* arg, int t) {
     SplitNode& nd = r->nodes[i];
     r->order->cut(nd.lo, nd.size, nd.axis, -1);
     const int32_t* arr = r->order->arr.data();
-    for (int k = nd.lo; k < nd.lo + nd.size; k++) r->sortarray[k] = r->vert[arr[k]];
+    if (r->vert) {
+        for (int k = nd.lo; k < nd.lo + nd.size; k++) r->sortarray[k] = r->vert[arr[k]];
+    } else {
+        for (int k = nd.lo; k < nd.lo + nd.size; k++) {
+            const int i = arr[k];
+            const vertex v = elas_vertex(r->m, r->base, i);
+            elas_fill(r->m, v, r->pts + 2 * (size_t)i);
+            r->sortarray[k] = v;
+        }
+    }
     node_pool(r, i);
     divconqrecurse(node_mesh(r, i), r->b, r->sortarray + nd.lo, nd.size, nd.axis, &nd.farleft, &nd.farright);
     for (int j = nd.parent; j >= 0; j = r->nodes[j].parent) {
@@ -203,60 +230,75 @@ void split_leaf(void* arg, int t) {
     }
 }
 
-// a node's live triangles in allocation order, corners as writeelements()
-// numbers them (src/triangle.cpp:7804-7860, order 1)
-void split_read(void* arg, int t) {
-    SplitRun* r = (SplitRun*)arg;
-    struct mesh* m = r->m;  // Triangle's vertex macros name the mesh `m`
-    SplitNode& nd = r->nodes[t];
-    struct memorypool* pool = &node_mesh(r, t)->triangles;
-    nd.tris.clear();
-    nd.tris.reserve(6 * (size_t)nd.size + 6);
-    traversalinit(pool);
-    struct otri tri;
-    tri.orient = 0;
-    vertex p1, p2, p3;
-    while ((tri.tri = (triangle*)traverse(pool)) != (triangle*)NULL) {
-        if (deadtri(tri.tri)) continue;
-        org(tri, p1);
-        dest(tri, p2);
-        apex(tri, p3);
-        if (r->vshift >= 0) {
-            nd.tris.push_back((int32_t)(((const char*)p1 - r->vbase) >> r->vshift));
-            nd.tris.push_back((int32_t)(((const char*)p2 - r->vbase) >> r->vshift));
-            nd.tris.push_back((int32_t)(((const char*)p3 - r->vbase) >> r->vshift));
-        } else {
-            nd.tris.push_back(vertexmark(p1));
-            nd.tris.push_back(vertexmark(p2));
-            nd.tris.push_back(vertexmark(p3));
-        }
+// A chunk's triangles in allocation order.  A pool whose items all lie in
+// its first block (the usual case) is read in chunks of item indices, each
+// visited as traverse() visits it: from the first block's first aligned
+// item (traversalinit(), :1786-1795) in steps of the item size.
+template <class F>
+void chunk_walk(SplitRun* r, const ReadChunk& ch, F&& f) {
+    struct memorypool* pool = &node_mesh(r, ch.node)->triangles;
+    if (ch.first >= 0) {
+        const unsigned long long alignptr = (unsigned long long)(pool->firstblock + 1);
+        char* item0 = (char*)(alignptr + (unsigned long long)pool->alignbytes - (alignptr % (unsigned long long)pool->alignbytes));
+        for (int j = ch.first; j < ch.first + ch.count; j++) f((triangle*)(item0 + (size_t)j * (size_t)pool->itembytes));
+    } else {
+        traversalinit(pool);
+        triangle* t;
+        while ((t = (triangle*)traverse(pool)) != (triangle*)NULL) f(t);
     }
-    if (t != r->root) pooldeinit(pool);
 }
 
-void split_copy(void* arg, int t) {
+// the live triangles of a chunk
+void split_count(void* arg, int t) {
     SplitRun* r = (SplitRun*)arg;
-    const SplitNode& nd = r->nodes[t];
-    if (nd.tris.empty()) return;
-    if (r->dst16) {
-        uint16_t* d = (uint16_t*)r->dst + nd.off;
-        for (size_t k = 0; k < nd.tris.size(); k++) d[k] = (uint16_t)nd.tris[k];
-    } else {
-        memcpy(r->dst + nd.off, nd.tris.data(), 4 * nd.tris.size());
-    }
+    ReadChunk& ch = r->chunks[t];
+    int live = 0;
+    chunk_walk(r, ch, [&](triangle* tp) { live += !deadtri(tp); });
+    ch.live = live;
+}
+
+// a chunk's live triangles into the element list at its offset, corners as
+// writeelements() numbers them (src/triangle.cpp:7804-7860, order 1)
+void split_write(void* arg, int t) {
+    SplitRun* r = (SplitRun*)arg;
+    struct mesh* m = r->m;  // Triangle's vertex macros name the mesh `m`
+    const ReadChunk& ch = r->chunks[t];
+    int32_t* d32 = r->dst + ch.off;
+    uint16_t* d16 = (uint16_t*)r->dst + ch.off;
+    size_t k = 0;
+    struct otri tri;
+    tri.orient = 0;
+    vertex p[3];
+    chunk_walk(r, ch, [&](triangle* tp) {
+        if (deadtri(tp)) return;
+        tri.tri = tp;
+        org(tri, p[0]);
+        dest(tri, p[1]);
+        apex(tri, p[2]);
+        for (int c = 0; c < 3; c++, k++) {
+            const int32_t id = r->vshift >= 0 ? (int32_t)(((const char*)p[c] - r->vbase) >> r->vshift) : vertexmark(p[c]);
+            if (r->dst16)
+                d16[k] = (uint16_t)id;
+            else
+                d32[k] = id;
+        }
+    });
 }
 
 // The merge over `vert` (count vertices; order->lx / ly hold their sort
-// orders) cut into at most 2^depth subtrees, the element list into *out, or
-// into dst when it holds cap triangles (as 16-bit indices with dst16);
-// returns ntri.
-int elas_finish_split(struct mesh& mesh, struct behavior& b, vertex* vert, int count, ElasTriOrder* order,
-                      int depth, ElasParallelFor par, void* parctx, std::vector<int32_t>* out, int32_t* dst,
-                      int64_t cap, bool dst16, bool topcut_done) {
+// orders; with vert null, the unfilled input vertices from base, filled from
+// pts by the leaves) cut into at most 2^depth subtrees, the element list
+// into *out, or into dst when it holds cap triangles (as 16-bit indices with
+// dst16); returns ntri.
+int elas_finish_split(struct mesh& mesh, struct behavior& b, vertex* vert, vertex base, const float* pts, int count,
+                      ElasTriOrder* order, int depth, ElasParallelFor par, void* parctx, std::vector<int32_t>* out,
+                      int32_t* dst, int64_t cap, bool dst16, bool topcut_done) {
     SplitRun r;
     r.m = &mesh;
     r.b = &b;
     r.vert = vert;
+    r.base = base;
+    r.pts = pts;
     r.order = order;
     r.sortarray = (vertex*)trimalloc(count * (int)sizeof(vertex));
     r.nodes.reserve(2u << std::min(depth, 12));
@@ -283,24 +325,39 @@ int elas_finish_split(struct mesh& mesh, struct behavior& b, vertex* vert, int c
         for (r.vshift = 0; (1 << r.vshift) < ib;) r.vshift++;
     else
         numbernodes(&mesh, &b);
-    par(parctx, (int)r.nodes.size(), split_read, &r);
+    // the pools in post-order (the nodes' order), each in allocation order
+    r.chunks.clear();
+    for (int i = 0; i < (int)r.nodes.size(); i++) {
+        const struct memorypool* pool = &node_mesh(&r, i)->triangles;
+        if (pool->maxitems <= pool->itemsfirstblock) {
+            const int items = (int)pool->maxitems;
+            for (int f = 0; f < items; f += ELAS_TRI_READ_CHUNK)
+                r.chunks.push_back(ReadChunk{i, f, std::min(ELAS_TRI_READ_CHUNK, items - f), 0, 0});
+        } else {
+            r.chunks.push_back(ReadChunk{i, -1, 0, 0, 0});
+        }
+    }
+    // count, place, then write each chunk's triangles where they belong
+    par(parctx, (int)r.chunks.size(), split_count, &r);
     size_t total = 0;
-    for (SplitNode& nd : r.nodes) {
-        nd.off = total;
-        total += nd.tris.size();
+    for (ReadChunk& ch : r.chunks) {
+        ch.off = total;
+        total += 3 * (size_t)ch.live;
     }
     const int nt = (int)(total / 3);
     if (dst) {
         if (nt <= cap) {
             r.dst = dst;
             r.dst16 = dst16;
-            par(parctx, (int)r.nodes.size(), split_copy, &r);
+            par(parctx, (int)r.chunks.size(), split_write, &r);
         }
     } else {
         out->resize(total);
         r.dst = out->data();
-        par(parctx, (int)r.nodes.size(), split_copy, &r);
+        par(parctx, (int)r.chunks.size(), split_write, &r);
     }
+    for (int i = 0; i < (int)r.nodes.size(); i++)
+        if (i != r.root) pooldeinit(&r.nodes[i].sub.triangles);
     triangledeinit(&mesh, &b);
     return nt;
 }
@@ -312,38 +369,58 @@ namespace {
 struct Setup {
     struct mesh mesh;
     struct behavior b;
-    vertex* byinput;
+    vertex base;  // input point i's vertex is item i of the vertex pool
+    const float* pts;
     int n;
 };
 
 }  // namespace
 
 // Triangle's setup of a "zQB" triangulation of n >= 3 points [x0, y0, ...]
-// (any thread); elas_tri_finish() takes it over.
+// (any thread; the vertices are taken, not yet filled); elas_tri_finish()
+// takes it over.
 void* elas_tri_setup(const float* pts, int n) {
     Setup* st = new Setup();
-    st->byinput = elas_begin(st->mesh, st->b, pts, n);
+    st->base = elas_begin(st->mesh, st->b, n);
+    st->pts = pts;
     st->n = n;
     return st;
 }
 
 // The triangulation, corners numbered by input position, into *out (or dst,
 // see elas_finish_split).  `order` is scratch; with `integer` set it already
-// holds the input points' sort orders (elas_tri_order_integer) and its top
-// `depth` levels of cuts, else
-// Triangle's vertexsort() and duplicate removal (:6176-6198) run first and
-// the order is built over the survivors.  Returns ntri, or -1 when a single
-// distinct point is left: Triangle's divconqrecurse() has no case for one
-// vertex and recurses until the stack overflows (two distinct points give an
-// empty mesh, as in Triangle).
+// holds the input points' sort orders and bounds (elas_tri_order_integer)
+// and its top `depth` levels of cuts, and the leaves fill the vertices, else
+// they are filled here, then Triangle's vertexsort() and duplicate removal
+// (:6176-6198) run and the order is built over the survivors.  Returns ntri,
+// or -1 when a single distinct point is left: Triangle's divconqrecurse() has
+// no case for one vertex and recurses until the stack overflows (two
+// distinct points give an empty mesh, as in Triangle).
 int elas_tri_finish(void* setup, ElasTriOrder* order, bool integer, int depth, ElasParallelFor par, void* parctx,
                     std::vector<int32_t>* out, int32_t* dst, int64_t cap, bool dst16) {
     Setup* st = (Setup*)setup;
     struct mesh* m = &st->mesh;  // Triangle's vertex macros name the mesh `m`
-    vertex* byinput = st->byinput;
     const int n = st->n;
+    const float* pts = st->pts;
     int count = n, nt = -1;
-    if (!integer) {
+    vertex* byinput = nullptr;
+    if (integer) {
+        // integer coordinates below 2^24: the bounds are exact as floats
+        elas_bounds(m, (float)order->lohi[0], (float)order->lohi[1], (float)order->lohi[2], (float)order->lohi[3]);
+    } else {
+        byinput = (vertex*)trimalloc(n * (int)sizeof(vertex));
+        float xmin = pts[0], xmax = pts[0], ymin = pts[1], ymax = pts[1];
+        for (int i = 0; i < n; i++) {
+            const vertex v = elas_vertex(m, st->base, i);
+            elas_fill(m, v, pts + 2 * (size_t)i);
+            byinput[i] = v;
+            const float x = pts[2 * i], y = pts[2 * i + 1];
+            xmin = (x < xmin) ? x : xmin;
+            xmax = (x > xmax) ? x : xmax;
+            ymin = (y < ymin) ? y : ymin;
+            ymax = (y > ymax) ? y : ymax;
+        }
+        elas_bounds(m, xmin, xmax, ymin, ymax);
         vertexsort(byinput, n);
         int i = 0;
         for (int j = 1; j < n; j++) {
@@ -366,13 +443,13 @@ int elas_tri_finish(void* setup, ElasTriOrder* order, bool integer, int depth, E
         }
     }
     if (count >= 2) {
-        nt = elas_finish_split(st->mesh, st->b, byinput, count, order, depth, par, parctx, out, dst, cap, dst16,
-                               integer);
+        nt = elas_finish_split(st->mesh, st->b, byinput, st->base, pts, count, order, depth, par, parctx, out, dst,
+                               cap, dst16, integer);
     } else {
         triangledeinit(&st->mesh, &st->b);
         if (out) out->clear();
     }
-    trifree((int*)byinput);
+    if (byinput) trifree((int*)byinput);
     delete st;
     return nt;
 }

@@ -237,7 +237,7 @@ void ElasTriOrder::pack() {
 
 void ElasTriOrder::cut(int lo, int size, int axis, int levels) {
     if (size <= 3) {
-        for (int k = 0; k < size; k++) arr[lo + k] = lx[px[lo + k] >> 32];
+        for (int k = 0; k < size; k++) arr[lo + k] = ly[(uint32_t)px[lo + k]];
         return;
     }
     if (levels == 0) return;
@@ -333,6 +333,29 @@ static __attribute__((target("avx2"))) int convert_keys(const float* p, int n, i
     return bad;
 }
 
+// Whether keys come in y order (bit 0) and in strictly increasing (y, x)
+// order (bit 1); eight neighbour pairs per step with AVX2.
+static __attribute__((target("avx2"))) int input_order(const int32_t* kx, const int32_t* ky, int n) {
+    int i = 1;
+    __m256i ydown = _mm256_setzero_si256(), notyx = _mm256_setzero_si256();
+    for (; i + 8 <= n; i += 8) {
+        const __m256i y1 = _mm256_loadu_si256((const __m256i*)(ky + i)), y0 = _mm256_loadu_si256((const __m256i*)(ky + i - 1));
+        const __m256i x1 = _mm256_loadu_si256((const __m256i*)(kx + i)), x0 = _mm256_loadu_si256((const __m256i*)(kx + i - 1));
+        const __m256i down = _mm256_cmpgt_epi32(y0, y1);  // y falls
+        const __m256i same = _mm256_cmpeq_epi32(y0, y1);
+        const __m256i xnotup = _mm256_xor_si256(_mm256_cmpgt_epi32(x1, x0), _mm256_set1_epi32(-1));
+        ydown = _mm256_or_si256(ydown, down);
+        notyx = _mm256_or_si256(notyx, _mm256_or_si256(down, _mm256_and_si256(same, xnotup)));
+    }
+    bool ys = _mm256_testz_si256(ydown, ydown), yxs = _mm256_testz_si256(notyx, notyx);
+    for (; i < n; i++) {
+        const int dy = ky[i] - ky[i - 1];
+        ys = ys && dy >= 0;
+        yxs = yxs && ((dy > 0) || (dy == 0 && kx[i] > kx[i - 1]));
+    }
+    return (ys ? 1 : 0) | (yxs ? 2 : 0);
+}
+
 // Support points have integer coordinates, so both sort orders come from
 // counting sorts, in O(n) instead of Triangle's randomised quicksort and
 // quickselect, which take about half of a support-point mesh's serial time.
@@ -356,13 +379,9 @@ bool elas_tri_order_integer(const float* p, int n, ElasTriOrder& o) {
     int bad = convert_keys(p, n, kx, ky, lohi);
     const int lox = lohi[0], hix = lohi[1], loy = lohi[2], hiy = lohi[3];
     if (bad) return false;
-    int ys = 1, yxs = 1;
-    for (int i = 1; i < n; i++) {
-        const int dy = ky[i] - ky[i - 1];
-        ys &= dy >= 0;
-        yxs &= (dy > 0) | ((dy == 0) & (kx[i] > kx[i - 1]));
-    }
-    const bool ysorted = ys, yxsorted = yxs;
+    for (int k = 0; k < 4; k++) o.lohi[k] = lohi[k];
+    int sorted = input_order(kx, ky, n);
+    const bool ysorted = sorted & 1, yxsorted = sorted & 2;
     const long rx = (long)hix - lox + 1, ry = (long)hiy - loy + 1;
     if (rx > 4L * n + 65536 || ry > 4L * n + 65536) return false;
     o.resize(n);
@@ -408,8 +427,8 @@ bool elas_tri_order_integer(const float* p, int n, ElasTriOrder& o) {
         for (int k = 1; k < n; k++)
             if (kx[ly[k]] == kx[ly[k - 1]] && ky[ly[k]] == ky[ly[k - 1]]) return false;
     }
-    // (x, y) order, ranks and entries: ly stably by x
-    int32_t* lx = o.lx.data();
+    // (x, y) order, ranks and entries: ly stably by x (a point's place in
+    // that order is its x rank; cut() maps entries to points through ly)
     uint64_t* px = o.px.data();
     uint64_t* py = o.py.data();
     cnt.assign((size_t)rx + 1, 0);
@@ -420,7 +439,6 @@ bool elas_tri_order_integer(const float* p, int n, ElasTriOrder& o) {
         const int32_t i = ly[k];
         const int32_t pos = c[kx[i] - lox]++;
         const uint64_t e = (uint64_t)(uint32_t)pos << 32 | (uint32_t)k;
-        lx[pos] = i;
         px[pos] = e;
         py[k] = e;
     }
@@ -480,24 +498,38 @@ struct TriCall {
     void* setup;
     bool integer;
     int depth;
+    void* pool;
 };
 
-// Triangle's setup on one thread; on the other the vertex order and, when
-// it was built here, the top `depth` levels of its cuts
+// the cuts below the root down to the split depth, one half of the order
+// per task
+void tri_half_cut(void* arg, int t) {
+    TriCall* c = (TriCall*)arg;
+    const int n = c->job->n, h = n >> 1;
+    c->order->cut(t ? h : 0, t ? n - h : h, 1, c->depth - 1);
+}
+
+// The vertex order and, when it was built here, the top `depth` levels of
+// its cuts: the root's here, its halves' on two threads (task 0, which the
+// calling thread takes first: the longer one); Triangle's setup on another
+// thread
 void tri_prepare(void* arg, int t) {
     TriCall* c = (TriCall*)arg;
-    if (t == 0) {
+    if (t == 1) {
         c->setup = kSetup[c->k](c->job->pts, c->job->n);
     } else {
         c->integer = elas_tri_order_integer(c->job->pts, c->job->n, *c->order);
-        if (c->integer) c->order->cut(0, c->job->n, 0, c->depth);
+        if (c->integer && c->depth > 0) {
+            c->order->cut(0, c->job->n, 0, 1);
+            if (c->depth > 1 && c->job->n > 3) pool_parallel_for(c->pool, 2, tri_half_cut, c);
+        }
     }
 }
 
 int tri_call(int k, ElasTriJob& j, bool split, void* pool) {
     thread_local ElasTriOrder order;
     const int depth = split ? job_depth(j.n, pool) : 0;
-    TriCall c{k, &j, &order, nullptr, false, depth};
+    TriCall c{k, &j, &order, nullptr, false, depth, pool};
     if (depth > 0) {
         pool_parallel_for(pool, 2, tri_prepare, &c);
     } else {
@@ -626,23 +658,27 @@ public:
         while (ts.done.load(std::memory_order_acquire) < ntasks) _mm_pause();
     }
     // Runs one task of a published set on this thread; false if there is none.
+    // The slot's reader count covers only the claim: once a task is claimed,
+    // its set lives until its publisher has counted it done, and a task may
+    // itself publish a set (which may land in this very slot, whose readers
+    // its publisher waits out).
     bool help_one() {
         if (helping_.load() == 0) return false;
         for (int k = 0; k < ELAS_TRI_SLOTS; k++) {
             if (!slots_[k].load(std::memory_order_relaxed)) continue;
             readers_[k].fetch_add(1);
             TaskSet* ts = slots_[k].load();
-            bool ran = false;
+            int i = -1;
             if (ts) {
-                const int i = ts->next.fetch_add(1);
-                if (i < ts->n) {
-                    ts->fn(ts->arg, i);
-                    ts->done.fetch_add(1, std::memory_order_release);
-                    ran = true;
-                }
+                i = ts->next.fetch_add(1);
+                if (i >= ts->n) i = -1;
             }
             readers_[k].fetch_sub(1);
-            if (ran) return true;
+            if (i >= 0) {
+                ts->fn(ts->arg, i);
+                ts->done.fetch_add(1, std::memory_order_release);
+                return true;
+            }
         }
         return false;
     }

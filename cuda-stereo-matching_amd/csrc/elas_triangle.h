@@ -24,8 +24,8 @@ struct ElasTriJob {
 
 // Triangle's divide-and-conquer vertex order (divconqdelaunay(),
 // src/triangle.cpp:6164-6230): the points sorted by (x, y), halved, and each
-// half cut alternately along y and x down to 2- and 3-point leaves.  lx and ly
-// hold the points in (x, y) and (y, x) order; cut() works on a node
+// half cut alternately along y and x down to 2- and 3-point leaves.  ly holds
+// the points in (y, x) order (and lx, for pack(), in (x, y) order); cut() works on a node
 // [lo, lo + size): a cut along `axis` keeps the size / 2 smallest points of
 // that order on the left (as vertexmedian() does, src/triangle.cpp:
 // 5517-5574) and splits the other list stably, so both children keep both
@@ -34,12 +34,14 @@ struct ElasTriJob {
 // (-1: down to the leaves): the top of the tree can be cut on one thread and
 // each subtree below it on another, since subtrees touch disjoint ranges.
 // The cuts work on px and py, the two orders as entries (x rank << 32 |
-// y rank), where a point's x rank is its place in lx and its y rank its
-// place in ly: a point belongs to the left of a cut along x exactly when its
-// x rank is at most that of the node's `divider`-th point in x order, so a
-// cut reads its entries in sequence and looks nothing up.  pack() builds px
-// and py from lx and ly.
+// y rank), where a point's x rank is its place in (x, y) order and its y
+// rank its place in ly: a point belongs to the left of a cut along x exactly
+// when its x rank is at most that of the node's `divider`-th point in x
+// order, so a cut reads its entries in sequence and looks nothing up; a leaf
+// maps its entries to points through ly.  pack() builds px and py from lx
+// and ly.
 struct ElasTriOrder {
+    int lohi[4];  // integer orders: min x, max x, min y, max y
     std::vector<int32_t> lx, ly, arr, rank;
     std::vector<uint64_t> px, py, tl, tr;
     void resize(int n);
