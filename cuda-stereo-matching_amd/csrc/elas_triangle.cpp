@@ -573,11 +573,21 @@ public:
     // remaining: jobs not finished, under m; left: the same count, lowered by
     // a worker as its last access of the batch (after releasing m), so a
     // waiter frees the batch only once left is 0
+    struct Batch;
+    struct Item {
+        ElasTriJob* job;
+        Batch* batch;
+    };
     struct Batch {
         std::mutex m;
         std::condition_variable cv;
         int remaining = 0;
         std::atomic<int> left{0};
+        Item* sticky[2] = {nullptr, nullptr};  // its jobs posted to a worker's box
+        ~Batch() {
+            delete sticky[0];
+            delete sticky[1];
+        }
     };
     // tasks of one step of a split merge, claimed through `next`; `done`
     // counts finished tasks
@@ -612,17 +622,37 @@ public:
         // after it released m: b (on this stack) outlives it
         while (b.left.load(std::memory_order_acquire) > 0) _mm_pause();
     }
+    // A batch of one or two jobs (the two meshes of one pair) goes to
+    // workers 1 and 2 when their boxes are free, so each mesh of a run of
+    // single-pair calls finds its thread's buffers where it left them; other
+    // jobs go to the shared queue.
     void enqueue(ElasTriJob* jobs, int n, Batch* b) {
         b->remaining = n;
         b->left.store(n, std::memory_order_relaxed);
+        bool boxed = false;
         {
             std::lock_guard<std::mutex> g(m_);
             ensure_workers();
-            for (int i = 0; i < n; i++) q_.push_back({&jobs[i], b});
+            int queued = 0;
+            for (int i = 0; i < n; i++) {
+                if (n <= 2 && (int)threads_.size() >= 2) {
+                    b->sticky[i] = new Item{&jobs[i], b};
+                    Item* expect = nullptr;
+                    if (box_[i + 1].compare_exchange_strong(expect, b->sticky[i])) {
+                        boxed = true;
+                        continue;
+                    }
+                    delete b->sticky[i];
+                    b->sticky[i] = nullptr;
+                }
+                q_.push_back({&jobs[i], b});
+                queued++;
+            }
             active_.fetch_add(n);
-            pending_.fetch_add(n, std::memory_order_release);
+            pending_.fetch_add(queued, std::memory_order_release);
         }
-        for (int i = 0; i < n; i++) cv_.notify_one();
+        if (boxed) cv_.notify_all();
+        else for (int i = 0; i < n; i++) cv_.notify_one();
     }
     // Runs the set's tasks here and on idle helpers; returns when all are done.
     void parallel_for(int ntasks, void (*fn)(void*, int), void* arg) {
@@ -684,10 +714,6 @@ public:
     }
 
 private:
-    struct Item {
-        ElasTriJob* job;
-        Batch* batch;
-    };
     // One worker per CPU this process may run on, less one for the caller
     // (at most ELAS_TRI_INSTANCES - 1: copy 0 is the caller's): a worker
     // that waits for a CPU while holding a claimed subtree stalls the whole
@@ -708,7 +734,9 @@ private:
                 continue;
             }
             Item it;
-            {
+            if (Item* bx = box_[k].exchange(nullptr, std::memory_order_acquire)) {
+                it = *bx;  // the Item itself is freed with its batch
+            } else {
                 if (pending_.load(std::memory_order_acquire) == 0) {
                     const double idle = now_s() - last;
                     if (idle < ELAS_TRI_SPIN_S) {
@@ -719,7 +747,7 @@ private:
                         continue;
                     }
                     std::unique_lock<std::mutex> g(m_);
-                    cv_.wait(g, [&] { return !q_.empty() || helping_.load() != 0; });
+                    cv_.wait(g, [&] { return !q_.empty() || helping_.load() != 0 || box_[k].load() != nullptr; });
                     last = now_s();  // woken: poll for a while before sleeping again
                     continue;
                 }
@@ -751,6 +779,7 @@ private:
     std::atomic<int> active_{0};   // jobs queued or running
     std::atomic<int> helping_{0};  // published task sets
     std::atomic<TaskSet*> slots_[ELAS_TRI_SLOTS] = {};
+    std::atomic<Item*> box_[ELAS_TRI_INSTANCES] = {};  // a job for worker k
     std::atomic<int> readers_[ELAS_TRI_SLOTS] = {};
     std::mutex m_, serial_;
     std::condition_variable cv_;
