@@ -216,15 +216,23 @@ void elas_tri_block_free(void* ptr) {
     free(p);
 }
 
-void ElasTriOrder::resize(int n) {
+void ElasTriOrder::resize(int n, bool narrow_entries) {
+    narrow = narrow_entries;
     lx.resize(n);
     ly.resize(n);
     arr.resize(n);
     rank.resize(n);
-    px.resize(n);
-    py.resize(n);
-    tl.resize(n);
-    tr.resize(n);
+    if (narrow) {
+        qx.resize(n);
+        qy.resize(n);
+        ql.resize(n);
+        qr.resize(n);
+    } else {
+        px.resize(n);
+        py.resize(n);
+        tl.resize(n);
+        tr.resize(n);
+    }
 }
 
 void ElasTriOrder::pack() {
@@ -235,38 +243,56 @@ void ElasTriOrder::pack() {
     for (int k = 0; k < n; k++) py[k] = (uint64_t)(uint32_t)rank[ly[k]] << 32 | (uint32_t)k;
 }
 
-void ElasTriOrder::cut(int lo, int size, int axis, int levels) {
+namespace {
+
+// cut() over entries of type E: (x rank << H | y rank), H = half E's bits
+template <typename E>
+void cut_entries(E* px, E* py, E* tl, E* tr, const int32_t* ly, int32_t* arr, int lo, int size, int axis,
+                 int levels) {
+    constexpr int H = (int)sizeof(E) * 4;
+    constexpr E LOW = (E)(((E)1 << H) - 1);
     if (size <= 3) {
-        for (int k = 0; k < size; k++) arr[lo + k] = ly[(uint32_t)px[lo + k]];
+        for (int k = 0; k < size; k++) arr[lo + k] = ly[px[lo + k] & LOW];
         return;
     }
     if (levels == 0) return;
     const int divider = size >> 1;
-    const uint64_t* cutl = axis ? py.data() : px.data();
-    uint64_t* other = axis ? px.data() : py.data();
-    const int sh = axis ? 0 : 32;
-    const uint32_t last = (uint32_t)(cutl[lo + divider - 1] >> sh);  // rank of the left's last point
+    const E* cutl = axis ? py : px;
+    E* other = axis ? px : py;
+    const int sh = axis ? 0 : H;
+    const E last = (E)((cutl[lo + divider - 1] >> sh) & LOW);  // rank of the left's last point
     // stable split of the other list without branches (the side of a point
     // is not predictable): every entry is stored to both sides and only its
     // own side's cursor moves on.  The store past a side's end lands inside
     // the node's range of tl / tr: a < divider and b < size - divider until
     // the last entry.
-    uint64_t* l = tl.data() + lo;
-    uint64_t* r = tr.data() + lo;
-    const uint64_t* o = other + lo;
+    E* l = tl + lo;
+    E* r = tr + lo;
+    const E* o = other + lo;
     int a = 0, b = 0;
     for (int k = 0; k < size; k++) {
-        const uint64_t e = o[k];
-        const int right = (uint32_t)(e >> sh) > last;
+        const E e = o[k];
+        const int right = (E)((e >> sh) & LOW) > last;
         l[a] = e;
         r[b] = e;
         a += 1 - right;
         b += right;
     }
-    memcpy(other + lo, l, sizeof(uint64_t) * (size_t)divider);
-    memcpy(other + lo + divider, r, sizeof(uint64_t) * (size_t)(size - divider));
-    cut(lo, divider, 1 - axis, levels - 1);
-    cut(lo + divider, size - divider, 1 - axis, levels - 1);
+    memcpy(other + lo, l, sizeof(E) * (size_t)divider);
+    memcpy(other + lo + divider, r, sizeof(E) * (size_t)(size - divider));
+    cut_entries(px, py, tl, tr, ly, arr, lo, divider, 1 - axis, levels - 1);
+    cut_entries(px, py, tl, tr, ly, arr, lo + divider, size - divider, 1 - axis, levels - 1);
+}
+
+}  // namespace
+
+void ElasTriOrder::cut(int lo, int size, int axis, int levels) {
+    if (narrow)
+        cut_entries<uint32_t>(qx.data(), qy.data(), ql.data(), qr.data(), ly.data(), arr.data(), lo, size, axis,
+                              levels);
+    else
+        cut_entries<uint64_t>(px.data(), py.data(), tl.data(), tr.data(), ly.data(), arr.data(), lo, size, axis,
+                              levels);
 }
 
 // The integer keys of n points [x0, y0, ...]: kx, ky, their bounds in
@@ -384,7 +410,7 @@ bool elas_tri_order_integer(const float* p, int n, ElasTriOrder& o) {
     const bool ysorted = sorted & 1, yxsorted = sorted & 2;
     const long rx = (long)hix - lox + 1, ry = (long)hiy - loy + 1;
     if (rx > 4L * n + 65536 || ry > 4L * n + 65536) return false;
-    o.resize(n);
+    o.resize(n, n <= 65536);
     int32_t* ly = o.ly.data();
     for (int i = 0; i < n; i++) ly[i] = i;
     bool done = yxsorted;  // strictly increasing (y, x): no equal points either
@@ -429,24 +455,36 @@ bool elas_tri_order_integer(const float* p, int n, ElasTriOrder& o) {
     }
     // (x, y) order, ranks and entries: ly stably by x (a point's place in
     // that order is its x rank; cut() maps entries to points through ly)
-    uint64_t* px = o.px.data();
-    uint64_t* py = o.py.data();
     cnt.assign((size_t)rx + 1, 0);
     for (int i = 0; i < n; i++) cnt[kx[i] - lox + 1]++;
     for (long b = 0; b < rx; b++) cnt[b + 1] += cnt[b];
     int32_t* c = cnt.data();
-    for (int k = 0; k < n; k++) {
-        const int32_t i = ly[k];
-        const int32_t pos = c[kx[i] - lox]++;
-        const uint64_t e = (uint64_t)(uint32_t)pos << 32 | (uint32_t)k;
-        px[pos] = e;
-        py[k] = e;
+    if (o.narrow) {
+        uint32_t* px = o.qx.data();
+        uint32_t* py = o.qy.data();
+        for (int k = 0; k < n; k++) {
+            const int32_t i = ly[k];
+            const int32_t pos = c[kx[i] - lox]++;
+            const uint32_t e = (uint32_t)pos << 16 | (uint32_t)k;
+            px[pos] = e;
+            py[k] = e;
+        }
+    } else {
+        uint64_t* px = o.px.data();
+        uint64_t* py = o.py.data();
+        for (int k = 0; k < n; k++) {
+            const int32_t i = ly[k];
+            const int32_t pos = c[kx[i] - lox]++;
+            const uint64_t e = (uint64_t)(uint32_t)pos << 32 | (uint32_t)k;
+            px[pos] = e;
+            py[k] = e;
+        }
     }
     return true;
 }
 
 void elas_tri_order_sorted(const float* xy, int count, ElasTriOrder& o) {
-    o.resize(count);
+    o.resize(count, false);
     for (int i = 0; i < count; i++) o.lx[i] = o.ly[i] = i;
     // (y, x) order: by y, the (x, y) order kept among equal y
     std::stable_sort(o.ly.begin(), o.ly.end(), [xy](int32_t a, int32_t b) { return xy[2 * a + 1] < xy[2 * b + 1]; });

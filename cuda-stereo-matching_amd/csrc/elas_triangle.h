@@ -43,8 +43,13 @@ struct ElasTriJob {
 struct ElasTriOrder {
     int lohi[4];  // integer orders: min x, max x, min y, max y
     std::vector<int32_t> lx, ly, arr, rank;
+    // entries: 64-bit (32-bit ranks), or with `narrow` (at most 65536
+    // points) 32-bit (16-bit ranks) in qx, qy, ql, qr, which halves the
+    // memory the cuts stream through
+    bool narrow = false;
     std::vector<uint64_t> px, py, tl, tr;
-    void resize(int n);
+    std::vector<uint32_t> qx, qy, ql, qr;
+    void resize(int n, bool narrow_entries);
     void pack();
     void cut(int lo, int size, int axis, int levels);
 };

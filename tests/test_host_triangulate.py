@@ -114,6 +114,18 @@ def test_random_integer_sets_every_size(pkg, oracle, n):
     assert np.array_equal(pkg.triangulate(pts), oracle.triangulate(pts))
 
 
+@pytest.mark.parametrize("n", [65536, 65537, 70001])
+def test_integer_sets_at_and_past_16_bit_ranks(pkg, oracle, n):
+    """The vertex order packs (x rank, y rank) into 32-bit entries up to
+    65536 points and into 64-bit entries beyond: both sides of the switch
+    give Triangle's mesh."""
+    rng = np.random.default_rng(n)
+    side = 400
+    flat = rng.choice(side * side, n, replace=False)
+    pts = np.stack([flat % side, flat // side], 1).astype(np.float32) * 2
+    assert np.array_equal(pkg.triangulate(pts), oracle.triangulate(pts))
+
+
 def test_full_lattice_cocircular(pkg, oracle):
     """Every 4 neighbours of a full lattice are cocircular: the mesh depends
     on the exact merge order, which the vertex order fixes."""
