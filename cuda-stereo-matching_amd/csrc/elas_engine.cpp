@@ -592,6 +592,10 @@ int elas_ctx::slot_phase_a(ElasSlot& S) {
         S.graphed = use_graphs;
     }
     HIPCHECK(hipEventRecord(S.ev[5], s1));
+    // a single-pair call: while the device computes the support points, the
+    // two workers that will take the meshes warm their order scratch (sized
+    // by the slot's last call)
+    if (S.gated && n == 1 && S.last_nsp >= 3 && tri_workers > 1) elas_tri_prewarm(S.last_nsp, S.last_nsp);
     // disparity grid on the second stream, overlapping the host Triangle
     // step; a gated chunk (one slot, phase B queued behind the gate) keeps it
     // on the first stream after phase A, where the device would only wait
@@ -625,6 +629,7 @@ int elas_ctx::slot_start_tri(ElasSlot& S) {
     // the compaction kernel wrote the counts and support points into the
     // pinned staging; phase A is complete, so they are all there
     for (int i = 0; i < n; i++) S.nsp[i] = S.dd[i].nblk > 0 ? S.hnsp[i] : 0;
+    if (n == 1) S.last_nsp = S.nsp[0];
     S.t_sp = clk::now();
     S.jobs.clear();
     S.job_of.assign(2 * (size_t)n, -1);

@@ -77,6 +77,7 @@ struct ElasSlot {
     std::vector<ElasPair> pairs;
     std::vector<int32_t> prefix;
     std::vector<int32_t> nsp, ntri0, ntri1, status;
+    int32_t last_nsp = 0;  // support points of the slot's last single-pair call
     int maxd = 0;
     size_t dense_lds[2] = {0, 0};  // dynamic LDS of the dense launches (1, 2 cells per wave)
     size_t plane_lds = 0;          // dynamic LDS of the plane launch (super-tile masks)

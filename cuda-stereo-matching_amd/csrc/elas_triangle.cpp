@@ -393,9 +393,24 @@ static __attribute__((target("avx2"))) int input_order(const int32_t* kx, const 
 // sort of that order by x then gives the (x, y) order, each point's x rank
 // and, with its y rank k, both packed entries.  Equal points are neighbours
 // in (y, x) order.
+namespace {
+
+// a thread's scratch for the vertex order of the meshes it runs
+struct OrderScratch {
+    std::vector<int32_t> cnt, kxv, kyv, tmp;
+    ElasTriOrder order;
+};
+OrderScratch& order_scratch() {
+    thread_local OrderScratch s;
+    return s;
+}
+
+}  // namespace
+
 bool elas_tri_order_integer(const float* p, int n, ElasTriOrder& o) {
     if (n < 3) return false;
-    thread_local std::vector<int32_t> cnt, kxv, kyv, tmp;
+    OrderScratch& sc = order_scratch();
+    std::vector<int32_t>&cnt = sc.cnt, &kxv = sc.kxv, &kyv = sc.kyv, &tmp = sc.tmp;
     kxv.resize(n);
     kyv.resize(n);
     int32_t* kx = kxv.data();
@@ -564,8 +579,30 @@ void tri_prepare(void* arg, int t) {
     }
 }
 
+// Brings a mesh of about n points' order scratch into this thread's cache
+// (written, so the order's scattered stores find their lines here): run by
+// the worker a mesh job is about to reach, while the device computes the
+// support points.
+void warm_order(int n) {
+    OrderScratch& s = order_scratch();
+    const bool narrow = n <= 65536;
+    s.kxv.resize(n);
+    s.kyv.resize(n);
+    s.order.resize(n, narrow);
+    memset(s.kxv.data(), 0, 4 * (size_t)n);
+    memset(s.kyv.data(), 0, 4 * (size_t)n);
+    memset(s.order.ly.data(), 0, 4 * (size_t)n);
+    if (narrow) {
+        for (std::vector<uint32_t>* v : {&s.order.qx, &s.order.qy, &s.order.ql, &s.order.qr})
+            memset(v->data(), 0, 4 * (size_t)n);
+    } else {
+        for (std::vector<uint64_t>* v : {&s.order.px, &s.order.py, &s.order.tl, &s.order.tr})
+            memset(v->data(), 0, 8 * (size_t)n);
+    }
+}
+
 int tri_call(int k, ElasTriJob& j, bool split, void* pool) {
-    thread_local ElasTriOrder order;
+    ElasTriOrder& order = order_scratch().order;
     const int depth = split ? job_depth(j.n, pool) : 0;
     TriCall c{k, &j, &order, nullptr, false, depth, pool};
     if (depth > 0) {
@@ -664,6 +701,18 @@ public:
     // workers 1 and 2 when their boxes are free, so each mesh of a run of
     // single-pair calls finds its thread's buffers where it left them; other
     // jobs go to the shared queue.
+    // Asks workers 1 and 2, which take the meshes of single-pair calls, to
+    // bring the order scratch of meshes of n0 and n1 points into their caches.
+    void prewarm(int n0, int n1) {
+        {
+            std::lock_guard<std::mutex> g(m_);
+            ensure_workers();
+            if (threads_.size() < 2) return;
+            warm_[1].store(n0, std::memory_order_relaxed);
+            warm_[2].store(n1, std::memory_order_relaxed);
+        }
+        cv_.notify_all();
+    }
     void enqueue(ElasTriJob* jobs, int n, Batch* b) {
         b->remaining = n;
         b->left.store(n, std::memory_order_relaxed);
@@ -771,6 +820,11 @@ private:
                 last = now_s();
                 continue;
             }
+            if (const int w = warm_[k].exchange(0, std::memory_order_relaxed)) {
+                warm_order(w);
+                last = now_s();
+                continue;
+            }
             Item it;
             if (Item* bx = box_[k].exchange(nullptr, std::memory_order_acquire)) {
                 it = *bx;  // the Item itself is freed with its batch
@@ -785,7 +839,9 @@ private:
                         continue;
                     }
                     std::unique_lock<std::mutex> g(m_);
-                    cv_.wait(g, [&] { return !q_.empty() || helping_.load() != 0 || box_[k].load() != nullptr; });
+                    cv_.wait(g, [&] {
+                        return !q_.empty() || helping_.load() != 0 || box_[k].load() != nullptr || warm_[k].load() != 0;
+                    });
                     last = now_s();  // woken: poll for a while before sleeping again
                     continue;
                 }
@@ -818,6 +874,7 @@ private:
     std::atomic<int> helping_{0};  // published task sets
     std::atomic<TaskSet*> slots_[ELAS_TRI_SLOTS] = {};
     std::atomic<Item*> box_[ELAS_TRI_INSTANCES] = {};  // a job for worker k
+    std::atomic<int> warm_[ELAS_TRI_INSTANCES] = {};   // points of a mesh worker k is about to get
     std::atomic<int> readers_[ELAS_TRI_SLOTS] = {};
     std::mutex m_, serial_;
     std::condition_variable cv_;
@@ -835,6 +892,10 @@ void pool_parallel_for(void* ctx, int ntasks, void (*task)(void*, int), void* ar
 
 void elas_triangulate_jobs(ElasTriJob* jobs, int n, int max_workers) {
     TriPool::get().run(jobs, n, max_workers);
+}
+
+void elas_tri_prewarm(int n0, int n1) {
+    if (n0 >= 3 && n1 >= 3) TriPool::get().prewarm(n0, n1);
 }
 
 void elas_tri_set_split(int max_depth) { g_split_max.store(max_depth < 0 ? -1 : std::min(max_depth, 12)); }

@@ -84,6 +84,11 @@ void elas_triangulate_jobs(ElasTriJob* jobs, int n, int max_workers);
 void elas_tri_set_split(int max_depth);
 int elas_tri_split_depth(int n);
 
+// A single-pair call about to submit its two meshes (of about n0 and n1
+// points): the workers that will take them warm their order scratch
+// meanwhile.  Optional; the meshes are the same either way.
+void elas_tri_prewarm(int n0, int n1);
+
 // Non-blocking form: queues the jobs and returns a ticket; elas_tri_done()
 // tells whether they have all finished, elas_tri_wait() blocks until then.
 // The jobs must stay alive until done; elas_tri_release() frees a finished
