@@ -601,6 +601,17 @@ void warm_order(int n) {
     }
 }
 
+// Brings a block of `bytes` into this thread's cache and its own block list,
+// from which the next pool of that size class this thread allocates comes
+// (a subtree's triangle pool): run by each thread while the device computes
+// the support points of a single-pair call.
+void warm_block(size_t bytes) {
+    void* b = elas_tri_block_alloc(bytes);
+    if (!b) return;
+    memset(b, 0, bytes);
+    elas_tri_block_free(b);
+}
+
 int tri_call(int k, ElasTriJob& j, bool split, void* pool) {
     ElasTriOrder& order = order_scratch().order;
     const int depth = split ? job_depth(j.n, pool) : 0;
@@ -703,15 +714,22 @@ public:
     // jobs go to the shared queue.
     // Asks workers 1 and 2, which take the meshes of single-pair calls, to
     // bring the order scratch of meshes of n0 and n1 points into their caches.
+    // Every thread (the caller too) also brings into its cache a block of the
+    // size a subtree's triangle pool takes.
     void prewarm(int n0, int n1) {
+        // a subtree of an 8-way split: its pool's first block holds
+        // 2 size + 8 triangles of 48 bytes ("zQB")
+        const size_t leaf = 48 * (2 * (size_t)(std::max(n0, n1) / 8 + 1) + 8) + 16;
         {
             std::lock_guard<std::mutex> g(m_);
             ensure_workers();
             if (threads_.size() < 2) return;
             warm_[1].store(n0, std::memory_order_relaxed);
             warm_[2].store(n1, std::memory_order_relaxed);
+            for (size_t k = 1; k <= threads_.size(); k++) warmb_[k].store((long)leaf, std::memory_order_relaxed);
         }
         cv_.notify_all();
+        warm_block(leaf);
     }
     void enqueue(ElasTriJob* jobs, int n, Batch* b) {
         b->remaining = n;
@@ -825,6 +843,11 @@ private:
                 last = now_s();
                 continue;
             }
+            if (const long wb = warmb_[k].exchange(0, std::memory_order_relaxed)) {
+                warm_block((size_t)wb);
+                last = now_s();
+                continue;
+            }
             Item it;
             if (Item* bx = box_[k].exchange(nullptr, std::memory_order_acquire)) {
                 it = *bx;  // the Item itself is freed with its batch
@@ -840,7 +863,8 @@ private:
                     }
                     std::unique_lock<std::mutex> g(m_);
                     cv_.wait(g, [&] {
-                        return !q_.empty() || helping_.load() != 0 || box_[k].load() != nullptr || warm_[k].load() != 0;
+                        return !q_.empty() || helping_.load() != 0 || box_[k].load() != nullptr || warm_[k].load() != 0 ||
+                               warmb_[k].load() != 0;
                     });
                     last = now_s();  // woken: poll for a while before sleeping again
                     continue;
@@ -875,6 +899,7 @@ private:
     std::atomic<TaskSet*> slots_[ELAS_TRI_SLOTS] = {};
     std::atomic<Item*> box_[ELAS_TRI_INSTANCES] = {};  // a job for worker k
     std::atomic<int> warm_[ELAS_TRI_INSTANCES] = {};   // points of a mesh worker k is about to get
+    std::atomic<long> warmb_[ELAS_TRI_INSTANCES] = {};  // bytes of a pool block worker k warms
     std::atomic<int> readers_[ELAS_TRI_SLOTS] = {};
     std::mutex m_, serial_;
     std::condition_variable cv_;
